@@ -1,0 +1,215 @@
+/*
+ * seismod_hip.h -- C ABI of the MI355X FDTD library (libseismod_hip.so).
+ *
+ * The reference (seismod) has no library API: its boundary is two programs,
+ * e_iso (2D elastic, split PML) and a_iso_3d (3D acoustic, CPML), whose time
+ * loops call plain C functions or run inline loops.  This ABI replaces
+ * those loops and their FP64 set-up; the drop-in programs in
+ * seismod_amd/host/ keep the reference's files and command line and call it.
+ *
+ * Conventions
+ *   - Every function returns an int status (SM_OK = 0); the library never
+ *     exits or aborts.  sm_last_error() gives a message for the calling
+ *     thread's last failure.
+ *   - Host arrays are FP64 in the reference's layouts: 2D index i*nz+j and
+ *     3D index (i*ny2+j)*nz2+k, z fastest.  Device layout and precision are
+ *     private to the library.
+ *   - One sm_ctx owns one HIP stream on one device.  Solvers created on a
+ *     context run on its stream; different contexts run concurrently.
+ *     A context and its solvers must not be used from two threads at once.
+ *   - Enumerations reuse the reference's numbering (structs.h:44-45).
+ */
+#ifndef SEISMOD_HIP_H
+#define SEISMOD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SM_ABI_VERSION 1
+
+/* status codes */
+#define SM_OK 0
+#define SM_ERR_ARG 1     /* bad argument or size */
+#define SM_ERR_HIP 2     /* HIP runtime error */
+#define SM_ERR_CFL 3     /* time step violates the stability limit */
+#define SM_ERR_NOMEM 4   /* host or device allocation failed */
+#define SM_ERR_STATE 5   /* call not valid in the solver's current state */
+#define SM_ERR_NODEV 6   /* no usable GPU */
+
+/* wavefield storage formats (arithmetic is FP32 except SM_STORE_F64) */
+#define SM_STORE_F32 0
+#define SM_STORE_F64 1   /* FP64 storage and arithmetic, same operation
+                            order as the reference */
+#define SM_STORE_BF16 2  /* bfloat16 with a per-field power-of-two scale */
+#define SM_STORE_F16 3   /* IEEE half with a per-field power-of-two scale */
+
+/* record components, numbering of enum component (structs.h:45) */
+#define SM_VX 0
+#define SM_VY 1
+#define SM_VZ 2
+#define SM_TXX 5
+#define SM_TZZ 6
+#define SM_TXZ 7
+#define SM_P 8
+
+/* source types, numbering of enum sourceType (structs.h:44) */
+#define SM_SX 0
+#define SM_SY 1
+#define SM_SZ 2
+#define SM_SXY 3
+#define SM_SXZ 4
+#define SM_SYZ 5
+#define SM_BULK 6
+#define SM_SF 7
+
+typedef struct sm_ctx sm_ctx;
+typedef struct sm_a3d sm_a3d;
+typedef struct sm_e2d sm_e2d;
+
+/* ------------------------------------------------------------------ */
+/* library and context                                                 */
+/* ------------------------------------------------------------------ */
+int sm_abi_version(void);
+const char *sm_strerror(int status);
+/* message of the calling thread's last failed call ("" if none) */
+const char *sm_last_error(void);
+
+/* One context = one device + one HIP stream (default priority). */
+int sm_ctx_create(int device, sm_ctx **out);
+int sm_ctx_destroy(sm_ctx *ctx);
+/* wait for all work queued on the context's stream */
+int sm_ctx_sync(sm_ctx *ctx);
+/* free and total device memory (hipMemGetInfo) of the context's device */
+int sm_ctx_mem_info(sm_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
+
+/* ------------------------------------------------------------------ */
+/* host-side FP64 set-up shared by both programs                       */
+/* ------------------------------------------------------------------ */
+/* Ricker length ceil(2/(f*dt)) -- replaces src.c:50-51. */
+size_t sm_ricker_length(double f, double dt);
+/* compute_src_fct (src.c:43-65) for n sources, fct concatenated (source s
+ * starts at sum(length[0..s-1])); keeps the reference's normalisation of
+ * every pass through source 0.  length[] from sm_ricker_length. */
+int sm_compute_src_fct(size_t n, const double *A, const double *f, double dt,
+                       const size_t *length, double *fct);
+/* time-step count of e_iso.c:575-576 and a_iso_3d.c:578-579 */
+size_t sm_nsteps(double duration, double dt);
+/* record predicate of e_iso.c:643-658 / a_iso_3d.c:700-718: non-zero when
+ * a record with start t0 [s] and interval rdt [s] samples after step it */
+int sm_record_due(size_t it, double dt, double t0, double rdt);
+
+/* ------------------------------------------------------------------ */
+/* 3D acoustic CPML solver -- replaces a_iso_3d.c:368-417, 494-693     */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    size_t nx, ny, nz;        /* interior nodes (struct grid3d, structs.h:72-78) */
+    size_t np;                /* CPML layers, 0 without absorbing boundary */
+    double dx, dy, dz;        /* m */
+    double dt;                /* s */
+    double vmax;              /* abs_params.vmax (5000 in the reference) */
+    double pml_order;         /* abs_params.pmlOrder (2) */
+    double Rc;                /* abs_params.Rc */
+    double kappa_max;         /* abs_params.kappa_max */
+    int alpha_max;            /* abs_params.alpha_max: 0 -> alpha = 0 */
+    double f0;                /* frequency of source 0 (alpha_max = pi*f0) */
+    int storage;              /* SM_STORE_* */
+    int flags;                /* SM_A3D_* */
+} sm_a3d_params;
+
+#define SM_A3D_NAIVE 1        /* two plain kernels per step (no fusion) */
+
+/* Build a solver from the padded model as read_model_a leaves it
+ * (io_utils.c:1029-1153): vp and rho hold nx2*ny2*nz2 values with the
+ * CPML cells already filled by edge replication.  Performs the set-up of
+ * a_iso_3d.c:368-417 (including its Vp maximum over the first nx*nz
+ * entries, 368-374), the CPML tables of 500-576, uploads the model and
+ * zeroes the wavefields (494-498).  Returns SM_ERR_CFL where the reference
+ * aborts (411-417). */
+int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *p, const double *vp,
+                  const double *rho, sm_a3d **out);
+int sm_a3d_destroy(sm_a3d *s);
+/* Point sources, a_iso_3d.c:689-693: after step it, while it < length[n],
+ * P[idx[n]] += dt*fct[n][it]*1e6.  fct concatenated as for
+ * sm_compute_src_fct.  Replaces any earlier set. */
+int sm_a3d_set_sources(sm_a3d *s, size_t nsrc, const size_t *idx,
+                       const size_t *length, const double *fct);
+/* Receivers: the value of comp[r] (SM_VX, SM_VY, SM_VZ, SM_P) at padded
+ * index idx[r] is recorded after every step, for up to max_steps steps
+ * from the current step.  The host applies the record predicate
+ * (sm_record_due) when writing traces (a_iso_3d.c:698-741). */
+int sm_a3d_set_receivers(sm_a3d *s, size_t nrec, const size_t *idx,
+                         const int *comp, size_t max_steps);
+/* Advance nsteps time steps (velocity sweep, pressure sweep, source,
+ * receivers; a_iso_3d.c:593-693).  Queued on the context stream. */
+int sm_a3d_run(sm_a3d *s, size_t nsteps);
+/* steps advanced since creation or the last reset */
+int sm_a3d_current_step(const sm_a3d *s, size_t *it);
+/* recorded values of steps [it0, it0+count) as out[count][nrec] (FP64);
+ * synchronises the stream */
+int sm_a3d_get_traces(sm_a3d *s, size_t it0, size_t count, double *out);
+/* whole padded field (SM_P, SM_VX, SM_VY, SM_VZ) in the reference layout */
+int sm_a3d_get_field(sm_a3d *s, int comp, double *out);
+/* nx2*nz2 x-z plane at padded y index iy, i-major (a_iso_3d.c:726-734) */
+int sm_a3d_get_slice_xz(sm_a3d *s, int comp, size_t iy, double *out);
+/* zero wavefields, memory variables and recorded traces; step count -> 0 */
+int sm_a3d_reset(sm_a3d *s);
+/* device bytes this solver holds, and the bytes sm_a3d_create would need
+ * for parameters p (for admission control) */
+int sm_a3d_device_bytes(const sm_a3d *s, size_t *bytes);
+size_t sm_a3d_bytes_needed(const sm_a3d_params *p, size_t nrec, size_t max_steps);
+
+/* ------------------------------------------------------------------ */
+/* 2D elastic split-PML solver -- replaces e_iso.c:423-500, 565-670,    */
+/* propagate.c:2511-2670, pml.c:39-1403, src.c:170-184                  */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    size_t nx, nz;            /* interior nodes (struct grid, np = 0) */
+    size_t npml;              /* PML layers (>= 1; e_iso.c:346, Q: abs 0) */
+    double h;                 /* dx == dz, m (e_iso.c:363-369) */
+    double dt;                /* s */
+    int storage;              /* SM_STORE_F32 or SM_STORE_F64 */
+    int q1_compat;            /* non-zero: use l2m/la[nx*nx-1] in the
+                                 lower-right corner of pml_t as pml.c:1044-1084
+                                 does (only where nx <= nz; otherwise that
+                                 read is out of bounds and the true corner
+                                 cell nx*nz-1 is used) */
+    double damping_scale;     /* 1.0 reproduces e_iso.c:490-500 */
+    int flags;                /* reserved, 0 */
+} sm_e2d_params;
+
+/* Build a solver from interior vp, vs, rho (nx*nz each, read_model_e,
+ * io_utils.c:892-947): Lame constants and staggered averages
+ * (e_iso.c:423-462), CFL check (476-479, SM_ERR_CFL), PML damping from
+ * the mean Vp (482-500); zero fields (565-572). */
+int sm_e2d_create(sm_ctx *ctx, const sm_e2d_params *p, const double *vp,
+                  const double *vs, const double *rho, sm_e2d **out);
+int sm_e2d_destroy(sm_e2d *s);
+/* add_src (src.c:170-184): type[n] (SM_SX/SM_SZ/SM_BULK/SM_SXZ; other
+ * types inject nothing), interior index idx[n] = i*nz+j. */
+int sm_e2d_set_sources(sm_e2d *s, size_t nsrc, const int *type,
+                       const size_t *idx, const size_t *length,
+                       const double *fct);
+/* receivers on interior indices; comp SM_VX, SM_VZ, SM_TXX, SM_TZZ, SM_TXZ */
+int sm_e2d_set_receivers(sm_e2d *s, size_t nrec, const size_t *idx,
+                         const int *comp, size_t max_steps);
+/* velocity phase, stress phase, source, receivers (e_iso.c:589-636) */
+int sm_e2d_run(sm_e2d *s, size_t nsteps);
+int sm_e2d_current_step(const sm_e2d *s, size_t *it);
+int sm_e2d_get_traces(sm_e2d *s, size_t it0, size_t count, double *out);
+/* interior field (nx*nz) as the reference's arrays hold it */
+int sm_e2d_get_field(sm_e2d *s, int comp, double *out);
+/* field on the padded (nx+2Np)x(nz+2Np) grid: interior values plus x+z of
+ * the split pair in PML cells (what pml.c reads as a neighbour value) */
+int sm_e2d_get_field_padded(sm_e2d *s, int comp, double *out);
+int sm_e2d_reset(sm_e2d *s);
+int sm_e2d_device_bytes(const sm_e2d *s, size_t *bytes);
+size_t sm_e2d_bytes_needed(const sm_e2d_params *p, size_t nrec, size_t max_steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEISMOD_HIP_H */

@@ -1,0 +1,165 @@
+"""ctypes binding of the FP64 oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_build" / "liboracle.so"
+
+O_OK, O_ERR_ARG, O_ERR_CFL, O_ERR_NOMEM = 0, 1, 3, 4
+
+_dp = C.POINTER(C.c_double)
+_sp = C.POINTER(C.c_size_t)
+_ip = C.POINTER(C.c_int)
+
+
+class A3dParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_size_t), ("ny", C.c_size_t), ("nz", C.c_size_t), ("np", C.c_size_t),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("vmax", C.c_double), ("pml_order", C.c_double), ("Rc", C.c_double),
+        ("kappa_max", C.c_double), ("alpha_max", C.c_int), ("f0", C.c_double),
+        ("nthreads", C.c_int), ("pressure_threads", C.c_int),
+    ]
+
+
+class E2dParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_size_t), ("nz", C.c_size_t), ("npml", C.c_size_t),
+        ("h", C.c_double), ("dt", C.c_double), ("q1_compat", C.c_int),
+        ("nthreads", C.c_int), ("damping_scale", C.c_double),
+    ]
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} not built; run `python -m seismod_amd.build`")
+    lib = C.CDLL(str(LIB_PATH))
+    sig = {
+        "o_ricker_length": (C.c_size_t, [C.c_double, C.c_double]),
+        "o_compute_src_fct": (None, [C.c_size_t, _dp, _dp, C.c_double, _sp, _dp]),
+        "o_nsteps": (C.c_size_t, [C.c_double, C.c_double]),
+        "o_record_due": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_double]),
+        "o_cpml_tables": (None, [C.c_size_t, C.c_size_t, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, C.c_double, C.c_double, C.c_double,
+                                 _dp, _dp, _dp, _dp, _dp, _dp]),
+        "o_a3d_run": (C.c_int, [C.POINTER(A3dParams), _dp, _dp, C.c_size_t, _sp, _sp, _dp,
+                                C.c_size_t, _sp, _ip, C.c_size_t, _dp, _dp]),
+        "o_e2d_damping": (None, [C.c_size_t, C.c_double, C.c_double, _dp, _dp, _dp]),
+        "o_e2d_material": (None, [C.c_size_t, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "o_e2d_run": (C.c_int, [C.POINTER(E2dParams), _dp, _dp, _dp, C.c_size_t, _ip, _sp, _sp,
+                                _dp, C.c_size_t, _sp, _ip, C.c_size_t, _dp, _dp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _s(a):
+    return a.ctypes.data_as(_sp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def ricker_length(f, dt):
+    return int(lib.o_ricker_length(f, dt))
+
+
+def source_functions(A, f, dt):
+    A = np.ascontiguousarray(np.atleast_1d(A), dtype=np.float64)
+    f = np.ascontiguousarray(np.atleast_1d(f), dtype=np.float64)
+    lengths = np.array([ricker_length(x, dt) for x in f], dtype=np.uintp)
+    fct = np.zeros(int(lengths.sum()), dtype=np.float64)
+    lib.o_compute_src_fct(len(A), _d(A), _d(f), dt, _s(lengths), _d(fct))
+    return lengths, fct
+
+
+def nsteps(duration, dt):
+    return int(lib.o_nsteps(duration, dt))
+
+
+def record_due(it, dt, t0, rdt):
+    return bool(lib.o_record_due(it, dt, t0, rdt))
+
+
+def cpml_tables(n2, np_, d, dt, vmax=5000.0, order=2.0, Rc=1e-5, kappa_max=1.0, alpha_max=0.0):
+    out = [np.zeros(n2) for _ in range(3)] + [np.zeros(max(n2 - 1, 0)) for _ in range(3)]
+    lib.o_cpml_tables(n2, np_, d, dt, vmax, order, Rc, kappa_max, alpha_max, *(_d(a) for a in out))
+    return dict(zip(("a", "b", "k", "a_h", "b_h", "k_h"), out))
+
+
+def e2d_damping(npml, h, vp_mean):
+    d, dh, dH = np.zeros(npml), np.zeros(npml), np.zeros(npml + 1)
+    lib.o_e2d_damping(npml, h, vp_mean, _d(d), _d(dh), _d(dH))
+    return d, dh, dH
+
+
+def e2d_material(vp, vs, rho):
+    nx, nz = vp.shape
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (vp, vs, rho)]
+    out = [np.zeros((nx, nz)) for _ in range(5)]
+    lib.o_e2d_material(nx, nz, *(_d(x) for x in a), *(_d(x) for x in out))
+    return dict(zip(("la", "mu", "l2m", "b", "bm"), out))
+
+
+def a3d_run(cfg: dict, vp, rho, src_idx, src_len, src_fct, rec_idx, rec_comp, nsteps,
+            want_fields=True, nthreads=1, pressure_threads=1):
+    """Run the a_iso_3d restatement.  cfg keys as A3dParams fields."""
+    p = A3dParams(cfg["nx"], cfg["ny"], cfg["nz"], cfg["np"], cfg["dx"], cfg["dy"], cfg["dz"],
+                  cfg["dt"], cfg.get("vmax", 5000.0), cfg.get("pml_order", 2.0),
+                  cfg.get("Rc", 1e-5), cfg.get("kappa_max", 1.0), cfg.get("alpha_max", 1),
+                  cfg["f0"], nthreads, pressure_threads)
+    vp = np.ascontiguousarray(vp, dtype=np.float64).reshape(-1)
+    rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+    si = np.ascontiguousarray(src_idx, dtype=np.uintp)
+    sl = np.ascontiguousarray(src_len, dtype=np.uintp)
+    sf = np.ascontiguousarray(src_fct, dtype=np.float64)
+    ri = np.ascontiguousarray(rec_idx, dtype=np.uintp)
+    rc = np.ascontiguousarray(rec_comp, dtype=np.intc)
+    traces = np.zeros((nsteps, len(ri)))
+    fields = np.zeros(4 * vp.size) if want_fields else None
+    st = lib.o_a3d_run(C.byref(p), _d(vp), _d(rho), len(si), _s(si), _s(sl), _d(sf), len(ri),
+                       _s(ri), _i(rc), nsteps, _d(traces), _d(fields))
+    if st != O_OK:
+        raise RuntimeError(f"o_a3d_run status {st}")
+    return traces, fields
+
+
+def e2d_run(cfg: dict, vp, vs, rho, src_type, src_idx, src_len, src_fct, rec_idx, rec_comp,
+            nsteps, want_fields=True, want_split=False, nthreads=1):
+    p = E2dParams(cfg["nx"], cfg["nz"], cfg["npml"], cfg["h"], cfg["dt"],
+                  1 if cfg.get("q1_compat", True) else 0, nthreads,
+                  cfg.get("damping_scale", 1.0))
+    m = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (vp, vs, rho)]
+    st_ = np.ascontiguousarray(src_type, dtype=np.intc)
+    si = np.ascontiguousarray(src_idx, dtype=np.uintp)
+    sl = np.ascontiguousarray(src_len, dtype=np.uintp)
+    sf = np.ascontiguousarray(src_fct, dtype=np.float64)
+    ri = np.ascontiguousarray(rec_idx, dtype=np.uintp)
+    rc = np.ascontiguousarray(rec_comp, dtype=np.intc)
+    NX, NZ = cfg["nx"] + 2 * cfg["npml"], cfg["nz"] + 2 * cfg["npml"]
+    traces = np.zeros((nsteps, len(ri)))
+    fields = np.zeros(5 * NX * NZ) if want_fields else None
+    split = np.zeros(10 * NX * NZ) if want_split else None
+    st = lib.o_e2d_run(C.byref(p), *(_d(a) for a in m), len(si), _i(st_), _s(si), _s(sl), _d(sf),
+                       len(ri), _s(ri), _i(rc), nsteps, _d(traces), _d(fields), _d(split))
+    if st != O_OK:
+        raise RuntimeError(f"o_e2d_run status {st}")
+    return traces, fields, split

@@ -1,0 +1,108 @@
+"""ctypes binding of libseismod_hip.so (declared in include/seismod_hip.h).
+
+The library is loaded from seismod_amd/lib/.  There is no fallback: if the
+library is missing or fails to load, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libseismod_hip.so"
+
+SM_OK, SM_ERR_ARG, SM_ERR_HIP, SM_ERR_CFL, SM_ERR_NOMEM, SM_ERR_STATE, SM_ERR_NODEV = range(7)
+SM_STORE_F32, SM_STORE_F64, SM_STORE_BF16, SM_STORE_F16 = range(4)
+SM_VX, SM_VY, SM_VZ, SM_TXX, SM_TZZ, SM_TXZ, SM_P = 0, 1, 2, 5, 6, 7, 8
+SM_SX, SM_SY, SM_SZ, SM_SXY, SM_SXZ, SM_SYZ, SM_BULK, SM_SF = range(8)
+SM_A3D_NAIVE = 1
+
+_size_p = C.POINTER(C.c_size_t)
+_dbl_p = C.POINTER(C.c_double)
+_int_p = C.POINTER(C.c_int)
+
+
+class A3dParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_size_t), ("ny", C.c_size_t), ("nz", C.c_size_t), ("np", C.c_size_t),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("vmax", C.c_double), ("pml_order", C.c_double), ("Rc", C.c_double),
+        ("kappa_max", C.c_double), ("alpha_max", C.c_int), ("f0", C.c_double),
+        ("storage", C.c_int), ("flags", C.c_int),
+    ]
+
+
+class E2dParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_size_t), ("nz", C.c_size_t), ("npml", C.c_size_t),
+        ("h", C.c_double), ("dt", C.c_double), ("storage", C.c_int),
+        ("q1_compat", C.c_int), ("damping_scale", C.c_double), ("flags", C.c_int),
+    ]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "sm_abi_version": (C.c_int, []),
+    "sm_strerror": (C.c_char_p, [C.c_int]),
+    "sm_last_error": (C.c_char_p, []),
+    "sm_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "sm_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "sm_ctx_sync": (C.c_int, [C.c_void_p]),
+    "sm_ctx_mem_info": (C.c_int, [C.c_void_p, _size_p, _size_p]),
+    "sm_ricker_length": (C.c_size_t, [C.c_double, C.c_double]),
+    "sm_compute_src_fct": (C.c_int, [C.c_size_t, _dbl_p, _dbl_p, C.c_double, _size_p, _dbl_p]),
+    "sm_nsteps": (C.c_size_t, [C.c_double, C.c_double]),
+    "sm_record_due": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_double]),
+    "sm_a3d_create": (C.c_int, [C.c_void_p, C.POINTER(A3dParams), _dbl_p, _dbl_p,
+                                C.POINTER(C.c_void_p)]),
+    "sm_a3d_destroy": (C.c_int, [C.c_void_p]),
+    "sm_a3d_set_sources": (C.c_int, [C.c_void_p, C.c_size_t, _size_p, _size_p, _dbl_p]),
+    "sm_a3d_set_receivers": (C.c_int, [C.c_void_p, C.c_size_t, _size_p, _int_p, C.c_size_t]),
+    "sm_a3d_run": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "sm_a3d_current_step": (C.c_int, [C.c_void_p, _size_p]),
+    "sm_a3d_get_traces": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, _dbl_p]),
+    "sm_a3d_get_field": (C.c_int, [C.c_void_p, C.c_int, _dbl_p]),
+    "sm_a3d_get_slice_xz": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, _dbl_p]),
+    "sm_a3d_reset": (C.c_int, [C.c_void_p]),
+    "sm_a3d_device_bytes": (C.c_int, [C.c_void_p, _size_p]),
+    "sm_a3d_bytes_needed": (C.c_size_t, [C.POINTER(A3dParams), C.c_size_t, C.c_size_t]),
+    "sm_e2d_create": (C.c_int, [C.c_void_p, C.POINTER(E2dParams), _dbl_p, _dbl_p, _dbl_p,
+                                C.POINTER(C.c_void_p)]),
+    "sm_e2d_destroy": (C.c_int, [C.c_void_p]),
+    "sm_e2d_set_sources": (C.c_int, [C.c_void_p, C.c_size_t, _int_p, _size_p, _size_p, _dbl_p]),
+    "sm_e2d_set_receivers": (C.c_int, [C.c_void_p, C.c_size_t, _size_p, _int_p, C.c_size_t]),
+    "sm_e2d_run": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "sm_e2d_current_step": (C.c_int, [C.c_void_p, _size_p]),
+    "sm_e2d_get_traces": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, _dbl_p]),
+    "sm_e2d_get_field": (C.c_int, [C.c_void_p, C.c_int, _dbl_p]),
+    "sm_e2d_get_field_padded": (C.c_int, [C.c_void_p, C.c_int, _dbl_p]),
+    "sm_e2d_reset": (C.c_int, [C.c_void_p]),
+    "sm_e2d_device_bytes": (C.c_int, [C.c_void_p, _size_p]),
+    "sm_e2d_bytes_needed": (C.c_size_t, [C.POINTER(E2dParams), C.c_size_t, C.c_size_t]),
+}
+
+
+class SeismodError(RuntimeError):
+    def __init__(self, status: int, where: str, msg: str):
+        super().__init__(f"{where}: status {status} ({msg})")
+        self.status = status
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m seismod_amd.build`")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+EXPORTED = tuple(_SIGS)
+
+
+def check(status: int, where: str) -> None:
+    if status != SM_OK:
+        msg = lib.sm_last_error().decode(errors="replace")
+        raise SeismodError(status, where, msg)

@@ -1,0 +1,613 @@
+// a3d.hip -- 3D acoustic CPML solver (replaces a_iso_3d.c:368-417, 494-693).
+//
+// Host side: FP64 set-up in the reference's order (CFL, CPML tables, K*dt),
+// device buffers, the time loop and the record buffers.
+// Device side: the "naive" two-kernel step (velocity sweep, pressure sweep)
+// in FP64 (reference operation order) and FP32, plus the fused single-pass
+// step in a3d_fused.hip.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "a3d_common.hpp"
+#include "sm_internal.hpp"
+
+namespace sm {
+
+template <typename T>
+struct A3dConst {
+    T dx, dy, dz;      // FP64 path: spacings; FP32 path: 1/spacing
+    T dt2;             // FP64: dt; FP32: 2*dt
+};
+
+// ---------------------------------------------------------------------
+// naive kernels: one thread per cell, velocity then pressure
+// ---------------------------------------------------------------------
+template <typename T, bool EXACT>
+__device__ inline void vel_component(T *v, T *mem, int s, int n, size_t c, size_t step,
+                                     const T *P, const T *rho, const CpmlAxis<T> &ax,
+                                     T h, T dt2)
+{
+    // a_iso_3d.c:611-615 (x), 628-632 (y), 644-648 (z)
+    T dP;
+    if constexpr (EXACT) dP = (P[c + step] - P[c]) / h;
+    else dP = (P[c + step] - P[c]) * h;
+    if (s >= 0) {
+        T m = ax.bh[n] * mem[0] + ax.ah[n] * dP;
+        mem[0] = m;
+        if constexpr (EXACT) dP = dP / ax.kh[n] + m;
+        else dP = dP * ax.kh[n] + m;
+    }
+    T irho;
+    if constexpr (EXACT) irho = dt2 * 2. / (rho[c] + rho[c + step]);
+    else irho = dt2 / (rho[c] + rho[c + step]);
+    v[c] -= irho * dP;
+}
+
+template <typename T, bool EXACT>
+__global__ void __launch_bounds__(256)
+a3d_vel_naive(A3dDev<T> d, CpmlAxis<T> cx, CpmlAxis<T> cy, CpmlAxis<T> cz, A3dConst<T> k)
+{
+    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (kk >= d.nz2 || j >= d.ny2) return;
+    const size_t sx = (size_t)d.ny2 * d.nz2, sy = d.nz2;
+    const size_t c = ((size_t)i * d.ny2 + j) * d.nz2 + kk;
+    const int np = d.np, np2 = 2 * np;
+    if (i < d.nx2 - 1) {
+        int s = slab_vel(i, d.nx2, np);
+        T *mem = s >= 0 ? d.mPx + ((size_t)s * d.ny2 + j) * d.nz2 + kk : nullptr;
+        vel_component<T, EXACT>(d.vx, mem, s, i, c, sx, d.P, d.rho, cx, k.dx, k.dt2);
+    }
+    if (j < d.ny2 - 1) {
+        int s = slab_vel(j, d.ny2, np);
+        T *mem = s >= 0 ? d.mPy + ((size_t)i * np2 + s) * d.nz2 + kk : nullptr;
+        vel_component<T, EXACT>(d.vy, mem, s, j, c, sy, d.P, d.rho, cy, k.dy, k.dt2);
+    }
+    if (kk < d.nz2 - 1) {
+        int s = slab_vel(kk, d.nz2, np);
+        T *mem = s >= 0 ? d.mPz + ((size_t)i * d.ny2 + j) * np2 + s : nullptr;
+        vel_component<T, EXACT>(d.vz, mem, s, kk, c, 1, d.P, d.rho, cz, k.dz, k.dt2);
+    }
+}
+
+template <typename T, bool EXACT>
+__device__ inline T pre_component(T dv, T *mem, int s, int n, const CpmlAxis<T> &ax, T h)
+{
+    // a_iso_3d.c:667-678
+    if constexpr (EXACT) dv = dv / h;
+    else dv = dv * h;
+    if (s >= 0) {
+        T m = ax.b[n] * mem[0] + ax.a[n] * dv;
+        mem[0] = m;
+        if constexpr (EXACT) dv = dv / ax.k[n] + m;
+        else dv = dv * ax.k[n] + m;
+    }
+    return dv;
+}
+
+template <typename T, bool EXACT>
+__global__ void __launch_bounds__(256)
+a3d_pre_naive(A3dDev<T> d, CpmlAxis<T> cx, CpmlAxis<T> cy, CpmlAxis<T> cz, A3dConst<T> k)
+{
+    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    // a_iso_3d.c:659-661: i, j, k >= 1
+    if (kk >= d.nz2 || j >= d.ny2 || i < 1 || j < 1 || kk < 1) return;
+    const size_t sx = (size_t)d.ny2 * d.nz2, sy = d.nz2;
+    const size_t c = ((size_t)i * d.ny2 + j) * d.nz2 + kk;
+    const int np = d.np, np2 = 2 * np;
+    int s;
+    s = slab_pre(i, d.nx2, np);
+    T dvdx = pre_component<T, EXACT>(d.vx[c] - d.vx[c - sx],
+                                     s >= 0 ? d.mvx + ((size_t)s * d.ny2 + j) * d.nz2 + kk : nullptr,
+                                     s, i, cx, k.dx);
+    s = slab_pre(j, d.ny2, np);
+    T dvdy = pre_component<T, EXACT>(d.vy[c] - d.vy[c - sy],
+                                     s >= 0 ? d.mvy + ((size_t)i * np2 + s) * d.nz2 + kk : nullptr,
+                                     s, j, cy, k.dy);
+    s = slab_pre(kk, d.nz2, np);
+    T dvdz = pre_component<T, EXACT>(d.vz[c] - d.vz[c - 1],
+                                     s >= 0 ? d.mvz + ((size_t)i * d.ny2 + j) * np2 + s : nullptr,
+                                     s, kk, cz, k.dz);
+    d.P[c] -= d.kdt[c] * (dvdx + dvdy + dvdz);   // a_iso_3d.c:680
+}
+
+// sources (a_iso_3d.c:689-693) then receivers for step it, one block
+template <typename T>
+__global__ void __launch_bounds__(256)
+a3d_src_rcv(A3dDev<T> d, int nsrc, const int64_t *sidx, const int64_t *soff, const int64_t *slen,
+            const T *sinc, int nrec, const int64_t *ridx, const int *rcomp, T *trace, int64_t it,
+            int64_t row)
+{
+    if (threadIdx.x == 0) {
+        for (int n = 0; n < nsrc; ++n)
+            if (it < slen[n]) d.P[sidx[n]] += sinc[soff[n] + it];
+    }
+    __syncthreads();
+    if (row < 0) return;
+    for (int r = threadIdx.x; r < nrec; r += blockDim.x) {
+        const int cp = rcomp[r];
+        const T *f = cp == SM_VX ? d.vx : cp == SM_VY ? d.vy : cp == SM_VZ ? d.vz : d.P;
+        trace[row * nrec + r] = f[ridx[r]];
+    }
+}
+
+}  // namespace sm
+
+using namespace sm;
+
+struct sm_a3d {
+    sm_ctx *ctx = nullptr;
+    sm_a3d_params p{};
+    int nx2 = 0, ny2 = 0, nz2 = 0;
+    size_t nnn = 0;
+    size_t elt = 0;                 // bytes per stored value
+    bool fused = false;
+    // device memory
+    sm_dbuf fields[2];              // P, vx, vy, vz (ping-pong when fused)
+    sm_dbuf mat;                    // rho, kdt
+    sm_dbuf memv[2];                // mPx, mPy, mPz (ping-pong when fused)
+    sm_dbuf memp;                   // mvx, mvy, mvz
+    sm_dbuf tables;                 // 18 CPML tables
+    sm_dbuf src;                    // idx, off, len (int64) + inc (T)
+    sm_dbuf rcv;                    // idx (int64), comp (int)
+    sm_dbuf trace;                  // [max_steps][nrec]
+    int cur = 0;                    // which ping-pong half holds the state
+    size_t step = 0;
+    size_t nsrc = 0, nrec = 0, max_steps = 0, rcv_step0 = 0;
+    size_t src_total = 0;
+    // host copies of the FP64 tables (exposed to tests through get_tables)
+    std::vector<double> htab;
+};
+
+namespace {
+
+size_t slab_cells(size_t nx2, size_t ny2, size_t nz2, size_t np)
+{
+    return 2 * np * (ny2 * nz2 + nx2 * nz2 + nx2 * ny2);
+}
+
+// CPML tables of one axis (a_iso_3d.c:500-576), same write order.
+void cpml_axis(size_t n2, size_t np, double d, double dt, double vmax, double order,
+               double Rc, double kappa_max, double alpha_max, double *a, double *b, double *k,
+               double *ah, double *bh, double *kh)
+{
+    for (size_t n = 0; n < n2; ++n) { a[n] = 0.0; b[n] = 1.0; k[n] = 1.0; }
+    for (size_t n = 0; n + 1 < n2; ++n) { ah[n] = 0.0; bh[n] = 1.0; kh[n] = 1.0; }
+    if (np == 0) return;
+    const double da = alpha_max / np;
+    const double L = np * d;
+    const double d0 = -(order + 1.) * vmax * std::log(Rc) / (2. * L);
+    for (size_t n = 0; n < np; ++n) {
+        const double kn = 1. + (kappa_max - 1.) * std::pow(((np - n) * d / L), order);
+        const double khn = 1. + (kappa_max - 1.) * std::pow(((np - n - 0.5) * d / L), order);
+        k[n] = k[n2 - 1 - n] = kn;
+        kh[n] = kh[n2 - 2 - n] = khn;
+        const double alpha = n * da;
+        const double alpha_h = (n + 0.5) * da;
+        const double dn = d0 * std::pow(((np - n) * d / L), order);
+        const double dnh = d0 * std::pow(((np - n - 0.5) * d / L), order);
+        const double bn = std::exp(-(dn / k[n] + alpha) * dt);
+        b[n] = b[n2 - 1 - n] = bn;
+        const double bhn = std::exp(-(dnh / kh[n] + alpha_h) * dt);
+        bh[n] = bh[n2 - 2 - n] = bhn;
+        const double an = dn * (b[n] - 1.) / (k[n] * (dn + k[n] * alpha));
+        a[n] = a[n2 - 1 - n] = an;
+        const double ahn = dnh * (bh[n] - 1.) / (kh[n] * (dnh + kh[n] * alpha_h));
+        ah[n] = ah[n2 - 2 - n] = ahn;
+    }
+}
+
+template <typename T>
+int upload(void *dst, const double *src, size_t n, hipStream_t st, bool invert = false)
+{
+    std::vector<T> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = (T)(invert ? 1.0 / src[i] : src[i]);
+    SM_HIP(hipMemcpyAsync(dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice, st));
+    SM_HIP(hipStreamSynchronize(st));
+    return SM_OK;
+}
+
+template <typename T>
+A3dDev<T> dev_view(sm_a3d *s, int half)
+{
+    A3dDev<T> d;
+    d.nx2 = s->nx2; d.ny2 = s->ny2; d.nz2 = s->nz2; d.np = (int)s->p.np;
+    T *f = (T *)s->fields[half].ptr;
+    d.P = f; d.vx = f + s->nnn; d.vy = f + 2 * s->nnn; d.vz = f + 3 * s->nnn;
+    const T *m = (const T *)s->mat.ptr;
+    d.rho = m; d.kdt = m + s->nnn;
+    const size_t np2 = 2 * s->p.np;
+    T *mv = (T *)s->memv[half].ptr;
+    d.mPx = mv; d.mPy = mv + np2 * s->ny2 * s->nz2; d.mPz = d.mPy + s->nx2 * np2 * s->nz2;
+    T *mp = (T *)s->memp.ptr;
+    d.mvx = mp; d.mvy = mp + np2 * s->ny2 * s->nz2; d.mvz = d.mvy + s->nx2 * np2 * s->nz2;
+    return d;
+}
+
+template <typename T>
+void axes(sm_a3d *s, CpmlAxis<T> ax[3])
+{
+    const T *t = (const T *)s->tables.ptr;
+    const size_t n2[3] = {(size_t)s->nx2, (size_t)s->ny2, (size_t)s->nz2};
+    for (int d = 0; d < 3; ++d) {
+        ax[d].a = t; ax[d].b = t + n2[d]; ax[d].k = t + 2 * n2[d];
+        ax[d].ah = t + 3 * n2[d]; ax[d].bh = t + 4 * n2[d]; ax[d].kh = t + 5 * n2[d];
+        t += 6 * n2[d];
+    }
+}
+
+template <typename T, bool EXACT>
+int run_naive(sm_a3d *s, size_t nsteps)
+{
+    hipStream_t st = s->ctx->stream;
+    A3dDev<T> d = dev_view<T>(s, s->cur);
+    CpmlAxis<T> ax[3];
+    axes<T>(s, ax);
+    A3dConst<T> k;
+    if constexpr (EXACT) {
+        k.dx = s->p.dx; k.dy = s->p.dy; k.dz = s->p.dz; k.dt2 = s->p.dt;
+    } else {
+        k.dx = (T)(1.0 / s->p.dx); k.dy = (T)(1.0 / s->p.dy); k.dz = (T)(1.0 / s->p.dz);
+        k.dt2 = (T)(2.0 * s->p.dt);
+    }
+    const dim3 blk(64, 4, 1);
+    const dim3 grd((s->nz2 + 63) / 64, (s->ny2 + 3) / 4, s->nx2);
+    const int64_t *sp = (const int64_t *)s->src.ptr;
+    const T *sinc = (const T *)(sp + 3 * s->nsrc);
+    const int64_t *ridx = (const int64_t *)s->rcv.ptr;
+    const int *rcomp = (const int *)(ridx + s->nrec);
+    for (size_t n = 0; n < nsteps; ++n) {
+        const size_t it = s->step + n;
+        a3d_vel_naive<T, EXACT><<<grd, blk, 0, st>>>(d, ax[0], ax[1], ax[2], k);
+        a3d_pre_naive<T, EXACT><<<grd, blk, 0, st>>>(d, ax[0], ax[1], ax[2], k);
+        int64_t row = -1;
+        if (s->nrec && it >= s->rcv_step0 && it - s->rcv_step0 < s->max_steps)
+            row = (int64_t)(it - s->rcv_step0);
+        if (s->nsrc || row >= 0)
+            a3d_src_rcv<T><<<1, 256, 0, st>>>(d, (int)s->nsrc, sp, sp + s->nsrc, sp + 2 * s->nsrc,
+                                              sinc, (int)s->nrec, ridx, rcomp, (T *)s->trace.ptr,
+                                              (int64_t)it, row);
+    }
+    SM_HIP(hipGetLastError());
+    s->step += nsteps;
+    return SM_OK;
+}
+
+template <typename T>
+int read_back(sm_a3d *s, const void *dev, size_t n, double *out)
+{
+    std::vector<T> tmp(n);
+    SM_HIP(hipMemcpyAsync(tmp.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost, s->ctx->stream));
+    SM_HIP(hipStreamSynchronize(s->ctx->stream));
+    for (size_t i = 0; i < n; ++i) out[i] = (double)tmp[i];
+    return SM_OK;
+}
+
+int zero_state(sm_a3d *s)
+{
+    hipStream_t st = s->ctx->stream;
+    for (int h = 0; h < 2; ++h) {
+        if (s->fields[h].ptr) SM_HIP(hipMemsetAsync(s->fields[h].ptr, 0, s->fields[h].bytes, st));
+        if (s->memv[h].ptr) SM_HIP(hipMemsetAsync(s->memv[h].ptr, 0, s->memv[h].bytes, st));
+    }
+    if (s->memp.ptr) SM_HIP(hipMemsetAsync(s->memp.ptr, 0, s->memp.bytes, st));
+    if (s->trace.ptr) SM_HIP(hipMemsetAsync(s->trace.ptr, 0, s->trace.bytes, st));
+    SM_HIP(hipStreamSynchronize(st));
+    s->cur = 0;
+    s->step = 0;
+    s->rcv_step0 = 0;
+    return SM_OK;
+}
+
+void release(sm_a3d *s)
+{
+    for (int h = 0; h < 2; ++h) { dfree(s->fields[h]); dfree(s->memv[h]); }
+    dfree(s->mat); dfree(s->memp); dfree(s->tables); dfree(s->src); dfree(s->rcv); dfree(s->trace);
+}
+
+size_t elt_size(int storage)
+{
+    switch (storage) {
+    case SM_STORE_F32: return 4;
+    case SM_STORE_F64: return 8;
+    default: return 0;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t sm_a3d_bytes_needed(const sm_a3d_params *p, size_t nrec, size_t max_steps)
+{
+    if (!p) return 0;
+    const size_t e = elt_size(p->storage);
+    if (!e) return 0;
+    const size_t nx2 = p->nx + 2 * p->np, ny2 = p->ny + 2 * p->np, nz2 = p->nz + 2 * p->np;
+    const size_t nnn = nx2 * ny2 * nz2;
+    const bool fused = false;
+    const size_t halves = fused ? 2 : 1;
+    const size_t slabs = slab_cells(nx2, ny2, nz2, p->np);
+    return e * (halves * 4 * nnn + 2 * nnn + (halves + 1) * slabs +
+                6 * (nx2 + ny2 + nz2) + nrec * max_steps) + 16 * nrec;
+}
+
+int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const double *rho,
+                  sm_a3d **out)
+{
+    SM_CHECK(ctx && pp && vp && rho && out, SM_ERR_ARG, "sm_a3d_create: NULL argument");
+    const sm_a3d_params &p = *pp;
+    SM_CHECK(p.nx >= 1 && p.ny >= 1 && p.nz >= 1, SM_ERR_ARG, "sm_a3d_create: empty grid");
+    SM_CHECK(p.dx > 0 && p.dy > 0 && p.dz > 0 && p.dt > 0, SM_ERR_ARG,
+             "sm_a3d_create: spacings and dt must be > 0");
+    SM_CHECK(p.dx == p.dz, SM_ERR_ARG,
+             "grid cell size must be equal in X and Z (a_iso_3d.c:298-301)");
+    const size_t e = elt_size(p.storage);
+    SM_CHECK(e != 0, SM_ERR_ARG, "sm_a3d_create: storage %d not supported", p.storage);
+    const size_t nx2 = p.nx + 2 * p.np, ny2 = p.ny + 2 * p.np, nz2 = p.nz + 2 * p.np;
+    SM_CHECK(nx2 < 65536 && ny2 < (1u << 30) && nz2 < (1u << 30), SM_ERR_ARG,
+             "sm_a3d_create: grid too large");
+    const size_t nnn = nx2 * ny2 * nz2;
+    SM_CHECK(nnn < ((size_t)1 << 34), SM_ERR_ARG, "sm_a3d_create: grid too large");
+
+    // a_iso_3d.c:368-374: Vp maximum over the first nx*nz padded entries
+    double vpmax = 0.0;
+    for (size_t i = 0; i < p.nx; ++i)
+        for (size_t j = 0; j < p.nz; ++j) {
+            const double v = vp[i * p.nz + j];
+            vpmax = v > vpmax ? v : vpmax;
+        }
+    // a_iso_3d.c:411-417
+    const double lim = p.dt * vpmax *
+                       std::sqrt(1. / (p.dx * p.dx) + 1. / (p.dy * p.dy) + 1. / (p.dz * p.dz));
+    if (lim > 1.)
+        return fail(SM_ERR_CFL, "time step too large: dt*vpmax*sqrt(sum 1/d^2) = %f > 1", lim);
+    for (size_t n = 0; n < nnn; ++n)
+        SM_CHECK(rho[n] > 0.0 && std::isfinite(vp[n]), SM_ERR_ARG,
+                 "sm_a3d_create: rho must be > 0 and vp finite (cell %zu)", n);
+
+    SM_CHECK(set_device(ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    sm_a3d *s = new sm_a3d;
+    s->ctx = ctx;
+    s->p = p;
+    s->nx2 = (int)nx2; s->ny2 = (int)ny2; s->nz2 = (int)nz2;
+    s->nnn = nnn;
+    s->elt = e;
+    s->fused = false;
+    // one set (velocity or pressure) of x, y and z slab arrays
+    const size_t slab_set = slab_cells(nx2, ny2, nz2, p.np);
+    int rc = SM_OK;
+    const int halves = s->fused ? 2 : 1;
+    for (int h = 0; h < halves && rc == SM_OK; ++h) {
+        rc = dalloc(s->fields[h], 4 * nnn * e);
+        if (rc == SM_OK) rc = dalloc(s->memv[h], slab_set * e);
+    }
+    if (rc == SM_OK) rc = dalloc(s->mat, 2 * nnn * e);
+    if (rc == SM_OK) rc = dalloc(s->memp, slab_set * e);
+    if (rc == SM_OK) rc = dalloc(s->tables, 6 * (nx2 + ny2 + nz2) * e);
+    if (rc != SM_OK) { release(s); delete s; return rc; }
+    {
+        // extent of the z slab set as dev_view lays it out: must end inside
+        // the allocation (x: 2np*ny2*nz2, y: nx2*2np*nz2, z: nx2*ny2*2np)
+        const size_t np2 = 2 * p.np;
+        const size_t end = np2 * ny2 * nz2 + nx2 * np2 * nz2 + nx2 * ny2 * np2;
+        if (end * e > s->memp.bytes || end * e > s->memv[0].bytes) {
+            release(s); delete s;
+            return fail(SM_ERR_STATE, "internal: slab layout exceeds its allocation");
+        }
+    }
+
+    // material: rho as read, K*dt = rho vp^2 dt (a_iso_3d.c:407-409)
+    {
+        std::vector<double> kdt(nnn);
+        for (size_t n = 0; n < nnn; ++n) kdt[n] = rho[n] * vp[n] * vp[n] * p.dt;
+        hipStream_t st = ctx->stream;
+        if (p.storage == SM_STORE_F64) {
+            rc = upload<double>(s->mat.ptr, rho, nnn, st);
+            if (rc == SM_OK) rc = upload<double>((double *)s->mat.ptr + nnn, kdt.data(), nnn, st);
+        } else {
+            rc = upload<float>(s->mat.ptr, rho, nnn, st);
+            if (rc == SM_OK) rc = upload<float>((float *)s->mat.ptr + nnn, kdt.data(), nnn, st);
+        }
+    }
+    // CPML tables (a_iso_3d.c:525-576)
+    if (rc == SM_OK) {
+        const double pi = 4.0 * std::atan(1.0);
+        double alpha_max = pi * p.f0;
+        if (p.alpha_max == 0) alpha_max = 0.0;
+        const size_t n2[3] = {nx2, ny2, nz2};
+        const double dd[3] = {p.dx, p.dy, p.dz};
+        s->htab.assign(6 * (nx2 + ny2 + nz2), 0.0);
+        double *t = s->htab.data();
+        for (int d = 0; d < 3; ++d) {
+            cpml_axis(n2[d], p.np, dd[d], p.dt, p.vmax, p.pml_order, p.Rc, p.kappa_max, alpha_max,
+                      t, t + n2[d], t + 2 * n2[d], t + 3 * n2[d], t + 4 * n2[d], t + 5 * n2[d]);
+            t += 6 * n2[d];
+        }
+        if (p.storage == SM_STORE_F64) {
+            rc = upload<double>(s->tables.ptr, s->htab.data(), s->htab.size(), ctx->stream);
+        } else {
+            // FP32 paths store 1/kappa in place of kappa
+            std::vector<double> inv(s->htab);
+            double *q = inv.data();
+            for (int d = 0; d < 3; ++d) {
+                for (size_t n = 0; n < n2[d]; ++n) q[2 * n2[d] + n] = 1.0 / q[2 * n2[d] + n];
+                for (size_t n = 0; n + 1 < n2[d]; ++n) q[5 * n2[d] + n] = 1.0 / q[5 * n2[d] + n];
+                q += 6 * n2[d];
+            }
+            rc = upload<float>(s->tables.ptr, inv.data(), inv.size(), ctx->stream);
+        }
+    }
+    if (rc == SM_OK) rc = zero_state(s);
+    if (rc != SM_OK) { release(s); delete s; return rc; }
+    *out = s;
+    return SM_OK;
+}
+
+int sm_a3d_destroy(sm_a3d *s)
+{
+    if (!s) return SM_OK;
+    (void)set_device(s->ctx);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    release(s);
+    delete s;
+    return SM_OK;
+}
+
+int sm_a3d_set_sources(sm_a3d *s, size_t nsrc, const size_t *idx, const size_t *length,
+                       const double *fct)
+{
+    SM_CHECK(s, SM_ERR_ARG, "sm_a3d_set_sources: NULL solver");
+    SM_CHECK(nsrc == 0 || (idx && length && fct), SM_ERR_ARG, "sm_a3d_set_sources: NULL array");
+    SM_CHECK(nsrc < (1u << 20), SM_ERR_ARG, "sm_a3d_set_sources: too many sources");
+    size_t total = 0;
+    for (size_t n = 0; n < nsrc; ++n) {
+        SM_CHECK(idx[n] < s->nnn, SM_ERR_ARG, "source %zu: index %zu outside the grid", n, idx[n]);
+        total += length[n];
+    }
+    SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    SM_HIP(hipStreamSynchronize(s->ctx->stream));
+    dfree(s->src);
+    s->nsrc = 0;
+    if (nsrc == 0) return SM_OK;
+    const size_t bytes = 3 * nsrc * sizeof(int64_t) + total * s->elt;
+    int rc = dalloc(s->src, bytes);
+    if (rc != SM_OK) return rc;
+    std::vector<unsigned char> h(bytes);
+    int64_t *hi = (int64_t *)h.data();
+    size_t off = 0;
+    for (size_t n = 0; n < nsrc; ++n) {
+        hi[n] = (int64_t)idx[n];
+        hi[nsrc + n] = (int64_t)off;
+        hi[2 * nsrc + n] = (int64_t)length[n];
+        off += length[n];
+    }
+    // increment dt*fct*1e6 (a_iso_3d.c:691), rounded once to the storage type
+    for (size_t q = 0; q < total; ++q) {
+        const double inc = s->p.dt * fct[q] * 1.e6;
+        if (s->elt == 8) ((double *)(hi + 3 * nsrc))[q] = inc;
+        else ((float *)(hi + 3 * nsrc))[q] = (float)inc;
+    }
+    SM_HIP(hipMemcpyAsync(s->src.ptr, h.data(), bytes, hipMemcpyHostToDevice, s->ctx->stream));
+    SM_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->nsrc = nsrc;
+    s->src_total = total;
+    return SM_OK;
+}
+
+int sm_a3d_set_receivers(sm_a3d *s, size_t nrec, const size_t *idx, const int *comp,
+                         size_t max_steps)
+{
+    SM_CHECK(s, SM_ERR_ARG, "sm_a3d_set_receivers: NULL solver");
+    SM_CHECK(nrec == 0 || (idx && comp), SM_ERR_ARG, "sm_a3d_set_receivers: NULL array");
+    SM_CHECK(nrec < (1u << 24), SM_ERR_ARG, "sm_a3d_set_receivers: too many receivers");
+    for (size_t r = 0; r < nrec; ++r) {
+        SM_CHECK(idx[r] < s->nnn, SM_ERR_ARG, "receiver %zu: index %zu outside the grid", r, idx[r]);
+        SM_CHECK(comp[r] == SM_VX || comp[r] == SM_VY || comp[r] == SM_VZ || comp[r] == SM_P,
+                 SM_ERR_ARG, "receiver %zu: component %d not recordable in 3D", r, comp[r]);
+    }
+    SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    SM_HIP(hipStreamSynchronize(s->ctx->stream));
+    dfree(s->rcv);
+    dfree(s->trace);
+    s->nrec = 0;
+    s->max_steps = 0;
+    if (nrec == 0 || max_steps == 0) return SM_OK;
+    int rc = dalloc(s->rcv, nrec * (sizeof(int64_t) + sizeof(int)));
+    if (rc == SM_OK) rc = dalloc(s->trace, nrec * max_steps * s->elt);
+    if (rc != SM_OK) { dfree(s->rcv); dfree(s->trace); return rc; }
+    std::vector<unsigned char> h(nrec * (sizeof(int64_t) + sizeof(int)));
+    int64_t *hi = (int64_t *)h.data();
+    int *hc = (int *)(hi + nrec);
+    for (size_t r = 0; r < nrec; ++r) { hi[r] = (int64_t)idx[r]; hc[r] = comp[r]; }
+    SM_HIP(hipMemcpyAsync(s->rcv.ptr, h.data(), h.size(), hipMemcpyHostToDevice, s->ctx->stream));
+    SM_HIP(hipMemsetAsync(s->trace.ptr, 0, s->trace.bytes, s->ctx->stream));
+    SM_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->nrec = nrec;
+    s->max_steps = max_steps;
+    s->rcv_step0 = s->step;
+    return SM_OK;
+}
+
+int sm_a3d_run(sm_a3d *s, size_t nsteps)
+{
+    SM_CHECK(s, SM_ERR_ARG, "sm_a3d_run: NULL solver");
+    if (nsteps == 0) return SM_OK;
+    SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    if (s->p.storage == SM_STORE_F64) return run_naive<double, true>(s, nsteps);
+    return run_naive<float, false>(s, nsteps);
+}
+
+int sm_a3d_current_step(const sm_a3d *s, size_t *it)
+{
+    SM_CHECK(s && it, SM_ERR_ARG, "sm_a3d_current_step: NULL argument");
+    *it = s->step;
+    return SM_OK;
+}
+
+int sm_a3d_get_traces(sm_a3d *s, size_t it0, size_t count, double *out)
+{
+    SM_CHECK(s && (out || count == 0), SM_ERR_ARG, "sm_a3d_get_traces: NULL argument");
+    SM_CHECK(it0 + count <= s->max_steps, SM_ERR_ARG,
+             "sm_a3d_get_traces: steps [%zu,%zu) beyond the %zu recorded", it0, it0 + count,
+             s->max_steps);
+    if (count == 0 || s->nrec == 0) return SM_OK;
+    SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    const size_t n = count * s->nrec;
+    const char *src = (const char *)s->trace.ptr + it0 * s->nrec * s->elt;
+    if (s->elt == 8) return read_back<double>(s, src, n, out);
+    return read_back<float>(s, src, n, out);
+}
+
+int sm_a3d_get_field(sm_a3d *s, int comp, double *out)
+{
+    SM_CHECK(s && out, SM_ERR_ARG, "sm_a3d_get_field: NULL argument");
+    int slot = comp == SM_P ? 0 : comp == SM_VX ? 1 : comp == SM_VY ? 2 : comp == SM_VZ ? 3 : -1;
+    SM_CHECK(slot >= 0, SM_ERR_ARG, "sm_a3d_get_field: component %d", comp);
+    SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    const char *src = (const char *)s->fields[s->cur].ptr + slot * s->nnn * s->elt;
+    if (s->elt == 8) return read_back<double>(s, src, s->nnn, out);
+    return read_back<float>(s, src, s->nnn, out);
+}
+
+int sm_a3d_get_slice_xz(sm_a3d *s, int comp, size_t iy, double *out)
+{
+    SM_CHECK(s && out, SM_ERR_ARG, "sm_a3d_get_slice_xz: NULL argument");
+    SM_CHECK(iy < (size_t)s->ny2, SM_ERR_ARG, "sm_a3d_get_slice_xz: iy %zu >= ny2", iy);
+    int slot = comp == SM_P ? 0 : comp == SM_VX ? 1 : comp == SM_VY ? 2 : comp == SM_VZ ? 3 : -1;
+    SM_CHECK(slot >= 0, SM_ERR_ARG, "sm_a3d_get_slice_xz: component %d", comp);
+    SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    const size_t e = s->elt, nz2 = s->nz2;
+    const char *base = (const char *)s->fields[s->cur].ptr + slot * s->nnn * e;
+    std::vector<unsigned char> tmp((size_t)s->nx2 * nz2 * e);
+    SM_HIP(hipMemcpy2DAsync(tmp.data(), nz2 * e, base + iy * nz2 * e, (size_t)s->ny2 * nz2 * e,
+                            nz2 * e, s->nx2, hipMemcpyDeviceToHost, s->ctx->stream));
+    SM_HIP(hipStreamSynchronize(s->ctx->stream));
+    const size_t n = (size_t)s->nx2 * nz2;
+    for (size_t q = 0; q < n; ++q)
+        out[q] = e == 8 ? ((const double *)tmp.data())[q] : (double)((const float *)tmp.data())[q];
+    return SM_OK;
+}
+
+int sm_a3d_reset(sm_a3d *s)
+{
+    SM_CHECK(s, SM_ERR_ARG, "sm_a3d_reset: NULL solver");
+    SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    return zero_state(s);
+}
+
+int sm_a3d_device_bytes(const sm_a3d *s, size_t *bytes)
+{
+    SM_CHECK(s && bytes, SM_ERR_ARG, "sm_a3d_device_bytes: NULL argument");
+    size_t b = s->mat.bytes + s->memp.bytes + s->tables.bytes + s->src.bytes + s->rcv.bytes +
+               s->trace.bytes;
+    for (int h = 0; h < 2; ++h) b += s->fields[h].bytes + s->memv[h].bytes;
+    *bytes = b;
+    return SM_OK;
+}
+
+}  // extern "C"

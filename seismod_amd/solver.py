@@ -1,0 +1,286 @@
+"""numpy-facing wrappers over the C ABI (used by tests, bench and the server).
+
+Arrays cross the boundary as FP64 numpy arrays in the reference layouts:
+2D index i*nz+j, 3D index (i*ny2+j)*nz2+k, z fastest.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _sp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_size_t))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def ricker_length(f: float, dt: float) -> int:
+    return int(lib.sm_ricker_length(f, dt))
+
+
+def source_functions(A, f, dt: float) -> tuple[np.ndarray, np.ndarray]:
+    """compute_src_fct (src.c:43-65): returns (lengths, concatenated fct)."""
+    A = _f64(np.atleast_1d(A))
+    f = _f64(np.atleast_1d(f))
+    lengths = np.array([ricker_length(fi, dt) for fi in f], dtype=np.uintp)
+    fct = np.zeros(int(lengths.sum()), dtype=np.float64)
+    check(lib.sm_compute_src_fct(len(A), _dp(A), _dp(f), dt, _sp(lengths), _dp(fct)),
+          "sm_compute_src_fct")
+    return lengths, fct
+
+
+def nsteps(duration: float, dt: float) -> int:
+    return int(lib.sm_nsteps(duration, dt))
+
+
+def record_due(it: int, dt: float, t0: float, rdt: float) -> bool:
+    return bool(lib.sm_record_due(it, dt, t0, rdt))
+
+
+class Context:
+    """One device + one HIP stream."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib.sm_ctx_create(device, C.byref(h)), "sm_ctx_create")
+        self.handle = h
+
+    def sync(self) -> None:
+        check(lib.sm_ctx_sync(self.handle), "sm_ctx_sync")
+
+    def mem_info(self) -> tuple[int, int]:
+        free, total = C.c_size_t(), C.c_size_t()
+        check(lib.sm_ctx_mem_info(self.handle, C.byref(free), C.byref(total)), "sm_ctx_mem_info")
+        return free.value, total.value
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib.sm_ctx_destroy(self.handle), "sm_ctx_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_STORAGE = {"f32": L.SM_STORE_F32, "f64": L.SM_STORE_F64, "bf16": L.SM_STORE_BF16,
+            "f16": L.SM_STORE_F16}
+
+
+@dataclass
+class A3dConfig:
+    nx: int
+    ny: int
+    nz: int
+    np: int
+    dx: float
+    dy: float
+    dz: float
+    dt: float
+    f0: float
+    vmax: float = 5000.0
+    pml_order: float = 2.0
+    Rc: float = 1e-5
+    kappa_max: float = 1.0
+    alpha_max: int = 1
+    storage: str = "f32"
+    naive: bool = False
+
+    def params(self) -> L.A3dParams:
+        return L.A3dParams(self.nx, self.ny, self.nz, self.np, self.dx, self.dy, self.dz,
+                           self.dt, self.vmax, self.pml_order, self.Rc, self.kappa_max,
+                           self.alpha_max, self.f0, _STORAGE[self.storage],
+                           L.SM_A3D_NAIVE if self.naive else 0)
+
+    @property
+    def shape(self) -> tuple[int, int, int]:
+        return (self.nx + 2 * self.np, self.ny + 2 * self.np, self.nz + 2 * self.np)
+
+
+class Acoustic3D:
+    """3D acoustic CPML solver (a_iso_3d)."""
+
+    def __init__(self, ctx: Context, cfg: A3dConfig, vp: np.ndarray, rho: np.ndarray):
+        self.ctx, self.cfg = ctx, cfg
+        n = int(np.prod(cfg.shape))
+        self._vp, self._rho = _f64(vp).reshape(-1), _f64(rho).reshape(-1)
+        if self._vp.size != n or self._rho.size != n:
+            raise ValueError(f"model must hold {n} padded cells")
+        self._p = cfg.params()
+        h = C.c_void_p()
+        check(lib.sm_a3d_create(ctx.handle, C.byref(self._p), _dp(self._vp), _dp(self._rho),
+                                C.byref(h)), "sm_a3d_create")
+        self.handle = h
+        self.nrec = 0
+
+    def set_sources(self, idx, lengths, fct) -> None:
+        idx = np.ascontiguousarray(idx, dtype=np.uintp)
+        lengths = np.ascontiguousarray(lengths, dtype=np.uintp)
+        fct = _f64(fct)
+        check(lib.sm_a3d_set_sources(self.handle, len(idx), _sp(idx), _sp(lengths), _dp(fct)),
+              "sm_a3d_set_sources")
+
+    def set_receivers(self, idx, comp, max_steps: int) -> None:
+        idx = np.ascontiguousarray(idx, dtype=np.uintp)
+        comp = np.ascontiguousarray(comp, dtype=np.intc)
+        check(lib.sm_a3d_set_receivers(self.handle, len(idx), _sp(idx), _ip(comp), max_steps),
+              "sm_a3d_set_receivers")
+        self.nrec = len(idx)
+
+    def run(self, nsteps: int) -> None:
+        check(lib.sm_a3d_run(self.handle, nsteps), "sm_a3d_run")
+
+    @property
+    def step(self) -> int:
+        it = C.c_size_t()
+        check(lib.sm_a3d_current_step(self.handle, C.byref(it)), "sm_a3d_current_step")
+        return it.value
+
+    def traces(self, it0: int, count: int) -> np.ndarray:
+        out = np.zeros((count, self.nrec), dtype=np.float64)
+        check(lib.sm_a3d_get_traces(self.handle, it0, count, _dp(out)), "sm_a3d_get_traces")
+        return out
+
+    def field(self, comp: int) -> np.ndarray:
+        out = np.zeros(self.cfg.shape, dtype=np.float64)
+        check(lib.sm_a3d_get_field(self.handle, comp, _dp(out)), "sm_a3d_get_field")
+        return out
+
+    def slice_xz(self, comp: int, iy: int) -> np.ndarray:
+        nx2, _, nz2 = self.cfg.shape
+        out = np.zeros((nx2, nz2), dtype=np.float64)
+        check(lib.sm_a3d_get_slice_xz(self.handle, comp, iy, _dp(out)), "sm_a3d_get_slice_xz")
+        return out
+
+    def reset(self) -> None:
+        check(lib.sm_a3d_reset(self.handle), "sm_a3d_reset")
+
+    def device_bytes(self) -> int:
+        b = C.c_size_t()
+        check(lib.sm_a3d_device_bytes(self.handle, C.byref(b)), "sm_a3d_device_bytes")
+        return b.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            check(lib.sm_a3d_destroy(self.handle), "sm_a3d_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class E2dConfig:
+    nx: int
+    nz: int
+    npml: int
+    h: float
+    dt: float
+    storage: str = "f32"
+    q1_compat: bool = True
+    damping_scale: float = 1.0
+
+    def params(self) -> L.E2dParams:
+        return L.E2dParams(self.nx, self.nz, self.npml, self.h, self.dt, _STORAGE[self.storage],
+                           1 if self.q1_compat else 0, self.damping_scale, 0)
+
+    @property
+    def padded(self) -> tuple[int, int]:
+        return (self.nx + 2 * self.npml, self.nz + 2 * self.npml)
+
+
+class Elastic2D:
+    """2D elastic split-PML solver (e_iso)."""
+
+    def __init__(self, ctx: Context, cfg: E2dConfig, vp, vs, rho):
+        self.ctx, self.cfg = ctx, cfg
+        n = cfg.nx * cfg.nz
+        self._m = [_f64(a).reshape(-1) for a in (vp, vs, rho)]
+        if any(a.size != n for a in self._m):
+            raise ValueError(f"model must hold {n} cells")
+        self._p = cfg.params()
+        h = C.c_void_p()
+        check(lib.sm_e2d_create(ctx.handle, C.byref(self._p), *(_dp(a) for a in self._m),
+                                C.byref(h)), "sm_e2d_create")
+        self.handle = h
+        self.nrec = 0
+
+    def set_sources(self, types, idx, lengths, fct) -> None:
+        types = np.ascontiguousarray(types, dtype=np.intc)
+        idx = np.ascontiguousarray(idx, dtype=np.uintp)
+        lengths = np.ascontiguousarray(lengths, dtype=np.uintp)
+        fct = _f64(fct)
+        check(lib.sm_e2d_set_sources(self.handle, len(idx), _ip(types), _sp(idx), _sp(lengths),
+                                     _dp(fct)), "sm_e2d_set_sources")
+
+    def set_receivers(self, idx, comp, max_steps: int) -> None:
+        idx = np.ascontiguousarray(idx, dtype=np.uintp)
+        comp = np.ascontiguousarray(comp, dtype=np.intc)
+        check(lib.sm_e2d_set_receivers(self.handle, len(idx), _sp(idx), _ip(comp), max_steps),
+              "sm_e2d_set_receivers")
+        self.nrec = len(idx)
+
+    def run(self, nsteps: int) -> None:
+        check(lib.sm_e2d_run(self.handle, nsteps), "sm_e2d_run")
+
+    @property
+    def step(self) -> int:
+        it = C.c_size_t()
+        check(lib.sm_e2d_current_step(self.handle, C.byref(it)), "sm_e2d_current_step")
+        return it.value
+
+    def traces(self, it0: int, count: int) -> np.ndarray:
+        out = np.zeros((count, self.nrec), dtype=np.float64)
+        check(lib.sm_e2d_get_traces(self.handle, it0, count, _dp(out)), "sm_e2d_get_traces")
+        return out
+
+    def field(self, comp: int) -> np.ndarray:
+        out = np.zeros((self.cfg.nx, self.cfg.nz), dtype=np.float64)
+        check(lib.sm_e2d_get_field(self.handle, comp, _dp(out)), "sm_e2d_get_field")
+        return out
+
+    def field_padded(self, comp: int) -> np.ndarray:
+        out = np.zeros(self.cfg.padded, dtype=np.float64)
+        check(lib.sm_e2d_get_field_padded(self.handle, comp, _dp(out)), "sm_e2d_get_field_padded")
+        return out
+
+    def reset(self) -> None:
+        check(lib.sm_e2d_reset(self.handle), "sm_e2d_reset")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            check(lib.sm_e2d_destroy(self.handle), "sm_e2d_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
