@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark: 3D acoustic CPML FDTD (a_iso_3d) on one MI355X.
+
+One bench step is one whole shot of the workload: zero the wavefields,
+run every time step (velocity, pressure, CPML, source, receivers), and copy
+the receiver traces back.  The model and geometry are resident in HBM
+before the timed region starts.
+
+Default workload (a_iso_3d 256^3): 256^3 interior nodes + 20 CPML layers
+(296^3 padded cells), dx = 10 m, seeded layered Vp 1500-4500 m/s with a
+Gaussian-smoothed (sigma 8 cells) +-5% perturbation, Gardner density,
+15 Hz Ricker Bulk source at (1280, 1280, 100) m, 32x32 Vz receivers at
+z = 50 m, dt = 0.6 ms, 1500 steps, FP32.
+
+Prints one JSON line.  value = padded cells x time steps x shots / wall
+time of the timed shots (Gcell-updates/s).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Gcell-updates/s & %HBM roofline, 3D acoustic CPML; p50/p99 shot latency"
+HBM_PEAK_GBS = 8000.0
+STATE_BYTES = {"f32": 4, "bf16": 2, "f16": 2}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5, help="timed shots")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
+    ap.add_argument("--n", type=int, default=256, help="interior nodes per axis")
+    ap.add_argument("--nsteps", type=int, default=None, help="time steps per shot")
+    ap.add_argument("--storage", default="f32", choices=sorted(STATE_BYTES))
+    ap.add_argument("--launch", default=None, help="rows_per_block,x_chunks")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    return ap.parse_args()
+
+
+def cpu_baseline(shot, nsteps_sample: int) -> dict:
+    """Oracle (FP64 C restatement, the reference's OpenMP shape: 3 threads for
+    the velocity sections, one for pressure) on the same model; time per step
+    from the difference of two runs so set-up is excluded."""
+    from oracle import pyoracle as O
+    cfg = dict(nx=shot.nx, ny=shot.ny, nz=shot.nz, np=shot.np, dx=shot.dx, dy=shot.dx,
+               dz=shot.dx, dt=shot.dt, f0=shot.f0)
+    L, fct = O.source_functions([1.0], [shot.f0], shot.dt)
+    times = []
+    for n in (2, 2 + nsteps_sample):
+        t0 = time.perf_counter()
+        O.a3d_run(cfg, shot.vp, shot.rho, shot.src_idx, L, fct, shot.rec_idx[:1],
+                  shot.rec_comp[:1], n, want_fields=False, nthreads=3, pressure_threads=1)
+        times.append(time.perf_counter() - t0)
+    per_step = max(times[1] - times[0], 1e-9) / nsteps_sample
+    return {"value": shot.cells / per_step / 1e9, "unit": "Gcell-updates/s", "cores": 3,
+            "kind": "port",
+            "sample": f"{shot.shape[0]}^3 padded model, {nsteps_sample} time steps (difference "
+                      f"of a {2 + nsteps_sample}-step and a 2-step run), FP64, OpenMP sections "
+                      f"as a_iso_3d.c:585 (3 threads velocity, 1 pressure)"}
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    p = ROOT / "profiles" / f"{name}.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(shots would shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from seismod_amd import synth
+    from seismod_amd.solver import A3dConfig, Acoustic3D, Context, source_functions
+
+    nsteps = args.nsteps or (1500 if args.n <= 256 else 2000)
+    dt = 6.0e-4 if args.n <= 256 else 5.5e-4
+    shot = synth.a3d_layered_shot(n=args.n, seed=2 if args.n <= 256 else 3, dt=dt, nsteps=nsteps,
+                                  vmax=4500.0 if args.n <= 256 else 5000.0)
+    ctx = Context(0)
+    cfg = A3dConfig(nx=shot.nx, ny=shot.ny, nz=shot.nz, np=shot.np, dx=shot.dx, dy=shot.dx,
+                    dz=shot.dx, dt=shot.dt, f0=shot.f0, storage=args.storage)
+    sim = Acoustic3D(ctx, cfg, shot.vp, shot.rho)
+    if args.launch:
+        tj, ch = (int(x) for x in args.launch.split(","))
+        sim.set_launch(tj, ch)
+    lengths, fct = source_functions([1.0], [shot.f0], shot.dt)
+    sim.set_sources(shot.src_idx, lengths, fct)
+
+    sim.set_receivers(shot.rec_idx, shot.rec_comp, nsteps)
+
+    def one_shot():
+        sim.reset()            # zero wavefields, memory variables and traces
+        sim.run(nsteps)
+        return sim.traces(0, nsteps)
+
+    for _ in range(args.warmup):
+        one_shot()
+    ctx.sync()
+    sim.set_timing(True)
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        tr = one_shot()
+        lat.append(time.perf_counter() - t0)
+    ctx.sync()
+    wall = time.perf_counter() - t_start
+    kms, launches = sim.timing()
+    sim.set_timing(False)
+    if not np.isfinite(tr).all() or np.abs(tr).max() == 0.0:
+        print("bench.py: traces are not finite or all zero", file=sys.stderr)
+        sys.exit(1)
+
+    cells = shot.cells
+    value = cells * nsteps * args.steps / wall / 1e9
+    nx2, ny2, nz2 = shot.shape
+    sb = STATE_BYTES[args.storage]
+    alg_b = synth.algorithmic_bytes_a3d(nx2, ny2, nz2, shot.np, state_bytes=sb, mat_bytes=4)
+    avg_ms = kms / max(launches, 1)
+    achieved = alg_b * cells / (avg_ms * 1e-3) / 1e9
+    kname = f"a3d_fused_{args.storage}_{args.n}"
+    tj, ch = sim.launch()
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.storage == "f32" else f"f32 compute, {args.storage} storage",
+        "data": "synthetic (seeded layered model, Gardner density)",
+        "config": {
+            "workload": f"a_iso_3d {args.n}^3 + CPML {shot.np}, {nsteps} time steps per shot, "
+                        f"{len(shot.rec_idx)} Vz receivers, {args.storage} storage",
+            "padded_grid": [nx2, ny2, nz2],
+            "time_steps_per_shot": nsteps,
+            "dt_s": shot.dt,
+            "launch": {"rows_per_block": tj, "x_chunks": ch},
+            "parallelism": "single GPU",
+        },
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "a3d_fused_step",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic(kname),
+            "algorithmic_bytes_per_cell": round(alg_b, 3),
+            "kernel_ms_avg": round(avg_ms, 5),
+            "launches_timed": launches,
+        },
+        "cpu_baseline": None,
+    }
+    sim.close()
+    ctx.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(shot, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

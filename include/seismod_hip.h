@@ -85,6 +85,12 @@ int sm_ctx_destroy(sm_ctx *ctx);
 int sm_ctx_sync(sm_ctx *ctx);
 /* free and total device memory (hipMemGetInfo) of the context's device */
 int sm_ctx_mem_info(sm_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
+/* Calibration: a plain streaming kernel reading `reads` and writing
+ * `writes` FP32 arrays of n values (known traffic 4n(reads+writes) bytes);
+ * returns the mean time of `iters` launches.  No counterpart in the
+ * reference: it scales HBM counters and measures achievable bandwidth. */
+int sm_probe_stream(sm_ctx *ctx, size_t n, int reads, int writes, int iters,
+                    double *ms_per_iter);
 
 /* ------------------------------------------------------------------ */
 /* host-side FP64 set-up shared by both programs                       */
@@ -157,6 +163,16 @@ int sm_a3d_get_field(sm_a3d *s, int comp, double *out);
 int sm_a3d_get_slice_xz(sm_a3d *s, int comp, size_t iy, double *out);
 /* zero wavefields, memory variables and recorded traces; step count -> 0 */
 int sm_a3d_reset(sm_a3d *s);
+/* Launch shape of the fused FP32/BF16/FP16 kernel: waves per block along y
+ * (1, 2, 4 or 8) and number of x chunks marched in parallel.  Results do
+ * not depend on it; the autotuner (seismod_amd/autotune.py) picks it. */
+int sm_a3d_set_launch(sm_a3d *s, int rows_per_block, int x_chunks);
+int sm_a3d_get_launch(const sm_a3d *s, int *rows_per_block, int *x_chunks);
+/* Per-launch timing: when on, every step kernel launch is bracketed by HIP
+ * events on the context stream; get_timing returns the summed kernel time
+ * and the number of timed launches since timing was switched on. */
+int sm_a3d_set_timing(sm_a3d *s, int on);
+int sm_a3d_get_timing(const sm_a3d *s, double *ms_total, size_t *launches);
 /* device bytes this solver holds, and the bytes sm_a3d_create would need
  * for parameters p (for admission control) */
 int sm_a3d_device_bytes(const sm_a3d *s, size_t *bytes);

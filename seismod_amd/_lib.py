@@ -48,6 +48,7 @@ _SIGS = {
     "sm_ctx_destroy": (C.c_int, [C.c_void_p]),
     "sm_ctx_sync": (C.c_int, [C.c_void_p]),
     "sm_ctx_mem_info": (C.c_int, [C.c_void_p, _size_p, _size_p]),
+    "sm_probe_stream": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, _dbl_p]),
     "sm_ricker_length": (C.c_size_t, [C.c_double, C.c_double]),
     "sm_compute_src_fct": (C.c_int, [C.c_size_t, _dbl_p, _dbl_p, C.c_double, _size_p, _dbl_p]),
     "sm_nsteps": (C.c_size_t, [C.c_double, C.c_double]),
@@ -63,6 +64,10 @@ _SIGS = {
     "sm_a3d_get_field": (C.c_int, [C.c_void_p, C.c_int, _dbl_p]),
     "sm_a3d_get_slice_xz": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, _dbl_p]),
     "sm_a3d_reset": (C.c_int, [C.c_void_p]),
+    "sm_a3d_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "sm_a3d_get_launch": (C.c_int, [C.c_void_p, _int_p, _int_p]),
+    "sm_a3d_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "sm_a3d_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),
     "sm_a3d_device_bytes": (C.c_int, [C.c_void_p, _size_p]),
     "sm_a3d_bytes_needed": (C.c_size_t, [C.POINTER(A3dParams), C.c_size_t, C.c_size_t]),
     "sm_e2d_create": (C.c_int, [C.c_void_p, C.POINTER(E2dParams), _dbl_p, _dbl_p, _dbl_p,

@@ -5,11 +5,13 @@
 // Device side: the "naive" two-kernel step (velocity sweep, pressure sweep)
 // in FP64 (reference operation order) and FP32, plus the fused single-pass
 // step in a3d_fused.hip.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
 
 #include "a3d_common.hpp"
+#include "a3d_fused.hpp"
 #include "sm_internal.hpp"
 
 namespace sm {
@@ -144,7 +146,9 @@ struct sm_a3d {
     sm_a3d_params p{};
     int nx2 = 0, ny2 = 0, nz2 = 0;
     size_t nnn = 0;
-    size_t elt = 0;                 // bytes per stored value
+    size_t elt = 0;                 // bytes per stored state value
+    size_t melt = 0;                // bytes per material / table value
+    size_t telt = 0;                // bytes per recorded trace value
     bool fused = false;
     // device memory
     sm_dbuf fields[2];              // P, vx, vy, vz (ping-pong when fused)
@@ -152,15 +156,21 @@ struct sm_a3d {
     sm_dbuf memv[2];                // mPx, mPy, mPz (ping-pong when fused)
     sm_dbuf memp;                   // mvx, mvy, mvz
     sm_dbuf tables;                 // 18 CPML tables
-    sm_dbuf src;                    // idx, off, len (int64) + inc (T)
+    sm_dbuf src;                    // cell, off, len (int64) | plane offsets (int) | increments
     sm_dbuf rcv;                    // idx (int64), comp (int)
     sm_dbuf trace;                  // [max_steps][nrec]
     int cur = 0;                    // which ping-pong half holds the state
     size_t step = 0;
     size_t nsrc = 0, nrec = 0, max_steps = 0, rcv_step0 = 0;
-    size_t src_total = 0;
-    // host copies of the FP64 tables (exposed to tests through get_tables)
-    std::vector<double> htab;
+    size_t src_plane_off = 0, src_inc_off = 0;   // byte offsets inside src
+    int tj = 4, nchunks = 1;        // fused launch shape
+    int nblocks = 0;                // persistent grid (0: one block per item)
+    int cus = 0;                    // compute units of the device
+    float sP = 1.f, sV = 1.f, sM = 1.f;          // storage scales of 16-bit formats
+    bool timing = false;
+    double timed_ms = 0.0;
+    size_t timed_launches = 0;
+    std::vector<double> htab;       // FP64 CPML tables as built on the host
 };
 
 namespace {
@@ -202,12 +212,70 @@ void cpml_axis(size_t n2, size_t np, double d, double dt, double vmax, double or
 }
 
 template <typename T>
-int upload(void *dst, const double *src, size_t n, hipStream_t st, bool invert = false)
+int upload(void *dst, const double *src, size_t n, hipStream_t st)
 {
     std::vector<T> tmp(n);
-    for (size_t i = 0; i < n; ++i) tmp[i] = (T)(invert ? 1.0 / src[i] : src[i]);
+    for (size_t i = 0; i < n; ++i) tmp[i] = (T)src[i];
     SM_HIP(hipMemcpyAsync(dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice, st));
     SM_HIP(hipStreamSynchronize(st));
+    return SM_OK;
+}
+
+// host conversions of the 16-bit storage formats
+float bf16_to_float(uint16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+float half_to_float(uint16_t h)
+{
+    const uint32_t sign = (uint32_t)(h >> 15) << 31;
+    uint32_t exp = (h >> 10) & 0x1f, man = h & 0x3ff, u;
+    if (exp == 0) {
+        if (man == 0) u = sign;
+        else {  // subnormal: renormalise
+            int e = -1;
+            do { man <<= 1; ++e; } while (!(man & 0x400));
+            u = sign | ((uint32_t)(127 - 15 - e) << 23) | ((man & 0x3ff) << 13);
+        }
+    } else if (exp == 31) u = sign | 0x7f800000u | (man << 13);
+    else u = sign | ((exp + 127 - 15) << 23) | (man << 13);
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// state values on the host as double, scaled back to physical units
+int state_to_host(sm_a3d *s, const void *dev, size_t n, size_t stride_elems, size_t count,
+                  size_t pitch_elems, float scale, double *out)
+{
+    // copies `count` rows of n values, rows `pitch_elems` apart on the device
+    const size_t e = s->elt;
+    std::vector<unsigned char> tmp(n * count * e);
+    SM_HIP(hipMemcpy2DAsync(tmp.data(), n * e, dev, pitch_elems * e, n * e, count,
+                            hipMemcpyDeviceToHost, s->ctx->stream));
+    SM_HIP(hipStreamSynchronize(s->ctx->stream));
+    (void)stride_elems;
+    const size_t tot = n * count;
+    switch (s->p.storage) {
+    case SM_STORE_F64:
+        std::memcpy(out, tmp.data(), tot * 8);
+        break;
+    case SM_STORE_F32:
+        for (size_t q = 0; q < tot; ++q) out[q] = ((const float *)tmp.data())[q];
+        break;
+    case SM_STORE_BF16:
+        for (size_t q = 0; q < tot; ++q)
+            out[q] = (double)bf16_to_float(((const uint16_t *)tmp.data())[q]) * scale;
+        break;
+    default:
+        for (size_t q = 0; q < tot; ++q)
+            out[q] = (double)half_to_float(((const uint16_t *)tmp.data())[q]) * scale;
+        break;
+    }
     return SM_OK;
 }
 
@@ -240,6 +308,45 @@ void axes(sm_a3d *s, CpmlAxis<T> ax[3])
     }
 }
 
+int64_t rcv_row(const sm_a3d *s, size_t it)
+{
+    if (s->nrec == 0 || it < s->rcv_step0 || it - s->rcv_step0 >= s->max_steps) return -1;
+    return (int64_t)(it - s->rcv_step0);
+}
+
+// per-launch timing with HIP events on the solver's stream
+struct LaunchTimer {
+    std::vector<hipEvent_t> ev;
+    bool on = false;
+    int begin(size_t n, bool enabled)
+    {
+        on = enabled;
+        if (!on) return SM_OK;
+        ev.resize(2 * n);
+        for (auto &e : ev) SM_HIP(hipEventCreate(&e));
+        return SM_OK;
+    }
+    int mark(hipStream_t st, size_t q)
+    {
+        if (on) SM_HIP(hipEventRecord(ev[q], st));
+        return SM_OK;
+    }
+    int finish(sm_a3d *s)
+    {
+        if (!on) return SM_OK;
+        SM_HIP(hipStreamSynchronize(s->ctx->stream));
+        for (size_t q = 0; q + 1 < ev.size(); q += 2) {
+            float ms = 0.f;
+            SM_HIP(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+            s->timed_ms += ms;
+            s->timed_launches += 1;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        return SM_OK;
+    }
+};
+
 template <typename T, bool EXACT>
 int run_naive(sm_a3d *s, size_t nsteps)
 {
@@ -257,16 +364,19 @@ int run_naive(sm_a3d *s, size_t nsteps)
     const dim3 blk(64, 4, 1);
     const dim3 grd((s->nz2 + 63) / 64, (s->ny2 + 3) / 4, s->nx2);
     const int64_t *sp = (const int64_t *)s->src.ptr;
-    const T *sinc = (const T *)(sp + 3 * s->nsrc);
+    const T *sinc = (const T *)((const char *)s->src.ptr + s->src_inc_off);
     const int64_t *ridx = (const int64_t *)s->rcv.ptr;
     const int *rcomp = (const int *)(ridx + s->nrec);
+    LaunchTimer tm;
+    int rc = tm.begin(nsteps, s->timing);
+    if (rc != SM_OK) return rc;
     for (size_t n = 0; n < nsteps; ++n) {
         const size_t it = s->step + n;
+        if ((rc = tm.mark(st, 2 * n)) != SM_OK) return rc;
         a3d_vel_naive<T, EXACT><<<grd, blk, 0, st>>>(d, ax[0], ax[1], ax[2], k);
         a3d_pre_naive<T, EXACT><<<grd, blk, 0, st>>>(d, ax[0], ax[1], ax[2], k);
-        int64_t row = -1;
-        if (s->nrec && it >= s->rcv_step0 && it - s->rcv_step0 < s->max_steps)
-            row = (int64_t)(it - s->rcv_step0);
+        if ((rc = tm.mark(st, 2 * n + 1)) != SM_OK) return rc;
+        const int64_t row = rcv_row(s, it);
         if (s->nsrc || row >= 0)
             a3d_src_rcv<T><<<1, 256, 0, st>>>(d, (int)s->nsrc, sp, sp + s->nsrc, sp + 2 * s->nsrc,
                                               sinc, (int)s->nrec, ridx, rcomp, (T *)s->trace.ptr,
@@ -274,17 +384,93 @@ int run_naive(sm_a3d *s, size_t nsteps)
     }
     SM_HIP(hipGetLastError());
     s->step += nsteps;
-    return SM_OK;
+    return tm.finish(s);
 }
 
-template <typename T>
-int read_back(sm_a3d *s, const void *dev, size_t n, double *out)
+template <typename S>
+FusedArgs<S> fused_args(sm_a3d *s)
 {
-    std::vector<T> tmp(n);
-    SM_HIP(hipMemcpyAsync(tmp.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost, s->ctx->stream));
-    SM_HIP(hipStreamSynchronize(s->ctx->stream));
-    for (size_t i = 0; i < n; ++i) out[i] = (double)tmp[i];
-    return SM_OK;
+    FusedArgs<S> a{};
+    a.nx2 = s->nx2; a.ny2 = s->ny2; a.nz2 = s->nz2; a.np = (int)s->p.np;
+    a.ntk = (s->nz2 + 63) / 64;
+    a.ntiles = a.ntk * ((s->ny2 + s->tj - 1) / s->tj);
+    a.chunk_len = (s->nx2 + s->nchunks - 1) / s->nchunks;
+    a.nreal = a.ntiles * s->nchunks;
+    a.nitems = (a.nreal + 7) / 8 * 8;
+    a.nblocks = s->nblocks > 0 ? std::min(s->nblocks, a.nitems) : a.nitems;
+    const float *m = (const float *)s->mat.ptr;
+    a.rho = m; a.kdt = m + s->nnn;
+    CpmlAxis<float> ax[3];
+    axes<float>(s, ax);
+    a.ax = ax[0].a; a.bx = ax[0].b; a.ikx = ax[0].k;
+    a.ahx = ax[0].ah; a.bhx = ax[0].bh; a.ikhx = ax[0].kh;
+    a.ay = ax[1].a; a.by = ax[1].b; a.iky = ax[1].k;
+    a.ahy = ax[1].ah; a.bhy = ax[1].bh; a.ikhy = ax[1].kh;
+    a.az = ax[2].a; a.bz = ax[2].b; a.ikz = ax[2].k;
+    a.ahz = ax[2].ah; a.bhz = ax[2].bh; a.ikhz = ax[2].kh;
+    a.idx = (float)(1.0 / s->p.dx); a.idy = (float)(1.0 / s->p.dy); a.idz = (float)(1.0 / s->p.dz);
+    a.two_dt = (float)(2.0 * s->p.dt);
+    a.sP = s->sP; a.isP = 1.f / s->sP; a.sV = s->sV; a.isV = 1.f / s->sV;
+    a.sM = s->sM; a.isM = 1.f / s->sM;
+    const char *sb = (const char *)s->src.ptr;
+    const int64_t *sp = (const int64_t *)sb;
+    a.src_cell = sp; a.src_off = sp + s->nsrc; a.src_len = sp + 2 * s->nsrc;
+    a.src_plane = (const int *)(sb + s->src_plane_off);
+    a.src_inc = (const float *)(sb + s->src_inc_off);
+    a.nrec = (int)s->nrec;
+    a.ridx = (const int64_t *)s->rcv.ptr;
+    a.rcomp = (const int *)(a.ridx + s->nrec);
+    a.trace = (float *)s->trace.ptr;
+    const size_t np2 = 2 * s->p.np;
+    S *mp = (S *)s->memp.ptr;
+    a.mvx = mp; a.mvy = mp + np2 * s->ny2 * s->nz2; a.mvz = a.mvy + s->nx2 * np2 * s->nz2;
+    return a;
+}
+
+template <typename S>
+void fused_bind(sm_a3d *s, FusedArgs<S> &a, int in)
+{
+    const size_t n = s->nnn, np2 = 2 * s->p.np;
+    const S *fi = (const S *)s->fields[in].ptr;
+    S *fo = (S *)s->fields[1 - in].ptr;
+    a.P_in = fi; a.vx_in = fi + n; a.vy_in = fi + 2 * n; a.vz_in = fi + 3 * n;
+    a.P_out = fo; a.vx_out = fo + n; a.vy_out = fo + 2 * n; a.vz_out = fo + 3 * n;
+    const S *mi = (const S *)s->memv[in].ptr;
+    S *mo = (S *)s->memv[1 - in].ptr;
+    const size_t sx = np2 * s->ny2 * s->nz2, sy = s->nx2 * np2 * s->nz2;
+    a.mPx_in = mi; a.mPy_in = mi + sx; a.mPz_in = mi + sx + sy;
+    a.mPx_out = mo; a.mPy_out = mo + sx; a.mPz_out = mo + sx + sy;
+}
+
+template <typename S>
+int run_fused(sm_a3d *s, size_t nsteps)
+{
+    hipStream_t st = s->ctx->stream;
+    FusedArgs<S> a = fused_args<S>(s);
+    const int per_gather = 64 * s->tj;
+    LaunchTimer tm;
+    int rc = tm.begin(nsteps, s->timing);
+    if (rc != SM_OK) return rc;
+    for (size_t n = 0; n < nsteps; ++n) {
+        const size_t it = s->step + n;
+        fused_bind<S>(s, a, s->cur);
+        a.step = (int64_t)it;
+        // receivers of the previous step, unless an earlier run() already did
+        a.gather_row = (n > 0 && it > 0) ? rcv_row(s, it - 1) : -1;
+        a.gather_blocks = a.gather_row >= 0 ? (int)((s->nrec + per_gather - 1) / per_gather) : 0;
+        if ((rc = tm.mark(st, 2 * n)) != SM_OK) return rc;
+        if ((rc = fused_launch<S>(a, s->tj, st)) != SM_OK)
+            return fail(rc, "fused launch failed (tj=%d): %s", s->tj,
+                        hipGetErrorString(hipGetLastError()));
+        if ((rc = tm.mark(st, 2 * n + 1)) != SM_OK) return rc;
+        s->cur = 1 - s->cur;
+    }
+    // receivers of the last step, from the state it produced
+    fused_bind<S>(s, a, s->cur);
+    a.gather_row = rcv_row(s, s->step + nsteps - 1);
+    if ((rc = gather_launch<S>(a, st)) != SM_OK) return fail(rc, "gather launch failed");
+    s->step += nsteps;
+    return tm.finish(s);
 }
 
 int zero_state(sm_a3d *s)
@@ -314,8 +500,50 @@ size_t elt_size(int storage)
     switch (storage) {
     case SM_STORE_F32: return 4;
     case SM_STORE_F64: return 8;
+    case SM_STORE_BF16: return 2;
+    case SM_STORE_F16: return 2;
     default: return 0;
     }
+}
+
+bool uses_fused(const sm_a3d_params &p)
+{
+    if (p.storage == SM_STORE_F64) return false;
+    if (p.storage == SM_STORE_F32) return !(p.flags & SM_A3D_NAIVE);
+    return true;   // 16-bit storage exists only in the fused kernel
+}
+
+int resident_blocks(sm_a3d *s, int tj)
+{
+    int per_cu = 0, rc = SM_OK;
+    switch (s->p.storage) {
+    case SM_STORE_F32: rc = fused_max_blocks_per_cu<float>(tj, &per_cu); break;
+    case SM_STORE_BF16: rc = fused_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu); break;
+    default: rc = fused_max_blocks_per_cu<__half>(tj, &per_cu); break;
+    }
+    if (rc != SM_OK || per_cu < 1) per_cu = 1;
+    return per_cu * std::max(s->cus, 1);
+}
+
+// chunk count with no empty chunk
+int valid_chunks(int nx2, int want)
+{
+    want = std::max(1, std::min(want, nx2));
+    const int len = (nx2 + want - 1) / want;
+    return (nx2 + len - 1) / len;
+}
+
+// default launch shape: 4 waves per block; x chunks so that every resident
+// block slot gets about three items; persistent grid of resident size
+void default_launch(sm_a3d *s)
+{
+    s->tj = 4;
+    const int ntiles = ((s->nz2 + 63) / 64) * ((s->ny2 + s->tj - 1) / s->tj);
+    const int slots = resident_blocks(s, s->tj);
+    int nch = (3 * slots + ntiles - 1) / ntiles;
+    nch = std::min(nch, std::max(1, s->nx2 / 24));
+    s->nchunks = valid_chunks(s->nx2, nch);
+    s->nblocks = slots;
 }
 
 }  // namespace
@@ -327,13 +555,13 @@ size_t sm_a3d_bytes_needed(const sm_a3d_params *p, size_t nrec, size_t max_steps
     if (!p) return 0;
     const size_t e = elt_size(p->storage);
     if (!e) return 0;
+    const size_t me = p->storage == SM_STORE_F64 ? 8 : 4;
     const size_t nx2 = p->nx + 2 * p->np, ny2 = p->ny + 2 * p->np, nz2 = p->nz + 2 * p->np;
     const size_t nnn = nx2 * ny2 * nz2;
-    const bool fused = false;
-    const size_t halves = fused ? 2 : 1;
+    const size_t halves = uses_fused(*p) ? 2 : 1;
     const size_t slabs = slab_cells(nx2, ny2, nz2, p->np);
-    return e * (halves * 4 * nnn + 2 * nnn + (halves + 1) * slabs +
-                6 * (nx2 + ny2 + nz2) + nrec * max_steps) + 16 * nrec;
+    return e * (halves * 4 * nnn + (halves + 1) * slabs) + me * (2 * nnn + 6 * (nx2 + ny2 + nz2)) +
+           me * nrec * max_steps + 16 * nrec + 4 * nx2;
 }
 
 int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const double *rho,
@@ -349,8 +577,7 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
     const size_t e = elt_size(p.storage);
     SM_CHECK(e != 0, SM_ERR_ARG, "sm_a3d_create: storage %d not supported", p.storage);
     const size_t nx2 = p.nx + 2 * p.np, ny2 = p.ny + 2 * p.np, nz2 = p.nz + 2 * p.np;
-    SM_CHECK(nx2 < 65536 && ny2 < (1u << 30) && nz2 < (1u << 30), SM_ERR_ARG,
-             "sm_a3d_create: grid too large");
+    SM_CHECK(nx2 < 65536 && ny2 < 65536 && nz2 < 65536, SM_ERR_ARG, "sm_a3d_create: grid too large");
     const size_t nnn = nx2 * ny2 * nz2;
     SM_CHECK(nnn < ((size_t)1 << 34), SM_ERR_ARG, "sm_a3d_create: grid too large");
 
@@ -377,7 +604,15 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
     s->nx2 = (int)nx2; s->ny2 = (int)ny2; s->nz2 = (int)nz2;
     s->nnn = nnn;
     s->elt = e;
-    s->fused = false;
+    s->melt = p.storage == SM_STORE_F64 ? 8 : 4;
+    s->telt = s->melt;
+    s->fused = uses_fused(p);
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess)
+            s->cus = prop.multiProcessorCount;
+    }
+    if (s->fused) default_launch(s);
     // one set (velocity or pressure) of x, y and z slab arrays
     const size_t slab_set = slab_cells(nx2, ny2, nz2, p.np);
     int rc = SM_OK;
@@ -386,12 +621,12 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
         rc = dalloc(s->fields[h], 4 * nnn * e);
         if (rc == SM_OK) rc = dalloc(s->memv[h], slab_set * e);
     }
-    if (rc == SM_OK) rc = dalloc(s->mat, 2 * nnn * e);
+    if (rc == SM_OK) rc = dalloc(s->mat, 2 * nnn * s->melt);
     if (rc == SM_OK) rc = dalloc(s->memp, slab_set * e);
-    if (rc == SM_OK) rc = dalloc(s->tables, 6 * (nx2 + ny2 + nz2) * e);
+    if (rc == SM_OK) rc = dalloc(s->tables, 6 * (nx2 + ny2 + nz2) * s->melt);
     if (rc != SM_OK) { release(s); delete s; return rc; }
     {
-        // extent of the z slab set as dev_view lays it out: must end inside
+        // extent of the slab set as dev_view lays it out: must end inside
         // the allocation (x: 2np*ny2*nz2, y: nx2*2np*nz2, z: nx2*ny2*2np)
         const size_t np2 = 2 * p.np;
         const size_t end = np2 * ny2 * nz2 + nx2 * np2 * nz2 + nx2 * ny2 * np2;
@@ -412,6 +647,16 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
         } else {
             rc = upload<float>(s->mat.ptr, rho, nnn, st);
             if (rc == SM_OK) rc = upload<float>((float *)s->mat.ptr + nnn, kdt.data(), nnn, st);
+        }
+        // 16-bit storage: power-of-two scales so the expected magnitudes sit
+        // well inside the format (P in Pa ~ 1e3..1e5, v ~ P/(rho c))
+        if (p.storage == SM_STORE_F16) {
+            double rc_min = 1e300;
+            for (size_t n = 0; n < nnn; ++n) rc_min = std::min(rc_min, rho[n] * std::max(vp[n], 1.0));
+            s->sP = std::ldexp(1.0f, 6);                      // P up to 4e6 Pa
+            const double vmax_est = 4.0e6 / rc_min;           // matching particle velocity
+            s->sV = std::ldexp(1.0f, (int)std::ceil(std::log2(vmax_est / 3.0e4)));
+            s->sM = std::ldexp(1.0f, (int)std::ceil(std::log2(vmax_est / p.dx / 3.0e4)));
         }
     }
     // CPML tables (a_iso_3d.c:525-576)
@@ -443,6 +688,8 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
         }
     }
     if (rc == SM_OK) rc = zero_state(s);
+    // an empty source set still needs the per-plane offsets of the fused path
+    if (rc == SM_OK) rc = sm_a3d_set_sources(s, 0, nullptr, nullptr, nullptr);
     if (rc != SM_OK) { release(s); delete s; return rc; }
     *out = s;
     return SM_OK;
@@ -473,29 +720,43 @@ int sm_a3d_set_sources(sm_a3d *s, size_t nsrc, const size_t *idx, const size_t *
     SM_HIP(hipStreamSynchronize(s->ctx->stream));
     dfree(s->src);
     s->nsrc = 0;
-    if (nsrc == 0) return SM_OK;
-    const size_t bytes = 3 * nsrc * sizeof(int64_t) + total * s->elt;
+    // sources in cell order, stable, so several sources on one cell add in
+    // the reference's order (a_iso_3d.c:689-693)
+    std::vector<size_t> order(nsrc), offs(nsrc);
+    size_t off = 0;
+    for (size_t n = 0; n < nsrc; ++n) { order[n] = n; offs[n] = off; off += length[n]; }
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return idx[a] < idx[b]; });
+    const size_t plane = (size_t)s->ny2 * s->nz2;
+    const size_t head = 3 * nsrc * sizeof(int64_t);
+    const size_t plane_bytes = ((size_t)s->nx2 + 1) * sizeof(int);
+    s->src_plane_off = head;
+    s->src_inc_off = (head + plane_bytes + 7) / 8 * 8;
+    const size_t bytes = s->src_inc_off + total * s->melt;
     int rc = dalloc(s->src, bytes);
     if (rc != SM_OK) return rc;
-    std::vector<unsigned char> h(bytes);
+    std::vector<unsigned char> h(bytes, 0);
     int64_t *hi = (int64_t *)h.data();
-    size_t off = 0;
-    for (size_t n = 0; n < nsrc; ++n) {
-        hi[n] = (int64_t)idx[n];
-        hi[nsrc + n] = (int64_t)off;
-        hi[2 * nsrc + n] = (int64_t)length[n];
-        off += length[n];
+    int *hp = (int *)(h.data() + s->src_plane_off);
+    size_t q = 0;
+    for (size_t r = 0; r < nsrc; ++r) {
+        const size_t n = order[r];
+        hi[r] = (int64_t)idx[n];
+        hi[nsrc + r] = (int64_t)q;
+        hi[2 * nsrc + r] = (int64_t)length[n];
+        // increment dt*fct*1e6 (a_iso_3d.c:691), rounded once to the working type
+        for (size_t t = 0; t < length[n]; ++t, ++q) {
+            const double inc = s->p.dt * fct[offs[n] + t] * 1.e6;
+            if (s->melt == 8) ((double *)(h.data() + s->src_inc_off))[q] = inc;
+            else ((float *)(h.data() + s->src_inc_off))[q] = (float)inc;
+        }
     }
-    // increment dt*fct*1e6 (a_iso_3d.c:691), rounded once to the storage type
-    for (size_t q = 0; q < total; ++q) {
-        const double inc = s->p.dt * fct[q] * 1.e6;
-        if (s->elt == 8) ((double *)(hi + 3 * nsrc))[q] = inc;
-        else ((float *)(hi + 3 * nsrc))[q] = (float)inc;
+    for (int i = 0, r = 0; i <= s->nx2; ++i) {
+        while (r < (int)nsrc && (size_t)hi[r] / plane < (size_t)i) ++r;
+        hp[i] = r;
     }
     SM_HIP(hipMemcpyAsync(s->src.ptr, h.data(), bytes, hipMemcpyHostToDevice, s->ctx->stream));
     SM_HIP(hipStreamSynchronize(s->ctx->stream));
     s->nsrc = nsrc;
-    s->src_total = total;
     return SM_OK;
 }
 
@@ -518,7 +779,7 @@ int sm_a3d_set_receivers(sm_a3d *s, size_t nrec, const size_t *idx, const int *c
     s->max_steps = 0;
     if (nrec == 0 || max_steps == 0) return SM_OK;
     int rc = dalloc(s->rcv, nrec * (sizeof(int64_t) + sizeof(int)));
-    if (rc == SM_OK) rc = dalloc(s->trace, nrec * max_steps * s->elt);
+    if (rc == SM_OK) rc = dalloc(s->trace, nrec * max_steps * s->telt);
     if (rc != SM_OK) { dfree(s->rcv); dfree(s->trace); return rc; }
     std::vector<unsigned char> h(nrec * (sizeof(int64_t) + sizeof(int)));
     int64_t *hi = (int64_t *)h.data();
@@ -533,13 +794,60 @@ int sm_a3d_set_receivers(sm_a3d *s, size_t nrec, const size_t *idx, const int *c
     return SM_OK;
 }
 
+int sm_a3d_set_launch(sm_a3d *s, int rows_per_block, int x_chunks)
+{
+    SM_CHECK(s, SM_ERR_ARG, "sm_a3d_set_launch: NULL solver");
+    SM_CHECK(s->fused, SM_ERR_STATE, "sm_a3d_set_launch: only the fused kernel has a launch shape");
+    SM_CHECK(rows_per_block == 1 || rows_per_block == 2 || rows_per_block == 4 ||
+                 rows_per_block == 8, SM_ERR_ARG, "rows_per_block must be 1, 2, 4 or 8");
+    SM_CHECK(x_chunks >= 1 && x_chunks <= s->nx2, SM_ERR_ARG, "x_chunks must be in [1, nx2]");
+    // every chunk must own at least one plane
+    const int len = (s->nx2 + x_chunks - 1) / x_chunks;
+    SM_CHECK((x_chunks - 1) * len < s->nx2, SM_ERR_ARG, "x_chunks %d leaves an empty chunk", x_chunks);
+    s->tj = rows_per_block;
+    s->nchunks = x_chunks;
+    s->nblocks = resident_blocks(s, rows_per_block);
+    return SM_OK;
+}
+
+int sm_a3d_get_launch(const sm_a3d *s, int *rows_per_block, int *x_chunks)
+{
+    SM_CHECK(s && rows_per_block && x_chunks, SM_ERR_ARG, "sm_a3d_get_launch: NULL argument");
+    *rows_per_block = s->fused ? s->tj : 0;
+    *x_chunks = s->fused ? s->nchunks : 0;
+    return SM_OK;
+}
+
+int sm_a3d_set_timing(sm_a3d *s, int on)
+{
+    SM_CHECK(s, SM_ERR_ARG, "sm_a3d_set_timing: NULL solver");
+    s->timing = on != 0;
+    s->timed_ms = 0.0;
+    s->timed_launches = 0;
+    return SM_OK;
+}
+
+int sm_a3d_get_timing(const sm_a3d *s, double *ms_total, size_t *launches)
+{
+    SM_CHECK(s && ms_total && launches, SM_ERR_ARG, "sm_a3d_get_timing: NULL argument");
+    *ms_total = s->timed_ms;
+    *launches = s->timed_launches;
+    return SM_OK;
+}
+
 int sm_a3d_run(sm_a3d *s, size_t nsteps)
 {
     SM_CHECK(s, SM_ERR_ARG, "sm_a3d_run: NULL solver");
     if (nsteps == 0) return SM_OK;
     SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
-    if (s->p.storage == SM_STORE_F64) return run_naive<double, true>(s, nsteps);
-    return run_naive<float, false>(s, nsteps);
+    switch (s->p.storage) {
+    case SM_STORE_F64: return run_naive<double, true>(s, nsteps);
+    case SM_STORE_F32:
+        return s->fused ? run_fused<float>(s, nsteps) : run_naive<float, false>(s, nsteps);
+    case SM_STORE_BF16: return run_fused<__hip_bfloat16>(s, nsteps);
+    case SM_STORE_F16: return run_fused<__half>(s, nsteps);
+    default: return fail(SM_ERR_STATE, "sm_a3d_run: bad storage");
+    }
 }
 
 int sm_a3d_current_step(const sm_a3d *s, size_t *it)
@@ -558,39 +866,41 @@ int sm_a3d_get_traces(sm_a3d *s, size_t it0, size_t count, double *out)
     if (count == 0 || s->nrec == 0) return SM_OK;
     SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
     const size_t n = count * s->nrec;
-    const char *src = (const char *)s->trace.ptr + it0 * s->nrec * s->elt;
-    if (s->elt == 8) return read_back<double>(s, src, n, out);
-    return read_back<float>(s, src, n, out);
+    std::vector<unsigned char> tmp(n * s->telt);
+    SM_HIP(hipMemcpyAsync(tmp.data(), (const char *)s->trace.ptr + it0 * s->nrec * s->telt,
+                          n * s->telt, hipMemcpyDeviceToHost, s->ctx->stream));
+    SM_HIP(hipStreamSynchronize(s->ctx->stream));
+    if (s->telt == 8) std::memcpy(out, tmp.data(), n * 8);
+    else for (size_t q = 0; q < n; ++q) out[q] = ((const float *)tmp.data())[q];
+    return SM_OK;
+}
+
+static int field_slot(int comp)
+{
+    return comp == SM_P ? 0 : comp == SM_VX ? 1 : comp == SM_VY ? 2 : comp == SM_VZ ? 3 : -1;
 }
 
 int sm_a3d_get_field(sm_a3d *s, int comp, double *out)
 {
     SM_CHECK(s && out, SM_ERR_ARG, "sm_a3d_get_field: NULL argument");
-    int slot = comp == SM_P ? 0 : comp == SM_VX ? 1 : comp == SM_VY ? 2 : comp == SM_VZ ? 3 : -1;
+    const int slot = field_slot(comp);
     SM_CHECK(slot >= 0, SM_ERR_ARG, "sm_a3d_get_field: component %d", comp);
     SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
     const char *src = (const char *)s->fields[s->cur].ptr + slot * s->nnn * s->elt;
-    if (s->elt == 8) return read_back<double>(s, src, s->nnn, out);
-    return read_back<float>(s, src, s->nnn, out);
+    return state_to_host(s, src, s->nnn, 0, 1, s->nnn, slot == 0 ? s->sP : s->sV, out);
 }
 
 int sm_a3d_get_slice_xz(sm_a3d *s, int comp, size_t iy, double *out)
 {
     SM_CHECK(s && out, SM_ERR_ARG, "sm_a3d_get_slice_xz: NULL argument");
     SM_CHECK(iy < (size_t)s->ny2, SM_ERR_ARG, "sm_a3d_get_slice_xz: iy %zu >= ny2", iy);
-    int slot = comp == SM_P ? 0 : comp == SM_VX ? 1 : comp == SM_VY ? 2 : comp == SM_VZ ? 3 : -1;
+    const int slot = field_slot(comp);
     SM_CHECK(slot >= 0, SM_ERR_ARG, "sm_a3d_get_slice_xz: component %d", comp);
     SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
-    const size_t e = s->elt, nz2 = s->nz2;
-    const char *base = (const char *)s->fields[s->cur].ptr + slot * s->nnn * e;
-    std::vector<unsigned char> tmp((size_t)s->nx2 * nz2 * e);
-    SM_HIP(hipMemcpy2DAsync(tmp.data(), nz2 * e, base + iy * nz2 * e, (size_t)s->ny2 * nz2 * e,
-                            nz2 * e, s->nx2, hipMemcpyDeviceToHost, s->ctx->stream));
-    SM_HIP(hipStreamSynchronize(s->ctx->stream));
-    const size_t n = (size_t)s->nx2 * nz2;
-    for (size_t q = 0; q < n; ++q)
-        out[q] = e == 8 ? ((const double *)tmp.data())[q] : (double)((const float *)tmp.data())[q];
-    return SM_OK;
+    const char *base = (const char *)s->fields[s->cur].ptr + slot * s->nnn * s->elt +
+                       iy * (size_t)s->nz2 * s->elt;
+    return state_to_host(s, base, s->nz2, 0, s->nx2, (size_t)s->ny2 * s->nz2,
+                         slot == 0 ? s->sP : s->sV, out);
 }
 
 int sm_a3d_reset(sm_a3d *s)

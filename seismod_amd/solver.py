@@ -69,6 +69,13 @@ class Context:
         check(lib.sm_ctx_mem_info(self.handle, C.byref(free), C.byref(total)), "sm_ctx_mem_info")
         return free.value, total.value
 
+    def probe_stream(self, n: int, reads: int, writes: int, iters: int = 20) -> float:
+        """ms per launch of a plain kernel streaming reads+writes FP32 arrays."""
+        ms = C.c_double()
+        check(lib.sm_probe_stream(self.handle, n, reads, writes, iters, C.byref(ms)),
+              "sm_probe_stream")
+        return ms.value
+
     def close(self) -> None:
         if self.handle:
             check(lib.sm_ctx_destroy(self.handle), "sm_ctx_destroy")
@@ -178,6 +185,22 @@ class Acoustic3D:
 
     def reset(self) -> None:
         check(lib.sm_a3d_reset(self.handle), "sm_a3d_reset")
+
+    def set_launch(self, rows_per_block: int, x_chunks: int) -> None:
+        check(lib.sm_a3d_set_launch(self.handle, rows_per_block, x_chunks), "sm_a3d_set_launch")
+
+    def launch(self) -> tuple[int, int]:
+        tj, ch = C.c_int(), C.c_int()
+        check(lib.sm_a3d_get_launch(self.handle, C.byref(tj), C.byref(ch)), "sm_a3d_get_launch")
+        return tj.value, ch.value
+
+    def set_timing(self, on: bool) -> None:
+        check(lib.sm_a3d_set_timing(self.handle, 1 if on else 0), "sm_a3d_set_timing")
+
+    def timing(self) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_size_t()
+        check(lib.sm_a3d_get_timing(self.handle, C.byref(ms), C.byref(n)), "sm_a3d_get_timing")
+        return ms.value, n.value
 
     def device_bytes(self) -> int:
         b = C.c_size_t()

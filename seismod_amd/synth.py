@@ -1,0 +1,107 @@
+"""Seeded synthetic models and shot geometries for the benchmark workloads.
+
+There is no field data in this project; every workload is generated here
+from a seed so runs are reproducible:
+  - a_iso_3d 256^3 (and 512^3): layered Vp with sorted random interface
+    depths, plus a Gaussian-smoothed random perturbation, Gardner density
+    rho = 310 Vp^0.25, padded into the CPML by edge replication exactly as
+    read_model_a does (io_utils.c:1089-1143);
+  - e_iso 400x400 two-layer model with a surface receiver line.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+VX, VY, VZ, P = 0, 1, 2, 8
+BULK = 6
+
+
+def smoothed_noise(shape, sigma: float, rng: np.random.Generator) -> np.ndarray:
+    """N(0,1) field smoothed by a Gaussian of `sigma` cells, peak |.| = 1."""
+    from scipy.ndimage import gaussian_filter
+    noise = rng.standard_normal(shape, dtype=np.float32)
+    noise = gaussian_filter(noise, sigma=sigma, mode="reflect", truncate=3.0)
+    noise /= max(float(np.abs(noise).max()), 1e-30)
+    return noise
+
+
+def layered_vp(nz: int, rng: np.random.Generator, vmin: float, vmax: float,
+               nlayers: tuple[int, int] = (4, 8)) -> np.ndarray:
+    """Vp(z) with a random number of layers, increasing with depth."""
+    nl = int(rng.integers(nlayers[0], nlayers[1] + 1))
+    tops = np.sort(rng.choice(np.arange(1, nz), size=nl - 1, replace=False))
+    vals = np.sort(rng.uniform(vmin, vmax, size=nl))
+    return vals[np.searchsorted(tops, np.arange(nz), side="right")]
+
+
+@dataclass
+class A3dShot:
+    nx: int
+    ny: int
+    nz: int
+    np: int
+    dx: float
+    dt: float
+    f0: float
+    nsteps: int
+    vp: np.ndarray          # padded (nx2, ny2, nz2), FP64
+    rho: np.ndarray
+    src_idx: np.ndarray
+    rec_idx: np.ndarray
+    rec_comp: np.ndarray
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def shape(self):
+        return self.vp.shape
+
+    @property
+    def cells(self) -> int:
+        return int(np.prod(self.vp.shape))
+
+
+def a3d_layered_shot(n: int = 256, np_: int = 20, dx: float = 10.0, seed: int = 2,
+                     vmin: float = 1500.0, vmax: float = 4500.0, perturb: float = 0.05,
+                     sigma: float = 8.0, dt: float = 6.0e-4, f0: float = 15.0,
+                     nsteps: int = 1500, nrec_side: int = 32) -> A3dShot:
+    """3D acoustic CPML shot: centre-top Bulk source, Vz receiver plane.
+
+    256^3 defaults: source at (1280, 1280, 100) m, receivers at z = 50 m on
+    x, y = 40 + 80 r m (r < 32), dt = 0.6 ms, 1500 steps.
+    """
+    rng = np.random.default_rng(seed)
+    vz = layered_vp(n, rng, vmin, vmax)
+    vp = np.empty((n, n, n), dtype=np.float64)
+    vp[:] = vz[None, None, :]
+    if perturb:
+        vp *= 1.0 + perturb * smoothed_noise((n, n, n), sigma, rng).astype(np.float64)
+    rho = 310.0 * vp ** 0.25
+    vp = np.pad(vp, np_, mode="edge")
+    rho = np.pad(rho, np_, mode="edge")
+    nx2 = ny2 = nz2 = n + 2 * np_
+    sx = sy = n // 2
+    sz = max(1, int(round(100.0 / dx)))
+    src = np.array([((sx + np_) * ny2 + sy + np_) * nz2 + sz + np_], dtype=np.uintp)
+    step = max(1, n // nrec_side)
+    off = step // 2
+    rz = int(round(50.0 / dx))
+    rec = [((off + step * a + np_) * ny2 + off + step * b + np_) * nz2 + rz + np_
+           for a in range(nrec_side) for b in range(nrec_side)]
+    return A3dShot(nx=n, ny=n, nz=n, np=np_, dx=dx, dt=dt, f0=f0, nsteps=nsteps, vp=vp,
+                   rho=rho, src_idx=src, rec_idx=np.array(rec, dtype=np.uintp),
+                   rec_comp=np.full(len(rec), VZ, dtype=np.intc),
+                   meta=dict(seed=seed, vmin=vmin, vmax=vmax, perturb=perturb, sigma=sigma))
+
+
+def algorithmic_bytes_a3d(nx2: int, ny2: int, nz2: int, np_: int, state_bytes: int = 4,
+                          mat_bytes: int = 4) -> float:
+    """Minimal HBM bytes per padded cell per step of the 3D CPML update.
+
+    P, vx, vy, vz read and written once, rho and K*dt read once, and the two
+    memory variables of each axis read and written only in that axis's
+    absorbing slabs (fraction 2np/n2 of the cells).
+    """
+    frac = sum(2.0 * np_ / n for n in (nx2, ny2, nz2))
+    return state_bytes * 8 + mat_bytes * 2 + state_bytes * 4 * frac

@@ -1,0 +1,290 @@
+"""CPU tests that pin the FP64 oracle.
+
+The reference cannot be built here (every source includes structs.h, which
+includes fftw3.h; io_utils.c includes netcdf.h; neither header is in the
+image) and has no tests or golden vectors.  The oracle is pinned instead by
+  - closed forms (Ricker wavelet, CPML and PML damping profiles),
+  - independent numpy formulations of both schemes (tests/numpy_ref.py),
+  - invariants (zero source, linearity, axis-permutation symmetry),
+  - an analytic 3D Green's function in a homogeneous medium,
+and the product's FP64 host set-up is checked bitwise against it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import decks
+import numpy_ref
+from oracle import pyoracle as O
+
+
+def rel_l2(a, b):
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+# ---------------------------------------------------------------- wavelet
+@pytest.mark.parametrize("f,dt", [(25.0, 5e-4), (15.0, 6e-4), (40.0, 1e-4), (7.3, 1.3e-3)])
+def test_ricker_single_source_closed_form(f, dt):
+    """One source: |A| times the Ricker wavelet normalised to peak 1 (src.c:43-65)."""
+    for A in (1.0, -3.5, 0.25):
+        L, fct = O.source_functions([A], [f], dt)
+        n = int(math.ceil(2.0 / (f * dt)))
+        assert L[0] == n
+        tau = np.arange(n) * dt - 1.0 / f
+        r = (1.0 - 2.0 * (np.pi * f * tau) ** 2) * np.exp(-(np.pi * f * tau) ** 2)
+        expect = abs(A) * r / np.abs(r).max()    # the sign of A is lost
+        np.testing.assert_allclose(fct, expect, rtol=0, atol=1e-13 * abs(A))
+
+
+def test_ricker_multi_source_closed_form():
+    """Later sources keep the raw A*2*pi^2*f^2*r(t); source 0 stays a Ricker."""
+    A, f, dt = [1.0, 2.0, -0.5], [30.0, 25.0, 40.0], 2e-4
+    L, fct = O.source_functions(A, f, dt)
+    off = np.concatenate([[0], np.cumsum(L)[:-1]]).astype(int)
+    for s in (1, 2):
+        tau = np.arange(L[s]) * dt - 1.0 / f[s]
+        K = (np.pi * f[s] * tau) ** 2
+        raw = A[s] * 2 * np.pi ** 2 * f[s] ** 2 * (1 - 2 * K) * np.exp(-K)
+        np.testing.assert_allclose(fct[off[s]:off[s] + int(L[s])], raw, rtol=1e-13, atol=1e-9)
+    tau = np.arange(L[0]) * dt - 1.0 / f[0]
+    r0 = (1.0 - 2.0 * (np.pi * f[0] * tau) ** 2) * np.exp(-(np.pi * f[0] * tau) ** 2)
+    w0 = fct[:L[0]]
+    scale = w0[np.argmax(np.abs(r0))] / r0[np.argmax(np.abs(r0))]
+    np.testing.assert_allclose(w0, scale * r0, rtol=1e-12, atol=1e-12 * abs(scale))
+
+
+def test_product_source_functions_match_oracle_bitwise():
+    from seismod_amd.solver import source_functions
+    for A, f, dt in (([1.0], [25.0], 5e-4), ([1.0, 0.5, -2.0], [25.0, 30.0, 20.0], 3e-4),
+                     ([-2.0], [15.0], 6e-4)):
+        Lo, fo = O.source_functions(A, f, dt)
+        Lp, fp = source_functions(A, f, dt)
+        assert np.array_equal(Lo, Lp)
+        assert np.array_equal(fo, fp)
+
+
+# ---------------------------------------------------------------- steps and records
+# 0.9 s at 0.6 ms gives 1501: 1500*0.0006 rounds to 0.8999999999999999 < 0.9
+@pytest.mark.parametrize("duration,dt,expect", [(1.0, 5e-4, 2000), (0.9, 6e-4, 1501),
+                                                (1.2, 1e-4, 12000), (0.1, 3e-4, 334)])
+def test_nsteps(duration, dt, expect):
+    n = O.nsteps(duration, dt)
+    assert n == expect
+    assert n * dt >= duration and (n - 1) * dt < duration
+
+
+def test_record_predicate_regular_sampling():
+    """rdt = m*dt from t0 = 0 samples every m-th step (t = dt*(it+1))."""
+    from seismod_amd.solver import record_due
+    dt = 5e-4
+    for m in (1, 2, 4, 10):
+        due = [it for it in range(2000) if O.record_due(it, dt, 0.0, m * dt)]
+        assert due == [it for it in range(2000) if (it + 1) % m == 0]
+        assert due == [it for it in range(2000) if record_due(it, dt, 0.0, m * dt)]
+    # a snapshot at one time (rdt <= 0) fires once, at the step nearest t0
+    due = [it for it in range(400) if O.record_due(it, dt, 0.1, 0.0)]
+    assert due == [199]
+
+
+# ---------------------------------------------------------------- CPML / PML tables
+@pytest.mark.parametrize("n2,np_,kmax,amax", [(296, 20, 1.0, 47.1), (40, 6, 3.0, 0.0),
+                                              (11, 5, 7.0, 31.4)])
+def test_cpml_tables_closed_form(n2, np_, kmax, amax):
+    d, dt = 10.0, 6e-4
+    t = O.cpml_tables(n2, np_, d, dt, kappa_max=kmax, alpha_max=amax)
+    a, b, k = numpy_ref.cpml_profile(n2, np_, d, dt, kappa_max=kmax, alpha_max=amax)
+    ah, bh, kh = numpy_ref.cpml_profile(n2, np_, d, dt, kappa_max=kmax, alpha_max=amax, half=True)
+    for got, want in ((t["a"], a), (t["b"], b), (t["k"], k), (t["a_h"], ah), (t["b_h"], bh),
+                      (t["k_h"], kh)):
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-15)
+    # mirror symmetry of the node tables
+    np.testing.assert_array_equal(t["b"], t["b"][::-1])
+    np.testing.assert_array_equal(t["b_h"], t["b_h"][::-1])
+
+
+def test_e2d_damping_closed_form():
+    npml, h, vmean = 20, 5.0, 3000.0
+    d, dh, dH = O.e2d_damping(npml, h, vmean)
+    delta = npml * h
+    d0 = 1.5 * math.log(1e3) * vmean / delta
+    n = np.arange(npml)
+    np.testing.assert_allclose(d, d0 * ((npml - n) * h / delta) ** 2, rtol=1e-13)
+    np.testing.assert_allclose(dh, d0 * ((npml - n - 0.5) * h / delta) ** 2, rtol=1e-13)
+    n = np.arange(npml + 1)
+    np.testing.assert_allclose(dH, d0 * ((npml - n + 0.5) * h / delta) ** 2, rtol=1e-13)
+
+
+def test_e2d_material_averages():
+    rng = np.random.default_rng(3)
+    vp = rng.uniform(2000, 4000, (7, 5))
+    vs = vp / rng.uniform(1.6, 2.0, (7, 5))
+    rho = rng.uniform(1800, 2600, (7, 5))
+    m = O.e2d_material(vp, vs, rho)
+    lam = rho * (vp ** 2 - 2 * vs ** 2)
+    mu = rho * vs ** 2
+    b = 1 / rho
+    # mu at (i, j+1/2), la/l2m at (i+1/2, j), bm at (i+1/2, j+1/2); last row/col kept
+    np.testing.assert_allclose(m["mu"][:, :-1], 0.5 * (mu[:, :-1] + mu[:, 1:]), rtol=1e-14)
+    np.testing.assert_allclose(m["mu"][:, -1], mu[:, -1], rtol=1e-14)
+    np.testing.assert_allclose(m["la"][:-1], 0.5 * (lam[:-1] + lam[1:]), rtol=1e-14)
+    np.testing.assert_allclose(m["l2m"][:-1], 0.5 * (lam[:-1] + lam[1:] + 2 * mu[:-1] + 2 * mu[1:]),
+                               rtol=1e-13)
+    np.testing.assert_allclose(m["b"], b, rtol=1e-15)
+    np.testing.assert_allclose(m["bm"][:-1, :-1],
+                               0.25 * (b[:-1, :-1] + b[1:, :-1] + b[:-1, 1:] + b[1:, 1:]), rtol=1e-14)
+    np.testing.assert_allclose(m["bm"][-1, -1], b[-1, -1], rtol=1e-15)
+
+
+# ---------------------------------------------------------------- 3D scheme
+def _a3d(d, A=None):
+    c = d["cfg"]
+    L, fct = O.source_functions(d["A"] if A is None else A, d["f"], c["dt"])
+    return O.a3d_run(c, d["vp"], d["rho"], d["src_idx"], L, fct, d["rec_idx"], d["rec_comp"],
+                     d["nsteps"]), (L, fct)
+
+
+@pytest.mark.parametrize("case", [dict(nx=8, ny=7, nz=6, np_=3, seed=1),
+                                  dict(nx=6, ny=9, nz=5, np_=2, seed=2, dy=7.5, kappa_max=2.5),
+                                  dict(nx=5, ny=5, nz=7, np_=0, seed=3),
+                                  dict(nx=4, ny=6, nz=5, np_=4, seed=4, alpha_max=0)])
+def test_a3d_oracle_matches_numpy_formulation(case):
+    d = decks.a3d_deck(nsteps=40, **case)
+    (tr, f), (L, fct) = _a3d(d)
+    tr_n, f_n = numpy_ref.a3d_numpy(d["cfg"], d["vp"], d["rho"], d["src_idx"], L, fct,
+                                    d["rec_idx"], d["rec_comp"], d["nsteps"])
+    assert np.abs(f).max() > 0
+    assert rel_l2(f, f_n) < 1e-12
+    assert rel_l2(tr, tr_n) < 1e-12
+
+
+def test_a3d_zero_source_stays_zero():
+    d = decks.a3d_deck(nx=8, ny=7, nz=6, np_=3, seed=5, nsteps=30)
+    c = d["cfg"]
+    tr, f = O.a3d_run(c, d["vp"], d["rho"], np.zeros(0, np.uintp), np.zeros(0, np.uintp),
+                      np.zeros(0), d["rec_idx"], d["rec_comp"], d["nsteps"])
+    assert not np.any(tr) and not np.any(f)
+
+
+def test_a3d_linear_in_amplitude():
+    d = decks.a3d_deck(nx=8, ny=7, nz=6, np_=3, seed=6, nsteps=30)
+    (tr1, f1), _ = _a3d(d, A=[1.0])
+    (tr3, f3), _ = _a3d(d, A=[3.0])
+    (trm, fm), _ = _a3d(d, A=[-3.0])     # the sign of A is lost (src.c:59-63)
+    assert rel_l2(f3, 3 * f1) < 1e-13
+    assert np.array_equal(f3, fm)
+
+
+def test_a3d_axis_permutation_symmetry():
+    """Cube, homogeneous medium, centred source: P is symmetric under
+    exchanging axes and each velocity maps to the permuted component."""
+    n, np_ = 9, 3
+    n2 = n + 2 * np_
+    vp = np.full((n2, n2, n2), 2500.0)
+    rho = np.full((n2, n2, n2), 2100.0)
+    dt = 0.4 * 10.0 / (2500.0 * math.sqrt(3))
+    cfg = dict(nx=n, ny=n, nz=n, np=np_, dx=10.0, dy=10.0, dz=10.0, dt=dt, f0=30.0)
+    c0 = n2 // 2
+    src = np.array([(c0 * n2 + c0) * n2 + c0], np.uintp)
+    L, fct = O.source_functions([1.0], [30.0], dt)
+    _, f = O.a3d_run(cfg, vp, rho, src, L, fct, np.zeros(0, np.uintp), np.zeros(0, np.intc), 50)
+    N = n2 ** 3
+    P, vx, vy, vz = (f[q * N:(q + 1) * N].reshape(n2, n2, n2) for q in range(4))
+    scale = np.abs(P).max()
+    assert np.abs(P - P.transpose(1, 0, 2)).max() < 1e-12 * scale
+    assert np.abs(P - P.transpose(2, 1, 0)).max() < 1e-12 * scale
+    vs = np.abs(vx).max()
+    assert np.abs(vx - vy.transpose(1, 0, 2)).max() < 1e-12 * vs
+    assert np.abs(vx - vz.transpose(2, 1, 0)).max() < 1e-12 * vs
+
+
+def test_a3d_homogeneous_green_function():
+    """Early pressure at distance r from the injection point against the
+    analytic 3D solution P(r,t) = h^3 q'(t - r/c) / (4 pi c^2 r), where
+    q(t) = 1e6 * fct(t) is the injection rate of a_iso_3d.c:689-693."""
+    n, np_, h, c, rho0, f0 = 64, 10, 10.0, 2000.0, 2000.0, 6.0
+    n2 = n + 2 * np_
+    vp = np.full((n2, n2, n2), c)
+    rho = np.full((n2, n2, n2), rho0)
+    dt = 0.4 * h / (c * math.sqrt(3))
+    cfg = dict(nx=n, ny=n, nz=n, np=np_, dx=h, dy=h, dz=h, dt=dt, f0=f0)
+    c0 = n2 // 2
+    src = np.array([(c0 * n2 + c0) * n2 + c0], np.uintp)
+    rcells = 12
+    rec = np.array([((c0 + rcells) * n2 + c0) * n2 + c0], np.uintp)
+    L, fct = O.source_functions([1.0], [f0], dt)
+    nsteps = int(0.30 / dt)
+    tr, _ = O.a3d_run(cfg, vp, rho, src, L, fct, rec, np.array([8], np.intc), nsteps,
+                      want_fields=False)
+    r = rcells * h
+    t = dt * (np.arange(nsteps) + 1)
+    tcut = 1.0 / f0
+
+    def qdot(tt):
+        # derivative of the normalised Ricker, evaluated continuously
+        tau = tt - tcut
+        a = (np.pi * f0) ** 2
+        w = (1 - 2 * a * tau ** 2) * np.exp(-a * tau ** 2)
+        dw = (-6 * a * tau + 4 * a * a * tau ** 3) * np.exp(-a * tau ** 2)
+        tau_s = np.arange(L[0]) * dt - tcut
+        peak = np.abs((1 - 2 * a * tau_s ** 2) * np.exp(-a * tau_s ** 2)).max()
+        return 1e6 * dw / peak * ((tt >= 0) & (tt <= 2 * tcut))
+
+    # injected after step it at time t; the sampled wavelet index it is time it*dt
+    best = min(rel_l2(tr[:, 0], h ** 3 * qdot(t - r / c - s) / (4 * np.pi * c * c * r))
+               for s in (0.0, 0.5 * dt, dt))
+    assert best < 0.06, best
+
+
+# ---------------------------------------------------------------- 2D scheme
+def _e2d(d, damping_scale=1.0):
+    c = dict(d["cfg"], damping_scale=damping_scale)
+    L, fct = O.source_functions(d["A"], d["f"], c["dt"])
+    tr, f, _ = O.e2d_run(c, d["vp"], d["vs"], d["rho"], d["src_type"], d["src_idx"], L, fct,
+                         d["rec_idx"], d["rec_comp"], d["nsteps"])
+    return tr, f, L, fct
+
+
+E2D_CASES = [dict(nx=16, nz=12, npml=3, seed=1),
+             dict(nx=10, nz=18, npml=4, seed=2),
+             dict(nx=9, nz=9, npml=1, seed=3, src_type=decks.SXZ),
+             dict(nx=6, nz=21, npml=5, seed=4, src_type=decks.SZ, q1_compat=False),
+             dict(nx=13, nz=8, npml=2, seed=5, src_type=decks.SX)]
+
+
+@pytest.mark.parametrize("case", E2D_CASES, ids=lambda c: f"{c['nx']}x{c['nz']}_np{c['npml']}")
+@pytest.mark.parametrize("scale", [1.0, 0.0])
+def test_e2d_oracle_matches_numpy_formulation(case, scale):
+    """The oracle's per-region transcription of pml.c against a padded-grid
+    formulation whose damping comes from distances (scale 0: PML off)."""
+    d = decks.e2d_deck(nsteps=60, **case)
+    tr, f, L, fct = _e2d(d, damping_scale=scale)
+    tr_n, f_n = numpy_ref.e2d_numpy(dict(d["cfg"]), d["vp"], d["vs"], d["rho"], d["src_type"],
+                                    d["src_idx"], L, fct, d["rec_idx"], d["rec_comp"],
+                                    d["nsteps"], damping_scale=scale)
+    assert np.abs(f).max() > 0
+    assert rel_l2(f, f_n) < 1e-11
+    assert rel_l2(tr, tr_n) < 1e-11
+
+
+def test_e2d_zero_source_and_linearity():
+    d = decks.e2d_deck(nx=12, nz=10, npml=3, seed=7, nsteps=40)
+    c = d["cfg"]
+    tr, f, _ = O.e2d_run(c, d["vp"], d["vs"], d["rho"], np.zeros(0, np.intc),
+                         np.zeros(0, np.uintp), np.zeros(0, np.uintp), np.zeros(0),
+                         d["rec_idx"], d["rec_comp"], d["nsteps"])
+    assert not np.any(tr) and not np.any(f)
+    _, f1, _, _ = _e2d(d)
+    d2 = dict(d, A=np.array([-4.0]))
+    _, f4, _, _ = _e2d(d2)
+    assert rel_l2(f4, 4 * f1) < 1e-13
+
+
+def test_e2d_pml_absorbs():
+    """With the PML the field left in the grid after the wave has crossed it
+    is much smaller than with damping switched off."""
+    d = decks.e2d_deck(nx=40, nz=40, npml=10, seed=8, nsteps=400, two_layer=False, f0=60.0)
+    _, f_on, _, _ = _e2d(d, 1.0)
+    _, f_off, _, _ = _e2d(d, 0.0)
+    assert np.linalg.norm(f_on) < 0.1 * np.linalg.norm(f_off)
