@@ -69,8 +69,12 @@ def build_lib() -> Path:
 
 
 def build_host() -> list[Path]:
-    """Drop-in e_iso and a_iso_3d programs linked against the library."""
+    """Drop-in e_iso and a_iso_3d programs linked against the library, and
+    the host I/O layer alone as libsm_hostio.so (tested on CPU)."""
     BINDIR.mkdir(exist_ok=True)
+    LIBDIR.mkdir(exist_ok=True)
+    _run(["gcc", *C_FLAGS, "-fPIC", "-shared", f"-I{HOST}", str(HOST / "sm_io.c"), "-lm",
+          "-o", str(LIBDIR / "libsm_hostio.so")])
     common = sorted(p for p in HOST.glob("*.c") if p.stem not in ("e_iso", "a_iso_3d"))
     outs = []
     for prog in ("e_iso", "a_iso_3d"):

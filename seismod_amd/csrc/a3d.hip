@@ -159,6 +159,8 @@ struct sm_a3d {
     sm_dbuf src;                    // cell, off, len (int64) | plane offsets (int) | increments
     sm_dbuf rcv;                    // idx (int64), comp (int)
     sm_dbuf trace;                  // [max_steps][nrec]
+    sm_dbuf xpl;                    // per-plane x constants of the fused kernel
+    std::vector<int> src_plane;     // sources of plane i: sorted entries [src_plane[i], [i+1])
     int cur = 0;                    // which ping-pong half holds the state
     size_t step = 0;
     size_t nsrc = 0, nrec = 0, max_steps = 0, rcv_step0 = 0;
@@ -392,54 +394,44 @@ FusedArgs<S> fused_args(sm_a3d *s)
 {
     FusedArgs<S> a{};
     a.nx2 = s->nx2; a.ny2 = s->ny2; a.nz2 = s->nz2; a.np = (int)s->p.np;
-    a.ntk = (s->nz2 + 63) / 64;
+    a.ntk = (s->nz2 + 62) / 63;      // 63 owned cells per wave along z
     a.ntiles = a.ntk * ((s->ny2 + s->tj - 1) / s->tj);
     a.chunk_len = (s->nx2 + s->nchunks - 1) / s->nchunks;
     a.nreal = a.ntiles * s->nchunks;
     a.nitems = (a.nreal + 7) / 8 * 8;
     a.nblocks = s->nblocks > 0 ? std::min(s->nblocks, a.nitems) : a.nitems;
-    const float *m = (const float *)s->mat.ptr;
-    a.rho = m; a.kdt = m + s->nnn;
-    CpmlAxis<float> ax[3];
-    axes<float>(s, ax);
-    a.ax = ax[0].a; a.bx = ax[0].b; a.ikx = ax[0].k;
-    a.ahx = ax[0].ah; a.bhx = ax[0].bh; a.ikhx = ax[0].kh;
-    a.ay = ax[1].a; a.by = ax[1].b; a.iky = ax[1].k;
-    a.ahy = ax[1].ah; a.bhy = ax[1].bh; a.ikhy = ax[1].kh;
-    a.az = ax[2].a; a.bz = ax[2].b; a.ikz = ax[2].k;
-    a.ahz = ax[2].ah; a.bhz = ax[2].bh; a.ikhz = ax[2].kh;
+    a.nnn = (unsigned)s->nnn;
+    const unsigned np2 = 2u * (unsigned)s->p.np;
+    a.memp = (S *)s->memp.ptr;
+    a.my = np2 * (unsigned)s->ny2 * (unsigned)s->nz2;
+    a.mz = a.my + (unsigned)s->nx2 * np2 * (unsigned)s->nz2;
+    a.mat = (const float *)s->mat.ptr;
+    a.xpl = (const XPlane *)s->xpl.ptr;
+    a.tab = (const float *)s->tables.ptr;
+    a.toff_y = 6 * s->nx2;
+    a.toff_z = 6 * (s->nx2 + s->ny2);
     a.idx = (float)(1.0 / s->p.dx); a.idy = (float)(1.0 / s->p.dy); a.idz = (float)(1.0 / s->p.dz);
     a.two_dt = (float)(2.0 * s->p.dt);
     a.sP = s->sP; a.isP = 1.f / s->sP; a.sV = s->sV; a.isV = 1.f / s->sV;
     a.sM = s->sM; a.isM = 1.f / s->sM;
     const char *sb = (const char *)s->src.ptr;
-    const int64_t *sp = (const int64_t *)sb;
-    a.src_cell = sp; a.src_off = sp + s->nsrc; a.src_len = sp + 2 * s->nsrc;
-    a.src_plane = (const int *)(sb + s->src_plane_off);
+    a.src = (const int64_t *)sb;
     a.src_inc = (const float *)(sb + s->src_inc_off);
+    a.nsrc = (int)s->nsrc;
     a.nrec = (int)s->nrec;
     a.ridx = (const int64_t *)s->rcv.ptr;
     a.rcomp = (const int *)(a.ridx + s->nrec);
     a.trace = (float *)s->trace.ptr;
-    const size_t np2 = 2 * s->p.np;
-    S *mp = (S *)s->memp.ptr;
-    a.mvx = mp; a.mvy = mp + np2 * s->ny2 * s->nz2; a.mvz = a.mvy + s->nx2 * np2 * s->nz2;
     return a;
 }
 
 template <typename S>
 void fused_bind(sm_a3d *s, FusedArgs<S> &a, int in)
 {
-    const size_t n = s->nnn, np2 = 2 * s->p.np;
-    const S *fi = (const S *)s->fields[in].ptr;
-    S *fo = (S *)s->fields[1 - in].ptr;
-    a.P_in = fi; a.vx_in = fi + n; a.vy_in = fi + 2 * n; a.vz_in = fi + 3 * n;
-    a.P_out = fo; a.vx_out = fo + n; a.vy_out = fo + 2 * n; a.vz_out = fo + 3 * n;
-    const S *mi = (const S *)s->memv[in].ptr;
-    S *mo = (S *)s->memv[1 - in].ptr;
-    const size_t sx = np2 * s->ny2 * s->nz2, sy = s->nx2 * np2 * s->nz2;
-    a.mPx_in = mi; a.mPy_in = mi + sx; a.mPz_in = mi + sx + sy;
-    a.mPx_out = mo; a.mPy_out = mo + sx; a.mPz_out = mo + sx + sy;
+    a.in = (const S *)s->fields[in].ptr;
+    a.out = (S *)s->fields[1 - in].ptr;
+    a.mem_in = (const S *)s->memv[in].ptr;
+    a.mem_out = (S *)s->memv[1 - in].ptr;
 }
 
 template <typename S>
@@ -493,6 +485,43 @@ void release(sm_a3d *s)
 {
     for (int h = 0; h < 2; ++h) { dfree(s->fields[h]); dfree(s->memv[h]); }
     dfree(s->mat); dfree(s->memp); dfree(s->tables); dfree(s->src); dfree(s->rcv); dfree(s->trace);
+    dfree(s->xpl);
+}
+
+// per-plane x records of the fused kernel from the FP64 tables and the
+// per-plane source ranges
+int upload_xplanes(sm_a3d *s)
+{
+    if (!s->fused) return SM_OK;
+    const int nx2 = s->nx2;
+    const double *t = s->htab.data();   // x axis: a, b, k, ah, bh, kh (nx2 each)
+    std::vector<XPlane> h(nx2);
+    for (int i = 0; i < nx2; ++i) {
+        XPlane &x = h[i];
+        std::memset(&x, 0, sizeof(x));
+        if (i < nx2 - 1) {
+            x.ah = (float)t[3 * nx2 + i];
+            x.bh = (float)t[4 * nx2 + i];
+            x.ikh = (float)(1.0 / t[5 * nx2 + i]);
+        } else {
+            x.ah = 0.f; x.bh = 1.f; x.ikh = 1.f;
+        }
+        x.a = (float)t[i];
+        x.b = (float)t[nx2 + i];
+        x.ik = (float)(1.0 / t[2 * nx2 + i]);
+        x.sxv = slab_vel(i, nx2, (int)s->p.np);
+        x.sxp = slab_pre(i, nx2, (int)s->p.np);
+        x.q0 = s->src_plane.empty() ? 0 : s->src_plane[i];
+        x.q1 = s->src_plane.empty() ? 0 : s->src_plane[i + 1];
+    }
+    if (!s->xpl.ptr) {
+        int rc = dalloc(s->xpl, nx2 * sizeof(XPlane));
+        if (rc != SM_OK) return rc;
+    }
+    SM_HIP(hipMemcpyAsync(s->xpl.ptr, h.data(), nx2 * sizeof(XPlane), hipMemcpyHostToDevice,
+                          s->ctx->stream));
+    SM_HIP(hipStreamSynchronize(s->ctx->stream));
+    return SM_OK;
 }
 
 size_t elt_size(int storage)
@@ -538,7 +567,7 @@ int valid_chunks(int nx2, int want)
 void default_launch(sm_a3d *s)
 {
     s->tj = 4;
-    const int ntiles = ((s->nz2 + 63) / 64) * ((s->ny2 + s->tj - 1) / s->tj);
+    const int ntiles = ((s->nz2 + 62) / 63) * ((s->ny2 + s->tj - 1) / s->tj);
     const int slots = resident_blocks(s, s->tj);
     int nch = (3 * slots + ntiles - 1) / ntiles;
     nch = std::min(nch, std::max(1, s->nx2 / 24));
@@ -607,6 +636,14 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
     s->melt = p.storage == SM_STORE_F64 ? 8 : 4;
     s->telt = s->melt;
     s->fused = uses_fused(p);
+    // the fused kernel addresses the four state fields of one buffer and
+    // the two material fields with 32-bit byte offsets
+    if (s->fused && (4 * nnn * e >= ((size_t)1 << 32) || 2 * nnn * 4 >= ((size_t)1 << 32))) {
+        delete s;
+        return fail(SM_ERR_ARG, "sm_a3d_create: %zu cells exceed the fused kernel's 32-bit "
+                    "offsets (limit 2^28 cells with FP32 storage, 2^29 with 16-bit); "
+                    "use SM_A3D_NAIVE with FP32 storage", nnn);
+    }
     {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess)
@@ -750,14 +787,16 @@ int sm_a3d_set_sources(sm_a3d *s, size_t nsrc, const size_t *idx, const size_t *
             else ((float *)(h.data() + s->src_inc_off))[q] = (float)inc;
         }
     }
+    s->src_plane.assign(s->nx2 + 1, 0);
     for (int i = 0, r = 0; i <= s->nx2; ++i) {
         while (r < (int)nsrc && (size_t)hi[r] / plane < (size_t)i) ++r;
         hp[i] = r;
+        s->src_plane[i] = r;
     }
     SM_HIP(hipMemcpyAsync(s->src.ptr, h.data(), bytes, hipMemcpyHostToDevice, s->ctx->stream));
     SM_HIP(hipStreamSynchronize(s->ctx->stream));
     s->nsrc = nsrc;
-    return SM_OK;
+    return upload_xplanes(s);
 }
 
 int sm_a3d_set_receivers(sm_a3d *s, size_t nrec, const size_t *idx, const int *comp,

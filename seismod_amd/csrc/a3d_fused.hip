@@ -36,33 +36,47 @@ namespace sm {
 template <typename S>
 struct Store;
 
+// Element access by a 32-bit element index turned into a 32-bit byte
+// offset from a uniform base, so loads use the SGPR-base + VGPR-offset form
+// (every device array here is well below 4 GiB).
+template <typename T>
+__device__ __forceinline__ T ld_at(const T *base, unsigned idx)
+{
+    return *(const T *)((const char *)base + idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st_at(T *base, unsigned idx, T v)
+{
+    *(T *)((char *)base + idx * (unsigned)sizeof(T)) = v;
+}
+
 template <>
 struct Store<float> {
-    __device__ static inline float ld(const float *p, size_t i, float) { return p[i]; }
-    __device__ static inline void st(float *p, size_t i, float v, float) { p[i] = v; }
+    __device__ static inline float ld(const float *p, unsigned i, float) { return ld_at(p, i); }
+    __device__ static inline void st(float *p, unsigned i, float v, float) { st_at(p, i, v); }
 };
 
 template <>
 struct Store<__hip_bfloat16> {
-    __device__ static inline float ld(const __hip_bfloat16 *p, size_t i, float scale)
+    __device__ static inline float ld(const __hip_bfloat16 *p, unsigned i, float scale)
     {
-        return __bfloat162float(p[i]) * scale;
+        return __bfloat162float(ld_at(p, i)) * scale;
     }
-    __device__ static inline void st(__hip_bfloat16 *p, size_t i, float v, float inv)
+    __device__ static inline void st(__hip_bfloat16 *p, unsigned i, float v, float inv)
     {
-        p[i] = __float2bfloat16(v * inv);
+        st_at(p, i, __float2bfloat16(v * inv));
     }
 };
 
 template <>
 struct Store<__half> {
-    __device__ static inline float ld(const __half *p, size_t i, float scale)
+    __device__ static inline float ld(const __half *p, unsigned i, float scale)
     {
-        return __half2float(p[i]) * scale;
+        return __half2float(ld_at(p, i)) * scale;
     }
-    __device__ static inline void st(__half *p, size_t i, float v, float inv)
+    __device__ static inline void st(__half *p, unsigned i, float v, float inv)
     {
-        p[i] = __float2half(v * inv);
+        st_at(p, i, __float2half(v * inv));
     }
 };
 
@@ -70,8 +84,9 @@ __device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // one velocity component at one cell: v - 2dt/(rho0+rho1) * dP, with the
 // CPML memory variable m (updated in place) where slab is true
-__device__ inline float vel_update(float v_old, float p0, float p1, float r0, float r1, float ih,
-                                   float two_dt, bool slab, float bh, float ah, float ikh, float &m)
+__device__ __forceinline__ float vel_update(float v_old, float p0, float p1, float r0, float r1,
+                                            float ih, float two_dt, bool slab, float bh, float ah,
+                                            float ikh, float &m)
 {
     float dP = (p1 - p0) * ih;
     if (slab) {
@@ -81,195 +96,166 @@ __device__ inline float vel_update(float v_old, float p0, float p1, float r0, fl
     return v_old - two_dt * fast_rcp(r0 + r1) * dP;
 }
 
-// everything one lane reads for plane i (loaded one plane ahead)
+// what one lane reads for plane i, loaded one plane ahead
 struct Col {
-    float vx, vy, vz, K;       // old velocities and K*dt at (i, j, k)
-    float Pn, rn;              // P and rho at (i+1, j, k)
-    float Pj, rj;              // P and rho at (i, j+1, k)
-    float Pe, re;              // lane 63: (i, j, k+1) across the tile edge
-    float Pjm, rjm, vyjm;      // wave 0: (i, j-1, k)
-    float Pkm, rkm, vzkm;      // lane 0: (i, j, k-1)
-    float mPx, mPy, mPz;       // velocity memory variables at (i, j, k)
-    float mPym, mPzm;          // ... at the halo cells
-    float mvx, mvy, mvz;       // pressure memory variables at (i, j, k)
+    float vx, vy, vz, K;    // old velocities and K*dt at (i, j, k)
+    float Pn, rn;           // P, rho at (i+1, j, k)   (clamped to the last plane)
+    float Pj, rj;           // P, rho at (i, j+1, k)   (clamped to the last row)
+    float Pk, rk;           // P, rho at (i, j, k+1)   (clamped to the last column)
+    float Pjm, rjm, vyjm;   // wave 0 only: (i, j-1, k)
+    float mPx, mPy, mPz;    // velocity memory variables (slab cells only)
+    float mPym;             // wave 0 only: at the halo row
+    float mvx, mvy, mvz;    // pressure memory variables (slab cells only)
+};
+
+// field q (0 P, 1 vx, 2 vy, 3 vz) of cell o: element q*nnn + o of in/out;
+// material: rho at o, K*dt at nnn + o
+#define F_VX 1u
+#define F_VY 2u
+#define F_VZ 3u
+
+// constants of one lane for one work item
+struct Lane {
+    unsigned o_jk, o_jp, o_kp, o_jm;   // in-plane offsets of the cell and its neighbours
+    int j, kk, kc;                     // j is wave-uniform
+    bool own;                          // this lane writes its cell
+    int syv, syp, syh;                 // y slab coordinates (wave-uniform)
+    int szv, szp;                      // z slab coordinates (per lane)
+    unsigned mz_v, mz_p;               // z-slab offsets within a plane
+    // CPML coefficients along y (wave-uniform) and z (per lane)
+    float ybh, yah, yikh, yb, ya, yik, hbh, hah, hikh;
+    float zbh, zah, zikh, zb, za, zik;
 };
 
 template <typename S>
-struct Ctx {
-    // restrict-qualified copies of the argument pointers
-    const S *__restrict__ P_in, *__restrict__ vx_in, *__restrict__ vy_in, *__restrict__ vz_in;
-    S *__restrict__ P_out, *__restrict__ vx_out, *__restrict__ vy_out, *__restrict__ vz_out;
-    const S *__restrict__ mPx_in, *__restrict__ mPy_in, *__restrict__ mPz_in;
-    S *__restrict__ mPx_out, *__restrict__ mPy_out, *__restrict__ mPz_out;
-    S *__restrict__ mvx, *__restrict__ mvy, *__restrict__ mvz;
-    const float *__restrict__ rho, *__restrict__ kdt;
-    int nx2, ny2, nz2, np, np2;
-    size_t plane;
-    float sP, isP, sV, isV, sM, isM;
-    // per lane
-    int lane, w, j, k, jc, kc;
-    bool active;
-    size_t jk;
-    int syv, szv, syp, szp, syh, szh;   // slab coordinates (y/z, vel/pre, halos)
-};
-
-template <typename S>
-__device__ inline void load_col(const Ctx<S> &x, int i, Col &c)
+__device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, int w, int i,
+                                         unsigned plane, int sxv, int sxp, Col &c)
 {
-    const size_t o = (size_t)i * x.plane + x.jk;
-    const float sP = x.sP, sV = x.sV, sM = x.sM;
-    c = Col{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f, 0.f, 1.f, 0.f, 1.f, 0.f,
-            0.f, 1.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (!x.active) return;
-    c.vx = Store<S>::ld(x.vx_in, o, sV);
-    c.vy = Store<S>::ld(x.vy_in, o, sV);
-    c.vz = Store<S>::ld(x.vz_in, o, sV);
-    c.K = x.kdt[o];
-    if (i + 1 < x.nx2) {
-        c.Pn = Store<S>::ld(x.P_in, o + x.plane, sP);
-        c.rn = x.rho[o + x.plane];
+    const unsigned o = (unsigned)i * plane;
+    const unsigned on = (unsigned)min(i + 1, a.nx2 - 1) * plane;
+    const unsigned np2 = 2u * a.np;
+    c.vx = Store<S>::ld(a.in, F_VX * a.nnn + o + L.o_jk, a.sV);
+    c.vy = Store<S>::ld(a.in, F_VY * a.nnn + o + L.o_jk, a.sV);
+    c.vz = Store<S>::ld(a.in, F_VZ * a.nnn + o + L.o_jk, a.sV);
+    c.K = ld_at(a.mat, a.nnn + o + L.o_jk);
+    c.Pn = Store<S>::ld(a.in, on + L.o_jk, a.sP);
+    c.rn = ld_at(a.mat, on + L.o_jk);
+    c.Pj = Store<S>::ld(a.in, o + L.o_jp, a.sP);
+    c.rj = ld_at(a.mat, o + L.o_jp);
+    c.Pk = Store<S>::ld(a.in, o + L.o_kp, a.sP);
+    c.rk = ld_at(a.mat, o + L.o_kp);
+    if (w == 0) {
+        c.Pjm = Store<S>::ld(a.in, o + L.o_jm, a.sP);
+        c.rjm = ld_at(a.mat, o + L.o_jm);
+        c.vyjm = Store<S>::ld(a.in, F_VY * a.nnn + o + L.o_jm, a.sV);
+        if (L.syh >= 0)
+            c.mPym = Store<S>::ld(a.mem_in, a.my + ((unsigned)i * np2 + L.syh) * a.nz2 + L.kc, a.sM);
     }
-    if (x.j < x.ny2 - 1) {
-        c.Pj = Store<S>::ld(x.P_in, o + x.nz2, sP);
-        c.rj = x.rho[o + x.nz2];
-    }
-    if (x.lane == 63 && x.k + 1 < x.nz2) {
-        c.Pe = Store<S>::ld(x.P_in, o + 1, sP);
-        c.re = x.rho[o + 1];
-    }
-    if (x.w == 0 && x.j >= 1) {
-        c.Pjm = Store<S>::ld(x.P_in, o - x.nz2, sP);
-        c.rjm = x.rho[o - x.nz2];
-        c.vyjm = Store<S>::ld(x.vy_in, o - x.nz2, sV);
-        if (x.syh >= 0)
-            c.mPym = Store<S>::ld(x.mPy_in, ((size_t)i * x.np2 + x.syh) * x.nz2 + x.kc, sM);
-    }
-    if (x.lane == 0 && x.k >= 1) {
-        c.Pkm = Store<S>::ld(x.P_in, o - 1, sP);
-        c.rkm = x.rho[o - 1];
-        c.vzkm = Store<S>::ld(x.vz_in, o - 1, sV);
-        if (x.szh >= 0)
-            c.mPzm = Store<S>::ld(x.mPz_in, ((size_t)i * x.ny2 + x.jc) * x.np2 + x.szh, sM);
-    }
-    const int sxv = slab_vel(i, x.nx2, x.np);
-    if (sxv >= 0 && i < x.nx2 - 1)
-        c.mPx = Store<S>::ld(x.mPx_in, ((size_t)sxv * x.ny2 + x.jc) * x.nz2 + x.kc, sM);
-    if (x.syv >= 0) c.mPy = Store<S>::ld(x.mPy_in, ((size_t)i * x.np2 + x.syv) * x.nz2 + x.kc, sM);
-    if (x.szv >= 0) c.mPz = Store<S>::ld(x.mPz_in, ((size_t)i * x.ny2 + x.jc) * x.np2 + x.szv, sM);
-    const bool pin = i >= 1 && x.j >= 1 && x.k >= 1;
-    const int sxp = slab_pre(i, x.nx2, x.np);
-    if (pin && sxp >= 0) c.mvx = Store<S>::ld(x.mvx, ((size_t)sxp * x.ny2 + x.jc) * x.nz2 + x.kc, sM);
-    if (pin && x.syp >= 0) c.mvy = Store<S>::ld(x.mvy, ((size_t)i * x.np2 + x.syp) * x.nz2 + x.kc, sM);
-    if (pin && x.szp >= 0) c.mvz = Store<S>::ld(x.mvz, ((size_t)i * x.ny2 + x.jc) * x.np2 + x.szp, sM);
+    if (sxv >= 0) c.mPx = Store<S>::ld(a.mem_in, (unsigned)sxv * plane + L.o_jk, a.sM);
+    if (L.syv >= 0)
+        c.mPy = Store<S>::ld(a.mem_in, a.my + ((unsigned)i * np2 + L.syv) * a.nz2 + L.kc, a.sM);
+    if (L.szv >= 0) c.mPz = Store<S>::ld(a.mem_in, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_v, a.sM);
+    if (sxp >= 0) c.mvx = Store<S>::ld(a.memp, (unsigned)sxp * plane + L.o_jk, a.sM);
+    if (L.syp >= 0)
+        c.mvy = Store<S>::ld(a.memp, a.my + ((unsigned)i * np2 + L.syp) * a.nz2 + L.kc, a.sM);
+    if (L.szp >= 0) c.mvz = Store<S>::ld(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, a.sM);
 }
 
-// one plane: load the next plane's column into nxt, update plane i from cur
+// one plane: prefetch plane i+1 into nxt, update plane i from cur
 template <typename S, int TJ>
-__device__ __forceinline__ void plane_step(const Ctx<S> &x, const FusedArgs<S> &a, int i, int i0,
-                                           int i1, Col &cur, Col &nxt, float &P_c, float &r_c,
-                                           float &vx_prev, float (*s_vy)[TJ][64])
+__device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L, int w, int i,
+                                           int i0, int i1, unsigned plane, Col &cur, Col &nxt,
+                                           float &P_c, float &r_c, float &vx_prev,
+                                           float (*s_vy)[TJ][64])
 {
-    if (i + 1 < i1) load_col<S>(x, i + 1, nxt);
-    const bool own = i >= i0;
-    const size_t o = (size_t)i * x.plane + x.jk;
-    const float isM = x.isM, isV = x.isV;
-
-    // vx at (i, j, k): a_iso_3d.c:605-619
-    float vxn = cur.vx;
-    if (x.active && i < x.nx2 - 1) {
-        const int sxv = slab_vel(i, x.nx2, x.np);
-        float m = cur.mPx;
-        vxn = vel_update(cur.vx, P_c, cur.Pn, r_c, cur.rn, a.idx, a.two_dt, sxv >= 0, a.bhx[i],
-                         a.ahx[i], a.ikhx[i], m);
-        if (own) {
-            Store<S>::st(x.vx_out, o, vxn, isV);
-            if (sxv >= 0)
-                Store<S>::st(x.mPx_out, ((size_t)sxv * x.ny2 + x.jc) * x.nz2 + x.kc, m, isM);
-        }
+    if (i + 1 < i1) {
+        const XPlane xn = a.xpl[i + 1];
+        load_col<S>(a, L, w, i + 1, plane, xn.sxv, xn.sxp, nxt);
     }
-    if (!own) {   // chunk start: only vx of the plane below is needed
+    const XPlane x = a.xpl[i];
+    const unsigned o = (unsigned)i * plane + L.o_jk;
+    const unsigned np2 = 2u * a.np;
+
+    // vx at (i, j, k), a_iso_3d.c:605-619.  On the last plane the clamped
+    // neighbour equals P_c, so dP = 0 and vx keeps its value.
+    float mx = cur.mPx;
+    const float vxn = vel_update(cur.vx, P_c, cur.Pn, r_c, cur.rn, a.idx, a.two_dt, x.sxv >= 0,
+                                 x.bh, x.ah, x.ikh, mx);
+    if (i < i0) {   // chunk start: only vx of the plane below is needed
         vx_prev = vxn;
         P_c = cur.Pn;
         r_c = cur.rn;
         return;
     }
-
-    // vy, vz at (i, j, k): a_iso_3d.c:621-651
-    float P_kp = __shfl_down(P_c, 1);
-    float r_kp = __shfl_down(r_c, 1);
-    if (x.lane == 63) { P_kp = cur.Pe; r_kp = cur.re; }
-    float vyn = cur.vy, vzn = cur.vz;
-    if (x.active && x.j < x.ny2 - 1) {
-        float m = cur.mPy;
-        const int n = x.j;
-        vyn = vel_update(cur.vy, P_c, cur.Pj, r_c, cur.rj, a.idy, a.two_dt, x.syv >= 0, a.bhy[n],
-                         a.ahy[n], a.ikhy[n], m);
-        Store<S>::st(x.vy_out, o, vyn, isV);
-        if (x.syv >= 0) Store<S>::st(x.mPy_out, ((size_t)i * x.np2 + x.syv) * x.nz2 + x.kc, m, isM);
+    // vy, vz at (i, j, k), a_iso_3d.c:621-651
+    float my = cur.mPy, mz = cur.mPz;
+    const float vyn = vel_update(cur.vy, P_c, cur.Pj, r_c, cur.rj, a.idy, a.two_dt, L.syv >= 0,
+                                 L.ybh, L.yah, L.yikh, my);
+    const float vzn = vel_update(cur.vz, P_c, cur.Pk, r_c, cur.rk, a.idz, a.two_dt, L.szv >= 0,
+                                 L.zbh, L.zah, L.zikh, mz);
+    if (L.own) {
+        Store<S>::st(a.out, F_VX * a.nnn + o, vxn, a.isV);
+        Store<S>::st(a.out, F_VY * a.nnn + o, vyn, a.isV);
+        Store<S>::st(a.out, F_VZ * a.nnn + o, vzn, a.isV);
+        if (x.sxv >= 0) Store<S>::st(a.mem_out, (unsigned)x.sxv * plane + L.o_jk, mx, a.isM);
+        if (L.syv >= 0)
+            Store<S>::st(a.mem_out, a.my + ((unsigned)i * np2 + L.syv) * a.nz2 + L.kc, my, a.isM);
+        if (L.szv >= 0) Store<S>::st(a.mem_out, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_v, mz, a.isM);
     }
-    if (x.active && x.k < x.nz2 - 1) {
-        float m = cur.mPz;
-        const int n = x.k;
-        vzn = vel_update(cur.vz, P_c, P_kp, r_c, r_kp, a.idz, a.two_dt, x.szv >= 0, a.bhz[n],
-                         a.ahz[n], a.ikhz[n], m);
-        Store<S>::st(x.vz_out, o, vzn, isV);
-        if (x.szv >= 0) Store<S>::st(x.mPz_out, ((size_t)i * x.ny2 + x.jc) * x.np2 + x.szv, m, isM);
+    // vy of the row below the tile (owned by the tile below; not stored)
+    float vy_halo = 0.f;
+    if (w == 0) {
+        float mh = cur.mPym;
+        vy_halo = vel_update(cur.vyjm, cur.Pjm, P_c, cur.rjm, r_c, a.idy, a.two_dt, L.syh >= 0,
+                             L.hbh, L.hah, L.hikh, mh);
     }
-    // halo velocities owned by neighbouring tiles (recomputed, not stored)
-    float vy_halo = 0.f, vz_halo = 0.f;
-    if (x.w == 0 && x.active && x.j >= 1) {
-        float m = cur.mPym;
-        const int n = x.j - 1;
-        vy_halo = vel_update(cur.vyjm, cur.Pjm, P_c, cur.rjm, r_c, a.idy, a.two_dt, x.syh >= 0,
-                             a.bhy[n], a.ahy[n], a.ikhy[n], m);
-    }
-    if (x.lane == 0 && x.active && x.k >= 1) {
-        float m = cur.mPzm;
-        const int n = x.k - 1;
-        vz_halo = vel_update(cur.vzkm, cur.Pkm, P_c, cur.rkm, r_c, a.idz, a.two_dt, x.szh >= 0,
-                             a.bhz[n], a.ahz[n], a.ikhz[n], m);
-    }
-    s_vy[i & 1][x.w][x.lane] = vyn;
+    s_vy[i & 1][w][threadIdx.x] = vyn;
     __syncthreads();
-    const float vy_m = x.w > 0 ? s_vy[i & 1][x.w - 1][x.lane] : vy_halo;
-    float vz_m = __shfl_up(vzn, 1);
-    if (x.lane == 0) vz_m = vz_halo;
+    const float vy_m = w > 0 ? s_vy[i & 1][w - 1][threadIdx.x] : vy_halo;
+    const float vz_m = __shfl_up(vzn, 1);   // lane 0 is the left halo lane
 
-    // pressure at (i, j, k): a_iso_3d.c:659-683
-    if (x.active) {
+    // pressure at (i, j, k), a_iso_3d.c:659-683
+    if (L.own) {
         float Pn = P_c;
-        if (i >= 1 && x.j >= 1 && x.k >= 1) {
+        if (i >= 1 && L.j >= 1 && L.kk >= 1) {
             float dvdx = (vxn - vx_prev) * a.idx;
             float dvdy = (vyn - vy_m) * a.idy;
             float dvdz = (vzn - vz_m) * a.idz;
-            const int sxp = slab_pre(i, x.nx2, x.np);
-            if (sxp >= 0) {
-                const float m = a.bx[i] * cur.mvx + a.ax[i] * dvdx;
-                Store<S>::st(x.mvx, ((size_t)sxp * x.ny2 + x.jc) * x.nz2 + x.kc, m, isM);
-                dvdx = dvdx * a.ikx[i] + m;
+            if (x.sxp >= 0) {
+                const float m = x.b * cur.mvx + x.a * dvdx;
+                Store<S>::st(a.memp, (unsigned)x.sxp * plane + L.o_jk, m, a.isM);
+                dvdx = dvdx * x.ik + m;
             }
-            if (x.syp >= 0) {
-                const float m = a.by[x.j] * cur.mvy + a.ay[x.j] * dvdy;
-                Store<S>::st(x.mvy, ((size_t)i * x.np2 + x.syp) * x.nz2 + x.kc, m, isM);
-                dvdy = dvdy * a.iky[x.j] + m;
+            if (L.syp >= 0) {
+                const float m = L.yb * cur.mvy + L.ya * dvdy;
+                Store<S>::st(a.memp, a.my + ((unsigned)i * np2 + L.syp) * a.nz2 + L.kc, m, a.isM);
+                dvdy = dvdy * L.yik + m;
             }
-            if (x.szp >= 0) {
-                const float m = a.bz[x.k] * cur.mvz + a.az[x.k] * dvdz;
-                Store<S>::st(x.mvz, ((size_t)i * x.ny2 + x.jc) * x.np2 + x.szp, m, isM);
-                dvdz = dvdz * a.ikz[x.k] + m;
+            if (L.szp >= 0) {
+                const float m = L.zb * cur.mvz + L.za * dvdz;
+                Store<S>::st(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, m, a.isM);
+                dvdz = dvdz * L.zik + m;
             }
             Pn = P_c - cur.K * (dvdx + dvdy + dvdz);
         }
         // sources of this plane (a_iso_3d.c:689-693), in source order
-        const int q0 = a.src_plane[i], q1 = a.src_plane[i + 1];
-        for (int q = q0; q < q1; ++q)
-            if ((size_t)a.src_cell[q] == o && a.step < a.src_len[q])
-                Pn += a.src_inc[a.src_off[q] + a.step];
-        Store<S>::st(x.P_out, o, Pn, x.isP);
+        for (int q = x.q0; q < x.q1; ++q)
+            if ((unsigned)a.src[q] == o && a.step < a.src[2 * a.nsrc + q])
+                Pn += a.src_inc[a.src[a.nsrc + q] + a.step];
+        Store<S>::st(a.out, o, Pn, a.isP);
     }
     vx_prev = vxn;
     P_c = cur.Pn;
     r_c = cur.rn;
+}
+
+template <typename S>
+__device__ __forceinline__ void gather_one(const FusedArgs<S> &a, int r)
+{
+    const int cp = a.rcomp[r];
+    const unsigned q = cp == SM_VX ? 1u : cp == SM_VY ? 2u : cp == SM_VZ ? 3u : 0u;
+    const float sc = cp == SM_P ? a.sP : a.sV;
+    a.trace[(size_t)a.gather_row * a.nrec + r] = Store<S>::ld(a.in, q * a.nnn + (unsigned)a.ridx[r], sc);
 }
 
 template <typename S, int TJ>
@@ -282,67 +268,80 @@ a3d_fused_step(FusedArgs<S> a)
     if ((int)blockIdx.x >= a.nblocks) {
         if (a.gather_row < 0) return;
         const int r = ((int)blockIdx.x - a.nblocks) * (64 * TJ) + threadIdx.y * 64 + threadIdx.x;
-        if (r < a.nrec) {
-            const int cp = a.rcomp[r];
-            const S *f = cp == SM_VX ? a.vx_in : cp == SM_VY ? a.vy_in : cp == SM_VZ ? a.vz_in : a.P_in;
-            const float sc = cp == SM_P ? a.sP : a.sV;
-            a.trace[(size_t)a.gather_row * a.nrec + r] = Store<S>::ld(f, (size_t)a.ridx[r], sc);
-        }
+        if (r < a.nrec) gather_one(a, r);
         return;
     }
 
-    Ctx<S> x;
-    x.P_in = a.P_in; x.vx_in = a.vx_in; x.vy_in = a.vy_in; x.vz_in = a.vz_in;
-    x.P_out = a.P_out; x.vx_out = a.vx_out; x.vy_out = a.vy_out; x.vz_out = a.vz_out;
-    x.mPx_in = a.mPx_in; x.mPy_in = a.mPy_in; x.mPz_in = a.mPz_in;
-    x.mPx_out = a.mPx_out; x.mPy_out = a.mPy_out; x.mPz_out = a.mPz_out;
-    x.mvx = a.mvx; x.mvy = a.mvy; x.mvz = a.mvz;
-    x.rho = a.rho; x.kdt = a.kdt;
-    x.nx2 = a.nx2; x.ny2 = a.ny2; x.nz2 = a.nz2; x.np = a.np; x.np2 = 2 * a.np;
-    x.plane = (size_t)a.ny2 * a.nz2;
-    x.sP = a.sP; x.isP = a.isP; x.sV = a.sV; x.isV = a.isV; x.sM = a.sM; x.isM = a.isM;
-    x.lane = threadIdx.x;
-    x.w = threadIdx.y;
+    const int lane = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);   // one wave per y row
+    const unsigned plane = (unsigned)a.ny2 * a.nz2;
+    const unsigned np2 = 2u * a.np;
 
     for (int item = blockIdx.x; item < a.nitems; item += a.nblocks) {
         // XCD-contiguous order: items of one XCD (item % 8) are consecutive
-        // logical tiles.  nitems is a multiple of 8; the padding items idle.
+        // logical tiles.  nitems is a multiple of 8; padding items idle.
         const int logical = (item % 8) * (a.nitems / 8) + item / 8;
-        __syncthreads();   // LDS of the previous item is no longer read
+        __syncthreads();   // the LDS of the previous item is no longer read
         if (logical >= a.nreal) continue;   // uniform per block
         const int chunk = logical / a.ntiles;
         const int tile = logical - chunk * a.ntiles;
         const int tj = tile / a.ntk;
         const int tk = tile - tj * a.ntk;
-        x.k = tk * 64 + x.lane;
-        x.j = tj * TJ + x.w;
-        x.active = x.k < x.nz2 && x.j < x.ny2;
-        x.kc = x.active ? x.k : 0;
-        x.jc = x.active ? x.j : 0;
-        x.jk = (size_t)x.jc * x.nz2 + x.kc;
-        x.syv = x.active && x.j < x.ny2 - 1 ? slab_vel(x.j, x.ny2, x.np) : -1;
-        x.szv = x.active && x.k < x.nz2 - 1 ? slab_vel(x.k, x.nz2, x.np) : -1;
-        x.syp = x.active ? slab_pre(x.j, x.ny2, x.np) : -1;
-        x.szp = x.active ? slab_pre(x.k, x.nz2, x.np) : -1;
-        x.syh = x.active && x.j >= 1 ? slab_vel(x.j - 1, x.ny2, x.np) : -1;
-        x.szh = x.active && x.k >= 1 ? slab_vel(x.k - 1, x.nz2, x.np) : -1;
+        // 63 owned cells per wave along z; lane 0 sits on the last cell of
+        // the tile to the left and only supplies vz at k0-1
+        Lane L;
+        L.kk = tk * 63 - 1 + lane;
+        L.j = tj * TJ + w;
+        const bool jv = L.j < a.ny2, kv = L.kk >= 0 && L.kk < a.nz2;
+        L.own = jv && kv && lane >= 1;
+        const int jc = min(L.j, a.ny2 - 1);
+        L.kc = min(max(L.kk, 0), a.nz2 - 1);
+        const int jp = min(jc + 1, a.ny2 - 1), kp = min(L.kc + 1, a.nz2 - 1), jm = max(jc - 1, 0);
+        L.o_jk = (unsigned)jc * a.nz2 + L.kc;
+        L.o_jp = (unsigned)jp * a.nz2 + L.kc;
+        L.o_kp = (unsigned)jc * a.nz2 + kp;
+        L.o_jm = (unsigned)jm * a.nz2 + L.kc;
+        L.syv = jv && L.j < a.ny2 - 1 ? slab_vel(L.j, a.ny2, a.np) : -1;
+        L.syp = jv ? slab_pre(L.j, a.ny2, a.np) : -1;
+        L.syh = jv && L.j >= 1 ? slab_vel(L.j - 1, a.ny2, a.np) : -1;
+        L.szv = kv && L.kk < a.nz2 - 1 ? slab_vel(L.kk, a.nz2, a.np) : -1;
+        L.szp = kv ? slab_pre(L.kk, a.nz2, a.np) : -1;
+        L.mz_v = (unsigned)jc * np2 + (L.szv < 0 ? 0 : L.szv);
+        L.mz_p = (unsigned)jc * np2 + (L.szp < 0 ? 0 : L.szp);
+        {
+            // y coefficients of the row (wave-uniform) and of the halo row,
+            // z coefficients of the lane's column; tables a, b, 1/k, ah, bh, 1/kh
+            const float *ty = a.tab + a.toff_y, *tz = a.tab + a.toff_z;
+            const int ny2 = a.ny2, nz2 = a.nz2;
+            const int jn = max(min(L.j, ny2 - 2), 0), jh = max(min(L.j - 1, ny2 - 2), 0);
+            const int kn = max(min(L.kc, nz2 - 2), 0);
+            L.ybh = ty[4 * ny2 + jn]; L.yah = ty[3 * ny2 + jn]; L.yikh = ty[5 * ny2 + jn];
+            L.hbh = ty[4 * ny2 + jh]; L.hah = ty[3 * ny2 + jh]; L.hikh = ty[5 * ny2 + jh];
+            L.yb = ty[ny2 + jc]; L.ya = ty[jc]; L.yik = ty[2 * ny2 + jc];
+            L.zbh = ld_at(tz, 4 * nz2 + kn); L.zah = ld_at(tz, 3 * nz2 + kn);
+            L.zikh = ld_at(tz, 5 * nz2 + kn);
+            L.zb = ld_at(tz, nz2 + L.kc); L.za = ld_at(tz, L.kc); L.zik = ld_at(tz, 2 * nz2 + L.kc);
+        }
         const int i0 = chunk * a.chunk_len;
-        const int i1 = min(x.nx2, i0 + a.chunk_len);
-        if (i0 >= x.nx2) continue;   // uniform; the host never makes empty chunks
+        const int i1 = min(a.nx2, i0 + a.chunk_len);
+        if (i0 >= a.nx2) continue;   // uniform; the host never makes empty chunks
         const int ist = i0 > 0 ? i0 - 1 : 0;
 
-        float P_c = 0.f, r_c = 1.f, vx_prev = 0.f;
-        if (x.active) {
-            const size_t o = (size_t)ist * x.plane + x.jk;
-            P_c = Store<S>::ld(x.P_in, o, x.sP);
-            r_c = x.rho[o];
-        }
+        float P_c = Store<S>::ld(a.in, (unsigned)ist * plane + L.o_jk, a.sP);
+        float r_c = ld_at(a.mat, (unsigned)ist * plane + L.o_jk);
+        float vx_prev = 0.f;
         Col A, B;
-        load_col<S>(x, ist, A);
+        A.Pjm = A.rjm = A.vyjm = A.mPym = 0.f;
+        A.mPx = A.mPy = A.mPz = A.mvx = A.mvy = A.mvz = 0.f;
+        B = A;
+        {
+            const XPlane xs = a.xpl[ist];
+            load_col<S>(a, L, w, ist, plane, xs.sxv, xs.sxp, A);
+        }
         for (int i = ist; i < i1; i += 2) {
-            plane_step<S, TJ>(x, a, i, i0, i1, A, B, P_c, r_c, vx_prev, s_vy);
+            plane_step<S, TJ>(a, L, w, i, i0, i1, plane, A, B, P_c, r_c, vx_prev, s_vy);
             if (i + 1 >= i1) break;
-            plane_step<S, TJ>(x, a, i + 1, i0, i1, B, A, P_c, r_c, vx_prev, s_vy);
+            plane_step<S, TJ>(a, L, w, i + 1, i0, i1, plane, B, A, P_c, r_c, vx_prev, s_vy);
         }
     }
 }
@@ -353,10 +352,7 @@ a3d_gather(FusedArgs<S> a)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (a.gather_row < 0 || r >= a.nrec) return;
-    const int cp = a.rcomp[r];
-    const S *f = cp == SM_VX ? a.vx_in : cp == SM_VY ? a.vy_in : cp == SM_VZ ? a.vz_in : a.P_in;
-    const float sc = cp == SM_P ? a.sP : a.sV;
-    a.trace[(size_t)a.gather_row * a.nrec + r] = Store<S>::ld(f, (size_t)a.ridx[r], sc);
+    gather_one(a, r);
 }
 
 template <typename S, int TJ>

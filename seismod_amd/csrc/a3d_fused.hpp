@@ -1,4 +1,7 @@
 // a3d_fused.hpp -- argument block of the fused 3D step kernel.
+//
+// Kept small on purpose: every field is uniform and lives in SGPRs for the
+// whole kernel, so arrays are passed as a few base pointers plus offsets.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,6 +13,15 @@
 
 namespace sm {
 
+// per-plane constants of the x axis, read with one scalar load per plane
+struct XPlane {
+    float bh, ah, ikh;   // half-node CPML coefficients (velocity at i)
+    float b, a, ik;      // node CPML coefficients (pressure at i)
+    int sxv, sxp;        // x slab coordinates (velocity, pressure), -1 outside
+    int q0, q1;          // sources of this plane: sorted entries [q0, q1)
+    int pad[6];
+};
+
 template <typename S>
 struct FusedArgs {
     int nx2, ny2, nz2, np;
@@ -18,25 +30,34 @@ struct FusedArgs {
     int nitems;           // nreal rounded up to a multiple of 8
     int nblocks;          // persistent stencil blocks (grid = nblocks + gather_blocks)
     int gather_blocks;
-    // state in/out (ping-pong) and memory variables
-    const S *P_in, *vx_in, *vy_in, *vz_in;
-    S *P_out, *vx_out, *vy_out, *vz_out;
-    const S *mPx_in, *mPy_in, *mPz_in;
-    S *mPx_out, *mPy_out, *mPz_out;
-    S *mvx, *mvy, *mvz;                    // pressure memory variables, in place
-    const float *rho, *kdt;
-    // CPML tables (node: a, b, 1/kappa; half node: ah, bh, 1/kappa_h)
-    const float *ax, *bx, *ikx, *ay, *by, *iky, *az, *bz, *ikz;
-    const float *ahx, *bhx, *ikhx, *ahy, *bhy, *ikhy, *ahz, *bhz, *ikhz;
+    unsigned nnn;         // cells per field
+    // state: P, vx, vy, vz at element q*nnn of in / out (ping-pong)
+    const S *in;
+    S *out;
+    // memory variables: x slabs at 0, y slabs at my, z slabs at mz
+    // (velocity set ping-ponged, pressure set updated in place)
+    const S *mem_in;
+    S *mem_out;
+    S *memp;
+    unsigned my, mz;
+    // material: rho at 0, K*dt at nnn
+    const float *mat;
+    // per-plane x constants, nx2 records
+    const XPlane *xpl;
+    // y and z CPML tables at tab + toff: a, b, 1/kappa, ah, bh, 1/kappa_h
+    // (n2 entries each)
+    const float *tab;
+    int toff_y, toff_z;
     float idx, idy, idz, two_dt;
     // per-field power-of-two scales: stored value * s = physical value
     float sP, isP, sV, isV, sM, isM;
-    // sources sorted by cell; src_plane[i]..src_plane[i+1] lists plane i
-    const int *src_plane;
-    const int64_t *src_cell, *src_off, *src_len;
+    // sources sorted by cell: cell, off, len (int64, nsrc each) at src,
+    // increments (float) at src_inc
+    const int64_t *src;
     const float *src_inc;
+    int nsrc;
     int64_t step;
-    // receivers of the previous step
+    // receivers of the previous step (used by the gather blocks only)
     int nrec;
     const int64_t *ridx;
     const int *rcomp;

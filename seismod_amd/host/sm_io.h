@@ -1,0 +1,103 @@
+/*
+ * sm_io.h -- input and output of the drop-in e_iso and a_iso_3d programs.
+ *
+ * The file formats, keywords, defaults and messages follow seismod's
+ * io_utils.c; each function names the routine it stands in for.
+ */
+#ifndef SM_IO_H
+#define SM_IO_H
+
+#include <stddef.h>
+#include <stdio.h>
+
+#define SM_NAME_LEN 80
+
+enum { REC_TRACE = 0, REC_SNAPSHOT = 1 };
+
+/* run parameters of the .par file (struct inputParams, structs.h:80-105) */
+typedef struct {
+    double dt;          /* s */
+    double duration;    /* s */
+    char modelfile[SM_NAME_LEN], sourcefile[SM_NAME_LEN], outputfile[SM_NAME_LEN];
+    char basename[SM_NAME_LEN];
+    int iwipe;          /* abs */
+    int plot_strips;    /* plstr */
+    int segy;
+    int shotpt_no;
+    int simulate_cmp;
+    /* absorbing layers (struct abs_params) */
+    size_t np;
+    double vmax, pml_order, Rc, kappa_max;
+    int alpha_max;
+} sm_params;
+
+typedef struct {
+    size_t nx, ny, nz;      /* ny = 1 for 2D */
+    double dx, dy, dz;
+    double x0, y0, z0;
+} sm_grid;
+
+typedef struct {
+    int type;               /* enum sourceType numbering */
+    double A, f, x, y, z;
+    size_t idx;
+} sm_source;
+
+typedef struct {
+    size_t nsrc, ntemplate;
+    sm_source *s;
+} sm_sources;
+
+typedef struct {
+    int comp;               /* enum component numbering */
+    int type;               /* REC_TRACE / REC_SNAPSHOT */
+    double x, y, z, t0, dt;
+    size_t idx;
+    FILE *fid;
+    float *data;            /* SEG-Y samples */
+    size_t nsamples;
+} sm_record;
+
+typedef struct {
+    size_t nrec;
+    sm_record *r;
+    char basename[SM_NAME_LEN];
+} sm_records;
+
+extern int sm_verbose;
+
+/* set_defaults / set_defaults_3d (io_utils.c:63-135) */
+void sm_set_defaults(sm_params *p, int three_d);
+/* process_args (io_utils.c:156-294): exits like the reference on -h,
+ * missing -p, unreadable .par; -c (checkpoint restart) is not supported
+ * by these programs and is rejected */
+void sm_process_args(int argc, char *const argv[], sm_params *p, const char *usage_text);
+/* read_grid_params / read_grid3d_params (io_utils.c:296-328, 384-401) */
+void sm_read_grid2d(const char *fn, sm_grid *g);
+void sm_read_grid3d(const char *fn, sm_grid *g);
+/* read_model_e (io_utils.c:892-947): nx*nz arrays, with the 1-D shortcut */
+void sm_read_model_e(const char *fn, const sm_grid *g, double *vp, double *vs, double *rho);
+/* read_model_a (io_utils.c:1029-1153): padded arrays, edges replicated */
+void sm_read_model_a(const char *fn, const sm_grid *g, size_t np, double *vp, double *rho);
+/* read_source / read_source_3d (io_utils.c:1432-1589) */
+void sm_read_source2d(const char *fn, const sm_grid *g, sm_sources *src);
+void sm_read_source3d(const char *fn, const sm_grid *g, size_t np, sm_sources *src);
+/* read_output / read_output_3d (io_utils.c:1592-1741) */
+void sm_read_output2d(const char *fn, const sm_grid *g, sm_records *out);
+void sm_read_output3d(const char *fn, const sm_grid *g, size_t np, sm_records *out);
+/* check_dt_trc (io_utils.c:3561-3574) */
+void sm_check_dt_trc(const sm_records *out, double dt);
+/* write_trace (io_utils.c:1744-1768): one "%le    %le" line, t and 1e3*v */
+void sm_write_trace(sm_records *out, size_t nr, double t, double value);
+void sm_close_traces(sm_records *out);
+/* write_snapshot_nc / write_snapshot3d_nc (io_utils.c:1822-2180): data is
+ * an nxp x nzp i-major slice (nxp = nx+2np, nzp = nz+2np) */
+int sm_write_snapshot_nc(const double *data, double t, size_t nx, size_t nz, size_t np,
+                         double x0, double z0, double dx, double dz, const char *basename,
+                         size_t nr, int plstr);
+/* save_segy (io_utils.c:3269-3559) */
+int sm_save_segy(const sm_params *p, const sm_records *out, const sm_sources *src);
+void sm_free_sources(sm_sources *s);
+void sm_free_records(sm_records *r);
+
+#endif

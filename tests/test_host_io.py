@@ -1,0 +1,281 @@
+"""CPU tests of the drop-in programs' input/output layer (seismod_amd/host).
+
+The formats are those of seismod's io_utils.c: the .par grammar
+(process_args, io_utils.c:156-294), model/source/record readers
+(io_utils.c:892-947, 1029-1153, 1432-1741), the trace text format
+(1744-1768), netCDF classic snapshots (1822-2180) and SEG-Y (3269-3559).
+Snapshots are read back with scipy's independent netCDF-3 reader.
+"""
+import ctypes as C
+import os
+import struct
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+LIB = C.CDLL(str(ROOT / "seismod_amd" / "lib" / "libsm_hostio.so"))
+LIBC = C.CDLL(None)
+BIN = ROOT / "seismod_amd" / "bin"
+
+
+class Params(C.Structure):
+    _fields_ = [("dt", C.c_double), ("duration", C.c_double),
+                ("modelfile", C.c_char * 80), ("sourcefile", C.c_char * 80),
+                ("outputfile", C.c_char * 80), ("basename", C.c_char * 80),
+                ("iwipe", C.c_int), ("plot_strips", C.c_int), ("segy", C.c_int),
+                ("shotpt_no", C.c_int), ("simulate_cmp", C.c_int), ("np", C.c_size_t),
+                ("vmax", C.c_double), ("pml_order", C.c_double), ("Rc", C.c_double),
+                ("kappa_max", C.c_double), ("alpha_max", C.c_int)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_size_t), ("ny", C.c_size_t), ("nz", C.c_size_t),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double)]
+
+
+class Source(C.Structure):
+    _fields_ = [("type", C.c_int), ("A", C.c_double), ("f", C.c_double), ("x", C.c_double),
+                ("y", C.c_double), ("z", C.c_double), ("idx", C.c_size_t)]
+
+
+class Sources(C.Structure):
+    _fields_ = [("nsrc", C.c_size_t), ("ntemplate", C.c_size_t), ("s", C.POINTER(Source))]
+
+
+class Record(C.Structure):
+    _fields_ = [("comp", C.c_int), ("type", C.c_int), ("x", C.c_double), ("y", C.c_double),
+                ("z", C.c_double), ("t0", C.c_double), ("dt", C.c_double), ("idx", C.c_size_t),
+                ("fid", C.c_void_p), ("data", C.POINTER(C.c_float)), ("nsamples", C.c_size_t)]
+
+
+class Records(C.Structure):
+    _fields_ = [("nrec", C.c_size_t), ("r", C.POINTER(Record)), ("basename", C.c_char * 80)]
+
+
+def parse_par(tmp_path, text, three_d=False):
+    par = tmp_path / "run.par"
+    par.write_text(text)
+    p = Params()
+    LIB.sm_set_defaults(C.byref(p), 1 if three_d else 0)
+    C.c_int.in_dll(LIBC, "optind").value = 1
+    argv = (C.c_char_p * 4)(b"prog", b"-p", str(par).encode(), None)
+    LIB.sm_process_args(3, argv, C.byref(p), None)
+    return p
+
+
+def test_par_grammar(tmp_path):
+    p = parse_par(tmp_path, """
+model.dat          # model, model file
+src.dat            # source, source file
+rcv.dat            # output, description of output records
+
+shot7              # basename, common for all output files
+0.9                # time, window in s
+0.6                # dt, time step in ms
+1                  # abs, absorbing boundary
+12                 # nl, number of PML layers
+3.5                # kappa_max, PML parameter
+0                  # alpha_max, PML parameter
+0.001              # Rc, reflection coefficient
+1                  # segy,save traces in segy format
+42                 # unknown_keyword, ignored
+""")
+    assert p.modelfile == b"model.dat" and p.sourcefile == b"src.dat" and p.outputfile == b"rcv.dat"
+    assert p.basename == b"shot7"
+    assert p.duration == 0.9 and p.dt == 0.6 * 0.001
+    assert p.np == 12 and p.kappa_max == 3.5 and p.alpha_max == 0 and p.Rc == 0.001
+    assert p.segy == 1
+
+
+def test_par_defaults_and_abs0(tmp_path):
+    p = parse_par(tmp_path, "m.dat # model,\n30 # nl, layers\n")
+    # io_utils.c:63-98 defaults; abs 0 (the default) removes the layers (293)
+    assert p.dt == 1e-4 and p.duration == 1.2 and p.basename == b"run01"
+    assert p.np == 0 and p.vmax == 5000.0 and p.pml_order == 2.0 and p.Rc == 1e-5
+
+
+def write_model3d(path, vp, rho, dx=10.0, origin=(0.0, 0.0, 0.0), column=False):
+    nx, ny, nz = vp.shape
+    with open(path, "w") as f:
+        f.write(f"{nx} {ny} {nz}\n{dx} {dx} {dx}\n{origin[0]} {origin[1]} {origin[2]}\n"
+                "isotropic_acoustic_3d\n")
+        cols = [(0, 0)] if column else [(i, j) for i in range(nx) for j in range(ny)]
+        for i, j in cols:
+            for k in range(nz):
+                f.write(f"{float(vp[i, j, k])!r} {float(rho[i, j, k])!r}\n")
+
+
+@pytest.mark.parametrize("column", [False, True])
+def test_read_model_a_pads_by_edge_replication(tmp_path, column):
+    rng = np.random.default_rng(1)
+    nx, ny, nz, np_ = 4, 3, 5, 2
+    vp = rng.uniform(1500, 4500, (nx, ny, nz))
+    rho = rng.uniform(1800, 2600, (nx, ny, nz))
+    if column:
+        vp[:] = vp[0:1, 0:1, :]
+        rho[:] = rho[0:1, 0:1, :]
+    fn = tmp_path / "m3.dat"
+    write_model3d(fn, vp, rho, column=column)
+    g = Grid()
+    LIB.sm_read_grid3d(str(fn).encode(), C.byref(g))
+    assert (g.nx, g.ny, g.nz, g.dx) == (nx, ny, nz, 10.0)
+    n2 = [n + 2 * np_ for n in (nx, ny, nz)]
+    out_vp = np.zeros(n2)
+    out_rho = np.zeros(n2)
+    LIB.sm_read_model_a(str(fn).encode(), C.byref(g), C.c_size_t(np_),
+                        out_vp.ctypes.data_as(C.c_void_p), out_rho.ctypes.data_as(C.c_void_p))
+    assert np.array_equal(out_vp, np.pad(vp, np_, mode="edge"))
+    assert np.array_equal(out_rho, np.pad(rho, np_, mode="edge"))
+
+
+def test_read_model_e_full_and_column(tmp_path):
+    nx, nz = 5, 4
+    rng = np.random.default_rng(2)
+    vals = rng.uniform(1000, 3000, (nx, nz, 3))
+    for column in (False, True):
+        fn = tmp_path / f"m2_{column}.dat"
+        with open(fn, "w") as f:
+            f.write(f"{nx} {nz}\n5.0 5.0\n0.0 0.0\nisotropic_elastic\n")
+            for i in range(1 if column else nx):
+                for j in range(nz):
+                    f.write(" ".join(repr(float(v)) for v in vals[i, j]) + "\n")
+        g = Grid()
+        LIB.sm_read_grid2d(str(fn).encode(), C.byref(g))
+        a = [np.zeros((nx, nz)) for _ in range(3)]
+        LIB.sm_read_model_e(str(fn).encode(), C.byref(g), *(x.ctypes.data_as(C.c_void_p) for x in a))
+        expect = np.broadcast_to(vals[0:1], vals.shape) if column else vals
+        for q in range(3):
+            assert np.array_equal(a[q], expect[:, :, q])
+
+
+def test_read_sources(tmp_path):
+    g = Grid(nx=400, ny=1, nz=400, dx=5.0, dy=5.0, dz=5.0)
+    fn = tmp_path / "src.dat"
+    fn.write_text("2\n1\nBulk\n1.0\n25.0\n1000.0 250.0\nSz\n-2.0\n30.0\n1002.4 247.6\n")
+    s = Sources()
+    LIB.sm_read_source2d(str(fn).encode(), C.byref(g), C.byref(s))
+    assert (s.nsrc, s.ntemplate) == (2, 1)
+    assert (s.s[0].type, s.s[0].idx) == (6, 200 * 400 + 50)        # Bulk at node (200, 50)
+    assert (s.s[1].type, s.s[1].A, s.s[1].idx) == (2, -2.0, 200 * 400 + 50)
+    # 9-point template: weights 1/2 centre, 1/4 edges, 1/8 corners (io_utils.c:1494-1535)
+    fn.write_text("1\n9\nBulk\n2.0\n25.0\n1000.0 250.0\n")
+    LIB.sm_read_source2d(str(fn).encode(), C.byref(g), C.byref(s))
+    A = [s.s[q].A for q in range(9)]
+    assert A == [1.0, 0.25, 0.5, 0.25, 0.5, 0.5, 0.25, 0.5, 0.25]
+    idx = sorted(s.s[q].idx for q in range(9))
+    assert idx == sorted((200 + di) * 400 + 50 + dj for di in (-1, 0, 1) for dj in (-1, 0, 1))
+    g3 = Grid(nx=256, ny=256, nz=256, dx=10.0, dy=10.0, dz=10.0)
+    fn.write_text("1\n1\nBulk\n1.0\n15.0\n1280.0 1280.0 100.0\n")
+    LIB.sm_read_source3d(str(fn).encode(), C.byref(g3), C.c_size_t(20), C.byref(s))
+    n2 = 296
+    assert s.s[0].idx == ((128 + 20) * n2 + 128 + 20) * n2 + 10 + 20
+
+
+def test_read_records(tmp_path):
+    g3 = Grid(nx=32, ny=30, nz=28, dx=10.0, dy=10.0, dz=10.0, x0=100.0, y0=-50.0, z0=0.0)
+    fn = tmp_path / "rcv.dat"
+    fn.write_text("3\nVz\nTrace\n154.9 -50.0 35.0\n0.0\n0.6\nP\nTrace\n100 0 0 0.1 1.2\n"
+                  "Vx\nSnapshot\n0.05\n0.0\n")
+    out = Records()
+    LIB.sm_read_output3d(str(fn).encode(), C.byref(g3), C.c_size_t(3), C.byref(out))
+    assert out.nrec == 3
+    r0, r1, r2 = out.r[0], out.r[1], out.r[2]
+    nx2, ny2, nz2 = 38, 36, 34
+    # lround: x 5.49 -> 5, y 0, z 3.5 -> 4 (io_utils.c:1722-1725)
+    assert (r0.comp, r0.type, r0.idx) == (2, 0, ((5 + 3) * ny2 + 3) * nz2 + 4 + 3)
+    assert abs(r0.dt - 0.6e-3) < 1e-18 and r0.t0 == 0.0
+    assert (r1.comp, r1.idx, r1.t0) == (8, ((0 + 3) * ny2 + 5 + 3) * nz2 + 3, 0.1)
+    assert (r2.comp, r2.type, r2.t0, r2.dt) == (0, 1, 0.05, 0.0)
+
+
+def test_snapshot_netcdf_roundtrip(tmp_path, monkeypatch):
+    from scipy.io import netcdf_file
+    monkeypatch.chdir(tmp_path)
+    nx, nz, np_ = 7, 5, 2
+    rng = np.random.default_rng(3)
+    data = rng.standard_normal((nx + 2 * np_, nz + 2 * np_)) * 1e-3
+    for plstr in (0, 1):
+        LIB.sm_write_snapshot_nc(data.ctypes.data_as(C.c_void_p), C.c_double(0.125), C.c_size_t(nx),
+                                 C.c_size_t(nz), C.c_size_t(np_), C.c_double(100.0),
+                                 C.c_double(-20.0), C.c_double(10.0), C.c_double(10.0), b"shot",
+                                 C.c_size_t(plstr), C.c_int(plstr))
+        fn = tmp_path / "ssh" / "shot" / f"shot_{plstr + 1}_0.125000.nc"
+        with netcdf_file(str(fn), "r", mmap=False) as f:
+            sl = slice(None) if plstr else slice(np_, -np_)
+            sub = data[sl, sl]
+            z = f.variables["z"]
+            assert z.dimensions == ("y", "x")
+            assert np.array_equal(z[:], (sub.astype(np.float32) * np.float32(1e3)).T)
+            x = f.variables["x"][:]
+            y = f.variables["y"][:]
+            off = np_ if plstr else 0
+            assert np.allclose(x, 100.0 + (np.arange(len(x)) - off) * 10.0)
+            assert np.allclose(y, -20.0 + (np.arange(len(y)) - off) * 10.0)
+            assert f.Conventions == b"COARDS/CF-1.0\x00"[:len(f.Conventions)] or \
+                f.Conventions.rstrip(b"\x00") == b"COARDS/CF-1.0"
+            assert f.title.rstrip(b"\x00") == b"pve_iso"
+            assert int(np.asarray(f.node_offset).ravel()[0]) == 0
+            assert np.allclose(z.actual_range, [1e3 * sub.min(), 1e3 * sub.max()])
+            assert float(np.asarray(z.scale_factor).ravel()[0]) == 1.0
+            assert np.isnan(np.asarray(z._FillValue).ravel()[0])
+            assert z.units.rstrip(b"\x00") == b"mm/s"
+
+
+def test_segy_layout(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    p = Params()
+    LIB.sm_set_defaults(C.byref(p), 1)
+    p.basename = b"line1"
+    p.duration = 0.01
+    p.shotpt_no = 7
+    ns = 10
+    recs = (Record * 2)()
+    bufs = []
+    for q in range(2):
+        recs[q].comp = 2
+        recs[q].type = 0
+        recs[q].x = 100.0 + 20 * q
+        recs[q].z = 30.0
+        recs[q].dt = 0.001
+        arr = (C.c_float * (ns + 1))(*[float(q * 100 + s) for s in range(ns + 1)])
+        bufs.append(arr)
+        recs[q].data = C.cast(arr, C.POINTER(C.c_float))
+        recs[q].nsamples = ns
+    out = Records(nrec=2, r=C.cast(recs, C.POINTER(Record)), basename=b"line1")
+    src = (Source * 1)()
+    src[0].x = 60.0
+    src[0].z = 5.0
+    srcs = Sources(nsrc=1, ntemplate=1, s=C.cast(src, C.POINTER(Source)))
+    LIB.sm_save_segy(C.byref(p), C.byref(out), C.byref(srcs))
+    b = (tmp_path / "segy" / "line1.sgy").read_bytes()
+    assert len(b) == 3600 + 2 * (240 + 4 * ns)
+    assert b[0:1] == b"C" and b[80:81] == b"C" and b[1:80] == b" " * 79
+    h = lambda off: struct.unpack_from("<h", b, off)[0]
+    i = lambda off: struct.unpack_from("<i", b, off)[0]
+    assert (i(3200), h(3212), h(3216), h(3220), h(3224), h(3226)) == (1, 2, 1000, 10, 5, 2)
+    assert h(3500) == 1
+    t1 = 3600 + 240 + 4 * ns
+    assert (i(3600), i(3616), h(3628), i(3636), i(3640), i(3648)) == (1, 7, 12, 40, -30, 5)
+    assert (i(t1), i(t1 + 36)) == (2, 60)
+    assert (h(3600 + 114), h(3600 + 116)) == (ns, 1000)
+    samples = np.frombuffer(b, dtype="<f4", count=ns, offset=3840)
+    assert np.array_equal(samples, np.arange(ns, dtype=np.float32))
+
+
+def run_bin(prog, *args, cwd=None):
+    return subprocess.run([str(BIN / prog), *args], capture_output=True, text=True, cwd=cwd,
+                          timeout=60)
+
+
+@pytest.mark.parametrize("prog", ["e_iso", "a_iso_3d"])
+def test_cli_errors(prog, tmp_path):
+    assert run_bin(prog, "-h").returncode == 0
+    r = run_bin(prog)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    r = run_bin(prog, "-p", str(tmp_path / "missing.par"))
+    assert r.returncode == 1 and "cannot open parameter file" in r.stderr
+    assert run_bin(prog, "-c", "chk.dat").returncode == 1
