@@ -168,7 +168,10 @@ struct sm_a3d {
     int tj = 4, nchunks = 1;        // fused launch shape
     int nblocks = 0;                // persistent grid (0: one block per item)
     int cus = 0;                    // compute units of the device
-    float sP = 1.f, sV = 1.f, sM = 1.f;          // storage scales of 16-bit formats
+    // storage scales of the 16-bit formats (P, v, velocity-side and
+    // pressure-side memory variables); 1 for FP32 and BF16
+    float sP = 1.f, sV = 1.f, sMv = 1.f, sMp = 1.f;
+    double rc_min = 0.0, d_min = 0.0;   // min rho*vp and min spacing, for the scales
     bool timing = false;
     double timed_ms = 0.0;
     size_t timed_launches = 0;
@@ -413,7 +416,7 @@ FusedArgs<S> fused_args(sm_a3d *s)
     a.idx = (float)(1.0 / s->p.dx); a.idy = (float)(1.0 / s->p.dy); a.idz = (float)(1.0 / s->p.dz);
     a.two_dt = (float)(2.0 * s->p.dt);
     a.sP = s->sP; a.isP = 1.f / s->sP; a.sV = s->sV; a.isV = 1.f / s->sV;
-    a.sM = s->sM; a.isM = 1.f / s->sM;
+    a.sMv = s->sMv; a.isMv = 1.f / s->sMv; a.sMp = s->sMp; a.isMp = 1.f / s->sMp;
     const char *sb = (const char *)s->src.ptr;
     a.src = (const int64_t *)sb;
     a.src_inc = (const float *)(sb + s->src_inc_off);
@@ -685,16 +688,10 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
             rc = upload<float>(s->mat.ptr, rho, nnn, st);
             if (rc == SM_OK) rc = upload<float>((float *)s->mat.ptr + nnn, kdt.data(), nnn, st);
         }
-        // 16-bit storage: power-of-two scales so the expected magnitudes sit
-        // well inside the format (P in Pa ~ 1e3..1e5, v ~ P/(rho c))
-        if (p.storage == SM_STORE_F16) {
-            double rc_min = 1e300;
-            for (size_t n = 0; n < nnn; ++n) rc_min = std::min(rc_min, rho[n] * std::max(vp[n], 1.0));
-            s->sP = std::ldexp(1.0f, 6);                      // P up to 4e6 Pa
-            const double vmax_est = 4.0e6 / rc_min;           // matching particle velocity
-            s->sV = std::ldexp(1.0f, (int)std::ceil(std::log2(vmax_est / 3.0e4)));
-            s->sM = std::ldexp(1.0f, (int)std::ceil(std::log2(vmax_est / p.dx / 3.0e4)));
-        }
+        double rcm = 1e300;
+        for (size_t n = 0; n < nnn; ++n) rcm = std::min(rcm, rho[n] * std::max(vp[n], 1.0));
+        s->rc_min = rcm;
+        s->d_min = std::min(p.dx, std::min(p.dy, p.dz));
     }
     // CPML tables (a_iso_3d.c:525-576)
     if (rc == SM_OK) {
@@ -796,6 +793,27 @@ int sm_a3d_set_sources(sm_a3d *s, size_t nsrc, const size_t *idx, const size_t *
     SM_HIP(hipMemcpyAsync(s->src.ptr, h.data(), bytes, hipMemcpyHostToDevice, s->ctx->stream));
     SM_HIP(hipStreamSynchronize(s->ctx->stream));
     s->nsrc = nsrc;
+    // FP16 storage: power-of-two scales from the injected pressure, while the
+    // state is still zero (a rescale of a live state would be needed later).
+    // The largest pressure is bounded by the total injected at one cell; the
+    // velocities, dP/dx and dv/dx scale with it.  Targets sit 8x below the
+    // FP16 maximum (65504).
+    if (s->p.storage == SM_STORE_F16 && s->step == 0 && nsrc > 0) {
+        double pmax = 0.0;
+        for (size_t n = 0; n < nsrc; ++n) {
+            double acc = 0.0;
+            for (size_t t = 0; t < length[n]; ++t) acc += std::fabs(s->p.dt * fct[offs[n] + t] * 1.e6);
+            pmax = std::max(pmax, acc);
+        }
+        if (pmax > 0.0) {
+            auto p2 = [](double v) { return std::ldexp(1.0f, (int)std::ceil(std::log2(v / 8192.0))); };
+            const double vmax = pmax / s->rc_min;
+            s->sP = p2(pmax);
+            s->sV = p2(vmax);
+            s->sMv = p2(2.0 * pmax / s->d_min);
+            s->sMp = p2(2.0 * vmax / s->d_min);
+        }
+    }
     return upload_xplanes(s);
 }
 

@@ -149,16 +149,16 @@ __device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, i
         c.rjm = ld_at(a.mat, o + L.o_jm);
         c.vyjm = Store<S>::ld(a.in, F_VY * a.nnn + o + L.o_jm, a.sV);
         if (L.syh >= 0)
-            c.mPym = Store<S>::ld(a.mem_in, a.my + ((unsigned)i * np2 + L.syh) * a.nz2 + L.kc, a.sM);
+            c.mPym = Store<S>::ld(a.mem_in, a.my + ((unsigned)i * np2 + L.syh) * a.nz2 + L.kc, a.sMv);
     }
-    if (sxv >= 0) c.mPx = Store<S>::ld(a.mem_in, (unsigned)sxv * plane + L.o_jk, a.sM);
+    if (sxv >= 0) c.mPx = Store<S>::ld(a.mem_in, (unsigned)sxv * plane + L.o_jk, a.sMv);
     if (L.syv >= 0)
-        c.mPy = Store<S>::ld(a.mem_in, a.my + ((unsigned)i * np2 + L.syv) * a.nz2 + L.kc, a.sM);
-    if (L.szv >= 0) c.mPz = Store<S>::ld(a.mem_in, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_v, a.sM);
-    if (sxp >= 0) c.mvx = Store<S>::ld(a.memp, (unsigned)sxp * plane + L.o_jk, a.sM);
+        c.mPy = Store<S>::ld(a.mem_in, a.my + ((unsigned)i * np2 + L.syv) * a.nz2 + L.kc, a.sMv);
+    if (L.szv >= 0) c.mPz = Store<S>::ld(a.mem_in, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_v, a.sMv);
+    if (sxp >= 0) c.mvx = Store<S>::ld(a.memp, (unsigned)sxp * plane + L.o_jk, a.sMp);
     if (L.syp >= 0)
-        c.mvy = Store<S>::ld(a.memp, a.my + ((unsigned)i * np2 + L.syp) * a.nz2 + L.kc, a.sM);
-    if (L.szp >= 0) c.mvz = Store<S>::ld(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, a.sM);
+        c.mvy = Store<S>::ld(a.memp, a.my + ((unsigned)i * np2 + L.syp) * a.nz2 + L.kc, a.sMp);
+    if (L.szp >= 0) c.mvz = Store<S>::ld(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, a.sMp);
 }
 
 // one plane: prefetch plane i+1 into nxt, update plane i from cur
@@ -197,10 +197,10 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
         Store<S>::st(a.out, F_VX * a.nnn + o, vxn, a.isV);
         Store<S>::st(a.out, F_VY * a.nnn + o, vyn, a.isV);
         Store<S>::st(a.out, F_VZ * a.nnn + o, vzn, a.isV);
-        if (x.sxv >= 0) Store<S>::st(a.mem_out, (unsigned)x.sxv * plane + L.o_jk, mx, a.isM);
+        if (x.sxv >= 0) Store<S>::st(a.mem_out, (unsigned)x.sxv * plane + L.o_jk, mx, a.isMv);
         if (L.syv >= 0)
-            Store<S>::st(a.mem_out, a.my + ((unsigned)i * np2 + L.syv) * a.nz2 + L.kc, my, a.isM);
-        if (L.szv >= 0) Store<S>::st(a.mem_out, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_v, mz, a.isM);
+            Store<S>::st(a.mem_out, a.my + ((unsigned)i * np2 + L.syv) * a.nz2 + L.kc, my, a.isMv);
+        if (L.szv >= 0) Store<S>::st(a.mem_out, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_v, mz, a.isMv);
     }
     // vy of the row below the tile (owned by the tile below; not stored)
     float vy_halo = 0.f;
@@ -223,17 +223,17 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
             float dvdz = (vzn - vz_m) * a.idz;
             if (x.sxp >= 0) {
                 const float m = x.b * cur.mvx + x.a * dvdx;
-                Store<S>::st(a.memp, (unsigned)x.sxp * plane + L.o_jk, m, a.isM);
+                Store<S>::st(a.memp, (unsigned)x.sxp * plane + L.o_jk, m, a.isMp);
                 dvdx = dvdx * x.ik + m;
             }
             if (L.syp >= 0) {
                 const float m = L.yb * cur.mvy + L.ya * dvdy;
-                Store<S>::st(a.memp, a.my + ((unsigned)i * np2 + L.syp) * a.nz2 + L.kc, m, a.isM);
+                Store<S>::st(a.memp, a.my + ((unsigned)i * np2 + L.syp) * a.nz2 + L.kc, m, a.isMp);
                 dvdy = dvdy * L.yik + m;
             }
             if (L.szp >= 0) {
                 const float m = L.zb * cur.mvz + L.za * dvdz;
-                Store<S>::st(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, m, a.isM);
+                Store<S>::st(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, m, a.isMp);
                 dvdz = dvdz * L.zik + m;
             }
             Pn = P_c - cur.K * (dvdx + dvdy + dvdz);

@@ -50,7 +50,8 @@ struct FusedArgs {
     int toff_y, toff_z;
     float idx, idy, idz, two_dt;
     // per-field power-of-two scales: stored value * s = physical value
-    float sP, isP, sV, isV, sM, isM;
+    // (P, velocities, velocity-side and pressure-side memory variables)
+    float sP, isP, sV, isV, sMv, isMv, sMp, isMp;
     // sources sorted by cell: cell, off, len (int64, nsrc each) at src,
     // increments (float) at src_inc
     const int64_t *src;
