@@ -86,10 +86,13 @@ int sm_ctx_sync(sm_ctx *ctx);
 /* free and total device memory (hipMemGetInfo) of the context's device */
 int sm_ctx_mem_info(sm_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
 /* Calibration: a plain streaming kernel reading `reads` and writing
- * `writes` FP32 arrays of n values (known traffic 4n(reads+writes) bytes);
- * returns the mean time of `iters` launches.  No counterpart in the
- * reference: it scales HBM counters and measures achievable bandwidth. */
-int sm_probe_stream(sm_ctx *ctx, size_t n, int reads, int writes, int iters,
+ * `writes` FP32 arrays of n values (known traffic 4n(reads+writes) bytes),
+ * each lane moving `vec` (1, 2 or 4) consecutive values per array; returns
+ * the mean time of `iters` launches.  (reads, writes) is one of (1,1)
+ * (2,1) (3,1) (4,2) (6,4) (8,4) (10,4) (12,8) (1,2) (5,5).  No counterpart
+ * in the reference: it scales HBM counters and measures achievable
+ * bandwidth. */
+int sm_probe_stream(sm_ctx *ctx, size_t n, int reads, int writes, int vec, int iters,
                     double *ms_per_iter);
 
 /* ------------------------------------------------------------------ */
@@ -194,8 +197,11 @@ typedef struct {
                                  read is out of bounds and the true corner
                                  cell nx*nz-1 is used) */
     double damping_scale;     /* 1.0 reproduces e_iso.c:490-500 */
-    int flags;                /* reserved, 0 */
+    int flags;                /* SM_E2D_* */
 } sm_e2d_params;
+
+#define SM_E2D_NAIVE 1        /* FP32 with the reference-order two-kernel
+                                 step instead of the fused one */
 
 /* Build a solver from interior vp, vs, rho (nx*nz each, read_model_e,
  * io_utils.c:892-947): Lame constants and staggered averages
@@ -224,6 +230,34 @@ int sm_e2d_get_field_padded(sm_e2d *s, int comp, double *out);
 int sm_e2d_reset(sm_e2d *s);
 int sm_e2d_device_bytes(const sm_e2d *s, size_t *bytes);
 size_t sm_e2d_bytes_needed(const sm_e2d_params *p, size_t nrec, size_t max_steps);
+
+/* Launch shape of the fused FP32 step (no counterpart in the reference):
+ * x rows each wavefront marches over, and steps per captured hipGraph
+ * (0 or 1: plain launches).  Results do not depend on either. */
+int sm_e2d_set_launch(sm_e2d *s, int rows_per_chunk, int graph_steps);
+int sm_e2d_get_launch(const sm_e2d *s, int *rows_per_chunk, int *graph_steps);
+/* HIP-event timing of every sm_e2d_run (no counterpart in the reference) */
+int sm_e2d_set_timing(sm_e2d *s, int on);
+int sm_e2d_get_timing(const sm_e2d *s, double *ms_total, size_t *launches);
+
+/* ------------------------------------------------------------------ */
+/* Batches of independent 2D shots (no counterpart in the reference,   */
+/* which runs one shot per process).  One launch per time step advances */
+/* every shot that still has steps to run; long stretches are replayed  */
+/* from captured hipGraphs.  Each shot's results are those of running   */
+/* it alone with sm_e2d_run.                                            */
+/* ------------------------------------------------------------------ */
+typedef struct sm_e2d_batch sm_e2d_batch;
+int sm_e2d_batch_create(sm_ctx *ctx, sm_e2d_batch **out);
+int sm_e2d_batch_destroy(sm_e2d_batch *b);
+/* steps per captured graph (0 or 1: plain launches; default 32) */
+int sm_e2d_batch_set_graph_steps(sm_e2d_batch *b, int graph_steps);
+/* advance shots[k] (FP32 fused solvers on the batch's device, each listed
+ * once) by nsteps[k] steps; returns when all are done */
+int sm_e2d_batch_run(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots,
+                     const size_t *nsteps);
+int sm_e2d_batch_set_timing(sm_e2d_batch *b, int on);
+int sm_e2d_batch_get_timing(const sm_e2d_batch *b, double *ms_total, size_t *launches);
 
 #ifdef __cplusplus
 }

@@ -48,7 +48,8 @@ _SIGS = {
     "sm_ctx_destroy": (C.c_int, [C.c_void_p]),
     "sm_ctx_sync": (C.c_int, [C.c_void_p]),
     "sm_ctx_mem_info": (C.c_int, [C.c_void_p, _size_p, _size_p]),
-    "sm_probe_stream": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, _dbl_p]),
+    "sm_probe_stream": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  _dbl_p]),
     "sm_ricker_length": (C.c_size_t, [C.c_double, C.c_double]),
     "sm_compute_src_fct": (C.c_int, [C.c_size_t, _dbl_p, _dbl_p, C.c_double, _size_p, _dbl_p]),
     "sm_nsteps": (C.c_size_t, [C.c_double, C.c_double]),
@@ -83,6 +84,16 @@ _SIGS = {
     "sm_e2d_reset": (C.c_int, [C.c_void_p]),
     "sm_e2d_device_bytes": (C.c_int, [C.c_void_p, _size_p]),
     "sm_e2d_bytes_needed": (C.c_size_t, [C.POINTER(E2dParams), C.c_size_t, C.c_size_t]),
+    "sm_e2d_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "sm_e2d_get_launch": (C.c_int, [C.c_void_p, _int_p, _int_p]),
+    "sm_e2d_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "sm_e2d_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),
+    "sm_e2d_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "sm_e2d_batch_destroy": (C.c_int, [C.c_void_p]),
+    "sm_e2d_batch_set_graph_steps": (C.c_int, [C.c_void_p, C.c_int]),
+    "sm_e2d_batch_run": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p), _size_p]),
+    "sm_e2d_batch_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "sm_e2d_batch_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),
 }
 
 

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <vector>
 
+#include "e2d_fused.hpp"
 #include "sm_internal.hpp"
 
 namespace sm {
@@ -158,16 +159,27 @@ e2d_src_rcv(E2dDev<T> d, int nsrc, const int64_t *sidx, const int64_t *soff, con
 
 using namespace sm;
 
+// Two device layouts:
+//  - reference-order kernels (FP64, or FP32 with SM_E2D_NAIVE): fields = 5
+//    totals + 10 split arrays, mat = b, bm, la, l2m, mu, damp = (c1, c2)
+//    pairs, src/rcv as int64 records;
+//  - fused FP32 kernel (default for FP32): fields = two parity buffers of
+//    E2_NSTATE arrays + E2_NSPLIT stress split arrays, mat = the five
+//    coefficients times dt/h, damp = (A, B) pairs, src/rcv as int4 records
+//    sorted by padded cell.
 struct sm_e2d {
     sm_ctx *ctx = nullptr;
     sm_e2d_params p{};
     int NX = 0, NZ = 0;
     size_t N = 0, elt = 0;
-    sm_dbuf fields;   // 5 totals + 10 split
-    sm_dbuf mat;      // b, bm, la, l2m, mu
-    sm_dbuf damp;     // 4 tables of [2] pairs
+    bool fused = false;
+    sm_dbuf fields;
+    sm_dbuf mat;
+    sm_dbuf damp;
     sm_dbuf src, rcv, trace;
     size_t step = 0, nsrc = 0, nrec = 0, max_steps = 0, rcv_step0 = 0;
+    int chunk = 8, graph_steps = 32;
+    sm::E2dRunner *runner = nullptr;
 };
 
 namespace {
@@ -175,6 +187,17 @@ namespace {
 void release(sm_e2d *s)
 {
     dfree(s->fields); dfree(s->mat); dfree(s->damp); dfree(s->src); dfree(s->rcv); dfree(s->trace);
+    delete s->runner;
+    s->runner = nullptr;
+}
+
+// rows per chunk for a grid: enough waves to cover the chip about twice
+int default_chunk(int NX, int NZ)
+{
+    const int ntz = (NZ + E2D_OWN - 1) / E2D_OWN;
+    const int target = 2 * 256 * 4;
+    int c = (int)((long long)NX * ntz / target);
+    return std::max(2, std::min(16, c));
 }
 
 template <typename T>
@@ -235,6 +258,62 @@ int run(sm_e2d *s, size_t nsteps)
     return SM_OK;
 }
 
+}  // namespace
+
+namespace sm {
+// Descriptor of a fused-layout solver for the next nsteps steps.
+E2dShot e2d_shot(sm_e2d *s, size_t nsteps)
+{
+    E2dShot d;
+    std::memset(&d, 0, sizeof(d));
+    d.NX = s->NX; d.NZ = s->NZ;
+    d.np = (int)s->p.npml; d.nx = (int)s->p.nx; d.nz = (int)s->p.nz;
+    d.chunk = std::max(1, std::min(s->chunk, s->NX));
+    e2d_items(d.NX, d.NZ, d.chunk, &d.ntz, &d.nch);
+    d.nsrc = (int)s->nsrc;
+    d.nrec = (int)s->nrec;
+    d.it0 = (int)s->step;
+    d.nsteps = (int)nsteps;
+    d.rcv0 = (int)s->rcv_step0;
+    d.rmax = (int)s->max_steps;
+    d.N = (unsigned)s->N;
+    float *f = (float *)s->fields.ptr;
+    d.st[0] = f;
+    d.st[1] = f + E2_NSTATE * s->N;
+    d.ss = f + 2 * E2_NSTATE * s->N;
+    d.mat = (const float *)s->mat.ptr;
+    d.dmp = (const float *)s->damp.ptr;
+    d.src = (const int4 *)s->src.ptr;
+    d.srow = s->src.ptr ? (const int *)((const int4 *)s->src.ptr + s->nsrc) : nullptr;
+    d.sfct = s->src.ptr ? (const float *)(d.srow + s->NX + 1) : nullptr;
+    d.rcv = (const int4 *)s->rcv.ptr;
+    d.rrow = s->rcv.ptr ? (const int *)((const int4 *)s->rcv.ptr + s->nrec) : nullptr;
+    d.trace = (float *)s->trace.ptr;
+    return d;
+}
+
+int e2d_check_steps(const sm_e2d *s, size_t nsteps)
+{
+    SM_CHECK(s->step + nsteps < (1ull << 31), SM_ERR_ARG,
+             "step count %zu beyond the fused kernel's 2^31 limit", s->step + nsteps);
+    return SM_OK;
+}
+}  // namespace sm
+
+namespace {
+
+int run_fused(sm_e2d *s, size_t nsteps)
+{
+    int rc = e2d_check_steps(s, nsteps);
+    if (rc != SM_OK) return rc;
+    if (!s->runner) s->runner = new E2dRunner;
+    std::vector<E2dShot> one{e2d_shot(s, nsteps)};
+    rc = s->runner->run(one, s->ctx->stream, s->graph_steps);
+    if (rc != SM_OK) return rc;
+    s->step += nsteps;
+    return SM_OK;
+}
+
 template <typename T>
 int read_back(sm_e2d *s, const void *dev, size_t n, double *out)
 {
@@ -277,6 +356,9 @@ size_t sm_e2d_bytes_needed(const sm_e2d_params *p, size_t nrec, size_t max_steps
     if (!p) return 0;
     const size_t e = p->storage == SM_STORE_F64 ? 8 : 4;
     const size_t NX = p->nx + 2 * p->npml, NZ = p->nz + 2 * p->npml, N = NX * NZ;
+    if (p->storage == SM_STORE_F32 && !(p->flags & SM_E2D_NAIVE))
+        return 4 * ((2 * E2_NSTATE + E2_NSPLIT + E2_NMAT) * N + 4 * (NX + NZ) + nrec * max_steps) +
+               16 * nrec + 4096;
     return e * (20 * N + 4 * (NX + NZ) + nrec * max_steps) + 16 * nrec;
 }
 
@@ -291,6 +373,7 @@ int sm_e2d_create(sm_ctx *ctx, const sm_e2d_params *pp, const double *vp, const 
     SM_CHECK(p.h > 0 && p.dt > 0, SM_ERR_ARG, "sm_e2d_create: h and dt must be > 0");
     SM_CHECK(p.storage == SM_STORE_F32 || p.storage == SM_STORE_F64, SM_ERR_ARG,
              "sm_e2d_create: storage %d not supported", p.storage);
+    SM_CHECK((p.flags & ~SM_E2D_NAIVE) == 0, SM_ERR_ARG, "sm_e2d_create: unknown flags %#x", p.flags);
     const size_t nx = p.nx, nz = p.nz, np = p.npml, n = nx * nz;
     const size_t NX = nx + 2 * np, NZ = nz + 2 * np, N = NX * NZ;
     SM_CHECK(NX < (1u << 20) && NZ < (1u << 20), SM_ERR_ARG, "sm_e2d_create: grid too large");
@@ -403,16 +486,54 @@ int sm_e2d_create(sm_ctx *ctx, const sm_e2d_params *pp, const double *vp, const 
     s->p = p;
     s->NX = (int)NX; s->NZ = (int)NZ; s->N = N;
     s->elt = p.storage == SM_STORE_F64 ? 8 : 4;
-    int rc = dalloc(s->fields, 15 * N * s->elt);
-    if (rc == SM_OK) rc = dalloc(s->mat, 5 * N * s->elt);
-    if (rc == SM_OK) rc = dalloc(s->damp, dt_tab.size() * s->elt);
-    if (rc == SM_OK) {
-        if (s->elt == 8) {
-            rc = upload<double>(s->mat.ptr, mat, ctx->stream);
-            if (rc == SM_OK) rc = upload<double>(s->damp.ptr, dt_tab, ctx->stream);
-        } else {
-            rc = upload<float>(s->mat.ptr, mat, ctx->stream);
-            if (rc == SM_OK) rc = upload<float>(s->damp.ptr, dt_tab, ctx->stream);
+    s->fused = p.storage == SM_STORE_F32 && !(p.flags & SM_E2D_NAIVE);
+    s->chunk = default_chunk((int)NX, (int)NZ);
+    int rc = SM_OK;
+    if (s->fused) {
+        // coefficients times dt/h (the interior forms multiply by dt/h,
+        // the damped split forms by c1 = 2dt/(2+dt d) and 1/h)
+        const double f = p.dt / p.h;
+        for (auto &v : mat) v *= f;
+        // (A, B) = (c1*c2, c1/dt) where damped, (1, 1) where not
+        std::vector<double> ab(4 * NX + 4 * NZ, 1.0);
+        auto put = [&](double *dst, double dd) {
+            dst[0] = (2. - p.dt * dd) / (2. + p.dt * dd);
+            dst[1] = 2. / (2. + p.dt * dd);
+        };
+        double *xi = ab.data(), *xh = xi + 2 * NX, *zi = xi + 4 * NX, *zh = zi + 2 * NZ;
+        for (size_t I = 0; I < NX; ++I) {
+            if (I < np) { put(xi + 2 * I, d[I]); put(xh + 2 * I, dh[I]); }
+            else if (I >= np + nx) {
+                const size_t r = np - 1 - (I - np - nx);
+                put(xi + 2 * I, d[r]); put(xh + 2 * I, dH[r]);
+            }
+        }
+        for (size_t J = 0; J < NZ; ++J) {
+            if (J < np) { put(zi + 2 * J, d[J]); put(zh + 2 * J, dh[J]); }
+            else if (J >= np + nz) {
+                const size_t r = np - 1 - (J - np - nz);
+                put(zi + 2 * J, d[r]); put(zh + 2 * J, dH[r]);
+            }
+        }
+        SM_CHECK((2 * E2_NSTATE + E2_NSPLIT) * N * 4 < (1ull << 32), SM_ERR_ARG,
+                 "sm_e2d_create: padded grid %zux%zu too large for 32-bit offsets", NX, NZ);
+        rc = dalloc(s->fields, (2 * E2_NSTATE + E2_NSPLIT) * N * 4);
+        if (rc == SM_OK) rc = dalloc(s->mat, E2_NMAT * N * 4);
+        if (rc == SM_OK) rc = dalloc(s->damp, ab.size() * 4);
+        if (rc == SM_OK) rc = upload<float>(s->mat.ptr, mat, ctx->stream);
+        if (rc == SM_OK) rc = upload<float>(s->damp.ptr, ab, ctx->stream);
+    } else {
+        rc = dalloc(s->fields, 15 * N * s->elt);
+        if (rc == SM_OK) rc = dalloc(s->mat, 5 * N * s->elt);
+        if (rc == SM_OK) rc = dalloc(s->damp, dt_tab.size() * s->elt);
+        if (rc == SM_OK) {
+            if (s->elt == 8) {
+                rc = upload<double>(s->mat.ptr, mat, ctx->stream);
+                if (rc == SM_OK) rc = upload<double>(s->damp.ptr, dt_tab, ctx->stream);
+            } else {
+                rc = upload<float>(s->mat.ptr, mat, ctx->stream);
+                if (rc == SM_OK) rc = upload<float>(s->damp.ptr, dt_tab, ctx->stream);
+            }
         }
     }
     if (rc == SM_OK) rc = zero_state(s);
@@ -450,6 +571,43 @@ int sm_e2d_set_sources(sm_e2d *s, size_t nsrc, const int *type, const size_t *id
     dfree(s->src);
     s->nsrc = 0;
     if (nsrc == 0) return SM_OK;
+    if (s->fused) {
+        // int4 (padded cell, type, length, offset) sorted by cell, then dt*fct
+        SM_CHECK(total < (1ull << 31), SM_ERR_ARG, "sm_e2d_set_sources: source samples too many");
+        std::vector<int> order(nsrc);
+        for (size_t k = 0; k < nsrc; ++k) order[k] = (int)k;
+        auto cell = [&](size_t k) {
+            const size_t i = idx[k] / s->p.nz, j = idx[k] % s->p.nz;
+            return (int)((i + s->p.npml) * s->NZ + j + s->p.npml);
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cell(a) < cell(b); });
+        std::vector<size_t> offs(nsrc);
+        for (size_t k = 0, o = 0; k < nsrc; ++k) { offs[k] = o; o += length[k]; }
+        const size_t NXp = (size_t)s->NX + 1;
+        const size_t bytes = nsrc * 16 + NXp * 4 + total * 4;
+        int rc = dalloc(s->src, bytes);
+        if (rc != SM_OK) return rc;
+        std::vector<unsigned char> hb(bytes);
+        int *rec = (int *)hb.data();
+        int *srow = (int *)(hb.data() + nsrc * 16);
+        float *hf = (float *)(hb.data() + nsrc * 16 + NXp * 4);
+        for (size_t q = 0; q < nsrc; ++q) {
+            const int k = order[q];
+            rec[4 * q] = cell(k);
+            rec[4 * q + 1] = type[k];
+            rec[4 * q + 2] = (int)std::min<size_t>(length[k], 1u << 30);
+            rec[4 * q + 3] = (int)offs[k];
+        }
+        for (size_t I = 0, q = 0; I < NXp; ++I) {
+            while (q < nsrc && rec[4 * q] < (int)(I * s->NZ)) ++q;
+            srow[I] = (int)q;
+        }
+        for (size_t q = 0; q < total; ++q) hf[q] = (float)(s->p.dt * fct[q]);
+        SM_HIP(hipMemcpyAsync(s->src.ptr, hb.data(), bytes, hipMemcpyHostToDevice, s->ctx->stream));
+        SM_HIP(hipStreamSynchronize(s->ctx->stream));
+        s->nsrc = nsrc;
+        return SM_OK;
+    }
     const size_t bytes = 4 * nsrc * sizeof(int64_t) + total * s->elt;
     int rc = dalloc(s->src, bytes);
     if (rc != SM_OK) return rc;
@@ -494,6 +652,39 @@ int sm_e2d_set_receivers(sm_e2d *s, size_t nrec, const size_t *idx, const int *c
     s->nrec = 0;
     s->max_steps = 0;
     if (nrec == 0 || max_steps == 0) return SM_OK;
+    if (s->fused) {
+        // int4 (padded cell, slot, trace column, 0) sorted by cell
+        SM_CHECK(max_steps < (1ull << 31), SM_ERR_ARG, "sm_e2d_set_receivers: max_steps too large");
+        std::vector<int> hr(4 * nrec + s->NX + 1);
+        std::vector<int> order(nrec);
+        for (size_t r = 0; r < nrec; ++r) order[r] = (int)r;
+        auto cell = [&](size_t r) {
+            const size_t i = idx[r] / s->p.nz, j = idx[r] % s->p.nz;
+            return (int)((i + s->p.npml) * s->NZ + j + s->p.npml);
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cell(a) < cell(b); });
+        for (size_t q = 0; q < nrec; ++q) {
+            const int r = order[q];
+            hr[4 * q] = cell(r);
+            hr[4 * q + 1] = comp_slot(comp[r]);
+            hr[4 * q + 2] = r;
+            hr[4 * q + 3] = 0;
+        }
+        for (size_t I = 0, q = 0; I <= (size_t)s->NX; ++I) {
+            while (q < nrec && hr[4 * q] < (int)(I * s->NZ)) ++q;
+            hr[4 * nrec + I] = (int)q;
+        }
+        int rc = dalloc(s->rcv, hr.size() * 4);
+        if (rc == SM_OK) rc = dalloc(s->trace, nrec * max_steps * 4);
+        if (rc != SM_OK) { dfree(s->rcv); dfree(s->trace); return rc; }
+        SM_HIP(hipMemcpyAsync(s->rcv.ptr, hr.data(), hr.size() * 4, hipMemcpyHostToDevice,
+                              s->ctx->stream));
+        SM_HIP(hipStreamSynchronize(s->ctx->stream));
+        s->nrec = nrec;
+        s->max_steps = max_steps;
+        s->rcv_step0 = s->step;
+        return SM_OK;
+    }
     int rc = dalloc(s->rcv, nrec * (sizeof(int64_t) + sizeof(int)));
     if (rc == SM_OK) rc = dalloc(s->trace, nrec * max_steps * s->elt);
     if (rc != SM_OK) { dfree(s->rcv); dfree(s->trace); return rc; }
@@ -519,6 +710,7 @@ int sm_e2d_run(sm_e2d *s, size_t nsteps)
     SM_CHECK(s, SM_ERR_ARG, "sm_e2d_run: NULL solver");
     if (nsteps == 0) return SM_OK;
     SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    if (s->fused) return run_fused(s, nsteps);
     if (s->elt == 8) return run<double>(s, nsteps);
     return run<float>(s, nsteps);
 }
@@ -549,7 +741,9 @@ int sm_e2d_get_field_padded(sm_e2d *s, int comp, double *out)
     const int slot = comp_slot(comp);
     SM_CHECK(slot >= 0, SM_ERR_ARG, "sm_e2d_get_field_padded: component %d", comp);
     SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
-    const char *src = (const char *)s->fields.ptr + slot * s->N * s->elt;
+    size_t arr = (size_t)slot;
+    if (s->fused) arr += (s->step & 1) * E2_NSTATE;   // parity buffer holding step s->step
+    const char *src = (const char *)s->fields.ptr + arr * s->N * s->elt;
     if (s->elt == 8) return read_back<double>(s, src, s->N, out);
     return read_back<float>(s, src, s->N, out);
 }
@@ -577,7 +771,130 @@ int sm_e2d_device_bytes(const sm_e2d *s, size_t *bytes)
 {
     SM_CHECK(s && bytes, SM_ERR_ARG, "sm_e2d_device_bytes: NULL argument");
     *bytes = s->fields.bytes + s->mat.bytes + s->damp.bytes + s->src.bytes + s->rcv.bytes +
-             s->trace.bytes;
+             s->trace.bytes + (s->runner ? s->runner->device_bytes() : 0);
+    return SM_OK;
+}
+
+int sm_e2d_set_launch(sm_e2d *s, int rows_per_chunk, int graph_steps)
+{
+    SM_CHECK(s, SM_ERR_ARG, "sm_e2d_set_launch: NULL solver");
+    SM_CHECK(s->fused, SM_ERR_ARG, "sm_e2d_set_launch: only the fused FP32 kernel has launch parameters");
+    SM_CHECK(rows_per_chunk >= 1 && rows_per_chunk <= 4096, SM_ERR_ARG,
+             "sm_e2d_set_launch: rows_per_chunk %d outside [1, 4096]", rows_per_chunk);
+    SM_CHECK(graph_steps >= 0 && graph_steps <= 4096, SM_ERR_ARG,
+             "sm_e2d_set_launch: graph_steps %d outside [0, 4096]", graph_steps);
+    s->chunk = rows_per_chunk;
+    s->graph_steps = graph_steps;
+    return SM_OK;
+}
+
+int sm_e2d_get_launch(const sm_e2d *s, int *rows_per_chunk, int *graph_steps)
+{
+    SM_CHECK(s && rows_per_chunk && graph_steps, SM_ERR_ARG, "sm_e2d_get_launch: NULL argument");
+    *rows_per_chunk = s->chunk;
+    *graph_steps = s->graph_steps;
+    return SM_OK;
+}
+
+int sm_e2d_set_timing(sm_e2d *s, int on)
+{
+    SM_CHECK(s, SM_ERR_ARG, "sm_e2d_set_timing: NULL solver");
+    SM_CHECK(s->fused, SM_ERR_ARG, "sm_e2d_set_timing: only the fused FP32 kernel is timed");
+    if (!s->runner) s->runner = new E2dRunner;
+    s->runner->timing = on != 0;
+    s->runner->ms_total = 0.0;
+    s->runner->launches = 0;
+    return SM_OK;
+}
+
+int sm_e2d_get_timing(const sm_e2d *s, double *ms_total, size_t *launches)
+{
+    SM_CHECK(s && ms_total && launches, SM_ERR_ARG, "sm_e2d_get_timing: NULL argument");
+    *ms_total = s->runner ? s->runner->ms_total : 0.0;
+    *launches = s->runner ? s->runner->launches : 0;
+    return SM_OK;
+}
+
+}  // extern "C"
+
+struct sm_e2d_batch {
+    sm_ctx *ctx = nullptr;
+    sm::E2dRunner runner;
+    int graph_steps = 32;
+};
+
+extern "C" {
+
+int sm_e2d_batch_create(sm_ctx *ctx, sm_e2d_batch **out)
+{
+    SM_CHECK(ctx && out, SM_ERR_ARG, "sm_e2d_batch_create: NULL argument");
+    sm_e2d_batch *b = new sm_e2d_batch;
+    b->ctx = ctx;
+    *out = b;
+    return SM_OK;
+}
+
+int sm_e2d_batch_destroy(sm_e2d_batch *b)
+{
+    if (!b) return SM_OK;
+    (void)set_device(b->ctx);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+    return SM_OK;
+}
+
+int sm_e2d_batch_set_graph_steps(sm_e2d_batch *b, int graph_steps)
+{
+    SM_CHECK(b, SM_ERR_ARG, "sm_e2d_batch_set_graph_steps: NULL batch");
+    SM_CHECK(graph_steps >= 0 && graph_steps <= 4096, SM_ERR_ARG,
+             "sm_e2d_batch_set_graph_steps: %d outside [0, 4096]", graph_steps);
+    b->graph_steps = graph_steps;
+    return SM_OK;
+}
+
+int sm_e2d_batch_run(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots, const size_t *nsteps)
+{
+    SM_CHECK(b && (nshots == 0 || (shots && nsteps)), SM_ERR_ARG, "sm_e2d_batch_run: NULL argument");
+    std::vector<E2dShot> d;
+    d.reserve(nshots);
+    for (size_t k = 0; k < nshots; ++k) {
+        sm_e2d *s = shots[k];
+        SM_CHECK(s, SM_ERR_ARG, "sm_e2d_batch_run: shot %zu is NULL", k);
+        SM_CHECK(s->fused, SM_ERR_ARG, "sm_e2d_batch_run: shot %zu is not an FP32 fused solver", k);
+        SM_CHECK(s->ctx->device == b->ctx->device, SM_ERR_ARG,
+                 "sm_e2d_batch_run: shot %zu is on device %d, the batch on %d", k, s->ctx->device,
+                 b->ctx->device);
+        for (size_t q = 0; q < k; ++q)
+            SM_CHECK(shots[q] != s, SM_ERR_ARG, "sm_e2d_batch_run: shot %zu listed twice", k);
+        int rc = e2d_check_steps(s, nsteps[k]);
+        if (rc != SM_OK) return rc;
+        d.push_back(e2d_shot(s, nsteps[k]));
+    }
+    SM_CHECK(set_device(b->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    // work queued on the shots' own streams completes before the batch reads them
+    for (size_t k = 0; k < nshots; ++k)
+        if (shots[k]->ctx->stream != b->ctx->stream) SM_HIP(hipStreamSynchronize(shots[k]->ctx->stream));
+    int rc = b->runner.run(d, b->ctx->stream, b->graph_steps);
+    if (rc != SM_OK) return rc;
+    SM_HIP(hipStreamSynchronize(b->ctx->stream));
+    for (size_t k = 0; k < nshots; ++k) shots[k]->step += nsteps[k];
+    return SM_OK;
+}
+
+int sm_e2d_batch_set_timing(sm_e2d_batch *b, int on)
+{
+    SM_CHECK(b, SM_ERR_ARG, "sm_e2d_batch_set_timing: NULL batch");
+    b->runner.timing = on != 0;
+    b->runner.ms_total = 0.0;
+    b->runner.launches = 0;
+    return SM_OK;
+}
+
+int sm_e2d_batch_get_timing(const sm_e2d_batch *b, double *ms_total, size_t *launches)
+{
+    SM_CHECK(b && ms_total && launches, SM_ERR_ARG, "sm_e2d_batch_get_timing: NULL argument");
+    *ms_total = b->runner.ms_total;
+    *launches = b->runner.launches;
     return SM_OK;
 }
 

@@ -1,0 +1,446 @@
+// e2d_fused.hip -- one-pass FP32 time step of the 2D elastic split-PML
+// solver over one shot or a batch of shots (replaces, per step,
+// e_iso.c:589-669: update_vx/update_vz + pml_v, update_txxzz/update_txz +
+// pml_t, add_src, receiver sampling).
+//
+// Arithmetic (all FP32, dt/h folded into the material arrays):
+//   interior   vx += cb*((txx - txx[i-1]) + (txz - txz[j-1]))
+//              vz += cbm*((txz[i+1] - txz) + (tzz[j+1] - tzz))
+//              txx += cl2m*dvx_x + cla*dvz_z ; tzz += cl2m*dvz_z + cla*dvx_x
+//              txz += cmu*(dvx_z + dvz_x)
+//   PML split  u = A*u + B*(c*delta) per component, with
+//              A = (2 - dt d)/(2 + dt d), B = 2/(2 + dt d) where the axis is
+//              damped (pml.c:59-63 folded: c1*c2 and c1*h/dt) and A = B = 1
+//              where it is not (pml.c:123); the total is x + z.
+// Neighbours off the padded grid read 0 (pml.c:58, 79).
+#include <algorithm>
+#include <cstring>
+
+#include "e2d_fused.hpp"
+#include "sm_internal.hpp"
+
+namespace sm {
+namespace {
+
+// Buffer-resource access: one 128-bit resource per array group (uniform,
+// in SGPRs), the array's start as the scalar offset and the cell's byte
+// offset as the per-lane offset.  A lane that must not touch memory passes
+// OFF_NONE: the hardware returns 0 for such loads and drops such stores, so
+// masking costs no branch.  Every resource covers less than 2^31 bytes, so
+// OFF_NONE plus any scalar offset stays out of range.
+constexpr unsigned OFF_NONE = 0x80000000u;
+constexpr int RSRC_WORD3 = 0x00020000;   // raw 32-bit buffer on gfx9
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, RSRC_WORD3);
+}
+// (the b32 builtins move raw 32-bit words: bit casts, not conversions)
+__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+// uniform read-only data through the scalar cache
+__device__ __forceinline__ int cld(const int *p)
+{
+    return *(const __attribute__((address_space(4))) int *)p;
+}
+__device__ __forceinline__ float cld(const float *p)
+{
+    return *(const __attribute__((address_space(4))) float *)p;
+}
+__device__ __forceinline__ float2 cld(const float2 *p)
+{
+    const float *f = (const float *)p;
+    return make_float2(cld(f), cld(f + 1));
+}
+__device__ __forceinline__ int4 cld(const int4 *p)
+{
+    const int *i = (const int *)p;
+    return make_int4(cld(i), cld(i + 1), cld(i + 2), cld(i + 3));
+}
+
+// value held by lane-1 / lane+1 (lanes 0 and 63 receive a value they do not use)
+__device__ __forceinline__ float from_left(float x) { return __shfl_up(x, 1, 64); }
+__device__ __forceinline__ float from_right(float x) { return __shfl_down(x, 1, 64); }
+
+struct VRow { float t, a, c, coef; };   // total or split pair, and its coefficient
+struct TRow { float la, l2m, mu, x[6]; }; // stress coefficients and split pairs
+
+__global__ void __launch_bounds__(256)
+e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot, int nitems,
+               const int *__restrict__ base, int k)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // consecutive blocks go to different XCDs: give each XCD a contiguous
+    // range of work items (the grid is a multiple of 8 blocks)
+    const int nb = gridDim.x;
+    const int b = blockIdx.x;
+    const int logical = (b & 7) * (nb >> 3) + (b >> 3);
+    const int g = logical * E2D_WAVES + w;
+    if (g >= nitems) return;
+    const E2dShot *S = shots + bshot[logical];
+    const int s = (base ? *base : 0) + k;
+    if (s >= S->nsteps) return;
+    const int q = g - S->item0;
+    const int ntz = S->ntz;
+    if (q >= ntz * S->nch) return;   // padding wave of the shot's last block
+    const int it = S->it0 + s;
+
+    const int NX = S->NX, NZ = S->NZ, np = S->np, nx = S->nx, nz = S->nz;
+    const int zt = q % ntz, ch = q / ntz;
+    const int I0 = ch * S->chunk;
+    const int I1 = min(I0 + S->chunk, NX);
+    const int J = zt * E2D_OWN - 1 + lane;
+    const bool jv = J >= 0 && J < NZ;
+    const bool own = jv && lane >= 1 && lane <= E2D_OWN;
+    const bool pzl = J < np || J >= np + nz;
+    // does any lane of this wave lie in the z PML (uniform)
+    const bool zpml = __any(jv && pzl);
+    const unsigned N = S->N, N4 = N * 4u;
+    const auto Rin = rsrc(S->st[it & 1], E2_NSTATE * N4);
+    const auto Rout = rsrc(S->st[(it + 1) & 1], E2_NSTATE * N4);
+    const auto Rss = rsrc(S->ss, E2_NSPLIT * N4);
+    const auto Rmat = rsrc(S->mat, E2_NMAT * N4);
+    const auto Rdmp = rsrc(S->dmp, (unsigned)(2 * NX + 2 * NZ) * 8u);
+    const float2 *xi = (const float2 *)S->dmp;
+    const float2 *xh = xi + NX;
+    // z damping of this lane: (A, B) at integer and half stagger
+    const unsigned zo = jv ? (unsigned)(2 * NX + J) * 8u : OFF_NONE;
+    const unsigned zho = jv ? (unsigned)(2 * NX + NZ + J) * 8u : OFF_NONE;
+    const float2 zi = make_float2(bld(Rdmp, zo, 0), bld(Rdmp, zo, 4));
+    const float2 zh = make_float2(bld(Rdmp, zho, 0), bld(Rdmp, zho, 4));
+
+    auto rv = [&](int R) { return R >= 0 && R < NX; };
+    auto px = [&](int R) { return R < np || R >= np + nx; };
+    // byte offset of row R in this lane, or OFF_NONE
+    auto voff = [&](int R, bool ok) { return ok ? ((unsigned)R * NZ + J) * 4u : OFF_NONE; };
+
+    // ---- loads, issued a row ahead of the arithmetic that uses them ----
+    auto fetch_s = [&](int R, float &sxx, float &szz, float &sxz) {
+        const unsigned o = voff(R, rv(R) && jv);
+        sxx = bld(Rin, o, E2_TXX * N4);
+        szz = bld(Rin, o, E2_TZZ * N4);
+        sxz = bld(Rin, o, E2_TXZ * N4);
+    };
+    // velocity component at row R: total, or split pair in the PML
+    auto fetch_v = [&](int R, int tot, int sx, int sz, int cf) {
+        VRow r{0.f, 0.f, 0.f, 0.f};
+        const unsigned o = voff(R, rv(R) && jv);
+        r.coef = bld(Rmat, o, cf * N4);
+        if (rv(R) && (px(R) || zpml)) {
+            const bool pm = px(R) || pzl;
+            r.a = bld(Rin, pm ? o : OFF_NONE, sx * N4);
+            r.c = bld(Rin, pm ? o : OFF_NONE, sz * N4);
+            r.t = bld(Rin, pm ? OFF_NONE : o, tot * N4);
+        } else {
+            r.t = bld(Rin, o, tot * N4);
+        }
+        return r;
+    };
+    auto fetch_t = [&](int R) {
+        TRow r{};
+        const unsigned o = voff(R, jv);
+        r.la = bld(Rmat, o, E2_CLA * N4);
+        r.l2m = bld(Rmat, o, E2_CL2M * N4);
+        r.mu = bld(Rmat, o, E2_CMU * N4);
+        if (px(R) || zpml) {
+            const unsigned op = (px(R) || pzl) && own ? o : OFF_NONE;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) r.x[c] = bld(Rss, op, c * N4);
+        }
+        return r;
+    };
+
+    // new vx of row R from old txx of rows R-1, R and old txz of row R
+    auto vx_new = [&](int R, const VRow &r, float sxx_m, float sxx_r, float sxz_r, bool store) {
+        const float dxs = sxx_r - sxx_m;
+        const float dzs = sxz_r - from_left(sxz_r);
+        const unsigned o = voff(R, store && own);
+        float vx = r.t + r.coef * (dxs + dzs);
+        if (rv(R) && (px(R) || zpml)) {
+            const bool pm = px(R) || pzl;
+            const float2 ax = cld(xi + R);
+            const float a = ax.x * r.a + ax.y * (r.coef * dxs);
+            const float c = zi.x * r.c + zi.y * (r.coef * dzs);
+            if (pm) vx = a + c;
+            bst(Rout, pm ? o : OFF_NONE, E2_VXX * N4, a);
+            bst(Rout, pm ? o : OFF_NONE, E2_VXZ * N4, c);
+        }
+        bst(Rout, o, E2_VX * N4, vx);
+        return vx;
+    };
+    // new vz of row R from old txz of rows R, R+1 and old tzz of row R
+    auto vz_new = [&](int R, const VRow &r, float sxz_r, float szz_r, float sxz_p, bool store) {
+        const float dxs = sxz_p - sxz_r;
+        const float dzs = from_right(szz_r) - szz_r;
+        const unsigned o = voff(R, store && own);
+        float vz = r.t + r.coef * (dxs + dzs);
+        if (rv(R) && (px(R) || zpml)) {
+            const bool pm = px(R) || pzl;
+            const float2 ax = cld(xh + R);
+            const float a = ax.x * r.a + ax.y * (r.coef * dxs);
+            const float c = zh.x * r.c + zh.y * (r.coef * dzs);
+            if (pm) vz = a + c;
+            bst(Rout, pm ? o : OFF_NONE, E2_VZX * N4, a);
+            bst(Rout, pm ? o : OFF_NONE, E2_VZZ * N4, c);
+        }
+        bst(Rout, o, E2_VZ * N4, vz);
+        return vz;
+    };
+
+    const int nsrc = S->nsrc, nrec = S->nrec;
+    const int4 *src = S->src;
+    const int4 *rcv = S->rcv;
+    const int row = it - S->rcv0;
+    const bool record = nrec > 0 && row >= 0 && row < S->rmax;
+    int sp = nsrc ? cld(S->srow + I0) : 0;
+    int rp = record ? cld(S->rrow + I0) : 0;
+
+    // preamble: vx of row I0 and vz of row I0-1, from rows I0-1 and I0;
+    // the loads of the first iteration go out with them
+    float sxx_m, szz_m, sxz_m, sxx0, szz0, sxz0, sxx1, szz1, sxz1;
+    fetch_s(I0 - 1, sxx_m, szz_m, sxz_m);
+    fetch_s(I0, sxx0, szz0, sxz0);
+    const VRow v_pre = fetch_v(I0, E2_VX, E2_VXX, E2_VXZ, E2_CB);
+    const VRow z_pre = fetch_v(I0 - 1, E2_VZ, E2_VZX, E2_VZZ, E2_CBM);
+    fetch_s(I0 + 1, sxx1, szz1, sxz1);
+    VRow vr = fetch_v(I0 + 1, E2_VX, E2_VXX, E2_VXZ, E2_CB);
+    VRow zr = fetch_v(I0, E2_VZ, E2_VZX, E2_VZZ, E2_CBM);
+    TRow tr = fetch_t(I0);
+    float vx0 = vx_new(I0, v_pre, sxx_m, sxx0, sxz0, true);
+    float vz_m = vz_new(I0 - 1, z_pre, sxz_m, szz_m, sxz0, false);
+
+    for (int I = I0; I < I1; ++I) {
+        // loads of the next iteration
+        float nsxx = 0.f, nszz = 0.f, nsxz = 0.f;
+        VRow nvr{0.f, 0.f, 0.f, 0.f}, nzr{0.f, 0.f, 0.f, 0.f};
+        TRow ntr{};
+        if (I + 1 < I1) {
+            fetch_s(I + 2, nsxx, nszz, nsxz);
+            nvr = fetch_v(I + 2, E2_VX, E2_VXX, E2_VXZ, E2_CB);
+            nzr = fetch_v(I + 1, E2_VZ, E2_VZX, E2_VZZ, E2_CBM);
+            ntr = fetch_t(I + 1);
+        }
+
+        const float vx1 = vx_new(I + 1, vr, sxx0, sxx1, sxz1, I + 1 < I1);
+        const float vz0 = vz_new(I, zr, sxz0, szz0, sxz1, true);
+
+        // stress of row I
+        const float dvxx = vx1 - vx0;
+        const float dvzz = vz0 - from_left(vz0);
+        const float dvxz = from_right(vx0) - vx0;
+        const float dvzx = vz0 - vz_m;
+        const unsigned o = voff(I, own);
+        float nxx = sxx0 + (tr.l2m * dvxx + tr.la * dvzz);
+        float nzz = szz0 + (tr.l2m * dvzz + tr.la * dvxx);
+        float nxz = sxz0 + tr.mu * (dvxz + dvzx);
+        if (px(I) || zpml) {
+            const bool pm = px(I) || pzl;
+            const float2 ah = cld(xh + I), ai = cld(xi + I);
+            const float txxx = ah.x * tr.x[E2_TXXX] + ah.y * (tr.l2m * dvxx);
+            const float txxz = zi.x * tr.x[E2_TXXZ] + zi.y * (tr.la * dvzz);
+            const float tzzx = ah.x * tr.x[E2_TZZX] + ah.y * (tr.la * dvxx);
+            const float tzzz = zi.x * tr.x[E2_TZZZ] + zi.y * (tr.l2m * dvzz);
+            const float txzx = ai.x * tr.x[E2_TXZX] + ai.y * (tr.mu * dvzx);
+            const float txzz = zh.x * tr.x[E2_TXZZ] + zh.y * (tr.mu * dvxz);
+            if (pm) {
+                nxx = txxx + txxz;
+                nzz = tzzx + tzzz;
+                nxz = txzx + txzz;
+            }
+            const unsigned op = pm ? o : OFF_NONE;
+            bst(Rss, op, E2_TXXX * N4, txxx); bst(Rss, op, E2_TXXZ * N4, txxz);
+            bst(Rss, op, E2_TZZX * N4, tzzx); bst(Rss, op, E2_TZZZ * N4, tzzz);
+            bst(Rss, op, E2_TXZX * N4, txzx); bst(Rss, op, E2_TXZZ * N4, txzz);
+        }
+
+        // add_src (src.c:170-184) on this row, after both phases
+        const int rowend = (I + 1) * NZ;
+        for (; sp < nsrc; ++sp) {
+            const int4 e = cld(src + sp);
+            if (e.x >= rowend) break;
+            if (it < e.z && own && J == e.x - I * NZ) {
+                const float inc = cld(S->sfct + e.w + it);
+                if (e.y == SM_SX || e.y == SM_BULK) nxx += inc;
+                if (e.y == SM_SZ || e.y == SM_BULK) nzz += inc;
+                if (e.y == SM_SXZ) nxz += inc;
+            }
+        }
+        bst(Rout, o, E2_TXX * N4, nxx);
+        bst(Rout, o, E2_TZZ * N4, nzz);
+        bst(Rout, o, E2_TXZ * N4, nxz);
+        // receivers on this row (e_iso.c:641-669 samples after add_src)
+        if (record) {
+            for (; rp < nrec; ++rp) {
+                const int4 e = cld(rcv + rp);
+                if (e.x >= rowend) break;
+                if (own && J == e.x - I * NZ) {
+                    const float val = e.y == 0 ? vx0 : e.y == 1 ? vz0 : e.y == 2 ? nxx
+                                    : e.y == 3 ? nzz : nxz;
+                    *(__attribute__((address_space(1))) float *)(S->trace + (size_t)row * nrec + e.z) = val;
+                }
+            }
+        }
+        sxx0 = sxx1; szz0 = szz1; sxz0 = sxz1;
+        sxx1 = nsxx; szz1 = nszz; sxz1 = nsxz;
+        vr = nvr; zr = nzr; tr = ntr;
+        vx0 = vx1;
+        vz_m = vz0;
+    }
+}
+
+__global__ void e2d_base_set(int *base, int value, int add)
+{
+    if (threadIdx.x == 0) *base = add ? *base + value : value;
+}
+
+int grid_blocks(int nitems)
+{
+    const int nb = (nitems + E2D_WAVES - 1) / E2D_WAVES;
+    return (nb + 7) & ~7;
+}
+
+}  // namespace
+
+E2dRunner::~E2dRunner()
+{
+    drop_graphs();
+    if (table_) (void)hipFree(table_);
+    if (base_) (void)hipFree(base_);
+}
+
+void E2dRunner::drop_graphs()
+{
+    for (auto &kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    graphs_.clear();
+}
+
+size_t E2dRunner::device_bytes() const { return table_cap_ + (base_ ? 256 : 0); }
+
+int E2dRunner::ensure(size_t bytes)
+{
+    if (!base_) SM_HIP(hipMalloc((void **)&base_, 256));
+    if (bytes <= table_cap_) return SM_OK;
+    drop_graphs();
+    if (table_) SM_HIP(hipFree(table_));
+    table_ = nullptr;
+    table_cap_ = 0;
+    const size_t cap = std::max<size_t>(bytes, 4096);
+    SM_HIP(hipMalloc(&table_, cap));
+    table_cap_ = cap;
+    return SM_OK;
+}
+
+int E2dRunner::graph_for(int nact, int nitems, int g, hipStream_t st, hipGraphExec_t *out)
+{
+    const auto key = std::make_tuple(nact, nitems, g);
+    auto f = graphs_.find(key);
+    if (f != graphs_.end()) { *out = f->second; return SM_OK; }
+    const E2dShot *tab = (const E2dShot *)table_;
+    const int *bshot = (const int *)((const char *)table_ + tab_bytes_);
+    const int nb = grid_blocks(nitems);
+    hipGraph_t graph = nullptr;
+    SM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < g; ++k)
+        e2d_fused_step<<<nb, 256, 0, st>>>(tab, bshot, nitems, base_, k);
+    e2d_base_set<<<1, 64, 0, st>>>(base_, g, 1);
+    SM_HIP(hipStreamEndCapture(st, &graph));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(SM_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    graphs_[key] = ex;
+    *out = ex;
+    return SM_OK;
+}
+
+int E2dRunner::run(std::vector<E2dShot> shots, hipStream_t st, int graph_steps)
+{
+    shots.erase(std::remove_if(shots.begin(), shots.end(),
+                               [](const E2dShot &d) { return d.nsteps <= 0; }),
+                shots.end());
+    if (shots.empty()) return SM_OK;
+    std::stable_sort(shots.begin(), shots.end(),
+                     [](const E2dShot &a, const E2dShot &b) { return a.nsteps > b.nsteps; });
+    const size_t n = shots.size();
+    std::vector<int> pref(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) {
+        shots[i].item0 = pref[i];
+        int ntz, nch;
+        const long long next =
+            (long long)pref[i] + e2d_items(shots[i].NX, shots[i].NZ, shots[i].chunk, &ntz, &nch);
+        SM_CHECK(next < (1ll << 30), SM_ERR_ARG, "batch too large: %lld work items", next);
+        pref[i + 1] = (int)next;
+    }
+    // block -> shot map (every shot's items are whole blocks)
+    const int nblk = pref[n] / E2D_WAVES;
+    tab_bytes_ = (n * sizeof(E2dShot) + 255) & ~(size_t)255;
+    std::vector<unsigned char> host(tab_bytes_ + (size_t)nblk * sizeof(int));
+    std::memcpy(host.data(), shots.data(), n * sizeof(E2dShot));
+    int *bs = (int *)(host.data() + tab_bytes_);
+    for (size_t i = 0; i < n; ++i)
+        for (int q = pref[i] / E2D_WAVES; q < pref[i + 1] / E2D_WAVES; ++q) bs[q] = (int)i;
+    if (host.size() > table_cap_ || tab_bytes_ != last_tab_bytes_) drop_graphs();
+    last_tab_bytes_ = tab_bytes_;
+    int rc = ensure(host.size());
+    if (rc != SM_OK) return rc;
+    // the table lives in device memory read by the kernels of this run;
+    // waiting for the copy keeps the host buffer's lifetime simple
+    SM_HIP(hipMemcpyAsync(table_, host.data(), host.size(), hipMemcpyHostToDevice, st));
+    SM_HIP(hipStreamSynchronize(st));
+
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (timing) {
+        SM_HIP(hipEventCreate(&t0));
+        SM_HIP(hipEventCreate(&t1));
+        SM_HIP(hipEventRecord(t0, st));
+    }
+    const E2dShot *tab = (const E2dShot *)table_;
+    const int *bshot = (const int *)((const char *)table_ + tab_bytes_);
+    size_t count = 0;
+    int s = 0;
+    const int total = shots[0].nsteps;
+    while (s < total) {
+        int m = (int)n;
+        while (m > 0 && shots[m - 1].nsteps <= s) --m;
+        const int e = shots[m - 1].nsteps;
+        const int nitems = pref[m];
+        int len = e - s;
+        if (graph_steps > 1 && len >= 2 * graph_steps) {
+            hipGraphExec_t ex = nullptr;
+            rc = graph_for(m, nitems, graph_steps, st, &ex);
+            if (rc != SM_OK) return rc;
+            e2d_base_set<<<1, 64, 0, st>>>(base_, s, 0);
+            const int reps = len / graph_steps;
+            for (int r = 0; r < reps; ++r) SM_HIP(hipGraphLaunch(ex, st));
+            s += reps * graph_steps;
+            count += (size_t)reps * graph_steps;
+            len = e - s;
+        }
+        const int nb = grid_blocks(nitems);
+        for (int k = 0; k < len; ++k)
+            e2d_fused_step<<<nb, 256, 0, st>>>(tab, bshot, nitems, nullptr, s + k);
+        s += len;
+        count += (size_t)len;
+    }
+    SM_HIP(hipGetLastError());
+    if (timing) {
+        SM_HIP(hipEventRecord(t1, st));
+        SM_HIP(hipEventSynchronize(t1));
+        float ms = 0.f;
+        SM_HIP(hipEventElapsedTime(&ms, t0, t1));
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        ms_total += ms;
+        launches += count;
+    }
+    return SM_OK;
+}
+
+}  // namespace sm

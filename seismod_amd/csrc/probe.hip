@@ -8,27 +8,64 @@
 
 namespace sm {
 
-// out[w][c] = sum_r in[r][c] for reads arrays in, writes arrays out
+// out[w][c] = sum_r in[r][c] for R arrays in, W arrays out; the array
+// counts are compile-time so every load of an element is issued before the
+// first is used (no latency chain between streams).  Each lane moves V
+// consecutive floats per array (V = 1, 2 or 4: dword, dwordx2, dwordx4).
+template <int V> struct VecT;
+template <> struct VecT<1> { using T = float; };
+template <> struct VecT<2> { using T = float2; };
+template <> struct VecT<4> { using T = float4; };
+
+template <int R, int W, int V>
 __global__ void __launch_bounds__(256)
-probe_stream(const float *__restrict__ in, float *__restrict__ out, size_t n, int reads, int writes)
+probe_stream(const float *__restrict__ in, float *__restrict__ out, size_t n)
 {
+    using T = typename VecT<V>::T;
+    const size_t nv = n / V;
+    const T *vin = (const T *)in;
+    T *vout = (T *)out;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
-        float s = 0.f;
-        for (int r = 0; r < reads; ++r) s += in[(size_t)r * n + c];
-        for (int w = 0; w < writes; ++w) out[(size_t)w * n + c] = s + (float)w;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nv; c += stride) {
+        T v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = vin[(size_t)r * nv + c];
+        T s = v[0];
+#pragma unroll
+        for (int r = 1; r < R; ++r) s = s + v[r];
+#pragma unroll
+        for (int w = 0; w < W; ++w) vout[(size_t)w * nv + c] = s;
     }
+}
+
+using ProbeFn = void (*)(const float *, float *, size_t);
+
+ProbeFn probe_kernel(int reads, int writes, int vec)
+{
+#define SM_PROBE(r, w)                                                              \
+    if (reads == r && writes == w)                                                  \
+        return vec == 4 ? probe_stream<r, w, 4> : vec == 2 ? probe_stream<r, w, 2>  \
+                                                           : probe_stream<r, w, 1>;
+    SM_PROBE(1, 1) SM_PROBE(2, 1) SM_PROBE(3, 1) SM_PROBE(4, 2) SM_PROBE(6, 4) SM_PROBE(8, 4)
+    SM_PROBE(10, 4) SM_PROBE(12, 8) SM_PROBE(1, 2) SM_PROBE(5, 5)
+#undef SM_PROBE
+    return nullptr;
 }
 
 }  // namespace sm
 
-extern "C" int sm_probe_stream(sm_ctx *ctx, size_t n, int reads, int writes, int iters,
-                               double *ms_per_iter)
+extern "C" int sm_probe_stream(sm_ctx *ctx, size_t n, int reads, int writes, int vec,
+                               int iters, double *ms_per_iter)
 {
     using namespace sm;
     SM_CHECK(ctx && ms_per_iter, SM_ERR_ARG, "sm_probe_stream: NULL argument");
-    SM_CHECK(n > 0 && reads >= 1 && reads <= 16 && writes >= 1 && writes <= 16 && iters >= 1,
-             SM_ERR_ARG, "sm_probe_stream: bad sizes");
+    SM_CHECK(n > 0 && iters >= 1, SM_ERR_ARG, "sm_probe_stream: bad sizes");
+    SM_CHECK(vec == 1 || vec == 2 || vec == 4, SM_ERR_ARG, "sm_probe_stream: vec must be 1, 2 or 4");
+    SM_CHECK(n % vec == 0, SM_ERR_ARG, "sm_probe_stream: n must be a multiple of vec");
+    const ProbeFn fn = probe_kernel(reads, writes, vec);
+    SM_CHECK(fn, SM_ERR_ARG,
+             "sm_probe_stream: (reads, writes) = (%d, %d) not built; available (1,1) (2,1) (3,1) "
+             "(4,2) (6,4) (8,4) (10,4) (12,8) (1,2) (5,5)", reads, writes);
     SM_CHECK(set_device(ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
     sm_dbuf a, b;
     int rc = dalloc(a, (size_t)reads * n * sizeof(float));
@@ -43,10 +80,9 @@ extern "C" int sm_probe_stream(sm_ctx *ctx, size_t n, int reads, int writes, int
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) dev_cus = prop.multiProcessorCount;
     const int grid = std::max(1, dev_cus) * 16;
-    probe_stream<<<grid, 256, 0, st>>>((const float *)a.ptr, (float *)b.ptr, n, reads, writes);
+    fn<<<grid, 256, 0, st>>>((const float *)a.ptr, (float *)b.ptr, n);
     SM_HIP(hipEventRecord(e0, st));
-    for (int it = 0; it < iters; ++it)
-        probe_stream<<<grid, 256, 0, st>>>((const float *)a.ptr, (float *)b.ptr, n, reads, writes);
+    for (int it = 0; it < iters; ++it) fn<<<grid, 256, 0, st>>>((const float *)a.ptr, (float *)b.ptr, n);
     SM_HIP(hipEventRecord(e1, st));
     SM_HIP(hipEventSynchronize(e1));
     float ms = 0.f;

@@ -5,7 +5,9 @@ Arrays cross the boundary as FP64 numpy arrays in the reference layouts:
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -53,6 +55,20 @@ def record_due(it: int, dt: float, t0: float, rdt: float) -> bool:
     return bool(lib.sm_record_due(it, dt, t0, rdt))
 
 
+_CONTEXTS = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all() -> None:
+    """Destroy live contexts (and their solvers) while the HIP runtime is
+    still fully up, rather than from interpreter teardown."""
+    for c in list(_CONTEXTS):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One device + one HIP stream."""
 
@@ -60,6 +76,13 @@ class Context:
         h = C.c_void_p()
         check(lib.sm_ctx_create(device, C.byref(h)), "sm_ctx_create")
         self.handle = h
+        # solvers and batches on this context; closed before the context so
+        # that no destroy call ever sees a freed stream
+        self._children = weakref.WeakSet()
+        _CONTEXTS.add(self)
+
+    def adopt(self, child) -> None:
+        self._children.add(child)
 
     def sync(self) -> None:
         check(lib.sm_ctx_sync(self.handle), "sm_ctx_sync")
@@ -69,15 +92,17 @@ class Context:
         check(lib.sm_ctx_mem_info(self.handle, C.byref(free), C.byref(total)), "sm_ctx_mem_info")
         return free.value, total.value
 
-    def probe_stream(self, n: int, reads: int, writes: int, iters: int = 20) -> float:
+    def probe_stream(self, n: int, reads: int, writes: int, iters: int = 20, vec: int = 1) -> float:
         """ms per launch of a plain kernel streaming reads+writes FP32 arrays."""
         ms = C.c_double()
-        check(lib.sm_probe_stream(self.handle, n, reads, writes, iters, C.byref(ms)),
+        check(lib.sm_probe_stream(self.handle, n, reads, writes, vec, iters, C.byref(ms)),
               "sm_probe_stream")
         return ms.value
 
     def close(self) -> None:
         if self.handle:
+            for c in list(self._children):
+                c.close()
             check(lib.sm_ctx_destroy(self.handle), "sm_ctx_destroy")
             self.handle = None
 
@@ -139,6 +164,7 @@ class Acoustic3D:
             raise ValueError(f"model must hold {n} padded cells")
         self._p = cfg.params()
         h = C.c_void_p()
+        ctx.adopt(self)
         check(lib.sm_a3d_create(ctx.handle, C.byref(self._p), _dp(self._vp), _dp(self._rho),
                                 C.byref(h)), "sm_a3d_create")
         self.handle = h
@@ -229,10 +255,12 @@ class E2dConfig:
     storage: str = "f32"
     q1_compat: bool = True
     damping_scale: float = 1.0
+    naive: bool = False       # FP32 with the two reference-order kernels
 
     def params(self) -> L.E2dParams:
         return L.E2dParams(self.nx, self.nz, self.npml, self.h, self.dt, _STORAGE[self.storage],
-                           1 if self.q1_compat else 0, self.damping_scale, 0)
+                           1 if self.q1_compat else 0, self.damping_scale,
+                           1 if self.naive else 0)
 
     @property
     def padded(self) -> tuple[int, int]:
@@ -250,6 +278,7 @@ class Elastic2D:
             raise ValueError(f"model must hold {n} cells")
         self._p = cfg.params()
         h = C.c_void_p()
+        ctx.adopt(self)
         check(lib.sm_e2d_create(ctx.handle, C.byref(self._p), *(_dp(a) for a in self._m),
                                 C.byref(h)), "sm_e2d_create")
         self.handle = h
@@ -297,9 +326,70 @@ class Elastic2D:
     def reset(self) -> None:
         check(lib.sm_e2d_reset(self.handle), "sm_e2d_reset")
 
+    def set_launch(self, rows_per_chunk: int, graph_steps: int = 32) -> None:
+        check(lib.sm_e2d_set_launch(self.handle, rows_per_chunk, graph_steps), "sm_e2d_set_launch")
+
+    def launch(self) -> tuple[int, int]:
+        a, b = C.c_int(), C.c_int()
+        check(lib.sm_e2d_get_launch(self.handle, C.byref(a), C.byref(b)), "sm_e2d_get_launch")
+        return a.value, b.value
+
+    def set_timing(self, on: bool) -> None:
+        check(lib.sm_e2d_set_timing(self.handle, 1 if on else 0), "sm_e2d_set_timing")
+
+    def timing(self) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_size_t()
+        check(lib.sm_e2d_get_timing(self.handle, C.byref(ms), C.byref(n)), "sm_e2d_get_timing")
+        return ms.value, n.value
+
+    def device_bytes(self) -> int:
+        b = C.c_size_t()
+        check(lib.sm_e2d_device_bytes(self.handle, C.byref(b)), "sm_e2d_device_bytes")
+        return b.value
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             check(lib.sm_e2d_destroy(self.handle), "sm_e2d_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class E2dBatch:
+    """Runs many Elastic2D shots (FP32 fused) together: one launch per time
+    step over all shots still running, replayed from hipGraphs."""
+
+    def __init__(self, ctx: Context, graph_steps: int = 32):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.adopt(self)
+        check(lib.sm_e2d_batch_create(ctx.handle, C.byref(h)), "sm_e2d_batch_create")
+        self.handle = h
+        check(lib.sm_e2d_batch_set_graph_steps(h, graph_steps), "sm_e2d_batch_set_graph_steps")
+
+    def run(self, shots, nsteps) -> None:
+        arr = (C.c_void_p * len(shots))(*[s.handle.value for s in shots])
+        steps = np.ascontiguousarray(nsteps, dtype=np.uintp)
+        if steps.size != len(shots):
+            raise ValueError("one step count per shot")
+        check(lib.sm_e2d_batch_run(self.handle, len(shots), arr, _sp(steps)), "sm_e2d_batch_run")
+
+    def set_timing(self, on: bool) -> None:
+        check(lib.sm_e2d_batch_set_timing(self.handle, 1 if on else 0), "sm_e2d_batch_set_timing")
+
+    def timing(self) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_size_t()
+        check(lib.sm_e2d_batch_get_timing(self.handle, C.byref(ms), C.byref(n)),
+              "sm_e2d_batch_get_timing")
+        return ms.value, n.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            check(lib.sm_e2d_batch_destroy(self.handle), "sm_e2d_batch_destroy")
             self.handle = None
 
     def __del__(self):

@@ -6,7 +6,8 @@ from a seed so runs are reproducible:
     depths, plus a Gaussian-smoothed random perturbation, Gardner density
     rho = 310 Vp^0.25, padded into the CPML by edge replication exactly as
     read_model_a does (io_utils.c:1089-1143);
-  - e_iso 400x400 two-layer model with a surface receiver line.
+  - e_iso 400x400 two-layer model with a surface receiver line;
+  - batches of e_iso shots of random size, layering, frequency and length.
 """
 from __future__ import annotations
 
@@ -105,3 +106,90 @@ def algorithmic_bytes_a3d(nx2: int, ny2: int, nz2: int, np_: int, state_bytes: i
     """
     frac = sum(2.0 * np_ / n for n in (nx2, ny2, nz2))
     return state_bytes * 8 + mat_bytes * 2 + state_bytes * 4 * frac
+
+
+def algorithmic_bytes_e2d(nx: int, nz: int, npml: int, state_bytes: int = 4) -> float:
+    """Minimal HBM bytes per padded cell per step of the 2D split-PML update.
+
+    Interior cell: vx, vz, txx, tzz, txz read and written once and three
+    material values (rho, lambda, mu) read once: 13 words.  PML cell: the ten
+    split fields read and written: 20 words.
+    """
+    NX, NZ = nx + 2 * npml, nz + 2 * npml
+    interior = nx * nz
+    pml = NX * NZ - interior
+    return state_bytes * (13.0 * interior + 20.0 * pml) / (NX * NZ)
+
+
+@dataclass
+class E2dShot:
+    nx: int
+    nz: int
+    npml: int
+    h: float
+    dt: float
+    f0: float
+    nsteps: int
+    vp: np.ndarray          # interior (nx, nz), FP64
+    vs: np.ndarray
+    rho: np.ndarray
+    src_type: np.ndarray
+    src_idx: np.ndarray
+    rec_idx: np.ndarray
+    rec_comp: np.ndarray
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def cells(self) -> int:
+        return (self.nx + 2 * self.npml) * (self.nz + 2 * self.npml)
+
+
+def e2d_two_layer_shot(n: int = 400, npml: int = 20, h: float = 5.0, dt: float = 5.0e-4,
+                       nsteps: int = 2000, f0: float = 25.0) -> E2dShot:
+    """e_iso two-layer shot: (Vp, Vs, rho) = (2500, 1400, 2000) above node
+    z = n/2 and (3500, 2000, 2400) below, Bulk source at node (n/2, n/8),
+    Vz receivers on the surface row every 4 nodes (100 at n = 400)."""
+    vp = np.empty((n, n)); vs = np.empty((n, n)); rho = np.empty((n, n))
+    zc = n // 2
+    vp[:, :zc], vs[:, :zc], rho[:, :zc] = 2500.0, 1400.0, 2000.0
+    vp[:, zc:], vs[:, zc:], rho[:, zc:] = 3500.0, 2000.0, 2400.0
+    src = np.array([(n // 2) * n + n // 8], dtype=np.uintp)
+    rec = np.array([4 * r * n for r in range(n // 4)], dtype=np.uintp)
+    return E2dShot(nx=n, nz=n, npml=npml, h=h, dt=dt, f0=f0, nsteps=nsteps, vp=vp, vs=vs,
+                   rho=rho, src_type=np.array([BULK], dtype=np.intc), src_idx=src,
+                   rec_idx=rec, rec_comp=np.full(len(rec), VZ, dtype=np.intc))
+
+
+def e2d_batch_shots(count: int = 64, seed: int = 4, npml: int = 20, h: float = 5.0,
+                    nx_range=(300, 1200), nz_range=(300, 800), steps_range=(1000, 3000),
+                    f_range=(10.0, 40.0)) -> list[E2dShot]:
+    """Independent e_iso shots: grid nx ~ U{300..1200} by nz ~ U{300..800},
+    3-6 layers with Vp ~ U[2000, 5000] (sorted with depth), Vs = Vp/U[1.7, 1.9],
+    rho ~ U[1800, 2700], Ricker f ~ U[10, 40] Hz, steps ~ U{1000..3000},
+    dt at half the CFL limit, Bulk source at the top centre, Vz receivers
+    every 4 nodes on row 0."""
+    rng = np.random.default_rng(seed)
+    shots = []
+    for _ in range(count):
+        nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        nz = int(rng.integers(nz_range[0], nz_range[1] + 1))
+        nl = int(rng.integers(3, 7))
+        tops = np.sort(rng.choice(np.arange(1, nz), size=nl - 1, replace=False))
+        vpl = np.sort(rng.uniform(2000.0, 5000.0, size=nl))
+        ratio = rng.uniform(1.7, 1.9, size=nl)
+        rhol = rng.uniform(1800.0, 2700.0, size=nl)
+        layer = np.searchsorted(tops, np.arange(nz), side="right")
+        vp = np.broadcast_to(vpl[layer], (nx, nz)).copy()
+        vs = np.broadcast_to((vpl / ratio)[layer], (nx, nz)).copy()
+        rho = np.broadcast_to(rhol[layer], (nx, nz)).copy()
+        f0 = float(rng.uniform(*f_range))
+        nsteps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+        dt = 0.5 * h / (np.sqrt(2.0) * vp.max())
+        src = np.array([(nx // 2) * nz + 1], dtype=np.uintp)
+        rec = np.array([4 * r * nz for r in range((nx + 3) // 4)], dtype=np.uintp)
+        shots.append(E2dShot(nx=nx, nz=nz, npml=npml, h=h, dt=dt, f0=f0, nsteps=nsteps,
+                             vp=vp, vs=vs, rho=rho, src_type=np.array([BULK], dtype=np.intc),
+                             src_idx=src, rec_idx=rec,
+                             rec_comp=np.full(len(rec), VZ, dtype=np.intc),
+                             meta=dict(seed=seed)))
+    return shots
