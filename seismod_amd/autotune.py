@@ -1,0 +1,127 @@
+"""Launch-shape autotuner for the fused kernels.
+
+The fused kernels give bit-identical results for every launch shape, so the
+shape is a pure performance choice.  The tuner times a few steps of each
+candidate on the solver itself (HIP-event timing of the kernel launches)
+and keeps the fastest; results are cached per grid shape (and storage) for
+the life of the process and, optionally, in a JSON file.
+
+Tuning runs steps and then resets the solver (fields, memory variables,
+traces and step counter back to zero), so call it right after set-up,
+before the solver's real run.
+
+  3D (Acoustic3D):  rows_per_block in {1, 2, 4, 8} x x_chunks
+  2D (Elastic2D):   rows_per_chunk in {1, 2, 3, 4, 6, 8, 12, 16}
+"""
+from __future__ import annotations
+
+import json
+import os
+import threading
+from pathlib import Path
+
+_LOCK = threading.Lock()
+_CACHE: dict[str, tuple] = {}
+
+
+def _key_a3d(sim) -> str:
+    c = sim.cfg
+    return f"a3d:{c.nx + 2 * c.np}x{c.ny + 2 * c.np}x{c.nz + 2 * c.np}:{c.storage}"
+
+
+def _key_e2d(sim) -> str:
+    # shapes within 128 cells of each other share a tuning
+    c = sim.cfg
+    return f"e2d:{(c.nx + 2 * c.npml) // 128}x{(c.nz + 2 * c.npml) // 128}"
+
+
+def _load(path):
+    if path and Path(path).exists():
+        try:
+            for k, v in json.loads(Path(path).read_text()).items():
+                _CACHE.setdefault(k, tuple(v))
+        except (OSError, ValueError):
+            pass
+
+
+def _save(path):
+    if not path:
+        return
+    tmp = Path(str(path) + f".{os.getpid()}.tmp")
+    tmp.write_text(json.dumps({k: list(v) for k, v in _CACHE.items()}, indent=1, sort_keys=True))
+    os.replace(tmp, path)
+
+
+def _time(sim, ctx, steps: int) -> float:
+    sim.reset()
+    sim.run(2)
+    ctx.sync()
+    sim.set_timing(True)
+    sim.run(steps)
+    ms, n = sim.timing()
+    sim.set_timing(False)
+    return ms / max(n, 1)
+
+
+def tune_a3d(sim, ctx, steps: int = 8, cache_file=None, candidates=None) -> tuple[int, int]:
+    """Pick (rows_per_block, x_chunks) for an Acoustic3D solver; applies it."""
+    key = _key_a3d(sim)
+    with _LOCK:
+        _load(cache_file)
+        if key in _CACHE:
+            sim.set_launch(*_CACHE[key])
+            return _CACHE[key]
+    nx2 = sim.cfg.nx + 2 * sim.cfg.np
+    if candidates is None:
+        candidates = [(tj, ch) for tj in (2, 4, 8) for ch in (2, 4, 6, 8, 12)
+                      if ch <= max(1, nx2 // 16)]
+    best = None
+    for tj, ch in candidates:
+        try:
+            sim.set_launch(tj, ch)
+        except Exception:
+            continue
+        t = _time(sim, ctx, steps)
+        if best is None or t < best[0]:
+            best = (t, tj, ch)
+    sim.reset()
+    if best is None:
+        return sim.launch()
+    choice = (best[1], best[2])
+    sim.set_launch(*choice)
+    with _LOCK:
+        _CACHE[key] = choice
+        _save(cache_file)
+    return choice
+
+
+def tune_e2d(sim, ctx, steps: int = 20, cache_file=None, candidates=None) -> tuple[int, int]:
+    """Pick rows_per_chunk for an Elastic2D solver (graph steps kept);
+    applies it.  Returns (rows_per_chunk, graph_steps)."""
+    key = _key_e2d(sim)
+    gs = sim.launch()[1]
+    with _LOCK:
+        _load(cache_file)
+        if key in _CACHE:
+            sim.set_launch(_CACHE[key][0], gs)
+            return _CACHE[key][0], gs
+    nx2 = sim.cfg.nx + 2 * sim.cfg.npml
+    if candidates is None:
+        candidates = [c for c in (1, 2, 3, 4, 6, 8, 12, 16) if c <= nx2]
+    best = None
+    for c in candidates:
+        sim.set_launch(c, 0)
+        t = _time(sim, ctx, steps)
+        if best is None or t < best[0]:
+            best = (t, c)
+    sim.reset()
+    sim.set_launch(best[1], gs)
+    with _LOCK:
+        _CACHE[key] = (best[1],)
+        _save(cache_file)
+    return best[1], gs
+
+
+def clear_cache() -> None:
+    with _LOCK:
+        _CACHE.clear()

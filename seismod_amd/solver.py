@@ -152,6 +152,11 @@ class A3dConfig:
     def shape(self) -> tuple[int, int, int]:
         return (self.nx + 2 * self.np, self.ny + 2 * self.np, self.nz + 2 * self.np)
 
+    def bytes_needed(self, nrec: int = 0, max_steps: int = 0) -> int:
+        """Device bytes a solver with this configuration will hold."""
+        p = self.params()
+        return int(lib.sm_a3d_bytes_needed(C.byref(p), nrec, max_steps))
+
 
 class Acoustic3D:
     """3D acoustic CPML solver (a_iso_3d)."""
@@ -265,6 +270,11 @@ class E2dConfig:
     @property
     def padded(self) -> tuple[int, int]:
         return (self.nx + 2 * self.npml, self.nz + 2 * self.npml)
+
+    def bytes_needed(self, nrec: int = 0, max_steps: int = 0) -> int:
+        """Device bytes a solver with this configuration will hold."""
+        p = self.params()
+        return int(lib.sm_e2d_bytes_needed(C.byref(p), nrec, max_steps))
 
 
 class Elastic2D:
