@@ -397,7 +397,7 @@ FusedArgs<S> fused_args(sm_a3d *s)
 {
     FusedArgs<S> a{};
     a.nx2 = s->nx2; a.ny2 = s->ny2; a.nz2 = s->nz2; a.np = (int)s->p.np;
-    a.ntk = (s->nz2 + 62) / 63;      // 63 owned cells per wave along z
+    a.ntk = (s->nz2 + 61) / 62;      // 62 owned cells per wave along z
     a.ntiles = a.ntk * ((s->ny2 + s->tj - 1) / s->tj);
     a.chunk_len = (s->nx2 + s->nchunks - 1) / s->nchunks;
     a.nreal = a.ntiles * s->nchunks;
@@ -570,7 +570,7 @@ int valid_chunks(int nx2, int want)
 void default_launch(sm_a3d *s)
 {
     s->tj = 4;
-    const int ntiles = ((s->nz2 + 62) / 63) * ((s->ny2 + s->tj - 1) / s->tj);
+    const int ntiles = ((s->nz2 + 61) / 62) * ((s->ny2 + s->tj - 1) / s->tj);
     const int slots = resident_blocks(s, s->tj);
     int nch = (3 * slots + ntiles - 1) / ntiles;
     nch = std::min(nch, std::max(1, s->nx2 / 24));

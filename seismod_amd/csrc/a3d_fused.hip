@@ -96,12 +96,13 @@ __device__ __forceinline__ float vel_update(float v_old, float p0, float p1, flo
     return v_old - two_dt * fast_rcp(r0 + r1) * dP;
 }
 
-// what one lane reads for plane i, loaded one plane ahead
+// what one lane reads for plane i, loaded one plane ahead.  The (j+1) and
+// (k+1) neighbours of P and rho come from other waves (LDS) and lanes
+// (shuffles); only the tile's top wave loads the row above the tile.
 struct Col {
     float vx, vy, vz, K;    // old velocities and K*dt at (i, j, k)
     float Pn, rn;           // P, rho at (i+1, j, k)   (clamped to the last plane)
-    float Pj, rj;           // P, rho at (i, j+1, k)   (clamped to the last row)
-    float Pk, rk;           // P, rho at (i, j, k+1)   (clamped to the last column)
+    float Pj, rj;           // top wave only: P, rho at (i, j+1, k) (clamped to the last row)
     float Pjm, rjm, vyjm;   // wave 0 only: (i, j-1, k)
     float mPx, mPy, mPz;    // velocity memory variables (slab cells only)
     float mPym;             // wave 0 only: at the halo row
@@ -116,7 +117,7 @@ struct Col {
 
 // constants of one lane for one work item
 struct Lane {
-    unsigned o_jk, o_jp, o_kp, o_jm;   // in-plane offsets of the cell and its neighbours
+    unsigned o_jk, o_jp, o_jm;         // in-plane offsets of the cell and its y neighbours
     int j, kk, kc;                     // j is wave-uniform
     bool own;                          // this lane writes its cell
     int syv, syp, syh;                 // y slab coordinates (wave-uniform)
@@ -127,7 +128,7 @@ struct Lane {
     float zbh, zah, zikh, zb, za, zik;
 };
 
-template <typename S>
+template <typename S, int TJ>
 __device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, int w, int i,
                                          unsigned plane, int sxv, int sxp, Col &c)
 {
@@ -140,10 +141,10 @@ __device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, i
     c.K = ld_at(a.mat, a.nnn + o + L.o_jk);
     c.Pn = Store<S>::ld(a.in, on + L.o_jk, a.sP);
     c.rn = ld_at(a.mat, on + L.o_jk);
-    c.Pj = Store<S>::ld(a.in, o + L.o_jp, a.sP);
-    c.rj = ld_at(a.mat, o + L.o_jp);
-    c.Pk = Store<S>::ld(a.in, o + L.o_kp, a.sP);
-    c.rk = ld_at(a.mat, o + L.o_kp);
+    if (w == TJ - 1) {
+        c.Pj = Store<S>::ld(a.in, o + L.o_jp, a.sP);
+        c.rj = ld_at(a.mat, o + L.o_jp);
+    }
     if (w == 0) {
         c.Pjm = Store<S>::ld(a.in, o + L.o_jm, a.sP);
         c.rjm = ld_at(a.mat, o + L.o_jm);
@@ -161,16 +162,25 @@ __device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, i
     if (L.szp >= 0) c.mvz = Store<S>::ld(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, a.sMp);
 }
 
+// LDS of one block: new vy of each row (for the row above), and old P and
+// rho of each row one plane ahead (for the row below); slots by plane parity
+template <int TJ>
+struct Shared {
+    float vy[2][TJ][64];
+    float P[2][TJ][64];
+    float r[2][TJ][64];
+};
+
 // one plane: prefetch plane i+1 into nxt, update plane i from cur
 template <typename S, int TJ>
 __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L, int w, int i,
                                            int i0, int i1, unsigned plane, Col &cur, Col &nxt,
                                            float &P_c, float &r_c, float &vx_prev,
-                                           float (*s_vy)[TJ][64])
+                                           Shared<TJ> &sh)
 {
     if (i + 1 < i1) {
         const XPlane xn = a.xpl[i + 1];
-        load_col<S>(a, L, w, i + 1, plane, xn.sxv, xn.sxp, nxt);
+        load_col<S, TJ>(a, L, w, i + 1, plane, xn.sxv, xn.sxp, nxt);
     }
     const XPlane x = a.xpl[i];
     const unsigned o = (unsigned)i * plane + L.o_jk;
@@ -181,17 +191,28 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
     float mx = cur.mPx;
     const float vxn = vel_update(cur.vx, P_c, cur.Pn, r_c, cur.rn, a.idx, a.two_dt, x.sxv >= 0,
                                  x.bh, x.ah, x.ikh, mx);
+    const int lane = threadIdx.x;
     if (i < i0) {   // chunk start: only vx of the plane below is needed
         vx_prev = vxn;
+        sh.P[(i + 1) & 1][w][lane] = cur.Pn;
+        sh.r[(i + 1) & 1][w][lane] = cur.rn;
+        __syncthreads();
         P_c = cur.Pn;
         r_c = cur.rn;
         return;
     }
+    // neighbours: (j+1) from the wave above (written one plane earlier) or,
+    // for the top wave, loaded; (k+1) from the next lane (lane 63 is a halo
+    // lane whose own vz is never used)
+    const float Pj = w == TJ - 1 ? cur.Pj : sh.P[i & 1][w + 1][lane];
+    const float rj = w == TJ - 1 ? cur.rj : sh.r[i & 1][w + 1][lane];
+    const float Pk = __shfl_down(P_c, 1);
+    const float rk = __shfl_down(r_c, 1);
     // vy, vz at (i, j, k), a_iso_3d.c:621-651
     float my = cur.mPy, mz = cur.mPz;
-    const float vyn = vel_update(cur.vy, P_c, cur.Pj, r_c, cur.rj, a.idy, a.two_dt, L.syv >= 0,
+    const float vyn = vel_update(cur.vy, P_c, Pj, r_c, rj, a.idy, a.two_dt, L.syv >= 0,
                                  L.ybh, L.yah, L.yikh, my);
-    const float vzn = vel_update(cur.vz, P_c, cur.Pk, r_c, cur.rk, a.idz, a.two_dt, L.szv >= 0,
+    const float vzn = vel_update(cur.vz, P_c, Pk, r_c, rk, a.idz, a.two_dt, L.szv >= 0,
                                  L.zbh, L.zah, L.zikh, mz);
     if (L.own) {
         Store<S>::st(a.out, F_VX * a.nnn + o, vxn, a.isV);
@@ -209,9 +230,11 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
         vy_halo = vel_update(cur.vyjm, cur.Pjm, P_c, cur.rjm, r_c, a.idy, a.two_dt, L.syh >= 0,
                              L.hbh, L.hah, L.hikh, mh);
     }
-    s_vy[i & 1][w][threadIdx.x] = vyn;
+    sh.vy[i & 1][w][lane] = vyn;
+    sh.P[(i + 1) & 1][w][lane] = cur.Pn;
+    sh.r[(i + 1) & 1][w][lane] = cur.rn;
     __syncthreads();
-    const float vy_m = w > 0 ? s_vy[i & 1][w - 1][threadIdx.x] : vy_halo;
+    const float vy_m = w > 0 ? sh.vy[i & 1][w - 1][lane] : vy_halo;
     const float vz_m = __shfl_up(vzn, 1);   // lane 0 is the left halo lane
 
     // pressure at (i, j, k), a_iso_3d.c:659-683
@@ -262,7 +285,7 @@ template <typename S, int TJ>
 __global__ void __launch_bounds__(64 * TJ)
 a3d_fused_step(FusedArgs<S> a)
 {
-    __shared__ float s_vy[2][TJ][64];
+    __shared__ Shared<TJ> sh;
 
     // ---------------- receiver gather blocks (previous step) -------------
     if ((int)blockIdx.x >= a.nblocks) {
@@ -287,19 +310,21 @@ a3d_fused_step(FusedArgs<S> a)
         const int tile = logical - chunk * a.ntiles;
         const int tj = tile / a.ntk;
         const int tk = tile - tj * a.ntk;
-        // 63 owned cells per wave along z; lane 0 sits on the last cell of
-        // the tile to the left and only supplies vz at k0-1
+        // 62 owned cells per wave along z (lanes 1..62); lane 0 sits on the
+        // last cell of the tile to the left and supplies vz at k0-1, lane 63
+        // on the first cell of the tile to the right and supplies P, rho at
+        // k0+62.  Cells past the grid edge are clamped to the last cell, so
+        // a clamped neighbour equals the cell and its difference is 0.
         Lane L;
-        L.kk = tk * 63 - 1 + lane;
+        L.kk = tk * 62 - 1 + lane;
         L.j = tj * TJ + w;
         const bool jv = L.j < a.ny2, kv = L.kk >= 0 && L.kk < a.nz2;
-        L.own = jv && kv && lane >= 1;
+        L.own = jv && kv && lane >= 1 && lane <= 62;
         const int jc = min(L.j, a.ny2 - 1);
         L.kc = min(max(L.kk, 0), a.nz2 - 1);
-        const int jp = min(jc + 1, a.ny2 - 1), kp = min(L.kc + 1, a.nz2 - 1), jm = max(jc - 1, 0);
+        const int jp = min(jc + 1, a.ny2 - 1), jm = max(jc - 1, 0);
         L.o_jk = (unsigned)jc * a.nz2 + L.kc;
         L.o_jp = (unsigned)jp * a.nz2 + L.kc;
-        L.o_kp = (unsigned)jc * a.nz2 + kp;
         L.o_jm = (unsigned)jm * a.nz2 + L.kc;
         L.syv = jv && L.j < a.ny2 - 1 ? slab_vel(L.j, a.ny2, a.np) : -1;
         L.syp = jv ? slab_pre(L.j, a.ny2, a.np) : -1;
@@ -331,17 +356,22 @@ a3d_fused_step(FusedArgs<S> a)
         float r_c = ld_at(a.mat, (unsigned)ist * plane + L.o_jk);
         float vx_prev = 0.f;
         Col A, B;
+        A.Pj = A.rj = 0.f;
         A.Pjm = A.rjm = A.vyjm = A.mPym = 0.f;
         A.mPx = A.mPy = A.mPz = A.mvx = A.mvy = A.mvz = 0.f;
         B = A;
         {
             const XPlane xs = a.xpl[ist];
-            load_col<S>(a, L, w, ist, plane, xs.sxv, xs.sxp, A);
+            load_col<S, TJ>(a, L, w, ist, plane, xs.sxv, xs.sxp, A);
         }
+        // old P and rho of the first plane, for the row below
+        sh.P[ist & 1][w][lane] = P_c;
+        sh.r[ist & 1][w][lane] = r_c;
+        __syncthreads();
         for (int i = ist; i < i1; i += 2) {
-            plane_step<S, TJ>(a, L, w, i, i0, i1, plane, A, B, P_c, r_c, vx_prev, s_vy);
+            plane_step<S, TJ>(a, L, w, i, i0, i1, plane, A, B, P_c, r_c, vx_prev, sh);
             if (i + 1 >= i1) break;
-            plane_step<S, TJ>(a, L, w, i + 1, i0, i1, plane, B, A, P_c, r_c, vx_prev, s_vy);
+            plane_step<S, TJ>(a, L, w, i + 1, i0, i1, plane, B, A, P_c, r_c, vx_prev, sh);
         }
     }
 }

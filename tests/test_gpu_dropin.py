@@ -1,0 +1,209 @@
+"""GPU tests of the drop-in programs bin/e_iso and bin/a_iso_3d.
+
+Each program runs a small deck written in the reference's formats (.par,
+model, source and record files) with FP64 storage (SEISMOD_STORAGE=f64),
+and its output tree is compared with the FP64 oracle run on the same
+inputs:
+  - trace files trc/<base>/<base>_<n>.trc must be text-identical to the
+    oracle's samples printed in the reference's format ("%le    %le", t and
+    1e3 x value, io_utils.c:1744-1768), at the steps the record predicate
+    (e_iso.c:643-658) selects;
+  - netCDF snapshots ssh/<base>/... must hold the oracle's field at the
+    snapshot step, in mm/s, rounded to FP32 (io_utils.c:1822-2180).
+The FP32 default storage is checked against the same oracle within the
+FP32 tolerance of the other tests.
+"""
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+BIN = ROOT / "seismod_amd" / "bin"
+VX, VY, VZ, P = 0, 1, 2, 8
+
+
+def run(prog, cwd, storage):
+    env = dict(os.environ, SEISMOD_STORAGE=storage)
+    r = subprocess.run([str(BIN / prog), "-v", "-p", "run.par"], cwd=cwd, env=env,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "End of computation" in r.stdout
+    return r
+
+
+def trace_lines(values, dt, t0, rdt):
+    from seismod_amd.solver import record_due
+    return [f"{dt * (it + 1):e}    {1e3 * v:e}" for it, v in enumerate(values)
+            if record_due(it, dt, t0, rdt)]
+
+
+def read_trc(path):
+    return Path(path).read_text().splitlines()
+
+
+def nc_field(path):
+    from scipy.io import netcdf_file
+    with netcdf_file(str(path), "r", mmap=False) as f:
+        return np.array(f.variables["z"].data, dtype=np.float64)
+
+
+# ------------------------------------------------------------------ 2D --
+def deck_2d(tmp, nx=40, nz=30, npml=5, h=5.0, dt_ms=0.4, time_s=0.06):
+    rng = np.random.default_rng(11)
+    vp = np.where(np.arange(nz)[None, :] < nz // 2, 2500.0, 3500.0) * np.ones((nx, 1))
+    vs = vp / 1.75
+    rho = np.where(np.arange(nz)[None, :] < nz // 2, 2000.0, 2400.0) * np.ones((nx, 1))
+    vp = vp * (1 + 0.02 * rng.uniform(-1, 1, (nx, nz)))
+    with open(tmp / "model.dat", "w") as f:
+        f.write(f"{nx} {nz}\n{h} {h}\n0.0 0.0\nisotropic_elastic\n")
+        for i in range(nx):
+            for j in range(nz):
+                f.write(f"{vp[i, j]!r} {vs[i, j]!r} {rho[i, j]!r}\n")
+    (tmp / "src.dat").write_text("1\n1\nBulk\n1.0\n40.0\n100.0 50.0\n")
+    recs = [("Vx", "Trace", "50.0 25.0", 0.0, dt_ms), ("Vz", "Trace", "150.0 75.0", 0.01, 2 * dt_ms),
+            ("Vz", "Trace", "195.0 145.0", 0.0, dt_ms), ("Vz", "Snapshot", None, 0.03, 0.0)]
+    with open(tmp / "rcv.dat", "w") as f:
+        f.write(f"{len(recs)}\n")
+        for comp, kind, xz, t0, rdt in recs:
+            f.write(f"{comp}\n{kind}\n" + (f"{xz}\n" if xz else "") + f"{t0}\n{rdt}\n")
+    (tmp / "run.par").write_text(
+        "model.dat # model, file\nsrc.dat # source, file\nrcv.dat # output, records\n"
+        f"d2 # basename, name\n{time_s} # time, s\n{dt_ms} # dt, ms\n1 # abs, on\n{npml} # nl, layers\n")
+    src_idx = np.array([20 * nz + 10], dtype=np.uintp)
+    rec = [(10 * nz + 5, VX, 0.0, dt_ms), (30 * nz + 15, VZ, 0.01, 2 * dt_ms),
+           (39 * nz + 29, VZ, 0.0, dt_ms)]
+    return dict(nx=nx, nz=nz, npml=npml, h=h, dt=dt_ms * 1e-3, nsteps=int(round(time_s / (dt_ms * 1e-3))),
+                vp=vp, vs=vs, rho=rho, src_idx=src_idx, rec=rec, snap_t0=0.03)
+
+
+def oracle_2d(d, nsteps):
+    cfg = dict(nx=d["nx"], nz=d["nz"], npml=d["npml"], h=d["h"], dt=d["dt"], q1_compat=True)
+    L, fct = O.source_functions([1.0], [40.0], d["dt"])
+    ridx = np.array([r[0] for r in d["rec"]], dtype=np.uintp)
+    rcomp = np.array([r[1] for r in d["rec"]], dtype=np.intc)
+    tr, f, _ = O.e2d_run(cfg, d["vp"], d["vs"], d["rho"], np.array([6], dtype=np.intc),
+                         d["src_idx"], L, fct, ridx, rcomp, nsteps)
+    NX, NZ = d["nx"] + 2 * d["npml"], d["nz"] + 2 * d["npml"]
+    vz = f[NX * NZ:2 * NX * NZ].reshape(NX, NZ)
+    p = d["npml"]
+    return tr, vz[p:p + d["nx"], p:p + d["nz"]]
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_e_iso_program(tmp_path, storage):
+    from seismod_amd.solver import nsteps as sm_nsteps, record_due
+    d = deck_2d(tmp_path)
+    run("e_iso", tmp_path, storage)
+    ns = sm_nsteps(0.06, d["dt"])
+    tr, _ = oracle_2d(d, ns)
+    for q, (_, _, t0, rdt) in enumerate(d["rec"]):
+        got = read_trc(tmp_path / "trc" / "d2" / f"d2_{q + 1}.trc")
+        want = trace_lines(tr[:, q], d["dt"], t0, rdt * 1e-3)
+        assert len(got) == len(want) > 0
+        if storage == "f64":
+            assert got == want, q
+        else:
+            g = np.array([float(l.split()[1]) for l in got])
+            w = np.array([float(l.split()[1]) for l in want])
+            assert np.linalg.norm(g - w) <= 2e-4 * np.linalg.norm(w), q
+    snaps = sorted((tmp_path / "ssh" / "d2").glob("d2_4_*.nc"))
+    assert len(snaps) == 1
+    it_snap = [it for it in range(ns) if record_due(it, d["dt"], d["snap_t0"], 0.0)]
+    assert len(it_snap) == 1
+    _, vz = oracle_2d(d, it_snap[0] + 1)
+    z = nc_field(snaps[0])
+    assert z.shape == (d["nz"], d["nx"])
+    want = (1e3 * vz.T).astype(np.float32).astype(np.float64)
+    if storage == "f64":
+        assert np.array_equal(z, want)
+    else:
+        assert np.linalg.norm(z - want) <= 2e-4 * np.linalg.norm(want)
+
+
+# ------------------------------------------------------------------ 3D --
+def deck_3d(tmp, n=(20, 18, 16), np_=5, dx=10.0, dt_ms=0.8, time_s=0.08):
+    nx, ny, nz = n
+    rng = np.random.default_rng(12)
+    vz = np.where(np.arange(nz) < nz // 2, 1800.0, 3000.0)
+    vp = np.broadcast_to(vz, n) * (1 + 0.02 * rng.uniform(-1, 1, n))
+    rho = 310.0 * vp ** 0.25
+    with open(tmp / "model.dat", "w") as f:
+        f.write(f"{nx} {ny} {nz}\n{dx} {dx} {dx}\n0.0 0.0 0.0\nisotropic_acoustic_3d\n")
+        for i in range(nx):
+            for j in range(ny):
+                for k in range(nz):
+                    f.write(f"{vp[i, j, k]!r} {rho[i, j, k]!r}\n")
+    (tmp / "src.dat").write_text("1\n1\nBulk\n1.0\n30.0\n100.0 90.0 40.0\n")
+    recs = [("Vz", "Trace", "50.0 40.0 30.0", 0.0, dt_ms), ("Vx", "Trace", "150.0 90.0 100.0", 0.008, dt_ms),
+            ("P", "Trace", "100.0 120.0 70.0", 0.0, 2 * dt_ms), ("Vz", "Snapshot", None, 0.04, 0.0)]
+    with open(tmp / "rcv.dat", "w") as f:
+        f.write(f"{len(recs)}\n")
+        for comp, kind, xyz, t0, rdt in recs:
+            f.write(f"{comp}\n{kind}\n" + (f"{xyz}\n" if xyz else "") + f"{t0}\n{rdt}\n")
+    (tmp / "run.par").write_text(
+        "model.dat # model, file\nsrc.dat # source, file\nrcv.dat # output, records\n"
+        f"d3 # basename, name\n{time_s} # time, s\n{dt_ms} # dt, ms\n1 # abs, on\n{np_} # nl, layers\n"
+        "5000 # vmax, m/s\n2 # pml_order\n1e-5 # Rc\n1 # kappa_max\n1 # alpha_max\n")
+    n2 = [m + 2 * np_ for m in n]
+    cell = lambda i, j, k: ((i + np_) * n2[1] + j + np_) * n2[2] + k + np_
+    rec = [(cell(5, 4, 3), VZ, 0.0, dt_ms), (cell(15, 9, 10), VX, 0.008, dt_ms),
+           (cell(10, 12, 7), P, 0.0, 2 * dt_ms)]
+    vpp = np.pad(vp, np_, mode="edge")
+    rhop = np.pad(rho, np_, mode="edge")
+    return dict(n=n, np=np_, dx=dx, dt=dt_ms * 1e-3, vp=vpp, rho=rhop,
+                src_idx=np.array([cell(10, 9, 4)], dtype=np.uintp), rec=rec, snap_t0=0.04)
+
+
+def oracle_3d(d, nsteps):
+    nx, ny, nz = d["n"]
+    cfg = dict(nx=nx, ny=ny, nz=nz, np=d["np"], dx=d["dx"], dy=d["dx"], dz=d["dx"], dt=d["dt"],
+               f0=30.0, vmax=5000.0, pml_order=2.0, Rc=1e-5, kappa_max=1.0, alpha_max=1)
+    L, fct = O.source_functions([1.0], [30.0], d["dt"])
+    ridx = np.array([r[0] for r in d["rec"]], dtype=np.uintp)
+    rcomp = np.array([r[1] for r in d["rec"]], dtype=np.intc)
+    return O.a3d_run(cfg, d["vp"], d["rho"], d["src_idx"], L, fct, ridx, rcomp, nsteps)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_a_iso_3d_program(tmp_path, storage):
+    from seismod_amd.solver import nsteps as sm_nsteps, record_due
+    d = deck_3d(tmp_path)
+    run("a_iso_3d", tmp_path, storage)
+    ns = sm_nsteps(0.08, d["dt"])
+    tr, _ = oracle_3d(d, ns)
+    for q, (_, _, t0, rdt) in enumerate(d["rec"]):
+        got = read_trc(tmp_path / "trc" / "d3" / f"d3_{q + 1}.trc")
+        want = trace_lines(tr[:, q], d["dt"], t0, rdt * 1e-3)
+        assert len(got) == len(want) > 0
+        if storage == "f64":
+            assert got == want, q
+        else:
+            g = np.array([float(l.split()[1]) for l in got])
+            w = np.array([float(l.split()[1]) for l in want])
+            assert np.linalg.norm(g - w) <= 2e-4 * np.linalg.norm(w), q
+    snaps = sorted((tmp_path / "ssh" / "d3").glob("d3_4_*.nc"))
+    assert len(snaps) == 1
+    it_snap = [it for it in range(ns) if record_due(it, d["dt"], d["snap_t0"], 0.0)]
+    assert len(it_snap) == 1
+    _, f = oracle_3d(d, it_snap[0] + 1)
+    nx, ny, nz = d["n"]
+    np_ = d["np"]
+    n2 = [m + 2 * np_ for m in d["n"]]
+    nnn = n2[0] * n2[1] * n2[2]
+    vz = f[3 * nnn:4 * nnn].reshape(n2)
+    iy = int(90.0 / d["dx"])          # (y_src - y0)/dy without +np (a_iso_3d.c:726)
+    sl = vz[np_:np_ + nx, iy, np_:np_ + nz]
+    z = nc_field(snaps[0])
+    assert z.shape == (nz, nx)
+    want = (1e3 * sl.T).astype(np.float32).astype(np.float64)
+    if storage == "f64":
+        assert np.array_equal(z, want)
+    else:
+        assert np.linalg.norm(z - want) <= 2e-4 * np.linalg.norm(want) + 1e-30
