@@ -130,6 +130,10 @@ typedef struct {
 } sm_a3d_params;
 
 #define SM_A3D_NAIVE 1        /* two plain kernels per step (no fusion) */
+#define SM_A3D_SROUND 2       /* BF16/FP16 storage: stochastic rounding of the
+                                 stored values (deterministic pseudo-random
+                                 bits from element index and step) instead
+                                 of round-to-nearest-even */
 
 /* Build a solver from the padded model as read_model_a leaves it
  * (io_utils.c:1029-1153): vp and rho hold nx2*ny2*nz2 values with the

@@ -15,6 +15,7 @@ SM_STORE_F32, SM_STORE_F64, SM_STORE_BF16, SM_STORE_F16 = range(4)
 SM_VX, SM_VY, SM_VZ, SM_TXX, SM_TZZ, SM_TXZ, SM_P = 0, 1, 2, 5, 6, 7, 8
 SM_SX, SM_SY, SM_SZ, SM_SXY, SM_SXZ, SM_SYZ, SM_BULK, SM_SF = range(8)
 SM_A3D_NAIVE = 1
+SM_A3D_SROUND = 2
 
 _size_p = C.POINTER(C.c_size_t)
 _dbl_p = C.POINTER(C.c_double)

@@ -166,6 +166,7 @@ struct sm_a3d {
     size_t nsrc = 0, nrec = 0, max_steps = 0, rcv_step0 = 0;
     size_t src_plane_off = 0, src_inc_off = 0;   // byte offsets inside src
     int tj = 4, nchunks = 1;        // fused launch shape
+    int sr = 0;                     // stochastic rounding of 16-bit stores
     int nblocks = 0;                // persistent grid (0: one block per item)
     int cus = 0;                    // compute units of the device
     // storage scales of the 16-bit formats (P, v, velocity-side and
@@ -313,6 +314,15 @@ void axes(sm_a3d *s, CpmlAxis<T> ax[3])
     }
 }
 
+// per-step salt of the stochastic rounding (splitmix64 finaliser of the step)
+unsigned step_salt(size_t it)
+{
+    uint64_t z = (uint64_t)it + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (unsigned)(z ^ (z >> 31));
+}
+
 int64_t rcv_row(const sm_a3d *s, size_t it)
 {
     if (s->nrec == 0 || it < s->rcv_step0 || it - s->rcv_step0 >= s->max_steps) return -1;
@@ -450,6 +460,8 @@ int run_fused(sm_a3d *s, size_t nsteps)
         const size_t it = s->step + n;
         fused_bind<S>(s, a, s->cur);
         a.step = (int64_t)it;
+        a.salt = step_salt(it);
+        a.sr = s->sr;
         // receivers of the previous step, unless an earlier run() already did
         a.gather_row = (n > 0 && it > 0) ? rcv_row(s, it - 1) : -1;
         a.gather_blocks = a.gather_row >= 0 ? (int)((s->nrec + per_gather - 1) / per_gather) : 0;
@@ -608,6 +620,8 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
              "grid cell size must be equal in X and Z (a_iso_3d.c:298-301)");
     const size_t e = elt_size(p.storage);
     SM_CHECK(e != 0, SM_ERR_ARG, "sm_a3d_create: storage %d not supported", p.storage);
+    SM_CHECK((p.flags & ~(SM_A3D_NAIVE | SM_A3D_SROUND)) == 0, SM_ERR_ARG,
+             "sm_a3d_create: unknown flags %#x", p.flags);
     const size_t nx2 = p.nx + 2 * p.np, ny2 = p.ny + 2 * p.np, nz2 = p.nz + 2 * p.np;
     SM_CHECK(nx2 < 65536 && ny2 < 65536 && nz2 < 65536, SM_ERR_ARG, "sm_a3d_create: grid too large");
     const size_t nnn = nx2 * ny2 * nz2;
@@ -639,6 +653,7 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
     s->melt = p.storage == SM_STORE_F64 ? 8 : 4;
     s->telt = s->melt;
     s->fused = uses_fused(p);
+    s->sr = (p.flags & SM_A3D_SROUND) && (p.storage == SM_STORE_BF16 || p.storage == SM_STORE_F16);
     // the fused kernel addresses the four state fields of one buffer and
     // the two material fields with 32-bit byte offsets
     if (s->fused && (4 * nnn * e >= ((size_t)1 << 32) || 2 * nnn * 4 >= ((size_t)1 << 32))) {

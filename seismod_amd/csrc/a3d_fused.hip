@@ -50,10 +50,32 @@ __device__ __forceinline__ void st_at(T *base, unsigned idx, T v)
     *(T *)((char *)base + idx * (unsigned)sizeof(T)) = v;
 }
 
+// Stochastic rounding of 16-bit stores (optional): the discarded low bits
+// of the FP32 value are compared with a pseudo-random threshold, so a value
+// rounds up in magnitude with probability equal to its fraction of an ulp.
+// Increments smaller than half an ulp then survive on average instead of
+// being rounded away every step.  The random bits are a hash of the
+// element index and a per-step, per-buffer salt, so results are
+// reproducible and independent of the launch shape.
+constexpr unsigned SALT_MV = 0x68E31DA4u;   // velocity-side memory variables
+constexpr unsigned SALT_MP = 0xB5297A4Du;   // pressure-side memory variables
+
+__device__ __forceinline__ unsigned sr_bits(unsigned i, unsigned salt)
+{
+    unsigned h = (i ^ salt) * 0x9E3779B1u;
+    h ^= h >> 15;
+    h *= 0x85EBCA77u;
+    h ^= h >> 13;
+    return h;
+}
+
 template <>
 struct Store<float> {
     __device__ static inline float ld(const float *p, unsigned i, float) { return ld_at(p, i); }
-    __device__ static inline void st(float *p, unsigned i, float v, float) { st_at(p, i, v); }
+    __device__ static inline void st(float *p, unsigned i, float v, float, unsigned, int)
+    {
+        st_at(p, i, v);
+    }
 };
 
 template <>
@@ -62,9 +84,17 @@ struct Store<__hip_bfloat16> {
     {
         return __bfloat162float(ld_at(p, i)) * scale;
     }
-    __device__ static inline void st(__hip_bfloat16 *p, unsigned i, float v, float inv)
+    __device__ static inline void st(__hip_bfloat16 *p, unsigned i, float v, float inv,
+                                     unsigned salt, int sr)
     {
-        st_at(p, i, __float2bfloat16(v * inv));
+        const float x = v * inv;
+        if (sr) {
+            // add 16 random bits below the BF16 mantissa, then truncate
+            const unsigned b = __float_as_uint(x) + (sr_bits(i, salt) & 0xFFFFu);
+            st_at(p, i, __builtin_bit_cast(__hip_bfloat16, (unsigned short)(b >> 16)));
+        } else {
+            st_at(p, i, __float2bfloat16(x));
+        }
     }
 };
 
@@ -74,9 +104,21 @@ struct Store<__half> {
     {
         return __half2float(ld_at(p, i)) * scale;
     }
-    __device__ static inline void st(__half *p, unsigned i, float v, float inv)
+    __device__ static inline void st(__half *p, unsigned i, float v, float inv, unsigned salt,
+                                     int sr)
     {
-        st_at(p, i, __float2half(v * inv));
+        float x = v * inv;
+        if (sr) {
+            // normal FP16 range [2^-14, 65504): 13 random bits below the FP16
+            // mantissa, then truncate to a value FP16 holds exactly
+            unsigned b = __float_as_uint(x);
+            const unsigned mag = b & 0x7FFFFFFFu;
+            if (mag >= 0x38800000u && mag < 0x477FE000u) {
+                b = (b + (sr_bits(i, salt) & 0x1FFFu)) & ~0x1FFFu;
+                x = __uint_as_float(b);
+            }
+        }
+        st_at(p, i, __float2half(x));
     }
 };
 
@@ -215,13 +257,18 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
     const float vzn = vel_update(cur.vz, P_c, Pk, r_c, rk, a.idz, a.two_dt, L.szv >= 0,
                                  L.zbh, L.zah, L.zikh, mz);
     if (L.own) {
-        Store<S>::st(a.out, F_VX * a.nnn + o, vxn, a.isV);
-        Store<S>::st(a.out, F_VY * a.nnn + o, vyn, a.isV);
-        Store<S>::st(a.out, F_VZ * a.nnn + o, vzn, a.isV);
-        if (x.sxv >= 0) Store<S>::st(a.mem_out, (unsigned)x.sxv * plane + L.o_jk, mx, a.isMv);
+        Store<S>::st(a.out, F_VX * a.nnn + o, vxn, a.isV, a.salt, a.sr);
+        Store<S>::st(a.out, F_VY * a.nnn + o, vyn, a.isV, a.salt, a.sr);
+        Store<S>::st(a.out, F_VZ * a.nnn + o, vzn, a.isV, a.salt, a.sr);
+        if (x.sxv >= 0)
+            Store<S>::st(a.mem_out, (unsigned)x.sxv * plane + L.o_jk, mx, a.isMv, a.salt ^ SALT_MV,
+                         a.sr);
         if (L.syv >= 0)
-            Store<S>::st(a.mem_out, a.my + ((unsigned)i * np2 + L.syv) * a.nz2 + L.kc, my, a.isMv);
-        if (L.szv >= 0) Store<S>::st(a.mem_out, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_v, mz, a.isMv);
+            Store<S>::st(a.mem_out, a.my + ((unsigned)i * np2 + L.syv) * a.nz2 + L.kc, my, a.isMv,
+                         a.salt ^ SALT_MV, a.sr);
+        if (L.szv >= 0)
+            Store<S>::st(a.mem_out, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_v, mz, a.isMv,
+                         a.salt ^ SALT_MV, a.sr);
     }
     // vy of the row below the tile (owned by the tile below; not stored)
     float vy_halo = 0.f;
@@ -246,17 +293,20 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
             float dvdz = (vzn - vz_m) * a.idz;
             if (x.sxp >= 0) {
                 const float m = x.b * cur.mvx + x.a * dvdx;
-                Store<S>::st(a.memp, (unsigned)x.sxp * plane + L.o_jk, m, a.isMp);
+                Store<S>::st(a.memp, (unsigned)x.sxp * plane + L.o_jk, m, a.isMp, a.salt ^ SALT_MP,
+                             a.sr);
                 dvdx = dvdx * x.ik + m;
             }
             if (L.syp >= 0) {
                 const float m = L.yb * cur.mvy + L.ya * dvdy;
-                Store<S>::st(a.memp, a.my + ((unsigned)i * np2 + L.syp) * a.nz2 + L.kc, m, a.isMp);
+                Store<S>::st(a.memp, a.my + ((unsigned)i * np2 + L.syp) * a.nz2 + L.kc, m, a.isMp,
+                             a.salt ^ SALT_MP, a.sr);
                 dvdy = dvdy * L.yik + m;
             }
             if (L.szp >= 0) {
                 const float m = L.zb * cur.mvz + L.za * dvdz;
-                Store<S>::st(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, m, a.isMp);
+                Store<S>::st(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, m, a.isMp,
+                             a.salt ^ SALT_MP, a.sr);
                 dvdz = dvdz * L.zik + m;
             }
             Pn = P_c - cur.K * (dvdx + dvdy + dvdz);
@@ -265,7 +315,7 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
         for (int q = x.q0; q < x.q1; ++q)
             if ((unsigned)a.src[q] == o && a.step < a.src[2 * a.nsrc + q])
                 Pn += a.src_inc[a.src[a.nsrc + q] + a.step];
-        Store<S>::st(a.out, o, Pn, a.isP);
+        Store<S>::st(a.out, o, Pn, a.isP, a.salt, a.sr);
     }
     vx_prev = vxn;
     P_c = cur.Pn;

@@ -58,6 +58,8 @@ struct FusedArgs {
     const float *src_inc;
     int nsrc;
     int64_t step;
+    unsigned salt;   // per-step salt of the stochastic rounding of 16-bit stores
+    int sr;          // 1: stochastic rounding of 16-bit stores
     // receivers of the previous step (used by the gather blocks only)
     int nrec;
     const int64_t *ridx;

@@ -141,12 +141,14 @@ class A3dConfig:
     alpha_max: int = 1
     storage: str = "f32"
     naive: bool = False
+    sround: bool = False      # BF16/FP16: stochastic rounding of stored values
 
     def params(self) -> L.A3dParams:
         return L.A3dParams(self.nx, self.ny, self.nz, self.np, self.dx, self.dy, self.dz,
                            self.dt, self.vmax, self.pml_order, self.Rc, self.kappa_max,
                            self.alpha_max, self.f0, _STORAGE[self.storage],
-                           L.SM_A3D_NAIVE if self.naive else 0)
+                           (L.SM_A3D_NAIVE if self.naive else 0)
+                           | (L.SM_A3D_SROUND if self.sround else 0))
 
     @property
     def shape(self) -> tuple[int, int, int]:

@@ -65,7 +65,7 @@ def deck_2d(tmp, nx=40, nz=30, npml=5, h=5.0, dt_ms=0.4, time_s=0.06):
         f.write(f"{nx} {nz}\n{h} {h}\n0.0 0.0\nisotropic_elastic\n")
         for i in range(nx):
             for j in range(nz):
-                f.write(f"{vp[i, j]!r} {vs[i, j]!r} {rho[i, j]!r}\n")
+                f.write(f"{float(vp[i, j])!r} {float(vs[i, j])!r} {float(rho[i, j])!r}\n")
     (tmp / "src.dat").write_text("1\n1\nBulk\n1.0\n40.0\n100.0 50.0\n")
     recs = [("Vx", "Trace", "50.0 25.0", 0.0, dt_ms), ("Vz", "Trace", "150.0 75.0", 0.01, 2 * dt_ms),
             ("Vz", "Trace", "195.0 145.0", 0.0, dt_ms), ("Vz", "Snapshot", None, 0.03, 0.0)]
@@ -139,7 +139,7 @@ def deck_3d(tmp, n=(20, 18, 16), np_=5, dx=10.0, dt_ms=0.8, time_s=0.08):
         for i in range(nx):
             for j in range(ny):
                 for k in range(nz):
-                    f.write(f"{vp[i, j, k]!r} {rho[i, j, k]!r}\n")
+                    f.write(f"{float(vp[i, j, k])!r} {float(rho[i, j, k])!r}\n")
     (tmp / "src.dat").write_text("1\n1\nBulk\n1.0\n30.0\n100.0 90.0 40.0\n")
     recs = [("Vz", "Trace", "50.0 40.0 30.0", 0.0, dt_ms), ("Vx", "Trace", "150.0 90.0 100.0", 0.008, dt_ms),
             ("P", "Trace", "100.0 120.0 70.0", 0.0, 2 * dt_ms), ("Vz", "Snapshot", None, 0.04, 0.0)]

@@ -26,6 +26,8 @@ TOL = {
     "f32": dict(traces=2e-4, field=2e-4),
     "f16": dict(traces=5e-3, field=1e-2),
     "bf16": dict(traces=5e-2, field=1e-1),
+    "f16-sr": dict(traces=5e-3, field=1e-2),
+    "bf16-sr": dict(traces=5e-2, field=1e-1),
 }
 MID = 200
 
@@ -62,12 +64,13 @@ def deck(n=48, np_=10, nsteps=400, seed=3, amp=1.0):
 
 
 def gpu_run(ctx, d, storage):
+    """storage: f32, bf16, f16, or bf16-sr / f16-sr (stochastic rounding)"""
     from seismod_amd.solver import A3dConfig, Acoustic3D, source_functions
     c = d["cfg"]
     cfg = A3dConfig(nx=c["nx"], ny=c["ny"], nz=c["nz"], np=c["np"], dx=c["dx"], dy=c["dy"],
                     dz=c["dz"], dt=c["dt"], f0=c["f0"], vmax=c["vmax"], pml_order=c["pml_order"],
                     Rc=c["Rc"], kappa_max=c["kappa_max"], alpha_max=c["alpha_max"],
-                    storage=storage)
+                    storage=storage.split("-")[0], sround=storage.endswith("-sr"))
     s = Acoustic3D(ctx, cfg, d["vp"], d["rho"])
     lengths, fct = source_functions(d["A"], d["f"], c["dt"])
     s.set_sources(d["src_idx"], lengths, fct)
@@ -96,7 +99,7 @@ def ref():
     return d, oracle_run(d)
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16", "f16-sr", "bf16-sr"])
 def test_storage_error_vs_fp64_oracle(ctx, ref, storage):
     d, (tr_ref, p_ref) = ref
     tr, p = gpu_run(ctx, d, storage)
@@ -122,3 +125,27 @@ def test_f16_scaling_follows_source_amplitude(ctx, amp):
     e = rel_l2(tr16, tr32)
     print(f"\namp {amp:g}: f16 vs f32 traces rel-L2 {e:.3e}")
     assert e <= TOL["f16"]["traces"], e
+
+
+@pytest.mark.parametrize("storage", ["bf16-sr", "f16-sr"])
+def test_stochastic_rounding_is_reproducible_and_launch_invariant(ctx, storage):
+    """The rounding bits depend on element index and step only."""
+    from seismod_amd.solver import A3dConfig, Acoustic3D, source_functions
+    d = deck(n=32, np_=8, nsteps=120)
+    c = d["cfg"]
+    outs = []
+    for launch in ((4, 2), (2, 5), (8, 1)):
+        cfg = A3dConfig(nx=c["nx"], ny=c["ny"], nz=c["nz"], np=c["np"], dx=c["dx"], dy=c["dy"],
+                        dz=c["dz"], dt=c["dt"], f0=c["f0"], storage=storage.split("-")[0],
+                        sround=True)
+        s = Acoustic3D(ctx, cfg, d["vp"], d["rho"])
+        s.set_launch(*launch)
+        L, fct = source_functions(d["A"], d["f"], c["dt"])
+        s.set_sources(d["src_idx"], L, fct)
+        s.set_receivers(d["rec_idx"], d["rec_comp"], d["nsteps"])
+        s.run(50)
+        s.run(70)
+        outs.append((s.traces(0, d["nsteps"]), s.field(P)))
+        s.close()
+    for tr, p in outs[1:]:
+        assert np.array_equal(tr, outs[0][0]) and np.array_equal(p, outs[0][1])
