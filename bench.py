@@ -42,7 +42,9 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="interior nodes per axis")
     ap.add_argument("--nsteps", type=int, default=None, help="time steps per shot")
     ap.add_argument("--storage", default="f32", choices=sorted(STATE_BYTES))
-    ap.add_argument("--launch", default=None, help="rows_per_block,x_chunks")
+    ap.add_argument("--launch", default="auto",
+                    help="rows_per_block,x_chunks, or 'auto' (the autotuner picks it during "
+                         "set-up, before the timed region)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=10)
     return ap.parse_args()
@@ -98,11 +100,14 @@ def main():
     cfg = A3dConfig(nx=shot.nx, ny=shot.ny, nz=shot.nz, np=shot.np, dx=shot.dx, dy=shot.dx,
                     dz=shot.dx, dt=shot.dt, f0=shot.f0, storage=args.storage)
     sim = Acoustic3D(ctx, cfg, shot.vp, shot.rho)
-    if args.launch:
-        tj, ch = (int(x) for x in args.launch.split(","))
-        sim.set_launch(tj, ch)
     lengths, fct = source_functions([1.0], [shot.f0], shot.dt)
     sim.set_sources(shot.src_idx, lengths, fct)
+    if args.launch == "auto":
+        from seismod_amd import autotune
+        autotune.tune_a3d(sim, ctx)       # times candidates, then resets the solver
+    elif args.launch:
+        tj, ch = (int(x) for x in args.launch.split(","))
+        sim.set_launch(tj, ch)
 
     sim.set_receivers(shot.rec_idx, shot.rec_comp, nsteps)
 
@@ -157,7 +162,8 @@ def main():
             "padded_grid": [nx2, ny2, nz2],
             "time_steps_per_shot": nsteps,
             "dt_s": shot.dt,
-            "launch": {"rows_per_block": tj, "x_chunks": ch},
+            "launch": {"rows_per_block": tj, "x_chunks": ch,
+                       "chosen_by": "autotuner" if args.launch == "auto" else "command line"},
             "parallelism": "single GPU",
         },
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),

@@ -16,6 +16,8 @@ SM_VX, SM_VY, SM_VZ, SM_TXX, SM_TZZ, SM_TXZ, SM_P = 0, 1, 2, 5, 6, 7, 8
 SM_SX, SM_SY, SM_SZ, SM_SXY, SM_SXZ, SM_SYZ, SM_BULK, SM_SF = range(8)
 SM_A3D_NAIVE = 1
 SM_A3D_SROUND = 2
+SM_A3D_CELL = 0
+SM_A3D_PAIR = 1
 
 _size_p = C.POINTER(C.c_size_t)
 _dbl_p = C.POINTER(C.c_double)
@@ -67,6 +69,8 @@ _SIGS = {
     "sm_a3d_get_slice_xz": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, _dbl_p]),
     "sm_a3d_reset": (C.c_int, [C.c_void_p]),
     "sm_a3d_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "sm_a3d_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
+    "sm_a3d_get_variant": (C.c_int, [C.c_void_p, _int_p]),
     "sm_a3d_get_launch": (C.c_int, [C.c_void_p, _int_p, _int_p]),
     "sm_a3d_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sm_a3d_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),

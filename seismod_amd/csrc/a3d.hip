@@ -166,6 +166,7 @@ struct sm_a3d {
     size_t nsrc = 0, nrec = 0, max_steps = 0, rcv_step0 = 0;
     size_t src_plane_off = 0, src_inc_off = 0;   // byte offsets inside src
     int tj = 4, nchunks = 1;        // fused launch shape
+    int variant = SM_A3D_CELL;      // fused kernel variant
     int sr = 0;                     // stochastic rounding of 16-bit stores
     int nblocks = 0;                // persistent grid (0: one block per item)
     int cus = 0;                    // compute units of the device
@@ -407,8 +408,14 @@ FusedArgs<S> fused_args(sm_a3d *s)
 {
     FusedArgs<S> a{};
     a.nx2 = s->nx2; a.ny2 = s->ny2; a.nz2 = s->nz2; a.np = (int)s->p.np;
-    a.ntk = (s->nz2 + 61) / 62;      // 62 owned cells per wave along z
-    a.ntiles = a.ntk * ((s->ny2 + s->tj - 1) / s->tj);
+    if (s->variant == SM_A3D_PAIR) {
+        // z tiles of 62 owned cells starting at even cells, two rows per wave
+        a.ntk = (s->nz2 + 60) / 62;
+        a.ntiles = a.ntk * ((s->ny2 + 2 * s->tj - 1) / (2 * s->tj));
+    } else {
+        a.ntk = (s->nz2 + 62) / 63;      // 63 owned cells per wave along z
+        a.ntiles = a.ntk * ((s->ny2 + s->tj - 1) / s->tj);
+    }
     a.chunk_len = (s->nx2 + s->nchunks - 1) / s->nchunks;
     a.nreal = a.ntiles * s->nchunks;
     a.nitems = (a.nreal + 7) / 8 * 8;
@@ -466,7 +473,8 @@ int run_fused(sm_a3d *s, size_t nsteps)
         a.gather_row = (n > 0 && it > 0) ? rcv_row(s, it - 1) : -1;
         a.gather_blocks = a.gather_row >= 0 ? (int)((s->nrec + per_gather - 1) / per_gather) : 0;
         if ((rc = tm.mark(st, 2 * n)) != SM_OK) return rc;
-        if ((rc = fused_launch<S>(a, s->tj, st)) != SM_OK)
+        if ((rc = s->variant == SM_A3D_PAIR ? pair_launch<S>(a, s->tj, st)
+                                            : fused_launch<S>(a, s->tj, st)) != SM_OK)
             return fail(rc, "fused launch failed (tj=%d): %s", s->tj,
                         hipGetErrorString(hipGetLastError()));
         if ((rc = tm.mark(st, 2 * n + 1)) != SM_OK) return rc;
@@ -560,10 +568,20 @@ bool uses_fused(const sm_a3d_params &p)
 int resident_blocks(sm_a3d *s, int tj)
 {
     int per_cu = 0, rc = SM_OK;
+    const bool pair = s->variant == SM_A3D_PAIR;
     switch (s->p.storage) {
-    case SM_STORE_F32: rc = fused_max_blocks_per_cu<float>(tj, &per_cu); break;
-    case SM_STORE_BF16: rc = fused_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu); break;
-    default: rc = fused_max_blocks_per_cu<__half>(tj, &per_cu); break;
+    case SM_STORE_F32:
+        rc = pair ? pair_max_blocks_per_cu<float>(tj, &per_cu)
+                  : fused_max_blocks_per_cu<float>(tj, &per_cu);
+        break;
+    case SM_STORE_BF16:
+        rc = pair ? pair_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu)
+                  : fused_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu);
+        break;
+    default:
+        rc = pair ? pair_max_blocks_per_cu<__half>(tj, &per_cu)
+                  : fused_max_blocks_per_cu<__half>(tj, &per_cu);
+        break;
     }
     if (rc != SM_OK || per_cu < 1) per_cu = 1;
     return per_cu * std::max(s->cus, 1);
@@ -582,7 +600,9 @@ int valid_chunks(int nx2, int want)
 void default_launch(sm_a3d *s)
 {
     s->tj = 4;
-    const int ntiles = ((s->nz2 + 61) / 62) * ((s->ny2 + s->tj - 1) / s->tj);
+    const int ntiles = s->variant == SM_A3D_PAIR
+                           ? ((s->nz2 + 60) / 62) * ((s->ny2 + 2 * s->tj - 1) / (2 * s->tj))
+                           : ((s->nz2 + 62) / 63) * ((s->ny2 + s->tj - 1) / s->tj);
     const int slots = resident_blocks(s, s->tj);
     int nch = (3 * slots + ntiles - 1) / ntiles;
     nch = std::min(nch, std::max(1, s->nx2 / 24));
@@ -879,6 +899,27 @@ int sm_a3d_set_launch(sm_a3d *s, int rows_per_block, int x_chunks)
     s->tj = rows_per_block;
     s->nchunks = x_chunks;
     s->nblocks = resident_blocks(s, rows_per_block);
+    return SM_OK;
+}
+
+int sm_a3d_set_variant(sm_a3d *s, int variant)
+{
+    SM_CHECK(s, SM_ERR_ARG, "sm_a3d_set_variant: NULL solver");
+    SM_CHECK(s->fused, SM_ERR_STATE, "sm_a3d_set_variant: only the fused kernel has variants");
+    SM_CHECK(variant == SM_A3D_CELL || variant == SM_A3D_PAIR, SM_ERR_ARG,
+             "sm_a3d_set_variant: unknown variant %d", variant);
+    SM_CHECK(variant != SM_A3D_PAIR || (s->nz2 % 2 == 0 && s->nz2 >= 4), SM_ERR_ARG,
+             "sm_a3d_set_variant: the pair kernel needs an even padded z extent (nz2 = %d)", s->nz2);
+    if (variant == s->variant) return SM_OK;
+    s->variant = variant;
+    default_launch(s);
+    return SM_OK;
+}
+
+int sm_a3d_get_variant(const sm_a3d *s, int *variant)
+{
+    SM_CHECK(s && variant, SM_ERR_ARG, "sm_a3d_get_variant: NULL argument");
+    *variant = s->fused ? s->variant : -1;
     return SM_OK;
 }
 

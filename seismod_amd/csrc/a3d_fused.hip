@@ -29,122 +29,17 @@
 
 #include "a3d_common.hpp"
 #include "a3d_fused.hpp"
+#include "a3d_store.hpp"
 #include "seismod_hip.h"
 
 namespace sm {
 
-template <typename S>
-struct Store;
-
-// Element access by a 32-bit element index turned into a 32-bit byte
-// offset from a uniform base, so loads use the SGPR-base + VGPR-offset form
-// (every device array here is well below 4 GiB).
-template <typename T>
-__device__ __forceinline__ T ld_at(const T *base, unsigned idx)
-{
-    return *(const T *)((const char *)base + idx * (unsigned)sizeof(T));
-}
-template <typename T>
-__device__ __forceinline__ void st_at(T *base, unsigned idx, T v)
-{
-    *(T *)((char *)base + idx * (unsigned)sizeof(T)) = v;
-}
-
-// Stochastic rounding of 16-bit stores (optional): the discarded low bits
-// of the FP32 value are compared with a pseudo-random threshold, so a value
-// rounds up in magnitude with probability equal to its fraction of an ulp.
-// Increments smaller than half an ulp then survive on average instead of
-// being rounded away every step.  The random bits are a hash of the
-// element index and a per-step, per-buffer salt, so results are
-// reproducible and independent of the launch shape.
-constexpr unsigned SALT_MV = 0x68E31DA4u;   // velocity-side memory variables
-constexpr unsigned SALT_MP = 0xB5297A4Du;   // pressure-side memory variables
-
-__device__ __forceinline__ unsigned sr_bits(unsigned i, unsigned salt)
-{
-    unsigned h = (i ^ salt) * 0x9E3779B1u;
-    h ^= h >> 15;
-    h *= 0x85EBCA77u;
-    h ^= h >> 13;
-    return h;
-}
-
-template <>
-struct Store<float> {
-    __device__ static inline float ld(const float *p, unsigned i, float) { return ld_at(p, i); }
-    __device__ static inline void st(float *p, unsigned i, float v, float, unsigned, int)
-    {
-        st_at(p, i, v);
-    }
-};
-
-template <>
-struct Store<__hip_bfloat16> {
-    __device__ static inline float ld(const __hip_bfloat16 *p, unsigned i, float scale)
-    {
-        return __bfloat162float(ld_at(p, i)) * scale;
-    }
-    __device__ static inline void st(__hip_bfloat16 *p, unsigned i, float v, float inv,
-                                     unsigned salt, int sr)
-    {
-        const float x = v * inv;
-        if (sr) {
-            // add 16 random bits below the BF16 mantissa, then truncate
-            const unsigned b = __float_as_uint(x) + (sr_bits(i, salt) & 0xFFFFu);
-            st_at(p, i, __builtin_bit_cast(__hip_bfloat16, (unsigned short)(b >> 16)));
-        } else {
-            st_at(p, i, __float2bfloat16(x));
-        }
-    }
-};
-
-template <>
-struct Store<__half> {
-    __device__ static inline float ld(const __half *p, unsigned i, float scale)
-    {
-        return __half2float(ld_at(p, i)) * scale;
-    }
-    __device__ static inline void st(__half *p, unsigned i, float v, float inv, unsigned salt,
-                                     int sr)
-    {
-        float x = v * inv;
-        if (sr) {
-            // normal FP16 range [2^-14, 65504): 13 random bits below the FP16
-            // mantissa, then truncate to a value FP16 holds exactly
-            unsigned b = __float_as_uint(x);
-            const unsigned mag = b & 0x7FFFFFFFu;
-            if (mag >= 0x38800000u && mag < 0x477FE000u) {
-                b = (b + (sr_bits(i, salt) & 0x1FFFu)) & ~0x1FFFu;
-                x = __uint_as_float(b);
-            }
-        }
-        st_at(p, i, __float2half(x));
-    }
-};
-
-__device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-
-// one velocity component at one cell: v - 2dt/(rho0+rho1) * dP, with the
-// CPML memory variable m (updated in place) where slab is true
-__device__ __forceinline__ float vel_update(float v_old, float p0, float p1, float r0, float r1,
-                                            float ih, float two_dt, bool slab, float bh, float ah,
-                                            float ikh, float &m)
-{
-    float dP = (p1 - p0) * ih;
-    if (slab) {
-        m = bh * m + ah * dP;
-        dP = dP * ikh + m;
-    }
-    return v_old - two_dt * fast_rcp(r0 + r1) * dP;
-}
-
-// what one lane reads for plane i, loaded one plane ahead.  The (j+1) and
-// (k+1) neighbours of P and rho come from other waves (LDS) and lanes
-// (shuffles); only the tile's top wave loads the row above the tile.
+// what one lane reads for plane i, loaded one plane ahead
 struct Col {
     float vx, vy, vz, K;    // old velocities and K*dt at (i, j, k)
     float Pn, rn;           // P, rho at (i+1, j, k)   (clamped to the last plane)
-    float Pj, rj;           // top wave only: P, rho at (i, j+1, k) (clamped to the last row)
+    float Pj, rj;           // P, rho at (i, j+1, k)   (clamped to the last row)
+    float Pk, rk;           // P, rho at (i, j, k+1)   (clamped to the last column)
     float Pjm, rjm, vyjm;   // wave 0 only: (i, j-1, k)
     float mPx, mPy, mPz;    // velocity memory variables (slab cells only)
     float mPym;             // wave 0 only: at the halo row
@@ -159,7 +54,7 @@ struct Col {
 
 // constants of one lane for one work item
 struct Lane {
-    unsigned o_jk, o_jp, o_jm;         // in-plane offsets of the cell and its y neighbours
+    unsigned o_jk, o_jp, o_kp, o_jm;   // in-plane offsets of the cell and its neighbours
     int j, kk, kc;                     // j is wave-uniform
     bool own;                          // this lane writes its cell
     int syv, syp, syh;                 // y slab coordinates (wave-uniform)
@@ -170,7 +65,7 @@ struct Lane {
     float zbh, zah, zikh, zb, za, zik;
 };
 
-template <typename S, int TJ>
+template <typename S>
 __device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, int w, int i,
                                          unsigned plane, int sxv, int sxp, Col &c)
 {
@@ -183,10 +78,10 @@ __device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, i
     c.K = ld_at(a.mat, a.nnn + o + L.o_jk);
     c.Pn = Store<S>::ld(a.in, on + L.o_jk, a.sP);
     c.rn = ld_at(a.mat, on + L.o_jk);
-    if (w == TJ - 1) {
-        c.Pj = Store<S>::ld(a.in, o + L.o_jp, a.sP);
-        c.rj = ld_at(a.mat, o + L.o_jp);
-    }
+    c.Pj = Store<S>::ld(a.in, o + L.o_jp, a.sP);
+    c.rj = ld_at(a.mat, o + L.o_jp);
+    c.Pk = Store<S>::ld(a.in, o + L.o_kp, a.sP);
+    c.rk = ld_at(a.mat, o + L.o_kp);
     if (w == 0) {
         c.Pjm = Store<S>::ld(a.in, o + L.o_jm, a.sP);
         c.rjm = ld_at(a.mat, o + L.o_jm);
@@ -204,25 +99,16 @@ __device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, i
     if (L.szp >= 0) c.mvz = Store<S>::ld(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, a.sMp);
 }
 
-// LDS of one block: new vy of each row (for the row above), and old P and
-// rho of each row one plane ahead (for the row below); slots by plane parity
-template <int TJ>
-struct Shared {
-    float vy[2][TJ][64];
-    float P[2][TJ][64];
-    float r[2][TJ][64];
-};
-
 // one plane: prefetch plane i+1 into nxt, update plane i from cur
 template <typename S, int TJ>
 __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L, int w, int i,
                                            int i0, int i1, unsigned plane, Col &cur, Col &nxt,
                                            float &P_c, float &r_c, float &vx_prev,
-                                           Shared<TJ> &sh)
+                                           float (*s_vy)[TJ][64])
 {
     if (i + 1 < i1) {
         const XPlane xn = a.xpl[i + 1];
-        load_col<S, TJ>(a, L, w, i + 1, plane, xn.sxv, xn.sxp, nxt);
+        load_col<S>(a, L, w, i + 1, plane, xn.sxv, xn.sxp, nxt);
     }
     const XPlane x = a.xpl[i];
     const unsigned o = (unsigned)i * plane + L.o_jk;
@@ -233,36 +119,25 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
     float mx = cur.mPx;
     const float vxn = vel_update(cur.vx, P_c, cur.Pn, r_c, cur.rn, a.idx, a.two_dt, x.sxv >= 0,
                                  x.bh, x.ah, x.ikh, mx);
-    const int lane = threadIdx.x;
     if (i < i0) {   // chunk start: only vx of the plane below is needed
         vx_prev = vxn;
-        sh.P[(i + 1) & 1][w][lane] = cur.Pn;
-        sh.r[(i + 1) & 1][w][lane] = cur.rn;
-        __syncthreads();
         P_c = cur.Pn;
         r_c = cur.rn;
         return;
     }
-    // neighbours: (j+1) from the wave above (written one plane earlier) or,
-    // for the top wave, loaded; (k+1) from the next lane (lane 63 is a halo
-    // lane whose own vz is never used)
-    const float Pj = w == TJ - 1 ? cur.Pj : sh.P[i & 1][w + 1][lane];
-    const float rj = w == TJ - 1 ? cur.rj : sh.r[i & 1][w + 1][lane];
-    const float Pk = __shfl_down(P_c, 1);
-    const float rk = __shfl_down(r_c, 1);
     // vy, vz at (i, j, k), a_iso_3d.c:621-651
     float my = cur.mPy, mz = cur.mPz;
-    const float vyn = vel_update(cur.vy, P_c, Pj, r_c, rj, a.idy, a.two_dt, L.syv >= 0,
+    const float vyn = vel_update(cur.vy, P_c, cur.Pj, r_c, cur.rj, a.idy, a.two_dt, L.syv >= 0,
                                  L.ybh, L.yah, L.yikh, my);
-    const float vzn = vel_update(cur.vz, P_c, Pk, r_c, rk, a.idz, a.two_dt, L.szv >= 0,
+    const float vzn = vel_update(cur.vz, P_c, cur.Pk, r_c, cur.rk, a.idz, a.two_dt, L.szv >= 0,
                                  L.zbh, L.zah, L.zikh, mz);
     if (L.own) {
         Store<S>::st(a.out, F_VX * a.nnn + o, vxn, a.isV, a.salt, a.sr);
         Store<S>::st(a.out, F_VY * a.nnn + o, vyn, a.isV, a.salt, a.sr);
         Store<S>::st(a.out, F_VZ * a.nnn + o, vzn, a.isV, a.salt, a.sr);
         if (x.sxv >= 0)
-            Store<S>::st(a.mem_out, (unsigned)x.sxv * plane + L.o_jk, mx, a.isMv, a.salt ^ SALT_MV,
-                         a.sr);
+            Store<S>::st(a.mem_out, (unsigned)x.sxv * plane + L.o_jk, mx, a.isMv,
+                         a.salt ^ SALT_MV, a.sr);
         if (L.syv >= 0)
             Store<S>::st(a.mem_out, a.my + ((unsigned)i * np2 + L.syv) * a.nz2 + L.kc, my, a.isMv,
                          a.salt ^ SALT_MV, a.sr);
@@ -277,11 +152,9 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
         vy_halo = vel_update(cur.vyjm, cur.Pjm, P_c, cur.rjm, r_c, a.idy, a.two_dt, L.syh >= 0,
                              L.hbh, L.hah, L.hikh, mh);
     }
-    sh.vy[i & 1][w][lane] = vyn;
-    sh.P[(i + 1) & 1][w][lane] = cur.Pn;
-    sh.r[(i + 1) & 1][w][lane] = cur.rn;
+    s_vy[i & 1][w][threadIdx.x] = vyn;
     __syncthreads();
-    const float vy_m = w > 0 ? sh.vy[i & 1][w - 1][lane] : vy_halo;
+    const float vy_m = w > 0 ? s_vy[i & 1][w - 1][threadIdx.x] : vy_halo;
     const float vz_m = __shfl_up(vzn, 1);   // lane 0 is the left halo lane
 
     // pressure at (i, j, k), a_iso_3d.c:659-683
@@ -293,8 +166,8 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
             float dvdz = (vzn - vz_m) * a.idz;
             if (x.sxp >= 0) {
                 const float m = x.b * cur.mvx + x.a * dvdx;
-                Store<S>::st(a.memp, (unsigned)x.sxp * plane + L.o_jk, m, a.isMp, a.salt ^ SALT_MP,
-                             a.sr);
+                Store<S>::st(a.memp, (unsigned)x.sxp * plane + L.o_jk, m, a.isMp,
+                             a.salt ^ SALT_MP, a.sr);
                 dvdx = dvdx * x.ik + m;
             }
             if (L.syp >= 0) {
@@ -335,7 +208,7 @@ template <typename S, int TJ>
 __global__ void __launch_bounds__(64 * TJ)
 a3d_fused_step(FusedArgs<S> a)
 {
-    __shared__ Shared<TJ> sh;
+    __shared__ float s_vy[2][TJ][64];
 
     // ---------------- receiver gather blocks (previous step) -------------
     if ((int)blockIdx.x >= a.nblocks) {
@@ -360,21 +233,19 @@ a3d_fused_step(FusedArgs<S> a)
         const int tile = logical - chunk * a.ntiles;
         const int tj = tile / a.ntk;
         const int tk = tile - tj * a.ntk;
-        // 62 owned cells per wave along z (lanes 1..62); lane 0 sits on the
-        // last cell of the tile to the left and supplies vz at k0-1, lane 63
-        // on the first cell of the tile to the right and supplies P, rho at
-        // k0+62.  Cells past the grid edge are clamped to the last cell, so
-        // a clamped neighbour equals the cell and its difference is 0.
+        // 63 owned cells per wave along z; lane 0 sits on the last cell of
+        // the tile to the left and only supplies vz at k0-1
         Lane L;
-        L.kk = tk * 62 - 1 + lane;
+        L.kk = tk * 63 - 1 + lane;
         L.j = tj * TJ + w;
         const bool jv = L.j < a.ny2, kv = L.kk >= 0 && L.kk < a.nz2;
-        L.own = jv && kv && lane >= 1 && lane <= 62;
+        L.own = jv && kv && lane >= 1;
         const int jc = min(L.j, a.ny2 - 1);
         L.kc = min(max(L.kk, 0), a.nz2 - 1);
-        const int jp = min(jc + 1, a.ny2 - 1), jm = max(jc - 1, 0);
+        const int jp = min(jc + 1, a.ny2 - 1), kp = min(L.kc + 1, a.nz2 - 1), jm = max(jc - 1, 0);
         L.o_jk = (unsigned)jc * a.nz2 + L.kc;
         L.o_jp = (unsigned)jp * a.nz2 + L.kc;
+        L.o_kp = (unsigned)jc * a.nz2 + kp;
         L.o_jm = (unsigned)jm * a.nz2 + L.kc;
         L.syv = jv && L.j < a.ny2 - 1 ? slab_vel(L.j, a.ny2, a.np) : -1;
         L.syp = jv ? slab_pre(L.j, a.ny2, a.np) : -1;
@@ -406,22 +277,17 @@ a3d_fused_step(FusedArgs<S> a)
         float r_c = ld_at(a.mat, (unsigned)ist * plane + L.o_jk);
         float vx_prev = 0.f;
         Col A, B;
-        A.Pj = A.rj = 0.f;
         A.Pjm = A.rjm = A.vyjm = A.mPym = 0.f;
         A.mPx = A.mPy = A.mPz = A.mvx = A.mvy = A.mvz = 0.f;
         B = A;
         {
             const XPlane xs = a.xpl[ist];
-            load_col<S, TJ>(a, L, w, ist, plane, xs.sxv, xs.sxp, A);
+            load_col<S>(a, L, w, ist, plane, xs.sxv, xs.sxp, A);
         }
-        // old P and rho of the first plane, for the row below
-        sh.P[ist & 1][w][lane] = P_c;
-        sh.r[ist & 1][w][lane] = r_c;
-        __syncthreads();
         for (int i = ist; i < i1; i += 2) {
-            plane_step<S, TJ>(a, L, w, i, i0, i1, plane, A, B, P_c, r_c, vx_prev, sh);
+            plane_step<S, TJ>(a, L, w, i, i0, i1, plane, A, B, P_c, r_c, vx_prev, s_vy);
             if (i + 1 >= i1) break;
-            plane_step<S, TJ>(a, L, w, i + 1, i0, i1, plane, B, A, P_c, r_c, vx_prev, sh);
+            plane_step<S, TJ>(a, L, w, i + 1, i0, i1, plane, B, A, P_c, r_c, vx_prev, s_vy);
         }
     }
 }

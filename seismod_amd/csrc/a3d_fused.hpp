@@ -75,5 +75,10 @@ int gather_launch(const FusedArgs<S> &a, hipStream_t st);
 // resident blocks per CU of the stencil kernel with TJ = tj
 template <typename S>
 int fused_max_blocks_per_cu(int tj, int *blocks);
+// the same for the two-cells-per-lane kernel (a3d_pair.hip)
+template <typename S>
+int pair_launch(const FusedArgs<S> &a, int tj, hipStream_t st);
+template <typename S>
+int pair_max_blocks_per_cu(int tj, int *blocks);
 
 }  // namespace sm

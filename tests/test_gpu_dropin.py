@@ -9,7 +9,8 @@ inputs:
     1e3 x value, io_utils.c:1744-1768), at the steps the record predicate
     (e_iso.c:643-658) selects;
   - netCDF snapshots ssh/<base>/... must hold the oracle's field at the
-    snapshot step, in mm/s, rounded to FP32 (io_utils.c:1822-2180).
+    snapshot step in mm/s, converted as the reference does: rounded to FP32,
+    then times 1e3 in FP32 (io_utils.c:1964, 2142).
 The FP32 default storage is checked against the same oracle within the
 FP32 tolerance of the other tests.
 """
@@ -120,7 +121,7 @@ def test_e_iso_program(tmp_path, storage):
     _, vz = oracle_2d(d, it_snap[0] + 1)
     z = nc_field(snaps[0])
     assert z.shape == (d["nz"], d["nx"])
-    want = (1e3 * vz.T).astype(np.float32).astype(np.float64)
+    want = (vz.T.astype(np.float32) * np.float32(1e3)).astype(np.float64)
     if storage == "f64":
         assert np.array_equal(z, want)
     else:
@@ -202,7 +203,7 @@ def test_a_iso_3d_program(tmp_path, storage):
     sl = vz[np_:np_ + nx, iy, np_:np_ + nz]
     z = nc_field(snaps[0])
     assert z.shape == (nz, nx)
-    want = (1e3 * sl.T).astype(np.float32).astype(np.float64)
+    want = (sl.T.astype(np.float32) * np.float32(1e3)).astype(np.float64)
     if storage == "f64":
         assert np.array_equal(z, want)
     else:

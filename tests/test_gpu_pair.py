@@ -1,0 +1,96 @@
+"""GPU tests of the two-cells-per-lane 3D kernel variant ("pair").
+
+It performs the same per-cell arithmetic as the one-cell-per-lane variant,
+only with a different mapping of cells to lanes and memory accesses, so
+traces and all four fields must be bit-identical to it for every storage
+type, launch shape and run splitting.  The one-cell variant is itself held
+to the FP64 oracle by test_gpu_fused.py and test_gpu_precision.py; the
+oracle check is repeated here for FP32.
+"""
+import numpy as np
+import pytest
+
+import decks
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    dict(nx=10, ny=8, nz=6, np_=3, seed=1),
+    dict(nx=7, ny=9, nz=70, np_=4, seed=2, kappa_max=3.0),      # two z tiles, odd ny2
+    dict(nx=12, ny=5, nz=8, np_=2, seed=3, dy=7.5, alpha_max=0),
+    dict(nx=9, ny=11, nz=126, np_=0, seed=4),                    # no CPML, three z tiles
+    dict(nx=6, ny=17, nz=64, np_=5, seed=5, nsteps=90),          # nz2 = 74
+]
+
+
+def ids(c):
+    return f"{c['nx']}x{c['ny']}x{c['nz']}_np{c['np_']}"
+
+
+def run(ctx, d, storage, variant, launch=None, sround=False, split=None):
+    from seismod_amd.solver import A3dConfig, Acoustic3D, source_functions
+    c = d["cfg"]
+    cfg = A3dConfig(nx=c["nx"], ny=c["ny"], nz=c["nz"], np=c["np"], dx=c["dx"], dy=c["dy"],
+                    dz=c["dz"], dt=c["dt"], f0=c["f0"], vmax=c["vmax"], pml_order=c["pml_order"],
+                    Rc=c["Rc"], kappa_max=c["kappa_max"], alpha_max=c["alpha_max"],
+                    storage=storage, sround=sround)
+    s = Acoustic3D(ctx, cfg, d["vp"], d["rho"])
+    s.set_variant(variant)
+    if launch is not None:
+        s.set_launch(*launch)
+    L, fct = source_functions(d["A"], d["f"], c["dt"])
+    s.set_sources(d["src_idx"], L, fct)
+    s.set_receivers(d["rec_idx"], d["rec_comp"], d["nsteps"])
+    for n in (split or [d["nsteps"]]):
+        s.run(n)
+    tr = s.traces(0, d["nsteps"])
+    f = np.concatenate([s.field(k).reshape(-1) for k in (8, 0, 1, 2)])
+    s.close()
+    return tr, f
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16", "f16"])
+@pytest.mark.parametrize("case", CASES, ids=ids)
+def test_pair_equals_cell(ctx, case, storage):
+    d = decks.a3d_deck(**case)
+    assert (d["cfg"]["nz"] + 2 * d["cfg"]["np"]) % 2 == 0
+    tr_c, f_c = run(ctx, d, storage, "cell")
+    tr_p, f_p = run(ctx, d, storage, "pair")
+    assert np.abs(tr_c).max() > 0
+    assert np.array_equal(tr_p, tr_c), np.abs(tr_p - tr_c).max()
+    assert np.array_equal(f_p, f_c), np.abs(f_p - f_c).max()
+
+
+@pytest.mark.parametrize("launch", [(1, 1), (2, 3), (4, 7), (8, 2), (4, 26)])
+def test_pair_launch_shapes_and_splitting(ctx, launch):
+    d = decks.a3d_deck(nx=26, ny=21, nz=68, np_=4, seed=7, nsteps=70)
+    ref = run(ctx, d, "f32", "cell")
+    got = run(ctx, d, "f32", "pair", launch=launch, split=[1, 20, 49])
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_pair_stochastic_rounding_equals_cell(ctx, storage):
+    d = decks.a3d_deck(nx=14, ny=12, nz=68, np_=4, seed=8, nsteps=80)
+    a = run(ctx, d, storage, "cell", sround=True)
+    b = run(ctx, d, storage, "pair", sround=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_pair_fp32_vs_oracle(ctx):
+    d = decks.a3d_deck(nx=11, ny=10, nz=40, np_=5, seed=9, nsteps=100)
+    c = d["cfg"]
+    L, fct = O.source_functions(d["A"], d["f"], c["dt"])
+    tr_o, f_o = O.a3d_run(c, d["vp"], d["rho"], d["src_idx"], L, fct, d["rec_idx"],
+                          d["rec_comp"], d["nsteps"])
+    tr, f = run(ctx, d, "f32", "pair")
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(tr, tr_o) < 2e-4 and rel(f, f_o) < 2e-4
+
+
+def test_pair_needs_even_z(ctx):
+    from seismod_amd._lib import SeismodError
+    d = decks.a3d_deck(nx=8, ny=8, nz=7, np_=3, seed=10, nsteps=5)   # nz2 = 13
+    with pytest.raises(SeismodError):
+        run(ctx, d, "f32", "pair")
