@@ -176,11 +176,13 @@ int sm_a3d_reset(sm_a3d *s);
 int sm_a3d_set_launch(sm_a3d *s, int rows_per_block, int x_chunks);
 int sm_a3d_get_launch(const sm_a3d *s, int *rows_per_block, int *x_chunks);
 /* Fused kernel variant (no counterpart in the reference): SM_A3D_CELL, one
- * cell per lane; SM_A3D_PAIR, two z-adjacent cells per lane moved as one
+ * cell per lane, loads one plane ahead; SM_A3D_CELL2, the same with loads
+ * two planes ahead; SM_A3D_PAIR, two z-adjacent cells per lane moved as one
  * access (needs an even padded z extent).  The variants compute the same
  * values bit for bit; setting one resets the launch shape to its default. */
 #define SM_A3D_CELL 0
 #define SM_A3D_PAIR 1
+#define SM_A3D_CELL2 2
 int sm_a3d_set_variant(sm_a3d *s, int variant);
 int sm_a3d_get_variant(const sm_a3d *s, int *variant);
 /* Per-launch timing: when on, every step kernel launch is bracketed by HIP

@@ -18,6 +18,7 @@ SM_A3D_NAIVE = 1
 SM_A3D_SROUND = 2
 SM_A3D_CELL = 0
 SM_A3D_PAIR = 1
+SM_A3D_CELL2 = 2
 
 _size_p = C.POINTER(C.c_size_t)
 _dbl_p = C.POINTER(C.c_double)

@@ -473,8 +473,9 @@ int run_fused(sm_a3d *s, size_t nsteps)
         a.gather_row = (n > 0 && it > 0) ? rcv_row(s, it - 1) : -1;
         a.gather_blocks = a.gather_row >= 0 ? (int)((s->nrec + per_gather - 1) / per_gather) : 0;
         if ((rc = tm.mark(st, 2 * n)) != SM_OK) return rc;
-        if ((rc = s->variant == SM_A3D_PAIR ? pair_launch<S>(a, s->tj, st)
-                                            : fused_launch<S>(a, s->tj, st)) != SM_OK)
+        if ((rc = s->variant == SM_A3D_PAIR
+                      ? pair_launch<S>(a, s->tj, st)
+                      : fused_launch<S>(a, s->tj, s->variant == SM_A3D_CELL2 ? 2 : 1, st)) != SM_OK)
             return fail(rc, "fused launch failed (tj=%d): %s", s->tj,
                         hipGetErrorString(hipGetLastError()));
         if ((rc = tm.mark(st, 2 * n + 1)) != SM_OK) return rc;
@@ -569,18 +570,19 @@ int resident_blocks(sm_a3d *s, int tj)
 {
     int per_cu = 0, rc = SM_OK;
     const bool pair = s->variant == SM_A3D_PAIR;
+    const int pf = s->variant == SM_A3D_CELL2 ? 2 : 1;
     switch (s->p.storage) {
     case SM_STORE_F32:
         rc = pair ? pair_max_blocks_per_cu<float>(tj, &per_cu)
-                  : fused_max_blocks_per_cu<float>(tj, &per_cu);
+                  : fused_max_blocks_per_cu<float>(tj, pf, &per_cu);
         break;
     case SM_STORE_BF16:
         rc = pair ? pair_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu)
-                  : fused_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu);
+                  : fused_max_blocks_per_cu<__hip_bfloat16>(tj, pf, &per_cu);
         break;
     default:
         rc = pair ? pair_max_blocks_per_cu<__half>(tj, &per_cu)
-                  : fused_max_blocks_per_cu<__half>(tj, &per_cu);
+                  : fused_max_blocks_per_cu<__half>(tj, pf, &per_cu);
         break;
     }
     if (rc != SM_OK || per_cu < 1) per_cu = 1;
@@ -906,7 +908,7 @@ int sm_a3d_set_variant(sm_a3d *s, int variant)
 {
     SM_CHECK(s, SM_ERR_ARG, "sm_a3d_set_variant: NULL solver");
     SM_CHECK(s->fused, SM_ERR_STATE, "sm_a3d_set_variant: only the fused kernel has variants");
-    SM_CHECK(variant == SM_A3D_CELL || variant == SM_A3D_PAIR, SM_ERR_ARG,
+    SM_CHECK(variant == SM_A3D_CELL || variant == SM_A3D_PAIR || variant == SM_A3D_CELL2, SM_ERR_ARG,
              "sm_a3d_set_variant: unknown variant %d", variant);
     SM_CHECK(variant != SM_A3D_PAIR || (s->nz2 % 2 == 0 && s->nz2 >= 4), SM_ERR_ARG,
              "sm_a3d_set_variant: the pair kernel needs an even padded z extent (nz2 = %d)", s->nz2);

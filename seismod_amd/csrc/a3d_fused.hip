@@ -99,16 +99,16 @@ __device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, i
     if (L.szp >= 0) c.mvz = Store<S>::ld(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, a.sMp);
 }
 
-// one plane: prefetch plane i+1 into nxt, update plane i from cur
-template <typename S, int TJ>
+// one plane: prefetch plane i+PF into fill, update plane i from cur
+template <typename S, int TJ, int PF>
 __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L, int w, int i,
-                                           int i0, int i1, unsigned plane, Col &cur, Col &nxt,
+                                           int i0, int i1, unsigned plane, Col &cur, Col &fill,
                                            float &P_c, float &r_c, float &vx_prev,
                                            float (*s_vy)[TJ][64])
 {
-    if (i + 1 < i1) {
-        const XPlane xn = a.xpl[i + 1];
-        load_col<S>(a, L, w, i + 1, plane, xn.sxv, xn.sxp, nxt);
+    if (i + PF < i1) {
+        const XPlane xn = a.xpl[i + PF];
+        load_col<S>(a, L, w, i + PF, plane, xn.sxv, xn.sxp, fill);
     }
     const XPlane x = a.xpl[i];
     const unsigned o = (unsigned)i * plane + L.o_jk;
@@ -204,7 +204,8 @@ __device__ __forceinline__ void gather_one(const FusedArgs<S> &a, int r)
     a.trace[(size_t)a.gather_row * a.nrec + r] = Store<S>::ld(a.in, q * a.nnn + (unsigned)a.ridx[r], sc);
 }
 
-template <typename S, int TJ>
+// PF: planes prefetched ahead of the one being updated (1 or 2)
+template <typename S, int TJ, int PF>
 __global__ void __launch_bounds__(64 * TJ)
 a3d_fused_step(FusedArgs<S> a)
 {
@@ -284,10 +285,26 @@ a3d_fused_step(FusedArgs<S> a)
             const XPlane xs = a.xpl[ist];
             load_col<S>(a, L, w, ist, plane, xs.sxv, xs.sxp, A);
         }
-        for (int i = ist; i < i1; i += 2) {
-            plane_step<S, TJ>(a, L, w, i, i0, i1, plane, A, B, P_c, r_c, vx_prev, s_vy);
-            if (i + 1 >= i1) break;
-            plane_step<S, TJ>(a, L, w, i + 1, i0, i1, plane, B, A, P_c, r_c, vx_prev, s_vy);
+        if constexpr (PF == 1) {
+            for (int i = ist; i < i1; i += 2) {
+                plane_step<S, TJ, 1>(a, L, w, i, i0, i1, plane, A, B, P_c, r_c, vx_prev, s_vy);
+                if (i + 1 >= i1) break;
+                plane_step<S, TJ, 1>(a, L, w, i + 1, i0, i1, plane, B, A, P_c, r_c, vx_prev, s_vy);
+            }
+        } else {
+            // three register sets in rotation: plane i in use, i+1 and i+2 in flight
+            Col C = B;   // B still holds the zeroed template (its loads follow)
+            if (ist + 1 < i1) {
+                const XPlane xs = a.xpl[ist + 1];
+                load_col<S>(a, L, w, ist + 1, plane, xs.sxv, xs.sxp, B);
+            }
+            for (int i = ist; i < i1; i += 3) {
+                plane_step<S, TJ, 2>(a, L, w, i, i0, i1, plane, A, C, P_c, r_c, vx_prev, s_vy);
+                if (i + 1 >= i1) break;
+                plane_step<S, TJ, 2>(a, L, w, i + 1, i0, i1, plane, B, A, P_c, r_c, vx_prev, s_vy);
+                if (i + 2 >= i1) break;
+                plane_step<S, TJ, 2>(a, L, w, i + 2, i0, i1, plane, C, B, P_c, r_c, vx_prev, s_vy);
+            }
         }
     }
 }
@@ -301,21 +318,25 @@ a3d_gather(FusedArgs<S> a)
     gather_one(a, r);
 }
 
-template <typename S, int TJ>
+template <typename S, int TJ, int PF>
 static const void *kernel_ptr()
 {
-    return reinterpret_cast<const void *>(&a3d_fused_step<S, TJ>);
+    return reinterpret_cast<const void *>(&a3d_fused_step<S, TJ, PF>);
 }
 
 template <typename S>
-int fused_max_blocks_per_cu(int tj, int *blocks)
+int fused_max_blocks_per_cu(int tj, int pf, int *blocks)
 {
     const void *f = nullptr;
-    switch (tj) {
-    case 1: f = kernel_ptr<S, 1>(); break;
-    case 2: f = kernel_ptr<S, 2>(); break;
-    case 4: f = kernel_ptr<S, 4>(); break;
-    case 8: f = kernel_ptr<S, 8>(); break;
+    switch (tj * 10 + pf) {
+    case 11: f = kernel_ptr<S, 1, 1>(); break;
+    case 21: f = kernel_ptr<S, 2, 1>(); break;
+    case 41: f = kernel_ptr<S, 4, 1>(); break;
+    case 81: f = kernel_ptr<S, 8, 1>(); break;
+    case 12: f = kernel_ptr<S, 1, 2>(); break;
+    case 22: f = kernel_ptr<S, 2, 2>(); break;
+    case 42: f = kernel_ptr<S, 4, 2>(); break;
+    case 82: f = kernel_ptr<S, 8, 2>(); break;
     default: return SM_ERR_ARG;
     }
     int nb = 0;
@@ -326,14 +347,18 @@ int fused_max_blocks_per_cu(int tj, int *blocks)
 }
 
 template <typename S>
-int fused_launch(const FusedArgs<S> &a, int tj, hipStream_t st)
+int fused_launch(const FusedArgs<S> &a, int tj, int pf, hipStream_t st)
 {
     const int blocks = a.nblocks + a.gather_blocks;
-    switch (tj) {
-    case 1: a3d_fused_step<S, 1><<<blocks, dim3(64, 1), 0, st>>>(a); break;
-    case 2: a3d_fused_step<S, 2><<<blocks, dim3(64, 2), 0, st>>>(a); break;
-    case 4: a3d_fused_step<S, 4><<<blocks, dim3(64, 4), 0, st>>>(a); break;
-    case 8: a3d_fused_step<S, 8><<<blocks, dim3(64, 8), 0, st>>>(a); break;
+    switch (tj * 10 + pf) {
+    case 11: a3d_fused_step<S, 1, 1><<<blocks, dim3(64, 1), 0, st>>>(a); break;
+    case 21: a3d_fused_step<S, 2, 1><<<blocks, dim3(64, 2), 0, st>>>(a); break;
+    case 41: a3d_fused_step<S, 4, 1><<<blocks, dim3(64, 4), 0, st>>>(a); break;
+    case 81: a3d_fused_step<S, 8, 1><<<blocks, dim3(64, 8), 0, st>>>(a); break;
+    case 12: a3d_fused_step<S, 1, 2><<<blocks, dim3(64, 1), 0, st>>>(a); break;
+    case 22: a3d_fused_step<S, 2, 2><<<blocks, dim3(64, 2), 0, st>>>(a); break;
+    case 42: a3d_fused_step<S, 4, 2><<<blocks, dim3(64, 4), 0, st>>>(a); break;
+    case 82: a3d_fused_step<S, 8, 2><<<blocks, dim3(64, 8), 0, st>>>(a); break;
     default: return SM_ERR_ARG;
     }
     return hipGetLastError() == hipSuccess ? SM_OK : SM_ERR_HIP;
@@ -347,14 +372,14 @@ int gather_launch(const FusedArgs<S> &a, hipStream_t st)
     return hipGetLastError() == hipSuccess ? SM_OK : SM_ERR_HIP;
 }
 
-template int fused_launch<float>(const FusedArgs<float> &, int, hipStream_t);
-template int fused_launch<__hip_bfloat16>(const FusedArgs<__hip_bfloat16> &, int, hipStream_t);
-template int fused_launch<__half>(const FusedArgs<__half> &, int, hipStream_t);
+template int fused_launch<float>(const FusedArgs<float> &, int, int, hipStream_t);
+template int fused_launch<__hip_bfloat16>(const FusedArgs<__hip_bfloat16> &, int, int, hipStream_t);
+template int fused_launch<__half>(const FusedArgs<__half> &, int, int, hipStream_t);
 template int gather_launch<float>(const FusedArgs<float> &, hipStream_t);
 template int gather_launch<__hip_bfloat16>(const FusedArgs<__hip_bfloat16> &, hipStream_t);
 template int gather_launch<__half>(const FusedArgs<__half> &, hipStream_t);
-template int fused_max_blocks_per_cu<float>(int, int *);
-template int fused_max_blocks_per_cu<__hip_bfloat16>(int, int *);
-template int fused_max_blocks_per_cu<__half>(int, int *);
+template int fused_max_blocks_per_cu<float>(int, int, int *);
+template int fused_max_blocks_per_cu<__hip_bfloat16>(int, int, int *);
+template int fused_max_blocks_per_cu<__half>(int, int, int *);
 
 }  // namespace sm

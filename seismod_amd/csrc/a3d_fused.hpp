@@ -68,13 +68,14 @@ struct FusedArgs {
     int64_t gather_row;   // -1: nothing to gather
 };
 
+// pf: planes prefetched ahead (1 or 2)
 template <typename S>
-int fused_launch(const FusedArgs<S> &a, int tj, hipStream_t st);
+int fused_launch(const FusedArgs<S> &a, int tj, int pf, hipStream_t st);
 template <typename S>
 int gather_launch(const FusedArgs<S> &a, hipStream_t st);
 // resident blocks per CU of the stencil kernel with TJ = tj
 template <typename S>
-int fused_max_blocks_per_cu(int tj, int *blocks);
+int fused_max_blocks_per_cu(int tj, int pf, int *blocks);
 // the same for the two-cells-per-lane kernel (a3d_pair.hip)
 template <typename S>
 int pair_launch(const FusedArgs<S> &a, int tj, hipStream_t st);

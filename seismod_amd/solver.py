@@ -227,17 +227,18 @@ class Acoustic3D:
         check(lib.sm_a3d_get_launch(self.handle, C.byref(tj), C.byref(ch)), "sm_a3d_get_launch")
         return tj.value, ch.value
 
-    VARIANTS = {"cell": L.SM_A3D_CELL, "pair": L.SM_A3D_PAIR}
+    VARIANTS = {"cell": L.SM_A3D_CELL, "pair": L.SM_A3D_PAIR, "cell2": L.SM_A3D_CELL2}
 
     def set_variant(self, variant: str) -> None:
-        """Fused kernel variant: "cell" (one cell per lane) or "pair" (two
-        z-adjacent cells per lane); results are bit-identical."""
+        """Fused kernel variant: "cell" (one cell per lane), "cell2" (the same,
+        loading two planes ahead) or "pair" (two z-adjacent cells per lane);
+        results are bit-identical."""
         check(lib.sm_a3d_set_variant(self.handle, self.VARIANTS[variant]), "sm_a3d_set_variant")
 
     def variant(self) -> str:
         v = C.c_int()
         check(lib.sm_a3d_get_variant(self.handle, C.byref(v)), "sm_a3d_get_variant")
-        return {0: "cell", 1: "pair"}.get(v.value, "none")
+        return {0: "cell", 1: "pair", 2: "cell2"}.get(v.value, "none")
 
     def set_timing(self, on: bool) -> None:
         check(lib.sm_a3d_set_timing(self.handle, 1 if on else 0), "sm_a3d_set_timing")

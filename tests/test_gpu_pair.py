@@ -1,4 +1,5 @@
-"""GPU tests of the two-cells-per-lane 3D kernel variant ("pair").
+"""GPU tests of the 3D kernel variants: two cells per lane ("pair") and
+one cell per lane with loads two planes ahead ("cell2").
 
 It performs the same per-cell arithmetic as the one-cell-per-lane variant,
 only with a different mapping of cells to lanes and memory accesses, so
@@ -50,23 +51,25 @@ def run(ctx, d, storage, variant, launch=None, sround=False, split=None):
     return tr, f
 
 
+@pytest.mark.parametrize("variant", ["pair", "cell2"])
 @pytest.mark.parametrize("storage", ["f32", "bf16", "f16"])
 @pytest.mark.parametrize("case", CASES, ids=ids)
-def test_pair_equals_cell(ctx, case, storage):
+def test_variant_equals_cell(ctx, case, storage, variant):
     d = decks.a3d_deck(**case)
     assert (d["cfg"]["nz"] + 2 * d["cfg"]["np"]) % 2 == 0
     tr_c, f_c = run(ctx, d, storage, "cell")
-    tr_p, f_p = run(ctx, d, storage, "pair")
+    tr_p, f_p = run(ctx, d, storage, variant)
     assert np.abs(tr_c).max() > 0
     assert np.array_equal(tr_p, tr_c), np.abs(tr_p - tr_c).max()
     assert np.array_equal(f_p, f_c), np.abs(f_p - f_c).max()
 
 
-@pytest.mark.parametrize("launch", [(1, 1), (2, 3), (4, 7), (8, 2), (4, 26)])
-def test_pair_launch_shapes_and_splitting(ctx, launch):
+@pytest.mark.parametrize("variant", ["pair", "cell2"])
+@pytest.mark.parametrize("launch", [(1, 1), (2, 3), (4, 7), (8, 2), (4, 17)])
+def test_variant_launch_shapes_and_splitting(ctx, launch, variant):
     d = decks.a3d_deck(nx=26, ny=21, nz=68, np_=4, seed=7, nsteps=70)
     ref = run(ctx, d, "f32", "cell")
-    got = run(ctx, d, "f32", "pair", launch=launch, split=[1, 20, 49])
+    got = run(ctx, d, "f32", variant, launch=launch, split=[1, 20, 49])
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
 
 
