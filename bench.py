@@ -143,6 +143,7 @@ def main():
     achieved = alg_b * cells / (avg_ms * 1e-3) / 1e9
     kname = f"a3d_fused_{args.storage}_{args.n}"
     tj, ch = sim.launch()
+    variant = sim.variant()
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -162,7 +163,7 @@ def main():
             "padded_grid": [nx2, ny2, nz2],
             "time_steps_per_shot": nsteps,
             "dt_s": shot.dt,
-            "launch": {"rows_per_block": tj, "x_chunks": ch,
+            "launch": {"kernel_variant": variant, "rows_per_block": tj, "x_chunks": ch,
                        "chosen_by": "autotuner" if args.launch == "auto" else "command line"},
             "parallelism": "single GPU",
         },

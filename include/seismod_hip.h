@@ -130,6 +130,8 @@ typedef struct {
 } sm_a3d_params;
 
 #define SM_A3D_NAIVE 1        /* two plain kernels per step (no fusion) */
+#define SM_A3D_PAD64 4        /* fused kernels: device rows padded to a multiple
+                                 of 64 cells (needed by SM_A3D_CELL64*) */
 #define SM_A3D_SROUND 2       /* BF16/FP16 storage: stochastic rounding of the
                                  stored values (deterministic pseudo-random
                                  bits from element index and step) instead
@@ -176,13 +178,18 @@ int sm_a3d_reset(sm_a3d *s);
 int sm_a3d_set_launch(sm_a3d *s, int rows_per_block, int x_chunks);
 int sm_a3d_get_launch(const sm_a3d *s, int *rows_per_block, int *x_chunks);
 /* Fused kernel variant (no counterpart in the reference): SM_A3D_CELL, one
- * cell per lane, loads one plane ahead; SM_A3D_CELL2, the same with loads
- * two planes ahead; SM_A3D_PAIR, two z-adjacent cells per lane moved as one
- * access (needs an even padded z extent).  The variants compute the same
- * values bit for bit; setting one resets the launch shape to its default. */
+ * cell per lane in z tiles of 63 owned cells, loads one plane ahead;
+ * SM_A3D_CELL2, the same with loads two planes ahead; SM_A3D_PAIR, two
+ * z-adjacent cells per lane moved as one access (needs an even padded z
+ * extent); SM_A3D_CELL64 / SM_A3D_CELL64X2, z tiles of 64 cells aligned to
+ * the device rows (whole cache lines per access), loads one / two planes
+ * ahead.  The variants compute the same values bit for bit; setting one
+ * resets the launch shape to its default. */
 #define SM_A3D_CELL 0
 #define SM_A3D_PAIR 1
 #define SM_A3D_CELL2 2
+#define SM_A3D_CELL64 3
+#define SM_A3D_CELL64X2 4
 int sm_a3d_set_variant(sm_a3d *s, int variant);
 int sm_a3d_get_variant(const sm_a3d *s, int *variant);
 /* Per-launch timing: when on, every step kernel launch is bracketed by HIP

@@ -16,9 +16,12 @@ SM_VX, SM_VY, SM_VZ, SM_TXX, SM_TZZ, SM_TXZ, SM_P = 0, 1, 2, 5, 6, 7, 8
 SM_SX, SM_SY, SM_SZ, SM_SXY, SM_SXZ, SM_SYZ, SM_BULK, SM_SF = range(8)
 SM_A3D_NAIVE = 1
 SM_A3D_SROUND = 2
+SM_A3D_PAD64 = 4
 SM_A3D_CELL = 0
 SM_A3D_PAIR = 1
 SM_A3D_CELL2 = 2
+SM_A3D_CELL64 = 3
+SM_A3D_CELL64X2 = 4
 
 _size_p = C.POINTER(C.c_size_t)
 _dbl_p = C.POINTER(C.c_double)

@@ -10,7 +10,7 @@ Tuning runs steps and then resets the solver (fields, memory variables,
 traces and step counter back to zero), so call it right after set-up,
 before the solver's real run.
 
-  3D (Acoustic3D):  rows_per_block in {1, 2, 4, 8} x x_chunks
+  3D (Acoustic3D):  kernel variant (cell, cell2) x rows_per_block x x_chunks
   2D (Elastic2D):   rows_per_chunk in {1, 2, 3, 4, 6, 8, 12, 16}
 """
 from __future__ import annotations
@@ -63,32 +63,43 @@ def _time(sim, ctx, steps: int) -> float:
     return ms / max(n, 1)
 
 
-def tune_a3d(sim, ctx, steps: int = 8, cache_file=None, candidates=None) -> tuple[int, int]:
-    """Pick (rows_per_block, x_chunks) for an Acoustic3D solver; applies it."""
+def tune_a3d(sim, ctx, steps: int = 8, cache_file=None, candidates=None,
+             variants=("cell", "cell2")) -> tuple:
+    """Pick the kernel variant and (rows_per_block, x_chunks) for an
+    Acoustic3D solver; applies them.  Returns (variant, rows_per_block,
+    x_chunks)."""
     key = _key_a3d(sim)
     with _LOCK:
         _load(cache_file)
         if key in _CACHE:
-            sim.set_launch(*_CACHE[key])
+            v, tj, ch = _CACHE[key]
+            sim.set_variant(v)
+            sim.set_launch(tj, ch)
             return _CACHE[key]
     nx2 = sim.cfg.nx + 2 * sim.cfg.np
     if candidates is None:
-        candidates = [(tj, ch) for tj in (2, 4, 8) for ch in (2, 4, 6, 8, 12)
+        candidates = [(tj, ch) for tj in (2, 4, 8) for ch in (2, 3, 4, 6, 8, 12)
                       if ch <= max(1, nx2 // 16)]
     best = None
-    for tj, ch in candidates:
+    for v in variants:
         try:
-            sim.set_launch(tj, ch)
+            sim.set_variant(v)
         except Exception:
             continue
-        t = _time(sim, ctx, steps)
-        if best is None or t < best[0]:
-            best = (t, tj, ch)
+        for tj, ch in candidates:
+            try:
+                sim.set_launch(tj, ch)
+            except Exception:
+                continue
+            t = _time(sim, ctx, steps)
+            if best is None or t < best[0]:
+                best = (t, v, tj, ch)
     sim.reset()
     if best is None:
-        return sim.launch()
-    choice = (best[1], best[2])
-    sim.set_launch(*choice)
+        return (sim.variant(),) + sim.launch()
+    choice = (best[1], best[2], best[3])
+    sim.set_variant(choice[0])
+    sim.set_launch(choice[1], choice[2])
     with _LOCK:
         _CACHE[key] = choice
         _save(cache_file)

@@ -145,7 +145,9 @@ struct sm_a3d {
     sm_ctx *ctx = nullptr;
     sm_a3d_params p{};
     int nx2 = 0, ny2 = 0, nz2 = 0;
-    size_t nnn = 0;
+    size_t nnn = 0;                 // cells per field (reference layout)
+    int pz = 0;                     // device row pitch along z (cells)
+    size_t nnp = 0;                 // cells per field on the device: nx2*ny2*pz
     size_t elt = 0;                 // bytes per stored state value
     size_t melt = 0;                // bytes per material / table value
     size_t telt = 0;                // bytes per recorded trace value
@@ -182,9 +184,46 @@ struct sm_a3d {
 
 namespace {
 
-size_t slab_cells(size_t nx2, size_t ny2, size_t nz2, size_t np)
+// one set of x, y and z slab arrays, with rows of pz cells along z
+size_t slab_cells(size_t nx2, size_t ny2, size_t pz, size_t np)
 {
-    return 2 * np * (ny2 * nz2 + nx2 * nz2 + nx2 * ny2);
+    return 2 * np * (ny2 * pz + nx2 * pz + nx2 * ny2);
+}
+
+bool uses_fused(const sm_a3d_params &p)
+{
+    if (p.storage == SM_STORE_F64) return false;
+    if (p.storage == SM_STORE_F32) return !(p.flags & SM_A3D_NAIVE);
+    return true;   // 16-bit storage exists only in the fused kernel
+}
+
+// device row pitch: rows padded to a multiple of 64 cells on request
+// (SM_A3D_PAD64, fused kernels only; the 64-cell-tile variants need it),
+// otherwise the reference layout
+size_t row_pitch(size_t nz2, const sm_a3d_params &p)
+{
+    return (p.flags & SM_A3D_PAD64) && uses_fused(p) ? (nz2 + 63) / 64 * 64 : nz2;
+}
+
+// reference-layout cell index -> device index
+size_t dev_index(const sm_a3d *s, size_t idx)
+{
+    return idx / s->nz2 * s->pz + idx % s->nz2;
+}
+
+// rows of nz2 values from the host (reference layout) into rows of pz
+// values on the device; the padding repeats each row's last value
+template <typename T>
+int upload_rows(void *dst, const double *src, size_t rows, size_t nz2, size_t pz, hipStream_t st)
+{
+    std::vector<T> tmp(rows * pz);
+    for (size_t r = 0; r < rows; ++r) {
+        for (size_t k = 0; k < nz2; ++k) tmp[r * pz + k] = (T)src[r * nz2 + k];
+        for (size_t k = nz2; k < pz; ++k) tmp[r * pz + k] = (T)src[r * nz2 + nz2 - 1];
+    }
+    SM_HIP(hipMemcpyAsync(dst, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    SM_HIP(hipStreamSynchronize(st));
+    return SM_OK;
 }
 
 // CPML tables of one axis (a_iso_3d.c:500-576), same write order.
@@ -292,9 +331,9 @@ A3dDev<T> dev_view(sm_a3d *s, int half)
     A3dDev<T> d;
     d.nx2 = s->nx2; d.ny2 = s->ny2; d.nz2 = s->nz2; d.np = (int)s->p.np;
     T *f = (T *)s->fields[half].ptr;
-    d.P = f; d.vx = f + s->nnn; d.vy = f + 2 * s->nnn; d.vz = f + 3 * s->nnn;
+    d.P = f; d.vx = f + s->nnp; d.vy = f + 2 * s->nnp; d.vz = f + 3 * s->nnp;
     const T *m = (const T *)s->mat.ptr;
-    d.rho = m; d.kdt = m + s->nnn;
+    d.rho = m; d.kdt = m + s->nnp;
     const size_t np2 = 2 * s->p.np;
     T *mv = (T *)s->memv[half].ptr;
     d.mPx = mv; d.mPy = mv + np2 * s->ny2 * s->nz2; d.mPz = d.mPy + s->nx2 * np2 * s->nz2;
@@ -322,6 +361,25 @@ unsigned step_salt(size_t it)
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return (unsigned)(z ^ (z >> 31));
+}
+
+// kernel variant properties: planes prefetched, 64-cell aligned z tiles
+int variant_pf(int v) { return v == SM_A3D_CELL2 || v == SM_A3D_CELL64X2 ? 2 : 1; }
+bool variant_a64(int v) { return v == SM_A3D_CELL64 || v == SM_A3D_CELL64X2; }
+
+// z tiles: 62 owned cells from even cells (pair), 64 aligned (cell64), 63
+int z_tiles(const sm_a3d *s)
+{
+    if (s->variant == SM_A3D_PAIR) return (s->nz2 + 60) / 62;
+    if (variant_a64(s->variant)) return (s->nz2 + 63) / 64;
+    return (s->nz2 + 62) / 63;
+}
+
+// y tiles: rows_per_block rows per block (two rows per wave for pair)
+int y_tiles(const sm_a3d *s)
+{
+    const int rows = s->variant == SM_A3D_PAIR ? 2 * s->tj : s->tj;
+    return (s->ny2 + rows - 1) / rows;
 }
 
 int64_t rcv_row(const sm_a3d *s, size_t it)
@@ -408,23 +466,18 @@ FusedArgs<S> fused_args(sm_a3d *s)
 {
     FusedArgs<S> a{};
     a.nx2 = s->nx2; a.ny2 = s->ny2; a.nz2 = s->nz2; a.np = (int)s->p.np;
-    if (s->variant == SM_A3D_PAIR) {
-        // z tiles of 62 owned cells starting at even cells, two rows per wave
-        a.ntk = (s->nz2 + 60) / 62;
-        a.ntiles = a.ntk * ((s->ny2 + 2 * s->tj - 1) / (2 * s->tj));
-    } else {
-        a.ntk = (s->nz2 + 62) / 63;      // 63 owned cells per wave along z
-        a.ntiles = a.ntk * ((s->ny2 + s->tj - 1) / s->tj);
-    }
+    a.ntk = z_tiles(s);
+    a.ntiles = a.ntk * y_tiles(s);
     a.chunk_len = (s->nx2 + s->nchunks - 1) / s->nchunks;
     a.nreal = a.ntiles * s->nchunks;
     a.nitems = (a.nreal + 7) / 8 * 8;
     a.nblocks = s->nblocks > 0 ? std::min(s->nblocks, a.nitems) : a.nitems;
-    a.nnn = (unsigned)s->nnn;
+    a.nnn = (unsigned)s->nnp;
+    a.pz = s->pz;
     const unsigned np2 = 2u * (unsigned)s->p.np;
     a.memp = (S *)s->memp.ptr;
-    a.my = np2 * (unsigned)s->ny2 * (unsigned)s->nz2;
-    a.mz = a.my + (unsigned)s->nx2 * np2 * (unsigned)s->nz2;
+    a.my = np2 * (unsigned)s->ny2 * (unsigned)s->pz;
+    a.mz = a.my + (unsigned)s->nx2 * np2 * (unsigned)s->pz;
     a.mat = (const float *)s->mat.ptr;
     a.xpl = (const XPlane *)s->xpl.ptr;
     a.tab = (const float *)s->tables.ptr;
@@ -475,7 +528,8 @@ int run_fused(sm_a3d *s, size_t nsteps)
         if ((rc = tm.mark(st, 2 * n)) != SM_OK) return rc;
         if ((rc = s->variant == SM_A3D_PAIR
                       ? pair_launch<S>(a, s->tj, st)
-                      : fused_launch<S>(a, s->tj, s->variant == SM_A3D_CELL2 ? 2 : 1, st)) != SM_OK)
+                      : fused_launch<S>(a, s->tj, variant_pf(s->variant), variant_a64(s->variant),
+                                        st)) != SM_OK)
             return fail(rc, "fused launch failed (tj=%d): %s", s->tj,
                         hipGetErrorString(hipGetLastError()));
         if ((rc = tm.mark(st, 2 * n + 1)) != SM_OK) return rc;
@@ -559,30 +613,25 @@ size_t elt_size(int storage)
     }
 }
 
-bool uses_fused(const sm_a3d_params &p)
-{
-    if (p.storage == SM_STORE_F64) return false;
-    if (p.storage == SM_STORE_F32) return !(p.flags & SM_A3D_NAIVE);
-    return true;   // 16-bit storage exists only in the fused kernel
-}
 
 int resident_blocks(sm_a3d *s, int tj)
 {
     int per_cu = 0, rc = SM_OK;
     const bool pair = s->variant == SM_A3D_PAIR;
-    const int pf = s->variant == SM_A3D_CELL2 ? 2 : 1;
+    const int pf = variant_pf(s->variant);
+    const bool a64 = variant_a64(s->variant);
     switch (s->p.storage) {
     case SM_STORE_F32:
         rc = pair ? pair_max_blocks_per_cu<float>(tj, &per_cu)
-                  : fused_max_blocks_per_cu<float>(tj, pf, &per_cu);
+                  : fused_max_blocks_per_cu<float>(tj, pf, a64, &per_cu);
         break;
     case SM_STORE_BF16:
         rc = pair ? pair_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu)
-                  : fused_max_blocks_per_cu<__hip_bfloat16>(tj, pf, &per_cu);
+                  : fused_max_blocks_per_cu<__hip_bfloat16>(tj, pf, a64, &per_cu);
         break;
     default:
         rc = pair ? pair_max_blocks_per_cu<__half>(tj, &per_cu)
-                  : fused_max_blocks_per_cu<__half>(tj, pf, &per_cu);
+                  : fused_max_blocks_per_cu<__half>(tj, pf, a64, &per_cu);
         break;
     }
     if (rc != SM_OK || per_cu < 1) per_cu = 1;
@@ -602,9 +651,7 @@ int valid_chunks(int nx2, int want)
 void default_launch(sm_a3d *s)
 {
     s->tj = 4;
-    const int ntiles = s->variant == SM_A3D_PAIR
-                           ? ((s->nz2 + 60) / 62) * ((s->ny2 + 2 * s->tj - 1) / (2 * s->tj))
-                           : ((s->nz2 + 62) / 63) * ((s->ny2 + s->tj - 1) / s->tj);
+    const int ntiles = z_tiles(s) * y_tiles(s);
     const int slots = resident_blocks(s, s->tj);
     int nch = (3 * slots + ntiles - 1) / ntiles;
     nch = std::min(nch, std::max(1, s->nx2 / 24));
@@ -623,9 +670,9 @@ size_t sm_a3d_bytes_needed(const sm_a3d_params *p, size_t nrec, size_t max_steps
     if (!e) return 0;
     const size_t me = p->storage == SM_STORE_F64 ? 8 : 4;
     const size_t nx2 = p->nx + 2 * p->np, ny2 = p->ny + 2 * p->np, nz2 = p->nz + 2 * p->np;
-    const size_t nnn = nx2 * ny2 * nz2;
+    const size_t nnn = nx2 * ny2 * row_pitch(nz2, *p);
     const size_t halves = uses_fused(*p) ? 2 : 1;
-    const size_t slabs = slab_cells(nx2, ny2, nz2, p->np);
+    const size_t slabs = slab_cells(nx2, ny2, row_pitch(nz2, *p), p->np);
     return e * (halves * 4 * nnn + (halves + 1) * slabs) + me * (2 * nnn + 6 * (nx2 + ny2 + nz2)) +
            me * nrec * max_steps + 16 * nrec + 4 * nx2;
 }
@@ -642,7 +689,7 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
              "grid cell size must be equal in X and Z (a_iso_3d.c:298-301)");
     const size_t e = elt_size(p.storage);
     SM_CHECK(e != 0, SM_ERR_ARG, "sm_a3d_create: storage %d not supported", p.storage);
-    SM_CHECK((p.flags & ~(SM_A3D_NAIVE | SM_A3D_SROUND)) == 0, SM_ERR_ARG,
+    SM_CHECK((p.flags & ~(SM_A3D_NAIVE | SM_A3D_SROUND | SM_A3D_PAD64)) == 0, SM_ERR_ARG,
              "sm_a3d_create: unknown flags %#x", p.flags);
     const size_t nx2 = p.nx + 2 * p.np, ny2 = p.ny + 2 * p.np, nz2 = p.nz + 2 * p.np;
     SM_CHECK(nx2 < 65536 && ny2 < 65536 && nz2 < 65536, SM_ERR_ARG, "sm_a3d_create: grid too large");
@@ -675,14 +722,17 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
     s->melt = p.storage == SM_STORE_F64 ? 8 : 4;
     s->telt = s->melt;
     s->fused = uses_fused(p);
+    s->pz = (int)row_pitch(nz2, p);
+    s->nnp = nx2 * ny2 * (size_t)s->pz;
+    const size_t nnp = s->nnp, pz = (size_t)s->pz;
     s->sr = (p.flags & SM_A3D_SROUND) && (p.storage == SM_STORE_BF16 || p.storage == SM_STORE_F16);
     // the fused kernel addresses the four state fields of one buffer and
     // the two material fields with 32-bit byte offsets
-    if (s->fused && (4 * nnn * e >= ((size_t)1 << 32) || 2 * nnn * 4 >= ((size_t)1 << 32))) {
+    if (s->fused && (4 * nnp * e >= ((size_t)1 << 32) || 2 * nnp * 4 >= ((size_t)1 << 32))) {
         delete s;
-        return fail(SM_ERR_ARG, "sm_a3d_create: %zu cells exceed the fused kernel's 32-bit "
-                    "offsets (limit 2^28 cells with FP32 storage, 2^29 with 16-bit); "
-                    "use SM_A3D_NAIVE with FP32 storage", nnn);
+        return fail(SM_ERR_ARG, "sm_a3d_create: %zu cells (rows padded to %zu) exceed the fused "
+                    "kernel's 32-bit offsets (limit 2^28 cells with FP32 storage, 2^29 with "
+                    "16-bit); use SM_A3D_NAIVE with FP32 storage", nnn, pz);
     }
     {
         hipDeviceProp_t prop;
@@ -691,22 +741,22 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
     }
     if (s->fused) default_launch(s);
     // one set (velocity or pressure) of x, y and z slab arrays
-    const size_t slab_set = slab_cells(nx2, ny2, nz2, p.np);
+    const size_t slab_set = slab_cells(nx2, ny2, pz, p.np);
     int rc = SM_OK;
     const int halves = s->fused ? 2 : 1;
     for (int h = 0; h < halves && rc == SM_OK; ++h) {
-        rc = dalloc(s->fields[h], 4 * nnn * e);
+        rc = dalloc(s->fields[h], 4 * nnp * e);
         if (rc == SM_OK) rc = dalloc(s->memv[h], slab_set * e);
     }
-    if (rc == SM_OK) rc = dalloc(s->mat, 2 * nnn * s->melt);
+    if (rc == SM_OK) rc = dalloc(s->mat, 2 * nnp * s->melt);
     if (rc == SM_OK) rc = dalloc(s->memp, slab_set * e);
     if (rc == SM_OK) rc = dalloc(s->tables, 6 * (nx2 + ny2 + nz2) * s->melt);
     if (rc != SM_OK) { release(s); delete s; return rc; }
     {
         // extent of the slab set as dev_view lays it out: must end inside
-        // the allocation (x: 2np*ny2*nz2, y: nx2*2np*nz2, z: nx2*ny2*2np)
+        // the allocation (x: 2np*ny2*pz, y: nx2*2np*pz, z: nx2*ny2*2np)
         const size_t np2 = 2 * p.np;
-        const size_t end = np2 * ny2 * nz2 + nx2 * np2 * nz2 + nx2 * ny2 * np2;
+        const size_t end = np2 * ny2 * pz + nx2 * np2 * pz + nx2 * ny2 * np2;
         if (end * e > s->memp.bytes || end * e > s->memv[0].bytes) {
             release(s); delete s;
             return fail(SM_ERR_STATE, "internal: slab layout exceeds its allocation");
@@ -718,12 +768,15 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
         std::vector<double> kdt(nnn);
         for (size_t n = 0; n < nnn; ++n) kdt[n] = rho[n] * vp[n] * vp[n] * p.dt;
         hipStream_t st = ctx->stream;
+        const size_t rows = nx2 * ny2;
         if (p.storage == SM_STORE_F64) {
-            rc = upload<double>(s->mat.ptr, rho, nnn, st);
-            if (rc == SM_OK) rc = upload<double>((double *)s->mat.ptr + nnn, kdt.data(), nnn, st);
+            rc = upload_rows<double>(s->mat.ptr, rho, rows, nz2, pz, st);
+            if (rc == SM_OK)
+                rc = upload_rows<double>((double *)s->mat.ptr + nnp, kdt.data(), rows, nz2, pz, st);
         } else {
-            rc = upload<float>(s->mat.ptr, rho, nnn, st);
-            if (rc == SM_OK) rc = upload<float>((float *)s->mat.ptr + nnn, kdt.data(), nnn, st);
+            rc = upload_rows<float>(s->mat.ptr, rho, rows, nz2, pz, st);
+            if (rc == SM_OK)
+                rc = upload_rows<float>((float *)s->mat.ptr + nnp, kdt.data(), rows, nz2, pz, st);
         }
         double rcm = 1e300;
         for (size_t n = 0; n < nnn; ++n) rcm = std::min(rcm, rho[n] * std::max(vp[n], 1.0));
@@ -797,7 +850,7 @@ int sm_a3d_set_sources(sm_a3d *s, size_t nsrc, const size_t *idx, const size_t *
     size_t off = 0;
     for (size_t n = 0; n < nsrc; ++n) { order[n] = n; offs[n] = off; off += length[n]; }
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return idx[a] < idx[b]; });
-    const size_t plane = (size_t)s->ny2 * s->nz2;
+    const size_t plane = (size_t)s->ny2 * s->pz;
     const size_t head = 3 * nsrc * sizeof(int64_t);
     const size_t plane_bytes = ((size_t)s->nx2 + 1) * sizeof(int);
     s->src_plane_off = head;
@@ -811,7 +864,7 @@ int sm_a3d_set_sources(sm_a3d *s, size_t nsrc, const size_t *idx, const size_t *
     size_t q = 0;
     for (size_t r = 0; r < nsrc; ++r) {
         const size_t n = order[r];
-        hi[r] = (int64_t)idx[n];
+        hi[r] = (int64_t)dev_index(s, idx[n]);
         hi[nsrc + r] = (int64_t)q;
         hi[2 * nsrc + r] = (int64_t)length[n];
         // increment dt*fct*1e6 (a_iso_3d.c:691), rounded once to the working type
@@ -878,7 +931,7 @@ int sm_a3d_set_receivers(sm_a3d *s, size_t nrec, const size_t *idx, const int *c
     std::vector<unsigned char> h(nrec * (sizeof(int64_t) + sizeof(int)));
     int64_t *hi = (int64_t *)h.data();
     int *hc = (int *)(hi + nrec);
-    for (size_t r = 0; r < nrec; ++r) { hi[r] = (int64_t)idx[r]; hc[r] = comp[r]; }
+    for (size_t r = 0; r < nrec; ++r) { hi[r] = (int64_t)dev_index(s, idx[r]); hc[r] = comp[r]; }
     SM_HIP(hipMemcpyAsync(s->rcv.ptr, h.data(), h.size(), hipMemcpyHostToDevice, s->ctx->stream));
     SM_HIP(hipMemsetAsync(s->trace.ptr, 0, s->trace.bytes, s->ctx->stream));
     SM_HIP(hipStreamSynchronize(s->ctx->stream));
@@ -908,8 +961,11 @@ int sm_a3d_set_variant(sm_a3d *s, int variant)
 {
     SM_CHECK(s, SM_ERR_ARG, "sm_a3d_set_variant: NULL solver");
     SM_CHECK(s->fused, SM_ERR_STATE, "sm_a3d_set_variant: only the fused kernel has variants");
-    SM_CHECK(variant == SM_A3D_CELL || variant == SM_A3D_PAIR || variant == SM_A3D_CELL2, SM_ERR_ARG,
+    SM_CHECK(variant >= SM_A3D_CELL && variant <= SM_A3D_CELL64X2, SM_ERR_ARG,
              "sm_a3d_set_variant: unknown variant %d", variant);
+    SM_CHECK(!variant_a64(variant) || s->pz % 64 == 0, SM_ERR_ARG,
+             "sm_a3d_set_variant: the 64-cell-tile variants need rows padded to 64 cells "
+             "(SM_A3D_PAD64 at creation)");
     SM_CHECK(variant != SM_A3D_PAIR || (s->nz2 % 2 == 0 && s->nz2 >= 4), SM_ERR_ARG,
              "sm_a3d_set_variant: the pair kernel needs an even padded z extent (nz2 = %d)", s->nz2);
     if (variant == s->variant) return SM_OK;
@@ -1001,8 +1057,9 @@ int sm_a3d_get_field(sm_a3d *s, int comp, double *out)
     const int slot = field_slot(comp);
     SM_CHECK(slot >= 0, SM_ERR_ARG, "sm_a3d_get_field: component %d", comp);
     SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
-    const char *src = (const char *)s->fields[s->cur].ptr + slot * s->nnn * s->elt;
-    return state_to_host(s, src, s->nnn, 0, 1, s->nnn, slot == 0 ? s->sP : s->sV, out);
+    const char *src = (const char *)s->fields[s->cur].ptr + slot * s->nnp * s->elt;
+    return state_to_host(s, src, s->nz2, 0, (size_t)s->nx2 * s->ny2, s->pz,
+                         slot == 0 ? s->sP : s->sV, out);
 }
 
 int sm_a3d_get_slice_xz(sm_a3d *s, int comp, size_t iy, double *out)
@@ -1012,9 +1069,9 @@ int sm_a3d_get_slice_xz(sm_a3d *s, int comp, size_t iy, double *out)
     const int slot = field_slot(comp);
     SM_CHECK(slot >= 0, SM_ERR_ARG, "sm_a3d_get_slice_xz: component %d", comp);
     SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
-    const char *base = (const char *)s->fields[s->cur].ptr + slot * s->nnn * s->elt +
-                       iy * (size_t)s->nz2 * s->elt;
-    return state_to_host(s, base, s->nz2, 0, s->nx2, (size_t)s->ny2 * s->nz2,
+    const char *base = (const char *)s->fields[s->cur].ptr + slot * s->nnp * s->elt +
+                       iy * (size_t)s->pz * s->elt;
+    return state_to_host(s, base, s->nz2, 0, s->nx2, (size_t)s->ny2 * s->pz,
                          slot == 0 ? s->sP : s->sV, out);
 }
 

@@ -25,12 +25,13 @@ struct XPlane {
 template <typename S>
 struct FusedArgs {
     int nx2, ny2, nz2, np;
+    int pz;               // device row pitch along z (cells, a multiple of 64)
     int ntk, ntiles, chunk_len;
     int nreal;            // tiles * chunks
     int nitems;           // nreal rounded up to a multiple of 8
     int nblocks;          // persistent stencil blocks (grid = nblocks + gather_blocks)
     int gather_blocks;
-    unsigned nnn;         // cells per field
+    unsigned nnn;         // cells per field on the device (nx2 * ny2 * pz)
     // state: P, vx, vy, vz at element q*nnn of in / out (ping-pong)
     const S *in;
     S *out;
@@ -68,14 +69,14 @@ struct FusedArgs {
     int64_t gather_row;   // -1: nothing to gather
 };
 
-// pf: planes prefetched ahead (1 or 2)
+// pf: planes prefetched ahead (1 or 2); a64: 64-cell aligned z tiles
 template <typename S>
-int fused_launch(const FusedArgs<S> &a, int tj, int pf, hipStream_t st);
+int fused_launch(const FusedArgs<S> &a, int tj, int pf, bool a64, hipStream_t st);
 template <typename S>
 int gather_launch(const FusedArgs<S> &a, hipStream_t st);
 // resident blocks per CU of the stencil kernel with TJ = tj
 template <typename S>
-int fused_max_blocks_per_cu(int tj, int pf, int *blocks);
+int fused_max_blocks_per_cu(int tj, int pf, bool a64, int *blocks);
 // the same for the two-cells-per-lane kernel (a3d_pair.hip)
 template <typename S>
 int pair_launch(const FusedArgs<S> &a, int tj, hipStream_t st);

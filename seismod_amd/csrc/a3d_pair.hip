@@ -96,7 +96,7 @@ __device__ __forceinline__ unsigned mx_off(const FusedArgs<S> &a, int sx, unsign
 template <typename S>
 __device__ __forceinline__ unsigned my_off(const FusedArgs<S> &a, int i, int sy, int k)
 {
-    return a.my + ((unsigned)i * 2u * a.np + sy) * a.nz2 + k;
+    return a.my + ((unsigned)i * 2u * a.np + sy) * a.pz + k;
 }
 template <typename S>
 __device__ __forceinline__ unsigned mz_off(const FusedArgs<S> &a, int i, int j, int sz)
@@ -142,7 +142,7 @@ a3d_pair_step(FusedArgs<S> a)
 
     const int lane = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    const unsigned plane = (unsigned)a.ny2 * a.nz2;
+    const unsigned plane = (unsigned)a.ny2 * a.pz;
     const int nx2 = a.nx2, ny2 = a.ny2, nz2 = a.nz2, np = a.np;
     const float *ty = a.tab + a.toff_y, *tz = a.tab + a.toff_z;
 
@@ -165,9 +165,9 @@ a3d_pair_step(FusedArgs<S> a)
         L.own1 = jv && L.k0 + 1 < nz2 && m < 31;
         const int jc = min(L.j, ny2 - 1);
         const int kp = min(L.k0, nz2 - 2);    // pair start, clamped to the last pair
-        L.o = (unsigned)jc * nz2 + kp;
-        L.o_jp = (unsigned)min(jc + 1, ny2 - 1) * nz2 + kp;
-        L.o_jm = (unsigned)max(jc - 1, 0) * nz2 + kp;
+        L.o = (unsigned)jc * a.pz + kp;
+        L.o_jp = (unsigned)min(jc + 1, ny2 - 1) * a.pz + kp;
+        L.o_jm = (unsigned)max(jc - 1, 0) * a.pz + kp;
         // y CPML of this row (velocity: rows j < ny2-1; pressure: all rows)
         // and of the row below (wave 0's halo row)
         const int syv = jv && L.j < ny2 - 1 ? slab_vel(L.j, ny2, np) : -1;

@@ -142,13 +142,15 @@ class A3dConfig:
     storage: str = "f32"
     naive: bool = False
     sround: bool = False      # BF16/FP16: stochastic rounding of stored values
+    pad64: bool = False       # device rows padded to 64 cells (cell64 variants)
 
     def params(self) -> L.A3dParams:
         return L.A3dParams(self.nx, self.ny, self.nz, self.np, self.dx, self.dy, self.dz,
                            self.dt, self.vmax, self.pml_order, self.Rc, self.kappa_max,
                            self.alpha_max, self.f0, _STORAGE[self.storage],
                            (L.SM_A3D_NAIVE if self.naive else 0)
-                           | (L.SM_A3D_SROUND if self.sround else 0))
+                           | (L.SM_A3D_SROUND if self.sround else 0)
+                           | (L.SM_A3D_PAD64 if self.pad64 else 0))
 
     @property
     def shape(self) -> tuple[int, int, int]:
@@ -227,18 +229,20 @@ class Acoustic3D:
         check(lib.sm_a3d_get_launch(self.handle, C.byref(tj), C.byref(ch)), "sm_a3d_get_launch")
         return tj.value, ch.value
 
-    VARIANTS = {"cell": L.SM_A3D_CELL, "pair": L.SM_A3D_PAIR, "cell2": L.SM_A3D_CELL2}
+    VARIANTS = {"cell": L.SM_A3D_CELL, "pair": L.SM_A3D_PAIR, "cell2": L.SM_A3D_CELL2,
+                "cell64": L.SM_A3D_CELL64, "cell64x2": L.SM_A3D_CELL64X2}
 
     def set_variant(self, variant: str) -> None:
-        """Fused kernel variant: "cell" (one cell per lane), "cell2" (the same,
-        loading two planes ahead) or "pair" (two z-adjacent cells per lane);
-        results are bit-identical."""
+        """Fused kernel variant: "cell" (one cell per lane, 63-cell z tiles),
+        "cell2" (the same, loading two planes ahead), "pair" (two z-adjacent
+        cells per lane), "cell64"/"cell64x2" (64-cell z tiles aligned to the
+        device rows, one/two planes ahead); results are bit-identical."""
         check(lib.sm_a3d_set_variant(self.handle, self.VARIANTS[variant]), "sm_a3d_set_variant")
 
     def variant(self) -> str:
         v = C.c_int()
         check(lib.sm_a3d_get_variant(self.handle, C.byref(v)), "sm_a3d_get_variant")
-        return {0: "cell", 1: "pair", 2: "cell2"}.get(v.value, "none")
+        return {0: "cell", 1: "pair", 2: "cell2", 3: "cell64", 4: "cell64x2"}.get(v.value, "none")
 
     def set_timing(self, on: bool) -> None:
         check(lib.sm_a3d_set_timing(self.handle, 1 if on else 0), "sm_a3d_set_timing")

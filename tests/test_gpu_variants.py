@@ -1,5 +1,6 @@
-"""GPU tests of the 3D kernel variants: two cells per lane ("pair") and
-one cell per lane with loads two planes ahead ("cell2").
+"""GPU tests of the 3D kernel variants: two cells per lane ("pair"), one
+cell per lane with loads two planes ahead ("cell2"), and 64-cell z tiles
+aligned to the device rows ("cell64", "cell64x2").
 
 It performs the same per-cell arithmetic as the one-cell-per-lane variant,
 only with a different mapping of cells to lanes and memory accesses, so
@@ -29,13 +30,14 @@ def ids(c):
     return f"{c['nx']}x{c['ny']}x{c['nz']}_np{c['np_']}"
 
 
-def run(ctx, d, storage, variant, launch=None, sround=False, split=None):
+def run(ctx, d, storage, variant, launch=None, sround=False, split=None, pad64=None):
     from seismod_amd.solver import A3dConfig, Acoustic3D, source_functions
     c = d["cfg"]
     cfg = A3dConfig(nx=c["nx"], ny=c["ny"], nz=c["nz"], np=c["np"], dx=c["dx"], dy=c["dy"],
                     dz=c["dz"], dt=c["dt"], f0=c["f0"], vmax=c["vmax"], pml_order=c["pml_order"],
                     Rc=c["Rc"], kappa_max=c["kappa_max"], alpha_max=c["alpha_max"],
-                    storage=storage, sround=sround)
+                    storage=storage, sround=sround,
+                    pad64=variant.startswith("cell64") if pad64 is None else pad64)
     s = Acoustic3D(ctx, cfg, d["vp"], d["rho"])
     s.set_variant(variant)
     if launch is not None:
@@ -51,7 +53,7 @@ def run(ctx, d, storage, variant, launch=None, sround=False, split=None):
     return tr, f
 
 
-@pytest.mark.parametrize("variant", ["pair", "cell2"])
+@pytest.mark.parametrize("variant", ["pair", "cell2", "cell64", "cell64x2"])
 @pytest.mark.parametrize("storage", ["f32", "bf16", "f16"])
 @pytest.mark.parametrize("case", CASES, ids=ids)
 def test_variant_equals_cell(ctx, case, storage, variant):
@@ -64,7 +66,7 @@ def test_variant_equals_cell(ctx, case, storage, variant):
     assert np.array_equal(f_p, f_c), np.abs(f_p - f_c).max()
 
 
-@pytest.mark.parametrize("variant", ["pair", "cell2"])
+@pytest.mark.parametrize("variant", ["pair", "cell2", "cell64", "cell64x2"])
 @pytest.mark.parametrize("launch", [(1, 1), (2, 3), (4, 7), (8, 2), (4, 17)])
 def test_variant_launch_shapes_and_splitting(ctx, launch, variant):
     d = decks.a3d_deck(nx=26, ny=21, nz=68, np_=4, seed=7, nsteps=70)
@@ -73,11 +75,14 @@ def test_variant_launch_shapes_and_splitting(ctx, launch, variant):
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
 
 
+@pytest.mark.parametrize("variant", ["pair", "cell64"])
 @pytest.mark.parametrize("storage", ["bf16", "f16"])
-def test_pair_stochastic_rounding_equals_cell(ctx, storage):
+def test_variant_stochastic_rounding_equals_cell(ctx, storage, variant):
+    # the rounding bits hash the device element index: compare on one layout
     d = decks.a3d_deck(nx=14, ny=12, nz=68, np_=4, seed=8, nsteps=80)
-    a = run(ctx, d, storage, "cell", sround=True)
-    b = run(ctx, d, storage, "pair", sround=True)
+    pad = variant.startswith("cell64")
+    a = run(ctx, d, storage, "cell", sround=True, pad64=pad)
+    b = run(ctx, d, storage, variant, sround=True, pad64=pad)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
@@ -97,3 +102,19 @@ def test_pair_needs_even_z(ctx):
     d = decks.a3d_deck(nx=8, ny=8, nz=7, np_=3, seed=10, nsteps=5)   # nz2 = 13
     with pytest.raises(SeismodError):
         run(ctx, d, "f32", "pair")
+
+
+def test_cell64_needs_padded_rows(ctx):
+    from seismod_amd._lib import SeismodError
+    d = decks.a3d_deck(nx=8, ny=8, nz=10, np_=3, seed=11, nsteps=5)
+    with pytest.raises(SeismodError):
+        run(ctx, d, "f32", "cell64", pad64=False)
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_padded_rows_equal_reference_rows(ctx, storage):
+    """The same kernel on padded and unpadded device rows."""
+    d = decks.a3d_deck(nx=12, ny=9, nz=40, np_=4, seed=12, nsteps=60)
+    a = run(ctx, d, storage, "cell", pad64=False)
+    b = run(ctx, d, storage, "cell", pad64=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
