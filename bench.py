@@ -43,8 +43,8 @@ def parse():
     ap.add_argument("--nsteps", type=int, default=None, help="time steps per shot")
     ap.add_argument("--storage", default="f32", choices=sorted(STATE_BYTES))
     ap.add_argument("--launch", default="auto",
-                    help="rows_per_block,x_chunks, or 'auto' (the autotuner picks it during "
-                         "set-up, before the timed region)")
+                    help="[variant,]rows_per_block,x_chunks, or 'auto' (the autotuner picks "
+                         "them during set-up, before the timed region)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=10)
     return ap.parse_args()
@@ -106,8 +106,11 @@ def main():
         from seismod_amd import autotune
         autotune.tune_a3d(sim, ctx)       # times candidates, then resets the solver
     elif args.launch:
-        tj, ch = (int(x) for x in args.launch.split(","))
-        sim.set_launch(tj, ch)
+        parts = args.launch.split(",")
+        if len(parts) == 3:
+            sim.set_variant(parts[0])
+            parts = parts[1:]
+        sim.set_launch(int(parts[0]), int(parts[1]))
 
     sim.set_receivers(shot.rec_idx, shot.rec_comp, nsteps)
 
