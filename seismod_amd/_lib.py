@@ -22,6 +22,7 @@ SM_A3D_PAIR = 1
 SM_A3D_CELL2 = 2
 SM_A3D_CELL64 = 3
 SM_A3D_CELL64X2 = 4
+SM_A3D_TILE = 5
 
 _size_p = C.POINTER(C.c_size_t)
 _dbl_p = C.POINTER(C.c_double)

@@ -197,12 +197,15 @@ bool uses_fused(const sm_a3d_params &p)
     return true;   // 16-bit storage exists only in the fused kernel
 }
 
-// device row pitch: rows padded to a multiple of 64 cells on request
-// (SM_A3D_PAD64, fused kernels only; the 64-cell-tile variants need it),
-// otherwise the reference layout
+// device row pitch of the fused kernels: rows padded to a multiple of 64
+// cells on request (SM_A3D_PAD64; the CELL64 variants need it), otherwise
+// to a multiple of 8 cells (16 bytes of 16-bit storage; the TILE variant
+// copies rows in 16-byte chunks).  The two-kernel paths keep the reference
+// layout.
 size_t row_pitch(size_t nz2, const sm_a3d_params &p)
 {
-    return (p.flags & SM_A3D_PAD64) && uses_fused(p) ? (nz2 + 63) / 64 * 64 : nz2;
+    if (!uses_fused(p)) return nz2;
+    return (p.flags & SM_A3D_PAD64) ? (nz2 + 63) / 64 * 64 : (nz2 + 7) / 8 * 8;
 }
 
 // reference-layout cell index -> device index
@@ -366,12 +369,14 @@ unsigned step_salt(size_t it)
 // kernel variant properties: planes prefetched, 64-cell aligned z tiles
 int variant_pf(int v) { return v == SM_A3D_CELL2 || v == SM_A3D_CELL64X2 ? 2 : 1; }
 bool variant_a64(int v) { return v == SM_A3D_CELL64 || v == SM_A3D_CELL64X2; }
+// variants with z tiles of 64 cells starting at multiples of 64
+bool variant_z64(int v) { return variant_a64(v) || v == SM_A3D_TILE; }
 
 // z tiles: 62 owned cells from even cells (pair), 64 aligned (cell64), 63
 int z_tiles(const sm_a3d *s)
 {
     if (s->variant == SM_A3D_PAIR) return (s->nz2 + 60) / 62;
-    if (variant_a64(s->variant)) return (s->nz2 + 63) / 64;
+    if (variant_z64(s->variant)) return (s->nz2 + 63) / 64;
     return (s->nz2 + 62) / 63;
 }
 
@@ -526,10 +531,10 @@ int run_fused(sm_a3d *s, size_t nsteps)
         a.gather_row = (n > 0 && it > 0) ? rcv_row(s, it - 1) : -1;
         a.gather_blocks = a.gather_row >= 0 ? (int)((s->nrec + per_gather - 1) / per_gather) : 0;
         if ((rc = tm.mark(st, 2 * n)) != SM_OK) return rc;
-        if ((rc = s->variant == SM_A3D_PAIR
-                      ? pair_launch<S>(a, s->tj, st)
-                      : fused_launch<S>(a, s->tj, variant_pf(s->variant), variant_a64(s->variant),
-                                        st)) != SM_OK)
+        if ((rc = s->variant == SM_A3D_PAIR   ? pair_launch<S>(a, s->tj, st)
+                  : s->variant == SM_A3D_TILE ? tile_launch<S>(a, s->tj, st)
+                                              : fused_launch<S>(a, s->tj, variant_pf(s->variant),
+                                                                variant_a64(s->variant), st)) != SM_OK)
             return fail(rc, "fused launch failed (tj=%d): %s", s->tj,
                         hipGetErrorString(hipGetLastError()));
         if ((rc = tm.mark(st, 2 * n + 1)) != SM_OK) return rc;
@@ -617,21 +622,24 @@ size_t elt_size(int storage)
 int resident_blocks(sm_a3d *s, int tj)
 {
     int per_cu = 0, rc = SM_OK;
-    const bool pair = s->variant == SM_A3D_PAIR;
+    const bool pair = s->variant == SM_A3D_PAIR, tile = s->variant == SM_A3D_TILE;
     const int pf = variant_pf(s->variant);
     const bool a64 = variant_a64(s->variant);
     switch (s->p.storage) {
     case SM_STORE_F32:
-        rc = pair ? pair_max_blocks_per_cu<float>(tj, &per_cu)
-                  : fused_max_blocks_per_cu<float>(tj, pf, a64, &per_cu);
+        rc = pair   ? pair_max_blocks_per_cu<float>(tj, &per_cu)
+             : tile ? tile_max_blocks_per_cu<float>(tj, &per_cu)
+                    : fused_max_blocks_per_cu<float>(tj, pf, a64, &per_cu);
         break;
     case SM_STORE_BF16:
-        rc = pair ? pair_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu)
-                  : fused_max_blocks_per_cu<__hip_bfloat16>(tj, pf, a64, &per_cu);
+        rc = pair   ? pair_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu)
+             : tile ? tile_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu)
+                    : fused_max_blocks_per_cu<__hip_bfloat16>(tj, pf, a64, &per_cu);
         break;
     default:
-        rc = pair ? pair_max_blocks_per_cu<__half>(tj, &per_cu)
-                  : fused_max_blocks_per_cu<__half>(tj, pf, a64, &per_cu);
+        rc = pair   ? pair_max_blocks_per_cu<__half>(tj, &per_cu)
+             : tile ? tile_max_blocks_per_cu<__half>(tj, &per_cu)
+                    : fused_max_blocks_per_cu<__half>(tj, pf, a64, &per_cu);
         break;
     }
     if (rc != SM_OK || per_cu < 1) per_cu = 1;
@@ -961,11 +969,13 @@ int sm_a3d_set_variant(sm_a3d *s, int variant)
 {
     SM_CHECK(s, SM_ERR_ARG, "sm_a3d_set_variant: NULL solver");
     SM_CHECK(s->fused, SM_ERR_STATE, "sm_a3d_set_variant: only the fused kernel has variants");
-    SM_CHECK(variant >= SM_A3D_CELL && variant <= SM_A3D_CELL64X2, SM_ERR_ARG,
+    SM_CHECK(variant >= SM_A3D_CELL && variant <= SM_A3D_TILE, SM_ERR_ARG,
              "sm_a3d_set_variant: unknown variant %d", variant);
     SM_CHECK(!variant_a64(variant) || s->pz % 64 == 0, SM_ERR_ARG,
              "sm_a3d_set_variant: the 64-cell-tile variants need rows padded to 64 cells "
              "(SM_A3D_PAD64 at creation)");
+    SM_CHECK(variant != SM_A3D_TILE || s->pz % 8 == 0, SM_ERR_ARG,
+             "sm_a3d_set_variant: the tile kernel needs a row pitch of a multiple of 8 cells");
     SM_CHECK(variant != SM_A3D_PAIR || (s->nz2 % 2 == 0 && s->nz2 >= 4), SM_ERR_ARG,
              "sm_a3d_set_variant: the pair kernel needs an even padded z extent (nz2 = %d)", s->nz2);
     if (variant == s->variant) return SM_OK;

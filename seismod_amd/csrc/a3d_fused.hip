@@ -49,12 +49,6 @@ struct Col {
     float Pkp, rkp, Pkm, rkm, vzkm, mPzm;
 };
 
-// field q (0 P, 1 vx, 2 vy, 3 vz) of cell o: element q*nnn + o of in/out;
-// material: rho at o, K*dt at nnn + o
-#define F_VX 1u
-#define F_VY 2u
-#define F_VZ 3u
-
 // constants of one lane for one work item
 struct Lane {
     unsigned o_jk, o_jp, o_kp, o_jm;   // in-plane offsets of the cell and its neighbours
@@ -120,11 +114,6 @@ __device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, i
     if (L.szp >= 0) c.mvz = Store<S>::ld(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, a.sMp);
 }
 
-// z-halo constants of a 64-cell tile (uniform): CPML of the velocity at k0-1
-struct Halo {
-    float bh, ah, ikh;
-};
-
 // one plane: prefetch plane i+PF into fill, update plane i from cur
 template <typename S, int TJ, int PF, bool A64>
 __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L, int w, int i,
@@ -133,10 +122,10 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
                                            float (*s_vy)[TJ][64], const Halo &hz)
 {
     if (i + PF < i1) {
-        const XPlane xn = a.xpl[i + PF];
+        const XPlane xn = xplane(a.xpl, i + PF);
         load_col<S, A64>(a, L, w, i + PF, plane, xn.sxv, xn.sxp, fill);
     }
-    const XPlane x = a.xpl[i];
+    const XPlane x = xplane(a.xpl, i);
     const unsigned o = (unsigned)i * plane + L.o_jk;
     const unsigned np2 = 2u * a.np;
 
@@ -345,7 +334,7 @@ a3d_fused_step(FusedArgs<S> a)
         A.Pk = A.rk = A.Pkp = A.rkp = A.Pkm = A.rkm = A.vzkm = A.mPzm = 0.f;
         B = A;
         {
-            const XPlane xs = a.xpl[ist];
+            const XPlane xs = xplane(a.xpl, ist);
             load_col<S, A64>(a, L, w, ist, plane, xs.sxv, xs.sxp, A);
         }
         if constexpr (PF == 1) {
@@ -358,7 +347,7 @@ a3d_fused_step(FusedArgs<S> a)
             // three register sets in rotation: plane i in use, i+1 and i+2 in flight
             Col C = B;   // B still holds the zeroed template (its loads follow)
             if (ist + 1 < i1) {
-                const XPlane xs = a.xpl[ist + 1];
+                const XPlane xs = xplane(a.xpl, ist + 1);
                 load_col<S, A64>(a, L, w, ist + 1, plane, xs.sxv, xs.sxp, B);
             }
             for (int i = ist; i < i1; i += 3) {

@@ -22,10 +22,37 @@ struct XPlane {
     int pad[6];
 };
 
+// Plane record i through the constant address space: the record is
+// read-only for the whole launch and i is wave-uniform (readfirstlane makes
+// that visible to the compiler), so this compiles to one scalar load
+// (s_load_dwordx16) instead of vector loads and a wait on them.
+__device__ __forceinline__ XPlane xplane(const XPlane *p, int i)
+{
+    i = __builtin_amdgcn_readfirstlane(i);
+    const __attribute__((address_space(4))) int *w =
+        (const __attribute__((address_space(4))) int *)(p + i);
+    XPlane x;
+    x.bh = __int_as_float(w[0]); x.ah = __int_as_float(w[1]); x.ikh = __int_as_float(w[2]);
+    x.b = __int_as_float(w[3]); x.a = __int_as_float(w[4]); x.ik = __int_as_float(w[5]);
+    x.sxv = w[6]; x.sxp = w[7]; x.q0 = w[8]; x.q1 = w[9];
+    return x;
+}
+
+// field q (0 P, 1 vx, 2 vy, 3 vz) of cell o: element q*nnn + o of in/out;
+// material: rho at o, K*dt at nnn + o
+#define F_VX 1u
+#define F_VY 2u
+#define F_VZ 3u
+
+// z-halo constants of a 64-cell tile (uniform): CPML of the velocity at k0-1
+struct Halo {
+    float bh, ah, ikh;
+};
+
 template <typename S>
 struct FusedArgs {
     int nx2, ny2, nz2, np;
-    int pz;               // device row pitch along z (cells, a multiple of 64)
+    int pz;               // device row pitch along z (cells, >= nz2)
     int ntk, ntiles, chunk_len;
     int nreal;            // tiles * chunks
     int nitems;           // nreal rounded up to a multiple of 8
@@ -82,5 +109,11 @@ template <typename S>
 int pair_launch(const FusedArgs<S> &a, int tj, hipStream_t st);
 template <typename S>
 int pair_max_blocks_per_cu(int tj, int *blocks);
+// the same for the LDS-staged tile kernel (a3d_tile.hip); the device row
+// pitch must be a multiple of 8 cells
+template <typename S>
+int tile_launch(const FusedArgs<S> &a, int tj, hipStream_t st);
+template <typename S>
+int tile_max_blocks_per_cu(int tj, int *blocks);
 
 }  // namespace sm

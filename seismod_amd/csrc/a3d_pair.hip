@@ -32,10 +32,6 @@
 namespace sm {
 namespace {
 
-#define F_VX 1u
-#define F_VY 2u
-#define F_VZ 3u
-
 // what one lane reads for plane i (two cells), loaded one plane ahead and
 // kept as stored (packed for 16-bit storage) until used
 template <typename S>
@@ -201,7 +197,7 @@ a3d_pair_step(FusedArgs<S> a)
         // one plane: prefetch plane i+1 into nxt, update plane i from cur
         auto plane_step = [&](int i, PCol<S> &cur, PCol<S> &nxt) {
             if (i + 1 < i1) pload<S, TJ>(a, L, w, i + 1, plane, nxt);
-            const XPlane x = a.xpl[i];
+            const XPlane x = xplane(a.xpl, i);
             const unsigned o = (unsigned)i * plane + L.o;   // first cell of the pair
             const float2 Pn_ = Pair<S>::unpack(cur.Pn, a.sP);
             const float2 vx_ = Pair<S>::unpack(cur.vx, a.sV);

@@ -51,8 +51,8 @@ __device__ __forceinline__ unsigned sr_bits(unsigned i, unsigned salt)
 
 template <>
 struct Store<float> {
-    __device__ static inline float ld(const float *p, unsigned i, float) { return ld_at(p, i); }
-    __device__ static inline void st(float *p, unsigned i, float v, float, unsigned, int)
+    __device__ static __forceinline__ float ld(const float *p, unsigned i, float) { return ld_at(p, i); }
+    __device__ static __forceinline__ void st(float *p, unsigned i, float v, float, unsigned, int)
     {
         st_at(p, i, v);
     }
@@ -60,11 +60,11 @@ struct Store<float> {
 
 template <>
 struct Store<__hip_bfloat16> {
-    __device__ static inline float ld(const __hip_bfloat16 *p, unsigned i, float scale)
+    __device__ static __forceinline__ float ld(const __hip_bfloat16 *p, unsigned i, float scale)
     {
         return __bfloat162float(ld_at(p, i)) * scale;
     }
-    __device__ static inline void st(__hip_bfloat16 *p, unsigned i, float v, float inv,
+    __device__ static __forceinline__ void st(__hip_bfloat16 *p, unsigned i, float v, float inv,
                                      unsigned salt, int sr)
     {
         const float x = v * inv;
@@ -80,11 +80,11 @@ struct Store<__hip_bfloat16> {
 
 template <>
 struct Store<__half> {
-    __device__ static inline float ld(const __half *p, unsigned i, float scale)
+    __device__ static __forceinline__ float ld(const __half *p, unsigned i, float scale)
     {
         return __half2float(ld_at(p, i)) * scale;
     }
-    __device__ static inline void st(__half *p, unsigned i, float v, float inv, unsigned salt,
+    __device__ static __forceinline__ void st(__half *p, unsigned i, float v, float inv, unsigned salt,
                                      int sr)
     {
         float x = v * inv;
@@ -102,7 +102,7 @@ struct Store<__half> {
     }
 };
 
-__device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // one velocity component at one cell: v - 2dt/(rho0+rho1) * dP, with the
 // CPML memory variable m (updated in place) where slab is true
@@ -129,16 +129,16 @@ struct Pair;
 template <>
 struct Pair<float> {
     using Raw = float2;
-    __device__ static inline Raw raw(const float *p, unsigned i)
+    __device__ static __forceinline__ Raw raw(const float *p, unsigned i)
     {
         return ld_at((const float2 *)p, i >> 1);
     }
-    __device__ static inline float2 unpack(Raw r, float) { return r; }
-    __device__ static inline float2 ld(const float *p, unsigned i, float)
+    __device__ static __forceinline__ float2 unpack(Raw r, float) { return r; }
+    __device__ static __forceinline__ float2 ld(const float *p, unsigned i, float)
     {
         return ld_at((const float2 *)p, i >> 1);
     }
-    __device__ static inline void st(float *p, unsigned i, float2 v, float, unsigned, int)
+    __device__ static __forceinline__ void st(float *p, unsigned i, float2 v, float, unsigned, int)
     {
         st_at((float2 *)p, i >> 1, v);
     }
@@ -147,20 +147,20 @@ struct Pair<float> {
 template <>
 struct Pair<__hip_bfloat16> {
     using Raw = unsigned;
-    __device__ static inline Raw raw(const __hip_bfloat16 *p, unsigned i)
+    __device__ static __forceinline__ Raw raw(const __hip_bfloat16 *p, unsigned i)
     {
         return ld_at((const unsigned *)p, i >> 1);
     }
-    __device__ static inline float2 unpack(Raw w, float scale)
+    __device__ static __forceinline__ float2 unpack(Raw w, float scale)
     {
         return make_float2(__uint_as_float(w << 16) * scale,
                            __uint_as_float(w & 0xFFFF0000u) * scale);
     }
-    __device__ static inline float2 ld(const __hip_bfloat16 *p, unsigned i, float scale)
+    __device__ static __forceinline__ float2 ld(const __hip_bfloat16 *p, unsigned i, float scale)
     {
         return unpack(raw(p, i), scale);
     }
-    __device__ static inline void st(__hip_bfloat16 *p, unsigned i, float2 v, float inv,
+    __device__ static __forceinline__ void st(__hip_bfloat16 *p, unsigned i, float2 v, float inv,
                                      unsigned salt, int sr)
     {
         unsigned lo, hi;
@@ -179,20 +179,20 @@ struct Pair<__hip_bfloat16> {
 template <>
 struct Pair<__half> {
     using Raw = unsigned;
-    __device__ static inline Raw raw(const __half *p, unsigned i)
+    __device__ static __forceinline__ Raw raw(const __half *p, unsigned i)
     {
         return ld_at((const unsigned *)p, i >> 1);
     }
-    __device__ static inline float2 unpack(Raw w, float scale)
+    __device__ static __forceinline__ float2 unpack(Raw w, float scale)
     {
         const float2 f = __half22float2(__builtin_bit_cast(__half2, w));
         return make_float2(f.x * scale, f.y * scale);
     }
-    __device__ static inline float2 ld(const __half *p, unsigned i, float scale)
+    __device__ static __forceinline__ float2 ld(const __half *p, unsigned i, float scale)
     {
         return unpack(raw(p, i), scale);
     }
-    __device__ static inline void st(__half *p, unsigned i, float2 v, float inv, unsigned salt,
+    __device__ static __forceinline__ void st(__half *p, unsigned i, float2 v, float inv, unsigned salt,
                                      int sr)
     {
         if (sr) {
