@@ -736,7 +736,7 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
     s->sr = (p.flags & SM_A3D_SROUND) && (p.storage == SM_STORE_BF16 || p.storage == SM_STORE_F16);
     // the fused kernel addresses the four state fields of one buffer and
     // the two material fields with 32-bit byte offsets
-    if (s->fused && (4 * nnp * e >= ((size_t)1 << 32) || 2 * nnp * 4 >= ((size_t)1 << 32))) {
+    if (s->fused && (4 * nnp * e > (size_t)0xFFFFFF00u || 2 * nnp * 4 > (size_t)0xFFFFFF00u)) {
         delete s;
         return fail(SM_ERR_ARG, "sm_a3d_create: %zu cells (rows padded to %zu) exceed the fused "
                     "kernel's 32-bit offsets (limit 2^28 cells with FP32 storage, 2^29 with "

@@ -49,8 +49,18 @@ __device__ __forceinline__ unsigned sr_bits(unsigned i, unsigned salt)
     return h;
 }
 
+// Store<S>: ld/st of element i with the field's scale; enc/dec convert
+// between a physical FP32 value and the stored bits (Bits), which buffer
+// loads and stores move.  st is enc followed by the store, so every kernel
+// rounds the same way.
 template <>
 struct Store<float> {
+    using Bits = unsigned;
+    __device__ static __forceinline__ float dec(Bits b, float) { return __uint_as_float(b); }
+    __device__ static __forceinline__ Bits enc(float v, float, unsigned, unsigned, int)
+    {
+        return __float_as_uint(v);
+    }
     __device__ static __forceinline__ float ld(const float *p, unsigned i, float) { return ld_at(p, i); }
     __device__ static __forceinline__ void st(float *p, unsigned i, float v, float, unsigned, int)
     {
@@ -60,6 +70,21 @@ struct Store<float> {
 
 template <>
 struct Store<__hip_bfloat16> {
+    using Bits = unsigned short;
+    __device__ static __forceinline__ float dec(Bits b, float scale)
+    {
+        return __uint_as_float((unsigned)b << 16) * scale;
+    }
+    __device__ static __forceinline__ Bits enc(float v, float inv, unsigned i, unsigned salt, int sr)
+    {
+        const float x = v * inv;
+        if (sr) {
+            // add 16 random bits below the BF16 mantissa, then truncate
+            const unsigned b = __float_as_uint(x) + (sr_bits(i, salt) & 0xFFFFu);
+            return (Bits)(b >> 16);
+        }
+        return __builtin_bit_cast(Bits, __float2bfloat16(x));
+    }
     __device__ static __forceinline__ float ld(const __hip_bfloat16 *p, unsigned i, float scale)
     {
         return __bfloat162float(ld_at(p, i)) * scale;
@@ -67,25 +92,18 @@ struct Store<__hip_bfloat16> {
     __device__ static __forceinline__ void st(__hip_bfloat16 *p, unsigned i, float v, float inv,
                                      unsigned salt, int sr)
     {
-        const float x = v * inv;
-        if (sr) {
-            // add 16 random bits below the BF16 mantissa, then truncate
-            const unsigned b = __float_as_uint(x) + (sr_bits(i, salt) & 0xFFFFu);
-            st_at(p, i, __builtin_bit_cast(__hip_bfloat16, (unsigned short)(b >> 16)));
-        } else {
-            st_at(p, i, __float2bfloat16(x));
-        }
+        st_at((Bits *)p, i, enc(v, inv, i, salt, sr));
     }
 };
 
 template <>
 struct Store<__half> {
-    __device__ static __forceinline__ float ld(const __half *p, unsigned i, float scale)
+    using Bits = unsigned short;
+    __device__ static __forceinline__ float dec(Bits b, float scale)
     {
-        return __half2float(ld_at(p, i)) * scale;
+        return __half2float(__builtin_bit_cast(__half, b)) * scale;
     }
-    __device__ static __forceinline__ void st(__half *p, unsigned i, float v, float inv, unsigned salt,
-                                     int sr)
+    __device__ static __forceinline__ Bits enc(float v, float inv, unsigned i, unsigned salt, int sr)
     {
         float x = v * inv;
         if (sr) {
@@ -98,7 +116,16 @@ struct Store<__half> {
                 x = __uint_as_float(b);
             }
         }
-        st_at(p, i, __float2half(x));
+        return __builtin_bit_cast(Bits, __float2half(x));
+    }
+    __device__ static __forceinline__ float ld(const __half *p, unsigned i, float scale)
+    {
+        return __half2float(ld_at(p, i)) * scale;
+    }
+    __device__ static __forceinline__ void st(__half *p, unsigned i, float v, float inv, unsigned salt,
+                                     int sr)
+    {
+        st_at((Bits *)p, i, enc(v, inv, i, salt, sr));
     }
 };
 

@@ -9,28 +9,31 @@
 //   - a block owns TJ rows (one wave per row along y) by 64 cells (lanes
 //     along z, tiles aligned to 64 cells) and marches along x over a chunk of
 //     planes, as the other variants do (persistent grid, XCD-contiguous
-//     item order, chunks with i0 > 0 recompute vx of plane i0-1).
+//     item order, chunks with i0 > 0 first compute vx of plane i0-1).
 //   - for every plane the block copies what it reads into LDS: P and rho
 //     with one row and one 16-byte chunk of halo on each side, old vy with
 //     the row below, old vz with the chunk to the left, old vx and K*dt.
-//     Each lane copies 16-byte chunks (global_load_dwordx4, ds_write_b128),
+//     Each lane copies 16-byte chunks (buffer_load_dwordx4, ds_write_b128),
 //     so every row of every field crosses the memory pipeline once per
-//     plane, in whole chunks, and all neighbour values come from LDS.  The
-//     other variants issue about twice as many vector-memory instructions
-//     per cell, most of them 4-byte-per-lane loads of rows other waves also
-//     load, which made their speed depend on L2 reuse.
-//   - three plane slots rotate: while plane i is updated from slots i and
-//     i+1, the chunks of plane i+2 (loaded during the previous plane) are
-//     written to the third slot and the loads of plane i+3 are issued.  One
-//     barrier per plane.
+//     plane, in whole chunks, and all neighbour values come from LDS.
+//   - three plane slots rotate: plane i is updated from slots i and i+1
+//     while plane i+2 is written to the third.  Chunk loads and CPML
+//     memory-variable loads are issued two planes ahead of their use, into
+//     two alternating register sets, so HBM latency hides behind two planes
+//     of work.  One barrier per plane.
+//   - every vector-memory access inside the plane loop is unconditional: a
+//     lane with nothing to load or store passes an offset past the end of
+//     its buffer resource (OFF_NONE), for which the hardware returns 0 and
+//     drops stores without touching memory.  The compiler then knows the
+//     exact number of accesses in flight and waits only for the ones a
+//     value needs (vmcnt), instead of draining the queue every plane.
 //   - each wave recomputes new vy of the row below its own and lane 0 new
 //     vz of the cell left of the tile, with the expressions their owners use,
 //     so no values pass between waves within a plane.
-//   - CPML memory variables exist only in the slabs; each lane loads its own
-//     one plane ahead.
 //
 // Requirements: the device row pitch is a multiple of 8 cells (16 bytes of
-// 16-bit storage), so chunks never straddle a row end.
+// 16-bit storage), so chunks never straddle a row end; every buffer is
+// smaller than OFF_NONE bytes (checked at creation).
 #include <cstdint>
 
 #include "a3d_common.hpp"
@@ -42,46 +45,78 @@ namespace sm {
 
 namespace {
 
-// Tile fields of one plane slot, in LDS order.
-enum { T_P = 0, T_R = 1, T_VY = 2, T_VZ = 3, T_VX = 4, T_K = 5, T_NF = 6 };
+constexpr unsigned OFF_NONE = 0xFFFFFF00u;   // past the end of every buffer
+constexpr int RSRC_WORD3 = 0x00020000;       // raw 32-bit buffer on gfx9
+
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+using Rsrc = __amdgpu_buffer_rsrc_t;
+
+__device__ __forceinline__ Rsrc rsrc(const void *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, RSRC_WORD3);
+}
+
+// load / store one element of storage type S at byte offset voff
+template <typename S>
+__device__ __forceinline__ float bld(Rsrc r, unsigned voff, float scale)
+{
+    if constexpr (sizeof(S) == 4)
+        return Store<S>::dec(__builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0), scale);
+    else
+        return Store<S>::dec(__builtin_amdgcn_raw_buffer_load_b16(r, voff, 0, 0), scale);
+}
+template <typename S>
+__device__ __forceinline__ void bst(Rsrc r, unsigned voff, typename Store<S>::Bits b)
+{
+    if constexpr (sizeof(S) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(b, r, voff, 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b16(b, r, voff, 0, 0);
+}
+
+// Tile fields of one plane slot.  The fields of the state buffer come
+// first, those of the material buffer start at a multiple of 64 chunks, so
+// every wave copies from one buffer per round.
+enum { T_P = 0, T_VY = 1, T_VZ = 2, T_VX = 3, T_R = 4, T_K = 5, T_NF = 6 };
 
 // Geometry of the staged tiles for storage type S and TJ rows per block.
-// Rows are relative to j0 (the block's first row), cells to k0 (first cell).
+// Rows are relative to j0 (the block's first row), cells to k0 (first
+// cell).  Chunk c of a slot lives at byte c * 16 of the slot.
 template <typename S, int TJ>
 struct TileShape {
     static constexpr int ES = (int)sizeof(S);
-    static constexpr int esz(int f) { return (f == T_R || f == T_K) ? 4 : ES; }
+    static constexpr int esz(int f) { return f >= T_R ? 4 : ES; }
     static constexpr int cpc(int f) { return 16 / esz(f); }   // cells per 16-byte chunk
-    static constexpr int rows(int f) { return f <= T_R ? TJ + 2 : f == T_VY ? TJ + 1 : TJ; }
-    static constexpr int row0(int f) { return f <= T_VY ? -1 : 0; }
-    static constexpr int c0(int f) { return (f <= T_R || f == T_VZ) ? -cpc(f) : 0; }
+    static constexpr int rows(int f)
+    {
+        return (f == T_P || f == T_R) ? TJ + 2 : f == T_VY ? TJ + 1 : TJ;
+    }
+    static constexpr int row0(int f) { return (f == T_P || f == T_R || f == T_VY) ? -1 : 0; }
+    static constexpr int c0(int f) { return (f == T_P || f == T_R || f == T_VZ) ? -cpc(f) : 0; }
     static constexpr int cells(int f)
     {
-        return f <= T_R ? 64 + 2 * cpc(f) : f == T_VZ ? 64 + cpc(f) : 64;
+        return (f == T_P || f == T_R) ? 64 + 2 * cpc(f) : f == T_VZ ? 64 + cpc(f) : 64;
     }
     static constexpr int row_bytes(int f) { return cells(f) * esz(f); }
     static constexpr int cpr(int f) { return row_bytes(f) / 16; }   // chunks per row
     static constexpr int chunks(int f) { return rows(f) * cpr(f); }
-    static constexpr int chunk0(int f) { return f == 0 ? 0 : chunk0(f - 1) + chunks(f - 1); }
-    static constexpr int off(int f) { return f == 0 ? 0 : off(f - 1) + rows(f - 1) * row_bytes(f - 1); }
-    static constexpr int total_chunks = chunk0(T_NF - 1) + chunks(T_NF - 1);
-    static constexpr int slot_bytes = off(T_NF - 1) + rows(T_NF - 1) * row_bytes(T_NF - 1);
+    static constexpr int state_chunks = chunks(T_P) + chunks(T_VY) + chunks(T_VZ) + chunks(T_VX);
+    static constexpr int MAT0 = (state_chunks + 63) / 64 * 64;
+    static constexpr int chunk0(int f)
+    {
+        return f == T_P ? 0 : f == T_VY ? chunks(T_P) : f == T_VZ ? chunks(T_P) + chunks(T_VY)
+             : f == T_VX ? chunks(T_P) + chunks(T_VY) + chunks(T_VZ)
+             : f == T_R ? MAT0 : MAT0 + chunks(T_R);
+    }
+    static constexpr int off(int f) { return chunk0(f) * 16; }
+    static constexpr int total_chunks = MAT0 + chunks(T_R) + chunks(T_K);
     static constexpr int NT = 64 * TJ;                                // threads per block
     static constexpr int NR = (total_chunks + NT - 1) / NT;          // chunks per thread
-};
-
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-
-// One staged 16-byte chunk of a lane: its address in plane 0, bytes per
-// plane, and its byte offset within a slot (base == nullptr: no chunk).
-struct Chunk {
-    const char *base;
-    unsigned pstride;
-    unsigned lds;
+    static constexpr int slot_bytes = NR * NT * 16;                   // every thread's chunks
 };
 
 // Per-item constants of one lane (the cell (j, kk) it owns, clamped copies,
-// slab coordinates, CPML coefficients and LDS offsets of its reads).
+// slab coordinates, CPML coefficients, tile coordinates of its reads).
 struct TLane {
     int j, kk, kc;
     bool own;
@@ -90,51 +125,11 @@ struct TLane {
     unsigned mz_v, mz_p, mz_h;           // z-slab offsets within a plane
     float ybh, yah, yikh, yb, ya, yik, hbh, hah, hikh;
     float zbh, zah, zikh, zb, za, zik;
-    // tile coordinates of the lane's reads: rows (relative to j0, wave-
-    // uniform) of the cell, j+1 and j-1; columns (relative to k0) of the
-    // cell and k+1 (per lane) and of k0-1 (uniform)
+    // rows (relative to j0, wave-uniform) of the cell, j+1 and j-1;
+    // columns (relative to k0) of the cell and k+1 (per lane), of k0-1
     int tc, tjp, tjm;
     int cc, ckp, ckm;
 };
-
-// LDS byte offset within a slot of field F at tile row t, column c (the
-// field is a template argument so the geometry folds to constants)
-template <typename TS, int F>
-__device__ __forceinline__ unsigned tof(int t, int c)
-{
-    constexpr int O = TS::off(F), R0 = TS::row0(F), RB = TS::row_bytes(F), C0 = TS::c0(F),
-                  E = TS::esz(F);
-    return (unsigned)(O + (t - R0) * RB + (c - C0) * E);
-}
-
-// The chunk c of a slot if it belongs to field F: its address in plane 0,
-// bytes per plane and LDS offset.  Tile rows are clamped to the grid;
-// chunks past the row end (no cell of the grid) and before k = 0 (halo of
-// the first tile, never read) are clamped into the row.
-template <typename S, typename TS, int F>
-__device__ __forceinline__ void chunk_of(const FusedArgs<S> &a, int c, int j0, int k0,
-                                         unsigned plane, Chunk &ck)
-{
-    constexpr int C0 = TS::chunk0(F), NC = TS::chunks(F), CPR = TS::cpr(F), CPC = TS::cpc(F),
-                  R0 = TS::row0(F), Z0 = TS::c0(F), E = TS::esz(F), O = TS::off(F),
-                  RB = TS::row_bytes(F);
-    if (c < C0 || c >= C0 + NC) return;
-    const int q = c - C0;
-    const int row = q / CPR, cc = q - row * CPR;
-    const int jr = min(max(j0 + R0 + row, 0), a.ny2 - 1);
-    int cs = k0 + Z0 + cc * CPC;
-    cs = cs < 0 ? 0 : cs >= a.pz ? a.pz - CPC : cs;
-    const char *fb;
-    if constexpr (F == T_R) fb = (const char *)a.mat;
-    else if constexpr (F == T_K) fb = (const char *)(a.mat + a.nnn);
-    else {
-        constexpr unsigned qf = F == T_P ? 0u : F == T_VX ? F_VX : F == T_VY ? F_VY : F_VZ;
-        fb = (const char *)(a.in + (size_t)qf * a.nnn);
-    }
-    ck.base = fb + ((size_t)jr * a.pz + cs) * E;
-    ck.pstride = plane * (unsigned)E;
-    ck.lds = (unsigned)(O + row * RB + cc * 16);
-}
 
 // CPML memory variables one lane reads for one plane (0 outside the slabs)
 struct Psi {
@@ -143,6 +138,20 @@ struct Psi {
     float mPzm;            // velocity set (z) at k0-1 (lane 0)
     float mvx, mvy, mvz;   // pressure set at the own cell
 };
+
+// the buffers of one launch
+struct Bufs {
+    Rsrc in, mat, out, mi, mo, mp;
+};
+
+// LDS byte offset within a slot of field F at tile row t, column c
+template <typename TS, int F>
+__device__ __forceinline__ unsigned tof(int t, int c)
+{
+    constexpr int O = TS::off(F), R0 = TS::row0(F), RB = TS::row_bytes(F), C0 = TS::c0(F),
+                  E = TS::esz(F);
+    return (unsigned)(O + (t - R0) * RB + (c - C0) * E);
+}
 
 template <typename S>
 __device__ __forceinline__ float lds_state(const char *slot, unsigned off, float scale)
@@ -154,145 +163,201 @@ __device__ __forceinline__ float lds_f32(const char *slot, unsigned off)
     return *(const float *)(slot + off);
 }
 
+// byte offset (within its buffer, plane 0) of chunk c if it belongs to
+// field F.  Tile rows are clamped to the grid; chunks past the row end (no
+// cell of the grid) and before k = 0 (halo of the first tile, never read)
+// are clamped into the row.
+template <typename S, typename TS, int F>
+__device__ __forceinline__ void chunk_of(const FusedArgs<S> &a, int c, int j0, int k0, unsigned &off)
+{
+    constexpr int C0 = TS::chunk0(F), NC = TS::chunks(F), CPR = TS::cpr(F), CPC = TS::cpc(F),
+                  R0 = TS::row0(F), Z0 = TS::c0(F), E = TS::esz(F);
+    if (c < C0 || c >= C0 + NC) return;
+    const int q = c - C0;
+    const int row = q / CPR, cc = q - row * CPR;
+    const int jr = min(max(j0 + R0 + row, 0), a.ny2 - 1);
+    int cs = k0 + Z0 + cc * CPC;
+    cs = cs < 0 ? 0 : cs >= a.pz ? a.pz - CPC : cs;
+    unsigned fo = 0;   // field offset within its buffer, elements
+    if constexpr (F == T_K) fo = a.nnn;
+    else if constexpr (F == T_VX) fo = F_VX * a.nnn;
+    else if constexpr (F == T_VY) fo = F_VY * a.nnn;
+    else if constexpr (F == T_VZ) fo = F_VZ * a.nnn;
+    off = (fo + (unsigned)jr * a.pz + (unsigned)cs) * (unsigned)E;
+}
+
+// the chunk loads of plane p into v (OFF_NONE when p is not needed)
+template <typename TS, int NR>
+__device__ __forceinline__ void stage_issue(const Bufs &B, const unsigned (&coff)[NR], int w,
+                                            unsigned plane, int p, int pmax, int plast,
+                                            u32x4 (&v)[NR])
+{
+    const unsigned pc = (unsigned)min(p, plast);
+    const bool need = p <= pmax;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const bool mat = r * TS::NT + w * 64 >= TS::MAT0;   // wave-uniform
+        const unsigned ps = plane * (mat ? 4u : (unsigned)TS::ES);
+        const unsigned vo = need && coff[r] != OFF_NONE ? coff[r] + pc * ps : OFF_NONE;
+        v[r] = __builtin_amdgcn_raw_buffer_load_b128(mat ? B.mat : B.in, vo, 0, 0);
+    }
+}
+
+template <typename TS, int NR>
+__device__ __forceinline__ void stage_write(char *slot, int tid, const u32x4 (&v)[NR])
+{
+#pragma unroll
+    for (int r = 0; r < NR; ++r) *(u32x4 *)(slot + (r * TS::NT + tid) * 16) = v[r];
+}
+
 template <typename S>
-__device__ __forceinline__ void load_psi(const FusedArgs<S> &a, const TLane &L, int i,
-                                         unsigned plane, Psi &m)
+__device__ __forceinline__ void psi_issue(const FusedArgs<S> &a, const Bufs &B, const TLane &L, int p,
+                                          bool need, unsigned plane, Psi &m)
+{
+    const XPlane x = xplane(a.xpl, min(p, a.nx2 - 1));
+    const unsigned np2 = 2u * a.np;
+    const unsigned yrow = (unsigned)p * np2;
+    const unsigned zpl = a.mz + (unsigned)p * a.ny2 * np2;
+    constexpr unsigned E = sizeof(S);
+    auto o = [&](bool ok, unsigned idx) { return need && ok ? idx * E : OFF_NONE; };
+    m.mPx = bld<S>(B.mi, o(x.sxv >= 0, (unsigned)x.sxv * plane + L.o_jk), a.sMv);
+    m.mPy = bld<S>(B.mi, o(L.syv >= 0, a.my + (yrow + L.syv) * a.pz + L.kc), a.sMv);
+    m.mPz = bld<S>(B.mi, o(L.szv >= 0, zpl + L.mz_v), a.sMv);
+    m.mPym = bld<S>(B.mi, o(L.syh >= 0, a.my + (yrow + L.syh) * a.pz + L.kc), a.sMv);
+    m.mPzm = bld<S>(B.mi, o(L.szh >= 0, zpl + L.mz_h), a.sMv);
+    m.mvx = bld<S>(B.mp, o(x.sxp >= 0, (unsigned)x.sxp * plane + L.o_jk), a.sMp);
+    m.mvy = bld<S>(B.mp, o(L.syp >= 0, a.my + (yrow + L.syp) * a.pz + L.kc), a.sMp);
+    m.mvz = bld<S>(B.mp, o(L.szp >= 0, zpl + L.mz_p), a.sMp);
+}
+
+// vx of plane i only (the plane before a chunk's first plane)
+template <typename S, typename TS>
+__device__ __forceinline__ float tile_vx(const FusedArgs<S> &a, const TLane &L, int i,
+                                         const char *cs, const char *ns, const Psi &m)
 {
     const XPlane x = xplane(a.xpl, i);
-    const unsigned np2 = 2u * a.np;
-    const unsigned yrow = (unsigned)i * np2;
-    const unsigned zpl = a.mz + (unsigned)i * a.ny2 * np2;
-    m.mPx = x.sxv >= 0 ? Store<S>::ld(a.mem_in, (unsigned)x.sxv * plane + L.o_jk, a.sMv) : 0.f;
-    m.mPy = L.syv >= 0 ? Store<S>::ld(a.mem_in, a.my + (yrow + L.syv) * a.pz + L.kc, a.sMv) : 0.f;
-    m.mPz = L.szv >= 0 ? Store<S>::ld(a.mem_in, zpl + L.mz_v, a.sMv) : 0.f;
-    m.mPym = L.syh >= 0 ? Store<S>::ld(a.mem_in, a.my + (yrow + L.syh) * a.pz + L.kc, a.sMv) : 0.f;
-    m.mPzm = L.szh >= 0 ? Store<S>::ld(a.mem_in, zpl + L.mz_h, a.sMv) : 0.f;
-    m.mvx = x.sxp >= 0 ? Store<S>::ld(a.memp, (unsigned)x.sxp * plane + L.o_jk, a.sMp) : 0.f;
-    m.mvy = L.syp >= 0 ? Store<S>::ld(a.memp, a.my + (yrow + L.syp) * a.pz + L.kc, a.sMp) : 0.f;
-    m.mvz = L.szp >= 0 ? Store<S>::ld(a.memp, zpl + L.mz_p, a.sMp) : 0.f;
-}
-
-// issue the chunk loads of plane p (clamped to the last plane)
-template <int NR>
-__device__ __forceinline__ void stage_issue(const Chunk (&ck)[NR], int p, u32x4 (&v)[NR])
-{
-#pragma unroll
-    for (int r = 0; r < NR; ++r)
-        if (ck[r].base) v[r] = *(const u32x4 *)(ck[r].base + (size_t)p * ck[r].pstride);
-}
-
-template <int NR>
-__device__ __forceinline__ void stage_write(const Chunk (&ck)[NR], char *slot, const u32x4 (&v)[NR])
-{
-#pragma unroll
-    for (int r = 0; r < NR; ++r)
-        if (ck[r].base) *(u32x4 *)(slot + ck[r].lds) = v[r];
+    const unsigned p_c = tof<TS, T_P>(L.tc, L.cc), r_c_ = tof<TS, T_R>(L.tc, L.cc);
+    float mx = m.mPx;
+    return vel_update(lds_state<S>(cs, tof<TS, T_VX>(L.tc, L.cc), a.sV), lds_state<S>(cs, p_c, a.sP),
+                      lds_state<S>(ns, p_c, a.sP), lds_f32(cs, r_c_), lds_f32(ns, r_c_), a.idx, a.two_dt,
+                      x.sxv >= 0, x.bh, x.ah, x.ikh, mx);
 }
 
 // update plane i from slot cs (plane i) and ns (plane i+1); same expressions
 // and operand order as the other variants' plane update
 template <typename S, typename TS>
-__device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const TLane &L, int i, int i0,
+__device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B, const TLane &L, int i,
                                            unsigned plane, const char *cs, const char *ns,
                                            const Psi &m, const Halo &hz, float &vx_prev)
 {
-    const unsigned p_c = tof<TS, T_P>(L.tc, L.cc), r_c_ = tof<TS, T_R>(L.tc, L.cc);
+    constexpr unsigned E = sizeof(S);
     const XPlane x = xplane(a.xpl, i);
     const unsigned o = (unsigned)i * plane + L.o_jk;
     const unsigned np2 = 2u * a.np;
+    const unsigned p_c = tof<TS, T_P>(L.tc, L.cc), r_c_ = tof<TS, T_R>(L.tc, L.cc);
     const float P_c = lds_state<S>(cs, p_c, a.sP);
     const float r_c = lds_f32(cs, r_c_);
 
     // vx at (i, j, k), a_iso_3d.c:605-619 (the next plane clamped to the last)
     float mx = m.mPx;
     const float vxn = vel_update(lds_state<S>(cs, tof<TS, T_VX>(L.tc, L.cc), a.sV), P_c,
-                                 lds_state<S>(ns, p_c, a.sP), r_c, lds_f32(ns, r_c_), a.idx, a.two_dt, x.sxv >= 0, x.bh, x.ah,
-                                 x.ikh, mx);
-    if (i < i0) {   // chunk start: only vx of the plane below is needed
-        vx_prev = vxn;
-        return;
-    }
+                                 lds_state<S>(ns, p_c, a.sP), r_c, lds_f32(ns, r_c_), a.idx,
+                                 a.two_dt, x.sxv >= 0, x.bh, x.ah, x.ikh, mx);
     // vy, vz at (i, j, k), a_iso_3d.c:621-651
     float my = m.mPy, mz = m.mPz;
     const float vyn = vel_update(lds_state<S>(cs, tof<TS, T_VY>(L.tc, L.cc), a.sV), P_c,
                                  lds_state<S>(cs, tof<TS, T_P>(L.tjp, L.cc), a.sP), r_c,
-                                 lds_f32(cs, tof<TS, T_R>(L.tjp, L.cc)), a.idy, a.two_dt, L.syv >= 0, L.ybh, L.yah,
-                                 L.yikh, my);
+                                 lds_f32(cs, tof<TS, T_R>(L.tjp, L.cc)), a.idy, a.two_dt, L.syv >= 0,
+                                 L.ybh, L.yah, L.yikh, my);
     const float vzn = vel_update(lds_state<S>(cs, tof<TS, T_VZ>(L.tc, L.cc), a.sV), P_c,
                                  lds_state<S>(cs, tof<TS, T_P>(L.tc, L.ckp), a.sP), r_c,
-                                 lds_f32(cs, tof<TS, T_R>(L.tc, L.ckp)), a.idz, a.two_dt, L.szv >= 0, L.zbh, L.zah,
-                                 L.zikh, mz);
-    if (L.own) {
-        Store<S>::st(a.out, F_VX * a.nnn + o, vxn, a.isV, a.salt, a.sr);
-        Store<S>::st(a.out, F_VY * a.nnn + o, vyn, a.isV, a.salt, a.sr);
-        Store<S>::st(a.out, F_VZ * a.nnn + o, vzn, a.isV, a.salt, a.sr);
-        if (x.sxv >= 0)
-            Store<S>::st(a.mem_out, (unsigned)x.sxv * plane + L.o_jk, mx, a.isMv,
-                         a.salt ^ SALT_MV, a.sr);
-        if (L.syv >= 0)
-            Store<S>::st(a.mem_out, a.my + ((unsigned)i * np2 + L.syv) * a.pz + L.kc, my, a.isMv,
-                         a.salt ^ SALT_MV, a.sr);
-        if (L.szv >= 0)
-            Store<S>::st(a.mem_out, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_v, mz, a.isMv,
-                         a.salt ^ SALT_MV, a.sr);
+                                 lds_f32(cs, tof<TS, T_R>(L.tc, L.ckp)), a.idz, a.two_dt, L.szv >= 0,
+                                 L.zbh, L.zah, L.zikh, mz);
+    // stores: a lane that must not write passes OFF_NONE
+    bst<S>(B.out, L.own ? (F_VX * a.nnn + o) * E : OFF_NONE,
+           Store<S>::enc(vxn, a.isV, F_VX * a.nnn + o, a.salt, a.sr));
+    bst<S>(B.out, L.own ? (F_VY * a.nnn + o) * E : OFF_NONE,
+           Store<S>::enc(vyn, a.isV, F_VY * a.nnn + o, a.salt, a.sr));
+    bst<S>(B.out, L.own ? (F_VZ * a.nnn + o) * E : OFF_NONE,
+           Store<S>::enc(vzn, a.isV, F_VZ * a.nnn + o, a.salt, a.sr));
+    {
+        const unsigned ix = (unsigned)x.sxv * plane + L.o_jk;
+        const unsigned iy = a.my + ((unsigned)i * np2 + L.syv) * a.pz + L.kc;
+        const unsigned iz = a.mz + (unsigned)i * a.ny2 * np2 + L.mz_v;
+        bst<S>(B.mo, L.own && x.sxv >= 0 ? ix * E : OFF_NONE,
+               Store<S>::enc(mx, a.isMv, ix, a.salt ^ SALT_MV, a.sr));
+        bst<S>(B.mo, L.own && L.syv >= 0 ? iy * E : OFF_NONE,
+               Store<S>::enc(my, a.isMv, iy, a.salt ^ SALT_MV, a.sr));
+        bst<S>(B.mo, L.own && L.szv >= 0 ? iz * E : OFF_NONE,
+               Store<S>::enc(mz, a.isMv, iz, a.salt ^ SALT_MV, a.sr));
     }
     // new vy of row j-1 and new vz of k0-1 (owned by other waves / tiles;
     // recomputed here, not stored)
     float mh = m.mPym;
     const float vy_m = vel_update(lds_state<S>(cs, tof<TS, T_VY>(L.tjm, L.cc), a.sV),
                                   lds_state<S>(cs, tof<TS, T_P>(L.tjm, L.cc), a.sP), P_c,
-                                  lds_f32(cs, tof<TS, T_R>(L.tjm, L.cc)), r_c, a.idy, a.two_dt, L.syh >= 0, L.hbh, L.hah,
-                                  L.hikh, mh);
+                                  lds_f32(cs, tof<TS, T_R>(L.tjm, L.cc)), r_c, a.idy, a.two_dt,
+                                  L.syh >= 0, L.hbh, L.hah, L.hikh, mh);
     float vz_m = __shfl_up(vzn, 1);
     {
         float mzh = m.mPzm;
         const float vz_h = vel_update(lds_state<S>(cs, tof<TS, T_VZ>(L.tc, L.ckm), a.sV),
                                       lds_state<S>(cs, tof<TS, T_P>(L.tc, L.ckm), a.sP), P_c,
-                                      lds_f32(cs, tof<TS, T_R>(L.tc, L.ckm)), r_c, a.idz, a.two_dt, L.szh >= 0, hz.bh,
-                                      hz.ah, hz.ikh, mzh);
+                                      lds_f32(cs, tof<TS, T_R>(L.tc, L.ckm)), r_c, a.idz, a.two_dt,
+                                      L.szh >= 0, hz.bh, hz.ah, hz.ikh, mzh);
         if (threadIdx.x == 0) vz_m = vz_h;
     }
 
-    // pressure at (i, j, k), a_iso_3d.c:659-683
-    if (L.own) {
-        float Pn = P_c;
-        if (i >= 1 && L.j >= 1 && L.kk >= 1) {
-            float dvdx = (vxn - vx_prev) * a.idx;
-            float dvdy = (vyn - vy_m) * a.idy;
-            float dvdz = (vzn - vz_m) * a.idz;
-            if (x.sxp >= 0) {
-                const float mm = x.b * m.mvx + x.a * dvdx;
-                Store<S>::st(a.memp, (unsigned)x.sxp * plane + L.o_jk, mm, a.isMp,
-                             a.salt ^ SALT_MP, a.sr);
-                dvdx = dvdx * x.ik + mm;
-            }
-            if (L.syp >= 0) {
-                const float mm = L.yb * m.mvy + L.ya * dvdy;
-                Store<S>::st(a.memp, a.my + ((unsigned)i * np2 + L.syp) * a.pz + L.kc, mm, a.isMp,
-                             a.salt ^ SALT_MP, a.sr);
-                dvdy = dvdy * L.yik + mm;
-            }
-            if (L.szp >= 0) {
-                const float mm = L.zb * m.mvz + L.za * dvdz;
-                Store<S>::st(a.memp, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p, mm, a.isMp,
-                             a.salt ^ SALT_MP, a.sr);
-                dvdz = dvdz * L.zik + mm;
-            }
-            Pn = P_c - lds_f32(cs, tof<TS, T_K>(L.tc, L.cc)) * (dvdx + dvdy + dvdz);
-        }
-        // sources of this plane (a_iso_3d.c:689-693), in source order
-        for (int q = x.q0; q < x.q1; ++q)
-            if ((unsigned)a.src[q] == o && a.step < a.src[2 * a.nsrc + q])
-                Pn += a.src_inc[a.src[a.nsrc + q] + a.step];
-        Store<S>::st(a.out, o, Pn, a.isP, a.salt, a.sr);
+    // pressure at (i, j, k), a_iso_3d.c:659-683; memory variables of the
+    // pressure set are written only where the reference updates them
+    const bool upd = L.own && i >= 1 && L.j >= 1 && L.kk >= 1;
+    float dvdx = (vxn - vx_prev) * a.idx;
+    float dvdy = (vyn - vy_m) * a.idy;
+    float dvdz = (vzn - vz_m) * a.idz;
+    const unsigned jx = (unsigned)x.sxp * plane + L.o_jk;
+    const unsigned jy = a.my + ((unsigned)i * np2 + L.syp) * a.pz + L.kc;
+    const unsigned jz = a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p;
+    float mmx = 0.f, mmy = 0.f, mmz = 0.f;
+    if (x.sxp >= 0) {
+        mmx = x.b * m.mvx + x.a * dvdx;
+        dvdx = dvdx * x.ik + mmx;
     }
+    if (L.syp >= 0) {
+        mmy = L.yb * m.mvy + L.ya * dvdy;
+        dvdy = dvdy * L.yik + mmy;
+    }
+    if (L.szp >= 0) {
+        mmz = L.zb * m.mvz + L.za * dvdz;
+        dvdz = dvdz * L.zik + mmz;
+    }
+    bst<S>(B.mp, upd && x.sxp >= 0 ? jx * E : OFF_NONE,
+           Store<S>::enc(mmx, a.isMp, jx, a.salt ^ SALT_MP, a.sr));
+    bst<S>(B.mp, upd && L.syp >= 0 ? jy * E : OFF_NONE,
+           Store<S>::enc(mmy, a.isMp, jy, a.salt ^ SALT_MP, a.sr));
+    bst<S>(B.mp, upd && L.szp >= 0 ? jz * E : OFF_NONE,
+           Store<S>::enc(mmz, a.isMp, jz, a.salt ^ SALT_MP, a.sr));
+    float Pn = upd ? P_c - lds_f32(cs, tof<TS, T_K>(L.tc, L.cc)) * (dvdx + dvdy + dvdz) : P_c;
+    if (x.q0 < x.q1) {
+        // sources of this plane (a_iso_3d.c:689-693), in source order, from
+        // the source table through scalar loads
+        typedef const __attribute__((address_space(4))) int64_t *C64;
+        typedef const __attribute__((address_space(4))) float *CF;
+        for (int q = x.q0; q < x.q1; ++q) {
+            const int64_t cell = ((C64)a.src)[q];
+            const int64_t off = ((C64)a.src)[a.nsrc + q];
+            const int64_t len = ((C64)a.src)[2 * a.nsrc + q];
+            if (L.own && (unsigned)cell == o && a.step < len) Pn += ((CF)a.src_inc)[off + a.step];
+        }
+    }
+    bst<S>(B.out, L.own ? o * E : OFF_NONE, Store<S>::enc(Pn, a.isP, o, a.salt, a.sr));
     vx_prev = vxn;
 }
 
 }  // namespace
 
+// at least 4 waves per SIMD (at most 128 registers per lane)
 template <typename S, int TJ>
-__global__ void __launch_bounds__(64 * TJ)
+__global__ void __launch_bounds__(64 * TJ, 4)
 a3d_tile_step(FusedArgs<S> a)
 {
     using TS = TileShape<S, TJ>;
@@ -318,6 +383,15 @@ a3d_tile_step(FusedArgs<S> a)
     const int tid = w * 64 + lane;
     const unsigned plane = (unsigned)a.ny2 * a.pz;
     const unsigned np2 = 2u * a.np;
+    const unsigned state_bytes = 4u * a.nnn * (unsigned)sizeof(S);
+    const unsigned slab_elems = a.mz + (unsigned)a.nx2 * a.ny2 * np2;
+    Bufs B;
+    B.in = rsrc(a.in, state_bytes);
+    B.out = rsrc(a.out, state_bytes);
+    B.mat = rsrc(a.mat, 2u * a.nnn * 4u);
+    B.mi = rsrc(a.mem_in, slab_elems * (unsigned)sizeof(S));
+    B.mo = rsrc(a.mem_out, slab_elems * (unsigned)sizeof(S));
+    B.mp = rsrc(a.memp, slab_elems * (unsigned)sizeof(S));
 
     for (int item = blockIdx.x; item < a.nitems; item += a.nblocks) {
         const int logical = (item % 8) * (a.nitems / 8) + item / 8;
@@ -351,6 +425,8 @@ a3d_tile_step(FusedArgs<S> a)
         L.mz_v = (unsigned)jc * np2 + (L.szv < 0 ? 0 : L.szv);
         L.mz_p = (unsigned)jc * np2 + (L.szp < 0 ? 0 : L.szp);
         L.mz_h = (unsigned)jc * np2 + (L.szh < 0 ? 0 : L.szh);
+        L.tc = jc - j0; L.tjp = jp - j0; L.tjm = jm - j0;
+        L.cc = L.kc - k0; L.ckp = kp - k0; L.ckm = km - k0;
         Halo hz{0.f, 1.f, 1.f};
         {
             // CPML tables a, b, 1/k, ah, bh, 1/kh along y (row and row below)
@@ -368,62 +444,89 @@ a3d_tile_step(FusedArgs<S> a)
             const int khn = max(min(hk, nz2 - 2), 0);
             hz.bh = tz[4 * nz2 + khn]; hz.ah = tz[3 * nz2 + khn]; hz.ikh = tz[5 * nz2 + khn];
         }
-        // ---- tile coordinates of the lane's reads
-        L.tc = jc - j0; L.tjp = jp - j0; L.tjm = jm - j0;
-        L.cc = L.kc - k0; L.ckp = kp - k0; L.ckm = km - k0;
-        // ---- the 16-byte chunks this thread copies
-        Chunk ck[NR];
+        // ---- the 16-byte chunks this thread copies (offsets in plane 0)
+        unsigned coff[NR];
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             const int c = r * TS::NT + tid;
-            ck[r].base = nullptr;
-            ck[r].pstride = 0;
-            ck[r].lds = 0;
-            chunk_of<S, TS, T_P>(a, c, j0, k0, plane, ck[r]);
-            chunk_of<S, TS, T_R>(a, c, j0, k0, plane, ck[r]);
-            chunk_of<S, TS, T_VY>(a, c, j0, k0, plane, ck[r]);
-            chunk_of<S, TS, T_VZ>(a, c, j0, k0, plane, ck[r]);
-            chunk_of<S, TS, T_VX>(a, c, j0, k0, plane, ck[r]);
-            chunk_of<S, TS, T_K>(a, c, j0, k0, plane, ck[r]);
+            coff[r] = OFF_NONE;
+            chunk_of<S, TS, T_P>(a, c, j0, k0, coff[r]);
+            chunk_of<S, TS, T_VY>(a, c, j0, k0, coff[r]);
+            chunk_of<S, TS, T_VZ>(a, c, j0, k0, coff[r]);
+            chunk_of<S, TS, T_VX>(a, c, j0, k0, coff[r]);
+            chunk_of<S, TS, T_R>(a, c, j0, k0, coff[r]);
+            chunk_of<S, TS, T_K>(a, c, j0, k0, coff[r]);
         }
 
         const int i0 = chunk * a.chunk_len;
         const int i1 = min(a.nx2, i0 + a.chunk_len);
         if (i0 >= a.nx2) continue;   // uniform; the host never makes empty chunks
-        const int ist = i0 > 0 ? i0 - 1 : 0;
         const int plast = a.nx2 - 1;
+        const int pmax = i1;   // last plane read: i1 (clamped to plast) for vx of plane i1-1
 
-        // ---- prologue: planes ist and ist+1 into their slots, plane ist+2
-        // in flight; slot of plane p is p mod 3
-        int s0 = ist % 3;                     // slot of plane i
+        // ---- prologue (synchronous): planes i0, i0+1 and, for a chunk
+        // after the first, i0-1 into their slots (plane p in slot p mod 3);
+        // vx of plane i0-1
+        int s0 = i0 % 3;                      // slot of plane i
         int s1 = s0 == 2 ? 0 : s0 + 1;        // plane i+1
-        int s2 = s1 == 2 ? 0 : s1 + 1;        // plane i+2
-        u32x4 v[NR];
-        stage_issue<NR>(ck, ist, v);
-        stage_write<NR>(ck, s_tile + s0 * TS::slot_bytes, v);
-        stage_issue<NR>(ck, min(ist + 1, plast), v);
-        stage_write<NR>(ck, s_tile + s1 * TS::slot_bytes, v);
-        if (ist + 2 <= i1) stage_issue<NR>(ck, min(ist + 2, plast), v);
-        Psi mc;
-        load_psi<S>(a, L, ist, plane, mc);
-        __syncthreads();
-
+        int s2 = s1 == 2 ? 0 : s1 + 1;        // plane i+2 (and i-1)
+        char *const base = s_tile;
+        u32x4 va[NR], vb[NR];
+        Psi m0, m1;
         float vx_prev = 0.f;
-        for (int i = ist; i < i1; ++i) {
-            Psi mn{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            if (i + 1 < i1) load_psi<S>(a, L, i + 1, plane, mn);
-            tile_plane<S, TS>(a, L, i, i0, plane, s_tile + s0 * TS::slot_bytes, s_tile + s1 * TS::slot_bytes, mc,
-                          hz, vx_prev);
-            // plane i+2 into the slot plane i-1 used (read last before the
-            // previous barrier); it is read from the next plane on
-            if (i + 2 <= i1) stage_write<NR>(ck, s_tile + s2 * TS::slot_bytes, v);
+        stage_issue<TS, NR>(B, coff, w, plane, i0, pmax, plast, va);
+        stage_issue<TS, NR>(B, coff, w, plane, i0 + 1, pmax, plast, vb);
+        stage_write<TS, NR>(base + s0 * TS::slot_bytes, tid, va);
+        stage_write<TS, NR>(base + s1 * TS::slot_bytes, tid, vb);
+        if (i0 > 0) {
+            stage_issue<TS, NR>(B, coff, w, plane, i0 - 1, pmax, plast, va);
+            psi_issue<S>(a, B, L, i0 - 1, true, plane, m0);
+            stage_write<TS, NR>(base + s2 * TS::slot_bytes, tid, va);
             __syncthreads();
-            if (i + 3 <= i1) stage_issue<NR>(ck, min(i + 3, plast), v);
-            mc = mn;
-            const int t = s0;
-            s0 = s1;
-            s1 = s2;
-            s2 = t;
+            vx_prev = tile_vx<S, TS>(a, L, i0 - 1, base + s2 * TS::slot_bytes,
+                                     base + s0 * TS::slot_bytes, m0);
+        }
+        __syncthreads();   // slots of planes i0, i0+1 written; slot s2 free
+        stage_issue<TS, NR>(B, coff, w, plane, i0 + 2, pmax, plast, va);
+        psi_issue<S>(a, B, L, i0, true, plane, m0);
+        stage_issue<TS, NR>(B, coff, w, plane, i0 + 3, pmax, plast, vb);
+        psi_issue<S>(a, B, L, i0 + 1, i0 + 1 < i1, plane, m1);
+
+        // ---- plane loop, two planes per trip: at plane i the register
+        // sets hold the chunks of planes i+2 (va) and i+3 (vb), and the
+        // memory variables of planes i (m0) and i+1 (m1)
+        for (int i = i0; i < i1; i += 2) {
+            {
+                Psi m2;
+                psi_issue<S>(a, B, L, i + 2, i + 2 < i1, plane, m2);
+                tile_plane<S, TS>(a, B, L, i, plane, base + s0 * TS::slot_bytes,
+                                  base + s1 * TS::slot_bytes, m0, hz, vx_prev);
+                stage_write<TS, NR>(base + s2 * TS::slot_bytes, tid, va);
+                __syncthreads();
+                stage_issue<TS, NR>(B, coff, w, plane, i + 4, pmax, plast, va);
+                m0 = m1;
+                m1 = m2;
+                const int t = s0;
+                s0 = s1;
+                s1 = s2;
+                s2 = t;
+            }
+            if (i + 1 >= i1) break;
+            {
+                Psi m2;
+                psi_issue<S>(a, B, L, i + 3, i + 3 < i1, plane, m2);
+                tile_plane<S, TS>(a, B, L, i + 1, plane, base + s0 * TS::slot_bytes,
+                                  base + s1 * TS::slot_bytes, m0, hz, vx_prev);
+                stage_write<TS, NR>(base + s2 * TS::slot_bytes, tid, vb);
+                __syncthreads();
+                stage_issue<TS, NR>(B, coff, w, plane, i + 5, pmax, plast, vb);
+                m0 = m1;
+                m1 = m2;
+                const int t = s0;
+                s0 = s1;
+                s1 = s2;
+                s2 = t;
+            }
         }
     }
 }
