@@ -72,15 +72,25 @@ def cpu_baseline(shot, nsteps_sample: int) -> dict:
                       f"as a_iso_3d.c:585 (3 threads velocity, 1 pressure)"}
 
 
-def load_traffic(name: str):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+# kernel launched by each variant of the fused 3D path
+KERNELS = {"cell": "a3d_fused_step", "cell2": "a3d_fused_step", "cell64": "a3d_fused_step",
+           "cell64x2": "a3d_fused_step", "pair": "a3d_pair_step", "tile": "a3d_tile_step"}
+
+
+def load_traffic(name: str, variant: str, rows: int):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary of
+    this workload, if it was measured on the same kernel variant and rows
+    per block (None otherwise)."""
     p = ROOT / "profiles" / f"{name}.json"
     if not p.exists():
         return None
     try:
-        return json.loads(p.read_text()).get("hbm_bytes_per_launch")
+        d = json.loads(p.read_text())
     except (OSError, ValueError):
         return None
+    if d.get("variant") != variant or d.get("rows_per_block") != rows:
+        return None
+    return d.get("hbm_bytes_per_launch")
 
 
 def main():
@@ -174,12 +184,12 @@ def main():
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
         "roofline": {
             "bound": "hbm",
-            "kernel": "a3d_fused_step",
+            "kernel": KERNELS.get(variant, variant),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic(kname),
+            "traffic": load_traffic(kname, variant, tj),
             "algorithmic_bytes_per_cell": round(alg_b, 3),
             "kernel_ms_avg": round(avg_ms, 5),
             "launches_timed": launches,

@@ -242,7 +242,7 @@ class Acoustic3D:
     def variant(self) -> str:
         v = C.c_int()
         check(lib.sm_a3d_get_variant(self.handle, C.byref(v)), "sm_a3d_get_variant")
-        return {0: "cell", 1: "pair", 2: "cell2", 3: "cell64", 4: "cell64x2"}.get(v.value, "none")
+        return {n: k for k, n in self.VARIANTS.items()}.get(v.value, "none")
 
     def set_timing(self, on: bool) -> None:
         check(lib.sm_a3d_set_timing(self.handle, 1 if on else 0), "sm_a3d_set_timing")
