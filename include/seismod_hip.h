@@ -89,9 +89,13 @@ int sm_ctx_mem_info(sm_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
  * `writes` FP32 arrays of n values (known traffic 4n(reads+writes) bytes),
  * each lane moving `vec` (1, 2 or 4) consecutive values per array; returns
  * the mean time of `iters` launches.  (reads, writes) is one of (1,1)
- * (2,1) (3,1) (4,2) (6,4) (8,4) (10,4) (12,8) (1,2) (5,5).  No counterpart
- * in the reference: it scales HBM counters and measures achievable
- * bandwidth. */
+ * (2,1) (3,1) (4,2) (6,4) (8,4) (10,4) (12,8) (1,2) (5,5).  Optional bits
+ * in `vec`: SM_PROBE_NT (non-temporal accesses), SM_PROBE_TILES (one
+ * 256-element tile per workgroup instead of a persistent grid of 16
+ * workgroups per CU).  No counterpart in the reference: it scales HBM
+ * counters and measures achievable bandwidth. */
+#define SM_PROBE_NT 16
+#define SM_PROBE_TILES 32
 int sm_probe_stream(sm_ctx *ctx, size_t n, int reads, int writes, int vec, int iters,
                     double *ms_per_iter);
 

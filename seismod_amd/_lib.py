@@ -23,6 +23,8 @@ SM_A3D_CELL2 = 2
 SM_A3D_CELL64 = 3
 SM_A3D_CELL64X2 = 4
 SM_A3D_TILE = 5
+SM_PROBE_NT = 16
+SM_PROBE_TILES = 32
 
 _size_p = C.POINTER(C.c_size_t)
 _dbl_p = C.POINTER(C.c_double)
