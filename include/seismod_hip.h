@@ -189,14 +189,18 @@ int sm_a3d_get_launch(const sm_a3d *s, int *rows_per_block, int *x_chunks);
  * the device rows (whole cache lines per access), loads one / two planes
  * ahead; SM_A3D_TILE, z tiles of 64 cells aligned to the device rows whose
  * fields are staged plane by plane through LDS in 16-byte chunks (every
- * row read once per plane).  The variants compute the same values bit for
- * bit; setting one resets the launch shape to its default. */
+ * row read once per plane); SM_A3D_TILE_SPLIT, the same with the CPML
+ * work of each direction compiled in only for tiles and ranges of planes
+ * that touch that direction's absorbing layer.  The variants compute the
+ * same values bit for bit; setting one resets the launch shape to its
+ * default. */
 #define SM_A3D_CELL 0
 #define SM_A3D_PAIR 1
 #define SM_A3D_CELL2 2
 #define SM_A3D_CELL64 3
 #define SM_A3D_CELL64X2 4
 #define SM_A3D_TILE 5
+#define SM_A3D_TILE_SPLIT 6
 int sm_a3d_set_variant(sm_a3d *s, int variant);
 int sm_a3d_get_variant(const sm_a3d *s, int *variant);
 /* Per-launch timing: when on, every step kernel launch is bracketed by HIP

@@ -23,6 +23,7 @@ SM_A3D_CELL2 = 2
 SM_A3D_CELL64 = 3
 SM_A3D_CELL64X2 = 4
 SM_A3D_TILE = 5
+SM_A3D_TILE_SPLIT = 6
 SM_PROBE_NT = 16
 SM_PROBE_TILES = 32
 

@@ -370,7 +370,8 @@ unsigned step_salt(size_t it)
 int variant_pf(int v) { return v == SM_A3D_CELL2 || v == SM_A3D_CELL64X2 ? 2 : 1; }
 bool variant_a64(int v) { return v == SM_A3D_CELL64 || v == SM_A3D_CELL64X2; }
 // variants with z tiles of 64 cells starting at multiples of 64
-bool variant_z64(int v) { return variant_a64(v) || v == SM_A3D_TILE; }
+bool variant_tile(int v) { return v == SM_A3D_TILE || v == SM_A3D_TILE_SPLIT; }
+bool variant_z64(int v) { return variant_a64(v) || variant_tile(v); }
 
 // z tiles: 62 owned cells from even cells (pair), 64 aligned (cell64), 63
 int z_tiles(const sm_a3d *s)
@@ -532,7 +533,7 @@ int run_fused(sm_a3d *s, size_t nsteps)
         a.gather_blocks = a.gather_row >= 0 ? (int)((s->nrec + per_gather - 1) / per_gather) : 0;
         if ((rc = tm.mark(st, 2 * n)) != SM_OK) return rc;
         if ((rc = s->variant == SM_A3D_PAIR   ? pair_launch<S>(a, s->tj, st)
-                  : s->variant == SM_A3D_TILE ? tile_launch<S>(a, s->tj, st)
+                  : variant_tile(s->variant) ? tile_launch<S>(a, s->tj, s->variant == SM_A3D_TILE_SPLIT, st)
                                               : fused_launch<S>(a, s->tj, variant_pf(s->variant),
                                                                 variant_a64(s->variant), st)) != SM_OK)
             return fail(rc, "fused launch failed (tj=%d): %s", s->tj,
@@ -622,23 +623,24 @@ size_t elt_size(int storage)
 int resident_blocks(sm_a3d *s, int tj)
 {
     int per_cu = 0, rc = SM_OK;
-    const bool pair = s->variant == SM_A3D_PAIR, tile = s->variant == SM_A3D_TILE;
+    const bool pair = s->variant == SM_A3D_PAIR, tile = variant_tile(s->variant);
+    const bool split = s->variant == SM_A3D_TILE_SPLIT;
     const int pf = variant_pf(s->variant);
     const bool a64 = variant_a64(s->variant);
     switch (s->p.storage) {
     case SM_STORE_F32:
         rc = pair   ? pair_max_blocks_per_cu<float>(tj, &per_cu)
-             : tile ? tile_max_blocks_per_cu<float>(tj, &per_cu)
+             : tile ? tile_max_blocks_per_cu<float>(tj, split, &per_cu)
                     : fused_max_blocks_per_cu<float>(tj, pf, a64, &per_cu);
         break;
     case SM_STORE_BF16:
         rc = pair   ? pair_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu)
-             : tile ? tile_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu)
+             : tile ? tile_max_blocks_per_cu<__hip_bfloat16>(tj, split, &per_cu)
                     : fused_max_blocks_per_cu<__hip_bfloat16>(tj, pf, a64, &per_cu);
         break;
     default:
         rc = pair   ? pair_max_blocks_per_cu<__half>(tj, &per_cu)
-             : tile ? tile_max_blocks_per_cu<__half>(tj, &per_cu)
+             : tile ? tile_max_blocks_per_cu<__half>(tj, split, &per_cu)
                     : fused_max_blocks_per_cu<__half>(tj, pf, a64, &per_cu);
         break;
     }
@@ -969,12 +971,12 @@ int sm_a3d_set_variant(sm_a3d *s, int variant)
 {
     SM_CHECK(s, SM_ERR_ARG, "sm_a3d_set_variant: NULL solver");
     SM_CHECK(s->fused, SM_ERR_STATE, "sm_a3d_set_variant: only the fused kernel has variants");
-    SM_CHECK(variant >= SM_A3D_CELL && variant <= SM_A3D_TILE, SM_ERR_ARG,
+    SM_CHECK(variant >= SM_A3D_CELL && variant <= SM_A3D_TILE_SPLIT, SM_ERR_ARG,
              "sm_a3d_set_variant: unknown variant %d", variant);
     SM_CHECK(!variant_a64(variant) || s->pz % 64 == 0, SM_ERR_ARG,
              "sm_a3d_set_variant: the 64-cell-tile variants need rows padded to 64 cells "
              "(SM_A3D_PAD64 at creation)");
-    SM_CHECK(variant != SM_A3D_TILE || s->pz % 8 == 0, SM_ERR_ARG,
+    SM_CHECK(!variant_tile(variant) || s->pz % 8 == 0, SM_ERR_ARG,
              "sm_a3d_set_variant: the tile kernel needs a row pitch of a multiple of 8 cells");
     SM_CHECK(variant != SM_A3D_PAIR || (s->nz2 % 2 == 0 && s->nz2 >= 4), SM_ERR_ARG,
              "sm_a3d_set_variant: the pair kernel needs an even padded z extent (nz2 = %d)", s->nz2);

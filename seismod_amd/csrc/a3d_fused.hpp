@@ -109,11 +109,12 @@ template <typename S>
 int pair_launch(const FusedArgs<S> &a, int tj, hipStream_t st);
 template <typename S>
 int pair_max_blocks_per_cu(int tj, int *blocks);
-// the same for the LDS-staged tile kernel (a3d_tile.hip); the device row
-// pitch must be a multiple of 8 cells
+// the same for the LDS-staged tile kernel (a3d_tile.hip; split: slab
+// groups decided per tile and range of planes); the device row pitch must
+// be a multiple of 8 cells
 template <typename S>
-int tile_launch(const FusedArgs<S> &a, int tj, hipStream_t st);
+int tile_launch(const FusedArgs<S> &a, int tj, bool split, hipStream_t st);
 template <typename S>
-int tile_max_blocks_per_cu(int tj, int *blocks);
+int tile_max_blocks_per_cu(int tj, bool split, int *blocks);
 
 }  // namespace sm

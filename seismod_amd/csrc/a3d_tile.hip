@@ -210,7 +210,10 @@ __device__ __forceinline__ void stage_write(char *slot, int tid, const u32x4 (&v
     for (int r = 0; r < NR; ++r) *(u32x4 *)(slot + (r * TS::NT + tid) * 16) = v[r];
 }
 
-template <typename S>
+// Memory variables of plane p.  X, Y, Z: whether the x, y, z groups can
+// be non-zero here (set outside the plane loops, so every loop issues a
+// fixed set of loads); a group that is off is 0 without any load.
+template <typename S, bool X, bool Y, bool Z>
 __device__ __forceinline__ void psi_issue(const FusedArgs<S> &a, const Bufs &B, const TLane &L, int p,
                                           bool need, unsigned plane, Psi &m)
 {
@@ -220,14 +223,21 @@ __device__ __forceinline__ void psi_issue(const FusedArgs<S> &a, const Bufs &B, 
     const unsigned zpl = a.mz + (unsigned)p * a.ny2 * np2;
     constexpr unsigned E = sizeof(S);
     auto o = [&](bool ok, unsigned idx) { return need && ok ? idx * E : OFF_NONE; };
-    m.mPx = bld<S>(B.mi, o(x.sxv >= 0, (unsigned)x.sxv * plane + L.o_jk), a.sMv);
-    m.mPy = bld<S>(B.mi, o(L.syv >= 0, a.my + (yrow + L.syv) * a.pz + L.kc), a.sMv);
-    m.mPz = bld<S>(B.mi, o(L.szv >= 0, zpl + L.mz_v), a.sMv);
-    m.mPym = bld<S>(B.mi, o(L.syh >= 0, a.my + (yrow + L.syh) * a.pz + L.kc), a.sMv);
-    m.mPzm = bld<S>(B.mi, o(L.szh >= 0, zpl + L.mz_h), a.sMv);
-    m.mvx = bld<S>(B.mp, o(x.sxp >= 0, (unsigned)x.sxp * plane + L.o_jk), a.sMp);
-    m.mvy = bld<S>(B.mp, o(L.syp >= 0, a.my + (yrow + L.syp) * a.pz + L.kc), a.sMp);
-    m.mvz = bld<S>(B.mp, o(L.szp >= 0, zpl + L.mz_p), a.sMp);
+    m.mPx = m.mvx = m.mPy = m.mvy = m.mPym = m.mPz = m.mvz = m.mPzm = 0.f;
+    if constexpr (X) {
+        m.mPx = bld<S>(B.mi, o(x.sxv >= 0, (unsigned)x.sxv * plane + L.o_jk), a.sMv);
+        m.mvx = bld<S>(B.mp, o(x.sxp >= 0, (unsigned)x.sxp * plane + L.o_jk), a.sMp);
+    }
+    if constexpr (Y) {
+        m.mPy = bld<S>(B.mi, o(L.syv >= 0, a.my + (yrow + L.syv) * a.pz + L.kc), a.sMv);
+        m.mPym = bld<S>(B.mi, o(L.syh >= 0, a.my + (yrow + L.syh) * a.pz + L.kc), a.sMv);
+        m.mvy = bld<S>(B.mp, o(L.syp >= 0, a.my + (yrow + L.syp) * a.pz + L.kc), a.sMp);
+    }
+    if constexpr (Z) {
+        m.mPz = bld<S>(B.mi, o(L.szv >= 0, zpl + L.mz_v), a.sMv);
+        m.mPzm = bld<S>(B.mi, o(L.szh >= 0, zpl + L.mz_h), a.sMv);
+        m.mvz = bld<S>(B.mp, o(L.szp >= 0, zpl + L.mz_p), a.sMp);
+    }
 }
 
 // vx of plane i only (the plane before a chunk's first plane)
@@ -244,8 +254,10 @@ __device__ __forceinline__ float tile_vx(const FusedArgs<S> &a, const TLane &L, 
 }
 
 // update plane i from slot cs (plane i) and ns (plane i+1); same expressions
-// and operand order as the other variants' plane update
-template <typename S, typename TS>
+// and operand order as the other variants' plane update.  X, Y, Z: whether
+// plane i / the tile can have x, y, z slab cells (when off, every slab test
+// of that direction is false here, so its code is left out).
+template <typename S, typename TS, bool X, bool Y, bool Z>
 __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B, const TLane &L, int i,
                                            unsigned plane, const char *cs, const char *ns,
                                            const Psi &m, const Halo &hz, float &vx_prev)
@@ -262,16 +274,16 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
     float mx = m.mPx;
     const float vxn = vel_update(lds_state<S>(cs, tof<TS, T_VX>(L.tc, L.cc), a.sV), P_c,
                                  lds_state<S>(ns, p_c, a.sP), r_c, lds_f32(ns, r_c_), a.idx,
-                                 a.two_dt, x.sxv >= 0, x.bh, x.ah, x.ikh, mx);
+                                 a.two_dt, X && x.sxv >= 0, x.bh, x.ah, x.ikh, mx);
     // vy, vz at (i, j, k), a_iso_3d.c:621-651
     float my = m.mPy, mz = m.mPz;
     const float vyn = vel_update(lds_state<S>(cs, tof<TS, T_VY>(L.tc, L.cc), a.sV), P_c,
                                  lds_state<S>(cs, tof<TS, T_P>(L.tjp, L.cc), a.sP), r_c,
-                                 lds_f32(cs, tof<TS, T_R>(L.tjp, L.cc)), a.idy, a.two_dt, L.syv >= 0,
+                                 lds_f32(cs, tof<TS, T_R>(L.tjp, L.cc)), a.idy, a.two_dt, Y && L.syv >= 0,
                                  L.ybh, L.yah, L.yikh, my);
     const float vzn = vel_update(lds_state<S>(cs, tof<TS, T_VZ>(L.tc, L.cc), a.sV), P_c,
                                  lds_state<S>(cs, tof<TS, T_P>(L.tc, L.ckp), a.sP), r_c,
-                                 lds_f32(cs, tof<TS, T_R>(L.tc, L.ckp)), a.idz, a.two_dt, L.szv >= 0,
+                                 lds_f32(cs, tof<TS, T_R>(L.tc, L.ckp)), a.idz, a.two_dt, Z && L.szv >= 0,
                                  L.zbh, L.zah, L.zikh, mz);
     // stores: a lane that must not write passes OFF_NONE
     bst<S>(B.out, L.own ? (F_VX * a.nnn + o) * E : OFF_NONE,
@@ -284,12 +296,15 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
         const unsigned ix = (unsigned)x.sxv * plane + L.o_jk;
         const unsigned iy = a.my + ((unsigned)i * np2 + L.syv) * a.pz + L.kc;
         const unsigned iz = a.mz + (unsigned)i * a.ny2 * np2 + L.mz_v;
-        bst<S>(B.mo, L.own && x.sxv >= 0 ? ix * E : OFF_NONE,
-               Store<S>::enc(mx, a.isMv, ix, a.salt ^ SALT_MV, a.sr));
-        bst<S>(B.mo, L.own && L.syv >= 0 ? iy * E : OFF_NONE,
-               Store<S>::enc(my, a.isMv, iy, a.salt ^ SALT_MV, a.sr));
-        bst<S>(B.mo, L.own && L.szv >= 0 ? iz * E : OFF_NONE,
-               Store<S>::enc(mz, a.isMv, iz, a.salt ^ SALT_MV, a.sr));
+        if constexpr (X)
+            bst<S>(B.mo, L.own && x.sxv >= 0 ? ix * E : OFF_NONE,
+                   Store<S>::enc(mx, a.isMv, ix, a.salt ^ SALT_MV, a.sr));
+        if constexpr (Y)
+            bst<S>(B.mo, L.own && L.syv >= 0 ? iy * E : OFF_NONE,
+                   Store<S>::enc(my, a.isMv, iy, a.salt ^ SALT_MV, a.sr));
+        if constexpr (Z)
+            bst<S>(B.mo, L.own && L.szv >= 0 ? iz * E : OFF_NONE,
+                   Store<S>::enc(mz, a.isMv, iz, a.salt ^ SALT_MV, a.sr));
     }
     // new vy of row j-1 and new vz of k0-1 (owned by other waves / tiles;
     // recomputed here, not stored)
@@ -297,14 +312,14 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
     const float vy_m = vel_update(lds_state<S>(cs, tof<TS, T_VY>(L.tjm, L.cc), a.sV),
                                   lds_state<S>(cs, tof<TS, T_P>(L.tjm, L.cc), a.sP), P_c,
                                   lds_f32(cs, tof<TS, T_R>(L.tjm, L.cc)), r_c, a.idy, a.two_dt,
-                                  L.syh >= 0, L.hbh, L.hah, L.hikh, mh);
+                                  Y && L.syh >= 0, L.hbh, L.hah, L.hikh, mh);
     float vz_m = __shfl_up(vzn, 1);
     {
         float mzh = m.mPzm;
         const float vz_h = vel_update(lds_state<S>(cs, tof<TS, T_VZ>(L.tc, L.ckm), a.sV),
                                       lds_state<S>(cs, tof<TS, T_P>(L.tc, L.ckm), a.sP), P_c,
                                       lds_f32(cs, tof<TS, T_R>(L.tc, L.ckm)), r_c, a.idz, a.two_dt,
-                                      L.szh >= 0, hz.bh, hz.ah, hz.ikh, mzh);
+                                      Z && L.szh >= 0, hz.bh, hz.ah, hz.ikh, mzh);
         if (threadIdx.x == 0) vz_m = vz_h;
     }
 
@@ -318,24 +333,27 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
     const unsigned jy = a.my + ((unsigned)i * np2 + L.syp) * a.pz + L.kc;
     const unsigned jz = a.mz + (unsigned)i * a.ny2 * np2 + L.mz_p;
     float mmx = 0.f, mmy = 0.f, mmz = 0.f;
-    if (x.sxp >= 0) {
+    if (X && x.sxp >= 0) {
         mmx = x.b * m.mvx + x.a * dvdx;
         dvdx = dvdx * x.ik + mmx;
     }
-    if (L.syp >= 0) {
+    if (Y && L.syp >= 0) {
         mmy = L.yb * m.mvy + L.ya * dvdy;
         dvdy = dvdy * L.yik + mmy;
     }
-    if (L.szp >= 0) {
+    if (Z && L.szp >= 0) {
         mmz = L.zb * m.mvz + L.za * dvdz;
         dvdz = dvdz * L.zik + mmz;
     }
-    bst<S>(B.mp, upd && x.sxp >= 0 ? jx * E : OFF_NONE,
-           Store<S>::enc(mmx, a.isMp, jx, a.salt ^ SALT_MP, a.sr));
-    bst<S>(B.mp, upd && L.syp >= 0 ? jy * E : OFF_NONE,
-           Store<S>::enc(mmy, a.isMp, jy, a.salt ^ SALT_MP, a.sr));
-    bst<S>(B.mp, upd && L.szp >= 0 ? jz * E : OFF_NONE,
-           Store<S>::enc(mmz, a.isMp, jz, a.salt ^ SALT_MP, a.sr));
+    if constexpr (X)
+        bst<S>(B.mp, upd && x.sxp >= 0 ? jx * E : OFF_NONE,
+               Store<S>::enc(mmx, a.isMp, jx, a.salt ^ SALT_MP, a.sr));
+    if constexpr (Y)
+        bst<S>(B.mp, upd && L.syp >= 0 ? jy * E : OFF_NONE,
+               Store<S>::enc(mmy, a.isMp, jy, a.salt ^ SALT_MP, a.sr));
+    if constexpr (Z)
+        bst<S>(B.mp, upd && L.szp >= 0 ? jz * E : OFF_NONE,
+               Store<S>::enc(mmz, a.isMp, jz, a.salt ^ SALT_MP, a.sr));
     float Pn = upd ? P_c - lds_f32(cs, tof<TS, T_K>(L.tc, L.cc)) * (dvdx + dvdy + dvdz) : P_c;
     if (x.q0 < x.q1) {
         // sources of this plane (a_iso_3d.c:689-693), in source order, from
@@ -353,11 +371,120 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
     vx_prev = vxn;
 }
 
+// Loop state of a chunk [i0, i1).  At plane i the register set of the
+// parity of i - i0 holds the chunks of plane i+2, m0 / m1 the memory
+// variables of planes i / i+1; slots s0, s1, s2 hold planes i, i+1 and the
+// free one.
+struct Ring {
+    int s0, s1, s2;
+    Psi m0, m1;
+    float vx_prev;
+};
+
+// one plane of a range: memory variables of plane i+2 issued, plane i
+// updated, plane i+2 (in v) written to the free slot, loads of plane i+4
+// issued into v
+template <typename S, typename TS, bool X, bool Y, bool Z>
+__device__ __forceinline__ void plane_iter(const FusedArgs<S> &a, const Bufs &B, const TLane &L,
+                                           const Halo &hz, const unsigned (&coff)[TS::NR], int w,
+                                           int tid, unsigned plane, int i1, int i, char *base,
+                                           u32x4 (&v)[TS::NR], Ring &R)
+{
+    constexpr int NR = TS::NR;
+    Psi m2;
+    psi_issue<S, X, Y, Z>(a, B, L, i + 2, i + 2 < i1, plane, m2);
+    tile_plane<S, TS, X, Y, Z>(a, B, L, i, plane, base + R.s0 * TS::slot_bytes,
+                               base + R.s1 * TS::slot_bytes, R.m0, hz, R.vx_prev);
+    // plane i+2 into the slot plane i-1 used (read last before the previous
+    // barrier), then the loads of plane i+4 into the same register set
+    stage_write<TS, NR>(base + R.s2 * TS::slot_bytes, tid, v);
+    __syncthreads();
+    stage_issue<TS, NR>(B, coff, w, plane, i + 4, i1, a.nx2 - 1, v);
+    R.m0 = R.m1;
+    R.m1 = m2;
+    const int t = R.s0;
+    R.s0 = R.s1;
+    R.s1 = R.s2;
+    R.s2 = t;
+}
+
+// Planes [ib, ie) of a chunk ending at i1 with the x flag fixed, two per
+// trip: v0 holds the chunks of plane ib+2, v1 those of plane ib+3.
+template <typename S, typename TS, bool X, bool Y, bool Z>
+__device__ __forceinline__ void plane_range(const FusedArgs<S> &a, const Bufs &B, const TLane &L,
+                                            const Halo &hz, const unsigned (&coff)[TS::NR], int w,
+                                            int tid, unsigned plane, int i1, int ib, int ie,
+                                            char *base, u32x4 (&v0)[TS::NR], u32x4 (&v1)[TS::NR],
+                                            Ring &R)
+{
+    for (int i = ib; i < ie; i += 2) {
+        plane_iter<S, TS, X, Y, Z>(a, B, L, hz, coff, w, tid, plane, i1, i, base, v0, R);
+        if (i + 1 >= ie) break;
+        plane_iter<S, TS, X, Y, Z>(a, B, L, hz, coff, w, tid, plane, i1, i + 1, base, v1, R);
+    }
+}
+
+// One work item: planes [i0, i1) of one tile.  Y, Z: the tile has cells
+// (or neighbours it recomputes) in a y / z slab.  SPLIT: the chunk is
+// marched in up to three ranges, x-slab planes at either end (or planes two
+// before them, whose memory variables are loaded ahead) with the x group
+// and the interior without; otherwise in one range with every group on.
+template <typename S, typename TS, bool Y, bool Z, bool SPLIT>
+__device__ __forceinline__ void march(const FusedArgs<S> &a, const Bufs &B, const TLane &L,
+                                      const Halo &hz, const unsigned (&coff)[TS::NR], int w, int tid,
+                                      unsigned plane, int i0, int i1, char *base)
+{
+    constexpr int NR = TS::NR;
+    const int pmax = i1, plast = a.nx2 - 1;
+    // prologue (synchronous): planes i0, i0+1 and, for a chunk after the
+    // first, i0-1 into their slots (plane p in slot p mod 3); vx of plane
+    // i0-1
+    Ring R;
+    R.s0 = i0 % 3;
+    R.s1 = R.s0 == 2 ? 0 : R.s0 + 1;
+    R.s2 = R.s1 == 2 ? 0 : R.s1 + 1;
+    R.vx_prev = 0.f;
+    u32x4 va[NR], vb[NR];
+    stage_issue<TS, NR>(B, coff, w, plane, i0, pmax, plast, va);
+    stage_issue<TS, NR>(B, coff, w, plane, i0 + 1, pmax, plast, vb);
+    stage_write<TS, NR>(base + R.s0 * TS::slot_bytes, tid, va);
+    stage_write<TS, NR>(base + R.s1 * TS::slot_bytes, tid, vb);
+    if (i0 > 0) {
+        stage_issue<TS, NR>(B, coff, w, plane, i0 - 1, pmax, plast, va);
+        psi_issue<S, true, Y, Z>(a, B, L, i0 - 1, true, plane, R.m0);
+        stage_write<TS, NR>(base + R.s2 * TS::slot_bytes, tid, va);
+        __syncthreads();
+        R.vx_prev = tile_vx<S, TS>(a, L, i0 - 1, base + R.s2 * TS::slot_bytes,
+                                   base + R.s0 * TS::slot_bytes, R.m0);
+    }
+    __syncthreads();   // slots of planes i0, i0+1 written; slot s2 free
+    stage_issue<TS, NR>(B, coff, w, plane, i0 + 2, pmax, plast, va);
+    psi_issue<S, true, Y, Z>(a, B, L, i0, true, plane, R.m0);
+    stage_issue<TS, NR>(B, coff, w, plane, i0 + 3, pmax, plast, vb);
+    psi_issue<S, true, Y, Z>(a, B, L, i0 + 1, i0 + 1 < i1, plane, R.m1);
+    // x-slab planes: i < np or i >= nx2-1-np (velocity or pressure set);
+    // the x group is on where plane i or plane i+2 is one.  The interior
+    // range [b1, b2) starts and has a length even in planes from i0, so
+    // every range starts with register set va (the x ranges grow by at
+    // most one plane; the x group is correct anywhere)
+    const int b1 = min(i0 + (max(a.np - i0, 0) + 1) / 2 * 2, i1);
+    const int b2 = b1 + max(min(a.nx2 - 3 - a.np, i1) - b1, 0) / 2 * 2;
+    if constexpr (SPLIT) {
+        plane_range<S, TS, true, Y, Z>(a, B, L, hz, coff, w, tid, plane, i1, i0, b1, base, va, vb, R);
+        plane_range<S, TS, false, Y, Z>(a, B, L, hz, coff, w, tid, plane, i1, b1, b2, base, va, vb, R);
+        plane_range<S, TS, true, Y, Z>(a, B, L, hz, coff, w, tid, plane, i1, b2, i1, base, va, vb, R);
+    } else {
+        plane_range<S, TS, true, Y, Z>(a, B, L, hz, coff, w, tid, plane, i1, i0, i1, base, va, vb, R);
+    }
+}
+
 }  // namespace
 
-// at least 4 waves per SIMD (at most 128 registers per lane)
-template <typename S, int TJ>
-__global__ void __launch_bounds__(64 * TJ, 4)
+// SPLIT: slab groups decided per tile and per range of planes (more code,
+// fewer instructions per interior cell); otherwise one loop with every
+// group, held to at least 4 waves per SIMD (at most 128 registers per lane)
+template <typename S, int TJ, bool SPLIT>
+__global__ void __launch_bounds__(64 * TJ, SPLIT ? 1 : 4)
 a3d_tile_step(FusedArgs<S> a)
 {
     using TS = TileShape<S, TJ>;
@@ -461,93 +588,41 @@ a3d_tile_step(FusedArgs<S> a)
         const int i0 = chunk * a.chunk_len;
         const int i1 = min(a.nx2, i0 + a.chunk_len);
         if (i0 >= a.nx2) continue;   // uniform; the host never makes empty chunks
-        const int plast = a.nx2 - 1;
-        const int pmax = i1;   // last plane read: i1 (clamped to plast) for vx of plane i1-1
-
-        // ---- prologue (synchronous): planes i0, i0+1 and, for a chunk
-        // after the first, i0-1 into their slots (plane p in slot p mod 3);
-        // vx of plane i0-1
-        int s0 = i0 % 3;                      // slot of plane i
-        int s1 = s0 == 2 ? 0 : s0 + 1;        // plane i+1
-        int s2 = s1 == 2 ? 0 : s1 + 1;        // plane i+2 (and i-1)
-        char *const base = s_tile;
-        u32x4 va[NR], vb[NR];
-        Psi m0, m1;
-        float vx_prev = 0.f;
-        stage_issue<TS, NR>(B, coff, w, plane, i0, pmax, plast, va);
-        stage_issue<TS, NR>(B, coff, w, plane, i0 + 1, pmax, plast, vb);
-        stage_write<TS, NR>(base + s0 * TS::slot_bytes, tid, va);
-        stage_write<TS, NR>(base + s1 * TS::slot_bytes, tid, vb);
-        if (i0 > 0) {
-            stage_issue<TS, NR>(B, coff, w, plane, i0 - 1, pmax, plast, va);
-            psi_issue<S>(a, B, L, i0 - 1, true, plane, m0);
-            stage_write<TS, NR>(base + s2 * TS::slot_bytes, tid, va);
-            __syncthreads();
-            vx_prev = tile_vx<S, TS>(a, L, i0 - 1, base + s2 * TS::slot_bytes,
-                                     base + s0 * TS::slot_bytes, m0);
-        }
-        __syncthreads();   // slots of planes i0, i0+1 written; slot s2 free
-        stage_issue<TS, NR>(B, coff, w, plane, i0 + 2, pmax, plast, va);
-        psi_issue<S>(a, B, L, i0, true, plane, m0);
-        stage_issue<TS, NR>(B, coff, w, plane, i0 + 3, pmax, plast, vb);
-        psi_issue<S>(a, B, L, i0 + 1, i0 + 1 < i1, plane, m1);
-
-        // ---- plane loop, two planes per trip: at plane i the register
-        // sets hold the chunks of planes i+2 (va) and i+3 (vb), and the
-        // memory variables of planes i (m0) and i+1 (m1)
-        for (int i = i0; i < i1; i += 2) {
-            {
-                Psi m2;
-                psi_issue<S>(a, B, L, i + 2, i + 2 < i1, plane, m2);
-                tile_plane<S, TS>(a, B, L, i, plane, base + s0 * TS::slot_bytes,
-                                  base + s1 * TS::slot_bytes, m0, hz, vx_prev);
-                stage_write<TS, NR>(base + s2 * TS::slot_bytes, tid, va);
-                __syncthreads();
-                stage_issue<TS, NR>(B, coff, w, plane, i + 4, pmax, plast, va);
-                m0 = m1;
-                m1 = m2;
-                const int t = s0;
-                s0 = s1;
-                s1 = s2;
-                s2 = t;
+        // the tile's rows (and the row below) or cells (and k0-1) in a y / z
+        // slab: rows j <= np or j >= ny2-1-np, cells likewise
+        if constexpr (SPLIT) {
+            const bool ys = j0 <= a.np || j0 + TJ - 1 >= a.ny2 - 1 - a.np;
+            const bool zs = k0 <= a.np || k0 + 63 >= a.nz2 - 1 - a.np;
+            if (ys) {
+                if (zs) march<S, TS, true, true, true>(a, B, L, hz, coff, w, tid, plane, i0, i1, s_tile);
+                else march<S, TS, true, false, true>(a, B, L, hz, coff, w, tid, plane, i0, i1, s_tile);
+            } else {
+                if (zs) march<S, TS, false, true, true>(a, B, L, hz, coff, w, tid, plane, i0, i1, s_tile);
+                else march<S, TS, false, false, true>(a, B, L, hz, coff, w, tid, plane, i0, i1, s_tile);
             }
-            if (i + 1 >= i1) break;
-            {
-                Psi m2;
-                psi_issue<S>(a, B, L, i + 3, i + 3 < i1, plane, m2);
-                tile_plane<S, TS>(a, B, L, i + 1, plane, base + s0 * TS::slot_bytes,
-                                  base + s1 * TS::slot_bytes, m0, hz, vx_prev);
-                stage_write<TS, NR>(base + s2 * TS::slot_bytes, tid, vb);
-                __syncthreads();
-                stage_issue<TS, NR>(B, coff, w, plane, i + 5, pmax, plast, vb);
-                m0 = m1;
-                m1 = m2;
-                const int t = s0;
-                s0 = s1;
-                s1 = s2;
-                s2 = t;
-            }
+        } else {
+            march<S, TS, true, true, false>(a, B, L, hz, coff, w, tid, plane, i0, i1, s_tile);
         }
     }
 }
 
-template <typename S, int TJ>
+template <typename S, int TJ, bool SPLIT>
 static const void *tile_kernel_ptr()
 {
-    return reinterpret_cast<const void *>(&a3d_tile_step<S, TJ>);
+    return reinterpret_cast<const void *>(&a3d_tile_step<S, TJ, SPLIT>);
 }
 
+#define SM_TILE_CASES(X) X(1, false) X(2, false) X(4, false) X(8, false) \
+                         X(1, true) X(2, true) X(4, true) X(8, true)
+
 template <typename S>
-int tile_max_blocks_per_cu(int tj, int *blocks)
+int tile_max_blocks_per_cu(int tj, bool split, int *blocks)
 {
     const void *f = nullptr;
-    switch (tj) {
-    case 1: f = tile_kernel_ptr<S, 1>(); break;
-    case 2: f = tile_kernel_ptr<S, 2>(); break;
-    case 4: f = tile_kernel_ptr<S, 4>(); break;
-    case 8: f = tile_kernel_ptr<S, 8>(); break;
-    default: return SM_ERR_ARG;
-    }
+#define SM_X(T, P) if (tj == T && split == P) f = tile_kernel_ptr<S, T, P>();
+    SM_TILE_CASES(SM_X)
+#undef SM_X
+    if (!f) return SM_ERR_ARG;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * tj, 0) != hipSuccess)
         return SM_ERR_HIP;
@@ -556,24 +631,26 @@ int tile_max_blocks_per_cu(int tj, int *blocks)
 }
 
 template <typename S>
-int tile_launch(const FusedArgs<S> &a, int tj, hipStream_t st)
+int tile_launch(const FusedArgs<S> &a, int tj, bool split, hipStream_t st)
 {
     const int blocks = a.nblocks + a.gather_blocks;
-    switch (tj) {
-    case 1: a3d_tile_step<S, 1><<<blocks, dim3(64, 1), 0, st>>>(a); break;
-    case 2: a3d_tile_step<S, 2><<<blocks, dim3(64, 2), 0, st>>>(a); break;
-    case 4: a3d_tile_step<S, 4><<<blocks, dim3(64, 4), 0, st>>>(a); break;
-    case 8: a3d_tile_step<S, 8><<<blocks, dim3(64, 8), 0, st>>>(a); break;
-    default: return SM_ERR_ARG;
+    bool done = false;
+#define SM_X(T, P)                                                                   \
+    if (!done && tj == T && split == P) {                                            \
+        a3d_tile_step<S, T, P><<<blocks, dim3(64, T), 0, st>>>(a);                   \
+        done = true;                                                                 \
     }
+    SM_TILE_CASES(SM_X)
+#undef SM_X
+    if (!done) return SM_ERR_ARG;
     return hipGetLastError() == hipSuccess ? SM_OK : SM_ERR_HIP;
 }
 
-template int tile_launch<float>(const FusedArgs<float> &, int, hipStream_t);
-template int tile_launch<__hip_bfloat16>(const FusedArgs<__hip_bfloat16> &, int, hipStream_t);
-template int tile_launch<__half>(const FusedArgs<__half> &, int, hipStream_t);
-template int tile_max_blocks_per_cu<float>(int, int *);
-template int tile_max_blocks_per_cu<__hip_bfloat16>(int, int *);
-template int tile_max_blocks_per_cu<__half>(int, int *);
+template int tile_launch<float>(const FusedArgs<float> &, int, bool, hipStream_t);
+template int tile_launch<__hip_bfloat16>(const FusedArgs<__hip_bfloat16> &, int, bool, hipStream_t);
+template int tile_launch<__half>(const FusedArgs<__half> &, int, bool, hipStream_t);
+template int tile_max_blocks_per_cu<float>(int, bool, int *);
+template int tile_max_blocks_per_cu<__hip_bfloat16>(int, bool, int *);
+template int tile_max_blocks_per_cu<__half>(int, bool, int *);
 
 }  // namespace sm

@@ -230,7 +230,8 @@ class Acoustic3D:
         return tj.value, ch.value
 
     VARIANTS = {"cell": L.SM_A3D_CELL, "pair": L.SM_A3D_PAIR, "cell2": L.SM_A3D_CELL2,
-                "cell64": L.SM_A3D_CELL64, "cell64x2": L.SM_A3D_CELL64X2, "tile": L.SM_A3D_TILE}
+                "cell64": L.SM_A3D_CELL64, "cell64x2": L.SM_A3D_CELL64X2, "tile": L.SM_A3D_TILE,
+                "tilesplit": L.SM_A3D_TILE_SPLIT}
 
     def set_variant(self, variant: str) -> None:
         """Fused kernel variant: "cell" (one cell per lane, 63-cell z tiles),

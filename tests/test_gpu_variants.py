@@ -54,7 +54,7 @@ def run(ctx, d, storage, variant, launch=None, sround=False, split=None, pad64=N
     return tr, f
 
 
-@pytest.mark.parametrize("variant", ["pair", "cell2", "cell64", "cell64x2", "tile"])
+@pytest.mark.parametrize("variant", ["pair", "cell2", "cell64", "cell64x2", "tile", "tilesplit"])
 @pytest.mark.parametrize("storage", ["f32", "bf16", "f16"])
 @pytest.mark.parametrize("case", CASES, ids=ids)
 def test_variant_equals_cell(ctx, case, storage, variant):
@@ -67,7 +67,7 @@ def test_variant_equals_cell(ctx, case, storage, variant):
     assert np.array_equal(f_p, f_c), np.abs(f_p - f_c).max()
 
 
-@pytest.mark.parametrize("variant", ["pair", "cell2", "cell64", "cell64x2", "tile"])
+@pytest.mark.parametrize("variant", ["pair", "cell2", "cell64", "cell64x2", "tile", "tilesplit"])
 @pytest.mark.parametrize("launch", [(1, 1), (2, 3), (4, 7), (8, 2), (4, 17)])
 def test_variant_launch_shapes_and_splitting(ctx, launch, variant):
     d = decks.a3d_deck(nx=26, ny=21, nz=68, np_=4, seed=7, nsteps=70)
@@ -76,7 +76,7 @@ def test_variant_launch_shapes_and_splitting(ctx, launch, variant):
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
 
 
-@pytest.mark.parametrize("variant", ["pair", "cell64", "tile"])
+@pytest.mark.parametrize("variant", ["pair", "cell64", "tile", "tilesplit"])
 @pytest.mark.parametrize("storage", ["bf16", "f16"])
 def test_variant_stochastic_rounding_equals_cell(ctx, storage, variant):
     # the rounding bits hash the device element index: compare on one layout
@@ -87,7 +87,7 @@ def test_variant_stochastic_rounding_equals_cell(ctx, storage, variant):
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
-@pytest.mark.parametrize("variant", ["pair", "tile"])
+@pytest.mark.parametrize("variant", ["pair", "tile", "tilesplit"])
 def test_variant_fp32_vs_oracle(ctx, variant):
     d = decks.a3d_deck(nx=11, ny=10, nz=40, np_=5, seed=9, nsteps=100)
     c = d["cfg"]
@@ -102,13 +102,26 @@ def test_variant_fp32_vs_oracle(ctx, variant):
 @pytest.mark.parametrize("storage", ["f32", "bf16"])
 @pytest.mark.parametrize("shape", [(8, 8, 7, 3), (9, 13, 1, 2), (5, 6, 131, 3)],
                          ids=["nz2_13", "nz2_5", "nz2_137"])
-def test_tile_odd_rows_equal_cell(ctx, storage, shape):
+@pytest.mark.parametrize("variant", ["tile", "tilesplit"])
+def test_tile_odd_rows_equal_cell(ctx, storage, shape, variant):
     """Row pitch rounded up from an odd z extent: the chunks past the last
     cell of a row and the halo chunk of the first tile are clamped."""
     nx, ny, nz, np_ = shape
     d = decks.a3d_deck(nx=nx, ny=ny, nz=nz, np_=np_, seed=13, nsteps=40)
     a = run(ctx, d, storage, "cell")
-    b = run(ctx, d, storage, "tile", launch=(2, 2))
+    b = run(ctx, d, storage, variant, launch=(2, 2))
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+@pytest.mark.parametrize("nchunks", [1, 3, 7, 14, 21])
+def test_tilesplit_ranges_equal_cell(ctx, storage, nchunks):
+    """x chunks that start and end inside, at and across the x absorbing
+    layers (chunk boundaries at every parity), y and z tiles in and out of
+    the y / z layers."""
+    d = decks.a3d_deck(nx=32, ny=30, nz=140, np_=5, seed=14, nsteps=30)
+    a = run(ctx, d, storage, "cell")
+    b = run(ctx, d, storage, "tilesplit", launch=(2, nchunks))
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
