@@ -6,7 +6,8 @@ plain per-shot launches, on one GPU.
 Workload (seismod_amd.synth.e2d_batch_shots): grids nx ~ U{300..1200} by
 nz ~ U{300..800}, 20-cell PML, 3-6 layers (Vp 2000-5000 m/s, Vs = Vp/1.7-1.9,
 rho 1800-2700), Ricker 10-40 Hz, 1000-3000 steps, dt at half the CFL limit,
-FP32.  Set-up (model upload, autotuning of rows per chunk) is outside the
+FP32.  Set-up (model upload, autotuning of rows per chunk: per shot for
+the plain launches, for the whole batch for the batched run) is outside the
 timed region; each timed run starts from zero state.  Prints one JSON line:
 aggregate cell-updates per second batched (hipGraph replay) and one shot
 after another with plain launches, and the batch's kernel time per step.
@@ -49,6 +50,8 @@ def main() -> None:
         s.set_launch(s.launch()[0], args.graph_steps)
         sims.append(s)
     work = sum(sh.cells * n for sh, n in zip(shots, steps))
+    alone = [s.launch()[0] for s in sims]          # per-shot choice, for the plain runs
+    batch_chunk = autotune.tune_e2d_batch(sims, ctx)
 
     batch = E2dBatch(ctx, graph_steps=args.graph_steps)
     for s in sims:
@@ -67,13 +70,13 @@ def main() -> None:
            "steps_total": sum(steps),
            "batched": {"wall_s": round(wall_b, 4), "gcell_updates_per_s": round(work / wall_b / 1e9, 2),
                        "launches": launches, "kernel_ms_per_launch": round(kms / max(launches, 1), 4),
-                       "graph_steps": args.graph_steps},
+                       "graph_steps": args.graph_steps, "rows_per_chunk": batch_chunk},
            "alg_bytes_per_cell": round(sum(synth.algorithmic_bytes_e2d(sh.nx, sh.nz, sh.npml) * sh.cells * n
                                            for sh, n in zip(shots, steps)) / work, 3)}
     if not args.no_sequential:
-        for s in sims:
+        for s, c in zip(sims, alone):
             s.reset()
-            s.set_launch(s.launch()[0], 0)
+            s.set_launch(c, 0)
         ctx.sync()
         t0 = time.perf_counter()
         for s, n in zip(sims, steps):
