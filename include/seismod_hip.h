@@ -268,6 +268,14 @@ size_t sm_e2d_bytes_needed(const sm_e2d_params *p, size_t nrec, size_t max_steps
  * (0 or 1: plain launches).  Results do not depend on either. */
 int sm_e2d_set_launch(sm_e2d *s, int rows_per_chunk, int graph_steps);
 int sm_e2d_get_launch(const sm_e2d *s, int *rows_per_chunk, int *graph_steps);
+/* Kernel form of the fused FP32 step (no counterpart in the reference):
+ * SM_E2D_ROWS, each lane loads the rows of the next x row itself;
+ * SM_E2D_STAGED, the rows every step needs arrive as 16-byte chunk loads
+ * through LDS.  Results are identical. */
+#define SM_E2D_ROWS 0
+#define SM_E2D_STAGED 1
+int sm_e2d_set_variant(sm_e2d *s, int variant);
+int sm_e2d_get_variant(const sm_e2d *s, int *variant);
 /* HIP-event timing of every sm_e2d_run (no counterpart in the reference) */
 int sm_e2d_set_timing(sm_e2d *s, int on);
 int sm_e2d_get_timing(const sm_e2d *s, double *ms_total, size_t *launches);
@@ -288,6 +296,9 @@ int sm_e2d_batch_set_graph_steps(sm_e2d_batch *b, int graph_steps);
  * once) by nsteps[k] steps; returns when all are done */
 int sm_e2d_batch_run(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots,
                      const size_t *nsteps);
+/* kernel form for the batch (SM_E2D_ROWS or SM_E2D_STAGED; the shots'
+ * own settings apply to sm_e2d_run only) */
+int sm_e2d_batch_set_variant(sm_e2d_batch *b, int variant);
 int sm_e2d_batch_set_timing(sm_e2d_batch *b, int on);
 int sm_e2d_batch_get_timing(const sm_e2d_batch *b, double *ms_total, size_t *launches);
 

@@ -25,6 +25,8 @@ SM_A3D_CELL64X2 = 4
 SM_A3D_TILE = 5
 SM_A3D_TILE_SPLIT = 6
 SM_PROBE_NT = 16
+SM_E2D_ROWS = 0
+SM_E2D_STAGED = 1
 SM_PROBE_TILES = 32
 
 _size_p = C.POINTER(C.c_size_t)
@@ -98,6 +100,9 @@ _SIGS = {
     "sm_e2d_device_bytes": (C.c_int, [C.c_void_p, _size_p]),
     "sm_e2d_bytes_needed": (C.c_size_t, [C.POINTER(E2dParams), C.c_size_t, C.c_size_t]),
     "sm_e2d_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "sm_e2d_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
+    "sm_e2d_get_variant": (C.c_int, [C.c_void_p, _int_p]),
+    "sm_e2d_batch_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "sm_e2d_get_launch": (C.c_int, [C.c_void_p, _int_p, _int_p]),
     "sm_e2d_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sm_e2d_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),

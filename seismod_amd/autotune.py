@@ -133,36 +133,40 @@ def tune_e2d(sim, ctx, steps: int = 20, cache_file=None, candidates=None) -> tup
     return best[1], gs
 
 
-def tune_e2d_batch(sims, ctx, steps: int = 8, candidates=(1, 2, 3, 4)) -> int:
-    """Pick one rows_per_chunk for a batch of Elastic2D solvers advanced
-    together (sm_e2d_batch_run), timing the whole batch; applies it to every
-    solver and resets them.  Many shots in flight favour other chunk lengths
+def tune_e2d_batch(sims, ctx, steps: int = 8, candidates=(1, 2, 3, 4),
+                   variants=("rows", "staged")) -> int:
+    """Pick one rows_per_chunk and kernel form for a batch of Elastic2D
+    solvers advanced together (sm_e2d_batch_run), timing the whole batch;
+    applies the chunk to every solver and resets them.  Many shots in flight favour other chunk lengths
     than one shot alone: each wave's rows are processed one after another,
     so the chunk trades waves in flight against the halo rows every chunk
-    recomputes.  Returns rows_per_chunk."""
+    recomputes.  Returns (rows_per_chunk, variant); pass the variant to
+    E2dBatch.set_variant."""
     from seismod_amd.solver import E2dBatch
     gs = [s.launch()[1] for s in sims]
     best = None
     batch = E2dBatch(ctx, graph_steps=0)
     try:
-        for c in candidates:
-            for s in sims:
-                s.reset()
-                s.set_launch(c, 0)
-            batch.run(sims, [2] * len(sims))
-            batch.set_timing(True)
-            batch.run(sims, [steps] * len(sims))
-            ms, n = batch.timing()
-            batch.set_timing(False)
-            t = ms / max(n, 1)
-            if best is None or t < best[0]:
-                best = (t, c)
+        for v in variants:
+            batch.set_variant(v)
+            for c in candidates:
+                for s in sims:
+                    s.reset()
+                    s.set_launch(c, 0)
+                batch.run(sims, [2] * len(sims))
+                batch.set_timing(True)
+                batch.run(sims, [steps] * len(sims))
+                ms, n = batch.timing()
+                batch.set_timing(False)
+                t = ms / max(n, 1)
+                if best is None or t < best[0]:
+                    best = (t, c, v)
     finally:
         batch.close()
     for s, g in zip(sims, gs):
         s.reset()
         s.set_launch(best[1], g)
-    return best[1]
+    return best[1], best[2]
 
 
 def clear_cache() -> None:

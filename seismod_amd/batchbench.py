@@ -51,9 +51,10 @@ def main() -> None:
         sims.append(s)
     work = sum(sh.cells * n for sh, n in zip(shots, steps))
     alone = [s.launch()[0] for s in sims]          # per-shot choice, for the plain runs
-    batch_chunk = autotune.tune_e2d_batch(sims, ctx)
+    batch_chunk, batch_variant = autotune.tune_e2d_batch(sims, ctx)
 
     batch = E2dBatch(ctx, graph_steps=args.graph_steps)
+    batch.set_variant(batch_variant)
     for s in sims:
         s.reset()
     batch.run(sims, [2] * len(sims))          # warm-up: graphs, code objects
@@ -70,7 +71,8 @@ def main() -> None:
            "steps_total": sum(steps),
            "batched": {"wall_s": round(wall_b, 4), "gcell_updates_per_s": round(work / wall_b / 1e9, 2),
                        "launches": launches, "kernel_ms_per_launch": round(kms / max(launches, 1), 4),
-                       "graph_steps": args.graph_steps, "rows_per_chunk": batch_chunk},
+                       "graph_steps": args.graph_steps, "rows_per_chunk": batch_chunk,
+                       "kernel_variant": batch_variant},
            "alg_bytes_per_cell": round(sum(synth.algorithmic_bytes_e2d(sh.nx, sh.nz, sh.npml) * sh.cells * n
                                            for sh, n in zip(shots, steps)) / work, 3)}
     if not args.no_sequential:

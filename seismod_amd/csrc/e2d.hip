@@ -179,6 +179,7 @@ struct sm_e2d {
     sm_dbuf src, rcv, trace;
     size_t step = 0, nsrc = 0, nrec = 0, max_steps = 0, rcv_step0 = 0;
     int chunk = 8, graph_steps = 32;
+    int variant = SM_E2D_ROWS;
     sm::E2dRunner *runner = nullptr;
 };
 
@@ -308,6 +309,7 @@ int run_fused(sm_e2d *s, size_t nsteps)
     if (rc != SM_OK) return rc;
     if (!s->runner) s->runner = new E2dRunner;
     std::vector<E2dShot> one{e2d_shot(s, nsteps)};
+    s->runner->staged = s->variant == SM_E2D_STAGED;
     rc = s->runner->run(one, s->ctx->stream, s->graph_steps);
     if (rc != SM_OK) return rc;
     s->step += nsteps;
@@ -518,7 +520,9 @@ int sm_e2d_create(sm_ctx *ctx, const sm_e2d_params *pp, const double *vp, const 
         SM_CHECK((2 * E2_NSTATE + E2_NSPLIT) * N * 4 < (1ull << 32), SM_ERR_ARG,
                  "sm_e2d_create: padded grid %zux%zu too large for 32-bit offsets", NX, NZ);
         rc = dalloc(s->fields, (2 * E2_NSTATE + E2_NSPLIT) * N * 4);
-        if (rc == SM_OK) rc = dalloc(s->mat, E2_NMAT * N * 4);
+        // 16 floats past the last material array: the staged kernel's
+        // 16-byte chunks of the last row may extend into them
+        if (rc == SM_OK) rc = dalloc(s->mat, (E2_NMAT * N + 16) * 4);
         if (rc == SM_OK) rc = dalloc(s->damp, ab.size() * 4);
         if (rc == SM_OK) rc = upload<float>(s->mat.ptr, mat, ctx->stream);
         if (rc == SM_OK) rc = upload<float>(s->damp.ptr, ab, ctx->stream);
@@ -788,6 +792,23 @@ int sm_e2d_set_launch(sm_e2d *s, int rows_per_chunk, int graph_steps)
     return SM_OK;
 }
 
+int sm_e2d_set_variant(sm_e2d *s, int variant)
+{
+    SM_CHECK(s, SM_ERR_ARG, "sm_e2d_set_variant: NULL solver");
+    SM_CHECK(s->fused, SM_ERR_ARG, "sm_e2d_set_variant: only the fused FP32 kernel has variants");
+    SM_CHECK(variant == SM_E2D_ROWS || variant == SM_E2D_STAGED, SM_ERR_ARG,
+             "sm_e2d_set_variant: unknown variant %d", variant);
+    s->variant = variant;
+    return SM_OK;
+}
+
+int sm_e2d_get_variant(const sm_e2d *s, int *variant)
+{
+    SM_CHECK(s && variant, SM_ERR_ARG, "sm_e2d_get_variant: NULL argument");
+    *variant = s->fused ? s->variant : -1;
+    return SM_OK;
+}
+
 int sm_e2d_get_launch(const sm_e2d *s, int *rows_per_chunk, int *graph_steps)
 {
     SM_CHECK(s && rows_per_chunk && graph_steps, SM_ERR_ARG, "sm_e2d_get_launch: NULL argument");
@@ -821,6 +842,7 @@ struct sm_e2d_batch {
     sm_ctx *ctx = nullptr;
     sm::E2dRunner runner;
     int graph_steps = 32;
+    int variant = SM_E2D_ROWS;
 };
 
 extern "C" {
@@ -874,10 +896,20 @@ int sm_e2d_batch_run(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots, const
     // work queued on the shots' own streams completes before the batch reads them
     for (size_t k = 0; k < nshots; ++k)
         if (shots[k]->ctx->stream != b->ctx->stream) SM_HIP(hipStreamSynchronize(shots[k]->ctx->stream));
+    b->runner.staged = b->variant == SM_E2D_STAGED;
     int rc = b->runner.run(d, b->ctx->stream, b->graph_steps);
     if (rc != SM_OK) return rc;
     SM_HIP(hipStreamSynchronize(b->ctx->stream));
     for (size_t k = 0; k < nshots; ++k) shots[k]->step += nsteps[k];
+    return SM_OK;
+}
+
+int sm_e2d_batch_set_variant(sm_e2d_batch *b, int variant)
+{
+    SM_CHECK(b, SM_ERR_ARG, "sm_e2d_batch_set_variant: NULL batch");
+    SM_CHECK(variant == SM_E2D_ROWS || variant == SM_E2D_STAGED, SM_ERR_ARG,
+             "sm_e2d_batch_set_variant: unknown variant %d", variant);
+    b->variant = variant;
     return SM_OK;
 }
 

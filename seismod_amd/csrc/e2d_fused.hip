@@ -69,12 +69,31 @@ __device__ __forceinline__ float from_left(float x) { return __shfl_up(x, 1, 64)
 __device__ __forceinline__ float from_right(float x) { return __shfl_down(x, 1, 64); }
 
 struct VRow { float t, a, c, coef; };   // total or split pair, and its coefficient
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// Staged loads: the rows every iteration needs (field-rows 0..4 from the
+// state buffer, 5..9 from the material buffer) copied in 16-byte chunks
+// through the wave's LDS.  Field-row fr: buffer field, row offset from the
+// iteration (+1 or +2), first chunk in the wave's LDS area.
+__host__ __device__ constexpr int sg_field(int fr)
+{
+    return fr == 0 ? E2_TXX : fr == 1 ? E2_TZZ : fr == 2 ? E2_TXZ : fr == 3 ? E2_VX : fr == 4 ? E2_VZ
+         : fr == 5 ? E2_CB : fr == 6 ? E2_CBM : fr == 7 ? E2_CLA : fr == 8 ? E2_CL2M : E2_CMU;
+}
+__host__ __device__ constexpr int sg_drow(int fr) { return (fr <= 3 || fr == 5) ? 2 : 1; }
+constexpr int SG_CHUNKS = 17;   // 68 cells: 64 plus up to 3 before for 16-byte alignment
+__host__ __device__ constexpr int sg_lds(int fr) { return fr < 5 ? fr * SG_CHUNKS : 128 + (fr - 5) * SG_CHUNKS; }
 struct TRow { float la, l2m, mu, x[6]; }; // stress coefficients and split pairs
 
+// STAGED: the rows every iteration needs arrive as 16-byte chunk loads
+// through LDS (4 loads per lane per iteration instead of 10); the
+// arithmetic is the same, so results are identical.
+template <bool STAGED>
 __global__ void __launch_bounds__(256)
 e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot, int nitems,
                const int *__restrict__ base, int k)
 {
+    __shared__ u32x4 s_stage[STAGED ? E2D_WAVES * 256 : 1];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // consecutive blocks go to different XCDs: give each XCD a contiguous
@@ -157,6 +176,47 @@ e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot,
         return r;
     };
 
+    // ---- staged loads (STAGED) ----
+    // The wave's 64 cells [J0m, J0m + 64) of a field-row lie in a window of
+    // 17 chunks starting at the 16-byte boundary at or before J0m, counted
+    // from the start of the state allocation (= st[0]) or of the material
+    // allocation (padded by 16 floats), both 256-byte aligned.  Chunks of
+    // rows outside the grid are not loaded (0); cells outside [0, NZ) are
+    // masked by the reader (jv), as the per-lane loads mask them.
+    const int J0m = zt * E2D_OWN - 1;
+    const int par_off = (int)(S->st[it & 1] - S->st[0]);   // floats: 0 or E2_NSTATE*N
+    const auto Rall = rsrc(S->st[0], (2 * E2_NSTATE + E2_NSPLIT) * N4);
+    const auto Rmat16 = rsrc(S->mat, E2_NMAT * N4 + 64);
+    u32x4 *const stage = s_stage + (STAGED ? w * 256 : 0);
+    // float index (within its allocation) of field-row fr's first cell
+    auto fr_base = [&](int fr) { return (fr < 5 ? par_off : 0) + sg_field(fr) * (int)N; };
+    // issue the chunk loads of iteration I (rows I+1, I+2); the lane's
+    // chunk of round r is c = (r & 1) * 64 + lane of the state (r < 2) or
+    // material (r >= 2) field-rows, 17 chunks each
+    auto stage_issue = [&](int I, u32x4 (&v)[4]) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int c = (r & 1) * 64 + lane;
+            const int fr = (r < 2 ? 0 : 5) + c / SG_CHUNKS;
+            const int R = I + ((fr <= 3 || fr == 5) ? 2 : 1);
+            const int a = (fr_base(fr) + R * NZ + J0m) & ~3;
+            const bool ok = c < 5 * SG_CHUNKS && rv(R);
+            const unsigned off = ok ? (unsigned)(a + 4 * (c % SG_CHUNKS)) * 4u : OFF_NONE;
+            v[r] = __builtin_amdgcn_raw_buffer_load_b128(r < 2 ? Rall : Rmat16, off, 0, 0);
+        }
+    };
+    auto stage_write = [&](const u32x4 (&v)[4]) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) stage[r * 64 + lane] = v[r];
+    };
+    // this lane's value of field-row fr for iteration I
+    auto stage_read = [&](int I, int fr) {
+        const int R = I + sg_drow(fr);
+        const int a = (fr_base(fr) + R * NZ + J0m) & 3;
+        const float x = ((const float *)stage)[sg_lds(fr) * 4 + lane + a];
+        return jv ? x : 0.f;
+    };
+
     // new vx of row R from old txx of rows R-1, R and old txz of row R
     auto vx_new = [&](int R, const VRow &r, float sxx_m, float sxx_r, float sxz_r, bool store) {
         const float dxs = sxx_r - sxx_m;
@@ -213,6 +273,9 @@ e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot,
     VRow vr = fetch_v(I0 + 1, E2_VX, E2_VXX, E2_VXZ, E2_CB);
     VRow zr = fetch_v(I0, E2_VZ, E2_VZX, E2_VZZ, E2_CBM);
     TRow tr = fetch_t(I0);
+    u32x4 sv[4];
+    if constexpr (STAGED)
+        if (I0 + 1 < I1) stage_issue(I0, sv);
     float vx0 = vx_new(I0, v_pre, sxx_m, sxx0, sxz0, true);
     float vz_m = vz_new(I0 - 1, z_pre, sxz_m, szz_m, sxz0, false);
 
@@ -222,10 +285,42 @@ e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot,
         VRow nvr{0.f, 0.f, 0.f, 0.f}, nzr{0.f, 0.f, 0.f, 0.f};
         TRow ntr{};
         if (I + 1 < I1) {
-            fetch_s(I + 2, nsxx, nszz, nsxz);
-            nvr = fetch_v(I + 2, E2_VX, E2_VXX, E2_VXZ, E2_CB);
-            nzr = fetch_v(I + 1, E2_VZ, E2_VZX, E2_VZZ, E2_CBM);
-            ntr = fetch_t(I + 1);
+            if constexpr (STAGED) {
+                stage_write(sv);
+                if (I + 2 < I1) stage_issue(I + 1, sv);
+                nsxx = stage_read(I, 0);
+                nszz = stage_read(I, 1);
+                nsxz = stage_read(I, 2);
+                nvr.t = stage_read(I, 3);
+                nvr.coef = stage_read(I, 5);
+                nzr.t = stage_read(I, 4);
+                nzr.coef = stage_read(I, 6);
+                ntr.la = stage_read(I, 7);
+                ntr.l2m = stage_read(I, 8);
+                ntr.mu = stage_read(I, 9);
+                // split velocities and stresses in the PML, per lane
+                // (totals of split lanes are loaded too and not used)
+                auto vsplit = [&](int R, int sx, int sz, VRow &r) {
+                    if (rv(R) && (px(R) || zpml)) {
+                        const bool pm = px(R) || pzl;
+                        const unsigned o = voff(R, rv(R) && jv);
+                        r.a = bld(Rin, pm ? o : OFF_NONE, sx * N4);
+                        r.c = bld(Rin, pm ? o : OFF_NONE, sz * N4);
+                    }
+                };
+                vsplit(I + 2, E2_VXX, E2_VXZ, nvr);
+                vsplit(I + 1, E2_VZX, E2_VZZ, nzr);
+                if (px(I + 1) || zpml) {
+                    const unsigned op = (px(I + 1) || pzl) && own ? voff(I + 1, jv) : OFF_NONE;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) ntr.x[c] = bld(Rss, op, c * N4);
+                }
+            } else {
+                fetch_s(I + 2, nsxx, nszz, nsxz);
+                nvr = fetch_v(I + 2, E2_VX, E2_VXX, E2_VXZ, E2_CB);
+                nzr = fetch_v(I + 1, E2_VZ, E2_VZX, E2_VZZ, E2_CBM);
+                ntr = fetch_t(I + 1);
+            }
         }
 
         const float vx1 = vx_new(I + 1, vr, sxx0, sxx1, sxz1, I + 1 < I1);
@@ -337,6 +432,13 @@ int E2dRunner::ensure(size_t bytes)
     return SM_OK;
 }
 
+void E2dRunner::launch_step(int nb, hipStream_t st, const E2dShot *tab, const int *bshot, int nitems,
+                            const int *base, int k)
+{
+    if (staged) e2d_fused_step<true><<<nb, 256, 0, st>>>(tab, bshot, nitems, base, k);
+    else e2d_fused_step<false><<<nb, 256, 0, st>>>(tab, bshot, nitems, base, k);
+}
+
 int E2dRunner::graph_for(int nact, int nitems, int g, hipStream_t st, hipGraphExec_t *out)
 {
     const auto key = std::make_tuple(nact, nitems, g);
@@ -348,7 +450,7 @@ int E2dRunner::graph_for(int nact, int nitems, int g, hipStream_t st, hipGraphEx
     hipGraph_t graph = nullptr;
     SM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < g; ++k)
-        e2d_fused_step<<<nb, 256, 0, st>>>(tab, bshot, nitems, base_, k);
+        launch_step(nb, st, tab, bshot, nitems, base_, k);
     e2d_base_set<<<1, 64, 0, st>>>(base_, g, 1);
     SM_HIP(hipStreamEndCapture(st, &graph));
     hipGraphExec_t ex = nullptr;
@@ -386,8 +488,10 @@ int E2dRunner::run(std::vector<E2dShot> shots, hipStream_t st, int graph_steps)
     int *bs = (int *)(host.data() + tab_bytes_);
     for (size_t i = 0; i < n; ++i)
         for (int q = pref[i] / E2D_WAVES; q < pref[i + 1] / E2D_WAVES; ++q) bs[q] = (int)i;
-    if (host.size() > table_cap_ || tab_bytes_ != last_tab_bytes_) drop_graphs();
+    if (host.size() > table_cap_ || tab_bytes_ != last_tab_bytes_ || staged != last_staged_)
+        drop_graphs();
     last_tab_bytes_ = tab_bytes_;
+    last_staged_ = staged;
     int rc = ensure(host.size());
     if (rc != SM_OK) return rc;
     // the table lives in device memory read by the kernels of this run;
@@ -425,7 +529,7 @@ int E2dRunner::run(std::vector<E2dShot> shots, hipStream_t st, int graph_steps)
         }
         const int nb = grid_blocks(nitems);
         for (int k = 0; k < len; ++k)
-            e2d_fused_step<<<nb, 256, 0, st>>>(tab, bshot, nitems, nullptr, s + k);
+            launch_step(nb, st, tab, bshot, nitems, nullptr, s + k);
         s += len;
         count += (size_t)len;
     }

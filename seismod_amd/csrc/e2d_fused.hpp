@@ -84,14 +84,20 @@ public:
     bool timing = false;
     double ms_total = 0.0;
     size_t launches = 0;
+    // kernel form: rows of the next iteration loaded per lane (false) or
+    // staged through LDS in 16-byte chunks (true); results are identical
+    bool staged = false;
 
 private:
+    void launch_step(int nb, hipStream_t st, const E2dShot *tab, const int *bshot, int nitems,
+                     const int *base, int k);
     int ensure(size_t bytes);
     int graph_for(int nact, int nitems, int g, hipStream_t st, hipGraphExec_t *out);
     void drop_graphs();
     void *table_ = nullptr;     // shot descriptors, then the block -> shot map
     size_t table_cap_ = 0;      // bytes
     size_t tab_bytes_ = 0, last_tab_bytes_ = 0;
+    bool last_staged_ = false;
     int *base_ = nullptr;
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs_;
 };

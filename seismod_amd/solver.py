@@ -364,6 +364,17 @@ class Elastic2D:
         check(lib.sm_e2d_get_launch(self.handle, C.byref(a), C.byref(b)), "sm_e2d_get_launch")
         return a.value, b.value
 
+    E2D_VARIANTS = {"rows": L.SM_E2D_ROWS, "staged": L.SM_E2D_STAGED}
+
+    def set_variant(self, variant: str) -> None:
+        """Kernel form of the fused FP32 step: "rows" or "staged" (identical results)."""
+        check(lib.sm_e2d_set_variant(self.handle, self.E2D_VARIANTS[variant]), "sm_e2d_set_variant")
+
+    def variant(self) -> str:
+        v = C.c_int()
+        check(lib.sm_e2d_get_variant(self.handle, C.byref(v)), "sm_e2d_get_variant")
+        return {n: k for k, n in self.E2D_VARIANTS.items()}.get(v.value, "none")
+
     def set_timing(self, on: bool) -> None:
         check(lib.sm_e2d_set_timing(self.handle, 1 if on else 0), "sm_e2d_set_timing")
 
@@ -407,6 +418,11 @@ class E2dBatch:
         if steps.size != len(shots):
             raise ValueError("one step count per shot")
         check(lib.sm_e2d_batch_run(self.handle, len(shots), arr, _sp(steps)), "sm_e2d_batch_run")
+
+    def set_variant(self, variant: str) -> None:
+        """Kernel form for the batch: "rows" or "staged" (identical results)."""
+        check(lib.sm_e2d_batch_set_variant(self.handle, Elastic2D.E2D_VARIANTS[variant]),
+              "sm_e2d_batch_set_variant")
 
     def set_timing(self, on: bool) -> None:
         check(lib.sm_e2d_batch_set_timing(self.handle, 1 if on else 0), "sm_e2d_batch_set_timing")

@@ -179,3 +179,48 @@ def test_batch_argument_errors(ctx):
         b.run([n], [1])
     b.run([a], [0])
     assert a.step == 0
+
+
+@pytest.mark.parametrize("case", CASES, ids=ids)
+@pytest.mark.parametrize("launch", [(1, 0), (2, 0), (5, 8), (64, 0)])
+def test_staged_equals_rows(ctx, case, launch):
+    """The staged kernel (rows of the next step as 16-byte chunks through
+    LDS) computes the same values as the per-lane form, for every chunk
+    length, grid width (rows not 16-byte aligned) and PML thickness."""
+    d = deck(**case)
+    a = make(ctx, d, launch=launch)
+    a.run(d["nsteps"])
+    ra = outputs(a)
+    a.close()
+    b = make(ctx, d, launch=launch)
+    b.set_variant("staged")
+    assert b.variant() == "staged"
+    b.run(d["nsteps"])
+    rb = outputs(b)
+    b.close()
+    assert np.abs(ra[0]).max() > 0
+    assert np.array_equal(ra[0], rb[0]) and np.array_equal(ra[1], rb[1])
+
+
+def test_staged_batch_equals_alone(ctx):
+    """A staged batch of shots of different sizes and step counts equals
+    each shot run alone with the per-lane kernel."""
+    from seismod_amd.solver import E2dBatch
+    specs = [dict(nx=37, nz=23, npml=4, seed=21, nsteps=60), dict(nx=16, nz=70, npml=3, seed=22, nsteps=45),
+             dict(nx=51, nz=130, npml=7, seed=23, nsteps=80), dict(nx=9, nz=9, npml=1, seed=24, nsteps=30)]
+    ds = [deck(**c) for c in specs]
+    alone = []
+    for d in ds:
+        s = make(ctx, d, launch=(2, 0))
+        s.run(d["nsteps"])
+        alone.append(outputs(s))
+        s.close()
+    sims = [make(ctx, d, launch=(2, 0)) for d in ds]
+    b = E2dBatch(ctx, graph_steps=8)
+    b.set_variant("staged")
+    b.run(sims, [d["nsteps"] for d in ds])
+    b.close()
+    for s, (tr, f) in zip(sims, alone):
+        got = outputs(s)
+        assert np.array_equal(got[0], tr) and np.array_equal(got[1], f)
+        s.close()
