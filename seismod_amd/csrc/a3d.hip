@@ -198,14 +198,17 @@ bool uses_fused(const sm_a3d_params &p)
 }
 
 // device row pitch of the fused kernels: rows padded to a multiple of 64
-// cells on request (SM_A3D_PAD64; the CELL64 variants need it), otherwise
-// to a multiple of 8 cells (16 bytes of 16-bit storage; the TILE variant
-// copies rows in 16-byte chunks).  The two-kernel paths keep the reference
-// layout.
+// cells (so every 64-cell wave access covers whole cache lines) on request
+// (SM_A3D_PAD64; the CELL64 variants need it) or when that adds at most 5%
+// cells, otherwise to a multiple of 8 cells (16 bytes of 16-bit storage;
+// the TILE variants copy rows in 16-byte chunks).  The two-kernel paths keep
+// the reference layout.
 size_t row_pitch(size_t nz2, const sm_a3d_params &p)
 {
     if (!uses_fused(p)) return nz2;
-    return (p.flags & SM_A3D_PAD64) ? (nz2 + 63) / 64 * 64 : (nz2 + 7) / 8 * 8;
+    const size_t p64 = (nz2 + 63) / 64 * 64;
+    if ((p.flags & SM_A3D_PAD64) || (p64 - nz2) * 20 <= nz2) return p64;
+    return (nz2 + 7) / 8 * 8;
 }
 
 // reference-layout cell index -> device index

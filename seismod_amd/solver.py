@@ -142,7 +142,7 @@ class A3dConfig:
     storage: str = "f32"
     naive: bool = False
     sround: bool = False      # BF16/FP16: stochastic rounding of stored values
-    pad64: bool = False       # device rows padded to 64 cells (cell64 variants)
+    pad64: bool = False       # rows always padded to 64 cells (cell64 variants); else when <= 5% extra
 
     def params(self) -> L.A3dParams:
         return L.A3dParams(self.nx, self.ny, self.nz, self.np, self.dx, self.dy, self.dz,
