@@ -136,8 +136,9 @@ typedef struct {
 #define SM_A3D_NAIVE 1        /* two plain kernels per step (no fusion) */
 #define SM_A3D_PAD64 4        /* fused kernels: device rows padded to a multiple
                                  of 64 cells (needed by SM_A3D_CELL64*); without
-                                 the flag rows are padded to 64 cells when that
-                                 adds at most 5% cells, else to 8 cells */
+                                 the flag rows start on 128-byte lines when that
+                                 adds at most 5% (FP32) / 10% (16-bit) cells,
+                                 else they are padded to 8 cells */
 #define SM_A3D_SROUND 2       /* BF16/FP16 storage: stochastic rounding of the
                                  stored values (deterministic pseudo-random
                                  bits from element index and step) instead

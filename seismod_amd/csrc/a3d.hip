@@ -197,17 +197,21 @@ bool uses_fused(const sm_a3d_params &p)
     return true;   // 16-bit storage exists only in the fused kernel
 }
 
-// device row pitch of the fused kernels: rows padded to a multiple of 64
-// cells (so every 64-cell wave access covers whole cache lines) on request
-// (SM_A3D_PAD64; the CELL64 variants need it) or when that adds at most 5%
-// cells, otherwise to a multiple of 8 cells (16 bytes of 16-bit storage;
-// the TILE variants copy rows in 16-byte chunks).  The two-kernel paths keep
-// the reference layout.
+// device row pitch of the fused kernels: rows start on 128-byte cache
+// lines (a multiple of 32 cells in FP32, 64 in 16-bit storage; every
+// 64-cell wave access then covers whole lines) when that adds at most 5%
+// (FP32) or 10% (16-bit) cells, otherwise a multiple of 8 cells (16 bytes
+// of 16-bit storage; the TILE variants copy rows in 16-byte chunks).
+// SM_A3D_PAD64 forces a multiple of 64 cells (the CELL64 variants need
+// it).  The two-kernel paths keep the reference layout.
 size_t row_pitch(size_t nz2, const sm_a3d_params &p)
 {
     if (!uses_fused(p)) return nz2;
-    const size_t p64 = (nz2 + 63) / 64 * 64;
-    if ((p.flags & SM_A3D_PAD64) || (p64 - nz2) * 20 <= nz2) return p64;
+    if (p.flags & SM_A3D_PAD64) return (nz2 + 63) / 64 * 64;
+    const bool wide = p.storage == SM_STORE_F32;
+    const size_t line = wide ? 32 : 64;
+    const size_t pl = (nz2 + line - 1) / line * line;
+    if ((pl - nz2) * (wide ? 20 : 10) <= nz2) return pl;
     return (nz2 + 7) / 8 * 8;
 }
 
