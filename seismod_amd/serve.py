@@ -8,10 +8,15 @@ solve with its own model, source, receivers and step count.  The server:
     exact device footprint from sm_*_bytes_needed), in arrival order;
   - keeps every admitted shot's wavefields resident on the device until it
     finishes;
-  - advances all admitted shots in ticks of `tick_steps` time steps: the 3D
-    shots are queued on their own streams first (asynchronous), then all
+  - advances admitted shots in ticks of `tick_steps` time steps: the 3D
+    shots are queued on their own streams first (asynchronous), then the
     2D shots advance together in one batched launch per time step
-    (sm_e2d_batch_run), so the device works on several shots at once;
+    (sm_e2d_batch_run), so the device works on several shots at once.
+    Which shots a tick advances is the policy: "srpt" (default) takes the
+    shots with the least remaining work (cells x steps left), of each kind,
+    until about one device's worth of cells is in flight, so short requests
+    finish first without leaving the device idle; "share" advances every
+    admitted shot (processor sharing);
   - between ticks, retires finished shots (traces copied back, device
     memory freed) and admits new arrivals, so requests join and leave
     while others are running.
@@ -22,7 +27,7 @@ the same bits alone, batched, or in any launch shape.
 `run_sequential` is the comparison point: one shot at a time in arrival
 order, each with plain per-step launches.
 
-CLI:  python -m seismod_amd.serve --rate 20 --count 24 [--seed 5]
+CLI:  python -m seismod_amd.serve --rate 20 --count 24 [--seed 5] [--policy srpt|share]
 prints one JSON line with p50/p99 latency and aggregate Gcell-updates/s for
 the server and for the sequential baseline on the same arrival trace.
 """
@@ -143,7 +148,12 @@ class Server:
     """Continuous-batching server on one device (see the module docstring)."""
 
     def __init__(self, device: int = 0, tick_steps: int = 64, reserve_bytes: int = 2 << 30,
-                 max_active: int = 256, graph_steps: int = 32):
+                 max_active: int = 256, graph_steps: int = 32, policy: str = "srpt",
+                 fill_3d: int = 16 << 20, fill_2d: int = 8 << 20):
+        if policy not in ("srpt", "share"):
+            raise ValueError(f"unknown policy {policy!r}")
+        self.policy = policy
+        self.fill = {"a3d": fill_3d, "e2d": fill_2d}   # cells in flight per tick and kind
         self.device = device
         self.tick = tick_steps
         self.reserve = reserve_bytes
@@ -182,19 +192,23 @@ class Server:
                     time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.05)))
                 continue
             # one tick: 3D shots queued on their own streams, then the 2D batch
+            run = self._select(active)
             for a in active:
+                a["step"] = 0
+            for a in run:
                 if a["req"].kind == "a3d":
                     a["step"] = min(self.tick, a["req"].nsteps - a["done"])
                     a["sim"].run(a["step"])
-            e2 = [a for a in active if a["req"].kind == "e2d"]
+            e2 = [a for a in run if a["req"].kind == "e2d"]
             if e2:
                 steps = [min(self.tick, a["req"].nsteps - a["done"]) for a in e2]
                 self.batch.run([a["sim"] for a in e2], steps)
                 for a, n in zip(e2, steps):
                     a["step"] = n
-            for a in active:
+            for a in run:
                 if a["req"].kind == "a3d":
                     a["ctx"].sync()
+            for a in active:
                 a["done"] += a["step"]
             # retire finished shots
             still = []
@@ -211,6 +225,22 @@ class Server:
                                       r.cells * r.nsteps, tr))
             active = still
         return sorted(results, key=lambda x: x.rid)
+
+    def _select(self, active: list) -> list:
+        """The shots one tick advances (see the module docstring)."""
+        if self.policy == "share":
+            return list(active)
+        left = lambda a: a["req"].cells * (a["req"].nsteps - a["done"])
+        out = []
+        for kind in ("a3d", "e2d"):
+            cells = 0
+            for a in sorted((a for a in active if a["req"].kind == kind),
+                            key=lambda a: (left(a), a["req"].rid)):
+                if cells >= self.fill[kind]:
+                    break
+                out.append(a)
+                cells += a["req"].cells
+        return out
 
     def close(self) -> None:
         self.batch.close()
@@ -259,19 +289,21 @@ def main() -> None:
     ap.add_argument("--frac-3d", type=float, default=0.4)
     ap.add_argument("--scale", type=float, default=1.0,
                     help="multiply grid edge ranges (smaller runs for quick checks)")
+    ap.add_argument("--policy", choices=("srpt", "share"), default="srpt",
+                    help="which admitted shots a tick advances")
     ap.add_argument("--no-baseline", action="store_true")
     args = ap.parse_args()
     sc = args.scale
     reqs = make_requests(args.count, args.rate, seed=args.seed, frac_3d=args.frac_3d,
                          e2d_range=(int(256 * sc), int(2048 * sc)),
                          a3d_range=(int(128 * sc), int(384 * sc)))
-    srv = Server(tick_steps=args.tick)
+    srv = Server(tick_steps=args.tick, policy=args.policy)
     # warm-up: kernels loaded, streams created
     srv.run(make_requests(2, 0.0, seed=99, e2d_range=(64, 64), a3d_range=(32, 32),
                           steps_range=(20, 20)), keep_traces=False)
     res = srv.run(reqs, keep_traces=False)
     srv.close()
-    line = {"rate_per_s": args.rate, "seed": args.seed, "tick_steps": args.tick,
+    line = {"rate_per_s": args.rate, "seed": args.seed, "tick_steps": args.tick, "policy": args.policy,
             "mix": {"e2d": sum(r.kind == "e2d" for r in reqs),
                     "a3d": sum(r.kind == "a3d" for r in reqs),
                     "a3d_bf16": sum(r.kind == "a3d" and r.storage == "bf16" for r in reqs)},

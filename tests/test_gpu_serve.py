@@ -8,13 +8,14 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_results_alone_equal_results_in_mix():
+@pytest.mark.parametrize("policy", ["srpt", "share"])
+def test_results_alone_equal_results_in_mix(policy):
     from seismod_amd import serve
     reqs = serve.make_requests(10, rate=400.0, seed=7, frac_3d=0.4, e2d_range=(40, 120),
                                a3d_range=(24, 48), steps_range=(40, 160), bf16_frac=0.5)
     kinds = {r.kind for r in reqs}
     assert kinds == {"e2d", "a3d"}
-    srv = serve.Server(tick_steps=16, graph_steps=8)
+    srv = serve.Server(tick_steps=16, graph_steps=8, policy=policy)
     mix = srv.run(reqs)
     srv.close()
     alone = serve.run_sequential(reqs)

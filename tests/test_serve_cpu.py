@@ -31,3 +31,39 @@ def test_summary_percentiles():
     assert abs(s["p50_s"] - np.percentile(1.0 + np.arange(100), 50)) < 1e-9
     assert abs(s["p99_s"] - np.percentile(1.0 + np.arange(100), 99)) < 1e-9
     assert s["makespan_s"] == 99.0 + 100.0
+
+
+class _Req:
+    def __init__(self, rid, kind, cells, nsteps):
+        self.rid, self.kind, self.cells, self.nsteps = rid, kind, cells, nsteps
+
+
+def _server(policy, fill_3d, fill_2d):
+    # the selection needs no device: build the object without __init__
+    s = serve.Server.__new__(serve.Server)
+    s.policy = policy
+    s.fill = {"a3d": fill_3d, "e2d": fill_2d}
+    return s
+
+
+def test_srpt_selects_least_remaining_work_per_kind():
+    act = [dict(req=_Req(0, "a3d", 100, 50), done=0),     # left 5000
+           dict(req=_Req(1, "a3d", 100, 50), done=40),    # left 1000
+           dict(req=_Req(2, "a3d", 300, 10), done=0),     # left 3000
+           dict(req=_Req(3, "e2d", 10, 100), done=0),     # left 1000
+           dict(req=_Req(4, "e2d", 10, 20), done=0),      # left 200
+           dict(req=_Req(5, "e2d", 10, 30), done=0)]      # left 300
+    s = _server("srpt", fill_3d=150, fill_2d=15)
+    got = [a["req"].rid for a in s._select(act)]
+    # 3D: rid 1 (100 cells), then rid 2 reaches 400 >= 150; 2D: 4 then 5
+    assert got == [1, 2, 4, 5]
+    s = _server("srpt", fill_3d=1, fill_2d=1)
+    assert [a["req"].rid for a in s._select(act)] == [1, 4]
+    s = _server("share", 1, 1)
+    assert [a["req"].rid for a in s._select(act)] == [0, 1, 2, 3, 4, 5]
+
+
+def test_unknown_policy_is_rejected():
+    import pytest
+    with pytest.raises(ValueError):
+        serve.Server(policy="fifo")
