@@ -482,9 +482,10 @@ __device__ __forceinline__ void march(const FusedArgs<S> &a, const Bufs &B, cons
 
 // SPLIT: slab groups decided per tile and per range of planes (more code,
 // fewer instructions per interior cell); otherwise one loop with every
-// group, held to at least 4 waves per SIMD (at most 128 registers per lane)
+// group.  Both are held to at least 4 waves per SIMD (at most 128
+// registers per lane).
 template <typename S, int TJ, bool SPLIT>
-__global__ void __launch_bounds__(64 * TJ, SPLIT ? 1 : 4)
+__global__ void __launch_bounds__(64 * TJ, 4)
 a3d_tile_step(FusedArgs<S> a)
 {
     using TS = TileShape<S, TJ>;
