@@ -637,17 +637,17 @@ int resident_blocks(sm_a3d *s, int tj)
     switch (s->p.storage) {
     case SM_STORE_F32:
         rc = pair   ? pair_max_blocks_per_cu<float>(tj, &per_cu)
-             : tile ? tile_max_blocks_per_cu<float>(tj, split, &per_cu)
+             : tile ? tile_max_blocks_per_cu<float>(tj, split, s->sr != 0, &per_cu)
                     : fused_max_blocks_per_cu<float>(tj, pf, a64, &per_cu);
         break;
     case SM_STORE_BF16:
         rc = pair   ? pair_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu)
-             : tile ? tile_max_blocks_per_cu<__hip_bfloat16>(tj, split, &per_cu)
+             : tile ? tile_max_blocks_per_cu<__hip_bfloat16>(tj, split, s->sr != 0, &per_cu)
                     : fused_max_blocks_per_cu<__hip_bfloat16>(tj, pf, a64, &per_cu);
         break;
     default:
         rc = pair   ? pair_max_blocks_per_cu<__half>(tj, &per_cu)
-             : tile ? tile_max_blocks_per_cu<__half>(tj, split, &per_cu)
+             : tile ? tile_max_blocks_per_cu<__half>(tj, split, s->sr != 0, &per_cu)
                     : fused_max_blocks_per_cu<__half>(tj, pf, a64, &per_cu);
         break;
     }

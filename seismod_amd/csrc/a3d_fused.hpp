@@ -115,6 +115,6 @@ int pair_max_blocks_per_cu(int tj, int *blocks);
 template <typename S>
 int tile_launch(const FusedArgs<S> &a, int tj, bool split, hipStream_t st);
 template <typename S>
-int tile_max_blocks_per_cu(int tj, bool split, int *blocks);
+int tile_max_blocks_per_cu(int tj, bool split, bool sr, int *blocks);
 
 }  // namespace sm

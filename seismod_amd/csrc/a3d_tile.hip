@@ -81,9 +81,12 @@ enum { T_P = 0, T_VY = 1, T_VZ = 2, T_VX = 3, T_R = 4, T_K = 5, T_NF = 6 };
 
 // Geometry of the staged tiles for storage type S and TJ rows per block.
 // Rows are relative to j0 (the block's first row), cells to k0 (first
-// cell).  Chunk c of a slot lives at byte c * 16 of the slot.
-template <typename S, int TJ>
+// cell).  Chunk c of a slot lives at byte c * 16 of the slot.  SR: 16-bit
+// stores round stochastically (a compile-time choice, so the default
+// round-to-nearest kernels carry no branch or salt for it).
+template <typename S, int TJ, bool SR = false>
 struct TileShape {
+    static constexpr int sr = SR ? 1 : 0;
     static constexpr int ES = (int)sizeof(S);
     static constexpr int esz(int f) { return f >= T_R ? 4 : ES; }
     static constexpr int cpc(int f) { return 16 / esz(f); }   // cells per 16-byte chunk
@@ -287,24 +290,24 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
                                  L.zbh, L.zah, L.zikh, mz);
     // stores: a lane that must not write passes OFF_NONE
     bst<S>(B.out, L.own ? (F_VX * a.nnn + o) * E : OFF_NONE,
-           Store<S>::enc(vxn, a.isV, F_VX * a.nnn + o, a.salt, a.sr));
+           Store<S>::enc(vxn, a.isV, F_VX * a.nnn + o, a.salt, TS::sr));
     bst<S>(B.out, L.own ? (F_VY * a.nnn + o) * E : OFF_NONE,
-           Store<S>::enc(vyn, a.isV, F_VY * a.nnn + o, a.salt, a.sr));
+           Store<S>::enc(vyn, a.isV, F_VY * a.nnn + o, a.salt, TS::sr));
     bst<S>(B.out, L.own ? (F_VZ * a.nnn + o) * E : OFF_NONE,
-           Store<S>::enc(vzn, a.isV, F_VZ * a.nnn + o, a.salt, a.sr));
+           Store<S>::enc(vzn, a.isV, F_VZ * a.nnn + o, a.salt, TS::sr));
     {
         const unsigned ix = (unsigned)x.sxv * plane + L.o_jk;
         const unsigned iy = a.my + ((unsigned)i * np2 + L.syv) * a.pz + L.kc;
         const unsigned iz = a.mz + (unsigned)i * a.ny2 * np2 + L.mz_v;
         if constexpr (X)
             bst<S>(B.mo, L.own && x.sxv >= 0 ? ix * E : OFF_NONE,
-                   Store<S>::enc(mx, a.isMv, ix, a.salt ^ SALT_MV, a.sr));
+                   Store<S>::enc(mx, a.isMv, ix, a.salt ^ SALT_MV, TS::sr));
         if constexpr (Y)
             bst<S>(B.mo, L.own && L.syv >= 0 ? iy * E : OFF_NONE,
-                   Store<S>::enc(my, a.isMv, iy, a.salt ^ SALT_MV, a.sr));
+                   Store<S>::enc(my, a.isMv, iy, a.salt ^ SALT_MV, TS::sr));
         if constexpr (Z)
             bst<S>(B.mo, L.own && L.szv >= 0 ? iz * E : OFF_NONE,
-                   Store<S>::enc(mz, a.isMv, iz, a.salt ^ SALT_MV, a.sr));
+                   Store<S>::enc(mz, a.isMv, iz, a.salt ^ SALT_MV, TS::sr));
     }
     // new vy of row j-1 and new vz of k0-1 (owned by other waves / tiles;
     // recomputed here, not stored)
@@ -347,13 +350,13 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
     }
     if constexpr (X)
         bst<S>(B.mp, upd && x.sxp >= 0 ? jx * E : OFF_NONE,
-               Store<S>::enc(mmx, a.isMp, jx, a.salt ^ SALT_MP, a.sr));
+               Store<S>::enc(mmx, a.isMp, jx, a.salt ^ SALT_MP, TS::sr));
     if constexpr (Y)
         bst<S>(B.mp, upd && L.syp >= 0 ? jy * E : OFF_NONE,
-               Store<S>::enc(mmy, a.isMp, jy, a.salt ^ SALT_MP, a.sr));
+               Store<S>::enc(mmy, a.isMp, jy, a.salt ^ SALT_MP, TS::sr));
     if constexpr (Z)
         bst<S>(B.mp, upd && L.szp >= 0 ? jz * E : OFF_NONE,
-               Store<S>::enc(mmz, a.isMp, jz, a.salt ^ SALT_MP, a.sr));
+               Store<S>::enc(mmz, a.isMp, jz, a.salt ^ SALT_MP, TS::sr));
     float Pn = upd ? P_c - lds_f32(cs, tof<TS, T_K>(L.tc, L.cc)) * (dvdx + dvdy + dvdz) : P_c;
     if (x.q0 < x.q1) {
         // sources of this plane (a_iso_3d.c:689-693), in source order, from
@@ -367,7 +370,7 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
             if (L.own && (unsigned)cell == o && a.step < len) Pn += ((CF)a.src_inc)[off + a.step];
         }
     }
-    bst<S>(B.out, L.own ? o * E : OFF_NONE, Store<S>::enc(Pn, a.isP, o, a.salt, a.sr));
+    bst<S>(B.out, L.own ? o * E : OFF_NONE, Store<S>::enc(Pn, a.isP, o, a.salt, TS::sr));
     vx_prev = vxn;
 }
 
@@ -484,11 +487,11 @@ __device__ __forceinline__ void march(const FusedArgs<S> &a, const Bufs &B, cons
 // fewer instructions per interior cell); otherwise one loop with every
 // group.  Both are held to at least 4 waves per SIMD (at most 128
 // registers per lane).
-template <typename S, int TJ, bool SPLIT>
+template <typename S, int TJ, bool SPLIT, bool SR>
 __global__ void __launch_bounds__(64 * TJ, 4)
 a3d_tile_step(FusedArgs<S> a)
 {
-    using TS = TileShape<S, TJ>;
+    using TS = TileShape<S, TJ, SR>;
     constexpr int NR = TS::NR;
     __shared__ __attribute__((aligned(16))) char s_tile[3 * TS::slot_bytes];
 
@@ -607,20 +610,26 @@ a3d_tile_step(FusedArgs<S> a)
     }
 }
 
-template <typename S, int TJ, bool SPLIT>
+// stochastic rounding exists only for the 16-bit storage types
+template <typename S>
+constexpr bool tile_has_sr() { return sizeof(S) == 2; }
+
+template <typename S, int TJ, bool SPLIT, bool SR>
 static const void *tile_kernel_ptr()
 {
-    return reinterpret_cast<const void *>(&a3d_tile_step<S, TJ, SPLIT>);
+    if constexpr (SR && !tile_has_sr<S>()) return nullptr;
+    else return reinterpret_cast<const void *>(&a3d_tile_step<S, TJ, SPLIT, SR>);
 }
 
 #define SM_TILE_CASES(X) X(1, false) X(2, false) X(4, false) X(8, false) \
                          X(1, true) X(2, true) X(4, true) X(8, true)
 
 template <typename S>
-int tile_max_blocks_per_cu(int tj, bool split, int *blocks)
+int tile_max_blocks_per_cu(int tj, bool split, bool sr, int *blocks)
 {
     const void *f = nullptr;
-#define SM_X(T, P) if (tj == T && split == P) f = tile_kernel_ptr<S, T, P>();
+#define SM_X(T, P) \
+    if (tj == T && split == P) f = sr ? tile_kernel_ptr<S, T, P, true>() : tile_kernel_ptr<S, T, P, false>();
     SM_TILE_CASES(SM_X)
 #undef SM_X
     if (!f) return SM_ERR_ARG;
@@ -638,7 +647,12 @@ int tile_launch(const FusedArgs<S> &a, int tj, bool split, hipStream_t st)
     bool done = false;
 #define SM_X(T, P)                                                                   \
     if (!done && tj == T && split == P) {                                            \
-        a3d_tile_step<S, T, P><<<blocks, dim3(64, T), 0, st>>>(a);                   \
+        if constexpr (tile_has_sr<S>()) {                                            \
+            if (a.sr) a3d_tile_step<S, T, P, true><<<blocks, dim3(64, T), 0, st>>>(a); \
+            else a3d_tile_step<S, T, P, false><<<blocks, dim3(64, T), 0, st>>>(a);    \
+        } else {                                                                     \
+            a3d_tile_step<S, T, P, false><<<blocks, dim3(64, T), 0, st>>>(a);        \
+        }                                                                            \
         done = true;                                                                 \
     }
     SM_TILE_CASES(SM_X)
@@ -650,8 +664,8 @@ int tile_launch(const FusedArgs<S> &a, int tj, bool split, hipStream_t st)
 template int tile_launch<float>(const FusedArgs<float> &, int, bool, hipStream_t);
 template int tile_launch<__hip_bfloat16>(const FusedArgs<__hip_bfloat16> &, int, bool, hipStream_t);
 template int tile_launch<__half>(const FusedArgs<__half> &, int, bool, hipStream_t);
-template int tile_max_blocks_per_cu<float>(int, bool, int *);
-template int tile_max_blocks_per_cu<__hip_bfloat16>(int, bool, int *);
-template int tile_max_blocks_per_cu<__half>(int, bool, int *);
+template int tile_max_blocks_per_cu<float>(int, bool, bool, int *);
+template int tile_max_blocks_per_cu<__hip_bfloat16>(int, bool, bool, int *);
+template int tile_max_blocks_per_cu<__half>(int, bool, bool, int *);
 
 }  // namespace sm
