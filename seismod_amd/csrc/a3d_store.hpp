@@ -68,16 +68,16 @@ struct Store<float> {
     }
 };
 
+// BF16 keeps FP32's exponent range, so its fields are never scaled (the
+// host leaves every BF16 scale at 1): the scale arguments are ignored, which
+// spares a multiply per access and keeps the scales out of the registers.
 template <>
 struct Store<__hip_bfloat16> {
     using Bits = unsigned short;
-    __device__ static __forceinline__ float dec(Bits b, float scale)
+    __device__ static __forceinline__ float dec(Bits b, float) { return __uint_as_float((unsigned)b << 16); }
+    __device__ static __forceinline__ Bits enc(float v, float, unsigned i, unsigned salt, int sr)
     {
-        return __uint_as_float((unsigned)b << 16) * scale;
-    }
-    __device__ static __forceinline__ Bits enc(float v, float inv, unsigned i, unsigned salt, int sr)
-    {
-        const float x = v * inv;
+        const float x = v;
         if (sr) {
             // add 16 random bits below the BF16 mantissa, then truncate
             const unsigned b = __float_as_uint(x) + (sr_bits(i, salt) & 0xFFFFu);
@@ -85,9 +85,9 @@ struct Store<__hip_bfloat16> {
         }
         return __builtin_bit_cast(Bits, __float2bfloat16(x));
     }
-    __device__ static __forceinline__ float ld(const __hip_bfloat16 *p, unsigned i, float scale)
+    __device__ static __forceinline__ float ld(const __hip_bfloat16 *p, unsigned i, float)
     {
-        return __bfloat162float(ld_at(p, i)) * scale;
+        return __bfloat162float(ld_at(p, i));
     }
     __device__ static __forceinline__ void st(__hip_bfloat16 *p, unsigned i, float v, float inv,
                                      unsigned salt, int sr)
@@ -178,20 +178,20 @@ struct Pair<__hip_bfloat16> {
     {
         return ld_at((const unsigned *)p, i >> 1);
     }
-    __device__ static __forceinline__ float2 unpack(Raw w, float scale)
+    // unscaled, as Store<__hip_bfloat16>
+    __device__ static __forceinline__ float2 unpack(Raw w, float)
     {
-        return make_float2(__uint_as_float(w << 16) * scale,
-                           __uint_as_float(w & 0xFFFF0000u) * scale);
+        return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xFFFF0000u));
     }
     __device__ static __forceinline__ float2 ld(const __hip_bfloat16 *p, unsigned i, float scale)
     {
         return unpack(raw(p, i), scale);
     }
-    __device__ static __forceinline__ void st(__hip_bfloat16 *p, unsigned i, float2 v, float inv,
+    __device__ static __forceinline__ void st(__hip_bfloat16 *p, unsigned i, float2 v, float,
                                      unsigned salt, int sr)
     {
         unsigned lo, hi;
-        const float x0 = v.x * inv, x1 = v.y * inv;
+        const float x0 = v.x, x1 = v.y;
         if (sr) {
             lo = (__float_as_uint(x0) + (sr_bits(i, salt) & 0xFFFFu)) >> 16;
             hi = (__float_as_uint(x1) + (sr_bits(i + 1, salt) & 0xFFFFu)) >> 16;
