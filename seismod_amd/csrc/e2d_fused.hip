@@ -15,6 +15,7 @@
 // Neighbours off the padded grid read 0 (pml.c:58, 79).
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "e2d_fused.hpp"
 #include "sm_internal.hpp"
@@ -140,42 +141,6 @@ e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot,
     // byte offset of row R in this lane, or OFF_NONE
     auto voff = [&](int R, bool ok) { return ok ? ((unsigned)R * NZ + J) * 4u : OFF_NONE; };
 
-    // ---- loads, issued a row ahead of the arithmetic that uses them ----
-    auto fetch_s = [&](int R, float &sxx, float &szz, float &sxz) {
-        const unsigned o = voff(R, rv(R) && jv);
-        sxx = bld(Rin, o, E2_TXX * N4);
-        szz = bld(Rin, o, E2_TZZ * N4);
-        sxz = bld(Rin, o, E2_TXZ * N4);
-    };
-    // velocity component at row R: total, or split pair in the PML
-    auto fetch_v = [&](int R, int tot, int sx, int sz, int cf) {
-        VRow r{0.f, 0.f, 0.f, 0.f};
-        const unsigned o = voff(R, rv(R) && jv);
-        r.coef = bld(Rmat, o, cf * N4);
-        if (rv(R) && (px(R) || zpml)) {
-            const bool pm = px(R) || pzl;
-            r.a = bld(Rin, pm ? o : OFF_NONE, sx * N4);
-            r.c = bld(Rin, pm ? o : OFF_NONE, sz * N4);
-            r.t = bld(Rin, pm ? OFF_NONE : o, tot * N4);
-        } else {
-            r.t = bld(Rin, o, tot * N4);
-        }
-        return r;
-    };
-    auto fetch_t = [&](int R) {
-        TRow r{};
-        const unsigned o = voff(R, jv);
-        r.la = bld(Rmat, o, E2_CLA * N4);
-        r.l2m = bld(Rmat, o, E2_CL2M * N4);
-        r.mu = bld(Rmat, o, E2_CMU * N4);
-        if (px(R) || zpml) {
-            const unsigned op = (px(R) || pzl) && own ? o : OFF_NONE;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) r.x[c] = bld(Rss, op, c * N4);
-        }
-        return r;
-    };
-
     // ---- staged loads (STAGED) ----
     // The wave's 64 cells [J0m, J0m + 64) of a field-row lie in a window of
     // 17 chunks starting at the 16-byte boundary at or before J0m, counted
@@ -192,15 +157,16 @@ e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot,
     auto fr_base = [&](int fr) { return (fr < 5 ? par_off : 0) + sg_field(fr) * (int)N; };
     // issue the chunk loads of iteration I (rows I+1, I+2); the lane's
     // chunk of round r is c = (r & 1) * 64 + lane of the state (r < 2) or
-    // material (r >= 2) field-rows, 17 chunks each
-    auto stage_issue = [&](int I, u32x4 (&v)[4]) {
+    // material (r >= 2) field-rows, 17 chunks each; go false: no memory
+    // access (the loads are still issued, so waits stay exact)
+    auto stage_issue = [&](int I, u32x4 (&v)[4], bool go) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int c = (r & 1) * 64 + lane;
             const int fr = (r < 2 ? 0 : 5) + c / SG_CHUNKS;
             const int R = I + ((fr <= 3 || fr == 5) ? 2 : 1);
             const int a = (fr_base(fr) + R * NZ + J0m) & ~3;
-            const bool ok = c < 5 * SG_CHUNKS && rv(R);
+            const bool ok = go && c < 5 * SG_CHUNKS && rv(R);
             const unsigned off = ok ? (unsigned)(a + 4 * (c % SG_CHUNKS)) * 4u : OFF_NONE;
             v[r] = __builtin_amdgcn_raw_buffer_load_b128(r < 2 ? Rall : Rmat16, off, 0, 0);
         }
@@ -217,43 +183,6 @@ e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot,
         return jv ? x : 0.f;
     };
 
-    // new vx of row R from old txx of rows R-1, R and old txz of row R
-    auto vx_new = [&](int R, const VRow &r, float sxx_m, float sxx_r, float sxz_r, bool store) {
-        const float dxs = sxx_r - sxx_m;
-        const float dzs = sxz_r - from_left(sxz_r);
-        const unsigned o = voff(R, store && own);
-        float vx = r.t + r.coef * (dxs + dzs);
-        if (rv(R) && (px(R) || zpml)) {
-            const bool pm = px(R) || pzl;
-            const float2 ax = cld(xi + R);
-            const float a = ax.x * r.a + ax.y * (r.coef * dxs);
-            const float c = zi.x * r.c + zi.y * (r.coef * dzs);
-            if (pm) vx = a + c;
-            bst(Rout, pm ? o : OFF_NONE, E2_VXX * N4, a);
-            bst(Rout, pm ? o : OFF_NONE, E2_VXZ * N4, c);
-        }
-        bst(Rout, o, E2_VX * N4, vx);
-        return vx;
-    };
-    // new vz of row R from old txz of rows R, R+1 and old tzz of row R
-    auto vz_new = [&](int R, const VRow &r, float sxz_r, float szz_r, float sxz_p, bool store) {
-        const float dxs = sxz_p - sxz_r;
-        const float dzs = from_right(szz_r) - szz_r;
-        const unsigned o = voff(R, store && own);
-        float vz = r.t + r.coef * (dxs + dzs);
-        if (rv(R) && (px(R) || zpml)) {
-            const bool pm = px(R) || pzl;
-            const float2 ax = cld(xh + R);
-            const float a = ax.x * r.a + ax.y * (r.coef * dxs);
-            const float c = zh.x * r.c + zh.y * (r.coef * dzs);
-            if (pm) vz = a + c;
-            bst(Rout, pm ? o : OFF_NONE, E2_VZX * N4, a);
-            bst(Rout, pm ? o : OFF_NONE, E2_VZZ * N4, c);
-        }
-        bst(Rout, o, E2_VZ * N4, vz);
-        return vz;
-    };
-
     const int nsrc = S->nsrc, nrec = S->nrec;
     const int4 *src = S->src;
     const int4 *rcv = S->rcv;
@@ -262,132 +191,263 @@ e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot,
     int sp = nsrc ? cld(S->srow + I0) : 0;
     int rp = record ? cld(S->rrow + I0) : 0;
 
-    // preamble: vx of row I0 and vz of row I0-1, from rows I0-1 and I0;
-    // the loads of the first iteration go out with them
-    float sxx_m, szz_m, sxz_m, sxx0, szz0, sxz0, sxx1, szz1, sxz1;
-    fetch_s(I0 - 1, sxx_m, szz_m, sxz_m);
-    fetch_s(I0, sxx0, szz0, sxz0);
-    const VRow v_pre = fetch_v(I0, E2_VX, E2_VXX, E2_VXZ, E2_CB);
-    const VRow z_pre = fetch_v(I0 - 1, E2_VZ, E2_VZX, E2_VZZ, E2_CBM);
-    fetch_s(I0 + 1, sxx1, szz1, sxz1);
-    VRow vr = fetch_v(I0 + 1, E2_VX, E2_VXX, E2_VXZ, E2_CB);
-    VRow zr = fetch_v(I0, E2_VZ, E2_VZX, E2_VZZ, E2_CBM);
-    TRow tr = fetch_t(I0);
-    u32x4 sv[4];
-    if constexpr (STAGED)
-        if (I0 + 1 < I1) stage_issue(I0, sv);
-    float vx0 = vx_new(I0, v_pre, sxx_m, sxx0, sxz0, true);
-    float vz_m = vz_new(I0 - 1, z_pre, sxz_m, szz_m, sxz0, false);
+    // The split fields are touched only in rows [I0-1, I1] of the x PML or
+    // by lanes in the z PML; a wave clear of both runs the loop compiled
+    // without them (PML false).  In both forms every per-lane load and store
+    // of the row loop is issued unconditionally, lanes and rows with nothing
+    // to move passing OFF_NONE: the compiler then knows how many accesses
+    // are in flight and waits only for the ones whose values it needs, where
+    // accesses behind branches made it wait for loads of the next row just
+    // issued.
+    const bool pmlw = zpml || px(I0 - 1) || px(I1);
+    auto body = [&](auto pml_tag) {
+        constexpr bool PML = decltype(pml_tag)::value;
 
-    for (int I = I0; I < I1; ++I) {
-        // loads of the next iteration
-        float nsxx = 0.f, nszz = 0.f, nsxz = 0.f;
-        VRow nvr{0.f, 0.f, 0.f, 0.f}, nzr{0.f, 0.f, 0.f, 0.f};
-        TRow ntr{};
-        if (I + 1 < I1) {
-            if constexpr (STAGED) {
-                stage_write(sv);
-                if (I + 2 < I1) stage_issue(I + 1, sv);
-                nsxx = stage_read(I, 0);
-                nszz = stage_read(I, 1);
-                nsxz = stage_read(I, 2);
-                nvr.t = stage_read(I, 3);
-                nvr.coef = stage_read(I, 5);
-                nzr.t = stage_read(I, 4);
-                nzr.coef = stage_read(I, 6);
-                ntr.la = stage_read(I, 7);
-                ntr.l2m = stage_read(I, 8);
-                ntr.mu = stage_read(I, 9);
-                // split velocities and stresses in the PML, per lane
-                // (totals of split lanes are loaded too and not used)
-                auto vsplit = [&](int R, int sx, int sz, VRow &r) {
-                    if (rv(R) && (px(R) || zpml)) {
-                        const bool pm = px(R) || pzl;
-                        const unsigned o = voff(R, rv(R) && jv);
-                        r.a = bld(Rin, pm ? o : OFF_NONE, sx * N4);
-                        r.c = bld(Rin, pm ? o : OFF_NONE, sz * N4);
-                    }
-                };
-                vsplit(I + 2, E2_VXX, E2_VXZ, nvr);
-                vsplit(I + 1, E2_VZX, E2_VZZ, nzr);
-                if (px(I + 1) || zpml) {
-                    const unsigned op = (px(I + 1) || pzl) && own ? voff(I + 1, jv) : OFF_NONE;
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) ntr.x[c] = bld(Rss, op, c * N4);
-                }
+        // ---- loads, issued a row ahead of the arithmetic that uses them;
+        // ok false: not needed (0, no memory access) ----
+        auto fetch_s = [&](int R, bool ok, float &sxx, float &szz, float &sxz) {
+            const unsigned o = voff(R, ok && rv(R) && jv);
+            sxx = bld(Rin, o, E2_TXX * N4);
+            szz = bld(Rin, o, E2_TZZ * N4);
+            sxz = bld(Rin, o, E2_TXZ * N4);
+        };
+        // velocity component at row R: total, or split pair where the
+        // lane's cell is in the PML
+        auto fetch_v = [&](int R, bool ok, int tot, int sx, int sz, int cf) {
+            VRow r{0.f, 0.f, 0.f, 0.f};
+            const unsigned o = voff(R, ok && rv(R) && jv);
+            r.coef = bld(Rmat, o, cf * N4);
+            if constexpr (PML) {
+                const bool pm = px(R) || pzl;
+                r.a = bld(Rin, pm ? o : OFF_NONE, sx * N4);
+                r.c = bld(Rin, pm ? o : OFF_NONE, sz * N4);
+                r.t = bld(Rin, pm ? OFF_NONE : o, tot * N4);
             } else {
-                fetch_s(I + 2, nsxx, nszz, nsxz);
-                nvr = fetch_v(I + 2, E2_VX, E2_VXX, E2_VXZ, E2_CB);
-                nzr = fetch_v(I + 1, E2_VZ, E2_VZX, E2_VZZ, E2_CBM);
-                ntr = fetch_t(I + 1);
+                r.t = bld(Rin, o, tot * N4);
             }
-        }
-
-        const float vx1 = vx_new(I + 1, vr, sxx0, sxx1, sxz1, I + 1 < I1);
-        const float vz0 = vz_new(I, zr, sxz0, szz0, sxz1, true);
-
-        // stress of row I
-        const float dvxx = vx1 - vx0;
-        const float dvzz = vz0 - from_left(vz0);
-        const float dvxz = from_right(vx0) - vx0;
-        const float dvzx = vz0 - vz_m;
-        const unsigned o = voff(I, own);
-        float nxx = sxx0 + (tr.l2m * dvxx + tr.la * dvzz);
-        float nzz = szz0 + (tr.l2m * dvzz + tr.la * dvxx);
-        float nxz = sxz0 + tr.mu * (dvxz + dvzx);
-        if (px(I) || zpml) {
-            const bool pm = px(I) || pzl;
-            const float2 ah = cld(xh + I), ai = cld(xi + I);
-            const float txxx = ah.x * tr.x[E2_TXXX] + ah.y * (tr.l2m * dvxx);
-            const float txxz = zi.x * tr.x[E2_TXXZ] + zi.y * (tr.la * dvzz);
-            const float tzzx = ah.x * tr.x[E2_TZZX] + ah.y * (tr.la * dvxx);
-            const float tzzz = zi.x * tr.x[E2_TZZZ] + zi.y * (tr.l2m * dvzz);
-            const float txzx = ai.x * tr.x[E2_TXZX] + ai.y * (tr.mu * dvzx);
-            const float txzz = zh.x * tr.x[E2_TXZZ] + zh.y * (tr.mu * dvxz);
-            if (pm) {
-                nxx = txxx + txxz;
-                nzz = tzzx + tzzz;
-                nxz = txzx + txzz;
+            return r;
+        };
+        auto fetch_t = [&](int R, bool ok) {
+            TRow r{};
+            const unsigned o = voff(R, ok && rv(R) && jv);
+            r.la = bld(Rmat, o, E2_CLA * N4);
+            r.l2m = bld(Rmat, o, E2_CL2M * N4);
+            r.mu = bld(Rmat, o, E2_CMU * N4);
+            if constexpr (PML) {
+                const unsigned op = (px(R) || pzl) && own ? o : OFF_NONE;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) r.x[c] = bld(Rss, op, c * N4);
             }
-            const unsigned op = pm ? o : OFF_NONE;
-            bst(Rss, op, E2_TXXX * N4, txxx); bst(Rss, op, E2_TXXZ * N4, txxz);
-            bst(Rss, op, E2_TZZX * N4, tzzx); bst(Rss, op, E2_TZZZ * N4, tzzz);
-            bst(Rss, op, E2_TXZX * N4, txzx); bst(Rss, op, E2_TXZZ * N4, txzz);
-        }
+            return r;
+        };
 
-        // add_src (src.c:170-184) on this row, after both phases
-        const int rowend = (I + 1) * NZ;
-        for (; sp < nsrc; ++sp) {
-            const int4 e = cld(src + sp);
-            if (e.x >= rowend) break;
-            if (it < e.z && own && J == e.x - I * NZ) {
-                const float inc = cld(S->sfct + e.w + it);
-                if (e.y == SM_SX || e.y == SM_BULK) nxx += inc;
-                if (e.y == SM_SZ || e.y == SM_BULK) nzz += inc;
-                if (e.y == SM_SXZ) nxz += inc;
+        // new vx of row R from old txx of rows R-1, R and old txz of row R
+        auto vx_new = [&](int R, const VRow &r, float sxx_m, float sxx_r, float sxz_r, bool store) {
+            const float dxs = sxx_r - sxx_m;
+            const float dzs = sxz_r - from_left(sxz_r);
+            const unsigned o = voff(R, store && own);
+            float vx = r.t + r.coef * (dxs + dzs);
+            if constexpr (PML) {
+                // (R lies in [0, NX]: the damping table read stays in the
+                // allocation; off the grid nothing is kept or stored)
+                const bool pm = rv(R) && (px(R) || pzl);
+                const float2 ax = cld(xi + R);
+                const float a = ax.x * r.a + ax.y * (r.coef * dxs);
+                const float c = zi.x * r.c + zi.y * (r.coef * dzs);
+                if (pm) vx = a + c;
+                bst(Rout, pm ? o : OFF_NONE, E2_VXX * N4, a);
+                bst(Rout, pm ? o : OFF_NONE, E2_VXZ * N4, c);
             }
-        }
-        bst(Rout, o, E2_TXX * N4, nxx);
-        bst(Rout, o, E2_TZZ * N4, nzz);
-        bst(Rout, o, E2_TXZ * N4, nxz);
-        // receivers on this row (e_iso.c:641-669 samples after add_src)
-        if (record) {
-            for (; rp < nrec; ++rp) {
-                const int4 e = cld(rcv + rp);
+            bst(Rout, o, E2_VX * N4, vx);
+            return vx;
+        };
+        // new vz of row R from old txz of rows R, R+1 and old tzz of row R
+        auto vz_new = [&](int R, const VRow &r, float sxz_r, float szz_r, float sxz_p, bool store) {
+            const float dxs = sxz_p - sxz_r;
+            const float dzs = from_right(szz_r) - szz_r;
+            const unsigned o = voff(R, store && own);
+            float vz = r.t + r.coef * (dxs + dzs);
+            if constexpr (PML) {
+                // (R lies in [-1, NX): the table read stays in the allocation)
+                const bool pm = rv(R) && (px(R) || pzl);
+                const float2 ax = cld(xh + R);
+                const float a = ax.x * r.a + ax.y * (r.coef * dxs);
+                const float c = zh.x * r.c + zh.y * (r.coef * dzs);
+                if (pm) vz = a + c;
+                bst(Rout, pm ? o : OFF_NONE, E2_VZX * N4, a);
+                bst(Rout, pm ? o : OFF_NONE, E2_VZZ * N4, c);
+            }
+            bst(Rout, o, E2_VZ * N4, vz);
+            return vz;
+        };
+
+        // Rows of one iteration: stress of row I+1, new-velocity inputs of
+        // rows I+1 (vx) and I (vz), stress inputs of row I
+        struct RowIn {
+            float sxx, szz, sxz;
+            VRow vr, zr;
+            TRow tr;
+        };
+        // per-lane loads of the inputs of row r (none where ok is false)
+        auto load_in = [&](int r, bool ok, RowIn &n) {
+            fetch_s(r + 1, ok, n.sxx, n.szz, n.sxz);
+            n.vr = fetch_v(r + 1, ok, E2_VX, E2_VXX, E2_VXZ, E2_CB);
+            n.zr = fetch_v(r, ok, E2_VZ, E2_VZX, E2_VZZ, E2_CBM);
+            n.tr = fetch_t(r, ok);
+        };
+
+        // preamble: vx of row I0 and vz of row I0-1, from rows I0-1 and I0;
+        // the loads of the first row go out with them.  A: inputs of row I0;
+        // C.sxx..: stress of row I0
+        float sxx_m, szz_m, sxz_m;
+        RowIn A, C;
+        fetch_s(I0 - 1, true, sxx_m, szz_m, sxz_m);
+        fetch_s(I0, true, C.sxx, C.szz, C.sxz);
+        const VRow v_pre = fetch_v(I0, true, E2_VX, E2_VXX, E2_VXZ, E2_CB);
+        const VRow z_pre = fetch_v(I0 - 1, true, E2_VZ, E2_VZX, E2_VZZ, E2_CBM);
+        load_in(I0, true, A);
+        u32x4 sv[4];
+        if constexpr (STAGED)
+            if (I0 + 1 < I1) stage_issue(I0, sv, true);
+        float vx0 = vx_new(I0, v_pre, sxx_m, C.sxx, C.sxz, true);
+        float vz_m = vz_new(I0 - 1, z_pre, sxz_m, szz_m, C.sxz, false);
+
+        // row I: new vx of row I+1, vz of row I, stress of row I, sources
+        // and receivers; sxx0..: stress of row I, c: the rest of its inputs
+        auto do_row = [&](int I, bool more, float sxx0, float szz0, float sxz0, const RowIn &c) {
+            const float vx1 = vx_new(I + 1, c.vr, sxx0, c.sxx, c.sxz, more);
+            const float vz0 = vz_new(I, c.zr, sxz0, szz0, c.sxz, true);
+            const TRow &tr = c.tr;
+
+            // stress of row I
+            const float dvxx = vx1 - vx0;
+            const float dvzz = vz0 - from_left(vz0);
+            const float dvxz = from_right(vx0) - vx0;
+            const float dvzx = vz0 - vz_m;
+            const unsigned o = voff(I, own);
+            float nxx = sxx0 + (tr.l2m * dvxx + tr.la * dvzz);
+            float nzz = szz0 + (tr.l2m * dvzz + tr.la * dvxx);
+            float nxz = sxz0 + tr.mu * (dvxz + dvzx);
+            if constexpr (PML) {
+                const bool pm = px(I) || pzl;
+                const float2 ah = cld(xh + I), ai = cld(xi + I);
+                const float txxx = ah.x * tr.x[E2_TXXX] + ah.y * (tr.l2m * dvxx);
+                const float txxz = zi.x * tr.x[E2_TXXZ] + zi.y * (tr.la * dvzz);
+                const float tzzx = ah.x * tr.x[E2_TZZX] + ah.y * (tr.la * dvxx);
+                const float tzzz = zi.x * tr.x[E2_TZZZ] + zi.y * (tr.l2m * dvzz);
+                const float txzx = ai.x * tr.x[E2_TXZX] + ai.y * (tr.mu * dvzx);
+                const float txzz = zh.x * tr.x[E2_TXZZ] + zh.y * (tr.mu * dvxz);
+                if (pm) {
+                    nxx = txxx + txxz;
+                    nzz = tzzx + tzzz;
+                    nxz = txzx + txzz;
+                }
+                const unsigned op = pm ? o : OFF_NONE;
+                bst(Rss, op, E2_TXXX * N4, txxx); bst(Rss, op, E2_TXXZ * N4, txxz);
+                bst(Rss, op, E2_TZZX * N4, tzzx); bst(Rss, op, E2_TZZZ * N4, tzzz);
+                bst(Rss, op, E2_TXZX * N4, txzx); bst(Rss, op, E2_TXZZ * N4, txzz);
+            }
+
+            // add_src (src.c:170-184) on this row, after both phases
+            const int rowend = (I + 1) * NZ;
+            for (; sp < nsrc; ++sp) {
+                const int4 e = cld(src + sp);
                 if (e.x >= rowend) break;
-                if (own && J == e.x - I * NZ) {
-                    const float val = e.y == 0 ? vx0 : e.y == 1 ? vz0 : e.y == 2 ? nxx
-                                    : e.y == 3 ? nzz : nxz;
-                    *(__attribute__((address_space(1))) float *)(S->trace + (size_t)row * nrec + e.z) = val;
+                if (it < e.z && own && J == e.x - I * NZ) {
+                    const float inc = cld(S->sfct + e.w + it);
+                    if (e.y == SM_SX || e.y == SM_BULK) nxx += inc;
+                    if (e.y == SM_SZ || e.y == SM_BULK) nzz += inc;
+                    if (e.y == SM_SXZ) nxz += inc;
+                }
+            }
+            bst(Rout, o, E2_TXX * N4, nxx);
+            bst(Rout, o, E2_TZZ * N4, nzz);
+            bst(Rout, o, E2_TXZ * N4, nxz);
+            // receivers on this row (e_iso.c:641-669 samples after add_src)
+            if (record) {
+                for (; rp < nrec; ++rp) {
+                    const int4 e = cld(rcv + rp);
+                    if (e.x >= rowend) break;
+                    if (own && J == e.x - I * NZ) {
+                        const float val = e.y == 0 ? vx0 : e.y == 1 ? vz0 : e.y == 2 ? nxx
+                                        : e.y == 3 ? nzz : nxz;
+                        *(__attribute__((address_space(1))) float *)(S->trace + (size_t)row * nrec + e.z) = val;
+                    }
+                }
+            }
+            vx0 = vx1;
+            vz_m = vz0;
+        };
+
+        if constexpr (STAGED) {
+            // (two alternating sets of chunk registers, loads issued before
+            // the wait for the previous row's, took 103 instead of 85
+            // registers, 4 instead of 5 waves per SIMD, and were slower)
+            RowIn cur = A;
+            float sxx0 = C.sxx, szz0 = C.szz, sxz0 = C.sxz;
+            for (int I = I0; I < I1; ++I) {
+                // loads of the next iteration, staged (none after the
+                // chunk's last row)
+                const bool more = I + 1 < I1;
+                RowIn nxt{0.f, 0.f, 0.f, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, TRow{}};
+                if (more) {
+                    stage_write(sv);
+                    if (I + 2 < I1) stage_issue(I + 1, sv, true);
+                    nxt.sxx = stage_read(I, 0);
+                    nxt.szz = stage_read(I, 1);
+                    nxt.sxz = stage_read(I, 2);
+                    nxt.vr.t = stage_read(I, 3);
+                    nxt.vr.coef = stage_read(I, 5);
+                    nxt.zr.t = stage_read(I, 4);
+                    nxt.zr.coef = stage_read(I, 6);
+                    nxt.tr.la = stage_read(I, 7);
+                    nxt.tr.l2m = stage_read(I, 8);
+                    nxt.tr.mu = stage_read(I, 9);
+                    if constexpr (PML) {
+                        // split velocities and stresses, per lane (totals of
+                        // split lanes are staged too and not used)
+                        auto vsplit = [&](int R, int sx, int sz, VRow &r) {
+                            const bool pm = rv(R) && (px(R) || pzl);
+                            const unsigned o = voff(R, pm && jv);
+                            r.a = bld(Rin, o, sx * N4);
+                            r.c = bld(Rin, o, sz * N4);
+                        };
+                        vsplit(I + 2, E2_VXX, E2_VXZ, nxt.vr);
+                        vsplit(I + 1, E2_VZX, E2_VZZ, nxt.zr);
+                        const unsigned op = (px(I + 1) || pzl) && own ? voff(I + 1, jv) : OFF_NONE;
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) nxt.tr.x[c] = bld(Rss, op, c * N4);
+                    }
+                }
+                do_row(I, more, sxx0, szz0, sxz0, cur);
+                sxx0 = cur.sxx; szz0 = cur.szz; sxz0 = cur.sxz;
+                cur = nxt;
+            }
+        } else {
+            // Two rows per trip with two register sets: the loads of one
+            // row land in the set the other row has finished with, so no
+            // value still in flight is ever copied (a copy would have to
+            // wait for it).  The stress of row I is taken out of that set
+            // before it is reloaded; it arrived a row ago.  (Loading two rows
+            // ahead from three sets took 84 instead of 62 registers, 5
+            // instead of 7 waves per SIMD, and made single shots slower.)
+            for (int I = I0; I < I1; I += 2) {
+                {
+                    const float x0 = C.sxx, z0 = C.szz, xz0 = C.sxz;
+                    load_in(I + 1, I + 1 < I1, C);
+                    do_row(I, I + 1 < I1, x0, z0, xz0, A);
+                }
+                if (I + 1 >= I1) break;
+                {
+                    const float x0 = A.sxx, z0 = A.szz, xz0 = A.sxz;
+                    load_in(I + 2, I + 2 < I1, A);
+                    do_row(I + 1, I + 2 < I1, x0, z0, xz0, C);
                 }
             }
         }
-        sxx0 = sxx1; szz0 = szz1; sxz0 = sxz1;
-        sxx1 = nsxx; szz1 = nszz; sxz1 = nsxz;
-        vr = nvr; zr = nzr; tr = ntr;
-        vx0 = vx1;
-        vz_m = vz0;
-    }
+    };
+    if (pmlw) body(std::true_type{});
+    else body(std::false_type{});
 }
 
 __global__ void e2d_base_set(int *base, int value, int add)
