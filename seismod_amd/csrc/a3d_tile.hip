@@ -17,10 +17,11 @@
 //     so every row of every field crosses the memory pipeline once per
 //     plane, in whole chunks, and all neighbour values come from LDS.
 //   - three plane slots rotate: plane i is updated from slots i and i+1
-//     while plane i+2 is written to the third.  Chunk loads and CPML
-//     memory-variable loads are issued two planes ahead of their use, into
-//     two alternating register sets, so HBM latency hides behind two planes
-//     of work.  One barrier per plane.
+//     while plane i+2 is written to the third.  Chunk loads are issued two
+//     planes ahead of their use and CPML memory-variable loads one plane
+//     ahead, each into two register sets that the two planes of a loop trip
+//     use in turn (no value still in flight is ever copied), so HBM latency
+//     hides behind two planes of work.  One barrier per plane.
 //   - every vector-memory access inside the plane loop is unconditional: a
 //     lane with nothing to load or store passes an offset past the end of
 //     its buffer resource (OFF_NONE), for which the hardware returns 0 and
@@ -375,36 +376,37 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
 }
 
 // Loop state of a chunk [i0, i1).  At plane i the register set of the
-// parity of i - i0 holds the chunks of plane i+2, m0 / m1 the memory
-// variables of planes i / i+1; slots s0, s1, s2 hold planes i, i+1 and the
-// free one.
+// parity of i - i0 holds the chunks of plane i+2, and the memory-variable
+// set of that parity (m0 even, m1 odd) those of plane i while the other
+// receives plane i+1's; slots s0, s1, s2 hold planes i, i+1 and the free
+// one.
 struct Ring {
     int s0, s1, s2;
     Psi m0, m1;
     float vx_prev;
 };
 
-// one plane of a range: memory variables of plane i+2 issued, plane i
-// updated, plane i+2 (in v) written to the free slot, loads of plane i+4
-// issued into v
+// one plane of a range: memory variables of plane i+1 issued into mn,
+// plane i updated with those in mc, plane i+2 (in v) written to the free
+// slot, loads of plane i+4 issued into v.  mc and mn are fixed register
+// sets used in turn by the two planes of a trip: a rotation by copy
+// (m0 = m1; m1 = m2) made every plane wait for the loads it had just
+// issued and, as loads complete in order, for the chunk loads before them.
 template <typename S, typename TS, bool X, bool Y, bool Z>
 __device__ __forceinline__ void plane_iter(const FusedArgs<S> &a, const Bufs &B, const TLane &L,
                                            const Halo &hz, const unsigned (&coff)[TS::NR], int w,
                                            int tid, unsigned plane, int i1, int i, char *base,
-                                           u32x4 (&v)[TS::NR], Ring &R)
+                                           u32x4 (&v)[TS::NR], Ring &R, const Psi &mc, Psi &mn)
 {
     constexpr int NR = TS::NR;
-    Psi m2;
-    psi_issue<S, X, Y, Z>(a, B, L, i + 2, i + 2 < i1, plane, m2);
+    psi_issue<S, X, Y, Z>(a, B, L, i + 1, i + 1 < i1, plane, mn);
     tile_plane<S, TS, X, Y, Z>(a, B, L, i, plane, base + R.s0 * TS::slot_bytes,
-                               base + R.s1 * TS::slot_bytes, R.m0, hz, R.vx_prev);
+                               base + R.s1 * TS::slot_bytes, mc, hz, R.vx_prev);
     // plane i+2 into the slot plane i-1 used (read last before the previous
     // barrier), then the loads of plane i+4 into the same register set
     stage_write<TS, NR>(base + R.s2 * TS::slot_bytes, tid, v);
     __syncthreads();
     stage_issue<TS, NR>(B, coff, w, plane, i + 4, i1, a.nx2 - 1, v);
-    R.m0 = R.m1;
-    R.m1 = m2;
     const int t = R.s0;
     R.s0 = R.s1;
     R.s1 = R.s2;
@@ -412,7 +414,8 @@ __device__ __forceinline__ void plane_iter(const FusedArgs<S> &a, const Bufs &B,
 }
 
 // Planes [ib, ie) of a chunk ending at i1 with the x flag fixed, two per
-// trip: v0 holds the chunks of plane ib+2, v1 those of plane ib+3.
+// trip: v0 holds the chunks of plane ib+2, v1 those of plane ib+3; R.m0
+// the memory variables of plane ib (ib - i0 is even).
 template <typename S, typename TS, bool X, bool Y, bool Z>
 __device__ __forceinline__ void plane_range(const FusedArgs<S> &a, const Bufs &B, const TLane &L,
                                             const Halo &hz, const unsigned (&coff)[TS::NR], int w,
@@ -421,16 +424,17 @@ __device__ __forceinline__ void plane_range(const FusedArgs<S> &a, const Bufs &B
                                             Ring &R)
 {
     for (int i = ib; i < ie; i += 2) {
-        plane_iter<S, TS, X, Y, Z>(a, B, L, hz, coff, w, tid, plane, i1, i, base, v0, R);
+        plane_iter<S, TS, X, Y, Z>(a, B, L, hz, coff, w, tid, plane, i1, i, base, v0, R, R.m0, R.m1);
         if (i + 1 >= ie) break;
-        plane_iter<S, TS, X, Y, Z>(a, B, L, hz, coff, w, tid, plane, i1, i + 1, base, v1, R);
+        plane_iter<S, TS, X, Y, Z>(a, B, L, hz, coff, w, tid, plane, i1, i + 1, base, v1, R, R.m1,
+                                   R.m0);
     }
 }
 
 // One work item: planes [i0, i1) of one tile.  Y, Z: the tile has cells
 // (or neighbours it recomputes) in a y / z slab.  SPLIT: the chunk is
-// marched in up to three ranges, x-slab planes at either end (or planes two
-// before them, whose memory variables are loaded ahead) with the x group
+// marched in up to three ranges, x-slab planes at either end (and the planes
+// before the upper ones, which load their memory variables) with the x group
 // and the interior without; otherwise in one range with every group on.
 template <typename S, typename TS, bool Y, bool Z, bool SPLIT>
 __device__ __forceinline__ void march(const FusedArgs<S> &a, const Bufs &B, const TLane &L,
@@ -464,12 +468,13 @@ __device__ __forceinline__ void march(const FusedArgs<S> &a, const Bufs &B, cons
     stage_issue<TS, NR>(B, coff, w, plane, i0 + 2, pmax, plast, va);
     psi_issue<S, true, Y, Z>(a, B, L, i0, true, plane, R.m0);
     stage_issue<TS, NR>(B, coff, w, plane, i0 + 3, pmax, plast, vb);
-    psi_issue<S, true, Y, Z>(a, B, L, i0 + 1, i0 + 1 < i1, plane, R.m1);
     // x-slab planes: i < np or i >= nx2-1-np (velocity or pressure set);
-    // the x group is on where plane i or plane i+2 is one.  The interior
-    // range [b1, b2) starts and has a length even in planes from i0, so
-    // every range starts with register set va (the x ranges grow by at
-    // most one plane; the x group is correct anywhere)
+    // the x group must be on where plane i or plane i+1 (whose memory
+    // variables plane i loads) is one, and is on two planes ahead of the
+    // upper slab.  The interior range [b1, b2) starts and has a length even
+    // in planes from i0, so every range starts with register set va and
+    // memory-variable set m0 (the x ranges grow by at most one plane; the x
+    // group is correct anywhere)
     const int b1 = min(i0 + (max(a.np - i0, 0) + 1) / 2 * 2, i1);
     const int b2 = b1 + max(min(a.nx2 - 3 - a.np, i1) - b1, 0) / 2 * 2;
     if constexpr (SPLIT) {
