@@ -281,8 +281,9 @@ e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot,
             return vz;
         };
 
-        // Rows of one iteration: stress of row I+1, new-velocity inputs of
-        // rows I+1 (vx) and I (vz), stress inputs of row I
+        // Inputs of row r beyond the stress of row r itself: the stress of
+        // row r+1, the old vx (and coefficient) of row r+1, the old vz of row
+        // r, and row r's stress coefficients (and split stresses)
         struct RowIn {
             float sxx, szz, sxz;
             VRow vr, zr;
