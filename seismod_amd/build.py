@@ -73,8 +73,8 @@ def build_host() -> list[Path]:
     the host I/O layer alone as libsm_hostio.so (tested on CPU)."""
     BINDIR.mkdir(exist_ok=True)
     LIBDIR.mkdir(exist_ok=True)
-    _run(["gcc", *C_FLAGS, "-fPIC", "-shared", f"-I{HOST}", str(HOST / "sm_io.c"), "-lm",
-          "-o", str(LIBDIR / "libsm_hostio.so")])
+    _run(["gcc", *C_FLAGS, "-fopenmp", "-fPIC", "-shared", f"-I{HOST}", str(HOST / "sm_io.c"),
+          "-lm", "-o", str(LIBDIR / "libsm_hostio.so")])
     common = sorted(p for p in HOST.glob("*.c") if p.stem not in ("e_iso", "a_iso_3d"))
     outs = []
     for prog in ("e_iso", "a_iso_3d"):
@@ -82,7 +82,7 @@ def build_host() -> list[Path]:
         if not src.exists():
             continue
         out = BINDIR / prog
-        _run(["gcc", *C_FLAGS, f"-I{INCLUDE}", f"-I{HOST}", str(src), *map(str, common),
+        _run(["gcc", *C_FLAGS, "-fopenmp", f"-I{INCLUDE}", f"-I{HOST}", str(src), *map(str, common),
               f"-L{LIBDIR}", "-lseismod_hip", "-lm",
               "-Wl,-rpath,$ORIGIN/../lib", "-o", str(out)])
         outs.append(out)

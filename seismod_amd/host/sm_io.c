@@ -18,6 +18,9 @@
 #include <sys/stat.h>
 #include <time.h>
 #include <unistd.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 int sm_verbose = 0;
 
@@ -150,26 +153,250 @@ void sm_read_grid3d(const char *fn, sm_grid *g)
     fclose(f);
 }
 
+/*
+ * Model bodies: the numbers after the header, whitespace-separated, read in
+ * parallel.  Each thread reads its share of the rest of the file into one
+ * buffer; the buffer is split into one range per thread at whitespace, the
+ * numbers of each range are counted, and each thread then converts its own
+ * exactly: fast_number where it applies, else strtod, the conversion
+ * fscanf's %lf uses (both correctly rounded, so the values are
+ * bit-identical to the reference's).  *nvalid is the count of leading
+ * numbers before the first token strtod does not take whole (or the end);
+ * out receives the first min(nvalid, want) of them.
+ */
+static int is_space(char c) { return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+
+/* Decimal number at p in the common form [sign] digits [. digits]
+ * [e|E [sign] digits], converted exactly without strtod when its digits
+ * make an integer M <= 2^53 and its power of ten 10^E has |E| <= 22: both
+ * are then exact doubles and one IEEE multiply or divide gives the
+ * correctly rounded value (Clinger's fast path), the same as strtod.
+ * Returns the end of the number, or NULL when the text is not in that form
+ * or outside that range (the caller then uses strtod). */
+static const char *fast_number(const char *p, double *out)
+{
+    static const double p10[23] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8, 1e9, 1e10, 1e11,
+                                   1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18, 1e19, 1e20, 1e21,
+                                   1e22};
+    int neg = 0, nd = 0, any = 0;
+    long e10 = 0;
+    uint64_t m = 0;
+    if (*p == '-' || *p == '+') neg = *p++ == '-';
+    for (; *p >= '0' && *p <= '9'; ++p) {
+        any = 1;
+        if (m == 0 && *p == '0') continue;           /* leading zeros */
+        if (++nd > 19) return NULL;
+        m = m * 10 + (uint64_t)(*p - '0');
+    }
+    if (*p == '.') {
+        for (++p; *p >= '0' && *p <= '9'; ++p) {
+            any = 1;
+            if (m == 0 && *p == '0') { --e10; continue; }
+            if (++nd > 19) return NULL;
+            m = m * 10 + (uint64_t)(*p - '0');
+            --e10;
+        }
+    }
+    if (!any) return NULL;
+    if (*p == 'e' || *p == 'E') {
+        const char *q = p + 1;
+        int eneg = 0;
+        long x = 0;
+        if (*q == '-' || *q == '+') eneg = *q++ == '-';
+        if (!(*q >= '0' && *q <= '9')) return NULL;
+        for (; *q >= '0' && *q <= '9'; ++q)
+            if ((x = x * 10 + (*q - '0')) > 100000) return NULL;
+        e10 += eneg ? -x : x;
+        p = q;
+    }
+    if (m > ((uint64_t)1 << 53)) return NULL;
+    double v = (double)m;
+    if (m != 0) {
+        if (e10 < -22 || e10 > 22) return NULL;
+        v = e10 < 0 ? v / p10[-e10] : v * p10[e10];
+    }
+    *out = neg ? -v : v;
+    return p;
+}
+
+static double *read_body(FILE *f, const char *fn, size_t want, size_t *nvalid)
+{
+    const long start = ftell(f);
+    if (start < 0 || fseek(f, 0, SEEK_END) != 0) die("Error: cannot read file %s\n", fn);
+    const long end = ftell(f);
+    if (end < start || fseek(f, start, SEEK_SET) != 0) die("Error: cannot read file %s\n", fn);
+    const size_t len = (size_t)(end - start);
+    char *buf = malloc(len + 1);
+    double *out = malloc((want ? want : 1) * sizeof(double));
+    if (!buf || !out) die("Error: out of memory reading %s\n", fn);
+    buf[len] = '\0';
+
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    if ((size_t)nt > len / 65536 + 1) nt = (int)(len / 65536 + 1);
+    /* each thread reads its share of the file */
+    int rd_ok = 1;
+    const int fd = fileno(f);
+#pragma omp parallel for num_threads(nt) schedule(static, 1) reduction(&& : rd_ok)
+    for (int t = 0; t < nt; ++t) {
+        size_t a = len / nt * t, b = t + 1 == nt ? len : len / nt * (t + 1);
+        while (a < b) {
+            const ssize_t r = pread(fd, buf + a, b - a, (off_t)start + (off_t)a);
+            if (r <= 0) { rd_ok = 0; break; }
+            a += (size_t)r;
+        }
+    }
+    if (!rd_ok) die("Error: cannot read file %s\n", fn);
+    size_t *lo = malloc((nt + 1) * sizeof(size_t)), *cnt = calloc(nt + 1, sizeof(size_t));
+    size_t *bad = malloc(nt * sizeof(size_t));
+    if (!lo || !cnt || !bad) die("Error: out of memory reading %s\n", fn);
+    /* range t starts at a whitespace character (or 0) and ends where t+1 starts */
+    lo[0] = 0;
+    lo[nt] = len;
+    for (int t = 1; t < nt; ++t) {
+        size_t q = len / nt * t;
+        if (q < lo[t - 1]) q = lo[t - 1];
+        while (q < len && !is_space(buf[q])) ++q;
+        lo[t] = q;
+    }
+    /* numbers (tokens) whose first character lies in each range */
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+    for (int t = 0; t < nt; ++t) {
+        size_t c = 0;
+        for (size_t q = lo[t]; q < lo[t + 1]; ++q)
+            if (!is_space(buf[q]) && (q == 0 || is_space(buf[q - 1]))) ++c;
+        cnt[t + 1] = c;
+    }
+    for (int t = 0; t < nt; ++t) cnt[t + 1] += cnt[t];   /* first token index of range t+1 */
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+    for (int t = 0; t < nt; ++t) {
+        size_t k = cnt[t];
+        bad[t] = (size_t)-1;
+        size_t q = lo[t];
+        while (q < lo[t + 1]) {
+            if (is_space(buf[q])) { ++q; continue; }
+            double v;
+            const char *e = fast_number(buf + q, &v);
+            if (!e || (*e != '\0' && !is_space(*e))) {
+                char *e2;
+                v = strtod(buf + q, &e2);
+                e = e2;
+            }
+            if (e == buf + q || (*e != '\0' && !is_space(*e))) { bad[t] = k; break; }
+            if (k < want) out[k] = v;
+            ++k;
+            q = (size_t)(e - buf);
+        }
+    }
+    size_t n = cnt[nt];
+    for (int t = 0; t < nt; ++t)
+        if (bad[t] < n) n = bad[t];
+    *nvalid = n;
+    free(lo);
+    free(cnt);
+    free(bad);
+    free(buf);
+    return out;
+}
+
+/*
+ * Model writers: the text read_model_e / read_model_a expect (a header of
+ * dimensions, spacings, origin and model type, then one line per cell), with
+ * every value printed with `digits` significant digits (17 makes the text
+ * read back to the very same doubles).  Cells are formatted in parallel in
+ * blocks and written in order.
+ */
+static int write_model_body(FILE *f, size_t ncell, int nval, const double *const *vals, int digits)
+{
+    const size_t block = (size_t)1 << 20;
+    const size_t line_max = (size_t)nval * 32 + 2;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    char *buf = malloc(block * line_max);
+    size_t *len = malloc((size_t)nt * sizeof(size_t));
+    if (!buf || !len) { free(buf); free(len); return -1; }
+    int rc = 0;
+    for (size_t c0 = 0; c0 < ncell && rc == 0; c0 += block) {
+        const size_t nb = ncell - c0 < block ? ncell - c0 : block;
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+        for (int t = 0; t < nt; ++t) {
+            const size_t a = nb * t / nt, b = nb * (t + 1) / nt;
+            char *o = buf + a * line_max;
+            size_t n = 0;
+            for (size_t c = a; c < b; ++c)
+                for (int v = 0; v < nval; ++v)
+                    n += (size_t)snprintf(o + n, 32, v + 1 < nval ? "%.*g " : "%.*g\n", digits,
+                                          vals[v][c0 + c]);
+            len[t] = n;
+        }
+        for (int t = 0; t < nt && rc == 0; ++t)
+            if (len[t] && fwrite(buf + nb * t / nt * line_max, 1, len[t], f) != len[t]) rc = -1;
+    }
+    free(buf);
+    free(len);
+    return rc;
+}
+
+int sm_write_model_a(const char *fn, size_t nx, size_t ny, size_t nz, double dx, double dy,
+                     double dz, double x0, double y0, double z0, const double *vp,
+                     const double *rho, int digits)
+{
+    /* the format io_utils.c:1029-1072 reads; vp, rho: nx*ny*nz, z fastest */
+    FILE *f = fopen(fn, "w");
+    if (!f) return -1;
+    const double *vals[2] = {vp, rho};
+    int rc = fprintf(f, "%zu %zu %zu\n%.17g %.17g %.17g\n%.17g %.17g %.17g\nisotropic_acoustic_3d\n",
+                     nx, ny, nz, dx, dy, dz, x0, y0, z0) < 0 ? -1 : 0;
+    if (rc == 0) rc = write_model_body(f, nx * ny * nz, 2, vals, digits);
+    if (fclose(f) != 0) rc = -1;
+    return rc;
+}
+
+int sm_write_model_e(const char *fn, size_t nx, size_t nz, double dx, double dz, double x0,
+                     double z0, const double *vp, const double *vs, const double *rho, int digits)
+{
+    /* the format io_utils.c:892-947 reads; vp, vs, rho: nx*nz, z fastest */
+    FILE *f = fopen(fn, "w");
+    if (!f) return -1;
+    const double *vals[3] = {vp, vs, rho};
+    int rc = fprintf(f, "%zu %zu\n%.17g %.17g\n%.17g %.17g\nisotropic_elastic\n", nx, nz, dx, dz,
+                     x0, z0) < 0 ? -1 : 0;
+    if (rc == 0) rc = write_model_body(f, nx * nz, 3, vals, digits);
+    if (fclose(f) != 0) rc = -1;
+    return rc;
+}
+
 void sm_read_model_e(const char *fn, const sm_grid *g, double *vp, double *vs, double *rho)
 {
-    /* io_utils.c:892-947 */
+    /* io_utils.c:892-947.  The reference reads nx columns of nz "Vp Vs rho"
+     * triples with fscanf; when nothing can be read where the second column
+     * starts, the file holds one column, which is copied to every x.  Any
+     * other shortfall is an error. */
     FILE *f = open_or_die(fn);
     char type[64];
     if (fscanf(f, "%*d %*d") < 0 || fscanf(f, "%*f %*f") < 0 || fscanf(f, "%*f %*f") < 0 ||
         fscanf(f, "%63s", type) != 1 || strcmp(type, "isotropic_elastic") != 0)
         die("Error, model in %s not isotropic_elastic\n", fn);
-    int flag1d = 0;
-    for (size_t i = 0; i < g->nx; ++i) {
-        size_t ind = i * g->nz;
-        int nread = fscanf(f, "%lf %lf %lf", &vp[ind], &vs[ind], &rho[ind]);
-        if (nread <= 0 && i == 1) { flag1d = 1; break; }   /* one column given */
-        for (size_t j = 1; j < g->nz; ++j) {
-            ind = i * g->nz + j;
-            if (fscanf(f, "%lf %lf %lf", &vp[ind], &vs[ind], &rho[ind]) != 3)
-                die("Error: something wrong with file %s\n", fn);
-        }
-    }
+    const size_t ncell = g->nx * g->nz;
+    size_t nv = 0;
+    double *v = read_body(f, fn, 3 * ncell, &nv);
     fclose(f);
+    int flag1d = 0;
+    if (nv < 3 * ncell) {
+        if (g->nx > 1 && nv == 3 * g->nz) flag1d = 1;
+        else die("Error: something wrong with file %s\n", fn);
+    }
+    const size_t ncopy = flag1d ? g->nz : ncell;
+    for (size_t c = 0; c < ncopy; ++c) {
+        vp[c] = v[3 * c];
+        vs[c] = v[3 * c + 1];
+        rho[c] = v[3 * c + 2];
+    }
+    free(v);
     if (flag1d)
         for (size_t i = 1; i < g->nx; ++i)
             for (size_t j = 0; j < g->nz; ++j) {
@@ -190,27 +417,43 @@ void sm_read_model_a(const char *fn, const sm_grid *g, size_t np, double *vp, do
         die("Error, model in %s not isotropic_acoustic_3d\n", fn);
     const size_t ny2 = g->ny + 2 * np, nz2 = g->nz + 2 * np, nx2 = g->nx + 2 * np;
 #define AT(i, j, k) ((((i)) * ny2 + (j)) * nz2 + (k))
-    int flag1d = 0;
-    for (size_t i = 0; i < g->nx && !flag1d; ++i)
-        for (size_t j = 0; j < g->ny; ++j) {
-            size_t ind = AT(i + np, j + np, np);
-            int nread = fscanf(f, "%lf %lf", &vp[ind], &rho[ind]);
-            if (nread <= 0 && j == 1) { flag1d = 1; break; }   /* one column given */
-            for (size_t k = 1; k < g->nz; ++k) {
-                ind = AT(i + np, j + np, k + np);
-                if (fscanf(f, "%lf %lf", &vp[ind], &rho[ind]) != 2)
-                    die("Error: something wrong with file %s\n", fn);
-            }
-        }
+    /* "Vp rho" pairs, x and y outer, z inner.  As io_utils.c:1049-1072
+     * reads them, the model is one z column (copied everywhere) when nothing
+     * can be read where column (i, 1) of any x plane i starts, that is when
+     * the numbers end right after column (i, 0); any other shortfall is an
+     * error. */
+    const size_t ncell = g->nx * g->ny * g->nz;
+    size_t nv = 0;
+    double *v = read_body(f, fn, 2 * ncell, &nv);
     fclose(f);
+    int flag1d = 0;
+    if (nv < 2 * ncell) {
+        if (g->ny > 1 && nv % (2 * g->nz * g->ny) == 2 * g->nz) flag1d = 1;
+        else die("Error: something wrong with file %s\n", fn);
+    }
+    const size_t ncol = flag1d ? 1 : g->nx * g->ny;
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < ncol; ++c) {
+        const size_t i = c / g->ny, j = c % g->ny;
+        for (size_t k = 0; k < g->nz; ++k) {
+            const size_t q = c * g->nz + k;
+            vp[AT(i + np, j + np, k + np)] = v[2 * q];
+            rho[AT(i + np, j + np, k + np)] = v[2 * q + 1];
+        }
+    }
+    free(v);
     if (flag1d)
+#pragma omp parallel for schedule(static)
         for (size_t i = 0; i < g->nx; ++i)
-            for (size_t j = 0; j < g->ny; ++j)
+            for (size_t j = i == 0 ? 1 : 0; j < g->ny; ++j)
                 for (size_t k = 0; k < g->nz; ++k) {
                     vp[AT(i + np, j + np, k + np)] = vp[AT(np, np, k + np)];
                     rho[AT(i + np, j + np, k + np)] = rho[AT(np, np, k + np)];
                 }
-    /* edge values into the absorbing layers: z, then y, then x */
+    /* edge values into the absorbing layers: z, then y, then x (each pass
+     * reads only what the earlier passes wrote; its outer iterations write
+     * disjoint cells) */
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < g->nx; ++i)
         for (size_t j = 0; j < g->ny; ++j)
             for (size_t k = 0; k < np; ++k) {
@@ -219,6 +462,7 @@ void sm_read_model_a(const char *fn, const sm_grid *g, size_t np, double *vp, do
                 vp[AT(i + np, j + np, np + g->nz + k)] = vp[AT(i + np, j + np, np + g->nz - 1)];
                 rho[AT(i + np, j + np, np + g->nz + k)] = rho[AT(i + np, j + np, np + g->nz - 1)];
             }
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < g->nx; ++i)
         for (size_t k = 0; k < nz2; ++k)
             for (size_t j = 0; j < np; ++j) {
@@ -227,6 +471,7 @@ void sm_read_model_a(const char *fn, const sm_grid *g, size_t np, double *vp, do
                 vp[AT(i + np, np + g->ny + j, k)] = vp[AT(i + np, np + g->ny - 1, k)];
                 rho[AT(i + np, np + g->ny + j, k)] = rho[AT(i + np, np + g->ny - 1, k)];
             }
+#pragma omp parallel for schedule(static)
     for (size_t j = 0; j < ny2; ++j)
         for (size_t k = 0; k < nz2; ++k)
             for (size_t i = 0; i < np; ++i) {

@@ -79,6 +79,13 @@ void sm_read_grid3d(const char *fn, sm_grid *g);
 void sm_read_model_e(const char *fn, const sm_grid *g, double *vp, double *vs, double *rho);
 /* read_model_a (io_utils.c:1029-1153): padded arrays, edges replicated */
 void sm_read_model_a(const char *fn, const sm_grid *g, size_t np, double *vp, double *rho);
+/* model files in the format the readers take, values with `digits`
+ * significant digits (17: read back bit for bit); 0 on success */
+int sm_write_model_a(const char *fn, size_t nx, size_t ny, size_t nz, double dx, double dy,
+                     double dz, double x0, double y0, double z0, const double *vp,
+                     const double *rho, int digits);
+int sm_write_model_e(const char *fn, size_t nx, size_t nz, double dx, double dz, double x0,
+                     double z0, const double *vp, const double *vs, const double *rho, int digits);
 /* read_source / read_source_3d (io_utils.c:1432-1589) */
 void sm_read_source2d(const char *fn, const sm_grid *g, sm_sources *src);
 void sm_read_source3d(const char *fn, const sm_grid *g, size_t np, sm_sources *src);

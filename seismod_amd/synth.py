@@ -193,3 +193,66 @@ def e2d_batch_shots(count: int = 64, seed: int = 4, npml: int = 20, h: float = 5
                              rec_comp=np.full(len(rec), VZ, dtype=np.intc),
                              meta=dict(seed=seed)))
     return shots
+
+
+# ---------------------------------------------------------------------------
+# Model files: the seeded models above written in the text format the
+# drop-in programs (and the reference) read, by the parallel writer of the
+# host I/O library (values with 17 significant digits read back bit for bit).
+
+def _hostio():
+    import ctypes as C
+    from pathlib import Path
+    lib = C.CDLL(str(Path(__file__).resolve().parent / "lib" / "libsm_hostio.so"))
+    dbl = C.POINTER(C.c_double)
+    lib.sm_write_model_a.argtypes = [C.c_char_p, C.c_size_t, C.c_size_t, C.c_size_t] + \
+        [C.c_double] * 6 + [dbl, dbl, C.c_int]
+    lib.sm_write_model_e.argtypes = [C.c_char_p, C.c_size_t, C.c_size_t] + [C.c_double] * 4 + \
+        [dbl, dbl, dbl, C.c_int]
+    return lib, dbl
+
+
+def write_model_a(path, shot: A3dShot, digits: int = 17) -> None:
+    """The interior of a 3D shot's model as an isotropic_acoustic_3d file
+    (io_utils.c:1029-1072 format), origin 0."""
+    lib, dbl = _hostio()
+    p = shot.np
+    vp = np.ascontiguousarray(shot.vp[p:-p, p:-p, p:-p] if p else shot.vp)
+    rho = np.ascontiguousarray(shot.rho[p:-p, p:-p, p:-p] if p else shot.rho)
+    if lib.sm_write_model_a(str(path).encode(), shot.nx, shot.ny, shot.nz, shot.dx, shot.dx, shot.dx,
+                            0.0, 0.0, 0.0, vp.ctypes.data_as(dbl), rho.ctypes.data_as(dbl),
+                            digits) != 0:
+        raise OSError(f"cannot write {path}")
+
+
+def write_model_e(path, shot: E2dShot, digits: int = 17) -> None:
+    """A 2D shot's model as an isotropic_elastic file (io_utils.c:892-947
+    format), origin 0."""
+    lib, dbl = _hostio()
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (shot.vp, shot.vs, shot.rho)]
+    if lib.sm_write_model_e(str(path).encode(), shot.nx, shot.nz, shot.h, shot.h, 0.0, 0.0,
+                            *(x.ctypes.data_as(dbl) for x in a), digits) != 0:
+        raise OSError(f"cannot write {path}")
+
+
+def main(argv=None) -> None:
+    """python -m seismod_amd.synth a3d --n 512 --seed 3 --out model.dat
+       python -m seismod_amd.synth e2d --n 400 --out model2d.dat"""
+    import argparse
+    ap = argparse.ArgumentParser(description="write a seeded model file")
+    ap.add_argument("kind", choices=["a3d", "e2d"])
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--vmax", type=float, default=4500.0)
+    ap.add_argument("--digits", type=int, default=17)
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args(argv)
+    if args.kind == "a3d":
+        write_model_a(args.out, a3d_layered_shot(n=args.n, seed=args.seed, vmax=args.vmax),
+                      args.digits)
+    else:
+        write_model_e(args.out, e2d_two_layer_shot(n=args.n), args.digits)
+
+
+if __name__ == "__main__":
+    main()

@@ -152,6 +152,108 @@ def test_read_model_e_full_and_column(tmp_path):
             assert np.array_equal(a[q], expect[:, :, q])
 
 
+def _read3d(fn, np_):
+    g = Grid()
+    LIB.sm_read_grid3d(str(fn).encode(), C.byref(g))
+    n2 = [n + 2 * np_ for n in (g.nx, g.ny, g.nz)]
+    vp, rho = np.zeros(n2), np.zeros(n2)
+    LIB.sm_read_model_a(str(fn).encode(), C.byref(g), C.c_size_t(np_),
+                        vp.ctypes.data_as(C.c_void_p), rho.ctypes.data_as(C.c_void_p))
+    return vp, rho
+
+
+def test_model_writer_round_trip_parallel_read(tmp_path):
+    """sm_write_model_a with 17 digits, read back by the parallel reader
+    (a 64x48x40 body of about 6 MB is split across threads): every value
+    comes back bit for bit, padded as before."""
+    rng = np.random.default_rng(5)
+    nx, ny, nz, np_ = 64, 48, 40, 3
+    vp = rng.uniform(1500.0, 4500.0, (nx, ny, nz)) * 10.0 ** rng.integers(-3, 4, (nx, ny, nz))
+    rho = -rng.standard_normal((nx, ny, nz)) * 1e-300          # tiny and subnormal, signed
+    fn = tmp_path / "w3.dat"
+    dbl = C.POINTER(C.c_double)
+    LIB.sm_write_model_a.argtypes = [C.c_char_p, C.c_size_t, C.c_size_t, C.c_size_t] + \
+        [C.c_double] * 6 + [dbl, dbl, C.c_int]
+    assert LIB.sm_write_model_a(str(fn).encode(), nx, ny, nz, 10.0, 10.0, 10.0, 0.0, 0.0, 0.0,
+                                vp.ctypes.data_as(dbl), rho.ctypes.data_as(dbl), 17) == 0
+    out_vp, out_rho = _read3d(fn, np_)
+    assert np.array_equal(out_vp, np.pad(vp, np_, mode="edge"))
+    assert np.array_equal(out_rho, np.pad(rho, np_, mode="edge"))
+    # the 2D writer and reader
+    vs = vp[:, 0, :] / 1.8
+    fn2 = tmp_path / "w2.dat"
+    LIB.sm_write_model_e.argtypes = [C.c_char_p, C.c_size_t, C.c_size_t] + [C.c_double] * 4 + \
+        [dbl, dbl, dbl, C.c_int]
+    a = [np.ascontiguousarray(x) for x in (vp[:, 0, :], vs, rho[:, 0, :])]
+    assert LIB.sm_write_model_e(str(fn2).encode(), nx, nz, 5.0, 5.0, 0.0, 0.0,
+                                *(x.ctypes.data_as(dbl) for x in a), 17) == 0
+    g = Grid()
+    LIB.sm_read_grid2d(str(fn2).encode(), C.byref(g))
+    b = [np.zeros((nx, nz)) for _ in range(3)]
+    LIB.sm_read_model_e(str(fn2).encode(), C.byref(g), *(x.ctypes.data_as(C.c_void_p) for x in b))
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_model_reader_number_forms_match_correct_rounding(tmp_path):
+    """Any whitespace between numbers, exponents, signs, leading dots, long
+    mantissas, no final newline: each value equals Python's correctly
+    rounded float() of the same text (as fscanf's %lf gives)."""
+    rng = np.random.default_rng(6)
+    nx, ny, nz = 3, 4, 5
+    forms = ["{:.3f}", "{:.6e}", "{:+.2E}", "{:.25g}", "{:.0f}", "{:.17g}"]
+    toks = []
+    for q in range(2 * nx * ny * nz):
+        v = float(rng.uniform(0.001, 9000.0))
+        t = forms[q % len(forms)].format(v)
+        if q % 7 == 3 and t.startswith("0."):
+            t = t[1:]                                   # ".5" form
+        toks.append(t)
+    seps = [" ", "\t", "\r\n", "\n\n ", "   "]
+    body = "".join(t + seps[q % len(seps)] for q, t in enumerate(toks)).rstrip()
+    fn = tmp_path / "forms.dat"
+    fn.write_text(f"{nx} {ny} {nz}\n10 10 10\n0 0 0\nisotropic_acoustic_3d\n" + body)
+    vp, rho = _read3d(fn, 0)
+    expect = np.array([float(t) for t in toks]).reshape(nx, ny, nz, 2)
+    assert np.array_equal(vp, expect[..., 0]) and np.array_equal(rho, expect[..., 1])
+
+
+def _read3d_subprocess(fn, np_):
+    code = (f"import ctypes as C, numpy as np, sys; sys.path.insert(0, {str(ROOT / 'tests')!r});"
+            f"import test_host_io as T; vp, rho = T._read3d({str(fn)!r}, {np_});"
+            "np.save(sys.argv[1], np.stack([vp, rho]))")
+    out = Path(str(fn) + ".npy")
+    r = subprocess.run(["python3", "-c", code, str(out)], capture_output=True, text=True)
+    return r.returncode, r.stderr, (np.load(out) if r.returncode == 0 else None)
+
+
+@pytest.mark.parametrize("cut", ["plane0", "plane1", "short", "bad_token"])
+def test_model_reader_column_and_error_cases(tmp_path, cut):
+    """io_utils.c:1049-1072: the model is one z column (copied everywhere)
+    when the numbers end right after column (i, 0) of an x plane; a body
+    ending anywhere else, or with a token that is not a number, stops the
+    program with the reference's message."""
+    rng = np.random.default_rng(7)
+    nx, ny, nz = 3, 2, 4
+    vals = rng.uniform(1000.0, 3000.0, (nx, ny, nz, 2))
+    toks = [repr(float(v)) for v in vals.reshape(-1)]
+    keep = {"plane0": 2 * nz, "plane1": 2 * nz * (ny + 1), "short": 2 * nz + 3,
+            "bad_token": len(toks)}[cut]
+    body = toks[:keep]
+    if cut == "bad_token":
+        body[2 * nz * ny + 5] = "12x"
+    fn = tmp_path / f"{cut}.dat"
+    fn.write_text(f"{nx} {ny} {nz}\n10 10 10\n0 0 0\nisotropic_acoustic_3d\n" + "\n".join(body) + "\n")
+    rc, err, arr = _read3d_subprocess(fn, 1)
+    if cut in ("plane0", "plane1"):
+        assert rc == 0, err
+        col = np.broadcast_to(vals[0:1, 0:1], vals.shape)
+        assert np.array_equal(arr[0], np.pad(col[..., 0], 1, mode="edge"))
+        assert np.array_equal(arr[1], np.pad(col[..., 1], 1, mode="edge"))
+    else:
+        assert rc == 1 and "Error: something wrong with file" in err
+
+
 def test_read_sources(tmp_path):
     g = Grid(nx=400, ny=1, nz=400, dx=5.0, dy=5.0, dz=5.0)
     fn = tmp_path / "src.dat"
@@ -279,3 +381,17 @@ def test_cli_errors(prog, tmp_path):
     r = run_bin(prog, "-p", str(tmp_path / "missing.par"))
     assert r.returncode == 1 and "cannot open parameter file" in r.stderr
     assert run_bin(prog, "-c", "chk.dat").returncode == 1
+
+
+def test_seeded_model_file_reads_back_the_shot(tmp_path):
+    """synth.write_model_a writes a seeded shot's interior model; the
+    drop-in reader returns exactly the padded arrays the shot holds (edge
+    replication into the CPML is the reader's, as io_utils.c:1089-1143)."""
+    import sys
+    sys.path.insert(0, str(ROOT))
+    from seismod_amd import synth
+    shot = synth.a3d_layered_shot(n=24, np_=6, seed=3, vmax=5000.0)
+    fn = tmp_path / "seeded.dat"
+    synth.write_model_a(fn, shot)
+    vp, rho = _read3d(fn, shot.np)
+    assert np.array_equal(vp, shot.vp) and np.array_equal(rho, shot.rho)
