@@ -310,6 +310,10 @@ static double *read_body(FILE *f, const char *fn, size_t want, size_t *nvalid)
  */
 static int write_model_body(FILE *f, size_t ncell, int nval, const double *const *vals, int digits)
 {
+    /* 1..17 significant digits: a value then takes at most 24 characters
+     * plus its separator, inside the 32 each one is given */
+    if (digits < 1) digits = 1;
+    if (digits > 17) digits = 17;
     const size_t block = (size_t)1 << 20;
     const size_t line_max = (size_t)nval * 32 + 2;
     int nt = 1;

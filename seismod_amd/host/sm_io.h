@@ -80,7 +80,8 @@ void sm_read_model_e(const char *fn, const sm_grid *g, double *vp, double *vs, d
 /* read_model_a (io_utils.c:1029-1153): padded arrays, edges replicated */
 void sm_read_model_a(const char *fn, const sm_grid *g, size_t np, double *vp, double *rho);
 /* model files in the format the readers take, values with `digits`
- * significant digits (17: read back bit for bit); 0 on success */
+ * significant digits (clamped to 1..17; 17: read back bit for bit); 0 on
+ * success */
 int sm_write_model_a(const char *fn, size_t nx, size_t ny, size_t nz, double dx, double dy,
                      double dz, double x0, double y0, double z0, const double *vp,
                      const double *rho, int digits);
