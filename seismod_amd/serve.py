@@ -12,11 +12,13 @@ solve with its own model, source, receivers and step count.  The server:
     shots are queued on their own streams first (asynchronous), then the
     2D shots advance together in one batched launch per time step
     (sm_e2d_batch_run), so the device works on several shots at once.
-    Which shots a tick advances is the policy: "srpt" (default) takes the
-    shots with the least remaining work (cells x steps left), of each kind,
-    until about one device's worth of cells is in flight, so short requests
-    finish first without leaving the device idle; "share" advances every
-    admitted shot (processor sharing);
+    Which shots a tick advances is the policy: "srpt-all" (default) takes
+    the admitted shots with the least remaining work (cells x steps left),
+    both kinds ranked together, until about one device's worth of cells is
+    in flight, so short requests finish first without leaving the device
+    idle and a long shot does not share the device with shorter ones;
+    "srpt" does the same per kind (each kind fills its own budget);
+    "share" advances every admitted shot (processor sharing);
   - between ticks, retires finished shots (traces copied back, device
     memory freed) and admits new arrivals, so requests join and leave
     while others are running.
@@ -27,7 +29,8 @@ the same bits alone, batched, or in any launch shape.
 `run_sequential` is the comparison point: one shot at a time in arrival
 order, each with plain per-step launches.
 
-CLI:  python -m seismod_amd.serve --rate 20 --count 24 [--seed 5] [--policy srpt|share]
+CLI:  python -m seismod_amd.serve --rate 20 --count 24 [--seed 5]
+          [--policy srpt|srpt-all|share]
 prints one JSON line with p50/p99 latency and aggregate Gcell-updates/s for
 the server and for the sequential baseline on the same arrival trace.
 """
@@ -45,6 +48,7 @@ from .solver import (A3dConfig, Acoustic3D, Context, E2dBatch, E2dConfig, Elasti
                      source_functions)
 
 VZ = 2
+POLICIES = ("srpt", "srpt-all", "share")
 
 
 @dataclass
@@ -148,9 +152,9 @@ class Server:
     """Continuous-batching server on one device (see the module docstring)."""
 
     def __init__(self, device: int = 0, tick_steps: int = 64, reserve_bytes: int = 2 << 30,
-                 max_active: int = 256, graph_steps: int = 32, policy: str = "srpt",
+                 max_active: int = 256, graph_steps: int = 32, policy: str = "srpt-all",
                  fill_3d: int = 16 << 20, fill_2d: int = 8 << 20):
-        if policy not in ("srpt", "share"):
+        if policy not in POLICIES:
             raise ValueError(f"unknown policy {policy!r}")
         self.policy = policy
         self.fill = {"a3d": fill_3d, "e2d": fill_2d}   # cells in flight per tick and kind
@@ -232,6 +236,15 @@ class Server:
             return list(active)
         left = lambda a: a["req"].cells * (a["req"].nsteps - a["done"])
         out = []
+        if self.policy == "srpt-all":
+            cells = 0
+            budget = max(self.fill.values())
+            for a in sorted(active, key=lambda a: (left(a), a["req"].rid)):
+                if cells >= budget:
+                    break
+                out.append(a)
+                cells += a["req"].cells
+            return out
         for kind in ("a3d", "e2d"):
             cells = 0
             for a in sorted((a for a in active if a["req"].kind == kind),
@@ -289,7 +302,7 @@ def main() -> None:
     ap.add_argument("--frac-3d", type=float, default=0.4)
     ap.add_argument("--scale", type=float, default=1.0,
                     help="multiply grid edge ranges (smaller runs for quick checks)")
-    ap.add_argument("--policy", choices=("srpt", "share"), default="srpt",
+    ap.add_argument("--policy", choices=POLICIES, default="srpt-all",
                     help="which admitted shots a tick advances")
     ap.add_argument("--no-baseline", action="store_true")
     args = ap.parse_args()

@@ -8,7 +8,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("policy", ["srpt", "share"])
+@pytest.mark.parametrize("policy", ["srpt", "srpt-all", "share"])
 def test_results_alone_equal_results_in_mix(policy):
     from seismod_amd import serve
     reqs = serve.make_requests(10, rate=400.0, seed=7, frac_3d=0.4, e2d_range=(40, 120),

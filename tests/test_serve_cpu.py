@@ -63,6 +63,23 @@ def test_srpt_selects_least_remaining_work_per_kind():
     assert [a["req"].rid for a in s._select(act)] == [0, 1, 2, 3, 4, 5]
 
 
+def test_srpt_all_ranks_both_kinds_against_one_budget():
+    act = [dict(req=_Req(0, "a3d", 100, 50), done=0),     # left 5000
+           dict(req=_Req(1, "a3d", 100, 50), done=40),    # left 1000
+           dict(req=_Req(2, "a3d", 300, 10), done=0),     # left 3000
+           dict(req=_Req(3, "e2d", 10, 100), done=0),     # left 1000
+           dict(req=_Req(4, "e2d", 10, 20), done=0),      # left 200
+           dict(req=_Req(5, "e2d", 10, 30), done=0)]      # left 300
+    # budget = the larger fill: 4 (10 cells), 5 (20), 1 (left 1000, before
+    # 3 in rid order) reaches 120 >= 120; with 121, 3 follows
+    s = _server("srpt-all", fill_3d=120, fill_2d=15)
+    assert [a["req"].rid for a in s._select(act)] == [4, 5, 1]
+    s = _server("srpt-all", fill_3d=121, fill_2d=15)
+    assert [a["req"].rid for a in s._select(act)] == [4, 5, 1, 3]
+    s = _server("srpt-all", fill_3d=1, fill_2d=1)
+    assert [a["req"].rid for a in s._select(act)] == [4]
+
+
 def test_unknown_policy_is_rejected():
     import pytest
     with pytest.raises(ValueError):
