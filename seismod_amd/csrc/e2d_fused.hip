@@ -157,16 +157,15 @@ e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot,
     auto fr_base = [&](int fr) { return (fr < 5 ? par_off : 0) + sg_field(fr) * (int)N; };
     // issue the chunk loads of iteration I (rows I+1, I+2); the lane's
     // chunk of round r is c = (r & 1) * 64 + lane of the state (r < 2) or
-    // material (r >= 2) field-rows, 17 chunks each; go false: no memory
-    // access (the loads are still issued, so waits stay exact)
-    auto stage_issue = [&](int I, u32x4 (&v)[4], bool go) {
+    // material (r >= 2) field-rows, 17 chunks each
+    auto stage_issue = [&](int I, u32x4 (&v)[4]) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int c = (r & 1) * 64 + lane;
             const int fr = (r < 2 ? 0 : 5) + c / SG_CHUNKS;
             const int R = I + ((fr <= 3 || fr == 5) ? 2 : 1);
             const int a = (fr_base(fr) + R * NZ + J0m) & ~3;
-            const bool ok = go && c < 5 * SG_CHUNKS && rv(R);
+            const bool ok = c < 5 * SG_CHUNKS && rv(R);
             const unsigned off = ok ? (unsigned)(a + 4 * (c % SG_CHUNKS)) * 4u : OFF_NONE;
             v[r] = __builtin_amdgcn_raw_buffer_load_b128(r < 2 ? Rall : Rmat16, off, 0, 0);
         }
@@ -309,7 +308,7 @@ e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot,
         load_in(I0, true, A);
         u32x4 sv[4];
         if constexpr (STAGED)
-            if (I0 + 1 < I1) stage_issue(I0, sv, true);
+            if (I0 + 1 < I1) stage_issue(I0, sv);
         float vx0 = vx_new(I0, v_pre, sxx_m, C.sxx, C.sxz, true);
         float vz_m = vz_new(I0 - 1, z_pre, sxz_m, szz_m, C.sxz, false);
 
@@ -393,7 +392,7 @@ e2d_fused_step(const E2dShot *__restrict__ shots, const int *__restrict__ bshot,
                 RowIn nxt{0.f, 0.f, 0.f, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, TRow{}};
                 if (more) {
                     stage_write(sv);
-                    if (I + 2 < I1) stage_issue(I + 1, sv, true);
+                    if (I + 2 < I1) stage_issue(I + 1, sv);
                     nxt.sxx = stage_read(I, 0);
                     nxt.szz = stage_read(I, 1);
                     nxt.sxz = stage_read(I, 2);
