@@ -53,23 +53,31 @@ def parse():
 def cpu_baseline(shot, nsteps_sample: int) -> dict:
     """Oracle (FP64 C restatement, the reference's OpenMP shape: 3 threads for
     the velocity sections, one for pressure) on the same model; time per step
-    from the difference of two runs so set-up is excluded."""
+    from the difference of two runs so set-up is excluded.  Also timed with
+    every sweep split over x planes on all host threads (same values)."""
     from oracle import pyoracle as O
     cfg = dict(nx=shot.nx, ny=shot.ny, nz=shot.nz, np=shot.np, dx=shot.dx, dy=shot.dx,
                dz=shot.dx, dt=shot.dt, f0=shot.f0)
     L, fct = O.source_functions([1.0], [shot.f0], shot.dt)
-    times = []
-    for n in (2, 2 + nsteps_sample):
-        t0 = time.perf_counter()
-        O.a3d_run(cfg, shot.vp, shot.rho, shot.src_idx, L, fct, shot.rec_idx[:1],
-                  shot.rec_comp[:1], n, want_fields=False, nthreads=3, pressure_threads=1)
-        times.append(time.perf_counter() - t0)
-    per_step = max(times[1] - times[0], 1e-9) / nsteps_sample
-    return {"value": shot.cells / per_step / 1e9, "unit": "Gcell-updates/s", "cores": 3,
+
+    def rate(**threads):
+        times = []
+        for n in (2, 2 + nsteps_sample):
+            t0 = time.perf_counter()
+            O.a3d_run(cfg, shot.vp, shot.rho, shot.src_idx, L, fct, shot.rec_idx[:1],
+                      shot.rec_comp[:1], n, want_fields=False, **threads)
+            times.append(time.perf_counter() - t0)
+        per_step = max(times[1] - times[0], 1e-9) / nsteps_sample
+        return shot.cells / per_step / 1e9
+
+    host = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    return {"value": rate(nthreads=3, pressure_threads=1), "unit": "Gcell-updates/s", "cores": 3,
             "kind": "port",
             "sample": f"{shot.shape[0]}^3 padded model, {nsteps_sample} time steps (difference "
                       f"of a {2 + nsteps_sample}-step and a 2-step run), FP64, OpenMP sections "
-                      f"as a_iso_3d.c:585 (3 threads velocity, 1 pressure)"}
+                      f"as a_iso_3d.c:585 (3 threads velocity, 1 pressure)",
+            "all_cores": {"value": rate(split_threads=host), "cores": host,
+                          "sample": "the same run with every sweep split over x planes"}}
 
 
 # kernel launched by each variant of the fused 3D path

@@ -69,6 +69,8 @@ typedef struct {
     double f0;
     int nthreads;          /* OpenMP threads for the velocity sections */
     int pressure_threads;  /* 1 = single section as in a_iso_3d.c:655-684 */
+    int split_threads;     /* > 0: instead, every sweep split over x planes on
+                              this many threads (same values, all cores) */
 } o_a3d_params;
 
 /* CPML tables of a_iso_3d.c:500-576 for one axis of n2 nodes.  node

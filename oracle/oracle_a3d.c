@@ -64,17 +64,19 @@ static inline size_t at(const a3d_state *s, size_t i, size_t j, size_t k)
     return (i * s->ny2 + j) * s->nz2 + k;
 }
 
-/* one velocity component sweep, a_iso_3d.c:605-651 (axis 0: x, 1: y, 2: z) */
-static void sweep_velocity(a3d_state *s, int axis)
+/* one velocity component sweep over x planes [i_begin, i_end),
+ * a_iso_3d.c:605-651 (axis 0: x, 1: y, 2: z) */
+static void sweep_velocity_rows(a3d_state *s, int axis, size_t i_begin, size_t i_end)
 {
     const size_t ni = s->nx2 - (axis == 0), nj = s->ny2 - (axis == 1),
                  nk = s->nz2 - (axis == 2);
+    if (i_end > ni) i_end = ni;
     const double h = axis == 0 ? s->dx : (axis == 1 ? s->dy : s->dz);
     const size_t step = axis == 0 ? s->ny2 * s->nz2 : (axis == 1 ? s->nz2 : 1);
     double *v = s->v[axis], *mem = s->mP[axis];
     const double *ir = s->irho[axis];
     const double *a = s->tah[axis], *b = s->tbh[axis], *kk = s->tkh[axis];
-    for (size_t i = 0; i < ni; ++i)
+    for (size_t i = i_begin; i < i_end; ++i)
         for (size_t j = 0; j < nj; ++j)
             for (size_t k = 0; k < nk; ++k) {
                 size_t c = at(s, i, j, k);
@@ -85,6 +87,8 @@ static void sweep_velocity(a3d_state *s, int axis)
                 v[c] -= ir[c] * dP;
             }
 }
+
+static void sweep_velocity(a3d_state *s, int axis) { sweep_velocity_rows(s, axis, 0, s->nx2); }
 
 /* pressure sweep, a_iso_3d.c:659-683 */
 static void sweep_pressure_rows(a3d_state *s, size_t i_begin, size_t i_end)
@@ -189,25 +193,37 @@ int o_a3d_run(const o_a3d_params *p, const double *vp_in, const double *rho,
 
     const int nth = p->nthreads > 0 ? p->nthreads : 1;
     const int pth = p->pressure_threads > 0 ? p->pressure_threads : 1;
-    (void)nth; (void)pth;
+    const int sth = p->split_threads;
+    (void)nth; (void)pth; (void)sth;
     for (size_t it = 0; it < nsteps; ++it) {
-        /* velocity phase: three independent sections (a_iso_3d.c:599-652) */
-#pragma omp parallel sections num_threads(nth)
-        {
-#pragma omp section
-            sweep_velocity(&s, 0);
-#pragma omp section
-            sweep_velocity(&s, 1);
-#pragma omp section
-            sweep_velocity(&s, 2);
-        }
-        /* pressure phase: one section in the reference (a_iso_3d.c:655-684);
-         * pressure_threads > 1 splits it over x rows (same arithmetic). */
-        if (pth <= 1) {
-            sweep_pressure_rows(&s, 1, s.nx2);
-        } else {
-#pragma omp parallel for num_threads(pth) schedule(static)
+        if (sth > 0) {
+            /* every sweep split over x planes: each cell's arithmetic is the
+             * same as in the sections below, so the values are identical */
+            for (int axis = 0; axis < 3; ++axis) {
+#pragma omp parallel for num_threads(sth) schedule(static)
+                for (size_t i = 0; i < s.nx2; ++i) sweep_velocity_rows(&s, axis, i, i + 1);
+            }
+#pragma omp parallel for num_threads(sth) schedule(static)
             for (size_t i = 1; i < s.nx2; ++i) sweep_pressure_rows(&s, i, i + 1);
+        } else {
+            /* velocity phase: three independent sections (a_iso_3d.c:599-652) */
+#pragma omp parallel sections num_threads(nth)
+            {
+#pragma omp section
+                sweep_velocity(&s, 0);
+#pragma omp section
+                sweep_velocity(&s, 1);
+#pragma omp section
+                sweep_velocity(&s, 2);
+            }
+            /* pressure phase: one section in the reference (a_iso_3d.c:655-684);
+             * pressure_threads > 1 splits it over x rows (same arithmetic). */
+            if (pth <= 1) {
+                sweep_pressure_rows(&s, 1, s.nx2);
+            } else {
+#pragma omp parallel for num_threads(pth) schedule(static)
+                for (size_t i = 1; i < s.nx2; ++i) sweep_pressure_rows(&s, i, i + 1);
+            }
         }
         /* source, a_iso_3d.c:689-693 */
         size_t off = 0;

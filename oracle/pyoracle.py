@@ -25,7 +25,7 @@ class A3dParams(C.Structure):
         ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
         ("vmax", C.c_double), ("pml_order", C.c_double), ("Rc", C.c_double),
         ("kappa_max", C.c_double), ("alpha_max", C.c_int), ("f0", C.c_double),
-        ("nthreads", C.c_int), ("pressure_threads", C.c_int),
+        ("nthreads", C.c_int), ("pressure_threads", C.c_int), ("split_threads", C.c_int),
     ]
 
 
@@ -120,12 +120,13 @@ def e2d_material(vp, vs, rho):
 
 
 def a3d_run(cfg: dict, vp, rho, src_idx, src_len, src_fct, rec_idx, rec_comp, nsteps,
-            want_fields=True, nthreads=1, pressure_threads=1):
-    """Run the a_iso_3d restatement.  cfg keys as A3dParams fields."""
+            want_fields=True, nthreads=1, pressure_threads=1, split_threads=0):
+    """Run the a_iso_3d restatement.  cfg keys as A3dParams fields;
+    split_threads > 0 splits every sweep over x planes (same values)."""
     p = A3dParams(cfg["nx"], cfg["ny"], cfg["nz"], cfg["np"], cfg["dx"], cfg["dy"], cfg["dz"],
                   cfg["dt"], cfg.get("vmax", 5000.0), cfg.get("pml_order", 2.0),
                   cfg.get("Rc", 1e-5), cfg.get("kappa_max", 1.0), cfg.get("alpha_max", 1),
-                  cfg["f0"], nthreads, pressure_threads)
+                  cfg["f0"], nthreads, pressure_threads, split_threads)
     vp = np.ascontiguousarray(vp, dtype=np.float64).reshape(-1)
     rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
     si = np.ascontiguousarray(src_idx, dtype=np.uintp)

@@ -159,6 +159,23 @@ def test_a3d_oracle_matches_numpy_formulation(case):
     assert rel_l2(tr, tr_n) < 1e-12
 
 
+def test_a3d_thread_shapes_give_identical_values():
+    """The reference's OpenMP shape (three velocity sections, one pressure
+    thread), pressure split over x rows, and every sweep split over x planes
+    on 8 threads (the all-cores CPU baseline) compute each cell the same
+    way: traces and fields are bit-identical."""
+    d = decks.a3d_deck(nx=12, ny=7, nz=9, np_=3, seed=8, nsteps=25)
+    c = d["cfg"]
+    L, fct = O.source_functions(d["A"], d["f"], c["dt"])
+    run = lambda **kw: O.a3d_run(c, d["vp"], d["rho"], d["src_idx"], L, fct, d["rec_idx"],
+                                 d["rec_comp"], d["nsteps"], **kw)
+    tr0, f0 = run(nthreads=3, pressure_threads=1)
+    assert np.abs(f0).max() > 0
+    for kw in (dict(nthreads=1, pressure_threads=4), dict(split_threads=8), dict(split_threads=1)):
+        tr, f = run(**kw)
+        assert np.array_equal(tr, tr0) and np.array_equal(f, f0), kw
+
+
 def test_a3d_zero_source_stays_zero():
     d = decks.a3d_deck(nx=8, ny=7, nz=6, np_=3, seed=5, nsteps=30)
     c = d["cfg"]
