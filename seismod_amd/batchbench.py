@@ -23,7 +23,7 @@ from pathlib import Path
 sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
 
 
-def main() -> None:
+def main(argv=None) -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--seed", type=int, default=4)
@@ -31,7 +31,7 @@ def main() -> None:
                     help="multiply every shot's step count (shorter runs for profiling)")
     ap.add_argument("--graph-steps", type=int, default=32)
     ap.add_argument("--no-sequential", action="store_true")
-    args = ap.parse_args()
+    args = ap.parse_args(argv)
 
     from seismod_amd import autotune, synth
     from seismod_amd.solver import Context, E2dBatch, E2dConfig, Elastic2D, source_functions

@@ -76,7 +76,7 @@ def run_mode(ctx, shot, mode: str, mid: int, tune: bool):
     return traces, field, info
 
 
-def main() -> None:
+def main(argv=None) -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--nsteps", type=int, default=2000)
@@ -86,7 +86,7 @@ def main() -> None:
     ap.add_argument("--storages", default="f32,bf16,f16,bf16-sr,f16-sr")
     ap.add_argument("--window", type=int, default=500)
     ap.add_argument("--no-tune", action="store_true")
-    args = ap.parse_args()
+    args = ap.parse_args(argv)
     if not 0 < args.mid < args.nsteps:
         sys.exit("--mid must lie strictly inside (0, nsteps)")
 

@@ -293,7 +293,7 @@ def summarize(results: list[Result]) -> dict:
             "gcell_updates_per_s": round(work / span / 1e9, 2)}
 
 
-def main() -> None:
+def main(argv=None) -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--rate", type=float, default=20.0, help="arrivals per second")
     ap.add_argument("--count", type=int, default=24)
@@ -305,7 +305,7 @@ def main() -> None:
     ap.add_argument("--policy", choices=POLICIES, default="srpt-all",
                     help="which admitted shots a tick advances")
     ap.add_argument("--no-baseline", action="store_true")
-    args = ap.parse_args()
+    args = ap.parse_args(argv)
     sc = args.scale
     reqs = make_requests(args.count, args.rate, seed=args.seed, frac_3d=args.frac_3d,
                          e2d_range=(int(256 * sc), int(2048 * sc)),
