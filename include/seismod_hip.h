@@ -245,8 +245,9 @@ typedef struct {
 int sm_e2d_create(sm_ctx *ctx, const sm_e2d_params *p, const double *vp,
                   const double *vs, const double *rho, sm_e2d **out);
 int sm_e2d_destroy(sm_e2d *s);
-/* add_src (src.c:170-184): type[n] (SM_SX/SM_SZ/SM_BULK/SM_SXZ; other
- * types inject nothing), interior index idx[n] = i*nz+j. */
+/* add_src (src.c:170-184): type[n] (SM_SX/SM_SZ/SM_BULK/SM_SXZ; the other
+ * types of enum sourceType, 0..12, inject nothing; values outside it are
+ * SM_ERR_ARG), interior index idx[n] = i*nz+j. */
 int sm_e2d_set_sources(sm_e2d *s, size_t nsrc, const int *type,
                        const size_t *idx, const size_t *length,
                        const double *fct);

@@ -568,6 +568,10 @@ int sm_e2d_set_sources(sm_e2d *s, size_t nsrc, const int *type, const size_t *id
     for (size_t k = 0; k < nsrc; ++k) {
         SM_CHECK(idx[k] < n, SM_ERR_ARG, "source %zu: index %zu outside the %zux%zu grid", k, idx[k],
                  s->p.nx, s->p.nz);
+        // enum sourceType (structs.h:44) has 13 values; add_src acts on
+        // four of them and ignores the others, values outside are errors
+        SM_CHECK(type[k] >= 0 && type[k] <= 12, SM_ERR_ARG, "source %zu: type %d is not a source type",
+                 k, type[k]);
         total += length[k];
     }
     SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
