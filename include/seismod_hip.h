@@ -265,6 +265,9 @@ int sm_e2d_get_field(sm_e2d *s, int comp, double *out);
 int sm_e2d_get_field_padded(sm_e2d *s, int comp, double *out);
 int sm_e2d_reset(sm_e2d *s);
 int sm_e2d_device_bytes(const sm_e2d *s, size_t *bytes);
+/* 0 when sm_e2d_create would refuse the grid for its size: the fused FP32
+ * solver needs its state and split fields, 96 bytes per padded cell,
+ * below 2 GiB (about 4730^2 padded cells) */
 size_t sm_e2d_bytes_needed(const sm_e2d_params *p, size_t nrec, size_t max_steps);
 
 /* Launch shape of the fused FP32 step (no counterpart in the reference):

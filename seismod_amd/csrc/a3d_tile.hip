@@ -49,6 +49,23 @@ namespace {
 constexpr unsigned OFF_NONE = 0xFFFFFF00u;   // past the end of every buffer
 constexpr int RSRC_WORD3 = 0x00020000;       // raw 32-bit buffer on gfx9
 
+// cache policy of the streaming accesses (buffer instruction aux bits:
+// 2 = nt, non-temporal) -- chunk loads of the state and material, and
+// stores of the state
+#ifndef SM_TILE_LD_CPOL
+#define SM_TILE_LD_CPOL 0
+#endif
+#ifndef SM_TILE_ST_CPOL
+#define SM_TILE_ST_CPOL 0
+#endif
+// 1: the four new state values of a wave's row are gathered in LDS and
+// written as 16-byte stores (one store instruction per wave and plane)
+#ifndef SM_TILE_PACK
+#define SM_TILE_PACK 0
+#endif
+constexpr int LD_CPOL = SM_TILE_LD_CPOL, ST_CPOL = SM_TILE_ST_CPOL;
+constexpr bool PACK = SM_TILE_PACK != 0;
+
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 using Rsrc = __amdgpu_buffer_rsrc_t;
 
@@ -73,6 +90,15 @@ __device__ __forceinline__ void bst(Rsrc r, unsigned voff, typename Store<S>::Bi
         __builtin_amdgcn_raw_buffer_store_b32(b, r, voff, 0, 0);
     else
         __builtin_amdgcn_raw_buffer_store_b16(b, r, voff, 0, 0);
+}
+// state store (streaming cache policy)
+template <typename S>
+__device__ __forceinline__ void bst_state(Rsrc r, unsigned voff, typename Store<S>::Bits b)
+{
+    if constexpr (sizeof(S) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(b, r, voff, 0, ST_CPOL);
+    else
+        __builtin_amdgcn_raw_buffer_store_b16(b, r, voff, 0, ST_CPOL);
 }
 
 // Tile fields of one plane slot.  The fields of the state buffer come
@@ -117,6 +143,11 @@ struct TileShape {
     static constexpr int NT = 64 * TJ;                                // threads per block
     static constexpr int NR = (total_chunks + NT - 1) / NT;          // chunks per thread
     static constexpr int slot_bytes = NR * NT * 16;                   // every thread's chunks
+    // packed stores: a wave's new P, vx, vy, vz (64 cells each) gathered in
+    // LDS, then written as 16-byte chunks, CPF chunks per field
+    static constexpr int CPF = 64 * ES / 16;
+    static constexpr int stash_bytes = 4 * 64 * ES;                  // per wave
+    static constexpr int lds_bytes = 3 * slot_bytes + (PACK ? TJ * stash_bytes : 0);
 };
 
 // Per-item constants of one lane (the cell (j, kk) it owns, clamped copies,
@@ -133,6 +164,10 @@ struct TLane {
     // columns (relative to k0) of the cell and k+1 (per lane), of k0-1
     int tc, tjp, tjm;
     int cc, ckp, ckm;
+    // packed stores: element offset (plane 0) of the lane's 16-byte chunk
+    // of the new state, and whether the lane writes one
+    unsigned pk_off;
+    bool pk_ok;
 };
 
 // CPML memory variables one lane reads for one plane (0 outside the slabs)
@@ -155,6 +190,22 @@ __device__ __forceinline__ unsigned tof(int t, int c)
     constexpr int O = TS::off(F), R0 = TS::row0(F), RB = TS::row_bytes(F), C0 = TS::c0(F),
                   E = TS::esz(F);
     return (unsigned)(O + (t - R0) * RB + (c - C0) * E);
+}
+
+// LDS of one wave written by its lanes and then read by other lanes of the
+// same wave: LDS operations of a wave complete in order, so only the
+// compiler must not move them across this point
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename S>
+__device__ __forceinline__ void stash_put(char *stash, int q, typename Store<S>::Bits b)
+{
+    ((typename Store<S>::Bits *)stash)[q * 64 + threadIdx.x] = b;
 }
 
 template <typename S>
@@ -203,7 +254,7 @@ __device__ __forceinline__ void stage_issue(const Bufs &B, const unsigned (&coff
         const bool mat = r * TS::NT + w * 64 >= TS::MAT0;   // wave-uniform
         const unsigned ps = plane * (mat ? 4u : (unsigned)TS::ES);
         const unsigned vo = need && coff[r] != OFF_NONE ? coff[r] + pc * ps : OFF_NONE;
-        v[r] = __builtin_amdgcn_raw_buffer_load_b128(mat ? B.mat : B.in, vo, 0, 0);
+        v[r] = __builtin_amdgcn_raw_buffer_load_b128(mat ? B.mat : B.in, vo, 0, LD_CPOL);
     }
 }
 
@@ -264,7 +315,8 @@ __device__ __forceinline__ float tile_vx(const FusedArgs<S> &a, const TLane &L, 
 template <typename S, typename TS, bool X, bool Y, bool Z>
 __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B, const TLane &L, int i,
                                            unsigned plane, const char *cs, const char *ns,
-                                           const Psi &m, const Halo &hz, float &vx_prev)
+                                           const Psi &m, const Halo &hz, float &vx_prev,
+                                           char *stash)
 {
     constexpr unsigned E = sizeof(S);
     const XPlane x = xplane(a.xpl, i);
@@ -290,12 +342,18 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
                                  lds_f32(cs, tof<TS, T_R>(L.tc, L.ckp)), a.idz, a.two_dt, Z && L.szv >= 0,
                                  L.zbh, L.zah, L.zikh, mz);
     // stores: a lane that must not write passes OFF_NONE
-    bst<S>(B.out, L.own ? (F_VX * a.nnn + o) * E : OFF_NONE,
-           Store<S>::enc(vxn, a.isV, F_VX * a.nnn + o, a.salt, TS::sr));
-    bst<S>(B.out, L.own ? (F_VY * a.nnn + o) * E : OFF_NONE,
-           Store<S>::enc(vyn, a.isV, F_VY * a.nnn + o, a.salt, TS::sr));
-    bst<S>(B.out, L.own ? (F_VZ * a.nnn + o) * E : OFF_NONE,
-           Store<S>::enc(vzn, a.isV, F_VZ * a.nnn + o, a.salt, TS::sr));
+    if constexpr (PACK) {
+        stash_put<S>(stash, 1, Store<S>::enc(vxn, a.isV, F_VX * a.nnn + o, a.salt, TS::sr));
+        stash_put<S>(stash, 2, Store<S>::enc(vyn, a.isV, F_VY * a.nnn + o, a.salt, TS::sr));
+        stash_put<S>(stash, 3, Store<S>::enc(vzn, a.isV, F_VZ * a.nnn + o, a.salt, TS::sr));
+    } else {
+        bst_state<S>(B.out, L.own ? (F_VX * a.nnn + o) * E : OFF_NONE,
+                     Store<S>::enc(vxn, a.isV, F_VX * a.nnn + o, a.salt, TS::sr));
+        bst_state<S>(B.out, L.own ? (F_VY * a.nnn + o) * E : OFF_NONE,
+                     Store<S>::enc(vyn, a.isV, F_VY * a.nnn + o, a.salt, TS::sr));
+        bst_state<S>(B.out, L.own ? (F_VZ * a.nnn + o) * E : OFF_NONE,
+                     Store<S>::enc(vzn, a.isV, F_VZ * a.nnn + o, a.salt, TS::sr));
+    }
     {
         const unsigned ix = (unsigned)x.sxv * plane + L.o_jk;
         const unsigned iy = a.my + ((unsigned)i * np2 + L.syv) * a.pz + L.kc;
@@ -371,7 +429,18 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
             if (L.own && (unsigned)cell == o && a.step < len) Pn += ((CF)a.src_inc)[off + a.step];
         }
     }
-    bst<S>(B.out, L.own ? o * E : OFF_NONE, Store<S>::enc(Pn, a.isP, o, a.salt, TS::sr));
+    if constexpr (PACK) {
+        // cells past the grid edge (pad cells of the row) receive the
+        // values their lanes computed; no update ever reads them
+        stash_put<S>(stash, 0, Store<S>::enc(Pn, a.isP, o, a.salt, TS::sr));
+        wave_lds_sync();
+        const u32x4 pk = *(const u32x4 *)(stash + (threadIdx.x % (4 * TS::CPF)) * 16);
+        __builtin_amdgcn_raw_buffer_store_b128(pk, B.out, L.pk_ok ? (L.pk_off + i * plane) * E : OFF_NONE,
+                                               0, ST_CPOL);
+        wave_lds_sync();
+    } else {
+        bst_state<S>(B.out, L.own ? o * E : OFF_NONE, Store<S>::enc(Pn, a.isP, o, a.salt, TS::sr));
+    }
     vx_prev = vxn;
 }
 
@@ -401,7 +470,8 @@ __device__ __forceinline__ void plane_iter(const FusedArgs<S> &a, const Bufs &B,
     constexpr int NR = TS::NR;
     psi_issue<S, X, Y, Z>(a, B, L, i + 1, i + 1 < i1, plane, mn);
     tile_plane<S, TS, X, Y, Z>(a, B, L, i, plane, base + R.s0 * TS::slot_bytes,
-                               base + R.s1 * TS::slot_bytes, mc, hz, R.vx_prev);
+                               base + R.s1 * TS::slot_bytes, mc, hz, R.vx_prev,
+                               base + 3 * TS::slot_bytes + w * TS::stash_bytes);
     // plane i+2 into the slot plane i-1 used (read last before the previous
     // barrier), then the loads of plane i+4 into the same register set
     stage_write<TS, NR>(base + R.s2 * TS::slot_bytes, tid, v);
@@ -498,7 +568,7 @@ a3d_tile_step(FusedArgs<S> a)
 {
     using TS = TileShape<S, TJ, SR>;
     constexpr int NR = TS::NR;
-    __shared__ __attribute__((aligned(16))) char s_tile[3 * TS::slot_bytes];
+    __shared__ __attribute__((aligned(16))) char s_tile[TS::lds_bytes];
 
     // ---------------- receiver gather blocks (previous step) -------------
     if ((int)blockIdx.x >= a.nblocks) {
@@ -563,6 +633,14 @@ a3d_tile_step(FusedArgs<S> a)
         L.mz_h = (unsigned)jc * np2 + (L.szh < 0 ? 0 : L.szh);
         L.tc = jc - j0; L.tjp = jp - j0; L.tjm = jm - j0;
         L.cc = L.kc - k0; L.ckp = kp - k0; L.ckm = km - k0;
+        {
+            // packed stores: lane l writes field l / CPF, cells from
+            // (l % CPF) * (16 / ES) of the tile (the pitch is a multiple of
+            // 8 cells, so a chunk starting inside the row ends inside it)
+            const int q = lane / TS::CPF, c = (lane % TS::CPF) * (16 / TS::ES);
+            L.pk_ok = jv && q < 4 && k0 + c < a.pz;
+            L.pk_off = (unsigned)q * a.nnn + (unsigned)jc * a.pz + (unsigned)(k0 + c);
+        }
         Halo hz{0.f, 1.f, 1.f};
         {
             // CPML tables a, b, 1/k, ah, bh, 1/kh along y (row and row below)

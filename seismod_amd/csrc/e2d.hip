@@ -353,14 +353,25 @@ int comp_slot(int comp)
 
 extern "C" {
 
+// The fused kernel masks an access by passing the per-lane offset 2^31
+// (e2d_fused.hip): that is out of range only while the largest buffer
+// resource, the two state parities plus the split fields, stays below
+// 2^31 bytes.
+static bool e2d_fused_fits(size_t N)
+{
+    return (2 * E2_NSTATE + E2_NSPLIT) * N * 4 < (1ull << 31);
+}
+
 size_t sm_e2d_bytes_needed(const sm_e2d_params *p, size_t nrec, size_t max_steps)
 {
     if (!p) return 0;
     const size_t e = p->storage == SM_STORE_F64 ? 8 : 4;
     const size_t NX = p->nx + 2 * p->npml, NZ = p->nz + 2 * p->npml, N = NX * NZ;
-    if (p->storage == SM_STORE_F32 && !(p->flags & SM_E2D_NAIVE))
+    if (p->storage == SM_STORE_F32 && !(p->flags & SM_E2D_NAIVE)) {
+        if (!e2d_fused_fits(N)) return 0;   // sm_e2d_create refuses the grid
         return 4 * ((2 * E2_NSTATE + E2_NSPLIT + E2_NMAT) * N + 4 * (NX + NZ) + nrec * max_steps) +
                16 * nrec + 4096;
+    }
     return e * (20 * N + 4 * (NX + NZ) + nrec * max_steps) + 16 * nrec;
 }
 
@@ -517,8 +528,9 @@ int sm_e2d_create(sm_ctx *ctx, const sm_e2d_params *pp, const double *vp, const 
                 put(zi + 2 * J, d[r]); put(zh + 2 * J, dH[r]);
             }
         }
-        SM_CHECK((2 * E2_NSTATE + E2_NSPLIT) * N * 4 < (1ull << 32), SM_ERR_ARG,
-                 "sm_e2d_create: padded grid %zux%zu too large for 32-bit offsets", NX, NZ);
+        SM_CHECK(e2d_fused_fits(N), SM_ERR_ARG,
+                 "sm_e2d_create: padded grid %zux%zu too large for the fused FP32 kernel "
+                 "(state and split fields must stay below 2 GiB; use SM_E2D_NAIVE or FP64)", NX, NZ);
         rc = dalloc(s->fields, (2 * E2_NSTATE + E2_NSPLIT) * N * 4);
         // 16 floats past the last material array: the staged kernel's
         // 16-byte chunks of the last row may extend into them

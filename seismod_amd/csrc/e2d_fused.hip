@@ -27,8 +27,9 @@ namespace {
 // in SGPRs), the array's start as the scalar offset and the cell's byte
 // offset as the per-lane offset.  A lane that must not touch memory passes
 // OFF_NONE: the hardware returns 0 for such loads and drops such stores, so
-// masking costs no branch.  Every resource covers less than 2^31 bytes, so
-// OFF_NONE plus any scalar offset stays out of range.
+// masking costs no branch.  Every resource covers less than 2^31 bytes
+// (sm_e2d_create refuses larger grids), so OFF_NONE plus any scalar offset
+// stays out of range.
 constexpr unsigned OFF_NONE = 0x80000000u;
 constexpr int RSRC_WORD3 = 0x00020000;   // raw 32-bit buffer on gfx9
 

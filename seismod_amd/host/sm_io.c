@@ -504,15 +504,23 @@ static int source_type(const char *t, int three_d)
     return -1;
 }
 
-/* grid index of a coordinate, round() as the reference; out of range -> error */
-static size_t node(double x, double x0, double d, size_t n, const char *what)
+/* padded grid index round((x - x0) / d) + np as the reference computes it
+ * (io_utils.c:1488-1492 with np = 0, 1582-1585); an index outside the
+ * padded axis of n + 2 np nodes, where the reference would write out of
+ * bounds, is an error.  3D positions inside the CPML layers are valid. */
+static size_t node_padded(double x, double x0, double d, size_t n, size_t np, const char *what)
 {
-    const double r = round((x - x0) / d);
-    if (!(r >= 0.0) || r >= (double)n) {
+    const double r = round((x - x0) / d) + (double)np;
+    if (!(r >= 0.0) || r >= (double)(n + 2 * np)) {
         fprintf(stderr, "Error: %s coordinate %g outside the grid\n", what, x);
         exit(1);
     }
     return (size_t)r;
+}
+
+static size_t node(double x, double x0, double d, size_t n, const char *what)
+{
+    return node_padded(x, x0, d, n, 0, what);
 }
 
 void sm_read_source2d(const char *fn, const sm_grid *g, sm_sources *src)
@@ -581,9 +589,9 @@ void sm_read_source3d(const char *fn, const sm_grid *g, size_t np, sm_sources *s
         if (fscanf(f, "%lf", &s->A) != 1 || fscanf(f, "%lf", &s->f) != 1 ||
             fscanf(f, "%lf %lf %lf", &s->x, &s->y, &s->z) != 3)
             die("Error: something wrong with file %s\n", fn);
-        size_t i = node(s->x, g->x0, g->dx, g->nx, "source x") + np;
-        size_t j = node(s->y, g->y0, g->dy, g->ny, "source y") + np;
-        size_t k = node(s->z, g->z0, g->dz, g->nz, "source z") + np;
+        size_t i = node_padded(s->x, g->x0, g->dx, g->nx, np, "source x");
+        size_t j = node_padded(s->y, g->y0, g->dy, g->ny, np, "source y");
+        size_t k = node_padded(s->z, g->z0, g->dz, g->nz, np, "source z");
         s->idx = (i * ny2 + j) * nz2 + k;
     }
     fclose(f);
@@ -628,15 +636,19 @@ static void read_output(const char *fn, const sm_grid *g, size_t np, sm_records 
             if (!ok || fscanf(f, "%lf", &r->t0) != 1 || fscanf(f, "%lf", &r->dt) != 1)
                 die("Error: something wrong with file %s\n", fn);
             r->dt *= 1.e-3;   /* ms -> s */
-            /* lround, io_utils.c:1661-1666, 1722-1725 */
-            const double fi = (double)lround((r->x - g->x0) / g->dx);
-            const double fk = (double)lround((r->z - g->z0) / g->dz);
-            const double fj = three_d ? (double)lround((r->y - g->y0) / g->dy) : 0.0;
-            if (fi < 0 || fi >= g->nx || fk < 0 || fk >= g->nz || fj < 0 || fj >= g->ny) {
+            /* lround(...) + np, io_utils.c:1661-1666, 1722-1725: a 3D
+             * record may sit in the CPML layers; outside the padded grid
+             * the reference would read out of bounds */
+            const double fi = (double)lround((r->x - g->x0) / g->dx) + (double)np;
+            const double fk = (double)lround((r->z - g->z0) / g->dz) + (double)np;
+            const double fj = three_d ? (double)lround((r->y - g->y0) / g->dy) + (double)np : 0.0;
+            const double nx2 = (double)(g->nx + 2 * np), nz2 = (double)(g->nz + 2 * np),
+                         ny2 = three_d ? (double)(g->ny + 2 * np) : 1.0;
+            if (fi < 0 || fi >= nx2 || fk < 0 || fk >= nz2 || fj < 0 || fj >= ny2) {
                 fprintf(stderr, "Error: record %zu is outside the grid\n", n + 1);
                 exit(1);
             }
-            const size_t i = (size_t)fi + np, j = (size_t)fj + np, k = (size_t)fk + np;
+            const size_t i = (size_t)fi, j = (size_t)fj, k = (size_t)fk;
             r->idx = three_d ? (i * ny2 + j) * nz2 + k : i * g->nz + k;
         } else if (!strcmp(t, "Snapshot") || !strcmp(t, "snapshot") || !strcmp(t, "SNAPSHOT")) {
             r->type = REC_SNAPSHOT;

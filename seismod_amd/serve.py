@@ -75,6 +75,7 @@ class Result:
     finished: float
     cell_updates: int
     traces: np.ndarray = field(repr=False, default=None)
+    error: str = ""            # non-empty: the request was rejected (no traces)
 
     @property
     def latency(self) -> float:
@@ -145,7 +146,21 @@ def _build(ctx: Context, r: Request, graph_steps: int):
 
 
 def _footprint(r: Request) -> int:
+    """Device bytes the request's solver holds; 0 when the library would
+    refuse the grid."""
     return _config(r).bytes_needed(len(r.shot.rec_idx), r.nsteps)
+
+
+def admission(footprint: int, free: int, reserve: int, n_active: int) -> str:
+    """"admit" when the request fits in free memory minus the reserve,
+    "wait" when it does not fit now but may once running shots retire, and
+    "reject" when it cannot run at all: the library refuses its grid
+    (footprint 0) or it does not fit with nothing else running."""
+    if footprint <= 0:
+        return "reject"
+    if footprint <= free - reserve:
+        return "admit"
+    return "wait" if n_active else "reject"
 
 
 class Server:
@@ -171,26 +186,52 @@ class Server:
         return self._streams.pop() if self._streams else Context(self.device)
 
     def run(self, requests: list[Request], keep_traces: bool = True) -> list[Result]:
-        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        """Serve `requests`; returns one Result per request (rejected ones
+        carry `error` and no traces).  Solvers still live when an error
+        escapes are closed before it propagates."""
         active = []     # dicts: req, sim, ctx, done, admitted
+        try:
+            return self._serve(requests, keep_traces, active)
+        finally:
+            for a in active:
+                a["sim"].close()
+                if a["req"].kind == "a3d":
+                    self._streams.append(a["ctx"])
+            active.clear()
+
+    def _serve(self, requests, keep_traces, active) -> list[Result]:
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         results = []
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0
+
+        def reject(r, why):
+            results.append(Result(r.rid, r.kind, r.arrival, clock(), clock(), 0, None, error=why))
+
         while pending or active:
             now = clock()
             # admission in arrival order while the device has room
             while pending and pending[0].arrival <= now and len(active) < self.max_active:
                 r = pending[0]
                 free, _ = self.ctx.mem_info()
-                if _footprint(r) > free - self.reserve:
-                    if not active:
-                        raise RuntimeError(f"request {r.rid} needs {_footprint(r)} device bytes; "
-                                           f"{free - self.reserve} are free with nothing running")
+                need = _footprint(r)
+                verdict = admission(need, free, self.reserve, len(active))
+                if verdict == "wait":
                     break
                 pending.pop(0)
+                if verdict == "reject":
+                    reject(r, f"needs {need} device bytes (0: grid refused by the library); "
+                              f"{free - self.reserve} free with {len(active)} shots running")
+                    continue
                 c = self._stream() if r.kind == "a3d" else self.ctx
-                active.append(dict(req=r, sim=_build(c, r, self.graph_steps), ctx=c, done=0,
-                                   admitted=clock()))
+                try:
+                    sim = _build(c, r, self.graph_steps)
+                except Exception as e:     # set-up refused (SM_ERR_*): this request only
+                    if r.kind == "a3d":
+                        self._streams.append(c)
+                    reject(r, str(e))
+                    continue
+                active.append(dict(req=r, sim=sim, ctx=c, done=0, admitted=clock()))
             if not active:
                 if pending:
                     time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.05)))
@@ -227,7 +268,7 @@ class Server:
                     self._streams.append(a["ctx"])
                 results.append(Result(r.rid, r.kind, r.arrival, a["admitted"], clock(),
                                       r.cells * r.nsteps, tr))
-            active = still
+            active[:] = still
         return sorted(results, key=lambda x: x.rid)
 
     def _select(self, active: list) -> list:
@@ -284,10 +325,14 @@ def run_sequential(requests: list[Request], device: int = 0, keep_traces: bool =
 
 
 def summarize(results: list[Result]) -> dict:
+    """p50/p99 latency and throughput over the served requests; rejected
+    requests are counted, not timed."""
+    rejected = sum(1 for r in results if r.error)
+    results = [r for r in results if not r.error]
     lat = np.array([r.latency for r in results])
     span = max(r.finished for r in results) - min(r.arrival for r in results)
     work = sum(r.cell_updates for r in results)
-    return {"requests": len(results), "p50_s": round(float(np.percentile(lat, 50)), 4),
+    return {"requests": len(results), "rejected": rejected, "p50_s": round(float(np.percentile(lat, 50)), 4),
             "p99_s": round(float(np.percentile(lat, 99)), 4),
             "mean_s": round(float(lat.mean()), 4), "makespan_s": round(span, 3),
             "gcell_updates_per_s": round(work / span / 1e9, 2)}

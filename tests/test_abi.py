@@ -46,3 +46,19 @@ def test_host_only_entry_points_without_gpu():
     assert record_due(1, 5e-4, 0.0, 1e-3) and not record_due(0, 5e-4, 0.0, 1e-3)
     L, f = source_functions([1.0], [25.0], 5e-4)
     assert L[0] == 160 and abs(abs(f).max() - 1.0) < 1e-15
+
+
+def test_e2d_fused_size_limit_without_gpu():
+    """The fused FP32 2D solver masks accesses with the offset 2^31, so its
+    largest buffer resource (state parities plus split fields, 96 bytes per
+    padded cell) must stay below 2 GiB: bytes_needed reports 0 for a grid
+    that sm_e2d_create would refuse.  FP64 and the two-kernel FP32 form have
+    no such limit."""
+    from seismod_amd.solver import E2dConfig
+    ok = E2dConfig(nx=4729 - 40, nz=4729 - 40, npml=20, h=5.0, dt=1e-4)
+    too_big = E2dConfig(nx=4730 - 40, nz=4730 - 40, npml=20, h=5.0, dt=1e-4)
+    assert 96 * 4729 ** 2 < 2 ** 31 <= 96 * 4730 ** 2
+    assert ok.bytes_needed() > 96 * 4729 ** 2
+    assert too_big.bytes_needed() == 0
+    assert E2dConfig(nx=4690, nz=4690, npml=20, h=5.0, dt=1e-4, storage="f64").bytes_needed() > 0
+    assert E2dConfig(nx=4690, nz=4690, npml=20, h=5.0, dt=1e-4, naive=True).bytes_needed() > 0

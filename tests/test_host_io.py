@@ -395,3 +395,42 @@ def test_seeded_model_file_reads_back_the_shot(tmp_path):
     synth.write_model_a(fn, shot)
     vp, rho = _read3d(fn, shot.np)
     assert np.array_equal(vp, shot.vp) and np.array_equal(rho, shot.rho)
+
+
+def _read_positions_subprocess(tmp_path, src_text, rcv_text, np_):
+    """Read a 3D source and record file in a child process (the readers exit
+    on a bad position); returns (status, stderr, source idx, record idx)."""
+    (tmp_path / "s.dat").write_text(src_text)
+    (tmp_path / "r.dat").write_text(rcv_text)
+    code = (f"import ctypes as C, sys; sys.path.insert(0, {str(ROOT / 'tests')!r});"
+            "import test_host_io as T;"
+            "g = T.Grid(nx=32, ny=30, nz=28, dx=10.0, dy=10.0, dz=10.0);"
+            "s = T.Sources(); o = T.Records();"
+            f"T.LIB.sm_read_source3d(b{str(tmp_path / 's.dat')!r}, C.byref(g), C.c_size_t({np_}), C.byref(s));"
+            f"T.LIB.sm_read_output3d(b{str(tmp_path / 'r.dat')!r}, C.byref(g), C.c_size_t({np_}), C.byref(o));"
+            "print(s.s[0].idx, o.r[0].idx)")
+    r = subprocess.run(["python3", "-c", code], capture_output=True, text=True)
+    idx = tuple(int(v) for v in r.stdout.split()) if r.returncode == 0 else None
+    return r.returncode, r.stderr, idx
+
+
+def test_3d_positions_inside_the_cpml_are_accepted(tmp_path):
+    """a_iso_3d indexes sources and records by round(...) + np in the padded
+    grid (io_utils.c:1582-1585, 1722-1725), so positions inside the CPML
+    layers are valid; only positions past the padded grid, where the
+    reference would index out of bounds, are refused."""
+    np_, ny2, nz2 = 3, 36, 34
+    rc, err, idx = _read_positions_subprocess(
+        tmp_path, "1\n1\nBulk\n1.0\n15.0\n-20.0 100.0 -30.0\n",
+        "1\nVz\nTrace\n330.0 -10.0 290.0 0.0 1.0\n", np_)
+    assert rc == 0, err
+    # source: x -2 -> 1, y 10 -> 13, z -3 -> 0; record: x 33 -> 36, y -1 -> 2, z 29 -> 32
+    assert idx == (((1 * ny2) + 13) * nz2 + 0, ((36 * ny2) + 2) * nz2 + 32)
+    rc, err, _ = _read_positions_subprocess(
+        tmp_path, "1\n1\nBulk\n1.0\n15.0\n-40.0 100.0 0.0\n",
+        "1\nVz\nTrace\n0.0 0.0 0.0 0.0 1.0\n", np_)
+    assert rc == 1 and "outside the grid" in err
+    rc, err, _ = _read_positions_subprocess(
+        tmp_path, "1\n1\nBulk\n1.0\n15.0\n0.0 0.0 0.0\n",
+        "1\nVz\nTrace\n0.0 0.0 310.0 0.0 1.0\n", np_)
+    assert rc == 1 and "outside the grid" in err

@@ -10,6 +10,7 @@ image) and has no tests or golden vectors.  The oracle is pinned instead by
 and the product's FP64 host set-up is checked bitwise against it.
 """
 import math
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -55,6 +56,42 @@ def test_ricker_multi_source_closed_form():
     np.testing.assert_allclose(w0, scale * r0, rtol=1e-12, atol=1e-12 * abs(scale))
 
 
+def _ricker_shape(f, dt, n):
+    """2 pi^2 f^2 (1 - 2K) e^-K at samples i*dt - 1/f (src.c:55-58, A = 1)."""
+    tau = np.arange(n) * dt - 1.0 / f
+    K = (np.pi * f * tau) ** 2
+    return 2 * np.pi ** 2 * f * f * (1 - 2 * K) * np.exp(-K)
+
+
+@pytest.mark.parametrize("A,f,dt", [
+    ([1.0, 2.0, -0.5], [30.0, 25.0, 40.0], 2e-4),      # later sources longer and shorter
+    ([1.0, -2.0, 0.5], [30.0, 25.0, 40.0], 2e-4),      # the sign of A_1 carries over
+    ([-1.5, 3.0], [10.0, 100.0], 2e-4),               # L_1 ends before source 0's peak
+    ([0.7, 1.0, 1.0, -4.0], [20.0, 20.0, 60.0, 5.0], 5e-4),
+])
+def test_ricker_source0_scale_after_every_pass(A, f, dt):
+    """src.c:52-63 rescales source 0 once per source s by A_s / Amax_s, where
+    Amax_s is the peak of |source 0| (as it stands then) over the first
+    min(L_s, L_0) samples.  Solved in closed form: after pass 0 source 0 is
+    |A_0| g / M_0; after pass s >= 1 it is sign(previous scale) A_s g / M_s,
+    g being source 0's unit-amplitude shape and M_s = max_{i < min(L_s, L_0)}
+    |g_i|.  Dropping the repeated rescales, their sign, or the window
+    min(L_s, L_0) each changes the result."""
+    L, fct = O.source_functions(A, f, dt)
+    g = _ricker_shape(f[0], dt, int(L[0]))
+    scale = abs(A[0]) / np.abs(g).max()
+    for s in range(1, len(A)):
+        m = np.abs(g[:min(int(L[s]), int(L[0]))]).max()
+        scale = math.copysign(1.0, scale) * A[s] / m
+    np.testing.assert_allclose(fct[:L[0]], scale * g, rtol=1e-12, atol=0)
+    # later sources stay raw (A_s times their shape)
+    off = int(L[0])
+    for s in range(1, len(A)):
+        np.testing.assert_allclose(fct[off:off + int(L[s])], A[s] * _ricker_shape(f[s], dt, int(L[s])),
+                                   rtol=1e-12, atol=1e-9)
+        off += int(L[s])
+
+
 def test_product_source_functions_match_oracle_bitwise():
     from seismod_amd.solver import source_functions
     for A, f, dt in (([1.0], [25.0], 5e-4), ([1.0, 0.5, -2.0], [25.0, 30.0, 20.0], 3e-4),
@@ -86,6 +123,38 @@ def test_record_predicate_regular_sampling():
     # a snapshot at one time (rdt <= 0) fires once, at the step nearest t0
     due = [it for it in range(400) if O.record_due(it, dt, 0.1, 0.0)]
     assert due == [199]
+
+
+def _record_due_exact(it, dt, t0, rdt):
+    """Record predicate of e_iso.c:643-658 in exact rational arithmetic: the
+    record fires within 0.99 dt of t0, or when t - t0 > 0 is a whole number
+    of record intervals (t = dt (it+1))."""
+    t = dt * (it + 1)
+    d = t - t0
+    if abs(d) < Fraction(99, 100) * dt:
+        return True
+    return t > t0 and rdt > 0 and (d / rdt).denominator == 1
+
+
+def test_record_predicate_against_exact_rationals():
+    """t0 != 0 and record intervals that are not multiples of dt, inputs as
+    the reference reads them (dt and the record interval in ms times 0.001,
+    io_utils.c:242-243, 1653-1677): the FP64 predicate with its fmod and
+    1e-15 / 1e-8 guards fires exactly where exact arithmetic says it should,
+    over 3000 steps of every combination."""
+    from seismod_amd.solver import record_due
+    bad = []
+    for dts in ("0.5", "0.6", "0.25", "1.3", "0.55", "0.1"):
+        for t0s in ("0", "0.0015", "0.01", "0.1234", "0.0003", "0.05"):
+            for rdts in ("0.75", "1.5", "2", "0.6", "4.4", "1", "0.5", "3.3", "0"):
+                dtF, t0F, rdF = Fraction(dts) / 1000, Fraction(t0s), Fraction(rdts) / 1000
+                dt, t0, rdt = float(dts) * 0.001, float(t0s), float(rdts) * 0.001
+                want = [_record_due_exact(it, dtF, t0F, rdF) for it in range(3000)]
+                got = [O.record_due(it, dt, t0, rdt) for it in range(3000)]
+                prod = [record_due(it, dt, t0, rdt) for it in range(0, 3000, 3)]
+                if got != want or prod != want[::3]:
+                    bad.append((dts, t0s, rdts))
+    assert not bad, bad
 
 
 # ---------------------------------------------------------------- CPML / PML tables
@@ -305,3 +374,110 @@ def test_e2d_pml_absorbs():
     _, f_on, _, _ = _e2d(d, 1.0)
     _, f_off, _, _ = _e2d(d, 0.0)
     assert np.linalg.norm(f_on) < 0.1 * np.linalg.norm(f_off)
+
+
+# ---------------------------------------------------------------- 2D physics
+def _e2d_homogeneous(src_type, n, h, f0, T, recs, vp0=3000.0, vs0=1700.0, rho0=2200.0, npml=20):
+    """Homogeneous e_iso run, source at interior node (n/2, n/2); recs are
+    (di, dj, comp) offsets from it."""
+    m = [np.full((n, n), v) for v in (vp0, vs0, rho0)]
+    dt = 0.4 * h / (math.sqrt(2) * vp0)
+    cfg = dict(nx=n, nz=n, npml=npml, h=h, dt=dt, q1_compat=True)
+    c0 = n // 2
+    L, fct = O.source_functions([1.0], [f0], dt)
+    ri = np.array([(c0 + a) * n + c0 + b for a, b, _ in recs], np.uintp)
+    rc = np.array([c for _, _, c in recs], np.intc)
+    tr, f, _ = O.e2d_run(cfg, *m, np.array([src_type], np.intc), np.array([c0 * n + c0], np.uintp),
+                         L, fct, ri, rc, int(T / dt))
+    return tr, f, dt, L
+
+
+def test_e2d_bulk_source_green_function():
+    """A Bulk source (txx and tzz += dt fct, src.c:170-184) in a homogeneous
+    medium radiates P only: v = grad(Phi) with Phi_tt - a^2 lap(Phi) =
+    (h^2/rho) delta(x) s(t), whose 2D solution is
+    Phi = h^2/(2 pi rho a^2) int_0^acosh(a t/r) s(t - (r/a) cosh u) du.
+    The injection of step it stands for the rate s at (it + 1/2) dt and the
+    velocity recorded after step it is the one at (it + 1/2) dt.  Receivers 40
+    to 60 cells away on the axes and a diagonal, vx and vz at their staggered
+    positions (vx at (i, j), vz at (i+1/2, j+1/2), the source at (i+1/2, j));
+    measured 1.1-2.5% (grid dispersion), held to 3%, where half a step of
+    timing error alone gives 4% and a misplaced half cell 7%."""
+    n, h, f0, vp0, rho0 = 240, 5.0, 15.0, 3000.0, 2200.0
+    recs = [(0, 40, decks.VZ), (0, -40, decks.VZ), (30, 30, decks.VX), (40, 0, decks.VX),
+            (30, 30, decks.VZ), (0, 60, decks.VZ)]
+    tr, _, dt, L = _e2d_homogeneous(decks.BULK, n, h, f0, 0.2, recs, vp0=vp0, rho0=rho0)
+    tcut = 1.0 / f0
+    a2 = (np.pi * f0) ** 2
+    ts = np.arange(L[0]) * dt - tcut
+    peak = np.abs((1 - 2 * a2 * ts ** 2) * np.exp(-a2 * ts ** 2)).max()
+
+    def rate(s):   # fct sampled continuously, delayed by half a step
+        tau = s - 0.5 * dt - tcut
+        return (1 - 2 * a2 * tau ** 2) * np.exp(-a2 * tau ** 2) / peak * ((tau >= -tcut) & (tau <= tcut))
+
+    def phi(r, t):
+        out = np.zeros_like(t)
+        for k, tt in enumerate(t):
+            if vp0 * tt > r:
+                u = np.linspace(0.0, math.acosh(vp0 * tt / r), 2001)
+                out[k] = np.trapezoid(rate(tt - (r / vp0) * np.cosh(u)), u)
+        return h * h / (2 * np.pi * rho0 * vp0 ** 2) * out
+
+    t = (np.arange(tr.shape[0]) + 0.5) * dt
+    for k, (a, b, comp) in enumerate(recs):
+        dx, dz = ((a * h, (b + 0.5) * h) if comp == decks.VZ else ((a - 0.5) * h, b * h))
+        r = math.hypot(dx, dz)
+        e = 1e-3 * r
+        dphi = (phi(r + e, t) - phi(r - e, t)) / (2 * e)
+        want = dphi * (dz if comp == decks.VZ else dx) / r
+        assert rel_l2(tr[:, k], want) < 0.03, (k, rel_l2(tr[:, k], want))
+
+
+def test_e2d_bulk_source_is_curl_free():
+    """In a homogeneous medium the staggered scheme keeps a Bulk source's
+    field exactly irrotational: the discrete curl at the txz positions,
+    (vx[i,j+1] - vx[i,j]) - (vz[i,j] - vz[i-1,j]), stays at rounding level
+    while the wave is inside the interior (measured 7e-16 of the strain).
+    A misplaced stagger, a swapped lambda / lambda+2mu or a dropped term
+    of update_txxzz breaks it."""
+    n, npml = 160, 20
+    _, f, _, _ = _e2d_homogeneous(decks.BULK, n, 5.0, 15.0, 0.10, [(0, 40, decks.VZ)], npml=npml)
+    N = n + 2 * npml
+    vx = f[:N * N].reshape(N, N)
+    vz = f[N * N:2 * N * N].reshape(N, N)
+    dvxz = vx[1:, 1:] - vx[1:, :-1]
+    dvzx = vz[1:, :-1] - vz[:-1, :-1]
+    inner = (slice(npml, -npml), slice(npml, -npml))
+    curl = np.abs((dvxz - dvzx)[inner]).max()
+    strain = np.abs((dvxz + dvzx)[inner]).max()
+    assert strain > 0 and curl < 1e-13 * strain, curl / strain
+
+
+def test_e2d_sxz_source_radiation_pattern():
+    """A txz source (Sxz) radiates P as sin(2 theta) and S as cos(2 theta):
+    on the x and z axes nearly all energy arrives in the S window (first
+    arrival at Vs, none at Vp), on the diagonal nearly all in the P window.
+    A Bulk source puts its energy in the P window everywhere.  Receivers at
+    100 cells (h = 2 m, 40 Hz: P and S windows of the wavelet separate)."""
+    R = 100
+    d = int(R / math.sqrt(2))
+    recs = [(R, 0, decks.VX), (R, 0, decks.VZ), (d, d, decks.VX), (d, d, decks.VZ),
+            (0, R, decks.VX), (0, R, decks.VZ)]
+    h, f0, vp0, vs0 = 2.0, 40.0, 3000.0, 1700.0
+    tc = 1.0 / f0
+    share = {}
+    for st in (decks.SXZ, decks.BULK):
+        tr, _, dt, _ = _e2d_homogeneous(st, 260, h, f0, 0.175, recs, vp0=vp0, vs0=vs0)
+        t = (np.arange(tr.shape[0]) + 0.5) * dt
+        for k in range(0, 6, 2):
+            e = tr[:, k] ** 2 + tr[:, k + 1] ** 2
+            r = math.hypot(recs[k][0], recs[k][1]) * h
+            pw = (t > r / vp0 - 0.2 * tc) & (t < r / vp0 + 2.2 * tc)
+            sw = (t > r / vs0 - 0.2 * tc) & (t < r / vs0 + 2.2 * tc)
+            share[(st, k)] = (e[pw].sum() / e.sum(), e[sw].sum() / e.sum())
+    for k in (0, 4):   # axes: S only
+        assert share[(decks.SXZ, k)][0] < 0.01 and share[(decks.SXZ, k)][1] > 0.99, share
+    assert share[(decks.SXZ, 2)][0] > 0.85, share      # diagonal: mostly P
+    for k in (0, 2, 4):
+        assert share[(decks.BULK, k)][1] < 0.01, share

@@ -84,3 +84,21 @@ def test_unknown_policy_is_rejected():
     import pytest
     with pytest.raises(ValueError):
         serve.Server(policy="fifo")
+
+
+def test_admission_rejects_only_what_cannot_run():
+    """A request larger than the device (or one whose grid the library
+    refuses: footprint 0) is rejected on its own; one that does not fit
+    while others run waits."""
+    GiB = 1 << 30
+    assert serve.admission(4 * GiB, free=10 * GiB, reserve=2 * GiB, n_active=3) == "admit"
+    assert serve.admission(9 * GiB, free=10 * GiB, reserve=2 * GiB, n_active=3) == "wait"
+    assert serve.admission(9 * GiB, free=10 * GiB, reserve=2 * GiB, n_active=0) == "reject"
+    assert serve.admission(0, free=10 * GiB, reserve=2 * GiB, n_active=3) == "reject"
+
+
+def test_summary_counts_rejected_requests_apart():
+    rs = [serve.Result(i, "e2d", 0.0, 0.0, 1.0, 10) for i in range(4)]
+    rs.append(serve.Result(9, "a3d", 0.0, 0.5, 0.5, 0, None, error="too large"))
+    s = serve.summarize(rs)
+    assert s["requests"] == 4 and s["rejected"] == 1 and s["p99_s"] == 1.0
