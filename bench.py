@@ -13,7 +13,10 @@ Gaussian-smoothed (sigma 8 cells) +-5% perturbation, Gardner density,
 z = 50 m, dt = 0.6 ms, 1500 steps, FP32.
 
 Prints one JSON line.  value = padded cells x time steps x shots / wall
-time of the timed shots (Gcell-updates/s).
+time of the timed shots (Gcell-updates/s).  After the timed region the
+same shot runs once with FP64 storage and the line states the trace error
+against it (error_vs_fp64; the run fails above the stated tolerance), and
+the 512^3 FP32 step is timed over a few hundred steps (fp32_512).
 """
 from __future__ import annotations
 
@@ -47,7 +50,18 @@ def parse():
                          "them during set-up, before the timed region)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-fp64-check", action="store_true",
+                    help="skip the FP64 run of the same shot that the error is stated against")
+    ap.add_argument("--no-512", action="store_true",
+                    help="skip the extra 512^3 FP32 measurement (target size)")
+    ap.add_argument("--steps-512", type=int, default=200,
+                    help="time steps of the extra 512^3 measurement")
     return ap.parse_args()
+
+
+# trace error of the timed storage against the FP64 run of the same shot
+# (relative L2 over all traces); the bench fails above it
+ERROR_TOL = {"f32": 2e-5, "f16": 4e-2, "bf16": 3e-1}
 
 
 def cpu_baseline(shot, nsteps_sample: int) -> dict:
@@ -100,6 +114,65 @@ def load_traffic(name: str, variant: str, rows: int):
     if d.get("variant") != variant or d.get("rows_per_block") != rows:
         return None
     return d.get("hbm_bytes_per_launch")
+
+
+def fp64_error(ctx, shot, traces, nsteps) -> dict:
+    """Run the timed shot once more with FP64 storage and arithmetic (the
+    path whose traces and fields equal the FP64 oracle's bit for bit) and
+    state the timed run's trace error against it."""
+    from seismod_amd.solver import A3dConfig, Acoustic3D, source_functions
+    cfg = A3dConfig(nx=shot.nx, ny=shot.ny, nz=shot.nz, np=shot.np, dx=shot.dx, dy=shot.dx,
+                    dz=shot.dx, dt=shot.dt, f0=shot.f0, storage="f64")
+    sim = Acoustic3D(ctx, cfg, shot.vp, shot.rho)
+    L, fct = source_functions([1.0], [shot.f0], shot.dt)
+    sim.set_sources(shot.src_idx, L, fct)
+    sim.set_receivers(shot.rec_idx, shot.rec_comp, nsteps)
+    sim.run(nsteps)
+    ref = sim.traces(0, nsteps)
+    sim.close()
+    err = float(np.linalg.norm(traces - ref) / np.linalg.norm(ref))
+    return {"traces_rel_l2": err, "against": "FP64 run of the same shot on the GPU (the FP64 path "
+            "equals the FP64 oracle bit for bit)", "traces": int(ref.shape[1]), "samples": nsteps}
+
+
+def measure_512(ctx, storage: str, nsteps: int) -> dict:
+    """Kernel time of the 3D step at 512^3 (+ CPML 20: 552^3 padded cells,
+    seed 3, Vp 1500-5000 m/s, dt 0.55 ms), autotuned launch, over nsteps
+    steps from a zeroed state."""
+    from seismod_amd import autotune, synth
+    from seismod_amd.solver import A3dConfig, Acoustic3D, source_functions
+    shot = synth.a3d_layered_shot(n=512, seed=3, dt=5.5e-4, nsteps=nsteps, vmax=5000.0)
+    cfg = A3dConfig(nx=shot.nx, ny=shot.ny, nz=shot.nz, np=shot.np, dx=shot.dx, dy=shot.dx,
+                    dz=shot.dx, dt=shot.dt, f0=shot.f0, storage=storage)
+    sim = Acoustic3D(ctx, cfg, shot.vp, shot.rho)
+    L, fct = source_functions([1.0], [shot.f0], shot.dt)
+    sim.set_sources(shot.src_idx, L, fct)
+    autotune.tune_a3d(sim, ctx)
+    sim.set_receivers(shot.rec_idx, shot.rec_comp, nsteps)
+    sim.reset()
+    ctx.sync()
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    sim.run(nsteps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    kms, launches = sim.timing()
+    tr = sim.traces(0, nsteps)
+    variant, (tj, ch) = sim.variant(), sim.launch()
+    sim.close()
+    if not np.isfinite(tr).all():
+        raise RuntimeError("512^3 traces are not finite")
+    nx2, ny2, nz2 = shot.shape
+    alg_b = synth.algorithmic_bytes_a3d(nx2, ny2, nz2, shot.np, state_bytes=STATE_BYTES[storage],
+                                        mat_bytes=4)
+    avg_ms = kms / max(launches, 1)
+    achieved = alg_b * shot.cells / (avg_ms * 1e-3) / 1e9
+    return {"workload": f"a_iso_3d 512^3 + CPML 20 (552^3 padded cells), {storage} storage, "
+                        f"{nsteps} time steps timed", "value": round(shot.cells * nsteps / wall / 1e9, 3),
+            "unit": "Gcell-updates/s", "kernel_ms_avg": round(avg_ms, 5),
+            "roofline_frac": round(achieved / HBM_PEAK_GBS, 4), "achieved_GBs": round(achieved, 1),
+            "algorithmic_bytes_per_cell": round(alg_b, 3),
+            "launch": {"kernel_variant": variant, "rows_per_block": tj, "x_chunks": ch}}
 
 
 def main():
@@ -206,6 +279,16 @@ def main():
         "cpu_baseline": None,
     }
     sim.close()
+    if not args.no_fp64_check and args.storage != "f64":
+        line["error_vs_fp64"] = fp64_error(ctx, shot, tr, nsteps)
+        line["error_vs_fp64"]["tolerance"] = ERROR_TOL[args.storage]
+        if line["error_vs_fp64"]["traces_rel_l2"] > ERROR_TOL[args.storage]:
+            print(json.dumps(line), flush=True)
+            print(f"bench.py: trace error {line['error_vs_fp64']['traces_rel_l2']:.3e} against the "
+                  f"FP64 run exceeds {ERROR_TOL[args.storage]}", file=sys.stderr)
+            sys.exit(1)
+    if not args.no_512 and args.n != 512:
+        line["fp32_512"] = measure_512(ctx, "f32", args.steps_512)
     ctx.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(shot, args.cpu_steps)

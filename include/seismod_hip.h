@@ -308,6 +308,10 @@ int sm_e2d_batch_run(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots,
 int sm_e2d_batch_set_variant(sm_e2d_batch *b, int variant);
 int sm_e2d_batch_set_timing(sm_e2d_batch *b, int on);
 int sm_e2d_batch_get_timing(const sm_e2d_batch *b, double *ms_total, size_t *launches);
+/* Captured hipGraphs the batch holds (at most 16: the least recently used
+ * one is destroyed to make room) and has instantiated since creation (no
+ * counterpart in the reference). */
+int sm_e2d_batch_get_graphs(const sm_e2d_batch *b, size_t *live, size_t *built);
 
 #ifdef __cplusplus
 }

@@ -128,6 +128,54 @@ int o_e2d_run(const o_e2d_params *p, const double *vp, const double *vs,
               const int *rec_comp, size_t nsteps, double *traces,
               double *fields, double *split);
 
+/* ------------------------------------------------------------------ */
+/* 2.5D elastic, cylindrical coordinates, CPML (e_cyl_iso)             */
+/* ------------------------------------------------------------------ */
+/* source types and record components of e_cyl_iso (structs.h:44-45) */
+enum { O_SRC_BULK = 6, O_SRC_FR = 9, O_SRC_FT = 10, O_SRC_FZ = 11, O_SRC_KURKJIAN = 12 };
+enum { O_DIV = 11, O_CURL = 12, O_VR = 13, O_VT = 14, O_TRR = 15, O_TRT = 16, O_TRZ = 17,
+       O_TTT = 18, O_TTZ = 19 };
+
+typedef struct {
+    size_t nx, nz;       /* interior nodes along r and z */
+    size_t np;           /* CPML layers (0: abs off); nx2 = nx+np, nz2 = nz+2np */
+    double dx, dz, dt;
+    int n;               /* azimuthal mode (azimuthal_mode, >= 0) */
+    double vmax, pml_order, Rc, kappa_max;
+    double alpha_max;    /* pi f0, or 0 when abs_params.alpha_max is 0 */
+} OCylParams;
+
+/* Staggered material of e_cyl_iso.c:342-415 on the padded nx2 x nz2 grid
+ * (lambda, lambda+2mu at cell centres, mu at nodes by 4-point harmonic
+ * means, mu and 1/rho averaged between r and z neighbours).  Returns 0, or
+ * 1 + the first cell holding a non-finite value (the reference aborts).
+ * Pinned: closed forms (tests/test_oracle_cyl.py). */
+int o_cyl_material(size_t nx2, size_t nz2, const double *vp, const double *vs, const double *rho,
+                   double *lij, double *l2mij, double *mu, double *mui, double *muj, double *bi,
+                   double *bj, double *bij, double *vpmax);
+
+/* CPML tables of pml.c:2107-2164 (see oracle_cyl.c for the order in t:
+ * 9 tables of np then 6 of np+1).  Pinned: closed form. */
+void o_cyl_cpml(size_t np, double dx, double dz, double dt, double vmax, double order, double Rc,
+                double kappa_max, double alpha_max, double *t);
+
+/* Kurkjian source function of src.c:202-229.  Pinned: closed form. */
+void o_kurkjian_fct(double lij_src, double dx, double dz, double x, int n, double A, double f,
+                    double dt, size_t length, double *fct);
+
+/* Whole e_cyl_iso run (e_cyl_iso.c:601-866).
+ *   vp, vs, rho: padded nx2 x nz2 model as read_model_e_cyl leaves it;
+ *   type: O_SRC_*; force sources (Fr, Ftheta, Fz, Bulk) after the velocity
+ *   update, a Kurkjian source as entry 0 after the stress update;
+ *   rcomp: O_VR .. O_TTZ, O_VZ, O_TZZ, O_DIV, O_CURL;
+ *   fields: vr, vt, vz, trr, ttt, tzz, trz, trt, ttz (nx2*nz2 each).
+ * O_ERR_CFL where e_cyl_iso.c:454-458 aborts, O_ERR_ARG for a material
+ * value that is not finite (417-452). */
+int o_cyl_run(const OCylParams *p, const double *vp, const double *vs, const double *rho,
+              size_t nsrc, const int *type, const size_t *idx, const size_t *len, const double *fct,
+              size_t nrec, const size_t *ridx, const int *rcomp, size_t nsteps, double *traces,
+              double *fields);
+
 #ifdef __cplusplus
 }
 #endif

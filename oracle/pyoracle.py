@@ -37,6 +37,15 @@ class E2dParams(C.Structure):
     ]
 
 
+class CylParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_size_t), ("nz", C.c_size_t), ("np", C.c_size_t),
+        ("dx", C.c_double), ("dz", C.c_double), ("dt", C.c_double), ("n", C.c_int),
+        ("vmax", C.c_double), ("pml_order", C.c_double), ("Rc", C.c_double),
+        ("kappa_max", C.c_double), ("alpha_max", C.c_double),
+    ]
+
+
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} not built; run `python -m seismod_amd.build`")
@@ -55,6 +64,12 @@ def _load() -> C.CDLL:
         "o_e2d_material": (None, [C.c_size_t, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "o_e2d_run": (C.c_int, [C.POINTER(E2dParams), _dp, _dp, _dp, C.c_size_t, _ip, _sp, _sp,
                                 _dp, C.c_size_t, _sp, _ip, C.c_size_t, _dp, _dp, _dp]),
+        "o_cyl_material": (C.c_int, [C.c_size_t, C.c_size_t, _dp, _dp, _dp] + [_dp] * 9),
+        "o_cyl_cpml": (None, [C.c_size_t] + [C.c_double] * 8 + [_dp]),
+        "o_kurkjian_fct": (None, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                  C.c_double, C.c_double, C.c_double, C.c_size_t, _dp]),
+        "o_cyl_run": (C.c_int, [C.POINTER(CylParams), _dp, _dp, _dp, C.c_size_t, _ip, _sp, _sp,
+                                _dp, C.c_size_t, _sp, _ip, C.c_size_t, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -164,3 +179,60 @@ def e2d_run(cfg: dict, vp, vs, rho, src_type, src_idx, src_len, src_fct, rec_idx
     if st != O_OK:
         raise RuntimeError(f"o_e2d_run status {st}")
     return traces, fields, split
+
+
+# ---------------------------------------------------------------- e_cyl_iso
+CYL_FIELDS = ("vr", "vt", "vz", "trr", "ttt", "tzz", "trz", "trt", "ttz")
+
+
+def cyl_material(vp, vs, rho):
+    """Staggered material of e_cyl_iso.c:342-415 on the padded grid; returns
+    (dict of arrays, vpmax, status)."""
+    nx2, nz2 = vp.shape
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (vp, vs, rho)]
+    out = [np.zeros((nx2, nz2)) for _ in range(8)]
+    vpmax = C.c_double()
+    st = lib.o_cyl_material(nx2, nz2, *(_d(x) for x in a), *(_d(x) for x in out), C.byref(vpmax))
+    names = ("lij", "l2mij", "mu", "mui", "muj", "bi", "bj", "bij")
+    return dict(zip(names, out)), vpmax.value, st
+
+
+def cyl_cpml(np_, dx, dz, dt, vmax=5000.0, order=2.0, Rc=1e-5, kappa_max=1.0, alpha_max=0.0):
+    t = np.zeros(9 * np_ + 6 * (np_ + 1))
+    lib.o_cyl_cpml(np_, dx, dz, dt, vmax, order, Rc, kappa_max, alpha_max, _d(t))
+    names = ("ik_r", "ik_z", "ikh_z", "b_r", "c_r", "b_z", "c_z", "bh_z", "ch_z")
+    out = {k: t[q * np_:(q + 1) * np_] for q, k in enumerate(names)}
+    H = t[9 * np_:]
+    for q, k in enumerate(("ikH_r", "ikH_z", "bH_r", "cH_r", "bH_z", "cH_z")):
+        out[k] = H[q * (np_ + 1):(q + 1) * (np_ + 1)]
+    return out
+
+
+def kurkjian_fct(lij_src, dx, dz, x, n, A, f, dt):
+    L = ricker_length(f, dt)
+    fct = np.zeros(L)
+    lib.o_kurkjian_fct(lij_src, dx, dz, x, n, A, f, dt, L, _d(fct))
+    return fct
+
+
+def cyl_run(cfg: dict, vp, vs, rho, src_type, src_idx, src_len, src_fct, rec_idx, rec_comp,
+            nsteps, want_fields=True):
+    """Run the e_cyl_iso restatement; cfg keys as CylParams fields."""
+    p = CylParams(cfg["nx"], cfg["nz"], cfg["np"], cfg["dx"], cfg["dz"], cfg["dt"], cfg.get("n", 0),
+                  cfg.get("vmax", 5000.0), cfg.get("pml_order", 2.0), cfg.get("Rc", 1e-5),
+                  cfg.get("kappa_max", 1.0), cfg.get("alpha_max", 0.0))
+    m = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (vp, vs, rho)]
+    st_ = np.ascontiguousarray(src_type, dtype=np.intc)
+    si = np.ascontiguousarray(src_idx, dtype=np.uintp)
+    sl = np.ascontiguousarray(src_len, dtype=np.uintp)
+    sf = np.ascontiguousarray(src_fct, dtype=np.float64)
+    ri = np.ascontiguousarray(rec_idx, dtype=np.uintp)
+    rc = np.ascontiguousarray(rec_comp, dtype=np.intc)
+    N = m[0].size
+    traces = np.zeros((nsteps, len(ri)))
+    fields = np.zeros(9 * N) if want_fields else None
+    st = lib.o_cyl_run(C.byref(p), *(_d(a) for a in m), len(si), _i(st_), _s(si), _s(sl), _d(sf),
+                       len(ri), _s(ri), _i(rc), nsteps, _d(traces), _d(fields))
+    if st != O_OK:
+        raise RuntimeError(f"o_cyl_run status {st}")
+    return traces, fields

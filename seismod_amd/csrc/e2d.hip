@@ -890,10 +890,12 @@ int sm_e2d_batch_set_graph_steps(sm_e2d_batch *b, int graph_steps)
     return SM_OK;
 }
 
-int sm_e2d_batch_run(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots, const size_t *nsteps)
+// Checks the shots of a batch run and builds their descriptors; work
+// queued on the shots' own streams is complete when it returns.
+static int batch_shots(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots, const size_t *nsteps,
+                       std::vector<E2dShot> &d)
 {
     SM_CHECK(b && (nshots == 0 || (shots && nsteps)), SM_ERR_ARG, "sm_e2d_batch_run: NULL argument");
-    std::vector<E2dShot> d;
     d.reserve(nshots);
     for (size_t k = 0; k < nshots; ++k) {
         sm_e2d *s = shots[k];
@@ -913,7 +915,15 @@ int sm_e2d_batch_run(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots, const
     for (size_t k = 0; k < nshots; ++k)
         if (shots[k]->ctx->stream != b->ctx->stream) SM_HIP(hipStreamSynchronize(shots[k]->ctx->stream));
     b->runner.staged = b->variant == SM_E2D_STAGED;
-    int rc = b->runner.run(d, b->ctx->stream, b->graph_steps);
+    return SM_OK;
+}
+
+int sm_e2d_batch_run(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots, const size_t *nsteps)
+{
+    std::vector<E2dShot> d;
+    int rc = batch_shots(b, nshots, shots, nsteps, d);
+    if (rc != SM_OK) return rc;
+    rc = b->runner.run(d, b->ctx->stream, b->graph_steps);
     if (rc != SM_OK) return rc;
     SM_HIP(hipStreamSynchronize(b->ctx->stream));
     for (size_t k = 0; k < nshots; ++k) shots[k]->step += nsteps[k];
@@ -943,6 +953,14 @@ int sm_e2d_batch_get_timing(const sm_e2d_batch *b, double *ms_total, size_t *lau
     SM_CHECK(b && ms_total && launches, SM_ERR_ARG, "sm_e2d_batch_get_timing: NULL argument");
     *ms_total = b->runner.ms_total;
     *launches = b->runner.launches;
+    return SM_OK;
+}
+
+int sm_e2d_batch_get_graphs(const sm_e2d_batch *b, size_t *live, size_t *built)
+{
+    SM_CHECK(b && live && built, SM_ERR_ARG, "sm_e2d_batch_get_graphs: NULL argument");
+    *live = b->runner.graphs_live();
+    *built = b->runner.graphs_built;
     return SM_OK;
 }
 

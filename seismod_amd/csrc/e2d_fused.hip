@@ -473,7 +473,7 @@ E2dRunner::~E2dRunner()
 
 void E2dRunner::drop_graphs()
 {
-    for (auto &kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : graphs_) (void)hipGraphExecDestroy(kv.second.exec);
     graphs_.clear();
 }
 
@@ -504,7 +504,20 @@ int E2dRunner::graph_for(int nact, int nitems, int g, hipStream_t st, hipGraphEx
 {
     const auto key = std::make_tuple(nact, nitems, g);
     auto f = graphs_.find(key);
-    if (f != graphs_.end()) { *out = f->second; return SM_OK; }
+    if (f != graphs_.end()) {
+        f->second.used = ++tick_;
+        *out = f->second.exec;
+        return SM_OK;
+    }
+    if (graphs_.size() >= MAX_GRAPHS) {
+        auto lru = graphs_.begin();
+        for (auto it = graphs_.begin(); it != graphs_.end(); ++it)
+            if (it->second.used < lru->second.used) lru = it;
+        // replays of it may still be queued on this stream
+        SM_HIP(hipStreamSynchronize(st));
+        (void)hipGraphExecDestroy(lru->second.exec);
+        graphs_.erase(lru);
+    }
     const E2dShot *tab = (const E2dShot *)table_;
     const int *bshot = (const int *)((const char *)table_ + tab_bytes_);
     const int nb = grid_blocks(nitems);
@@ -518,12 +531,19 @@ int E2dRunner::graph_for(int nact, int nitems, int g, hipStream_t st, hipGraphEx
     const hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(SM_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-    graphs_[key] = ex;
+    graphs_[key] = Cached{ex, ++tick_};
+    ++graphs_built;
     *out = ex;
     return SM_OK;
 }
 
-int E2dRunner::run(std::vector<E2dShot> shots, hipStream_t st, int graph_steps)
+// Sorts the shots by step count (the active set at any step is then a
+// prefix), gives each its first work item, and lays out the table the
+// kernel reads: the shot descriptors, then at tab_bytes the block -> shot
+// map (every shot's items are whole blocks).  pref[m] is the number of work
+// items of the first m shots.
+int E2dRunner::plan(std::vector<E2dShot> &shots, std::vector<int> &pref,
+                    std::vector<unsigned char> &host, size_t &tab_bytes)
 {
     shots.erase(std::remove_if(shots.begin(), shots.end(),
                                [](const E2dShot &d) { return d.nsteps <= 0; }),
@@ -532,7 +552,7 @@ int E2dRunner::run(std::vector<E2dShot> shots, hipStream_t st, int graph_steps)
     std::stable_sort(shots.begin(), shots.end(),
                      [](const E2dShot &a, const E2dShot &b) { return a.nsteps > b.nsteps; });
     const size_t n = shots.size();
-    std::vector<int> pref(n + 1, 0);
+    pref.assign(n + 1, 0);
     for (size_t i = 0; i < n; ++i) {
         shots[i].item0 = pref[i];
         int ntz, nch;
@@ -541,19 +561,68 @@ int E2dRunner::run(std::vector<E2dShot> shots, hipStream_t st, int graph_steps)
         SM_CHECK(next < (1ll << 30), SM_ERR_ARG, "batch too large: %lld work items", next);
         pref[i + 1] = (int)next;
     }
-    // block -> shot map (every shot's items are whole blocks)
     const int nblk = pref[n] / E2D_WAVES;
-    tab_bytes_ = (n * sizeof(E2dShot) + 255) & ~(size_t)255;
-    std::vector<unsigned char> host(tab_bytes_ + (size_t)nblk * sizeof(int));
+    tab_bytes = (n * sizeof(E2dShot) + 255) & ~(size_t)255;
+    host.assign(tab_bytes + (size_t)nblk * sizeof(int), 0);
     std::memcpy(host.data(), shots.data(), n * sizeof(E2dShot));
-    int *bs = (int *)(host.data() + tab_bytes_);
+    int *bs = (int *)(host.data() + tab_bytes);
     for (size_t i = 0; i < n; ++i)
         for (int q = pref[i] / E2D_WAVES; q < pref[i + 1] / E2D_WAVES; ++q) bs[q] = (int)i;
+    return SM_OK;
+}
+
+// One launch per step over the shots still running (a prefix of the sorted
+// list); with graph_steps > 1, stretches of at least two graphs' length are
+// replayed from captured graphs.  Returns the number of step launches.
+int E2dRunner::launch_all(const std::vector<E2dShot> &shots, const std::vector<int> &pref,
+                          const void *table, size_t tab_bytes, hipStream_t st, int graph_steps,
+                          size_t *count)
+{
+    const E2dShot *tab = (const E2dShot *)table;
+    const int *bshot = (const int *)((const char *)table + tab_bytes);
+    const int n = (int)shots.size();
+    int s = 0;
+    const int total = shots[0].nsteps;
+    while (s < total) {
+        int m = n;
+        while (m > 0 && shots[m - 1].nsteps <= s) --m;
+        const int e = shots[m - 1].nsteps;
+        const int nitems = pref[m];
+        int len = e - s;
+        if (graph_steps > 1 && len >= 2 * graph_steps) {
+            hipGraphExec_t ex = nullptr;
+            int rc = graph_for(m, nitems, graph_steps, st, &ex);
+            if (rc != SM_OK) return rc;
+            e2d_base_set<<<1, 64, 0, st>>>(base_, s, 0);
+            const int reps = len / graph_steps;
+            for (int r = 0; r < reps; ++r) SM_HIP(hipGraphLaunch(ex, st));
+            s += reps * graph_steps;
+            *count += (size_t)reps * graph_steps;
+            len = e - s;
+        }
+        const int nb = grid_blocks(nitems);
+        for (int k = 0; k < len; ++k)
+            launch_step(nb, st, tab, bshot, nitems, nullptr, s + k);
+        s += len;
+        *count += (size_t)len;
+    }
+    SM_HIP(hipGetLastError());
+    return SM_OK;
+}
+
+int E2dRunner::run(std::vector<E2dShot> shots, hipStream_t st, int graph_steps)
+{
+    std::vector<int> pref;
+    std::vector<unsigned char> host;
+    size_t tab_bytes = 0;
+    int rc = plan(shots, pref, host, tab_bytes);
+    if (rc != SM_OK || shots.empty()) return rc;
+    tab_bytes_ = tab_bytes;
     if (host.size() > table_cap_ || tab_bytes_ != last_tab_bytes_ || staged != last_staged_)
         drop_graphs();
     last_tab_bytes_ = tab_bytes_;
     last_staged_ = staged;
-    int rc = ensure(host.size());
+    rc = ensure(host.size());
     if (rc != SM_OK) return rc;
     // the table lives in device memory read by the kernels of this run;
     // waiting for the copy keeps the host buffer's lifetime simple
@@ -566,35 +635,9 @@ int E2dRunner::run(std::vector<E2dShot> shots, hipStream_t st, int graph_steps)
         SM_HIP(hipEventCreate(&t1));
         SM_HIP(hipEventRecord(t0, st));
     }
-    const E2dShot *tab = (const E2dShot *)table_;
-    const int *bshot = (const int *)((const char *)table_ + tab_bytes_);
     size_t count = 0;
-    int s = 0;
-    const int total = shots[0].nsteps;
-    while (s < total) {
-        int m = (int)n;
-        while (m > 0 && shots[m - 1].nsteps <= s) --m;
-        const int e = shots[m - 1].nsteps;
-        const int nitems = pref[m];
-        int len = e - s;
-        if (graph_steps > 1 && len >= 2 * graph_steps) {
-            hipGraphExec_t ex = nullptr;
-            rc = graph_for(m, nitems, graph_steps, st, &ex);
-            if (rc != SM_OK) return rc;
-            e2d_base_set<<<1, 64, 0, st>>>(base_, s, 0);
-            const int reps = len / graph_steps;
-            for (int r = 0; r < reps; ++r) SM_HIP(hipGraphLaunch(ex, st));
-            s += reps * graph_steps;
-            count += (size_t)reps * graph_steps;
-            len = e - s;
-        }
-        const int nb = grid_blocks(nitems);
-        for (int k = 0; k < len; ++k)
-            launch_step(nb, st, tab, bshot, nitems, nullptr, s + k);
-        s += len;
-        count += (size_t)len;
-    }
-    SM_HIP(hipGetLastError());
+    rc = launch_all(shots, pref, table_, tab_bytes_, st, graph_steps, &count);
+    if (rc != SM_OK) return rc;
     if (timing) {
         SM_HIP(hipEventRecord(t1, st));
         SM_HIP(hipEventSynchronize(t1));

@@ -93,13 +93,31 @@ private:
                      const int *base, int k);
     int ensure(size_t bytes);
     int graph_for(int nact, int nitems, int g, hipStream_t st, hipGraphExec_t *out);
+    static int plan(std::vector<E2dShot> &shots, std::vector<int> &pref,
+                    std::vector<unsigned char> &host, size_t &tab_bytes);
+    int launch_all(const std::vector<E2dShot> &shots, const std::vector<int> &pref,
+                   const void *table, size_t tab_bytes, hipStream_t st, int graph_steps,
+                   size_t *count);
     void drop_graphs();
     void *table_ = nullptr;     // shot descriptors, then the block -> shot map
     size_t table_cap_ = 0;      // bytes
     size_t tab_bytes_ = 0, last_tab_bytes_ = 0;
     bool last_staged_ = false;
     int *base_ = nullptr;
-    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs_;
+    // instantiated graphs by (active shots, work items, steps per graph),
+    // with the tick of their last use; at most MAX_GRAPHS live, the least
+    // recently used one destroyed (after its stream drains) to make room
+    struct Cached {
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    std::map<std::tuple<int, int, int>, Cached> graphs_;
+    uint64_t tick_ = 0;
+
+public:
+    static constexpr size_t MAX_GRAPHS = 16;
+    size_t graphs_live() const { return graphs_.size(); }
+    size_t graphs_built = 0;    // instantiations since creation
 };
 
 }  // namespace sm

@@ -433,6 +433,13 @@ class E2dBatch:
               "sm_e2d_batch_get_timing")
         return ms.value, n.value
 
+    def graphs(self) -> tuple[int, int]:
+        """(captured graphs held now, graphs instantiated since creation)."""
+        live, built = C.c_size_t(), C.c_size_t()
+        check(lib.sm_e2d_batch_get_graphs(self.handle, C.byref(live), C.byref(built)),
+              "sm_e2d_batch_get_graphs")
+        return live.value, built.value
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             check(lib.sm_e2d_batch_destroy(self.handle), "sm_e2d_batch_destroy")

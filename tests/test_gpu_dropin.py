@@ -20,6 +20,7 @@ from pathlib import Path
 
 import numpy as np
 import pytest
+from tolerances import FP32_REL_L2
 
 from oracle import pyoracle as O
 
@@ -113,7 +114,7 @@ def test_e_iso_program(tmp_path, storage):
         else:
             g = np.array([float(l.split()[1]) for l in got])
             w = np.array([float(l.split()[1]) for l in want])
-            assert np.linalg.norm(g - w) <= 2e-4 * np.linalg.norm(w), q
+            assert np.linalg.norm(g - w) <= FP32_REL_L2 * np.linalg.norm(w), q
     snaps = sorted((tmp_path / "ssh" / "d2").glob("d2_4_*.nc"))
     assert len(snaps) == 1
     it_snap = [it for it in range(ns) if record_due(it, d["dt"], d["snap_t0"], 0.0)]
@@ -125,7 +126,7 @@ def test_e_iso_program(tmp_path, storage):
     if storage == "f64":
         assert np.array_equal(z, want)
     else:
-        assert np.linalg.norm(z - want) <= 2e-4 * np.linalg.norm(want)
+        assert np.linalg.norm(z - want) <= FP32_REL_L2 * np.linalg.norm(want)
 
 
 # ------------------------------------------------------------------ 3D --
@@ -188,7 +189,7 @@ def test_a_iso_3d_program(tmp_path, storage):
         else:
             g = np.array([float(l.split()[1]) for l in got])
             w = np.array([float(l.split()[1]) for l in want])
-            assert np.linalg.norm(g - w) <= 2e-4 * np.linalg.norm(w), q
+            assert np.linalg.norm(g - w) <= FP32_REL_L2 * np.linalg.norm(w), q
     snaps = sorted((tmp_path / "ssh" / "d3").glob("d3_4_*.nc"))
     assert len(snaps) == 1
     it_snap = [it for it in range(ns) if record_due(it, d["dt"], d["snap_t0"], 0.0)]
@@ -207,4 +208,51 @@ def test_a_iso_3d_program(tmp_path, storage):
     if storage == "f64":
         assert np.array_equal(z, want)
     else:
-        assert np.linalg.norm(z - want) <= 2e-4 * np.linalg.norm(want) + 1e-30
+        assert np.linalg.norm(z - want) <= FP32_REL_L2 * np.linalg.norm(want) + 1e-30
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_a_iso_3d_segy_samples(tmp_path, storage):
+    """segy 1: the velocity traces go to segy/<base>.sgy instead of trc/
+    (a_iso_3d.c:454-462, 698-741).  Sample round(t/rdt) of each trace holds
+    the recorded value rounded to FP32 (fill_data_segy, io_utils.c:1770-1774)
+    at the steps the record predicate selects; the P record is not a SEG-Y
+    trace (io_utils.c:3269-3559 writes Vx, Vy, Vz, div, curl only).  The
+    reference's sample 0 is never written and its last one lands past the
+    end of the buffer: here they are 0 and dropped (documented deviation)."""
+    import struct
+    from seismod_amd.solver import nsteps as sm_nsteps, record_due
+    d = deck_3d(tmp_path)
+    with open(tmp_path / "run.par", "a") as f:
+        f.write("1 # segy, on\n12 # shotpoint, number\n")
+    run("a_iso_3d", tmp_path, storage)
+    assert not (tmp_path / "trc" / "d3" / "d3_1.trc").exists()
+    ns = sm_nsteps(0.08, d["dt"])
+    tr, _ = oracle_3d(d, ns)
+    b = (tmp_path / "segy" / "d3.sgy").read_bytes()
+    pos = 3600
+    ntr = 0
+    for q, (_, comp, t0, rdt_ms) in enumerate(d["rec"]):
+        if comp == P:
+            continue
+        rdt = rdt_ms * 1e-3
+        nsr = int(round(0.08 / rdt))
+        hdr = b[pos:pos + 240]
+        assert struct.unpack_from("<i", hdr, 0)[0] == ntr + 1            # trace sequence number
+        assert struct.unpack_from("<i", hdr, 16)[0] == 12                # shot point
+        assert struct.unpack_from("<h", hdr, 114)[0] == nsr
+        got = np.frombuffer(b, dtype="<f4", count=nsr, offset=pos + 240).astype(np.float64)
+        want = np.zeros(nsr)
+        for it in range(ns):
+            if record_due(it, d["dt"], t0, rdt):
+                s = int(round(d["dt"] * (it + 1) / rdt))
+                if s < nsr:
+                    want[s] = float(np.float32(tr[it, q]))
+        assert np.abs(want).max() > 0, q
+        if storage == "f64":
+            assert np.array_equal(got, want), q
+        else:
+            assert np.linalg.norm(got - want) <= 4e-6 * np.linalg.norm(want), q
+        pos += 240 + 4 * nsr
+        ntr += 1
+    assert ntr == 2 and pos == len(b)

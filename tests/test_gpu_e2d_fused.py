@@ -9,13 +9,14 @@ folding of dt/h and the PML coefficients differ).
 """
 import numpy as np
 import pytest
+from tolerances import FP32_REL_L2, note
 
 import decks
 from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
 
-TOL_ORACLE = 2e-4
+TOL_ORACLE = FP32_REL_L2
 TOL_NAIVE = 2e-5
 
 
@@ -83,7 +84,7 @@ def test_fused_vs_oracle(ctx, case):
     tr_o, f_o = oracle(d)
     assert np.abs(tr_o).max() > 0
     e_tr, e_f = rel_l2(tr, tr_o), rel_l2(f, f_o)
-    print(f"\n{ids(case)}: traces {e_tr:.2e} fields {e_f:.2e}")
+    note("e2d_fused_vs_oracle", case=ids(case), traces=e_tr, fields=e_f)
     assert e_tr < TOL_ORACLE and e_f < TOL_ORACLE
 
 

@@ -6,14 +6,15 @@ results are held to a relative-L2 tolerance against the same oracle run.
 """
 import numpy as np
 import pytest
+from tolerances import FP32_REL_L2, note
 
 import decks
 from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
 
-FP32_TRACE_TOL = 2e-4     # relative L2, FP32 arithmetic vs FP64 oracle
-FP32_FIELD_TOL = 2e-4
+FP32_TRACE_TOL = FP32_REL_L2   # relative L2, FP32 arithmetic vs FP64 oracle
+FP32_FIELD_TOL = FP32_REL_L2
 
 
 def rel_l2(a, b):
@@ -71,6 +72,7 @@ def test_a3d_fp32_naive_tolerance(ctx, case):
     d = decks.a3d_deck(**case)
     tr_o, f_o = run_a3d_oracle(d)
     tr_g, f_g = run_a3d_gpu(ctx, d, "f32", naive=True)
+    note("a3d_naive_fp32_vs_oracle", case=str(case), traces=rel_l2(tr_g, tr_o), fields=rel_l2(f_g, f_o))
     assert rel_l2(tr_g, tr_o) < FP32_TRACE_TOL
     assert rel_l2(f_g, f_o) < FP32_FIELD_TOL
 
@@ -124,5 +126,6 @@ def test_e2d_fp32_tolerance(ctx, case):
     d = decks.e2d_deck(**case)
     tr_o, f_o = run_e2d_oracle(d)
     tr_g, f_g = run_e2d_gpu(ctx, d, "f32")
+    note("e2d_fp32_vs_oracle", case=str(case), traces=rel_l2(tr_g, tr_o), fields=rel_l2(f_g, f_o))
     assert rel_l2(tr_g, tr_o) < FP32_TRACE_TOL
     assert rel_l2(f_g, f_o) < FP32_FIELD_TOL

@@ -14,6 +14,7 @@ left, whose relative error says little).
 """
 import numpy as np
 import pytest
+from tolerances import FP32_REL_L2
 
 from oracle import pyoracle as O
 
@@ -23,7 +24,7 @@ VZ, P = 2, 8
 
 # relative L2 vs the FP64 oracle
 TOL = {
-    "f32": dict(traces=2e-4, field=2e-4),
+    "f32": dict(traces=FP32_REL_L2, field=FP32_REL_L2),
     "f16": dict(traces=5e-3, field=1e-2),
     "bf16": dict(traces=5e-2, field=1e-1),
     "f16-sr": dict(traces=5e-3, field=1e-2),

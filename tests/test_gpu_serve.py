@@ -17,6 +17,8 @@ def test_results_alone_equal_results_in_mix(policy):
     assert kinds == {"e2d", "a3d"}
     srv = serve.Server(tick_steps=16, graph_steps=8, policy=policy)
     mix = srv.run(reqs)
+    live, built = srv.batch.graphs()
+    assert live <= min(built, 16)        # the graph cache stays bounded
     srv.close()
     alone = serve.run_sequential(reqs)
     assert [r.rid for r in mix] == [r.rid for r in alone] == sorted(r.rid for r in reqs)
@@ -27,3 +29,21 @@ def test_results_alone_equal_results_in_mix(policy):
     # requests overlapped: some request was admitted before another finished
     assert any(x.admitted < y.finished and y.admitted < x.finished
                for x in mix for y in mix if x.rid != y.rid)
+
+
+def test_request_that_cannot_run_is_rejected_alone():
+    """A 2D request whose grid the fused solver refuses (over 2 GiB of state)
+    comes back as a rejected result; the requests around it are served."""
+    from types import SimpleNamespace
+    from seismod_amd import serve
+    reqs = serve.make_requests(3, rate=0.0, seed=8, frac_3d=0.0, e2d_range=(40, 60),
+                               steps_range=(20, 40))
+    big = SimpleNamespace(nx=4800, nz=4800, npml=20, h=5.0, dt=5e-4, rec_idx=[0],
+                          cells=4840 * 4840)
+    reqs.append(serve.Request(99, "e2d", 0.0, 100, "f32", big))
+    srv = serve.Server(tick_steps=16, graph_steps=8)
+    res = srv.run(reqs)
+    srv.close()
+    by = {r.rid: r for r in res}
+    assert by[99].error and by[99].traces is None
+    assert all(not by[q.rid].error and by[q.rid].traces is not None for q in reqs[:3])
