@@ -95,9 +95,8 @@ def cpu_baseline(shot, nsteps_sample: int) -> dict:
 
 
 # kernel launched by each variant of the fused 3D path
-KERNELS = {"cell": "a3d_fused_step", "cell2": "a3d_fused_step", "cell64": "a3d_fused_step",
-           "cell64x2": "a3d_fused_step", "pair": "a3d_pair_step", "tile": "a3d_tile_step",
-           "tilesplit": "a3d_tile_step"}
+KERNELS = {"cell": "a3d_fused_step", "cell2": "a3d_fused_step", "tile": "a3d_tile_step",
+           "tilesplit": "a3d_tile_step", "tile_nt": "a3d_tile_step", "tilesplit_nt": "a3d_tile_step"}
 
 
 def load_traffic(name: str, variant: str, rows: int):

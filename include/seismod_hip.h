@@ -55,6 +55,15 @@ extern "C" {
 #define SM_TZZ 6
 #define SM_TXZ 7
 #define SM_P 8
+#define SM_DIV 11
+#define SM_CURL 12
+#define SM_VR 13
+#define SM_VT 14
+#define SM_TRR 15
+#define SM_TRT 16
+#define SM_TRZ 17
+#define SM_TTT 18
+#define SM_TTZ 19
 
 /* source types, numbering of enum sourceType (structs.h:44) */
 #define SM_SX 0
@@ -65,10 +74,16 @@ extern "C" {
 #define SM_SYZ 5
 #define SM_BULK 6
 #define SM_SF 7
+#define SM_BULK_S 8
+#define SM_FR 9
+#define SM_FT 10
+#define SM_FZ 11
+#define SM_KURKJIAN 12
 
 typedef struct sm_ctx sm_ctx;
 typedef struct sm_a3d sm_a3d;
 typedef struct sm_e2d sm_e2d;
+typedef struct sm_cyl sm_cyl;
 
 /* ------------------------------------------------------------------ */
 /* library and context                                                 */
@@ -135,7 +150,7 @@ typedef struct {
 
 #define SM_A3D_NAIVE 1        /* two plain kernels per step (no fusion) */
 #define SM_A3D_PAD64 4        /* fused kernels: device rows padded to a multiple
-                                 of 64 cells (needed by SM_A3D_CELL64*); without
+                                 of 64 cells; without
                                  the flag rows start on 128-byte lines when that
                                  adds at most 5% (FP32) / 10% (16-bit) cells,
                                  else they are padded to 8 cells */
@@ -186,24 +201,22 @@ int sm_a3d_set_launch(sm_a3d *s, int rows_per_block, int x_chunks);
 int sm_a3d_get_launch(const sm_a3d *s, int *rows_per_block, int *x_chunks);
 /* Fused kernel variant (no counterpart in the reference): SM_A3D_CELL, one
  * cell per lane in z tiles of 63 owned cells, loads one plane ahead;
- * SM_A3D_CELL2, the same with loads two planes ahead; SM_A3D_PAIR, two
- * z-adjacent cells per lane moved as one access (needs an even padded z
- * extent); SM_A3D_CELL64 / SM_A3D_CELL64X2, z tiles of 64 cells aligned to
- * the device rows (whole cache lines per access), loads one / two planes
- * ahead; SM_A3D_TILE, z tiles of 64 cells aligned to the device rows whose
- * fields are staged plane by plane through LDS in 16-byte chunks (every
- * row read once per plane); SM_A3D_TILE_SPLIT, the same with the CPML
- * work of each direction compiled in only for tiles and ranges of planes
- * that touch that direction's absorbing layer.  The variants compute the
- * same values bit for bit; setting one resets the launch shape to its
- * default. */
+ * SM_A3D_CELL2, the same with loads two planes ahead; SM_A3D_TILE, z tiles
+ * of 64 cells aligned to the device rows whose fields are staged plane by
+ * plane through LDS in 16-byte chunks (every row read once per plane) and
+ * whose new state leaves as 16-byte stores; SM_A3D_TILE_SPLIT, the same
+ * with the CPML work of each direction compiled in only for tiles and
+ * ranges of planes that touch that direction's absorbing layer;
+ * SM_A3D_TILE_NT / SM_A3D_TILE_SPLIT_NT, the same two with non-temporal
+ * state stores (rows_per_block 4 or 8 only).  Values 1, 3 and 4 are not
+ * variants (SM_ERR_ARG).  The variants compute the same values bit for
+ * bit; setting one resets the launch shape to its default. */
 #define SM_A3D_CELL 0
-#define SM_A3D_PAIR 1
 #define SM_A3D_CELL2 2
-#define SM_A3D_CELL64 3
-#define SM_A3D_CELL64X2 4
 #define SM_A3D_TILE 5
 #define SM_A3D_TILE_SPLIT 6
+#define SM_A3D_TILE_NT 7
+#define SM_A3D_TILE_SPLIT_NT 8
 int sm_a3d_set_variant(sm_a3d *s, int variant);
 int sm_a3d_get_variant(const sm_a3d *s, int *variant);
 /* Per-launch timing: when on, every step kernel launch is bracketed by HIP
@@ -312,6 +325,61 @@ int sm_e2d_batch_get_timing(const sm_e2d_batch *b, double *ms_total, size_t *lau
  * one is destroyed to make room) and has instantiated since creation (no
  * counterpart in the reference). */
 int sm_e2d_batch_get_graphs(const sm_e2d_batch *b, size_t *live, size_t *built);
+
+/* ------------------------------------------------------------------ */
+/* 2.5D elastic solver in cylindrical coordinates (e_cyl_iso) --        */
+/* replaces e_cyl_iso.c:342-458, 526-535, 601-866, propagate.c:2706-4119 */
+/* (update_*_cyl, update_*_cyl_n, compute_div_cyl, compute_curl_cyl),   */
+/* pml.c:2107-2164 (compute_cpml_cyl), src.c:186-238                    */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    size_t nx, nz;            /* interior nodes along r and z (struct grid) */
+    size_t np;                /* CPML layers: outer r side and both z ends
+                                 (0: abs off, e_cyl_iso runs without layers) */
+    double dr, dz;            /* m (g.dx, g.dz) */
+    double dt;                /* s */
+    int mode;                 /* azimuthal mode n (azimuthal_mode, >= 0) */
+    double vmax, pml_order, Rc, kappa_max;   /* struct abs_params */
+    int alpha_max;            /* abs_params.alpha_max: 0 -> alpha = 0 */
+    double f0;                /* frequency of source 0 (alpha_max = pi*f0) */
+    int storage;              /* SM_STORE_F32 or SM_STORE_F64 */
+    int flags;                /* 0 */
+} sm_cyl_params;
+
+/* Build a solver from the padded model as read_model_e_cyl leaves it
+ * (io_utils.c:949-1027): vp, vs, rho hold nx2*nz2 values, nx2 = nx+np,
+ * nz2 = nz+2np, index i*nz2+j with row i at r = i*dr.  Computes the
+ * staggered material (e_cyl_iso.c:342-415), refuses a non-finite value
+ * (SM_ERR_ARG, 417-452) and a time step beyond 0.5(dr+dz)/(sqrt(2) Vp max)
+ * (SM_ERR_CFL, 454-458), builds the CPML tables and zeroes the fields. */
+int sm_cyl_create(sm_ctx *ctx, const sm_cyl_params *p, const double *vp, const double *vs,
+                  const double *rho, sm_cyl **out);
+int sm_cyl_destroy(sm_cyl *s);
+/* Sources on padded indices.  SM_FR, SM_FT, SM_FZ and SM_BULK act as
+ * forces after the velocity update (add_force_src_cyl, src.c:186-200:
+ * v += dt * (1/rho) * fct); a SM_KURKJIAN source must be entry 0 and is
+ * added to tau_rr and tau_zz after the stress update (src.c:231-238);
+ * other types inject nothing.  fct concatenated as for sm_compute_src_fct
+ * (or from sm_kurkjian_fct). */
+int sm_cyl_set_sources(sm_cyl *s, size_t nsrc, const int *type, const size_t *idx,
+                       const size_t *length, const double *fct);
+/* Kurkjian source function (compute_kurkjian, src.c:202-229): lambda is the
+ * Lame constant at the source cell, r its radial coordinate, A the peak
+ * volume change. */
+int sm_kurkjian_fct(double lambda, double dr, double dz, double r, int mode, double A, double f,
+                    double dt, size_t length, double *fct);
+/* receivers on padded indices; comp SM_VR, SM_VT, SM_VZ, SM_TRR, SM_TRT,
+ * SM_TRZ, SM_TTT, SM_TTZ, SM_TZZ, SM_DIV, SM_CURL (e_cyl_iso.c:688-716) */
+int sm_cyl_set_receivers(sm_cyl *s, size_t nrec, const size_t *idx, const int *comp,
+                         size_t max_steps);
+int sm_cyl_run(sm_cyl *s, size_t nsteps);
+int sm_cyl_current_step(const sm_cyl *s, size_t *it);
+int sm_cyl_get_traces(sm_cyl *s, size_t it0, size_t count, double *out);
+/* padded nx2*nz2 field of comp (SM_VR ... SM_TTZ, SM_VZ, SM_TZZ) */
+int sm_cyl_get_field(sm_cyl *s, int comp, double *out);
+int sm_cyl_reset(sm_cyl *s);
+int sm_cyl_device_bytes(const sm_cyl *s, size_t *bytes);
+size_t sm_cyl_bytes_needed(const sm_cyl_params *p, size_t nrec, size_t max_steps);
 
 #ifdef __cplusplus
 }

@@ -10,7 +10,8 @@ Tuning runs steps and then resets the solver (fields, memory variables,
 traces and step counter back to zero), so call it right after set-up,
 before the solver's real run.
 
-  3D (Acoustic3D):  kernel variant (cell, cell2, tile, tilesplit) x rows_per_block x x_chunks
+  3D (Acoustic3D):  kernel variant (cell, cell2, tile, tilesplit, tile_nt, tilesplit_nt)
+                    x rows_per_block x x_chunks
   2D (Elastic2D):   rows_per_chunk in {1, 2, 3, 4, 6, 8, 12, 16}
 """
 from __future__ import annotations
@@ -64,7 +65,7 @@ def _time(sim, ctx, steps: int) -> float:
 
 
 def tune_a3d(sim, ctx, steps: int = 8, cache_file=None, candidates=None,
-             variants=("cell", "cell2", "tile", "tilesplit")) -> tuple:
+             variants=("cell", "cell2", "tile", "tilesplit", "tile_nt", "tilesplit_nt")) -> tuple:
     """Pick the kernel variant and (rows_per_block, x_chunks) for an
     Acoustic3D solver; applies them.  Returns (variant, rows_per_block,
     x_chunks)."""

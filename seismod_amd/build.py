@@ -2,7 +2,7 @@
 
 Everything is built in-tree so the artefacts travel with the source tree:
   seismod_amd/lib/libseismod_hip.so   HIP kernels + C ABI (hipcc, gfx950)
-  seismod_amd/bin/e_iso, a_iso_3d      drop-in programs (gcc, C99)
+  seismod_amd/bin/e_iso, a_iso_3d, e_cyl_iso   drop-in programs (gcc, C99)
   oracle/_build/liboracle.so           FP64 CPU restatement (test checker)
 
 Run ``python -m seismod_amd.build`` or call :func:`build_all`.
@@ -69,15 +69,16 @@ def build_lib() -> Path:
 
 
 def build_host() -> list[Path]:
-    """Drop-in e_iso and a_iso_3d programs linked against the library, and
+    """Drop-in e_iso, a_iso_3d and e_cyl_iso programs linked against the library, and
     the host I/O layer alone as libsm_hostio.so (tested on CPU)."""
     BINDIR.mkdir(exist_ok=True)
     LIBDIR.mkdir(exist_ok=True)
     _run(["gcc", *C_FLAGS, "-fopenmp", "-fPIC", "-shared", f"-I{HOST}", str(HOST / "sm_io.c"),
           "-lm", "-o", str(LIBDIR / "libsm_hostio.so")])
-    common = sorted(p for p in HOST.glob("*.c") if p.stem not in ("e_iso", "a_iso_3d"))
+    programs = ("e_iso", "a_iso_3d", "e_cyl_iso")
+    common = sorted(p for p in HOST.glob("*.c") if p.stem not in programs)
     outs = []
-    for prog in ("e_iso", "a_iso_3d"):
+    for prog in programs:
         src = HOST / f"{prog}.c"
         if not src.exists():
             continue

@@ -202,8 +202,8 @@ bool uses_fused(const sm_a3d_params &p)
 // 64-cell wave access then covers whole lines) when that adds at most 5%
 // (FP32) or 10% (16-bit) cells, otherwise a multiple of 8 cells (16 bytes
 // of 16-bit storage; the TILE variants copy rows in 16-byte chunks).
-// SM_A3D_PAD64 forces a multiple of 64 cells (the CELL64 variants need
-// it).  The two-kernel paths keep the reference layout.
+// SM_A3D_PAD64 forces a multiple of 64 cells.  The two-kernel paths keep
+// the reference layout.
 size_t row_pitch(size_t nz2, const sm_a3d_params &p)
 {
     if (!uses_fused(p)) return nz2;
@@ -373,27 +373,24 @@ unsigned step_salt(size_t it)
     return (unsigned)(z ^ (z >> 31));
 }
 
-// kernel variant properties: planes prefetched, 64-cell aligned z tiles
-int variant_pf(int v) { return v == SM_A3D_CELL2 || v == SM_A3D_CELL64X2 ? 2 : 1; }
-bool variant_a64(int v) { return v == SM_A3D_CELL64 || v == SM_A3D_CELL64X2; }
-// variants with z tiles of 64 cells starting at multiples of 64
-bool variant_tile(int v) { return v == SM_A3D_TILE || v == SM_A3D_TILE_SPLIT; }
-bool variant_z64(int v) { return variant_a64(v) || variant_tile(v); }
+// kernel variant properties: planes prefetched (cell kernels); LDS tile
+// kernels (z tiles of 64 cells starting at multiples of 64), with the CPML
+// work split by tile, with non-temporal state stores
+int variant_pf(int v) { return v == SM_A3D_CELL2 ? 2 : 1; }
+bool variant_tile(int v) { return v >= SM_A3D_TILE && v <= SM_A3D_TILE_SPLIT_NT; }
+bool variant_split(int v) { return v == SM_A3D_TILE_SPLIT || v == SM_A3D_TILE_SPLIT_NT; }
+bool variant_nt(int v) { return v == SM_A3D_TILE_NT || v == SM_A3D_TILE_SPLIT_NT; }
+bool variant_known(int v) { return v == SM_A3D_CELL || v == SM_A3D_CELL2 || variant_tile(v); }
 
-// z tiles: 62 owned cells from even cells (pair), 64 aligned (cell64), 63
+// z tiles: 64 aligned cells (tile kernels), else 63 owned cells
 int z_tiles(const sm_a3d *s)
 {
-    if (s->variant == SM_A3D_PAIR) return (s->nz2 + 60) / 62;
-    if (variant_z64(s->variant)) return (s->nz2 + 63) / 64;
+    if (variant_tile(s->variant)) return (s->nz2 + 63) / 64;
     return (s->nz2 + 62) / 63;
 }
 
-// y tiles: rows_per_block rows per block (two rows per wave for pair)
-int y_tiles(const sm_a3d *s)
-{
-    const int rows = s->variant == SM_A3D_PAIR ? 2 * s->tj : s->tj;
-    return (s->ny2 + rows - 1) / rows;
-}
+// y tiles: rows_per_block rows per block
+int y_tiles(const sm_a3d *s) { return (s->ny2 + s->tj - 1) / s->tj; }
 
 int64_t rcv_row(const sm_a3d *s, size_t it)
 {
@@ -539,10 +536,9 @@ int run_fused(sm_a3d *s, size_t nsteps)
         a.gather_row = (n > 0 && it > 0) ? rcv_row(s, it - 1) : -1;
         a.gather_blocks = a.gather_row >= 0 ? (int)((s->nrec + per_gather - 1) / per_gather) : 0;
         if ((rc = tm.mark(st, 2 * n)) != SM_OK) return rc;
-        if ((rc = s->variant == SM_A3D_PAIR   ? pair_launch<S>(a, s->tj, st)
-                  : variant_tile(s->variant) ? tile_launch<S>(a, s->tj, s->variant == SM_A3D_TILE_SPLIT, st)
-                                              : fused_launch<S>(a, s->tj, variant_pf(s->variant),
-                                                                variant_a64(s->variant), st)) != SM_OK)
+        if ((rc = variant_tile(s->variant)
+                      ? tile_launch<S>(a, s->tj, variant_split(s->variant), variant_nt(s->variant), st)
+                      : fused_launch<S>(a, s->tj, variant_pf(s->variant), st)) != SM_OK)
             return fail(rc, "fused launch failed (tj=%d): %s", s->tj,
                         hipGetErrorString(hipGetLastError()));
         if ((rc = tm.mark(st, 2 * n + 1)) != SM_OK) return rc;
@@ -630,25 +626,21 @@ size_t elt_size(int storage)
 int resident_blocks(sm_a3d *s, int tj)
 {
     int per_cu = 0, rc = SM_OK;
-    const bool pair = s->variant == SM_A3D_PAIR, tile = variant_tile(s->variant);
-    const bool split = s->variant == SM_A3D_TILE_SPLIT;
+    const bool tile = variant_tile(s->variant);
+    const bool split = variant_split(s->variant), nt = variant_nt(s->variant), sr = s->sr != 0;
     const int pf = variant_pf(s->variant);
-    const bool a64 = variant_a64(s->variant);
     switch (s->p.storage) {
     case SM_STORE_F32:
-        rc = pair   ? pair_max_blocks_per_cu<float>(tj, &per_cu)
-             : tile ? tile_max_blocks_per_cu<float>(tj, split, s->sr != 0, &per_cu)
-                    : fused_max_blocks_per_cu<float>(tj, pf, a64, &per_cu);
+        rc = tile ? tile_max_blocks_per_cu<float>(tj, split, nt, sr, &per_cu)
+                  : fused_max_blocks_per_cu<float>(tj, pf, &per_cu);
         break;
     case SM_STORE_BF16:
-        rc = pair   ? pair_max_blocks_per_cu<__hip_bfloat16>(tj, &per_cu)
-             : tile ? tile_max_blocks_per_cu<__hip_bfloat16>(tj, split, s->sr != 0, &per_cu)
-                    : fused_max_blocks_per_cu<__hip_bfloat16>(tj, pf, a64, &per_cu);
+        rc = tile ? tile_max_blocks_per_cu<__hip_bfloat16>(tj, split, nt, sr, &per_cu)
+                  : fused_max_blocks_per_cu<__hip_bfloat16>(tj, pf, &per_cu);
         break;
     default:
-        rc = pair   ? pair_max_blocks_per_cu<__half>(tj, &per_cu)
-             : tile ? tile_max_blocks_per_cu<__half>(tj, split, s->sr != 0, &per_cu)
-                    : fused_max_blocks_per_cu<__half>(tj, pf, a64, &per_cu);
+        rc = tile ? tile_max_blocks_per_cu<__half>(tj, split, nt, sr, &per_cu)
+                  : fused_max_blocks_per_cu<__half>(tj, pf, &per_cu);
         break;
     }
     if (rc != SM_OK || per_cu < 1) per_cu = 1;
@@ -964,6 +956,8 @@ int sm_a3d_set_launch(sm_a3d *s, int rows_per_block, int x_chunks)
     SM_CHECK(s->fused, SM_ERR_STATE, "sm_a3d_set_launch: only the fused kernel has a launch shape");
     SM_CHECK(rows_per_block == 1 || rows_per_block == 2 || rows_per_block == 4 ||
                  rows_per_block == 8, SM_ERR_ARG, "rows_per_block must be 1, 2, 4 or 8");
+    SM_CHECK(!variant_nt(s->variant) || rows_per_block >= 4, SM_ERR_ARG,
+             "the non-temporal tile variants take rows_per_block 4 or 8");
     SM_CHECK(x_chunks >= 1 && x_chunks <= s->nx2, SM_ERR_ARG, "x_chunks must be in [1, nx2]");
     // every chunk must own at least one plane
     const int len = (s->nx2 + x_chunks - 1) / x_chunks;
@@ -978,15 +972,9 @@ int sm_a3d_set_variant(sm_a3d *s, int variant)
 {
     SM_CHECK(s, SM_ERR_ARG, "sm_a3d_set_variant: NULL solver");
     SM_CHECK(s->fused, SM_ERR_STATE, "sm_a3d_set_variant: only the fused kernel has variants");
-    SM_CHECK(variant >= SM_A3D_CELL && variant <= SM_A3D_TILE_SPLIT, SM_ERR_ARG,
-             "sm_a3d_set_variant: unknown variant %d", variant);
-    SM_CHECK(!variant_a64(variant) || s->pz % 64 == 0, SM_ERR_ARG,
-             "sm_a3d_set_variant: the 64-cell-tile variants need rows padded to 64 cells "
-             "(SM_A3D_PAD64 at creation)");
+    SM_CHECK(variant_known(variant), SM_ERR_ARG, "sm_a3d_set_variant: unknown variant %d", variant);
     SM_CHECK(!variant_tile(variant) || s->pz % 8 == 0, SM_ERR_ARG,
              "sm_a3d_set_variant: the tile kernel needs a row pitch of a multiple of 8 cells");
-    SM_CHECK(variant != SM_A3D_PAIR || (s->nz2 % 2 == 0 && s->nz2 >= 4), SM_ERR_ARG,
-             "sm_a3d_set_variant: the pair kernel needs an even padded z extent (nz2 = %d)", s->nz2);
     if (variant == s->variant) return SM_OK;
     s->variant = variant;
     default_launch(s);

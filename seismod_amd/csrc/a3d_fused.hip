@@ -44,17 +44,11 @@ struct Col {
     float mPx, mPy, mPz;    // velocity memory variables (slab cells only)
     float mPym;             // wave 0 only: at the halo row
     float mvx, mvy, mvz;    // pressure memory variables (slab cells only)
-    // 64-cell tiles: lane 63 reads P, rho at k0+64; lane 0 reads P, rho,
-    // old vz (and the memory variable) at k0-1
-    float Pkp, rkp, Pkm, rkm, vzkm, mPzm;
 };
 
 // constants of one lane for one work item
 struct Lane {
     unsigned o_jk, o_jp, o_kp, o_jm;   // in-plane offsets of the cell and its neighbours
-    unsigned o_kpp, o_km;              // 64-cell tiles: offsets of k0+64 (lane 63), k0-1 (lane 0)
-    int szh;                           // 64-cell tiles: z slab of k0-1 (velocity set, uniform)
-    unsigned mz_h;                     // and its z-slab offset within a plane
     int j, kk, kc;                     // j is wave-uniform
     bool own;                          // this lane writes its cell
     int syv, syp, syh;                 // y slab coordinates (wave-uniform)
@@ -65,7 +59,7 @@ struct Lane {
     float zbh, zah, zikh, zb, za, zik;
 };
 
-template <typename S, bool A64>
+template <typename S>
 __device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, int w, int i,
                                          unsigned plane, int sxv, int sxp, Col &c)
 {
@@ -80,23 +74,8 @@ __device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, i
     c.rn = ld_at(a.mat, on + L.o_jk);
     c.Pj = Store<S>::ld(a.in, o + L.o_jp, a.sP);
     c.rj = ld_at(a.mat, o + L.o_jp);
-    if constexpr (A64) {
-        const int lane = threadIdx.x;
-        if (lane == 63) {
-            c.Pkp = Store<S>::ld(a.in, o + L.o_kpp, a.sP);
-            c.rkp = ld_at(a.mat, o + L.o_kpp);
-        }
-        if (lane == 0) {
-            c.Pkm = Store<S>::ld(a.in, o + L.o_km, a.sP);
-            c.rkm = ld_at(a.mat, o + L.o_km);
-            c.vzkm = Store<S>::ld(a.in, F_VZ * a.nnn + o + L.o_km, a.sV);
-            if (L.szh >= 0)
-                c.mPzm = Store<S>::ld(a.mem_in, a.mz + (unsigned)i * a.ny2 * np2 + L.mz_h, a.sMv);
-        }
-    } else {
-        c.Pk = Store<S>::ld(a.in, o + L.o_kp, a.sP);
-        c.rk = ld_at(a.mat, o + L.o_kp);
-    }
+    c.Pk = Store<S>::ld(a.in, o + L.o_kp, a.sP);
+    c.rk = ld_at(a.mat, o + L.o_kp);
     if (w == 0) {
         c.Pjm = Store<S>::ld(a.in, o + L.o_jm, a.sP);
         c.rjm = ld_at(a.mat, o + L.o_jm);
@@ -115,15 +94,15 @@ __device__ __forceinline__ void load_col(const FusedArgs<S> &a, const Lane &L, i
 }
 
 // one plane: prefetch plane i+PF into fill, update plane i from cur
-template <typename S, int TJ, int PF, bool A64>
+template <typename S, int TJ, int PF>
 __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L, int w, int i,
                                            int i0, int i1, unsigned plane, Col &cur, Col &fill,
                                            float &P_c, float &r_c, float &vx_prev,
-                                           float (*s_vy)[TJ][64], const Halo &hz)
+                                           float (*s_vy)[TJ][64])
 {
     if (i + PF < i1) {
         const XPlane xn = xplane(a.xpl, i + PF);
-        load_col<S, A64>(a, L, w, i + PF, plane, xn.sxv, xn.sxp, fill);
+        load_col<S>(a, L, w, i + PF, plane, xn.sxv, xn.sxp, fill);
     }
     const XPlane x = xplane(a.xpl, i);
     const unsigned o = (unsigned)i * plane + L.o_jk;
@@ -144,17 +123,7 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
     float my = cur.mPy, mz = cur.mPz;
     const float vyn = vel_update(cur.vy, P_c, cur.Pj, r_c, cur.rj, a.idy, a.two_dt, L.syv >= 0,
                                  L.ybh, L.yah, L.yikh, my);
-    float Pk, rk;
-    if constexpr (A64) {
-        // k+1 from the next lane; lane 63 loaded it
-        Pk = __shfl_down(P_c, 1);
-        rk = __shfl_down(r_c, 1);
-        if (threadIdx.x == 63) { Pk = cur.Pkp; rk = cur.rkp; }
-    } else {
-        Pk = cur.Pk;
-        rk = cur.rk;
-    }
-    const float vzn = vel_update(cur.vz, P_c, Pk, r_c, rk, a.idz, a.two_dt, L.szv >= 0,
+    const float vzn = vel_update(cur.vz, P_c, cur.Pk, r_c, cur.rk, a.idz, a.two_dt, L.szv >= 0,
                                  L.zbh, L.zah, L.zikh, mz);
     if (L.own) {
         Store<S>::st(a.out, F_VX * a.nnn + o, vxn, a.isV, a.salt, a.sr);
@@ -180,15 +149,7 @@ __device__ __forceinline__ void plane_step(const FusedArgs<S> &a, const Lane &L,
     s_vy[i & 1][w][threadIdx.x] = vyn;
     __syncthreads();
     const float vy_m = w > 0 ? s_vy[i & 1][w - 1][threadIdx.x] : vy_halo;
-    float vz_m = __shfl_up(vzn, 1);   // 63-cell tiles: lane 0 is the left halo lane
-    if constexpr (A64) {
-        // 64-cell tiles: lane 0 recomputes vz at k0-1 (owned by the tile to
-        // the left; not stored) from the values it loaded
-        float mh = cur.mPzm;
-        const float vz_h = vel_update(cur.vzkm, cur.Pkm, P_c, cur.rkm, r_c, a.idz, a.two_dt,
-                                      L.szh >= 0, hz.bh, hz.ah, hz.ikh, mh);
-        if (threadIdx.x == 0) vz_m = vz_h;
-    }
+    const float vz_m = __shfl_up(vzn, 1);   // lane 0 is the left halo lane
 
     // pressure at (i, j, k), a_iso_3d.c:659-683
     if (L.own) {
@@ -237,12 +198,9 @@ __device__ __forceinline__ void gather_one(const FusedArgs<S> &a, int r)
     a.trace[(size_t)a.gather_row * a.nrec + r] = Store<S>::ld(a.in, q * a.nnn + (unsigned)a.ridx[r], sc);
 }
 
-// PF: planes prefetched ahead of the one being updated (1 or 2).
-// A64: z tiles of 64 owned cells starting at multiples of 64 (every access
-// of a wave covers whole cache lines of the padded rows); the neighbours
-// k0-1 and k0+64 come from single-lane loads.  Otherwise tiles of 63 owned
-// cells with the left neighbour in lane 0.
-template <typename S, int TJ, int PF, bool A64>
+// PF: planes prefetched ahead of the one being updated (1 or 2).  Tiles of
+// 63 owned cells along z; lane 0 holds the left neighbour.
+template <typename S, int TJ, int PF>
 __global__ void __launch_bounds__(64 * TJ)
 a3d_fused_step(FusedArgs<S> a)
 {
@@ -271,16 +229,16 @@ a3d_fused_step(FusedArgs<S> a)
         const int tile = logical - chunk * a.ntiles;
         const int tj = tile / a.ntk;
         const int tk = tile - tj * a.ntk;
-        // A64: 64 owned cells per wave along z from k0 = 64 tk.  Otherwise
-        // 63 owned cells; lane 0 sits on the last cell of the tile to the
-        // left and only supplies vz at k0-1.  Cells past the grid edge read
-        // the last cell (its neighbour differences vanish) and are not owned.
+        // 63 owned cells per wave along z; lane 0 sits on the last cell of
+        // the tile to the left and only supplies vz at k0-1.  Cells past the
+        // grid edge read the last cell (its neighbour differences vanish)
+        // and are not owned.
         Lane L;
-        const int k0 = A64 ? tk * 64 : tk * 63;
-        L.kk = A64 ? k0 + lane : k0 - 1 + lane;
+        const int k0 = tk * 63;
+        L.kk = k0 - 1 + lane;
         L.j = tj * TJ + w;
         const bool jv = L.j < a.ny2, kv = L.kk >= 0 && L.kk < a.nz2;
-        L.own = jv && kv && (A64 || lane >= 1);
+        L.own = jv && kv && lane >= 1;
         const int jc = min(L.j, a.ny2 - 1);
         L.kc = min(max(L.kk, 0), a.nz2 - 1);
         const int jp = min(jc + 1, a.ny2 - 1), kp = min(L.kc + 1, a.nz2 - 1), jm = max(jc - 1, 0);
@@ -288,8 +246,6 @@ a3d_fused_step(FusedArgs<S> a)
         L.o_jp = (unsigned)jp * a.pz + L.kc;
         L.o_kp = (unsigned)jc * a.pz + kp;
         L.o_jm = (unsigned)jm * a.pz + L.kc;
-        L.o_kpp = (unsigned)jc * a.pz + min(k0 + 64, a.nz2 - 1);
-        L.o_km = (unsigned)jc * a.pz + max(k0 - 1, 0);
         L.syv = jv && L.j < a.ny2 - 1 ? slab_vel(L.j, a.ny2, a.np) : -1;
         L.syp = jv ? slab_pre(L.j, a.ny2, a.np) : -1;
         L.syh = jv && L.j >= 1 ? slab_vel(L.j - 1, a.ny2, a.np) : -1;
@@ -297,11 +253,7 @@ a3d_fused_step(FusedArgs<S> a)
         L.szp = kv ? slab_pre(L.kk, a.nz2, a.np) : -1;
         L.mz_v = (unsigned)jc * np2 + (L.szv < 0 ? 0 : L.szv);
         L.mz_p = (unsigned)jc * np2 + (L.szp < 0 ? 0 : L.szp);
-        // A64: the cell k0-1 whose new vz lane 0 recomputes (uniform)
-        const int hk = k0 - 1;
-        L.szh = A64 && hk >= 0 && hk < a.nz2 - 1 ? slab_vel(hk, a.nz2, a.np) : -1;
-        L.mz_h = (unsigned)jc * np2 + (L.szh < 0 ? 0 : L.szh);
-        Halo hz{0.f, 1.f, 1.f};
+
         {
             // y coefficients of the row (wave-uniform) and of the halo row,
             // z coefficients of the lane's column; tables a, b, 1/k, ah, bh, 1/kh
@@ -315,10 +267,6 @@ a3d_fused_step(FusedArgs<S> a)
             L.zbh = ld_at(tz, 4 * nz2 + kn); L.zah = ld_at(tz, 3 * nz2 + kn);
             L.zikh = ld_at(tz, 5 * nz2 + kn);
             L.zb = ld_at(tz, nz2 + L.kc); L.za = ld_at(tz, L.kc); L.zik = ld_at(tz, 2 * nz2 + L.kc);
-            if (A64) {
-                const int khn = max(min(hk, nz2 - 2), 0);
-                hz.bh = tz[4 * nz2 + khn]; hz.ah = tz[3 * nz2 + khn]; hz.ikh = tz[5 * nz2 + khn];
-            }
         }
         const int i0 = chunk * a.chunk_len;
         const int i1 = min(a.nx2, i0 + a.chunk_len);
@@ -331,31 +279,31 @@ a3d_fused_step(FusedArgs<S> a)
         Col A, B;
         A.Pjm = A.rjm = A.vyjm = A.mPym = 0.f;
         A.mPx = A.mPy = A.mPz = A.mvx = A.mvy = A.mvz = 0.f;
-        A.Pk = A.rk = A.Pkp = A.rkp = A.Pkm = A.rkm = A.vzkm = A.mPzm = 0.f;
+        A.Pk = A.rk = 0.f;
         B = A;
         {
             const XPlane xs = xplane(a.xpl, ist);
-            load_col<S, A64>(a, L, w, ist, plane, xs.sxv, xs.sxp, A);
+            load_col<S>(a, L, w, ist, plane, xs.sxv, xs.sxp, A);
         }
         if constexpr (PF == 1) {
             for (int i = ist; i < i1; i += 2) {
-                plane_step<S, TJ, 1, A64>(a, L, w, i, i0, i1, plane, A, B, P_c, r_c, vx_prev, s_vy, hz);
+                plane_step<S, TJ, 1>(a, L, w, i, i0, i1, plane, A, B, P_c, r_c, vx_prev, s_vy);
                 if (i + 1 >= i1) break;
-                plane_step<S, TJ, 1, A64>(a, L, w, i + 1, i0, i1, plane, B, A, P_c, r_c, vx_prev, s_vy, hz);
+                plane_step<S, TJ, 1>(a, L, w, i + 1, i0, i1, plane, B, A, P_c, r_c, vx_prev, s_vy);
             }
         } else {
             // three register sets in rotation: plane i in use, i+1 and i+2 in flight
             Col C = B;   // B still holds the zeroed template (its loads follow)
             if (ist + 1 < i1) {
                 const XPlane xs = xplane(a.xpl, ist + 1);
-                load_col<S, A64>(a, L, w, ist + 1, plane, xs.sxv, xs.sxp, B);
+                load_col<S>(a, L, w, ist + 1, plane, xs.sxv, xs.sxp, B);
             }
             for (int i = ist; i < i1; i += 3) {
-                plane_step<S, TJ, 2, A64>(a, L, w, i, i0, i1, plane, A, C, P_c, r_c, vx_prev, s_vy, hz);
+                plane_step<S, TJ, 2>(a, L, w, i, i0, i1, plane, A, C, P_c, r_c, vx_prev, s_vy);
                 if (i + 1 >= i1) break;
-                plane_step<S, TJ, 2, A64>(a, L, w, i + 1, i0, i1, plane, B, A, P_c, r_c, vx_prev, s_vy, hz);
+                plane_step<S, TJ, 2>(a, L, w, i + 1, i0, i1, plane, B, A, P_c, r_c, vx_prev, s_vy);
                 if (i + 2 >= i1) break;
-                plane_step<S, TJ, 2, A64>(a, L, w, i + 2, i0, i1, plane, C, B, P_c, r_c, vx_prev, s_vy, hz);
+                plane_step<S, TJ, 2>(a, L, w, i + 2, i0, i1, plane, C, B, P_c, r_c, vx_prev, s_vy);
             }
         }
     }
@@ -370,26 +318,22 @@ a3d_gather(FusedArgs<S> a)
     gather_one(a, r);
 }
 
-template <typename S, int TJ, int PF, bool A64>
+template <typename S, int TJ, int PF>
 static const void *kernel_ptr()
 {
-    return reinterpret_cast<const void *>(&a3d_fused_step<S, TJ, PF, A64>);
+    return reinterpret_cast<const void *>(&a3d_fused_step<S, TJ, PF>);
 }
 
-// the instantiated (TJ, PF, A64) combinations, as one switch key
-#define SM_FUSED_CASES(X)                                                      \
-    X(1, 1, false) X(2, 1, false) X(4, 1, false) X(8, 1, false)                \
-    X(1, 2, false) X(2, 2, false) X(4, 2, false) X(8, 2, false)                \
-    X(1, 1, true) X(2, 1, true) X(4, 1, true) X(8, 1, true)                    \
-    X(1, 2, true) X(2, 2, true) X(4, 2, true) X(8, 2, true)
-#define SM_FUSED_KEY(tj, pf, a64) ((tj) * 100 + (pf) * 10 + ((a64) ? 1 : 0))
+// the instantiated (TJ, PF) combinations, as one switch key
+#define SM_FUSED_CASES(X) X(1, 1) X(2, 1) X(4, 1) X(8, 1) X(1, 2) X(2, 2) X(4, 2) X(8, 2)
+#define SM_FUSED_KEY(tj, pf) ((tj) * 10 + (pf))
 
 template <typename S>
-int fused_max_blocks_per_cu(int tj, int pf, bool a64, int *blocks)
+int fused_max_blocks_per_cu(int tj, int pf, int *blocks)
 {
     const void *f = nullptr;
-    switch (SM_FUSED_KEY(tj, pf, a64)) {
-#define SM_X(T, P, A) case SM_FUSED_KEY(T, P, A): f = kernel_ptr<S, T, P, A>(); break;
+    switch (SM_FUSED_KEY(tj, pf)) {
+#define SM_X(T, P) case SM_FUSED_KEY(T, P): f = kernel_ptr<S, T, P>(); break;
         SM_FUSED_CASES(SM_X)
 #undef SM_X
     default: return SM_ERR_ARG;
@@ -402,13 +346,13 @@ int fused_max_blocks_per_cu(int tj, int pf, bool a64, int *blocks)
 }
 
 template <typename S>
-int fused_launch(const FusedArgs<S> &a, int tj, int pf, bool a64, hipStream_t st)
+int fused_launch(const FusedArgs<S> &a, int tj, int pf, hipStream_t st)
 {
     const int blocks = a.nblocks + a.gather_blocks;
-    switch (SM_FUSED_KEY(tj, pf, a64)) {
-#define SM_X(T, P, A)                                                                  \
-    case SM_FUSED_KEY(T, P, A):                                                        \
-        a3d_fused_step<S, T, P, A><<<blocks, dim3(64, T), 0, st>>>(a);                 \
+    switch (SM_FUSED_KEY(tj, pf)) {
+#define SM_X(T, P)                                                                     \
+    case SM_FUSED_KEY(T, P):                                                           \
+        a3d_fused_step<S, T, P><<<blocks, dim3(64, T), 0, st>>>(a);                    \
         break;
         SM_FUSED_CASES(SM_X)
 #undef SM_X
@@ -425,15 +369,14 @@ int gather_launch(const FusedArgs<S> &a, hipStream_t st)
     return hipGetLastError() == hipSuccess ? SM_OK : SM_ERR_HIP;
 }
 
-template int fused_launch<float>(const FusedArgs<float> &, int, int, bool, hipStream_t);
-template int fused_launch<__hip_bfloat16>(const FusedArgs<__hip_bfloat16> &, int, int, bool,
-                                          hipStream_t);
-template int fused_launch<__half>(const FusedArgs<__half> &, int, int, bool, hipStream_t);
+template int fused_launch<float>(const FusedArgs<float> &, int, int, hipStream_t);
+template int fused_launch<__hip_bfloat16>(const FusedArgs<__hip_bfloat16> &, int, int, hipStream_t);
+template int fused_launch<__half>(const FusedArgs<__half> &, int, int, hipStream_t);
 template int gather_launch<float>(const FusedArgs<float> &, hipStream_t);
 template int gather_launch<__hip_bfloat16>(const FusedArgs<__hip_bfloat16> &, hipStream_t);
 template int gather_launch<__half>(const FusedArgs<__half> &, hipStream_t);
-template int fused_max_blocks_per_cu<float>(int, int, bool, int *);
-template int fused_max_blocks_per_cu<__hip_bfloat16>(int, int, bool, int *);
-template int fused_max_blocks_per_cu<__half>(int, int, bool, int *);
+template int fused_max_blocks_per_cu<float>(int, int, int *);
+template int fused_max_blocks_per_cu<__hip_bfloat16>(int, int, int *);
+template int fused_max_blocks_per_cu<__half>(int, int, int *);
 
 }  // namespace sm

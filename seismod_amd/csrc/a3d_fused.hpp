@@ -96,25 +96,21 @@ struct FusedArgs {
     int64_t gather_row;   // -1: nothing to gather
 };
 
-// pf: planes prefetched ahead (1 or 2); a64: 64-cell aligned z tiles
+// pf: planes prefetched ahead (1 or 2)
 template <typename S>
-int fused_launch(const FusedArgs<S> &a, int tj, int pf, bool a64, hipStream_t st);
+int fused_launch(const FusedArgs<S> &a, int tj, int pf, hipStream_t st);
 template <typename S>
 int gather_launch(const FusedArgs<S> &a, hipStream_t st);
 // resident blocks per CU of the stencil kernel with TJ = tj
 template <typename S>
-int fused_max_blocks_per_cu(int tj, int pf, bool a64, int *blocks);
-// the same for the two-cells-per-lane kernel (a3d_pair.hip)
-template <typename S>
-int pair_launch(const FusedArgs<S> &a, int tj, hipStream_t st);
-template <typename S>
-int pair_max_blocks_per_cu(int tj, int *blocks);
+int fused_max_blocks_per_cu(int tj, int pf, int *blocks);
 // the same for the LDS-staged tile kernel (a3d_tile.hip; split: slab
-// groups decided per tile and range of planes); the device row pitch must
-// be a multiple of 8 cells
+// groups decided per tile and range of planes; nt: state stores
+// non-temporal, built for tj = 4 and 8); the device row pitch must be a
+// multiple of 8 cells
 template <typename S>
-int tile_launch(const FusedArgs<S> &a, int tj, bool split, hipStream_t st);
+int tile_launch(const FusedArgs<S> &a, int tj, bool split, bool nt, hipStream_t st);
 template <typename S>
-int tile_max_blocks_per_cu(int tj, bool split, bool sr, int *blocks);
+int tile_max_blocks_per_cu(int tj, bool split, bool nt, bool sr, int *blocks);
 
 }  // namespace sm

@@ -49,22 +49,9 @@ namespace {
 constexpr unsigned OFF_NONE = 0xFFFFFF00u;   // past the end of every buffer
 constexpr int RSRC_WORD3 = 0x00020000;       // raw 32-bit buffer on gfx9
 
-// cache policy of the streaming accesses (buffer instruction aux bits:
-// 2 = nt, non-temporal) -- chunk loads of the state and material, and
-// stores of the state
-#ifndef SM_TILE_LD_CPOL
-#define SM_TILE_LD_CPOL 0
-#endif
-#ifndef SM_TILE_ST_CPOL
-#define SM_TILE_ST_CPOL 0
-#endif
-// 1: the four new state values of a wave's row are gathered in LDS and
-// written as 16-byte stores (one store instruction per wave and plane)
-#ifndef SM_TILE_PACK
-#define SM_TILE_PACK 0
-#endif
-constexpr int LD_CPOL = SM_TILE_LD_CPOL, ST_CPOL = SM_TILE_ST_CPOL;
-constexpr bool PACK = SM_TILE_PACK != 0;
+// cache policy (buffer instruction aux bits) of the state stores when
+// they are non-temporal: streamed past the caches
+constexpr int CPOL_NT = 2;
 
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 using Rsrc = __amdgpu_buffer_rsrc_t;
@@ -91,15 +78,7 @@ __device__ __forceinline__ void bst(Rsrc r, unsigned voff, typename Store<S>::Bi
     else
         __builtin_amdgcn_raw_buffer_store_b16(b, r, voff, 0, 0);
 }
-// state store (streaming cache policy)
-template <typename S>
-__device__ __forceinline__ void bst_state(Rsrc r, unsigned voff, typename Store<S>::Bits b)
-{
-    if constexpr (sizeof(S) == 4)
-        __builtin_amdgcn_raw_buffer_store_b32(b, r, voff, 0, ST_CPOL);
-    else
-        __builtin_amdgcn_raw_buffer_store_b16(b, r, voff, 0, ST_CPOL);
-}
+
 
 // Tile fields of one plane slot.  The fields of the state buffer come
 // first, those of the material buffer start at a multiple of 64 chunks, so
@@ -110,10 +89,12 @@ enum { T_P = 0, T_VY = 1, T_VZ = 2, T_VX = 3, T_R = 4, T_K = 5, T_NF = 6 };
 // Rows are relative to j0 (the block's first row), cells to k0 (first
 // cell).  Chunk c of a slot lives at byte c * 16 of the slot.  SR: 16-bit
 // stores round stochastically (a compile-time choice, so the default
-// round-to-nearest kernels carry no branch or salt for it).
-template <typename S, int TJ, bool SR = false>
+// round-to-nearest kernels carry no branch or salt for it).  NTS: the
+// state stores are non-temporal.
+template <typename S, int TJ, bool SR = false, bool NTS = false>
 struct TileShape {
     static constexpr int sr = SR ? 1 : 0;
+    static constexpr int st_cpol = NTS ? CPOL_NT : 0;
     static constexpr int ES = (int)sizeof(S);
     static constexpr int esz(int f) { return f >= T_R ? 4 : ES; }
     static constexpr int cpc(int f) { return 16 / esz(f); }   // cells per 16-byte chunk
@@ -147,7 +128,7 @@ struct TileShape {
     // LDS, then written as 16-byte chunks, CPF chunks per field
     static constexpr int CPF = 64 * ES / 16;
     static constexpr int stash_bytes = 4 * 64 * ES;                  // per wave
-    static constexpr int lds_bytes = 3 * slot_bytes + (PACK ? TJ * stash_bytes : 0);
+    static constexpr int lds_bytes = 3 * slot_bytes + TJ * stash_bytes;
 };
 
 // Per-item constants of one lane (the cell (j, kk) it owns, clamped copies,
@@ -254,7 +235,7 @@ __device__ __forceinline__ void stage_issue(const Bufs &B, const unsigned (&coff
         const bool mat = r * TS::NT + w * 64 >= TS::MAT0;   // wave-uniform
         const unsigned ps = plane * (mat ? 4u : (unsigned)TS::ES);
         const unsigned vo = need && coff[r] != OFF_NONE ? coff[r] + pc * ps : OFF_NONE;
-        v[r] = __builtin_amdgcn_raw_buffer_load_b128(mat ? B.mat : B.in, vo, 0, LD_CPOL);
+        v[r] = __builtin_amdgcn_raw_buffer_load_b128(mat ? B.mat : B.in, vo, 0, 0);
     }
 }
 
@@ -341,19 +322,12 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
                                  lds_state<S>(cs, tof<TS, T_P>(L.tc, L.ckp), a.sP), r_c,
                                  lds_f32(cs, tof<TS, T_R>(L.tc, L.ckp)), a.idz, a.two_dt, Z && L.szv >= 0,
                                  L.zbh, L.zah, L.zikh, mz);
-    // stores: a lane that must not write passes OFF_NONE
-    if constexpr (PACK) {
-        stash_put<S>(stash, 1, Store<S>::enc(vxn, a.isV, F_VX * a.nnn + o, a.salt, TS::sr));
-        stash_put<S>(stash, 2, Store<S>::enc(vyn, a.isV, F_VY * a.nnn + o, a.salt, TS::sr));
-        stash_put<S>(stash, 3, Store<S>::enc(vzn, a.isV, F_VZ * a.nnn + o, a.salt, TS::sr));
-    } else {
-        bst_state<S>(B.out, L.own ? (F_VX * a.nnn + o) * E : OFF_NONE,
-                     Store<S>::enc(vxn, a.isV, F_VX * a.nnn + o, a.salt, TS::sr));
-        bst_state<S>(B.out, L.own ? (F_VY * a.nnn + o) * E : OFF_NONE,
-                     Store<S>::enc(vyn, a.isV, F_VY * a.nnn + o, a.salt, TS::sr));
-        bst_state<S>(B.out, L.own ? (F_VZ * a.nnn + o) * E : OFF_NONE,
-                     Store<S>::enc(vzn, a.isV, F_VZ * a.nnn + o, a.salt, TS::sr));
-    }
+    // the new velocities go to the wave's LDS stash (written out with P
+    // below as 16-byte chunks)
+    stash_put<S>(stash, 1, Store<S>::enc(vxn, a.isV, F_VX * a.nnn + o, a.salt, TS::sr));
+    stash_put<S>(stash, 2, Store<S>::enc(vyn, a.isV, F_VY * a.nnn + o, a.salt, TS::sr));
+    stash_put<S>(stash, 3, Store<S>::enc(vzn, a.isV, F_VZ * a.nnn + o, a.salt, TS::sr));
+    // memory-variable stores: a lane that must not write passes OFF_NONE
     {
         const unsigned ix = (unsigned)x.sxv * plane + L.o_jk;
         const unsigned iy = a.my + ((unsigned)i * np2 + L.syv) * a.pz + L.kc;
@@ -429,18 +403,16 @@ __device__ __forceinline__ void tile_plane(const FusedArgs<S> &a, const Bufs &B,
             if (L.own && (unsigned)cell == o && a.step < len) Pn += ((CF)a.src_inc)[off + a.step];
         }
     }
-    if constexpr (PACK) {
-        // cells past the grid edge (pad cells of the row) receive the
-        // values their lanes computed; no update ever reads them
-        stash_put<S>(stash, 0, Store<S>::enc(Pn, a.isP, o, a.salt, TS::sr));
-        wave_lds_sync();
-        const u32x4 pk = *(const u32x4 *)(stash + (threadIdx.x % (4 * TS::CPF)) * 16);
-        __builtin_amdgcn_raw_buffer_store_b128(pk, B.out, L.pk_ok ? (L.pk_off + i * plane) * E : OFF_NONE,
-                                               0, ST_CPOL);
-        wave_lds_sync();
-    } else {
-        bst_state<S>(B.out, L.own ? o * E : OFF_NONE, Store<S>::enc(Pn, a.isP, o, a.salt, TS::sr));
-    }
+    // the wave's four new rows leave as 16-byte stores, lane l taking
+    // chunk l % CPF of field l / CPF (lanes past the four fields and rows
+    // past the grid pass OFF_NONE); cells past the grid edge in a row (pad
+    // cells) receive the values their lanes computed, which no update reads
+    stash_put<S>(stash, 0, Store<S>::enc(Pn, a.isP, o, a.salt, TS::sr));
+    wave_lds_sync();
+    const u32x4 pk = *(const u32x4 *)(stash + (threadIdx.x % (4 * TS::CPF)) * 16);
+    __builtin_amdgcn_raw_buffer_store_b128(pk, B.out, L.pk_ok ? (L.pk_off + i * plane) * E : OFF_NONE,
+                                           0, TS::st_cpol);
+    wave_lds_sync();
     vx_prev = vxn;
 }
 
@@ -562,11 +534,11 @@ __device__ __forceinline__ void march(const FusedArgs<S> &a, const Bufs &B, cons
 // fewer instructions per interior cell); otherwise one loop with every
 // group.  Both are held to at least 4 waves per SIMD (at most 128
 // registers per lane).
-template <typename S, int TJ, bool SPLIT, bool SR>
+template <typename S, int TJ, bool SPLIT, bool SR, bool NT>
 __global__ void __launch_bounds__(64 * TJ, 4)
 a3d_tile_step(FusedArgs<S> a)
 {
-    using TS = TileShape<S, TJ, SR>;
+    using TS = TileShape<S, TJ, SR, NT>;
     constexpr int NR = TS::NR;
     __shared__ __attribute__((aligned(16))) char s_tile[TS::lds_bytes];
 
@@ -697,22 +669,25 @@ a3d_tile_step(FusedArgs<S> a)
 template <typename S>
 constexpr bool tile_has_sr() { return sizeof(S) == 2; }
 
-template <typename S, int TJ, bool SPLIT, bool SR>
+template <typename S, int TJ, bool SPLIT, bool SR, bool NT>
 static const void *tile_kernel_ptr()
 {
     if constexpr (SR && !tile_has_sr<S>()) return nullptr;
-    else return reinterpret_cast<const void *>(&a3d_tile_step<S, TJ, SPLIT, SR>);
+    else return reinterpret_cast<const void *>(&a3d_tile_step<S, TJ, SPLIT, SR, NT>);
 }
 
-#define SM_TILE_CASES(X) X(1, false) X(2, false) X(4, false) X(8, false) \
-                         X(1, true) X(2, true) X(4, true) X(8, true)
+// (rows per block, split, non-temporal stores) built
+#define SM_TILE_CASES(X) X(1, false, false) X(2, false, false) X(4, false, false) X(8, false, false) \
+                         X(1, true, false) X(2, true, false) X(4, true, false) X(8, true, false)     \
+                         X(4, false, true) X(8, false, true) X(4, true, true) X(8, true, true)
 
 template <typename S>
-int tile_max_blocks_per_cu(int tj, bool split, bool sr, int *blocks)
+int tile_max_blocks_per_cu(int tj, bool split, bool nt, bool sr, int *blocks)
 {
     const void *f = nullptr;
-#define SM_X(T, P) \
-    if (tj == T && split == P) f = sr ? tile_kernel_ptr<S, T, P, true>() : tile_kernel_ptr<S, T, P, false>();
+#define SM_X(T, P, N)                                                                          \
+    if (tj == T && split == P && nt == N)                                                      \
+        f = sr ? tile_kernel_ptr<S, T, P, true, N>() : tile_kernel_ptr<S, T, P, false, N>();
     SM_TILE_CASES(SM_X)
 #undef SM_X
     if (!f) return SM_ERR_ARG;
@@ -724,19 +699,19 @@ int tile_max_blocks_per_cu(int tj, bool split, bool sr, int *blocks)
 }
 
 template <typename S>
-int tile_launch(const FusedArgs<S> &a, int tj, bool split, hipStream_t st)
+int tile_launch(const FusedArgs<S> &a, int tj, bool split, bool nt, hipStream_t st)
 {
     const int blocks = a.nblocks + a.gather_blocks;
     bool done = false;
-#define SM_X(T, P)                                                                   \
-    if (!done && tj == T && split == P) {                                            \
-        if constexpr (tile_has_sr<S>()) {                                            \
-            if (a.sr) a3d_tile_step<S, T, P, true><<<blocks, dim3(64, T), 0, st>>>(a); \
-            else a3d_tile_step<S, T, P, false><<<blocks, dim3(64, T), 0, st>>>(a);    \
-        } else {                                                                     \
-            a3d_tile_step<S, T, P, false><<<blocks, dim3(64, T), 0, st>>>(a);        \
-        }                                                                            \
-        done = true;                                                                 \
+#define SM_X(T, P, N)                                                                           \
+    if (!done && tj == T && split == P && nt == N) {                                            \
+        if constexpr (tile_has_sr<S>()) {                                                       \
+            if (a.sr) a3d_tile_step<S, T, P, true, N><<<blocks, dim3(64, T), 0, st>>>(a);       \
+            else a3d_tile_step<S, T, P, false, N><<<blocks, dim3(64, T), 0, st>>>(a);           \
+        } else {                                                                                \
+            a3d_tile_step<S, T, P, false, N><<<blocks, dim3(64, T), 0, st>>>(a);                \
+        }                                                                                       \
+        done = true;                                                                            \
     }
     SM_TILE_CASES(SM_X)
 #undef SM_X
@@ -744,11 +719,12 @@ int tile_launch(const FusedArgs<S> &a, int tj, bool split, hipStream_t st)
     return hipGetLastError() == hipSuccess ? SM_OK : SM_ERR_HIP;
 }
 
-template int tile_launch<float>(const FusedArgs<float> &, int, bool, hipStream_t);
-template int tile_launch<__hip_bfloat16>(const FusedArgs<__hip_bfloat16> &, int, bool, hipStream_t);
-template int tile_launch<__half>(const FusedArgs<__half> &, int, bool, hipStream_t);
-template int tile_max_blocks_per_cu<float>(int, bool, bool, int *);
-template int tile_max_blocks_per_cu<__hip_bfloat16>(int, bool, bool, int *);
-template int tile_max_blocks_per_cu<__half>(int, bool, bool, int *);
+template int tile_launch<float>(const FusedArgs<float> &, int, bool, bool, hipStream_t);
+template int tile_launch<__hip_bfloat16>(const FusedArgs<__hip_bfloat16> &, int, bool, bool,
+                                         hipStream_t);
+template int tile_launch<__half>(const FusedArgs<__half> &, int, bool, bool, hipStream_t);
+template int tile_max_blocks_per_cu<float>(int, bool, bool, bool, int *);
+template int tile_max_blocks_per_cu<__hip_bfloat16>(int, bool, bool, bool, int *);
+template int tile_max_blocks_per_cu<__half>(int, bool, bool, bool, int *);
 
 }  // namespace sm

@@ -114,8 +114,9 @@ void sm_process_args(int argc, char *const argv[], sm_params *p, const char *usa
         else if (!strcmp(keyword, "alpha_max")) p->alpha_max = (short)atoi(value);
         else if (!strcmp(keyword, "shotpoint")) p->shotpt_no = atoi(value);
         else if (!strcmp(keyword, "simulateCMP")) p->simulate_cmp = atoi(value);
+        else if (!strcmp(keyword, "azimuthal_mode")) p->mode = atoi(value);
         /* other keywords (check_model, chkpt_inc, pml_median, saveEnergy,
-         * EnergyROI, azimuthal_mode) do not affect these programs */
+         * EnergyROI) do not affect these programs */
     }
     fclose(fid);
     if (p->iwipe == 0) p->np = 0;   /* io_utils.c:293 */
@@ -789,23 +790,27 @@ static void att_f64(buf *o, const char *name, const double *v, size_t n)
     for (size_t i = 0; i < n; ++i) put_f64(o, v[i]);
 }
 
-int sm_write_snapshot_nc(const double *data, double t, size_t nx, size_t nz, size_t np,
-                         double x0, double z0, double dx, double dz, const char *basename,
-                         size_t nr, int plstr)
+static int snapshot_nc(const double *data, double t, size_t nx, size_t nz, size_t np,
+                       double x0, double z0, double dx, double dz, const char *basename,
+                       size_t nr, int plstr, int cyl)
 {
     /* write_snapshot_nc / write_snapshot3d_nc (io_utils.c:1822-2180): dims
-     * x, y; double coordinate variables x(x), y(y); float z(y, x) in mm/s */
+     * x, y; double coordinate variables x(x), y(y); float z(y, x) in mm/s.
+     * Cylindrical grids (cyl) have no layer at the axis: nx + np rows, the
+     * x coordinates start at x0 (io_utils.c:1877-1881, 1946-1949); without
+     * strips the reference still skips np rows (io_utils.c:1985-1991), and
+     * so does this writer. */
     char fn[3 * SM_NAME_LEN + 64];
     make_dir("ssh");
     snprintf(fn, sizeof(fn), "ssh/%s", basename);
     make_dir(fn);
     snprintf(fn, sizeof(fn), "ssh/%s/%s_%zu_%lf.nc", basename, basename, nr + 1, t);
-    const size_t nx2 = nx + 2 * np, nz2 = nz + 2 * np;
+    const size_t nx2 = cyl ? nx + np : nx + 2 * np, nz2 = nz + 2 * np;
     const size_t mx = plstr ? nx2 : nx, mz = plstr ? nz2 : nz;
     const size_t ox = plstr ? 0 : np, oz = plstr ? 0 : np;
     double xr[2], yr[2], zr[2];
     if (plstr) {
-        xr[0] = x0 - np * dx; xr[1] = x0 + (nx + np - 1) * dx;
+        xr[0] = cyl ? x0 : x0 - np * dx; xr[1] = x0 + (nx + np - 1) * dx;
         yr[0] = z0 - np * dz; yr[1] = z0 + (nz + np - 1) * dz;
     } else {
         xr[0] = x0; xr[1] = x0 + (nx - 1) * dx;
@@ -865,7 +870,7 @@ int sm_write_snapshot_nc(const double *data, double t, size_t nx, size_t nz, siz
     }
     /* data: x, y coordinates then z row by row (y outer) */
     for (size_t i = 0; i < mx; ++i)
-        put_f64(&h, plstr ? x0 + ((1.0 * i) - np) * dx : x0 + i * dx);
+        put_f64(&h, plstr && !cyl ? x0 + ((1.0 * i) - np) * dx : x0 + i * dx);
     for (size_t j = 0; j < mz; ++j)
         put_f64(&h, plstr ? z0 + ((1.0 * j) - np) * dz : z0 + j * dz);
     for (size_t j = 0; j < mz; ++j)
@@ -882,6 +887,170 @@ int sm_write_snapshot_nc(const double *data, double t, size_t nx, size_t nz, siz
     fclose(f);
     free(h.b);
     return 0;
+}
+
+int sm_write_snapshot_nc(const double *data, double t, size_t nx, size_t nz, size_t np,
+                         double x0, double z0, double dx, double dz, const char *basename,
+                         size_t nr, int plstr)
+{
+    return snapshot_nc(data, t, nx, nz, np, x0, z0, dx, dz, basename, nr, plstr, 0);
+}
+
+int sm_write_snapshot_cyl_nc(const double *data, double t, size_t nx, size_t nz, size_t np,
+                             double x0, double z0, double dx, double dz, const char *basename,
+                             size_t nr, int plstr)
+{
+    return snapshot_nc(data, t, nx, nz, np, x0, z0, dx, dz, basename, nr, plstr, 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* e_cyl_iso inputs                                                   */
+/* ------------------------------------------------------------------ */
+void sm_read_model_e_cyl(const char *fn, const sm_grid *g, size_t np, double *vp, double *vs,
+                         double *rho)
+{
+    /* read_model_e_cyl (io_utils.c:949-1027): after the header, "Vp Vs rho"
+     * with r fastest, into rows i, columns j + np of the nx+np by nz+2np
+     * grid; a file holding one row of r values (the read of (0, 1) fails)
+     * is copied to every z; values are then copied into the z layers and
+     * the outer r layer.  A file ending elsewhere is an error here (the
+     * reference goes on with unset values). */
+    FILE *f = open_or_die(fn);
+    const size_t nx = g->nx, nz = g->nz, nz2 = nz + 2 * np, nx2 = nx + np;
+    char type[64];
+    if (fscanf(f, "%*d %*d") < 0 || fscanf(f, "%*f %*f") < 0 || fscanf(f, "%*f %*f") < 0 ||
+        fscanf(f, "%63s", type) != 1)
+        die("Error: something wrong with file %s\n", fn);
+    if (strcmp(type, "isotropic_elastic_cyl") != 0) die("Error, model in %s not isotropic_elastic_cyl\n", fn);
+    int column = 0;
+    for (size_t j = 0; j < nz && !column; ++j)
+        for (size_t i = 0; i < nx; ++i) {
+            const size_t c = i * nz2 + j + np;
+            const int nread = fscanf(f, "%lf %lf %lf", &vp[c], &vs[c], &rho[c]);
+            if (nread <= 0 && i == 0 && j == 1) { column = 1; break; }
+            if (nread != 3) die("Error: something wrong with file %s\n", fn);
+        }
+    fclose(f);
+    if (column)
+        for (size_t i = 0; i < nx; ++i)
+            for (size_t j = 1; j < nz; ++j) {
+                vp[i * nz2 + j + np] = vp[i * nz2 + np];
+                vs[i * nz2 + j + np] = vs[i * nz2 + np];
+                rho[i * nz2 + j + np] = rho[i * nz2 + np];
+            }
+    for (size_t i = 0; i < nx; ++i) {
+        for (size_t j = 0; j < np; ++j) {
+            vp[i * nz2 + j] = vp[i * nz2 + np];
+            vs[i * nz2 + j] = vs[i * nz2 + np];
+            rho[i * nz2 + j] = rho[i * nz2 + np];
+        }
+        for (size_t j = nz + np; j < nz2; ++j) {
+            vp[i * nz2 + j] = vp[i * nz2 + nz + np - 1];
+            vs[i * nz2 + j] = vs[i * nz2 + nz + np - 1];
+            rho[i * nz2 + j] = rho[i * nz2 + nz + np - 1];
+        }
+    }
+    for (size_t i = nx; i < nx2; ++i)
+        for (size_t j = 0; j < nz2; ++j) {
+            vp[i * nz2 + j] = vp[(nx - 1) * nz2 + j];
+            vs[i * nz2 + j] = vs[(nx - 1) * nz2 + j];
+            rho[i * nz2 + j] = rho[(nx - 1) * nz2 + j];
+        }
+}
+
+void sm_read_source_cyl(const char *fn, const sm_grid *g, size_t np, sm_sources *src)
+{
+    /* read_source (io_utils.c:1432-1538) on a cylindrical grid: index
+     * round((r - x0)/dr) * nz2 + round((z - z0)/dz) + np, the row without
+     * the np offset (1489-1490).  The 9-point template keeps the reference's
+     * offsets, which add np to the rows of its eight extra points
+     * (1515-1534); they take the main point's type. */
+    FILE *f = open_or_die(fn);
+    memset(src, 0, sizeof(*src));
+    if (fscanf(f, "%zu", &src->nsrc) != 1) die("Error: something wrong with file %s\n", fn);
+    if (fscanf(f, "%zu", &src->ntemplate) != 1) src->ntemplate = 1;
+    if (src->ntemplate != 1 && src->ntemplate != 9)
+        die("Error: source template must have 1 or 9 points (%s)\n", fn);
+    src->s = calloc(src->nsrc * src->ntemplate, sizeof(sm_source));
+    const size_t nx2 = g->nx + np, nz2 = g->nz + 2 * np;
+    char t[64];
+    for (size_t n = 0; n < src->nsrc * src->ntemplate; n += src->ntemplate) {
+        sm_source *s = &src->s[n];
+        if (fscanf(f, "%63s", t) != 1) die("Error: something wrong with file %s\n", fn);
+        s->type = source_type(t, 0);
+        if (s->type < 0) die("Error: type of source (%s) not implemented\n", t);
+        if (fscanf(f, "%lf", &s->A) != 1 || fscanf(f, "%lf", &s->f) != 1 ||
+            fscanf(f, "%lf %lf", &s->x, &s->z) != 2)
+            die("Error: something wrong with file %s\n", fn);
+        const size_t i = node_padded(s->x, g->x0, g->dx, nx2, 0, "source r");
+        const size_t j = node_padded(s->z, g->z0, g->dz, g->nz, np, "source z");
+        s->idx = i * nz2 + j;
+        if (src->ntemplate == 9) {
+            static const double w[9] = {0.5, 0.125, 0.25, 0.125, 0.25, 0.25, 0.125, 0.25, 0.125};
+            static const int di[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1};
+            static const int dj[9] = {0, -1, -1, -1, 0, 0, 1, 1, 1};
+            const double A = s->A;
+            for (int q = 0; q < 9; ++q) {
+                sm_source *e = &src->s[n + q];
+                e->type = s->type;
+                e->A = w[q] * A;
+                e->f = s->f;
+                e->x = s->x; e->z = s->z;
+                if (q == 0) continue;
+                const size_t ii = node_padded(s->x + di[q] * g->dx, g->x0, g->dx, nx2 - np, np,
+                                              "template r");
+                const size_t jj = node_padded(s->z + dj[q] * g->dz, g->z0, g->dz, g->nz, np,
+                                              "template z");
+                if (ii >= nx2) {
+                    fprintf(stderr, "Error: template r coordinate outside the grid\n");
+                    exit(1);
+                }
+                e->idx = ii * nz2 + jj;
+            }
+        }
+    }
+    fclose(f);
+}
+
+void sm_read_output_cyl(const char *fn, const sm_grid *g, size_t np, sm_records *out)
+{
+    /* read_output (io_utils.c:1592-1682) on a cylindrical grid: row
+     * round((r - x0)/dr), column lround((z - z0)/dz) + np (1661-1666) */
+    FILE *f = open_or_die(fn);
+    if (fscanf(f, "%zu", &out->nrec) != 1) die("Error: something wrong with file %s\n", fn);
+    out->r = calloc(out->nrec ? out->nrec : 1, sizeof(sm_record));
+    const size_t nx2 = g->nx + np, nz2 = g->nz + 2 * np;
+    char t[80];
+    for (size_t n = 0; n < out->nrec; ++n) {
+        sm_record *r = &out->r[n];
+        if (fscanf(f, "%79s", t) != 1) die("Error: something wrong with file %s\n", fn);
+        r->comp = comp_code(t, 0);
+        if (r->comp < 0) die("Error: particle velocity component (%s) does not exists\n", t);
+        if (fscanf(f, "%79s", t) != 1) die("Error: something wrong with file %s\n", fn);
+        if (!strcmp(t, "Trace") || !strcmp(t, "trace") || !strcmp(t, "TRACE")) {
+            r->type = REC_TRACE;
+            if (fscanf(f, "%lf %lf", &r->x, &r->z) != 2 || fscanf(f, "%lf", &r->t0) != 1 ||
+                fscanf(f, "%lf", &r->dt) != 1)
+                die("Error: something wrong with file %s\n", fn);
+            r->dt *= 1.e-3;
+            const double fi = round((r->x - g->x0) / g->dx);
+            const double fk = (double)lround((r->z - g->z0) / g->dz) + (double)np;
+            if (!(fi >= 0) || fi >= (double)nx2 || !(fk >= 0) || fk >= (double)nz2) {
+                fprintf(stderr, "Error: record %zu is outside the grid\n", n + 1);
+                exit(1);
+            }
+            r->idx = (size_t)fi * nz2 + (size_t)fk;
+        } else if (!strcmp(t, "Snapshot") || !strcmp(t, "snapshot") || !strcmp(t, "SNAPSHOT")) {
+            r->type = REC_SNAPSHOT;
+            if (fscanf(f, "%lf", &r->t0) != 1 || fscanf(f, "%lf", &r->dt) != 1)
+                die("Error: something wrong with file %s\n", fn);
+            r->dt *= 1.e-3;
+        } else {
+            die("Error: type of record (%s) not implemented\n", t);
+        }
+        r->fid = NULL;
+    }
+    fclose(f);
 }
 
 /* ------------------------------------------------------------------ */

@@ -29,6 +29,7 @@ typedef struct {
     size_t np;
     double vmax, pml_order, Rc, kappa_max;
     int alpha_max;
+    int mode;           /* azimuthal_mode (e_cyl_iso) */
 } sm_params;
 
 typedef struct {
@@ -103,6 +104,17 @@ void sm_close_traces(sm_records *out);
 int sm_write_snapshot_nc(const double *data, double t, size_t nx, size_t nz, size_t np,
                          double x0, double z0, double dx, double dz, const char *basename,
                          size_t nr, int plstr);
+/* the same for e_cyl_iso's cylindrical grid: data is (nx+np) x (nz+2np) */
+int sm_write_snapshot_cyl_nc(const double *data, double t, size_t nx, size_t nz, size_t np,
+                             double x0, double z0, double dx, double dz, const char *basename,
+                             size_t nr, int plstr);
+/* e_cyl_iso inputs on the (nx+np) x (nz+2np) grid, index i*nz2 + j:
+ * read_model_e_cyl (io_utils.c:949-1027), read_source and read_output
+ * with the cylindrical index rules (io_utils.c:1432-1538, 1592-1682) */
+void sm_read_model_e_cyl(const char *fn, const sm_grid *g, size_t np, double *vp, double *vs,
+                         double *rho);
+void sm_read_source_cyl(const char *fn, const sm_grid *g, size_t np, sm_sources *src);
+void sm_read_output_cyl(const char *fn, const sm_grid *g, size_t np, sm_records *out);
 /* save_segy (io_utils.c:3269-3559) */
 int sm_save_segy(const sm_params *p, const sm_records *out, const sm_sources *src);
 void sm_free_sources(sm_sources *s);

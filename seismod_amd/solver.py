@@ -142,7 +142,7 @@ class A3dConfig:
     storage: str = "f32"
     naive: bool = False
     sround: bool = False      # BF16/FP16: stochastic rounding of stored values
-    pad64: bool = False       # rows always padded to 64 cells (cell64 variants); else when <= 5% extra
+    pad64: bool = False       # rows always padded to 64 cells; else when <= 5% extra
 
     def params(self) -> L.A3dParams:
         return L.A3dParams(self.nx, self.ny, self.nz, self.np, self.dx, self.dy, self.dz,
@@ -229,15 +229,18 @@ class Acoustic3D:
         check(lib.sm_a3d_get_launch(self.handle, C.byref(tj), C.byref(ch)), "sm_a3d_get_launch")
         return tj.value, ch.value
 
-    VARIANTS = {"cell": L.SM_A3D_CELL, "pair": L.SM_A3D_PAIR, "cell2": L.SM_A3D_CELL2,
-                "cell64": L.SM_A3D_CELL64, "cell64x2": L.SM_A3D_CELL64X2, "tile": L.SM_A3D_TILE,
-                "tilesplit": L.SM_A3D_TILE_SPLIT}
+    VARIANTS = {"cell": L.SM_A3D_CELL, "cell2": L.SM_A3D_CELL2, "tile": L.SM_A3D_TILE,
+                "tilesplit": L.SM_A3D_TILE_SPLIT, "tile_nt": L.SM_A3D_TILE_NT,
+                "tilesplit_nt": L.SM_A3D_TILE_SPLIT_NT}
 
     def set_variant(self, variant: str) -> None:
         """Fused kernel variant: "cell" (one cell per lane, 63-cell z tiles),
-        "cell2" (the same, loading two planes ahead), "pair" (two z-adjacent
-        cells per lane), "cell64"/"cell64x2" (64-cell z tiles aligned to the
-        device rows, one/two planes ahead); results are bit-identical."""
+        "cell2" (the same, loading two planes ahead), "tile" (64-cell z tiles
+        aligned to the device rows, staged through LDS, 16-byte stores),
+        "tilesplit" (the same with the CPML work compiled in per tile only
+        where a layer is touched), "tile_nt"/"tilesplit_nt" (those two with
+        non-temporal state stores; rows_per_block 4 or 8); results are
+        bit-identical."""
         check(lib.sm_a3d_set_variant(self.handle, self.VARIANTS[variant]), "sm_a3d_set_variant")
 
     def variant(self) -> str:
@@ -443,6 +446,123 @@ class E2dBatch:
     def close(self) -> None:
         if getattr(self, "handle", None):
             check(lib.sm_e2d_batch_destroy(self.handle), "sm_e2d_batch_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class CylConfig:
+    """e_cyl_iso grid: nx radial and nz vertical interior nodes, np CPML
+    layers on the outer r side and both z ends (padded nx+np by nz+2np)."""
+    nx: int
+    nz: int
+    np: int
+    dr: float
+    dz: float
+    dt: float
+    f0: float
+    mode: int = 0             # azimuthal mode n
+    vmax: float = 5000.0
+    pml_order: float = 2.0
+    Rc: float = 1e-5
+    kappa_max: float = 1.0
+    alpha_max: int = 1
+    storage: str = "f64"
+
+    def params(self) -> L.CylParams:
+        return L.CylParams(self.nx, self.nz, self.np, self.dr, self.dz, self.dt, self.mode,
+                           self.vmax, self.pml_order, self.Rc, self.kappa_max, self.alpha_max,
+                           self.f0, _STORAGE[self.storage], 0)
+
+    @property
+    def shape(self) -> tuple[int, int]:
+        return (self.nx + self.np, self.nz + 2 * self.np)
+
+    def bytes_needed(self, nrec: int = 0, max_steps: int = 0) -> int:
+        p = self.params()
+        return int(lib.sm_cyl_bytes_needed(C.byref(p), nrec, max_steps))
+
+
+def kurkjian_fct(lam: float, dr: float, dz: float, r: float, mode: int, A: float, f: float,
+                 dt: float) -> np.ndarray:
+    """compute_kurkjian (src.c:202-229) for a source with Lame constant lam
+    at radius r."""
+    n = ricker_length(f, dt)
+    out = np.zeros(n)
+    check(lib.sm_kurkjian_fct(lam, dr, dz, r, mode, A, f, dt, n, _dp(out)), "sm_kurkjian_fct")
+    return out
+
+
+class Cylindrical:
+    """2.5D elastic solver in cylindrical coordinates (e_cyl_iso)."""
+
+    FIELDS = {"vr": L.SM_VR, "vt": L.SM_VT, "vz": L.SM_VZ, "trr": L.SM_TRR, "ttt": L.SM_TTT,
+              "tzz": L.SM_TZZ, "trz": L.SM_TRZ, "trt": L.SM_TRT, "ttz": L.SM_TTZ}
+
+    def __init__(self, ctx: Context, cfg: CylConfig, vp, vs, rho):
+        self.ctx, self.cfg = ctx, cfg
+        n = int(np.prod(cfg.shape))
+        self._m = [_f64(a).reshape(-1) for a in (vp, vs, rho)]
+        if any(a.size != n for a in self._m):
+            raise ValueError(f"model must hold {n} padded cells")
+        self._p = cfg.params()
+        h = C.c_void_p()
+        ctx.adopt(self)
+        check(lib.sm_cyl_create(ctx.handle, C.byref(self._p), *(_dp(a) for a in self._m),
+                                C.byref(h)), "sm_cyl_create")
+        self.handle = h
+        self.nrec = 0
+
+    def set_sources(self, types, idx, lengths, fct) -> None:
+        types = np.ascontiguousarray(types, dtype=np.intc)
+        idx = np.ascontiguousarray(idx, dtype=np.uintp)
+        lengths = np.ascontiguousarray(lengths, dtype=np.uintp)
+        fct = _f64(fct)
+        check(lib.sm_cyl_set_sources(self.handle, len(idx), _ip(types), _sp(idx), _sp(lengths),
+                                     _dp(fct)), "sm_cyl_set_sources")
+
+    def set_receivers(self, idx, comp, max_steps: int) -> None:
+        idx = np.ascontiguousarray(idx, dtype=np.uintp)
+        comp = np.ascontiguousarray(comp, dtype=np.intc)
+        check(lib.sm_cyl_set_receivers(self.handle, len(idx), _sp(idx), _ip(comp), max_steps),
+              "sm_cyl_set_receivers")
+        self.nrec = len(idx)
+
+    def run(self, nsteps: int) -> None:
+        check(lib.sm_cyl_run(self.handle, nsteps), "sm_cyl_run")
+
+    @property
+    def step(self) -> int:
+        it = C.c_size_t()
+        check(lib.sm_cyl_current_step(self.handle, C.byref(it)), "sm_cyl_current_step")
+        return it.value
+
+    def traces(self, it0: int, count: int) -> np.ndarray:
+        out = np.zeros((count, self.nrec), dtype=np.float64)
+        check(lib.sm_cyl_get_traces(self.handle, it0, count, _dp(out)), "sm_cyl_get_traces")
+        return out
+
+    def field(self, name: str) -> np.ndarray:
+        out = np.zeros(self.cfg.shape, dtype=np.float64)
+        check(lib.sm_cyl_get_field(self.handle, self.FIELDS[name], _dp(out)), "sm_cyl_get_field")
+        return out
+
+    def reset(self) -> None:
+        check(lib.sm_cyl_reset(self.handle), "sm_cyl_reset")
+
+    def device_bytes(self) -> int:
+        b = C.c_size_t()
+        check(lib.sm_cyl_device_bytes(self.handle, C.byref(b)), "sm_cyl_device_bytes")
+        return b.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            check(lib.sm_cyl_destroy(self.handle), "sm_cyl_destroy")
             self.handle = None
 
     def __del__(self):

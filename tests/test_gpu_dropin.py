@@ -20,7 +20,7 @@ from pathlib import Path
 
 import numpy as np
 import pytest
-from tolerances import FP32_REL_L2
+from tolerances import FP32_REL_L2, note
 
 from oracle import pyoracle as O
 
@@ -256,3 +256,117 @@ def test_a_iso_3d_segy_samples(tmp_path, storage):
         pos += 240 + 4 * nsr
         ntr += 1
     assert ntr == 2 and pos == len(b)
+
+
+# ------------------------------------------------------------ e_cyl_iso --
+def deck_cyl(tmp, nx=24, nz=30, np_=5, h=2.0, dt_ms=0.25, time_s=0.03, mode=1, plstr=1,
+             kurkjian=False):
+    rng = np.random.default_rng(13)
+    vp = np.where(np.arange(nz)[None, :] < nz // 2, 2500.0, 3200.0) * np.ones((nx, 1))
+    vp = vp * (1 + 0.02 * rng.uniform(-1, 1, (nx, nz)))
+    vs = vp / 1.8
+    vs[:2, :] = 0.0                                   # fluid-filled borehole
+    rho = np.where(np.arange(nx)[:, None] < 2, 1000.0, 2300.0) * np.ones((1, nz))
+    with open(tmp / "model.dat", "w") as f:
+        f.write(f"{nx} {nz}\n{h} {h}\n0.0 0.0\nisotropic_elastic_cyl\n")
+        for j in range(nz):                           # r fastest (io_utils.c:974-1000)
+            for i in range(nx):
+                f.write(f"{float(vp[i, j])!r} {float(vs[i, j])!r} {float(rho[i, j])!r}\n")
+    # the Kurkjian source on the axis in the fluid; the radial force off the
+    # axis in the solid (a radial force on the axis of a fluid does not
+    # couple to the stresses of the first cell)
+    src_r = 0.0 if kurkjian else 6.0
+    src = (f"1\n1\nKurkjian\n1e-6\n400.0\n{src_r} 30.0\n" if kurkjian
+           else f"1\n1\nFr\n1.0\n400.0\n{src_r} 30.0\n")
+    (tmp / "src.dat").write_text(src)
+    recs = [("Vr", "Trace", "10.0 40.0", 0.0, dt_ms), ("Vz", "Trace", "20.0 10.0", 0.005, 2 * dt_ms),
+            ("trr", "Trace", "6.0 30.0", 0.0, dt_ms), ("div", "Trace", "12.0 24.0", 0.0, dt_ms),
+            ("curl", "Trace", "14.0 50.0", 0.0, dt_ms), ("Vtheta", "Trace", "8.0 36.0", 0.0, dt_ms),
+            ("Vz", "Snapshot", None, 0.02, 0.0)]
+    with open(tmp / "rcv.dat", "w") as f:
+        f.write(f"{len(recs)}\n")
+        for comp, kind, xz, t0, rdt in recs:
+            f.write(f"{comp}\n{kind}\n" + (f"{xz}\n" if xz else "") + f"{t0}\n{rdt}\n")
+    (tmp / "run.par").write_text(
+        "model.dat # model, file\nsrc.dat # source, file\nrcv.dat # output, records\n"
+        f"c1 # basename, name\n{time_s} # time, s\n{dt_ms} # dt, ms\n1 # abs, on\n{np_} # nl, layers\n"
+        f"{plstr} # plstr, strips\n{mode} # azimuthal_mode, n\n")
+    nx2, nz2 = nx + np_, nz + 2 * np_
+    vpp = np.zeros((nx2, nz2)); vsp = np.zeros((nx2, nz2)); rhop = np.zeros((nx2, nz2))
+    for a, b in ((vpp, vp), (vsp, vs), (rhop, rho)):
+        a[:nx, np_:np_ + nz] = b
+        a[:nx, :np_] = b[:, :1]
+        a[:nx, np_ + nz:] = b[:, -1:]
+        a[nx:, :] = a[nx - 1:nx, :]
+    cell = lambda r, z: int(round(r / h)) * nz2 + int(round(z / h)) + np_
+    comp = {"Vr": 13, "Vz": 2, "trr": 15, "div": 11, "curl": 12, "Vtheta": 14}
+    rec = [(cell(*map(float, xz.split())), comp[c], t0, rdt) for c, k, xz, t0, rdt in recs if xz]
+    return dict(nx=nx, nz=nz, np=np_, h=h, dt=dt_ms * 1e-3, vp=vpp, vs=vsp, rho=rhop, mode=mode,
+                src=cell(src_r, 30.0), src_r=src_r, rec=rec, snap_t0=0.02, kurkjian=kurkjian,
+                f0=400.0)
+
+
+def oracle_cyl(d, nsteps):
+    import math
+    cfg = dict(nx=d["nx"], nz=d["nz"], np=d["np"], dx=d["h"], dz=d["h"], dt=d["dt"], n=d["mode"],
+               alpha_max=math.pi * d["f0"])
+    if d["kurkjian"]:
+        c = d["src"]
+        vp, vs, rho = (a.reshape(-1)[c] for a in (d["vp"], d["vs"], d["rho"]))
+        fct = O.kurkjian_fct(rho * (vp * vp - 2. * vs * vs), d["h"], d["h"], d["src_r"], d["mode"],
+                             1e-6, d["f0"], d["dt"])
+        L, st = np.array([len(fct)], np.uintp), 12
+    else:
+        L, fct = O.source_functions([1.0], [d["f0"]], d["dt"])
+        st = 9
+    ridx = np.array([r[0] for r in d["rec"]], dtype=np.uintp)
+    rcomp = np.array([r[1] for r in d["rec"]], dtype=np.intc)
+    return O.cyl_run(cfg, d["vp"], d["vs"], d["rho"], np.array([st], np.intc),
+                     np.array([d["src"]], np.uintp), L, fct, ridx, rcomp, nsteps)
+
+
+# FP32 e_cyl_iso deck: the 400 Hz source is four cells per P wavelength, so
+# the field is dominated by grid-scale oscillation and FP32 rounding in the
+# differences shows more than on the well-sampled decks (Vz snapshot measured
+# at 6.1e-6 relative L2 against the FP64 oracle); ten times that
+CYL_DROPIN_FP32 = 6e-5
+
+
+@pytest.mark.parametrize("storage,mode,plstr,kurk", [("f64", 1, 1, False), ("f64", 0, 0, True),
+                                                     ("f32", 1, 1, False)])
+def test_e_cyl_iso_program(tmp_path, storage, mode, plstr, kurk):
+    from seismod_amd.solver import nsteps as sm_nsteps, record_due
+    d = deck_cyl(tmp_path, mode=mode, plstr=plstr, kurkjian=kurk)
+    run("e_cyl_iso", tmp_path, storage)
+    ns = sm_nsteps(0.03, d["dt"])
+    tr, _ = oracle_cyl(d, ns)
+    for q, (_, _, t0, rdt) in enumerate(d["rec"]):
+        got = read_trc(tmp_path / "trc" / "c1" / f"c1_{q + 1}.trc")
+        want = trace_lines(tr[:, q], d["dt"], t0, rdt * 1e-3)
+        assert len(got) == len(want) > 0
+        if storage == "f64":
+            assert got == want, q
+        else:
+            g = np.array([float(l.split()[1]) for l in got])
+            w = np.array([float(l.split()[1]) for l in want])
+            e = np.linalg.norm(g - w) / max(np.linalg.norm(w), 1e-300)
+            note("e_cyl_iso_program_fp32_trace", record=q, rel_l2=e)
+            assert e <= CYL_DROPIN_FP32, q
+    snaps = sorted((tmp_path / "ssh" / "c1").glob("c1_7_*.nc"))
+    assert len(snaps) == 1
+    it_snap = [it for it in range(ns) if record_due(it, d["dt"], d["snap_t0"], 0.0)]
+    _, f = oracle_cyl(d, it_snap[0] + 1)
+    nx2, nz2 = d["nx"] + d["np"], d["nz"] + 2 * d["np"]
+    vz = f[2 * nx2 * nz2:3 * nx2 * nz2].reshape(nx2, nz2)
+    # with strips the whole (nx+np) x (nz+2np) grid; without, the reference
+    # skips np rows in r as well (io_utils.c:1985-1991)
+    sl = vz if plstr else vz[d["np"]:d["np"] + d["nx"], d["np"]:d["np"] + d["nz"]]
+    z = nc_field(snaps[0])
+    assert z.shape == (sl.shape[1], sl.shape[0])
+    want = (sl.T.astype(np.float32) * np.float32(1e3)).astype(np.float64)
+    if storage == "f64":
+        assert np.array_equal(z, want)
+    else:
+        e = np.linalg.norm(z - want) / np.linalg.norm(want)
+        note("e_cyl_iso_program_fp32_snapshot", rel_l2=e)
+        assert e <= CYL_DROPIN_FP32

@@ -1,9 +1,10 @@
-"""GPU tests of the 3D kernel variants: two cells per lane ("pair"), one
-cell per lane with loads two planes ahead ("cell2"), 64-cell z tiles
-aligned to the device rows ("cell64", "cell64x2"), and 64-cell tiles staged
-through LDS in 16-byte chunks ("tile").
+"""GPU tests of the 3D kernel variants: one cell per lane with loads two
+planes ahead ("cell2"), 64-cell tiles staged through LDS in 16-byte chunks
+with 16-byte state stores ("tile"), the same with the CPML work compiled in
+per tile only where a layer is touched ("tilesplit"), and those two with
+non-temporal state stores ("tile_nt", "tilesplit_nt").
 
-It performs the same per-cell arithmetic as the one-cell-per-lane variant,
+Each performs the same per-cell arithmetic as the one-cell-per-lane variant,
 only with a different mapping of cells to lanes and memory accesses, so
 traces and all four fields must be bit-identical to it for every storage
 type, launch shape and run splitting.  The one-cell variant is itself held
@@ -38,8 +39,7 @@ def run(ctx, d, storage, variant, launch=None, sround=False, split=None, pad64=N
     cfg = A3dConfig(nx=c["nx"], ny=c["ny"], nz=c["nz"], np=c["np"], dx=c["dx"], dy=c["dy"],
                     dz=c["dz"], dt=c["dt"], f0=c["f0"], vmax=c["vmax"], pml_order=c["pml_order"],
                     Rc=c["Rc"], kappa_max=c["kappa_max"], alpha_max=c["alpha_max"],
-                    storage=storage, sround=sround,
-                    pad64=variant.startswith("cell64") if pad64 is None else pad64)
+                    storage=storage, sround=sround, pad64=bool(pad64))
     s = Acoustic3D(ctx, cfg, d["vp"], d["rho"])
     s.set_variant(variant)
     if launch is not None:
@@ -55,7 +55,10 @@ def run(ctx, d, storage, variant, launch=None, sround=False, split=None, pad64=N
     return tr, f
 
 
-@pytest.mark.parametrize("variant", ["pair", "cell2", "cell64", "cell64x2", "tile", "tilesplit"])
+VARIANTS = ["cell2", "tile", "tilesplit", "tile_nt", "tilesplit_nt"]
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("storage", ["f32", "bf16", "f16"])
 @pytest.mark.parametrize("case", CASES, ids=ids)
 def test_variant_equals_cell(ctx, case, storage, variant):
@@ -68,27 +71,28 @@ def test_variant_equals_cell(ctx, case, storage, variant):
     assert np.array_equal(f_p, f_c), np.abs(f_p - f_c).max()
 
 
-@pytest.mark.parametrize("variant", ["pair", "cell2", "cell64", "cell64x2", "tile", "tilesplit"])
-@pytest.mark.parametrize("launch", [(1, 1), (2, 3), (4, 7), (8, 2), (4, 17)])
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("launch", [(1, 1), (2, 3), (4, 7), (8, 2), (4, 17), (8, 9)])
 def test_variant_launch_shapes_and_splitting(ctx, launch, variant):
+    if variant.endswith("_nt") and launch[0] < 4:
+        pytest.skip("the non-temporal variants are built for 4 and 8 rows per block")
     d = decks.a3d_deck(nx=26, ny=21, nz=68, np_=4, seed=7, nsteps=70)
     ref = run(ctx, d, "f32", "cell")
     got = run(ctx, d, "f32", variant, launch=launch, split=[1, 20, 49])
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
 
 
-@pytest.mark.parametrize("variant", ["pair", "cell64", "tile", "tilesplit"])
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("storage", ["bf16", "f16"])
 def test_variant_stochastic_rounding_equals_cell(ctx, storage, variant):
-    # the rounding bits hash the device element index: compare on one layout
+    # the rounding bits hash the device element index: same layout for both
     d = decks.a3d_deck(nx=14, ny=12, nz=68, np_=4, seed=8, nsteps=80)
-    pad = variant.startswith("cell64")
-    a = run(ctx, d, storage, "cell", sround=True, pad64=pad)
-    b = run(ctx, d, storage, variant, sround=True, pad64=pad)
+    a = run(ctx, d, storage, "cell", sround=True)
+    b = run(ctx, d, storage, variant, sround=True)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
-@pytest.mark.parametrize("variant", ["pair", "tile", "tilesplit"])
+@pytest.mark.parametrize("variant", ["tile", "tilesplit", "tilesplit_nt"])
 def test_variant_fp32_vs_oracle(ctx, variant):
     d = decks.a3d_deck(nx=11, ny=10, nz=40, np_=5, seed=9, nsteps=100)
     c = d["cfg"]
@@ -104,41 +108,50 @@ def test_variant_fp32_vs_oracle(ctx, variant):
 @pytest.mark.parametrize("storage", ["f32", "bf16"])
 @pytest.mark.parametrize("shape", [(8, 8, 7, 3), (9, 13, 1, 2), (5, 6, 131, 3)],
                          ids=["nz2_13", "nz2_5", "nz2_137"])
-@pytest.mark.parametrize("variant", ["tile", "tilesplit"])
+@pytest.mark.parametrize("variant", ["tile", "tilesplit", "tile_nt"])
 def test_tile_odd_rows_equal_cell(ctx, storage, shape, variant):
     """Row pitch rounded up from an odd z extent: the chunks past the last
     cell of a row and the halo chunk of the first tile are clamped."""
     nx, ny, nz, np_ = shape
     d = decks.a3d_deck(nx=nx, ny=ny, nz=nz, np_=np_, seed=13, nsteps=40)
     a = run(ctx, d, storage, "cell")
-    b = run(ctx, d, storage, variant, launch=(2, 2))
+    b = run(ctx, d, storage, variant, launch=(4 if variant.endswith("_nt") else 2, 2))
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("variant", ["tilesplit", "tilesplit_nt"])
 @pytest.mark.parametrize("storage", ["f32", "bf16"])
 @pytest.mark.parametrize("nchunks", [1, 3, 7, 14, 21])
-def test_tilesplit_ranges_equal_cell(ctx, storage, nchunks):
+def test_tilesplit_ranges_equal_cell(ctx, storage, nchunks, variant):
     """x chunks that start and end inside, at and across the x absorbing
     layers (chunk boundaries at every parity), y and z tiles in and out of
     the y / z layers."""
     d = decks.a3d_deck(nx=32, ny=30, nz=140, np_=5, seed=14, nsteps=30)
     a = run(ctx, d, storage, "cell")
-    b = run(ctx, d, storage, "tilesplit", launch=(2, nchunks))
+    b = run(ctx, d, storage, variant, launch=(4 if variant.endswith("_nt") else 2, nchunks))
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
-def test_pair_needs_even_z(ctx):
-    from seismod_amd._lib import SeismodError
-    d = decks.a3d_deck(nx=8, ny=8, nz=7, np_=3, seed=10, nsteps=5)   # nz2 = 13
-    with pytest.raises(SeismodError):
-        run(ctx, d, "f32", "pair")
-
-
-def test_cell64_needs_padded_rows(ctx):
-    from seismod_amd._lib import SeismodError
+def test_unknown_variant_ids_and_nt_row_counts_are_refused(ctx):
+    from seismod_amd import _lib as L
+    from seismod_amd._lib import SeismodError, check, lib
+    from seismod_amd.solver import A3dConfig, Acoustic3D
     d = decks.a3d_deck(nx=8, ny=8, nz=10, np_=3, seed=11, nsteps=5)
-    with pytest.raises(SeismodError):
-        run(ctx, d, "f32", "cell64", pad64=False)
+    c = d["cfg"]
+    s = Acoustic3D(ctx, A3dConfig(nx=c["nx"], ny=c["ny"], nz=c["nz"], np=c["np"], dx=c["dx"],
+                                  dy=c["dy"], dz=c["dz"], dt=c["dt"], f0=c["f0"], vmax=c["vmax"]),
+                   d["vp"], d["rho"])
+    for v in (1, 3, 4, 9, -1):
+        with pytest.raises(SeismodError) as e:
+            check(lib.sm_a3d_set_variant(s.handle, v), "sm_a3d_set_variant")
+        assert e.value.status == L.SM_ERR_ARG
+    s.set_variant("tile_nt")
+    for tj in (1, 2):
+        with pytest.raises(SeismodError):
+            s.set_launch(tj, 2)
+    s.set_launch(8, 2)
+    assert s.launch() == (8, 2)
+    s.close()
 
 
 @pytest.mark.parametrize("storage", ["f32", "bf16"])

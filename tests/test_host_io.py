@@ -434,3 +434,46 @@ def test_3d_positions_inside_the_cpml_are_accepted(tmp_path):
         tmp_path, "1\n1\nBulk\n1.0\n15.0\n0.0 0.0 0.0\n",
         "1\nVz\nTrace\n0.0 0.0 310.0 0.0 1.0\n", np_)
     assert rc == 1 and "outside the grid" in err
+
+
+@pytest.mark.parametrize("column", [False, True])
+def test_read_model_e_cyl_layout_and_padding(tmp_path, column):
+    """read_model_e_cyl (io_utils.c:949-1027): "Vp Vs rho" with r fastest
+    into the (nx+np) x (nz+2np) grid; one r profile fills every z; edges
+    copied into the z layers and the outer r layer (none at the axis)."""
+    rng = np.random.default_rng(5)
+    nx, nz, np_ = 5, 4, 2
+    m = rng.uniform(1000, 3000, (3, nx, nz))
+    if column:
+        m[:] = m[:, :, :1]
+    fn = tmp_path / "c.dat"
+    with open(fn, "w") as f:
+        f.write(f"{nx} {nz}\n1.5 2.0\n0.0 0.0\nisotropic_elastic_cyl\n")
+        for j in range(1 if column else nz):
+            for i in range(nx):
+                f.write(" ".join(repr(float(m[q, i, j])) for q in range(3)) + "\n")
+    g = Grid()
+    LIB.sm_read_grid2d(str(fn).encode(), C.byref(g))
+    out = [np.zeros((nx + np_, nz + 2 * np_)) for _ in range(3)]
+    LIB.sm_read_model_e_cyl(str(fn).encode(), C.byref(g), C.c_size_t(np_),
+                            *(o.ctypes.data_as(C.c_void_p) for o in out))
+    for q in range(3):
+        want = np.pad(m[q], ((0, np_), (np_, np_)), mode="edge")
+        assert np.array_equal(out[q], want), q
+
+
+def test_read_source_and_records_cyl(tmp_path):
+    """Cylindrical index rules: row round((r - r0)/dr) with no axis layer,
+    column round((z - z0)/dz) + np (io_utils.c:1488-1491, 1661-1666)."""
+    g = Grid(nx=20, ny=1, nz=30, dx=2.0, dy=2.0, dz=2.0)
+    np_ = 4
+    nz2 = 30 + 2 * np_
+    (tmp_path / "s.dat").write_text("1\n1\nKurkjian\n1e-6\n8000.0\n0.0 30.9\n")
+    s = Sources()
+    LIB.sm_read_source_cyl(str(tmp_path / "s.dat").encode(), C.byref(g), C.c_size_t(np_), C.byref(s))
+    assert (s.s[0].type, s.s[0].idx) == (12, 0 * nz2 + 15 + np_)
+    (tmp_path / "r.dat").write_text("2\nVr\nTrace\n7.1 -3.9 0.0 0.1\ntthetaz\nTrace\n40.0 10.0 0.0 0.1\n")
+    out = Records()
+    LIB.sm_read_output_cyl(str(tmp_path / "r.dat").encode(), C.byref(g), C.c_size_t(np_), C.byref(out))
+    assert (out.r[0].comp, out.r[0].idx) == (13, 4 * nz2 - 2 + np_)     # z in the top layer
+    assert (out.r[1].comp, out.r[1].idx) == (19, 20 * nz2 + 5 + np_)    # r in the outer layer
