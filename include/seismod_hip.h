@@ -321,6 +321,21 @@ int sm_e2d_batch_run(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots,
 int sm_e2d_batch_set_variant(sm_e2d_batch *b, int variant);
 int sm_e2d_batch_set_timing(sm_e2d_batch *b, int on);
 int sm_e2d_batch_get_timing(const sm_e2d_batch *b, double *ms_total, size_t *launches);
+/* Allocation-free variant of sm_e2d_batch_run.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory
+ * (serving loops and harnesses that forbid hipMalloc; it synchronises the
+ * context's stream, so it is not for use under graph capture).  The same
+ * kernels and results as sm_e2d_batch_run, with plain launches (no graph
+ * replay, no timing); the shot table the kernels read lives in a
+ * file-scope device array of 1 MiB per GPU (held until the process ends)
+ * instead of memory the batch allocates; no hipMalloc or hipFree is
+ * called.  A table larger than that (about
+ * 250 000 blocks of four waves: some 60 M padded cells at one row per
+ * chunk) is refused with SM_ERR_ARG.  Calls that overlap on one GPU, from
+ * two host threads, race on that array; calls on one context are
+ * serialised by the call itself. */
+int sm_e2d_batch_run_noalloc(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots,
+                             const size_t *nsteps);
 /* Captured hipGraphs the batch holds (at most 16: the least recently used
  * one is destroyed to make room) and has instantiated since creation (no
  * counterpart in the reference). */

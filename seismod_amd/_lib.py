@@ -124,6 +124,7 @@ _SIGS = {
     "sm_e2d_batch_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sm_e2d_batch_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),
     "sm_e2d_batch_get_graphs": (C.c_int, [C.c_void_p, _size_p, _size_p]),
+    "sm_e2d_batch_run_noalloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p), _size_p]),
     "sm_cyl_create": (C.c_int, [C.c_void_p, C.POINTER(CylParams), _dbl_p, _dbl_p, _dbl_p,
                                 C.POINTER(C.c_void_p)]),
     "sm_cyl_destroy": (C.c_int, [C.c_void_p]),

@@ -930,6 +930,19 @@ int sm_e2d_batch_run(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots, const
     return SM_OK;
 }
 
+int sm_e2d_batch_run_noalloc(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots,
+                             const size_t *nsteps)
+{
+    std::vector<E2dShot> d;
+    int rc = batch_shots(b, nshots, shots, nsteps, d);
+    if (rc != SM_OK) return rc;
+    rc = b->runner.run_static(d, b->ctx->device, b->ctx->stream);
+    if (rc != SM_OK) return rc;
+    SM_HIP(hipStreamSynchronize(b->ctx->stream));
+    for (size_t k = 0; k < nshots; ++k) shots[k]->step += nsteps[k];
+    return SM_OK;
+}
+
 int sm_e2d_batch_set_variant(sm_e2d_batch *b, int variant)
 {
     SM_CHECK(b, SM_ERR_ARG, "sm_e2d_batch_set_variant: NULL batch");

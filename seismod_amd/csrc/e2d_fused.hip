@@ -17,6 +17,8 @@
 #include <cstring>
 #include <type_traits>
 
+#include <mutex>
+
 #include "e2d_fused.hpp"
 #include "sm_internal.hpp"
 
@@ -649,6 +651,67 @@ int E2dRunner::run(std::vector<E2dShot> shots, hipStream_t st, int graph_steps)
         launches += count;
     }
     return SM_OK;
+}
+
+// ---------------------------------------------------------------------
+// Allocation-free batches.  The shot table and block -> shot map of a run
+// live in this file-scope device array instead of memory the runner
+// allocates.  Each device holds its own copy; host code reaches it only
+// through the address hipGetSymbolAddress returns on that device (the
+// array's name is not a device address in host code).  Every run on a
+// device overwrites the whole table it reads, so nothing needs clearing;
+// two runs that overlap on one device (from two host threads) would
+// overwrite each other's table.
+constexpr size_t E2D_SCRATCH_BYTES = 1u << 20;
+static __device__ __align__(256) unsigned char e2d_scratch[E2D_SCRATCH_BYTES];
+
+namespace {
+constexpr int E2D_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[E2D_MAX_DEVICES] = {};
+}  // namespace
+
+int e2d_scratch_init(int device)
+{
+    SM_CHECK(device >= 0 && device < E2D_MAX_DEVICES, SM_ERR_ARG, "device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return SM_OK;
+    int cur = -1;
+    SM_HIP(hipGetDevice(&cur));
+    SM_CHECK(cur == device, SM_ERR_STATE, "e2d_scratch_init: device %d is not current (%d)", device, cur);
+    void *p = nullptr;
+    SM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(e2d_scratch)));
+    SM_CHECK(p != nullptr, SM_ERR_HIP, "hipGetSymbolAddress returned NULL");
+    g_scratch[device] = p;
+    return SM_OK;
+}
+
+size_t e2d_scratch_bytes() { return E2D_SCRATCH_BYTES; }
+
+int E2dRunner::run_static(std::vector<E2dShot> shots, int device, hipStream_t st)
+{
+    void *scratch = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (device >= 0 && device < E2D_MAX_DEVICES) scratch = g_scratch[device];
+    }
+    SM_CHECK(scratch, SM_ERR_STATE, "allocation-free batch: no scratch address for device %d "
+             "(the context was not created by sm_ctx_create)", device);
+    std::vector<int> pref;
+    std::vector<unsigned char> host;
+    size_t tab_bytes = 0;
+    int rc = plan(shots, pref, host, tab_bytes);
+    if (rc != SM_OK || shots.empty()) return rc;
+    SM_CHECK(host.size() <= E2D_SCRATCH_BYTES, SM_ERR_ARG,
+             "allocation-free batch: the shot table needs %zu bytes, the device scratch holds %zu "
+             "(fewer shots or more rows per chunk)", host.size(), (size_t)E2D_SCRATCH_BYTES);
+    // on one stream the copy is ordered after the kernels of an earlier run
+    // that read the table; runs on other streams of the device must not
+    // overlap this one (see above)
+    SM_HIP(hipMemcpyAsync(scratch, host.data(), host.size(), hipMemcpyHostToDevice, st));
+    SM_HIP(hipStreamSynchronize(st));
+    size_t count = 0;
+    return launch_all(shots, pref, scratch, tab_bytes, st, 0, &count);
 }
 
 }  // namespace sm

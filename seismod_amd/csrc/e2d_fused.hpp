@@ -78,6 +78,9 @@ class E2dRunner {
 public:
     ~E2dRunner();
     int run(std::vector<E2dShot> shots, hipStream_t st, int graph_steps);
+    // the same without allocating: the table goes to the device's static
+    // scratch array (see e2d_scratch_init), steps are plain launches
+    int run_static(std::vector<E2dShot> shots, int device, hipStream_t st);
     size_t device_bytes() const;
     // with timing on, each run is bracketed by HIP events on its stream;
     // totals accumulate (graph replays count each of their step launches)
@@ -119,5 +122,10 @@ public:
     size_t graphs_live() const { return graphs_.size(); }
     size_t graphs_built = 0;    // instantiations since creation
 };
+
+// Address of the allocation-free batch scratch on `device` (which must be
+// current), looked up once per device; called by sm_ctx_create.
+int e2d_scratch_init(int device);
+size_t e2d_scratch_bytes();
 
 }  // namespace sm

@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "e2d_fused.hpp"
 #include "sm_internal.hpp"
 
 namespace sm {
@@ -79,6 +80,9 @@ int sm_ctx_create(int device, sm_ctx **out)
     SM_CHECK(device >= 0 && device < count, SM_ERR_ARG, "device %d out of range (%d devices)",
              device, count);
     SM_HIP(hipSetDevice(device));
+    // per-device address of the allocation-free batches' scratch array
+    const int rc = sm::e2d_scratch_init(device);
+    if (rc != SM_OK) return rc;
     hipStream_t st;
     SM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     sm_ctx *c = new sm_ctx;
