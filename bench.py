@@ -40,11 +40,17 @@ STATE_BYTES = {"f32": 4, "bf16": 2, "f16": 2}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="a_iso_3d", choices=["a_iso_3d", "e_cyl_iso"],
+                    help="a_iso_3d (the headline, default) or e_cyl_iso (the 2.5D cylindrical "
+                         "elastic solver on a borehole model, FP32 or FP64 storage)")
     ap.add_argument("--steps", type=int, default=5, help="timed shots")
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
     ap.add_argument("--n", type=int, default=256, help="interior nodes per axis")
     ap.add_argument("--nsteps", type=int, default=None, help="time steps per shot")
-    ap.add_argument("--storage", default="f32", choices=sorted(STATE_BYTES))
+    ap.add_argument("--storage", default="f32", choices=sorted(STATE_BYTES) + ["f64"])
+    ap.add_argument("--mat16", action="store_true",
+                    help="16-bit storage only: material (rho, K*dt) as FP16 with power-of-two "
+                         "scales (tile kernels)")
     ap.add_argument("--launch", default="auto",
                     help="[variant,]rows_per_block,x_chunks, or 'auto' (the autotuner picks "
                          "them during set-up, before the timed region)")
@@ -174,11 +180,143 @@ def measure_512(ctx, storage: str, nsteps: int) -> dict:
             "launch": {"kernel_variant": variant, "rows_per_block": tj, "x_chunks": ch}}
 
 
+def source_lambda(shot) -> float:
+    """Lame lambda = rho (vp^2 - 2 vs^2) of the model at the source cell, as
+    the e_cyl_iso program passes it to the Kurkjian wavelet."""
+    c = int(shot.src_idx[0])
+    vp, vs, rho = (float(a.reshape(-1)[c]) for a in (shot.vp, shot.vs, shot.rho))
+    return rho * (vp * vp - 2.0 * vs * vs)
+
+
+def cpu_baseline_cyl(shot, nsteps_sample: int) -> dict:
+    """Oracle (FP64 C restatement of e_cyl_iso, one thread) on the same
+    borehole model; time per step from the difference of two runs."""
+    from oracle import pyoracle as O
+    cfg = dict(nx=shot.nx, nz=shot.nz, np=shot.np, dx=shot.dr, dz=shot.dr, dt=shot.dt, n=shot.mode,
+               alpha_max=np.pi * shot.f0)
+    fct = O.kurkjian_fct(source_lambda(shot), shot.dr, shot.dr, shot.src_r, shot.mode, 1e-6,
+                         shot.f0, shot.dt)
+    L = np.array([len(fct)], np.uintp)
+    times = []
+    for n in (2, 2 + nsteps_sample):
+        t0 = time.perf_counter()
+        O.cyl_run(cfg, shot.vp, shot.vs, shot.rho, np.array([12], np.intc), shot.src_idx, L, fct,
+                  shot.rec_idx[:1], shot.rec_comp[:1], n, want_fields=False)
+        times.append(time.perf_counter() - t0)
+    per_step = max(times[1] - times[0], 1e-9) / nsteps_sample
+    return {"value": shot.cells / per_step / 1e9, "unit": "Gcell-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{shot.vp.shape[0]}x{shot.vp.shape[1]} padded borehole model, {nsteps_sample} "
+                      f"time steps (difference of a {2 + nsteps_sample}-step and a 2-step run), FP64, "
+                      f"one thread"}
+
+
+def main_cyl(args) -> None:
+    """The e_cyl_iso solver on the seeded borehole model (1000 x 4000 cells
+    of 1 cm plus a 20-cell CPML, water column of 10 cells, 8 kHz Kurkjian
+    monopole on the axis, 2000 steps); one bench step is one shot."""
+    from seismod_amd import synth
+    from seismod_amd.solver import Context, CylConfig, Cylindrical, kurkjian_fct
+    if args.storage not in ("f32", "f64"):
+        print("bench.py: e_cyl_iso stores FP32 or FP64", file=sys.stderr)
+        sys.exit(2)
+    nsteps = args.nsteps or 2000
+    shot = synth.cyl_borehole_shot(nsteps=nsteps)
+    ctx = Context(0)
+
+    def make(storage):
+        cfg = CylConfig(nx=shot.nx, nz=shot.nz, np=shot.np, dr=shot.dr, dz=shot.dr, dt=shot.dt,
+                        f0=shot.f0, mode=shot.mode, storage=storage)
+        sim = Cylindrical(ctx, cfg, shot.vp, shot.vs, shot.rho)
+        fct = kurkjian_fct(source_lambda(shot), shot.dr, shot.dr, shot.src_r, shot.mode, 1e-6,
+                           shot.f0, shot.dt)
+        sim.set_sources(np.array([12], np.intc), shot.src_idx, np.array([len(fct)], np.uintp), fct)
+        sim.set_receivers(shot.rec_idx, shot.rec_comp, nsteps)
+        return sim
+
+    sim = make(args.storage)
+
+    def one_shot():
+        sim.reset()
+        sim.run(nsteps)
+        return sim.traces(0, nsteps)
+
+    for _ in range(args.warmup):
+        one_shot()
+    ctx.sync()
+    sim.set_timing(True)
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        tr = one_shot()
+        lat.append(time.perf_counter() - t0)
+    ctx.sync()
+    wall = time.perf_counter() - t_start
+    kms, timed = sim.timing()
+    sim.close()
+    if not np.isfinite(tr).all() or np.abs(tr).max() == 0.0:
+        print("bench.py: traces are not finite or all zero", file=sys.stderr)
+        sys.exit(1)
+    nx2, nz2 = shot.vp.shape
+    elt = 4 if args.storage == "f32" else 8
+    alg_b = synth.algorithmic_bytes_cyl(nx2, nz2, shot.np, elt=elt)
+    avg_ms = kms / max(timed, 1)
+    achieved = alg_b * shot.cells / (avg_ms * 1e-3) / 1e9
+    line = {
+        "metric": "Gcell-updates/s & %HBM roofline, 2.5D cylindrical elastic CPML (e_cyl_iso)",
+        "value": round(shot.cells * nsteps * args.steps / wall / 1e9, 3),
+        "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.storage == "f32" else "f64",
+        "data": "synthetic (seeded borehole model: water column in a bedded formation)",
+        "config": {"workload": f"e_cyl_iso {shot.nx}x{shot.nz} + CPML {shot.np}, dr = dz = "
+                               f"{shot.dr} m, mode {shot.mode}, Kurkjian {shot.f0:g} Hz, {nsteps} "
+                               f"time steps per shot, {len(shot.rec_idx)} receivers",
+                   "padded_grid": [nx2, nz2], "time_steps_per_shot": nsteps, "dt_s": shot.dt,
+                   "parallelism": "single GPU"},
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+        "roofline": {"bound": "hbm", "kernel": "cyl_velocity + cyl_stress (one step)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_cell": round(alg_b, 3),
+                     "kernel_ms_avg": round(avg_ms, 5), "launches_timed": timed},
+        "cpu_baseline": None,
+    }
+    if not args.no_fp64_check and args.storage == "f32":
+        ref = make("f64")
+        ref.run(nsteps)
+        tr64 = ref.traces(0, nsteps)
+        ref.close()
+        e = float(np.linalg.norm(tr - tr64) / np.linalg.norm(tr64))
+        line["error_vs_fp64"] = {"traces_rel_l2": e, "against": "FP64 run of the same shot on the "
+                                 "GPU (the FP64 path equals the FP64 oracle bit for bit)",
+                                 "traces": len(shot.rec_idx), "samples": nsteps,
+                                 "tolerance": ERROR_TOL["f32"]}
+        if e > ERROR_TOL["f32"]:
+            print(json.dumps(line), flush=True)
+            print(f"bench.py: trace error {e:.3e} against the FP64 run exceeds "
+                  f"{ERROR_TOL['f32']}", file=sys.stderr)
+            sys.exit(1)
+    ctx.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_cyl(shot, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(shots would shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "e_cyl_iso":
+        main_cyl(args)
+        return
+    if args.storage == "f64":
+        print("bench.py: the a_iso_3d bench stores f32, bf16 or f16", file=sys.stderr)
         sys.exit(2)
     from seismod_amd import synth
     from seismod_amd.solver import A3dConfig, Acoustic3D, Context, source_functions
@@ -189,7 +327,7 @@ def main():
                                   vmax=4500.0 if args.n <= 256 else 5000.0)
     ctx = Context(0)
     cfg = A3dConfig(nx=shot.nx, ny=shot.ny, nz=shot.nz, np=shot.np, dx=shot.dx, dy=shot.dx,
-                    dz=shot.dx, dt=shot.dt, f0=shot.f0, storage=args.storage)
+                    dz=shot.dx, dt=shot.dt, f0=shot.f0, storage=args.storage, mat16=args.mat16)
     sim = Acoustic3D(ctx, cfg, shot.vp, shot.rho)
     lengths, fct = source_functions([1.0], [shot.f0], shot.dt)
     sim.set_sources(shot.src_idx, lengths, fct)
@@ -232,10 +370,11 @@ def main():
     value = cells * nsteps * args.steps / wall / 1e9
     nx2, ny2, nz2 = shot.shape
     sb = STATE_BYTES[args.storage]
-    alg_b = synth.algorithmic_bytes_a3d(nx2, ny2, nz2, shot.np, state_bytes=sb, mat_bytes=4)
+    alg_b = synth.algorithmic_bytes_a3d(nx2, ny2, nz2, shot.np, state_bytes=sb,
+                                        mat_bytes=2 if args.mat16 else 4)
     avg_ms = kms / max(launches, 1)
     achieved = alg_b * cells / (avg_ms * 1e-3) / 1e9
-    kname = f"a3d_fused_{args.storage}_{args.n}"
+    kname = f"a3d_fused_{args.storage}{'_mat16' if args.mat16 else ''}_{args.n}"
     tj, ch = sim.launch()
     variant = sim.variant()
     line = {
@@ -249,11 +388,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.storage == "f32" else f"f32 compute, {args.storage} storage",
+        "dtype": "f32" if args.storage == "f32" else
+                 f"f32 compute, {args.storage} storage" + (", fp16 material" if args.mat16 else ""),
         "data": "synthetic (seeded layered model, Gardner density)",
         "config": {
             "workload": f"a_iso_3d {args.n}^3 + CPML {shot.np}, {nsteps} time steps per shot, "
-                        f"{len(shot.rec_idx)} Vz receivers, {args.storage} storage",
+                        f"{len(shot.rec_idx)} Vz receivers, {args.storage} storage"
+                        + (", FP16 material" if args.mat16 else ""),
             "padded_grid": [nx2, ny2, nz2],
             "time_steps_per_shot": nsteps,
             "dt_s": shot.dt,

@@ -158,6 +158,14 @@ typedef struct {
                                  stored values (deterministic pseudo-random
                                  bits from element index and step) instead
                                  of round-to-nearest-even */
+#define SM_A3D_MAT16 8        /* BF16/FP16 storage without SM_A3D_SROUND: the
+                                 material (rho, K*dt) is held as FP16 with
+                                 one power-of-two scale per field (4 instead
+                                 of 8 bytes per cell; values rounded once from
+                                 FP64 to 11 significant bits; a field whose
+                                 nonzero magnitudes span more than 2^28 is
+                                 refused); tile variants only, rows_per_block
+                                 4 or 8 (no counterpart in the reference) */
 
 /* Build a solver from the padded model as read_model_a leaves it
  * (io_utils.c:1029-1153): vp and rho hold nx2*ny2*nz2 values with the
@@ -321,6 +329,14 @@ int sm_e2d_batch_run(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots,
 int sm_e2d_batch_set_variant(sm_e2d_batch *b, int variant);
 int sm_e2d_batch_set_timing(sm_e2d_batch *b, int on);
 int sm_e2d_batch_get_timing(const sm_e2d_batch *b, double *ms_total, size_t *launches);
+/* Shot grouping of sm_e2d_batch_run (no counterpart in the reference): with
+ * bytes > 0 the shots, longest run first, are packed into groups whose
+ * wavefield, material and damping memory stays within bytes (a larger shot
+ * is a group of its own), and each group runs all its steps before the
+ * next starts, so its fields stay in the last-level cache between steps.
+ * 0 (the default) advances every shot in every step launch.  Results do
+ * not depend on it. */
+int sm_e2d_batch_set_group_bytes(sm_e2d_batch *b, size_t bytes);
 /* Allocation-free variant of sm_e2d_batch_run.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory
  * (serving loops and harnesses that forbid hipMalloc; it synchronises the
@@ -388,6 +404,11 @@ int sm_kurkjian_fct(double lambda, double dr, double dz, double r, int mode, dou
 int sm_cyl_set_receivers(sm_cyl *s, size_t nrec, const size_t *idx, const int *comp,
                          size_t max_steps);
 int sm_cyl_run(sm_cyl *s, size_t nsteps);
+/* Per-step timing (no counterpart in the reference): when on, HIP events on
+ * the solver's stream bracket the two update launches of every step;
+ * totals accumulate until the next sm_cyl_set_timing. */
+int sm_cyl_set_timing(sm_cyl *s, int on);
+int sm_cyl_get_timing(const sm_cyl *s, double *ms_total, size_t *steps);
 int sm_cyl_current_step(const sm_cyl *s, size_t *it);
 int sm_cyl_get_traces(sm_cyl *s, size_t it0, size_t count, double *out);
 /* padded nx2*nz2 field of comp (SM_VR ... SM_TTZ, SM_VZ, SM_TZZ) */

@@ -19,6 +19,7 @@ SM_SX, SM_SY, SM_SZ, SM_SXY, SM_SXZ, SM_SYZ, SM_BULK, SM_SF = range(8)
 SM_A3D_NAIVE = 1
 SM_A3D_SROUND = 2
 SM_A3D_PAD64 = 4
+SM_A3D_MAT16 = 8
 SM_A3D_CELL = 0
 SM_A3D_CELL2 = 2
 SM_A3D_TILE = 5
@@ -120,6 +121,7 @@ _SIGS = {
     "sm_e2d_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "sm_e2d_batch_destroy": (C.c_int, [C.c_void_p]),
     "sm_e2d_batch_set_graph_steps": (C.c_int, [C.c_void_p, C.c_int]),
+    "sm_e2d_batch_set_group_bytes": (C.c_int, [C.c_void_p, C.c_size_t]),
     "sm_e2d_batch_run": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p), _size_p]),
     "sm_e2d_batch_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sm_e2d_batch_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),
@@ -133,6 +135,8 @@ _SIGS = {
                                   C.c_double, C.c_double, C.c_double, C.c_size_t, _dbl_p]),
     "sm_cyl_set_receivers": (C.c_int, [C.c_void_p, C.c_size_t, _size_p, _int_p, C.c_size_t]),
     "sm_cyl_run": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "sm_cyl_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "sm_cyl_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),
     "sm_cyl_current_step": (C.c_int, [C.c_void_p, _size_p]),
     "sm_cyl_get_traces": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, _dbl_p]),
     "sm_cyl_get_field": (C.c_int, [C.c_void_p, C.c_int, _dbl_p]),

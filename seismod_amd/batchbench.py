@@ -31,6 +31,9 @@ def main(argv=None) -> None:
                     help="multiply every shot's step count (shorter runs for profiling)")
     ap.add_argument("--graph-steps", type=int, default=32)
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--group-mb", default="0",
+                    help="comma list of shot-group budgets in MiB for the batched run "
+                         "(0: all shots in every launch); each is timed, the best is reported")
     args = ap.parse_args(argv)
 
     from seismod_amd import autotune, synth
@@ -55,16 +58,25 @@ def main(argv=None) -> None:
 
     batch = E2dBatch(ctx, graph_steps=args.graph_steps)
     batch.set_variant(batch_variant)
-    for s in sims:
-        s.reset()
-    batch.run(sims, [2] * len(sims))          # warm-up: graphs, code objects
-    for s in sims:
-        s.reset()
-    batch.set_timing(True)
-    t0 = time.perf_counter()
-    batch.run(sims, steps)
-    wall_b = time.perf_counter() - t0
-    kms, launches = batch.timing()
+    groups = {}
+    for mb in [float(x) for x in args.group_mb.split(",") if x]:
+        batch.set_group_bytes(int(mb * 2 ** 20))
+        for s in sims:
+            s.reset()
+        batch.run(sims, [2] * len(sims))          # warm-up: graphs, code objects
+        for s in sims:
+            s.reset()
+        batch.set_timing(True)
+        t0 = time.perf_counter()
+        batch.run(sims, steps)
+        wall = time.perf_counter() - t0
+        kms, launches = batch.timing()
+        batch.set_timing(False)
+        groups[mb] = (wall, kms, launches)
+        print(json.dumps({"group_mb": mb, "wall_s": round(wall, 4),
+                          "gcell_updates_per_s": round(work / wall / 1e9, 2)}), flush=True)
+    best_mb = min(groups, key=lambda k: groups[k][0])
+    wall_b, kms, launches = groups[best_mb]
     batch.close()
     out = {"shots": len(shots), "cell_updates": work,
            "padded_cells_mean": round(sum(s.cells for s in shots) / len(shots)),
@@ -72,7 +84,8 @@ def main(argv=None) -> None:
            "batched": {"wall_s": round(wall_b, 4), "gcell_updates_per_s": round(work / wall_b / 1e9, 2),
                        "launches": launches, "kernel_ms_per_launch": round(kms / max(launches, 1), 4),
                        "graph_steps": args.graph_steps, "rows_per_chunk": batch_chunk,
-                       "kernel_variant": batch_variant},
+                       "kernel_variant": batch_variant, "group_mb": best_mb,
+                       "by_group_mb": {str(k): round(work / v[0] / 1e9, 2) for k, v in groups.items()}},
            "alg_bytes_per_cell": round(sum(synth.algorithmic_bytes_e2d(sh.nx, sh.nz, sh.npml) * sh.cells * n
                                            for sh, n in zip(shots, steps)) / work, 3)}
     if not args.no_sequential:

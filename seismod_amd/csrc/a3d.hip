@@ -170,6 +170,8 @@ struct sm_a3d {
     int tj = 4, nchunks = 1;        // fused launch shape
     int variant = SM_A3D_CELL;      // fused kernel variant
     int sr = 0;                     // stochastic rounding of 16-bit stores
+    int m16 = 0;                    // FP16 material (tile kernels, 16-bit state)
+    float sR = 1.f, sK = 1.f;       // its scales: stored value * s = rho, K*dt
     int nblocks = 0;                // persistent grid (0: one block per item)
     int cus = 0;                    // compute units of the device
     // storage scales of the 16-bit formats (P, v, velocity-side and
@@ -233,6 +235,31 @@ int upload_rows(void *dst, const double *src, size_t rows, size_t nz2, size_t pz
     }
     SM_HIP(hipMemcpyAsync(dst, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, st));
     SM_HIP(hipStreamSynchronize(st));
+    return SM_OK;
+}
+
+// Divides x by the power-of-two scale sc (returned) that puts its largest
+// magnitude in [2^14, 2^15): FP16 then holds every value with 11 significant
+// bits down to 2^-14 * sc, so a field whose nonzero magnitudes span more
+// than 2^28 is refused rather than flushed towards zero.
+int fp16_scale(std::vector<double> &x, const char *what, float &sc)
+{
+    double mx = 0.0, mn = 0.0;
+    for (double v : x) {
+        const double a = std::fabs(v);
+        if (!std::isfinite(a)) return fail(SM_ERR_ARG, "16-bit material: %s is not finite", what);
+        mx = std::max(mx, a);
+        if (a > 0.0) mn = mn > 0.0 ? std::min(mn, a) : a;
+    }
+    if (mx == 0.0) { sc = 1.f; return SM_OK; }
+    int e = 0;
+    std::frexp(mx, &e);                       // mx = f * 2^e, f in [0.5, 1)
+    const double scale = std::ldexp(1.0, e - 15);
+    if (mn / scale < std::ldexp(1.0, -14))
+        return fail(SM_ERR_ARG, "16-bit material: %s spans %g to %g, more than FP16 holds at "
+                    "full precision under one scale", what, mn, mx);
+    for (double &v : x) v /= scale;
+    sc = (float)scale;
     return SM_OK;
 }
 
@@ -489,6 +516,7 @@ FusedArgs<S> fused_args(sm_a3d *s)
     a.my = np2 * (unsigned)s->ny2 * (unsigned)s->pz;
     a.mz = a.my + (unsigned)s->nx2 * np2 * (unsigned)s->pz;
     a.mat = (const float *)s->mat.ptr;
+    a.sR = s->sR; a.sK = s->sK;
     a.xpl = (const XPlane *)s->xpl.ptr;
     a.tab = (const float *)s->tables.ptr;
     a.toff_y = 6 * s->nx2;
@@ -537,7 +565,8 @@ int run_fused(sm_a3d *s, size_t nsteps)
         a.gather_blocks = a.gather_row >= 0 ? (int)((s->nrec + per_gather - 1) / per_gather) : 0;
         if ((rc = tm.mark(st, 2 * n)) != SM_OK) return rc;
         if ((rc = variant_tile(s->variant)
-                      ? tile_launch<S>(a, s->tj, variant_split(s->variant), variant_nt(s->variant), st)
+                      ? tile_launch<S>(a, s->tj, variant_split(s->variant), variant_nt(s->variant),
+                                       s->m16 != 0, st)
                       : fused_launch<S>(a, s->tj, variant_pf(s->variant), st)) != SM_OK)
             return fail(rc, "fused launch failed (tj=%d): %s", s->tj,
                         hipGetErrorString(hipGetLastError()));
@@ -628,18 +657,19 @@ int resident_blocks(sm_a3d *s, int tj)
     int per_cu = 0, rc = SM_OK;
     const bool tile = variant_tile(s->variant);
     const bool split = variant_split(s->variant), nt = variant_nt(s->variant), sr = s->sr != 0;
+    const bool m16 = s->m16 != 0;
     const int pf = variant_pf(s->variant);
     switch (s->p.storage) {
     case SM_STORE_F32:
-        rc = tile ? tile_max_blocks_per_cu<float>(tj, split, nt, sr, &per_cu)
+        rc = tile ? tile_max_blocks_per_cu<float>(tj, split, nt, sr, m16, &per_cu)
                   : fused_max_blocks_per_cu<float>(tj, pf, &per_cu);
         break;
     case SM_STORE_BF16:
-        rc = tile ? tile_max_blocks_per_cu<__hip_bfloat16>(tj, split, nt, sr, &per_cu)
+        rc = tile ? tile_max_blocks_per_cu<__hip_bfloat16>(tj, split, nt, sr, m16, &per_cu)
                   : fused_max_blocks_per_cu<__hip_bfloat16>(tj, pf, &per_cu);
         break;
     default:
-        rc = tile ? tile_max_blocks_per_cu<__half>(tj, split, nt, sr, &per_cu)
+        rc = tile ? tile_max_blocks_per_cu<__half>(tj, split, nt, sr, m16, &per_cu)
                   : fused_max_blocks_per_cu<__half>(tj, pf, &per_cu);
         break;
     }
@@ -678,12 +708,13 @@ size_t sm_a3d_bytes_needed(const sm_a3d_params *p, size_t nrec, size_t max_steps
     const size_t e = elt_size(p->storage);
     if (!e) return 0;
     const size_t me = p->storage == SM_STORE_F64 ? 8 : 4;
+    const size_t mat_e = (p->flags & SM_A3D_MAT16) ? 2 : me;
     const size_t nx2 = p->nx + 2 * p->np, ny2 = p->ny + 2 * p->np, nz2 = p->nz + 2 * p->np;
     const size_t nnn = nx2 * ny2 * row_pitch(nz2, *p);
     const size_t halves = uses_fused(*p) ? 2 : 1;
     const size_t slabs = slab_cells(nx2, ny2, row_pitch(nz2, *p), p->np);
-    return e * (halves * 4 * nnn + (halves + 1) * slabs) + me * (2 * nnn + 6 * (nx2 + ny2 + nz2)) +
-           me * nrec * max_steps + 16 * nrec + 4 * nx2;
+    return e * (halves * 4 * nnn + (halves + 1) * slabs) + mat_e * 2 * nnn +
+           me * 6 * (nx2 + ny2 + nz2) + me * nrec * max_steps + 16 * nrec + 4 * nx2;
 }
 
 int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const double *rho,
@@ -698,8 +729,12 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
              "grid cell size must be equal in X and Z (a_iso_3d.c:298-301)");
     const size_t e = elt_size(p.storage);
     SM_CHECK(e != 0, SM_ERR_ARG, "sm_a3d_create: storage %d not supported", p.storage);
-    SM_CHECK((p.flags & ~(SM_A3D_NAIVE | SM_A3D_SROUND | SM_A3D_PAD64)) == 0, SM_ERR_ARG,
-             "sm_a3d_create: unknown flags %#x", p.flags);
+    SM_CHECK((p.flags & ~(SM_A3D_NAIVE | SM_A3D_SROUND | SM_A3D_PAD64 | SM_A3D_MAT16)) == 0,
+             SM_ERR_ARG, "sm_a3d_create: unknown flags %#x", p.flags);
+    SM_CHECK(!(p.flags & SM_A3D_MAT16) ||
+                 ((p.storage == SM_STORE_BF16 || p.storage == SM_STORE_F16) && !(p.flags & SM_A3D_SROUND)),
+             SM_ERR_ARG, "sm_a3d_create: SM_A3D_MAT16 needs BF16 or FP16 storage with "
+             "round-to-nearest stores (no SM_A3D_SROUND)");
     const size_t nx2 = p.nx + 2 * p.np, ny2 = p.ny + 2 * p.np, nz2 = p.nz + 2 * p.np;
     SM_CHECK(nx2 < 65536 && ny2 < 65536 && nz2 < 65536, SM_ERR_ARG, "sm_a3d_create: grid too large");
     const size_t nnn = nx2 * ny2 * nz2;
@@ -735,6 +770,8 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
     s->nnp = nx2 * ny2 * (size_t)s->pz;
     const size_t nnp = s->nnp, pz = (size_t)s->pz;
     s->sr = (p.flags & SM_A3D_SROUND) && (p.storage == SM_STORE_BF16 || p.storage == SM_STORE_F16);
+    s->m16 = (p.flags & SM_A3D_MAT16) != 0;
+    if (s->m16) s->variant = SM_A3D_TILE_SPLIT;   // the cell kernels read FP32 material
     // the fused kernel addresses the four state fields of one buffer and
     // the two material fields with 32-bit byte offsets
     if (s->fused && (4 * nnp * e > (size_t)0xFFFFFF00u || 2 * nnp * 4 > (size_t)0xFFFFFF00u)) {
@@ -757,7 +794,7 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
         rc = dalloc(s->fields[h], 4 * nnp * e);
         if (rc == SM_OK) rc = dalloc(s->memv[h], slab_set * e);
     }
-    if (rc == SM_OK) rc = dalloc(s->mat, 2 * nnp * s->melt);
+    if (rc == SM_OK) rc = dalloc(s->mat, 2 * nnp * (s->m16 ? 2 : s->melt));
     if (rc == SM_OK) rc = dalloc(s->memp, slab_set * e);
     if (rc == SM_OK) rc = dalloc(s->tables, 6 * (nx2 + ny2 + nz2) * s->melt);
     if (rc != SM_OK) { release(s); delete s; return rc; }
@@ -782,6 +819,14 @@ int sm_a3d_create(sm_ctx *ctx, const sm_a3d_params *pp, const double *vp, const 
             rc = upload_rows<double>(s->mat.ptr, rho, rows, nz2, pz, st);
             if (rc == SM_OK)
                 rc = upload_rows<double>((double *)s->mat.ptr + nnp, kdt.data(), rows, nz2, pz, st);
+        } else if (s->m16) {
+            // FP16 with one power-of-two scale per field, rounded once from FP64
+            std::vector<double> r16(rho, rho + nnn);
+            rc = fp16_scale(r16, "rho", s->sR);
+            if (rc == SM_OK) rc = fp16_scale(kdt, "K*dt", s->sK);
+            if (rc == SM_OK) rc = upload_rows<_Float16>(s->mat.ptr, r16.data(), rows, nz2, pz, st);
+            if (rc == SM_OK)
+                rc = upload_rows<_Float16>((_Float16 *)s->mat.ptr + nnp, kdt.data(), rows, nz2, pz, st);
         } else {
             rc = upload_rows<float>(s->mat.ptr, rho, rows, nz2, pz, st);
             if (rc == SM_OK)
@@ -958,6 +1003,8 @@ int sm_a3d_set_launch(sm_a3d *s, int rows_per_block, int x_chunks)
                  rows_per_block == 8, SM_ERR_ARG, "rows_per_block must be 1, 2, 4 or 8");
     SM_CHECK(!variant_nt(s->variant) || rows_per_block >= 4, SM_ERR_ARG,
              "the non-temporal tile variants take rows_per_block 4 or 8");
+    SM_CHECK(!s->m16 || rows_per_block >= 4, SM_ERR_ARG,
+             "16-bit material (SM_A3D_MAT16) takes rows_per_block 4 or 8");
     SM_CHECK(x_chunks >= 1 && x_chunks <= s->nx2, SM_ERR_ARG, "x_chunks must be in [1, nx2]");
     // every chunk must own at least one plane
     const int len = (s->nx2 + x_chunks - 1) / x_chunks;
@@ -973,6 +1020,8 @@ int sm_a3d_set_variant(sm_a3d *s, int variant)
     SM_CHECK(s, SM_ERR_ARG, "sm_a3d_set_variant: NULL solver");
     SM_CHECK(s->fused, SM_ERR_STATE, "sm_a3d_set_variant: only the fused kernel has variants");
     SM_CHECK(variant_known(variant), SM_ERR_ARG, "sm_a3d_set_variant: unknown variant %d", variant);
+    SM_CHECK(!s->m16 || variant_tile(variant), SM_ERR_ARG,
+             "sm_a3d_set_variant: 16-bit material (SM_A3D_MAT16) runs on the tile variants only");
     SM_CHECK(!variant_tile(variant) || s->pz % 8 == 0, SM_ERR_ARG,
              "sm_a3d_set_variant: the tile kernel needs a row pitch of a multiple of 8 cells");
     if (variant == s->variant) return SM_OK;

@@ -68,8 +68,10 @@ struct FusedArgs {
     S *mem_out;
     S *memp;
     unsigned my, mz;
-    // material: rho at 0, K*dt at nnn
+    // material: rho at 0, K*dt at nnn; FP32, or FP16 (tile kernels with
+    // 16-bit material) whose value times sR / sK is the physical one
     const float *mat;
+    float sR, sK;
     // per-plane x constants, nx2 records
     const XPlane *xpl;
     // y and z CPML tables at tab + toff: a, b, 1/kappa, ah, bh, 1/kappa_h
@@ -106,11 +108,17 @@ template <typename S>
 int fused_max_blocks_per_cu(int tj, int pf, int *blocks);
 // the same for the LDS-staged tile kernel (a3d_tile.hip; split: slab
 // groups decided per tile and range of planes; nt: state stores
-// non-temporal, built for tj = 4 and 8); the device row pitch must be a
-// multiple of 8 cells
+// non-temporal, built for tj = 4 and 8; m16: FP16 material, built for
+// 16-bit state, tj = 4 and 8, round-to-nearest stores); the device row
+// pitch must be a multiple of 8 cells
 template <typename S>
-int tile_launch(const FusedArgs<S> &a, int tj, bool split, bool nt, hipStream_t st);
+int tile_launch(const FusedArgs<S> &a, int tj, bool split, bool nt, bool m16, hipStream_t st);
 template <typename S>
-int tile_max_blocks_per_cu(int tj, bool split, bool nt, bool sr, int *blocks);
+int tile_max_blocks_per_cu(int tj, bool split, bool nt, bool sr, bool m16, int *blocks);
+// the FP16-material kernels (a3d_tile_m16.hip)
+template <typename S>
+int tile_launch_m16(const FusedArgs<S> &a, int tj, bool split, bool nt, hipStream_t st);
+template <typename S>
+int tile_max_blocks_per_cu_m16(int tj, bool split, bool nt, bool sr, int *blocks);
 
 }  // namespace sm

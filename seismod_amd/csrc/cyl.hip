@@ -410,6 +410,9 @@ struct sm_cyl {
     size_t nx2 = 0, nz2 = 0, N = 0, elt = 8;
     sm_dbuf fields, mem, mat, tab, src, rcv, trace;
     size_t nsrc = 0, nrec = 0, max_steps = 0, rcv_step0 = 0, step = 0;
+    bool timing = false;            // HIP events around each step's two launches
+    double timed_ms = 0.0;
+    size_t timed_steps = 0;
 };
 
 namespace {
@@ -455,6 +458,16 @@ int upload(void *dst, const double *src, size_t n, hipStream_t st)
     return SM_OK;
 }
 
+// events of a timed run, destroyed on every exit path
+struct StepEvents {
+    std::vector<hipEvent_t> ev;
+    ~StepEvents()
+    {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
 template <typename T>
 int run(sm_cyl *s, size_t nsteps)
 {
@@ -463,17 +476,33 @@ int run(sm_cyl *s, size_t nsteps)
     const dim3 grd((unsigned)((s->nz2 + 63) / 64), (unsigned)((s->nx2 + 3) / 4), 1);
     const int64_t *ridx = (const int64_t *)s->rcv.ptr;
     const int *rcomp = ridx ? (const int *)(ridx + s->nrec) : nullptr;
+    StepEvents te;
+    if (s->timing) {
+        te.ev.assign(2 * nsteps, nullptr);
+        for (auto &e : te.ev) SM_HIP(hipEventCreate(&e));
+    }
     for (size_t k = 0; k < nsteps; ++k) {
         const size_t it = s->step + k;
         CylDev<T> d = view<T>(s, (int64_t)it);
+        if (s->timing) SM_HIP(hipEventRecord(te.ev[2 * k], st));
         cyl_velocity<T><<<grd, blk, 0, st>>>(d);
         cyl_stress<T><<<grd, blk, 0, st>>>(d);
+        if (s->timing) SM_HIP(hipEventRecord(te.ev[2 * k + 1], st));
         if (s->nrec && it >= s->rcv_step0 && it - s->rcv_step0 < s->max_steps)
             cyl_gather<T><<<(unsigned)((s->nrec + 255) / 256), 256, 0, st>>>(
                 d, (int)s->nrec, ridx, rcomp, (T *)s->trace.ptr, (int64_t)(it - s->rcv_step0));
     }
     SM_HIP(hipGetLastError());
     s->step += nsteps;
+    if (s->timing) {
+        SM_HIP(hipStreamSynchronize(st));
+        for (size_t k = 0; k < nsteps; ++k) {
+            float ms = 0.f;
+            SM_HIP(hipEventElapsedTime(&ms, te.ev[2 * k], te.ev[2 * k + 1]));
+            s->timed_ms += ms;
+            s->timed_steps += 1;
+        }
+    }
     return SM_OK;
 }
 
@@ -752,6 +781,23 @@ int sm_cyl_run(sm_cyl *s, size_t nsteps)
     SM_CHECK(s->step + nsteps < (1ull << 62), SM_ERR_ARG, "sm_cyl_run: step count too large");
     SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
     return s->elt == 8 ? run<double>(s, nsteps) : run<float>(s, nsteps);
+}
+
+int sm_cyl_set_timing(sm_cyl *s, int on)
+{
+    SM_CHECK(s, SM_ERR_ARG, "sm_cyl_set_timing: NULL solver");
+    s->timing = on != 0;
+    s->timed_ms = 0.0;
+    s->timed_steps = 0;
+    return SM_OK;
+}
+
+int sm_cyl_get_timing(const sm_cyl *s, double *ms_total, size_t *steps)
+{
+    SM_CHECK(s && ms_total && steps, SM_ERR_ARG, "sm_cyl_get_timing: NULL argument");
+    *ms_total = s->timed_ms;
+    *steps = s->timed_steps;
+    return SM_OK;
 }
 
 int sm_cyl_current_step(const sm_cyl *s, size_t *it)

@@ -859,6 +859,7 @@ struct sm_e2d_batch {
     sm::E2dRunner runner;
     int graph_steps = 32;
     int variant = SM_E2D_ROWS;
+    size_t group_bytes = 0;         // 0: all shots in every launch
 };
 
 extern "C" {
@@ -878,6 +879,13 @@ int sm_e2d_batch_destroy(sm_e2d_batch *b)
     (void)set_device(b->ctx);
     (void)hipStreamSynchronize(b->ctx->stream);
     delete b;
+    return SM_OK;
+}
+
+int sm_e2d_batch_set_group_bytes(sm_e2d_batch *b, size_t bytes)
+{
+    SM_CHECK(b, SM_ERR_ARG, "sm_e2d_batch_set_group_bytes: NULL batch");
+    b->group_bytes = bytes;
     return SM_OK;
 }
 
@@ -918,12 +926,43 @@ static int batch_shots(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots, con
     return SM_OK;
 }
 
+// Device memory a shot's step kernels touch: wavefields, material, damping.
+static size_t shot_footprint(const sm_e2d *s)
+{
+    return s->fields.bytes + s->mat.bytes + s->damp.bytes;
+}
+
 int sm_e2d_batch_run(sm_e2d_batch *b, size_t nshots, sm_e2d *const *shots, const size_t *nsteps)
 {
     std::vector<E2dShot> d;
     int rc = batch_shots(b, nshots, shots, nsteps, d);
     if (rc != SM_OK) return rc;
-    rc = b->runner.run(d, b->ctx->stream, b->graph_steps);
+    if (b->group_bytes == 0) {
+        rc = b->runner.run(d, b->ctx->stream, b->graph_steps);
+    } else {
+        // longest runs first, packed greedily into groups whose footprint
+        // stays within group_bytes (a larger shot runs alone); each group
+        // runs to its end before the next starts, so its fields stay in
+        // the device's last-level cache between steps
+        std::vector<size_t> ord(nshots);
+        for (size_t k = 0; k < nshots; ++k) ord[k] = k;
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](size_t x, size_t y) { return nsteps[x] > nsteps[y]; });
+        std::vector<E2dShot> g;
+        size_t acc = 0;
+        for (size_t q = 0; q <= nshots && rc == SM_OK; ++q) {
+            const size_t fb = q < nshots ? shot_footprint(shots[ord[q]]) : 0;
+            if (!g.empty() && (q == nshots || acc + fb > b->group_bytes)) {
+                rc = b->runner.run(g, b->ctx->stream, b->graph_steps);
+                g.clear();
+                acc = 0;
+            }
+            if (q < nshots) {
+                g.push_back(d[ord[q]]);
+                acc += fb;
+            }
+        }
+    }
     if (rc != SM_OK) return rc;
     SM_HIP(hipStreamSynchronize(b->ctx->stream));
     for (size_t k = 0; k < nshots; ++k) shots[k]->step += nsteps[k];

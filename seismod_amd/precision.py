@@ -2,7 +2,10 @@
 (power-of-two per-field scales) wavefield storage against FP64.
 
     python -m seismod_amd.precision [--n 512] [--nsteps 2000] [--mid 1000]
-                                    [--storages f32,bf16,f16,bf16-sr,f16-sr]
+                                    [--storages f32,bf16,f16,bf16-sr,f16-sr,bf16-m16]
+
+(a mode "-sr" rounds the 16-bit stores stochastically, "-m16" also stores
+the material as FP16 with power-of-two scales)
 
 Workload (seismod_amd.synth.a3d_layered_shot, seed 3): n^3 cells of 10 m,
 4-8 layers with Vp in [1500, 5000] m/s plus a Gaussian-smoothed (8 cells)
@@ -50,7 +53,7 @@ def run_mode(ctx, shot, mode: str, mid: int, tune: bool):
     storage = mode.split("-")[0]
     cfg = A3dConfig(nx=shot.nx, ny=shot.ny, nz=shot.nz, np=shot.np, dx=shot.dx, dy=shot.dx,
                     dz=shot.dx, dt=shot.dt, f0=shot.f0, vmax=5000.0, storage=storage,
-                    sround=mode.endswith("-sr"))
+                    sround=mode.endswith("-sr"), mat16=mode.endswith("-m16"))
     s = Acoustic3D(ctx, cfg, shot.vp, shot.rho)
     lengths, fct = source_functions([1.0], [shot.f0], shot.dt)
     s.set_sources(shot.src_idx, lengths, fct)

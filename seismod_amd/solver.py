@@ -143,6 +143,7 @@ class A3dConfig:
     naive: bool = False
     sround: bool = False      # BF16/FP16: stochastic rounding of stored values
     pad64: bool = False       # rows always padded to 64 cells; else when <= 5% extra
+    mat16: bool = False       # BF16/FP16: material as FP16 with power-of-two scales (tile variants)
 
     def params(self) -> L.A3dParams:
         return L.A3dParams(self.nx, self.ny, self.nz, self.np, self.dx, self.dy, self.dz,
@@ -150,7 +151,8 @@ class A3dConfig:
                            self.alpha_max, self.f0, _STORAGE[self.storage],
                            (L.SM_A3D_NAIVE if self.naive else 0)
                            | (L.SM_A3D_SROUND if self.sround else 0)
-                           | (L.SM_A3D_PAD64 if self.pad64 else 0))
+                           | (L.SM_A3D_PAD64 if self.pad64 else 0)
+                           | (L.SM_A3D_MAT16 if self.mat16 else 0))
 
     @property
     def shape(self) -> tuple[int, int, int]:
@@ -422,6 +424,14 @@ class E2dBatch:
             raise ValueError("one step count per shot")
         check(lib.sm_e2d_batch_run(self.handle, len(shots), arr, _sp(steps)), "sm_e2d_batch_run")
 
+    def set_group_bytes(self, nbytes: int) -> None:
+        """Run the shots in groups whose wavefield/material memory stays
+        within nbytes, one group after another (0: all shots in every step
+        launch).  Identical results; a group that fits the last-level cache
+        keeps its fields there between steps."""
+        check(lib.sm_e2d_batch_set_group_bytes(self.handle, int(nbytes)),
+              "sm_e2d_batch_set_group_bytes")
+
     def set_variant(self, variant: str) -> None:
         """Kernel form for the batch: "rows" or "staged" (identical results)."""
         check(lib.sm_e2d_batch_set_variant(self.handle, Elastic2D.E2D_VARIANTS[variant]),
@@ -535,6 +545,16 @@ class Cylindrical:
 
     def run(self, nsteps: int) -> None:
         check(lib.sm_cyl_run(self.handle, nsteps), "sm_cyl_run")
+
+    def set_timing(self, on: bool) -> None:
+        """HIP events around the two update launches of every step."""
+        check(lib.sm_cyl_set_timing(self.handle, 1 if on else 0), "sm_cyl_set_timing")
+
+    def timing(self) -> tuple[float, int]:
+        """(milliseconds, steps) timed since set_timing(True)."""
+        ms, n = C.c_double(), C.c_size_t()
+        check(lib.sm_cyl_get_timing(self.handle, C.byref(ms), C.byref(n)), "sm_cyl_get_timing")
+        return ms.value, n.value
 
     @property
     def step(self) -> int:

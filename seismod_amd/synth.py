@@ -7,7 +7,10 @@ from a seed so runs are reproducible:
     rho = 310 Vp^0.25, padded into the CPML by edge replication exactly as
     read_model_a does (io_utils.c:1089-1143);
   - e_iso 400x400 two-layer model with a surface receiver line;
-  - batches of e_iso shots of random size, layering, frequency and length.
+  - batches of e_iso shots of random size, layering, frequency and length;
+  - an e_cyl_iso borehole: fluid column around the axis inside a bedded
+    formation, Kurkjian monopole source on the axis, receiver array in the
+    fluid.
 """
 from __future__ import annotations
 
@@ -256,3 +259,86 @@ def main(argv=None) -> None:
 
 if __name__ == "__main__":
     main()
+
+
+@dataclass
+class CylShot:
+    nx: int
+    nz: int
+    np: int
+    dr: float
+    dt: float
+    f0: float
+    mode: int
+    nsteps: int
+    vp: np.ndarray          # padded (nx + np, nz + 2 np), row i at r = i dr, FP64
+    vs: np.ndarray
+    rho: np.ndarray
+    src_idx: np.ndarray     # one Kurkjian source on the axis
+    src_r: float
+    rec_idx: np.ndarray
+    rec_comp: np.ndarray
+
+    @property
+    def cells(self) -> int:
+        return int(self.vp.size)
+
+
+def cyl_borehole_shot(nx: int = 1000, nz: int = 4000, np_: int = 20, dr: float = 0.01,
+                      seed: int = 5, mode: int = 0, f0: float = 8000.0, nsteps: int = 2000,
+                      borehole_cells: int = 10, rec_every: int = 20) -> CylShot:
+    """e_cyl_iso borehole model: water (1500 m/s, 1000 kg/m^3) for r below
+    `borehole_cells` cells, a formation of 6-12 beds along z with Vp ~
+    U[2500, 4500], Vs = Vp / U[1.7, 1.9], rho ~ U[2100, 2600]; a Kurkjian
+    source on the axis at mid-depth, trr receivers in the fluid (row 1)
+    every `rec_every` cells and a few Vr / Vz receivers in the formation.
+    Padded like read_model_e_cyl: np extra rows at large r, np rows at each
+    z end, copied from the edge."""
+    rng = np.random.default_rng(seed)
+    nbeds = int(rng.integers(6, 13))
+    tops = np.sort(rng.choice(np.arange(1, nz), size=nbeds - 1, replace=False))
+    bed = np.searchsorted(tops, np.arange(nz), side="right")
+    vpb = rng.uniform(2500.0, 4500.0, size=nbeds)
+    vsb = vpb / rng.uniform(1.7, 1.9, size=nbeds)
+    rhob = rng.uniform(2100.0, 2600.0, size=nbeds)
+    vp = np.repeat(vpb[bed][None, :], nx, axis=0)
+    vs = np.repeat(vsb[bed][None, :], nx, axis=0)
+    rho = np.repeat(rhob[bed][None, :], nx, axis=0)
+    vp[:borehole_cells] = 1500.0
+    vs[:borehole_cells] = 0.0
+    rho[:borehole_cells] = 1000.0
+    nx2, nz2 = nx + np_, nz + 2 * np_
+
+    def pad(a):
+        out = np.empty((nx2, nz2))
+        out[:nx, np_:np_ + nz] = a
+        out[:nx, :np_] = a[:, :1]
+        out[:nx, np_ + nz:] = a[:, -1:]
+        out[nx:, :] = out[nx - 1:nx, :]
+        return out
+
+    vp, vs, rho = pad(vp), pad(vs), pad(rho)
+    dt = 0.6 * 0.5 * (dr + dr) / (np.sqrt(2.0) * vp.max())
+    zc = np_ + nz // 2
+    src_idx = np.array([0 * nz2 + zc], dtype=np.uintp)
+    TRR, VR, VZ_ = 15, 13, 2
+    rec = [(1 * nz2 + j, TRR) for j in range(np_, np_ + nz, rec_every)]
+    for i in (borehole_cells + 5, nx // 4, nx // 2):
+        rec += [(i * nz2 + zc + 200, VR), (i * nz2 + zc - 200, VZ_)]
+    return CylShot(nx=nx, nz=nz, np=np_, dr=dr, dt=float(dt), f0=f0, mode=mode, nsteps=nsteps,
+                   vp=vp, vs=vs, rho=rho, src_idx=src_idx, src_r=0.0,
+                   rec_idx=np.array([r[0] for r in rec], dtype=np.uintp),
+                   rec_comp=np.array([r[1] for r in rec], dtype=np.intc))
+
+
+def algorithmic_bytes_cyl(nx2: int, nz2: int, np_: int, elt: int = 4) -> float:
+    """Minimal HBM bytes per padded cell per step of the e_cyl_iso update.
+
+    The nine fields (v_r, v_theta, v_z and six stresses) read and written
+    once and the eight material arrays (lambda, lambda+2mu, three mu and
+    three buoyancy averages) read once: 26 values; plus the CPML memory
+    variables, read and written once where they exist (18 arrays over the
+    np+1 outer rows, 6 over the 2np+1 slots of every row)."""
+    cells = nx2 * nz2
+    mem = (18 * (np_ + 1) * nz2 + 6 * nx2 * (2 * np_ + 1)) if np_ else 0
+    return elt * (26.0 + 2.0 * mem / cells)

@@ -130,3 +130,26 @@ def test_cyl_cfl_and_material_errors(ctx):
     with pytest.raises(SeismodError) as e:
         Cylindrical(ctx, cfg, d["vp"], d["vs"], rho)
     assert e.value.status == SM_ERR_ARG
+
+
+def test_cyl_timing_counts_steps_and_leaves_results(ctx):
+    """Per-step timing brackets every step and changes no value."""
+    from seismod_amd.solver import CylConfig, Cylindrical
+    d = deck(n=1, seed=11, src=((FR, 3, 20),), nsteps=60)
+    tr0, f0 = run_gpu(ctx, d, "f32")
+    c = d["cfg"]
+    cfg = CylConfig(nx=c["nx"], nz=c["nz"], np=c["np"], dr=c["dx"], dz=c["dz"], dt=c["dt"],
+                    f0=d["f0"], mode=c["n"], kappa_max=c["kappa_max"], alpha_max=d["alpha_max"],
+                    storage="f32")
+    s = Cylindrical(ctx, cfg, d["vp"], d["vs"], d["rho"])
+    s.set_sources(d["types"], d["idx"], d["L"], d["fct"])
+    s.set_receivers(d["ridx"], d["rcomp"], d["nsteps"])
+    s.set_timing(True)
+    s.run(25)
+    s.run(35)
+    ms, n = s.timing()
+    assert n == 60 and ms > 0
+    assert np.array_equal(s.traces(0, d["nsteps"]), tr0)
+    s.set_timing(False)
+    assert s.timing() == (0.0, 0)
+    s.close()

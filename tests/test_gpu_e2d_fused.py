@@ -167,6 +167,45 @@ def test_batch_equals_alone(ctx):
         b.close()
 
 
+@pytest.mark.parametrize("group_kib", [1, 300, 900, 1 << 20])
+def test_grouped_batch_equals_alone(ctx, group_kib):
+    """Shots packed into groups by footprint (budgets below every shot, a
+    few shots per group, all shots in one group) and run group after
+    group give the same bits as each shot alone, with and without graph
+    replay, staged and per-lane forms."""
+    from seismod_amd.solver import E2dBatch
+    rng = np.random.default_rng(31)
+    specs = []
+    for k in range(9):
+        nx, nz = int(rng.integers(8, 120)), int(rng.integers(8, 150))
+        specs.append(dict(nx=nx, nz=nz, npml=int(rng.integers(1, 10)), seed=200 + k,
+                          nsteps=int(rng.integers(30, 140)),
+                          src_type=[decks.BULK, decks.SX, decks.SZ, decks.SXZ][k % 4]))
+    dks = [deck(**sp) for sp in specs]
+    alone = []
+    for d in dks:
+        s = make(ctx, d, launch=(2, 0))
+        s.run(d["nsteps"])
+        alone.append(outputs(s))
+        s.close()
+    shots = [make(ctx, d, launch=(2, 0)) for d in dks]
+    for gs, variant in ((0, "rows"), (8, "staged")):
+        for s in shots:
+            s.reset()
+        b = E2dBatch(ctx, graph_steps=gs)
+        b.set_variant(variant)
+        b.set_group_bytes(group_kib * 1024)
+        b.run(shots, [d["nsteps"] for d in dks])
+        b.close()
+        for k, (s, d) in enumerate(zip(shots, dks)):
+            assert s.step == d["nsteps"]
+            tr, f = outputs(s)
+            assert np.array_equal(tr, alone[k][0]), (gs, k)
+            assert np.array_equal(f, alone[k][1]), (gs, k)
+    for s in shots:
+        s.close()
+
+
 def test_batch_argument_errors(ctx):
     from seismod_amd.solver import E2dBatch
     from seismod_amd._lib import SeismodError
