@@ -13,6 +13,8 @@ before the solver's real run.
   3D (Acoustic3D):  kernel variant (cell, cell2, tile, tilesplit, tile_nt, tilesplit_nt)
                     x rows_per_block x x_chunks
   2D (Elastic2D):   rows_per_chunk in {1, 2, 3, 4, 6, 8, 12, 16}
+  2D batches:       rows_per_chunk and kernel form for the whole batch, then
+                    the shot-group budget (tune_e2d_groups)
 """
 from __future__ import annotations
 
@@ -27,7 +29,8 @@ _CACHE: dict[str, tuple] = {}
 
 def _key_a3d(sim) -> str:
     c = sim.cfg
-    return f"a3d:{c.nx + 2 * c.np}x{c.ny + 2 * c.np}x{c.nz + 2 * c.np}:{c.storage}"
+    return (f"a3d:{c.nx + 2 * c.np}x{c.ny + 2 * c.np}x{c.nz + 2 * c.np}:{c.storage}"
+            + (":mat16" if getattr(c, "mat16", False) else ""))
 
 
 def _key_e2d(sim) -> str:
@@ -79,7 +82,7 @@ def tune_a3d(sim, ctx, steps: int = 8, cache_file=None, candidates=None,
             return _CACHE[key]
     nx2 = sim.cfg.nx + 2 * sim.cfg.np
     if candidates is None:
-        candidates = [(tj, ch) for tj in (2, 4, 8) for ch in (2, 3, 4, 6, 8, 12, 16)
+        candidates = [(tj, ch) for tj in (2, 4, 8) for ch in range(2, 17)
                       if ch <= max(1, nx2 // 16)]
     best = None
     for v in variants:
@@ -168,6 +171,32 @@ def tune_e2d_batch(sims, ctx, steps: int = 8, candidates=(1, 2, 3, 4),
         s.reset()
         s.set_launch(best[1], g)
     return best[1], best[2]
+
+
+def tune_e2d_groups(batch, sims, steps: int = 16, candidates_mb=(0, 192, 256, 320, 384)) -> int:
+    """Pick the shot-group budget of an E2dBatch (sm_e2d_batch_set_group_bytes):
+    each candidate runs every shot `steps` steps, group after group, timed
+    on the wall clock (groups also save launches, which kernel timing would
+    not show); applies the fastest and resets the solvers.  Returns bytes."""
+    import time
+    best = None
+    for mb in candidates_mb:
+        nbytes = int(mb * 2 ** 20)
+        batch.set_group_bytes(nbytes)
+        for s in sims:
+            s.reset()
+        batch.run(sims, [2] * len(sims))
+        for s in sims:
+            s.reset()
+        t0 = time.perf_counter()
+        batch.run(sims, [steps] * len(sims))
+        t = time.perf_counter() - t0
+        if best is None or t < best[0]:
+            best = (t, nbytes)
+    for s in sims:
+        s.reset()
+    batch.set_group_bytes(best[1])
+    return best[1]
 
 
 def clear_cache() -> None:

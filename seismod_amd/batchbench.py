@@ -7,8 +7,9 @@ Workload (seismod_amd.synth.e2d_batch_shots): grids nx ~ U{300..1200} by
 nz ~ U{300..800}, 20-cell PML, 3-6 layers (Vp 2000-5000 m/s, Vs = Vp/1.7-1.9,
 rho 1800-2700), Ricker 10-40 Hz, 1000-3000 steps, dt at half the CFL limit,
 FP32.  Set-up (model upload, autotuning of rows per chunk: per shot for
-the plain launches, for the whole batch for the batched run) is outside the
-timed region; each timed run starts from zero state.  Prints one JSON line:
+the plain launches, for the whole batch for the batched run, then of the
+shot-group budget that keeps each group's fields in the last-level cache)
+is outside the timed region; each timed run starts from zero state.  Prints one JSON line:
 aggregate cell-updates per second batched (hipGraph replay) and one shot
 after another with plain launches, and the batch's kernel time per step.
 """
@@ -31,9 +32,10 @@ def main(argv=None) -> None:
                     help="multiply every shot's step count (shorter runs for profiling)")
     ap.add_argument("--graph-steps", type=int, default=32)
     ap.add_argument("--no-sequential", action="store_true")
-    ap.add_argument("--group-mb", default="0",
-                    help="comma list of shot-group budgets in MiB for the batched run "
-                         "(0: all shots in every launch); each is timed, the best is reported")
+    ap.add_argument("--group-mb", default="auto",
+                    help="shot-group budget in MiB for the batched run (0: all shots in every "
+                         "launch), a comma list (each timed, the best reported), or 'auto' "
+                         "(autotune.tune_e2d_groups picks it during set-up)")
     args = ap.parse_args(argv)
 
     from seismod_amd import autotune, synth
@@ -59,7 +61,11 @@ def main(argv=None) -> None:
     batch = E2dBatch(ctx, graph_steps=args.graph_steps)
     batch.set_variant(batch_variant)
     groups = {}
-    for mb in [float(x) for x in args.group_mb.split(",") if x]:
+    if args.group_mb == "auto":
+        mbs = [autotune.tune_e2d_groups(batch, sims) / 2 ** 20]
+    else:
+        mbs = [float(x) for x in args.group_mb.split(",") if x]
+    for mb in mbs:
         batch.set_group_bytes(int(mb * 2 ** 20))
         for s in sims:
             s.reset()
