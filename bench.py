@@ -40,9 +40,10 @@ STATE_BYTES = {"f32": 4, "bf16": 2, "f16": 2}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="a_iso_3d", choices=["a_iso_3d", "e_cyl_iso"],
-                    help="a_iso_3d (the headline, default) or e_cyl_iso (the 2.5D cylindrical "
-                         "elastic solver on a borehole model, FP32 or FP64 storage)")
+    ap.add_argument("--workload", default="a_iso_3d", choices=["a_iso_3d", "e_cyl_iso", "ve_vti_sh_pml"],
+                    help="a_iso_3d (the headline, default), e_cyl_iso (the 2.5D cylindrical "
+                         "elastic solver on a borehole model) or ve_vti_sh_pml (viscoelastic VTI SH, "
+                         "pseudo-spectral); the last two store FP32 or FP64")
     ap.add_argument("--steps", type=int, default=5, help="timed shots")
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
     ap.add_argument("--n", type=int, default=256, help="interior nodes per axis")
@@ -68,6 +69,8 @@ def parse():
 # trace error of the timed storage against the FP64 run of the same shot
 # (relative L2 over all traces); the bench fails above it
 ERROR_TOL = {"f32": 2e-5, "f16": 4e-2, "bf16": 3e-1}
+# FP32 pseudo-spectral SH: FFT rounding over 1000 steps of a 1M-cell grid
+SH_F32_TOL = 1e-5    # measured 1.05e-6 at 1000 steps of the 1000^2 layered shot
 
 
 def cpu_baseline(shot, nsteps_sample: int) -> dict:
@@ -306,6 +309,120 @@ def main_cyl(args) -> None:
     print(json.dumps(line), flush=True)
 
 
+def cpu_baseline_sh(shot, nsteps_sample: int) -> dict:
+    """Oracle (FP64 numpy restatement with pocketfft, one thread) on the same
+    model; time per step from the difference of two runs."""
+    from oracle import oracle_sh as S
+    from oracle import pyoracle as O
+    cfg = dict(nx=shot.nx, nz=shot.nz, np=shot.np, dx=shot.h, dz=shot.h, dt=shot.dt, vmax=5000.0,
+               alpha_max=np.pi * shot.f0, iwipe=1)
+    L, fct = O.source_functions([1.0], [shot.f0], 0.5 * shot.dt)
+    src = [(5, int(shot.src_idx[0]), fct / (shot.h * shot.h))]
+    times = []
+    for n in (1, 1 + nsteps_sample):
+        t0 = time.perf_counter()
+        S.sh_run(cfg, shot.mu, shot.rho, shot.gamma, shot.Q, shot.f, src, shot.rec_idx[:1],
+                 shot.rec_comp[:1], n)
+        times.append(time.perf_counter() - t0)
+    per_step = max(times[1] - times[0], 1e-9) / nsteps_sample
+    return {"value": shot.cells / per_step / 1e9, "unit": "Gcell-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{shot.mu.shape[0]}x{shot.mu.shape[1]} padded model, {nsteps_sample} time steps "
+                      f"(difference of a {1 + nsteps_sample}-step and a 1-step run), FP64 numpy, one thread"}
+
+
+def main_sh(args) -> None:
+    """ve_vti_sh_pml on the seeded layered VTI viscoelastic model (1000 x 1000
+    nodes of 5 m plus a 20-cell CPML, Syz source, 200 Vy receivers, 1000
+    steps); one bench step is one shot."""
+    from seismod_amd import synth
+    from seismod_amd.solver import Context, ShConfig, ViscoelasticSH, source_functions
+    if args.storage not in ("f32", "f64"):
+        print("bench.py: ve_vti_sh_pml stores FP32 or FP64", file=sys.stderr)
+        sys.exit(2)
+    nsteps = args.nsteps or 1000
+    shot = synth.sh_layered_shot(nsteps=nsteps)
+    ctx = Context(0)
+
+    def make(storage):
+        cfg = ShConfig(nx=shot.nx, nz=shot.nz, np=shot.np, dx=shot.h, dz=shot.h, dt=shot.dt, f0=shot.f0,
+                       storage=storage)
+        sim = ViscoelasticSH(ctx, cfg, shot.mu, shot.rho, shot.gamma, shot.Q, shot.f)
+        L, fct = source_functions([1.0], [shot.f0], 0.5 * shot.dt)
+        sim.set_sources(np.array([5], np.intc), shot.src_idx, L, fct / (shot.h * shot.h))
+        sim.set_receivers(shot.rec_idx, shot.rec_comp, nsteps)
+        return sim
+
+    sim = make(args.storage)
+
+    def one_shot():
+        sim.reset()            # zero fields, memory variables and traces
+        sim.run(nsteps)
+        return sim.traces(0, nsteps)
+
+    for _ in range(args.warmup):
+        one_shot()
+    ctx.sync()
+    sim.set_timing(True)
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        tr = one_shot()
+        lat.append(time.perf_counter() - t0)
+    ctx.sync()
+    wall = time.perf_counter() - t_start
+    kms, timed = sim.timing()
+    sim.close()
+    if not np.isfinite(tr).all() or np.abs(tr).max() == 0.0:
+        print("bench.py: traces are not finite or all zero", file=sys.stderr)
+        sys.exit(1)
+    elt = 4 if args.storage == "f32" else 8
+    alg_b = synth.algorithmic_bytes_sh(elt)
+    avg_ms = kms / max(timed, 1)
+    achieved = alg_b * shot.cells / (avg_ms * 1e-3) / 1e9
+    nx2, nz2 = shot.mu.shape
+    line = {
+        "metric": "Gcell-updates/s & %HBM roofline, 2D viscoelastic VTI SH, pseudo-spectral RK4 (ve_vti_sh_pml)",
+        "value": round(shot.cells * nsteps * args.steps / wall / 1e9, 3),
+        "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.storage == "f32" else "f64",
+        "data": "synthetic (seeded layered VTI viscoelastic model)",
+        "config": {"workload": f"ve_vti_sh_pml {shot.nx}x{shot.nz} + CPML {shot.np}, h = {shot.h} m, "
+                               f"Syz {shot.f0:g} Hz, {nsteps} time steps per shot, {len(shot.rec_idx)} receivers",
+                   "padded_grid": [nx2, nz2], "time_steps_per_shot": nsteps, "dt_s": shot.dt,
+                   "parallelism": "single GPU"},
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+        "roofline": {"bound": "hbm", "kernel": "one step (4 RK stages of hipFFT derivatives and pointwise kernels)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_cell": alg_b, "kernel_ms_avg": round(avg_ms, 5),
+                     "launches_timed": timed},
+        "cpu_baseline": None,
+    }
+    if not args.no_fp64_check and args.storage == "f32":
+        ref = make("f64")
+        ref.run(nsteps)
+        tr64 = ref.traces(0, nsteps)
+        ref.close()
+        e = float(np.linalg.norm(tr - tr64) / np.linalg.norm(tr64))
+        line["error_vs_fp64"] = {"traces_rel_l2": e, "against": "FP64 run of the same shot on the GPU "
+                                 "(the FP64 path equals the FP64 oracle to FFT rounding, 1e-15)",
+                                 "traces": len(shot.rec_idx), "samples": nsteps,
+                                 "tolerance": SH_F32_TOL}
+        if e > SH_F32_TOL:
+            print(json.dumps(line), flush=True)
+            print(f"bench.py: trace error {e:.3e} against the FP64 run exceeds {SH_F32_TOL}",
+                  file=sys.stderr)
+            sys.exit(1)
+    ctx.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_sh(shot, max(1, args.cpu_steps // 5))
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -314,6 +431,9 @@ def main():
         sys.exit(2)
     if args.workload == "e_cyl_iso":
         main_cyl(args)
+        return
+    if args.workload == "ve_vti_sh_pml":
+        main_sh(args)
         return
     if args.storage == "f64":
         print("bench.py: the a_iso_3d bench stores f32, bf16 or f16", file=sys.stderr)

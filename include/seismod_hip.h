@@ -55,6 +55,8 @@ extern "C" {
 #define SM_TZZ 6
 #define SM_TXZ 7
 #define SM_P 8
+#define SM_TXY 9
+#define SM_TYZ 10
 #define SM_DIV 11
 #define SM_CURL 12
 #define SM_VR 13
@@ -416,6 +418,62 @@ int sm_cyl_get_field(sm_cyl *s, int comp, double *out);
 int sm_cyl_reset(sm_cyl *s);
 int sm_cyl_device_bytes(const sm_cyl *s, size_t *bytes);
 size_t sm_cyl_bytes_needed(const sm_cyl_params *p, size_t nrec, size_t max_steps);
+
+/* ------------------------------------------------------------------ */
+/* Viscoelastic SH waves in a 2D VTI medium (ve_vti_sh_pml) --          */
+/* pseudo-spectral derivatives (hipFFT), CPML, fourth-order Runge-Kutta; */
+/* replaces ve_vti_sh_pml.c:475-700, propagate.c:540-591, 2280-2510     */
+/* (init_fftw_data_ve_sh, propagateVE_SH_CPML, partial_*_sh), pml.c:    */
+/* 1689-1757 and 1959-2106 (alloc_cpml_ve_sh, compute_cpml), src.c:     */
+/* 145-168 (add_src_ve_sh_rk4)                                          */
+/* ------------------------------------------------------------------ */
+typedef struct sm_sh sm_sh;
+typedef struct {
+    size_t nx, nz;            /* interior nodes (struct grid) */
+    size_t np;                /* CPML layers on every side (nl); forced to 0
+                                 when abs is 0 (io_utils.c:293) */
+    double dx, dz;            /* m */
+    double dt;                /* s */
+    double vmax, pml_order, Rc, kappa_max;   /* struct abs_params */
+    int alpha_max;            /* 0 -> alpha = 0, else alpha_max = pi*f0 */
+    double f0;                /* frequency of source 0 */
+    int abs;                  /* iwipe: 1 absorbing layers, 0 none */
+    int storage;              /* SM_STORE_F64 or SM_STORE_F32 */
+    int flags;                /* 0 */
+} sm_sh_params;
+
+/* Build a solver from the padded model as read_model_ve_sh leaves it
+ * (io_utils.c:1316-1429): mu = Vs0^2 rho 1e-9 (GPa), rho (kg/m^3), gamma,
+ * Q and f hold nx2*nz2 values, nx2 = nx+2np, index i*nz2+j.  Computes the
+ * staggered standard-linear-solid material (ve_vti_sh_pml.c:475-585), the
+ * CPML tables of the three Runge-Kutta stage sets and the FFT plans, and
+ * zeroes the fields.  Needs nx, nz >= 2 and rho, Q, f > 0 (SM_ERR_ARG). */
+int sm_sh_create(sm_ctx *ctx, const sm_sh_params *p, const double *mu, const double *rho,
+                 const double *gamma, const double *Q, const double *f, sm_sh **out);
+int sm_sh_destroy(sm_sh *s);
+/* Sources on padded indices: SM_SXY adds to tau_xy, SM_SYZ to tau_yz,
+ * other types inject nothing (add_src_ve_sh_rk4).  fct holds each
+ * function at half time steps, already divided by dx*dz, concatenated:
+ * sm_compute_src_fct with dt/2 (compute_src_fct(.., dt, 0.5)) then the
+ * 1/(dx dz) of normalize_src_fct (src.c:67-75); step it uses sample
+ * 2 it + 1. */
+int sm_sh_set_sources(sm_sh *s, size_t nsrc, const int *type, const size_t *idx,
+                      const size_t *length, const double *fct);
+/* receivers on padded indices; comp SM_VY, SM_TXY or SM_TYZ */
+int sm_sh_set_receivers(sm_sh *s, size_t nrec, const size_t *idx, const int *comp,
+                        size_t max_steps);
+int sm_sh_run(sm_sh *s, size_t nsteps);
+/* Per-step timing (no counterpart in the reference): HIP events around the
+ * four Runge-Kutta stages of every step. */
+int sm_sh_set_timing(sm_sh *s, int on);
+int sm_sh_get_timing(const sm_sh *s, double *ms_total, size_t *steps);
+int sm_sh_current_step(const sm_sh *s, size_t *it);
+int sm_sh_get_traces(sm_sh *s, size_t it0, size_t count, double *out);
+/* padded nx2*nz2 field: SM_VY, SM_TXY or SM_TYZ */
+int sm_sh_get_field(sm_sh *s, int comp, double *out);
+int sm_sh_reset(sm_sh *s);
+int sm_sh_device_bytes(const sm_sh *s, size_t *bytes);
+size_t sm_sh_bytes_needed(const sm_sh_params *p, size_t nrec, size_t max_steps);
 
 #ifdef __cplusplus
 }

@@ -63,19 +63,21 @@ def build_lib() -> Path:
     LIBDIR.mkdir(exist_ok=True)
     out = LIBDIR / "libseismod_hip.so"
     tmp = out.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
+    rocm_lib = Path(HIPCC).resolve().parent.parent / "lib"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), f"-L{rocm_lib}",
+          "-lhipfft", f"-Wl,-rpath,{rocm_lib}", "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 
 
 def build_host() -> list[Path]:
-    """Drop-in e_iso, a_iso_3d and e_cyl_iso programs linked against the library, and
+    """Drop-in e_iso, a_iso_3d, e_cyl_iso and ve_vti_sh_pml programs linked against the library, and
     the host I/O layer alone as libsm_hostio.so (tested on CPU)."""
     BINDIR.mkdir(exist_ok=True)
     LIBDIR.mkdir(exist_ok=True)
     _run(["gcc", *C_FLAGS, "-fopenmp", "-fPIC", "-shared", f"-I{HOST}", str(HOST / "sm_io.c"),
           "-lm", "-o", str(LIBDIR / "libsm_hostio.so")])
-    programs = ("e_iso", "a_iso_3d", "e_cyl_iso")
+    programs = ("e_iso", "a_iso_3d", "e_cyl_iso", "ve_vti_sh_pml")
     common = sorted(p for p in HOST.glob("*.c") if p.stem not in programs)
     outs = []
     for prog in programs:

@@ -411,6 +411,72 @@ void sm_read_model_e(const char *fn, const sm_grid *g, double *vp, double *vs, d
             }
 }
 
+void sm_read_grid_ve_sh(const char *fn, sm_grid *g)
+{
+    /* io_utils.c:330-354: the keyword compared in lower case */
+    FILE *f = open_or_die(fn);
+    memset(g, 0, sizeof(*g));
+    char type[64];
+    if (fscanf(f, "%zu %zu", &g->nx, &g->nz) != 2 || fscanf(f, "%lf %lf", &g->dx, &g->dz) != 2 ||
+        fscanf(f, "%lf %lf", &g->x0, &g->z0) != 2 || fscanf(f, "%63s", type) != 1)
+        die("Error: something wrong with file %s\n", fn);
+    for (char *c = type; *c; ++c) *c = (char)tolower((unsigned char)*c);
+    if (strcmp(type, "vti_sh_viscoelastic") != 0)
+        die("Error, model file %s not for a viscoelastic VTI medium\n", fn);
+    g->ny = 1;
+    g->dy = g->dx;
+    fclose(f);
+}
+
+void sm_read_model_ve_sh(const char *fn, const sm_grid *g, size_t np, double *mu, double *rho,
+                         double *gamma, double *Q, double *fr)
+{
+    /* io_utils.c:1316-1429: "Vs0 rho gamma Q f" per node, x outer and z
+     * inner, mu = Vs0^2 rho 1e-9 (GPa); one z column (copied to every x)
+     * when nothing can be read where column 1 starts; values copied into the
+     * absorbing layers from the nearest model node.  Any other shortfall is
+     * an error. */
+    FILE *f = open_or_die(fn);
+    char type[64];
+    if (fscanf(f, "%*d %*d") < 0 || fscanf(f, "%*f %*f") < 0 || fscanf(f, "%*f %*f") < 0 ||
+        fscanf(f, "%63s", type) != 1 || strcmp(type, "vti_sh_viscoelastic") != 0)
+        die("Error, model in %s not vti_sh_viscoelastic\n", fn);
+    const size_t nx = g->nx, nz = g->nz, nz2 = nz + 2 * np;
+    const size_t ncell = nx * nz;
+    size_t nv = 0;
+    double *v = read_body(f, fn, 5 * ncell, &nv);
+    fclose(f);
+    int flag1d = 0;
+    if (nv < 5 * ncell) {
+        if (nx > 1 && nv == 5 * nz) flag1d = 1;
+        else die("Error: something wrong with file %s\n", fn);
+    }
+    for (size_t i = 0; i < nx; ++i)
+        for (size_t j = 0; j < nz; ++j) {
+            const double *c = v + 5 * ((flag1d ? 0 : i) * nz + j);
+            const size_t ind = (i + np) * nz2 + j + np;
+            rho[ind] = c[1];
+            mu[ind] = c[0] * c[0] * rho[ind] * 1.e-9;
+            gamma[ind] = c[2];
+            Q[ind] = c[3];
+            fr[ind] = c[4];
+        }
+    free(v);
+    double *a[5] = {mu, rho, gamma, Q, fr};
+    for (size_t i = 0; i < nx; ++i)
+        for (size_t j = 0; j < np; ++j)
+            for (int q = 0; q < 5; ++q) {
+                a[q][(i + np) * nz2 + j] = a[q][(i + np) * nz2 + np];
+                a[q][(i + np) * nz2 + np + nz + j] = a[q][(i + np) * nz2 + np + nz - 1];
+            }
+    for (size_t i = 0; i < np; ++i)
+        for (size_t j = 0; j < nz2; ++j)
+            for (int q = 0; q < 5; ++q) {
+                a[q][i * nz2 + j] = a[q][np * nz2 + j];
+                a[q][(i + nx + np) * nz2 + j] = a[q][(nx + np - 1) * nz2 + j];
+            }
+}
+
 void sm_read_model_a(const char *fn, const sm_grid *g, size_t np, double *vp, double *rho)
 {
     /* io_utils.c:1029-1153 */
@@ -526,7 +592,14 @@ static size_t node(double x, double x0, double d, size_t n, const char *what)
 
 void sm_read_source2d(const char *fn, const sm_grid *g, sm_sources *src)
 {
-    /* io_utils.c:1432-1538, with np = 0 as e_iso uses it */
+    sm_read_source2d_np(fn, g, 0, src);   /* e_iso: np = 0 */
+}
+
+void sm_read_source2d_np(const char *fn, const sm_grid *g, size_t np, sm_sources *src)
+{
+    /* io_utils.c:1432-1538: indices round((x - x0)/dx) + np on the padded
+     * grid of nz + 2 np nodes along z */
+    const size_t nz2 = g->nz + 2 * np;
     FILE *f = open_or_die(fn);
     memset(src, 0, sizeof(*src));
     if (fscanf(f, "%zu", &src->nsrc) != 1) die("Error: something wrong with file %s\n", fn);
@@ -543,9 +616,9 @@ void sm_read_source2d(const char *fn, const sm_grid *g, sm_sources *src)
         if (fscanf(f, "%lf", &s->A) != 1 || fscanf(f, "%lf", &s->f) != 1 ||
             fscanf(f, "%lf %lf", &s->x, &s->z) != 2)
             die("Error: something wrong with file %s\n", fn);
-        size_t i = node(s->x, g->x0, g->dx, g->nx, "source x");
-        size_t j = node(s->z, g->z0, g->dz, g->nz, "source z");
-        s->idx = i * g->nz + j;
+        size_t i = node_padded(s->x, g->x0, g->dx, g->nx, np, "source x");
+        size_t j = node_padded(s->z, g->z0, g->dz, g->nz, np, "source z");
+        s->idx = i * nz2 + j;
         if (src->ntemplate == 9) {
             /* 9-point template (io_utils.c:1494-1535); the reference leaves the
              * type of the extra points unset, here they take the main type */
@@ -559,9 +632,9 @@ void sm_read_source2d(const char *fn, const sm_grid *g, sm_sources *src)
                 e->A = w[q] * A;
                 e->f = s->f;
                 e->x = s->x; e->z = s->z;
-                size_t ii = node(s->x + di[q] * g->dx, g->x0, g->dx, g->nx, "template x");
-                size_t jj = node(s->z + dj[q] * g->dz, g->z0, g->dz, g->nz, "template z");
-                e->idx = ii * g->nz + jj;
+                size_t ii = node_padded(s->x + di[q] * g->dx, g->x0, g->dx, g->nx, np, "template x");
+                size_t jj = node_padded(s->z + dj[q] * g->dz, g->z0, g->dz, g->nz, np, "template z");
+                e->idx = ii * nz2 + jj;
             }
         }
     }
@@ -650,7 +723,7 @@ static void read_output(const char *fn, const sm_grid *g, size_t np, sm_records 
                 exit(1);
             }
             const size_t i = (size_t)fi, j = (size_t)fj, k = (size_t)fk;
-            r->idx = three_d ? (i * ny2 + j) * nz2 + k : i * g->nz + k;
+            r->idx = three_d ? (i * ny2 + j) * nz2 + k : i * nz2 + k;
         } else if (!strcmp(t, "Snapshot") || !strcmp(t, "snapshot") || !strcmp(t, "SNAPSHOT")) {
             r->type = REC_SNAPSHOT;
             if (fscanf(f, "%lf", &r->t0) != 1 || fscanf(f, "%lf", &r->dt) != 1)
@@ -667,6 +740,11 @@ static void read_output(const char *fn, const sm_grid *g, size_t np, sm_records 
 void sm_read_output2d(const char *fn, const sm_grid *g, sm_records *out)
 {
     read_output(fn, g, 0, out, 0);   /* io_utils.c:1592-1682 */
+}
+
+void sm_read_output2d_np(const char *fn, const sm_grid *g, size_t np, sm_records *out)
+{
+    read_output(fn, g, np, out, 0);   /* io_utils.c:1592-1682 on the padded grid */
 }
 
 void sm_read_output3d(const char *fn, const sm_grid *g, size_t np, sm_records *out)

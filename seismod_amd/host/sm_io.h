@@ -90,9 +90,17 @@ int sm_write_model_e(const char *fn, size_t nx, size_t nz, double dx, double dz,
                      double z0, const double *vp, const double *vs, const double *rho, int digits);
 /* read_source / read_source_3d (io_utils.c:1432-1589) */
 void sm_read_source2d(const char *fn, const sm_grid *g, sm_sources *src);
+/* read_source on a grid padded by np nodes on every side (ve_vti_sh_pml) */
+void sm_read_source2d_np(const char *fn, const sm_grid *g, size_t np, sm_sources *src);
 void sm_read_source3d(const char *fn, const sm_grid *g, size_t np, sm_sources *src);
 /* read_output / read_output_3d (io_utils.c:1592-1741) */
 void sm_read_output2d(const char *fn, const sm_grid *g, sm_records *out);
+void sm_read_output2d_np(const char *fn, const sm_grid *g, size_t np, sm_records *out);
+/* read_grid_params_ve_sh / read_model_ve_sh (io_utils.c:330-354, 1316-1429):
+ * padded (nx + 2np) x (nz + 2np) arrays, mu in GPa */
+void sm_read_grid_ve_sh(const char *fn, sm_grid *g);
+void sm_read_model_ve_sh(const char *fn, const sm_grid *g, size_t np, double *mu, double *rho,
+                         double *gamma, double *Q, double *fr);
 void sm_read_output3d(const char *fn, const sm_grid *g, size_t np, sm_records *out);
 /* check_dt_trc (io_utils.c:3561-3574) */
 void sm_check_dt_trc(const sm_records *out, double dt);

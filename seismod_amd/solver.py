@@ -590,3 +590,119 @@ class Cylindrical:
             self.close()
         except Exception:
             pass
+
+
+@dataclass
+class ShConfig:
+    """ve_vti_sh_pml grid: nx by nz interior nodes and np CPML layers on every
+    side (none when abs is False), padded (nx + 2np) by (nz + 2np)."""
+    nx: int
+    nz: int
+    np: int
+    dx: float
+    dz: float
+    dt: float
+    f0: float
+    vmax: float = 5000.0
+    pml_order: float = 2.0
+    Rc: float = 1e-5
+    kappa_max: float = 1.0
+    alpha_max: int = 1
+    abs: bool = True
+    storage: str = "f64"
+
+    def params(self) -> L.ShParams:
+        return L.ShParams(self.nx, self.nz, self.np, self.dx, self.dz, self.dt, self.vmax,
+                          self.pml_order, self.Rc, self.kappa_max, self.alpha_max, self.f0,
+                          1 if self.abs else 0, _STORAGE[self.storage], 0)
+
+    @property
+    def shape(self) -> tuple[int, int]:
+        np_ = self.np if self.abs else 0
+        return (self.nx + 2 * np_, self.nz + 2 * np_)
+
+    def bytes_needed(self, nrec: int = 0, max_steps: int = 0) -> int:
+        p = self.params()
+        return int(lib.sm_sh_bytes_needed(C.byref(p), nrec, max_steps))
+
+
+class ViscoelasticSH:
+    """Viscoelastic SH waves in a 2D VTI medium (ve_vti_sh_pml): hipFFT
+    derivatives, CPML, fourth-order Runge-Kutta."""
+
+    FIELDS = {"vy": L.SM_VY, "txy": L.SM_TXY, "tyz": L.SM_TYZ}
+
+    def __init__(self, ctx: Context, cfg: ShConfig, mu, rho, gamma, Q, f):
+        self.ctx, self.cfg = ctx, cfg
+        n = int(np.prod(cfg.shape))
+        self._m = [_f64(a).reshape(-1) for a in (mu, rho, gamma, Q, f)]
+        if any(a.size != n for a in self._m):
+            raise ValueError(f"model must hold {n} padded cells")
+        self._p = cfg.params()
+        h = C.c_void_p()
+        ctx.adopt(self)
+        check(lib.sm_sh_create(ctx.handle, C.byref(self._p), *(_dp(a) for a in self._m), C.byref(h)),
+              "sm_sh_create")
+        self.handle = h
+        self.nrec = 0
+
+    def set_sources(self, types, idx, lengths, fct) -> None:
+        t = np.ascontiguousarray(types, dtype=np.intc)
+        i = np.ascontiguousarray(idx, dtype=np.uintp)
+        ln = np.ascontiguousarray(lengths, dtype=np.uintp)
+        f = _f64(fct)
+        check(lib.sm_sh_set_sources(self.handle, len(t), t.ctypes.data_as(C.POINTER(C.c_int)), _sp(i),
+                                    _sp(ln), _dp(f)), "sm_sh_set_sources")
+
+    def set_receivers(self, idx, comp, max_steps: int) -> None:
+        i = np.ascontiguousarray(idx, dtype=np.uintp)
+        c = np.ascontiguousarray(comp, dtype=np.intc)
+        check(lib.sm_sh_set_receivers(self.handle, len(i), _sp(i), c.ctypes.data_as(C.POINTER(C.c_int)),
+                                      max_steps), "sm_sh_set_receivers")
+        self.nrec = len(i)
+
+    def run(self, nsteps: int) -> None:
+        check(lib.sm_sh_run(self.handle, nsteps), "sm_sh_run")
+
+    def set_timing(self, on: bool) -> None:
+        check(lib.sm_sh_set_timing(self.handle, 1 if on else 0), "sm_sh_set_timing")
+
+    def timing(self) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_size_t()
+        check(lib.sm_sh_get_timing(self.handle, C.byref(ms), C.byref(n)), "sm_sh_get_timing")
+        return ms.value, n.value
+
+    @property
+    def step(self) -> int:
+        it = C.c_size_t()
+        check(lib.sm_sh_current_step(self.handle, C.byref(it)), "sm_sh_current_step")
+        return it.value
+
+    def traces(self, it0: int, count: int) -> np.ndarray:
+        out = np.zeros((count, self.nrec), dtype=np.float64)
+        check(lib.sm_sh_get_traces(self.handle, it0, count, _dp(out)), "sm_sh_get_traces")
+        return out
+
+    def field(self, name: str) -> np.ndarray:
+        out = np.zeros(self.cfg.shape, dtype=np.float64)
+        check(lib.sm_sh_get_field(self.handle, self.FIELDS[name], _dp(out)), "sm_sh_get_field")
+        return out
+
+    def reset(self) -> None:
+        check(lib.sm_sh_reset(self.handle), "sm_sh_reset")
+
+    def device_bytes(self) -> int:
+        b = C.c_size_t()
+        check(lib.sm_sh_device_bytes(self.handle, C.byref(b)), "sm_sh_device_bytes")
+        return b.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            check(lib.sm_sh_destroy(self.handle), "sm_sh_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -342,3 +342,65 @@ def algorithmic_bytes_cyl(nx2: int, nz2: int, np_: int, elt: int = 4) -> float:
     cells = nx2 * nz2
     mem = (18 * (np_ + 1) * nz2 + 6 * nx2 * (2 * np_ + 1)) if np_ else 0
     return elt * (26.0 + 2.0 * mem / cells)
+
+
+@dataclass
+class ShShot:
+    nx: int
+    nz: int
+    np: int
+    h: float
+    dt: float
+    f0: float
+    nsteps: int
+    mu: np.ndarray          # padded (nx + 2np, nz + 2np), GPa, as read_model_ve_sh leaves it
+    rho: np.ndarray
+    gamma: np.ndarray
+    Q: np.ndarray
+    f: np.ndarray
+    src_idx: np.ndarray     # one Syz source
+    rec_idx: np.ndarray
+    rec_comp: np.ndarray
+
+    @property
+    def cells(self) -> int:
+        return int(self.mu.size)
+
+
+def sh_layered_shot(n: int = 1000, np_: int = 20, h: float = 5.0, seed: int = 7, f0: float = 25.0,
+                    nsteps: int = 1000, nrec: int = 200) -> ShShot:
+    """ve_vti_sh_pml workload: n x n nodes of h metres plus np CPML layers,
+    5-10 horizontal layers with Vs0 ~ U[800, 2500] m/s, rho ~ U[1800, 2600],
+    gamma ~ U[0, 0.25], Q ~ U[20, 100], relaxation frequency f0; a Syz source
+    at (n/2, 20) nodes and a line of nrec Vy receivers at depth 10 nodes.
+    dt at 40% of the RK4 pseudo-spectral stability limit."""
+    rng = np.random.default_rng(seed)
+    nl = int(rng.integers(5, 11))
+    tops = np.sort(rng.choice(np.arange(1, n), size=nl - 1, replace=False))
+    lay = np.searchsorted(tops, np.arange(n), side="right")
+    vs = np.sort(rng.uniform(800.0, 2500.0, nl))[lay] * np.ones((n, 1))
+    rho = rng.uniform(1800.0, 2600.0, nl)[lay] * np.ones((n, 1))
+    gamma = rng.uniform(0.0, 0.25, nl)[lay] * np.ones((n, 1))
+    Q = rng.uniform(20.0, 100.0, nl)[lay] * np.ones((n, 1))
+    pad = lambda a: np.pad(a, np_, mode="edge")
+    mu = pad(vs * vs * rho * 1e-9)
+    vmax = float((vs * np.sqrt(1 + 2 * gamma)).max())
+    dt = 0.4 * 2.0 * h / (np.pi * vmax * np.sqrt(2.0))
+    nz2 = n + 2 * np_
+    src = np.array([(n // 2 + np_) * nz2 + 20 + np_], dtype=np.uintp)
+    xs = np.linspace(np_, n + np_ - 1, nrec).astype(int)
+    rec = np.array([x * nz2 + 10 + np_ for x in xs], dtype=np.uintp)
+    return ShShot(nx=n, nz=n, np=np_, h=h, dt=float(dt), f0=f0, nsteps=nsteps, mu=mu, rho=pad(rho),
+                  gamma=pad(gamma), Q=pad(Q), f=np.full(mu.shape, f0), src_idx=src, rec_idx=rec,
+                  rec_comp=np.full(nrec, 1, dtype=np.intc))
+
+
+def algorithmic_bytes_sh(elt: int = 8) -> float:
+    """HBM bytes per padded cell per step of the ve_vti_sh_pml update, as a
+    streaming minimum for its structure: per Runge-Kutta stage, three FFT
+    derivatives each reading a field and writing its derivative (2 values),
+    the right-hand sides reading the five state values and nine material
+    arrays and writing five rates (19), and the stage update reading W, Ws
+    and the rates and writing W and the next stage's input (25): 50 values
+    per stage, 200 per step."""
+    return elt * 200.0

@@ -477,3 +477,58 @@ def test_read_source_and_records_cyl(tmp_path):
     LIB.sm_read_output_cyl(str(tmp_path / "r.dat").encode(), C.byref(g), C.c_size_t(np_), C.byref(out))
     assert (out.r[0].comp, out.r[0].idx) == (13, 4 * nz2 - 2 + np_)     # z in the top layer
     assert (out.r[1].comp, out.r[1].idx) == (19, 20 * nz2 + 5 + np_)    # r in the outer layer
+
+
+@pytest.mark.parametrize("column", [False, True])
+def test_read_model_ve_sh_layout_and_padding(tmp_path, column):
+    """read_grid_params_ve_sh / read_model_ve_sh (io_utils.c:330-354,
+    1316-1429): "Vs0 rho gamma Q f" with z fastest into the (nx+2np) x
+    (nz+2np) grid, mu = Vs0^2 rho 1e-9; one column fills every x; edges
+    copied into all four layers; the keyword is case-insensitive in the
+    header check only."""
+    rng = np.random.default_rng(6)
+    nx, nz, np_ = 4, 5, 2
+    m = np.stack([rng.uniform(800, 2500, (nx, nz)), rng.uniform(1500, 2700, (nx, nz)),
+                  rng.uniform(0, 0.3, (nx, nz)), rng.uniform(10, 90, (nx, nz)), rng.uniform(5, 50, (nx, nz))])
+    if column:
+        m[:] = m[:, :1, :]
+    fn = tmp_path / "sh.dat"
+    with open(fn, "w") as f:
+        f.write(f"{nx} {nz}\n2.0 2.5\n0.0 0.0\nvti_sh_viscoelastic\n")
+        for i in range(1 if column else nx):
+            for j in range(nz):
+                f.write(" ".join(repr(float(m[q, i, j])) for q in range(5)) + "\n")
+    g = Grid()
+    LIB.sm_read_grid_ve_sh(str(fn).encode(), C.byref(g))
+    assert (g.nx, g.nz, g.dx, g.dz) == (nx, nz, 2.0, 2.5)
+    out = [np.zeros((nx + 2 * np_, nz + 2 * np_)) for _ in range(5)]
+    LIB.sm_read_model_ve_sh(str(fn).encode(), C.byref(g), C.c_size_t(np_),
+                            *(o.ctypes.data_as(C.c_void_p) for o in out))
+    mu = m[0] * m[0] * m[1] * 1e-9
+    for q, want in enumerate([mu, m[1], m[2], m[3], m[4]]):
+        assert np.array_equal(out[q], np.pad(want, np_, mode="edge")), q
+
+
+def test_read_source_and_records_padded_2d(tmp_path):
+    """ve_vti_sh_pml index rules: round((x - x0)/dx) + np along both axes of
+    the padded grid, for sources (1- and 9-point templates) and records
+    (io_utils.c:1488-1535, 1661-1666)."""
+    g = Grid(nx=20, ny=1, nz=30, dx=2.0, dy=2.0, dz=2.0)
+    np_ = 4
+    nz2 = 30 + 2 * np_
+    (tmp_path / "s.dat").write_text("1\n9\nSyz\n2.0\n25.0\n10.2 21.1\n")
+    s = Sources()
+    LIB.sm_read_source2d_np(str(tmp_path / "s.dat").encode(), C.byref(g), C.c_size_t(np_), C.byref(s))
+    assert s.nsrc == 1 and s.ntemplate == 9
+    i0, j0 = 5 + np_, 11 + np_
+    want = [(i0, j0), (i0 - 1, j0 - 1), (i0, j0 - 1), (i0 + 1, j0 - 1), (i0 - 1, j0), (i0 + 1, j0),
+            (i0 - 1, j0 + 1), (i0, j0 + 1), (i0 + 1, j0 + 1)]
+    wts = [0.5, 0.125, 0.25, 0.125, 0.25, 0.25, 0.125, 0.25, 0.125]
+    for q in range(9):
+        assert (s.s[q].type, s.s[q].idx) == (5, want[q][0] * nz2 + want[q][1])
+        assert s.s[q].A == wts[q] * 2.0
+    (tmp_path / "r.dat").write_text("2\nVy\nTrace\n7.1 -3.9 0.0 0.1\ntxy\nTrace\n40.0 10.0 0.0 0.1\n")
+    out = Records()
+    LIB.sm_read_output2d_np(str(tmp_path / "r.dat").encode(), C.byref(g), C.c_size_t(np_), C.byref(out))
+    assert (out.r[0].comp, out.r[0].idx) == (1, (4 + np_) * nz2 - 2 + np_)
+    assert (out.r[1].comp, out.r[1].idx) == (9, (20 + np_) * nz2 + 5 + np_)
