@@ -1,0 +1,385 @@
+// pstd.hpp -- the pseudo-spectral machinery shared by the viscoelastic
+// PSTD solvers (ve_vti_sh_pml, ve_vti_pml): staggered FFT derivatives on
+// the padded nx2 x nz2 grid (z fastest), the CPML stage tables and the
+// layer corrections of the derivatives, applied inside each solver's fused
+// stage kernel.
+//
+// Derivatives (propagate.c:2463-2510): f -> IFFT(H(k) FFT(f)) along one
+// axis, H(k) = i k e^{+-i k d / 2} with the 1/n of the unnormalised inverse
+// folded into k (propagate.c:363-374, 566-577).
+//   x (stride nz2): when nz2 is even, the real field is read as an
+//     nx2 x nz2/2 array of complex numbers, z_j = f(., 2j) + i f(., 2j+1);
+//     one complex FFT along x transforms two real columns at once, the
+//     multiplier is extended to the whole spectrum (H(n - k) = conj H(k),
+//     the real part of H at the Nyquist bin, as a complex-to-real inverse
+//     takes it), and the inverse complex FFT returns D f(., 2j) in the real
+//     and D f(., 2j+1) in the imaginary part.  No real/complex copy passes,
+//     half the transforms.  Odd nz2 falls back to real transforms.
+//   z (unit stride): real-to-complex and complex-to-real transforms.
+// Results equal the real-transform derivatives to rounding.
+//
+// CPML (pml.c:1959-2106): per stage (fp1, fp23, fp4) and axis the rows b,
+// c at nodes, bh, ch at inner half nodes, bH, cH at outer half nodes; the
+// k rows are shared by the stages.  Every row has np + 1 entries; entry np
+// of the node and inner-half rows (read by loops that run to nl = np) takes
+// the row's own formula there (k = 1, d = 0: b = 1, c = 0 for the node row).
+//
+// Layer sets: one memory variable per slot q, mapped to a cell as the
+// reference's region loops map their counter ipml:
+//   mode 0 (x layers, nz2 columns): q < np nz2 -> row nl = q / nz2 at cell
+//     (nl, j); then rows nl <= lastn at cell (nx2 - 1 - nl, j);
+//   mode 1 (z layers, nx2 columns): q < np nx2 -> nl = q / nx2 at (i, nl);
+//     then rows nl <= lastn at (i, nz2 - 1 - nl).
+// At a slot: m1 = b m0 + c d; d <- d / k + m2.  The copies the reference
+// makes between propagations are folded in: m2 <- m1 for q < lim2 (after
+// every propagation) and, in the last stage of a step, m0 <- m1 for
+// q < lim0 (the copy at the start of the next step).
+#pragma once
+
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "sm_internal.hpp"
+
+namespace sm {
+namespace pstd {
+
+template <typename T>
+struct Cx {
+    T re, im;
+};
+
+// CPML table rows per stage and axis, then shared k rows per axis
+enum { T_B, T_C, T_BH, T_CH, T_BHE, T_CHE, T_NST };
+enum { K_K, K_KH, K_KHE, K_NK };
+
+struct Cpml {
+    int abs = 0;
+    double vmax = 0, pml_order = 2, Rc = 1e-5, kappa_max = 1, alpha_max = 0;
+};
+
+struct Grid {
+    sm_ctx *ctx = nullptr;
+    int nx2 = 0, nz2 = 0, np = 0;
+    size_t N = 0, elt = 8;
+    int nxh = 0, nzh = 0;           // rfft bins along x and z
+    bool xpair = false;             // x derivatives by complex transforms of column pairs
+    sm_dbuf spec;                   // spectrum scratch (largest of the three layouts)
+    sm_dbuf kmul;                   // x multipliers (fwd, bwd), then z (fwd, bwd)
+    sm_dbuf tabs;                   // per stage 2 x T_NST rows, then 2 x K_NK rows (np + 1 each)
+    size_t kx_n = 0;                // entries per x multiplier row (nx2 paired, nxh otherwise)
+    hipfftHandle px_f = 0, px_i = 0, pz_f = 0, pz_i = 0;
+};
+
+// one layer set as the stage kernels see it
+template <typename T>
+struct Layer {
+    T *m0, *m1, *m2;
+    const T *b1, *c1, *k1;          // rows nl < np (first side)
+    const T *b2, *c2, *k2;          // rows nl <= lastn (far side)
+    int mode, lastn;
+    unsigned lim2, lim0;
+};
+
+// the derivative value d at cell (i, j) corrected in the layer slots of L
+template <typename T>
+__device__ __forceinline__ T layer_fix(const Layer<T> &L, T d, int i, int j, int nx2, int nz2, int np)
+{
+    int nl, col, cols;
+    bool first;
+    if (L.mode == 0) {
+        cols = nz2; col = j;
+        if (i < np) { first = true; nl = i; }
+        else if (i >= nx2 - 1 - L.lastn) { first = false; nl = nx2 - 1 - i; }
+        else return d;
+    } else {
+        cols = nx2; col = i;
+        if (j < np) { first = true; nl = j; }
+        else if (j >= nz2 - 1 - L.lastn) { first = false; nl = nz2 - 1 - j; }
+        else return d;
+    }
+    const unsigned q = (unsigned)((first ? 0 : np) + nl) * (unsigned)cols + (unsigned)col;
+    const T b = first ? L.b1[nl] : L.b2[nl], c = first ? L.c1[nl] : L.c2[nl];
+    const T k = first ? L.k1[nl] : L.k2[nl];
+    const T m = b * L.m0[q] + c * d;
+    L.m1[q] = m;
+    const T out = d / k + L.m2[q];
+    if (q < L.lim2) L.m2[q] = m;
+    if (q < L.lim0) L.m0[q] = m;
+    return out;
+}
+
+// complex multiply by the multiplier of bin l % m (ROWS false) or l / m
+template <typename T, bool ROWS>
+__global__ void pstd_kmul(Cx<T> *o, const Cx<T> *k, unsigned m, size_t n)
+{
+    const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const Cx<T> a = o[l], b = k[ROWS ? l / m : l % m];
+    o[l] = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+inline unsigned blocks(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+inline hipfftResult exec_r2c(hipfftHandle p, const double *in, void *out)
+{
+    return hipfftExecD2Z(p, const_cast<double *>(in), (hipfftDoubleComplex *)out);
+}
+inline hipfftResult exec_r2c(hipfftHandle p, const float *in, void *out)
+{
+    return hipfftExecR2C(p, const_cast<float *>(in), (hipfftComplex *)out);
+}
+inline hipfftResult exec_c2r(hipfftHandle p, void *in, double *out)
+{
+    return hipfftExecZ2D(p, (hipfftDoubleComplex *)in, out);
+}
+inline hipfftResult exec_c2r(hipfftHandle p, void *in, float *out)
+{
+    return hipfftExecC2R(p, (hipfftComplex *)in, out);
+}
+inline hipfftResult exec_c2c(hipfftHandle p, const double *in, void *out, int dir)
+{
+    return hipfftExecZ2Z(p, (hipfftDoubleComplex *)const_cast<double *>(in), (hipfftDoubleComplex *)out, dir);
+}
+inline hipfftResult exec_c2c(hipfftHandle p, const float *in, void *out, int dir)
+{
+    return hipfftExecC2C(p, (hipfftComplex *)const_cast<float *>(in), (hipfftComplex *)out, dir);
+}
+
+#define PSTD_FFT(call)                                                                   \
+    do {                                                                                 \
+        hipfftResult r_ = (call);                                                        \
+        if (r_ != HIPFFT_SUCCESS) return fail(SM_ERR_HIP, "%s: hipFFT error %d", #call, (int)r_); \
+    } while (0)
+
+// derivative of f along axis (0 x, 1 z), forward (D+) or backward (D-),
+// into out (nx2 x nz2, may not alias f)
+template <typename T>
+int deriv(Grid &g, const T *f, int axis, bool fwd, T *out)
+{
+    hipStream_t st = g.ctx->stream;
+    Cx<T> *spec = (Cx<T> *)g.spec.ptr;
+    const Cx<T> *km = (const Cx<T> *)g.kmul.ptr;
+    if (axis == 0 && g.xpair) {
+        const size_t n = g.N / 2;
+        PSTD_FFT(exec_c2c(g.px_f, f, spec, HIPFFT_FORWARD));
+        pstd_kmul<T, true><<<blocks(n), 256, 0, st>>>(spec, km + (fwd ? 0 : g.kx_n), (unsigned)(g.nz2 / 2), n);
+        PSTD_FFT(exec_c2c(g.px_f, (const T *)spec, out, HIPFFT_BACKWARD));
+    } else if (axis == 0) {
+        const size_t n = (size_t)g.nxh * g.nz2;
+        PSTD_FFT(exec_r2c(g.px_f, f, spec));
+        pstd_kmul<T, false><<<blocks(n), 256, 0, st>>>(spec, km + (fwd ? 0 : g.kx_n), (unsigned)g.nxh, n);
+        PSTD_FFT(exec_c2r(g.px_i, spec, out));
+    } else {
+        const size_t n = (size_t)g.nzh * g.nx2;
+        PSTD_FFT(exec_r2c(g.pz_f, f, spec));
+        pstd_kmul<T, false><<<blocks(n), 256, 0, st>>>(spec, km + 2 * g.kx_n + (fwd ? 0 : g.nzh),
+                                                       (unsigned)g.nzh, n);
+        PSTD_FFT(exec_c2r(g.pz_i, spec, out));
+    }
+    return SM_OK;
+}
+
+// device views of the CPML tables: stage 0 (fp1), 1 (fp23), 2 (fp4); axis 0 x, 1 z
+template <typename T>
+const T *tab(const Grid &g, int stage, int axis, int row)
+{
+    return (const T *)g.tabs.ptr + ((size_t)(stage * 2 + axis) * T_NST + row) * (g.np + 1);
+}
+template <typename T>
+const T *ktab(const Grid &g, int axis, int row)
+{
+    return (const T *)g.tabs.ptr + ((size_t)3 * 2 * T_NST + axis * K_NK + row) * (g.np + 1);
+}
+
+// a layer set: mode, last far-side row, tables (stage, axis, rows) of the
+// first and far sides, memory copies, copy limits
+template <typename T>
+Layer<T> layer(const Grid &g, int stage, T *m0, T *m1, T *m2, int mode, int lastn, int ax1, int b1, int kk1,
+               int ax2, int b2, int kk2, size_t lim2, size_t lim0)
+{
+    Layer<T> L;
+    L.m0 = m0; L.m1 = m1; L.m2 = m2;
+    L.b1 = tab<T>(g, stage, ax1, b1); L.c1 = tab<T>(g, stage, ax1, b1 + 1); L.k1 = ktab<T>(g, ax1, kk1);
+    L.b2 = tab<T>(g, stage, ax2, b2); L.c2 = tab<T>(g, stage, ax2, b2 + 1); L.k2 = ktab<T>(g, ax2, kk2);
+    L.mode = mode; L.lastn = lastn;
+    L.lim2 = (unsigned)lim2; L.lim0 = (unsigned)lim0;
+    return L;
+}
+
+// CPML tables of pml.c:1959-2106 for one axis
+inline void cpml_axis(const Cpml &p, size_t n, double d, double dt, double alpha_max,
+                      std::vector<double> st[3][T_NST], std::vector<double> kk[K_NK])
+{
+    for (int s = 0; s < 3; ++s)
+        for (int r = 0; r < T_NST; ++r) st[s][r].assign(n + 1, 0.0);
+    for (int r = 0; r < K_NK; ++r) kk[r].assign(n + 1, 1.0);
+    if (!(p.abs && n > 0)) return;
+    const double L = n * d;
+    const double d0 = -(p.pml_order + 1.) * p.vmax * std::log(p.Rc) / (2. * L);
+    const double da = alpha_max / n;
+    const double theta = 0.5;
+    auto bc = [&](double k, double dv, double a, double h, double &b, double &c) {
+        b = (k - (1. - theta) * h * (dv + k * a)) / (k + theta * h * (dv + k * a));
+        c = -dv * h / (k * (k + theta * h * (dv + k * a)));
+    };
+    for (size_t m = 0; m <= n; ++m) {
+        const double x = (double)(n - m) * d / L;
+        const double k = 1. + (p.kappa_max - 1.) * std::pow(x, p.pml_order);
+        const double dv = d0 * std::pow(x, p.pml_order);
+        const double al = m * da;
+        const bool inner = m < n;
+        const double xh = ((double)n - m - 0.5) * d / L;
+        const double kh = inner ? 1. + (p.kappa_max - 1.) * std::pow(xh, p.pml_order) : 1.;
+        const double dh = inner ? d0 * std::pow(xh, p.pml_order) : 0.;
+        const double alh = (m + 0.5) * da;
+        const double xH = ((double)n - m + 0.5) * d / L;
+        const double kH = 1. + (p.kappa_max - 1.) * std::pow(xH, p.pml_order);
+        const double dH = d0 * std::pow(xH, p.pml_order);
+        const double alH = (m - 0.5) * da;
+        kk[K_K][m] = k; kk[K_KH][m] = kh; kk[K_KHE][m] = kH;
+        st[0][T_B][m] = st[0][T_BH][m] = st[0][T_BHE][m] = 1.0;
+        for (int s = 1; s < 3; ++s) {
+            const double h = s == 1 ? dt * 0.5 : dt;
+            bc(k, dv, al, h, st[s][T_B][m], st[s][T_C][m]);
+            bc(kh, dh, alh, h, st[s][T_BH][m], st[s][T_CH][m]);
+            bc(kH, dH, alH, h, st[s][T_BHE][m], st[s][T_CHE][m]);
+        }
+    }
+}
+
+template <typename T>
+int upload(void *dst, const std::vector<double> &v, hipStream_t st)
+{
+    std::vector<T> t(v.begin(), v.end());
+    SM_HIP(hipMemcpyAsync(dst, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    SM_HIP(hipStreamSynchronize(st));
+    return SM_OK;
+}
+
+inline int upload_any(size_t elt, void *dst, const std::vector<double> &v, hipStream_t st)
+{
+    return elt == 8 ? upload<double>(dst, v, st) : upload<float>(dst, v, st);
+}
+
+inline int make_plan(hipfftHandle *h, int n, int howmany, int istride, int idist, int ostride, int odist,
+                     int in_total, int out_total, hipfftType type, hipStream_t st)
+{
+    int nn[1] = {n}, inembed[1] = {in_total}, onembed[1] = {out_total};
+    hipfftResult r = hipfftPlanMany(h, 1, nn, inembed, istride, idist, onembed, ostride, odist, type, howmany);
+    if (r != HIPFFT_SUCCESS) return fail(SM_ERR_HIP, "hipfftPlanMany (n %d, batch %d): error %d", n, howmany, (int)r);
+    r = hipfftSetStream(*h, st);
+    if (r != HIPFFT_SUCCESS) return fail(SM_ERR_HIP, "hipfftSetStream: error %d", (int)r);
+    return SM_OK;
+}
+
+inline void release(Grid &g)
+{
+    for (hipfftHandle *h : {&g.px_f, &g.px_i, &g.pz_f, &g.pz_i})
+        if (*h) { (void)hipfftDestroy(*h); *h = 0; }
+    for (sm_dbuf *b : {&g.spec, &g.kmul, &g.tabs}) dfree(*b);
+}
+
+inline size_t spec_bytes(size_t nx2, size_t nz2, size_t elt)
+{
+    const size_t a = (nx2 / 2 + 1) * nz2, b = nx2 * (nz2 / 2 + 1);
+    return std::max(a, b) * 2 * elt;
+}
+
+inline size_t table_bytes(size_t np, size_t elt) { return (3 * 2 * T_NST + 2 * K_NK) * (np + 1) * elt; }
+
+inline size_t kmul_bytes(size_t nx2, size_t nz2, size_t elt)
+{
+    return 2 * (std::max(nx2, nx2 / 2 + 1) + nz2 / 2 + 1) * 2 * elt;
+}
+
+// plans, multipliers and CPML tables for an nx2 x nz2 grid with np layers
+inline int init(Grid &g, sm_ctx *ctx, size_t nx2, size_t nz2, size_t np, size_t elt, double dx, double dz,
+                double dt, double f0, const Cpml &cp)
+{
+    g.ctx = ctx;
+    g.nx2 = (int)nx2; g.nz2 = (int)nz2; g.np = (int)np;
+    g.N = nx2 * nz2; g.elt = elt;
+    g.nxh = (int)(nx2 / 2 + 1); g.nzh = (int)(nz2 / 2 + 1);
+    g.xpair = nz2 % 2 == 0;
+    g.kx_n = g.xpair ? nx2 : (size_t)g.nxh;
+    hipStream_t st = ctx->stream;
+    int rc = dalloc(g.spec, spec_bytes(nx2, nz2, elt));
+    if (rc == SM_OK) rc = dalloc(g.kmul, kmul_bytes(nx2, nz2, elt));
+    if (rc == SM_OK) rc = dalloc(g.tabs, table_bytes(np, elt));
+    if (rc != SM_OK) return rc;
+    // multipliers i k e^{+-i k d/2}, k = 2 pi n / (d n2^2) (propagate.c:363-374)
+    {
+        std::vector<double> km;
+        auto h = [](size_t n2, double d, double sgn, size_t n, double &re, double &im) {
+            const double dn = 8.0 * std::atan(1.0) / (d * n2 * n2);
+            const double k = n * dn, ph = sgn * 0.5 * k * d * n2;
+            re = -k * std::sin(ph);   // re(i k e^{i ph})
+            im = k * std::cos(ph);
+        };
+        auto add_half = [&](size_t n2, double d, double sgn) {
+            for (size_t n = 0; n < n2 / 2 + 1; ++n) {
+                double re, im;
+                h(n2, d, sgn, n, re, im);
+                km.push_back(re); km.push_back(im);
+            }
+        };
+        auto add_full = [&](size_t n2, double d, double sgn) {
+            for (size_t n = 0; n < n2; ++n) {
+                double re, im;
+                if (2 * n < n2) h(n2, d, sgn, n, re, im);
+                else if (2 * n == n2) { h(n2, d, sgn, n, re, im); im = 0.0; }
+                else { h(n2, d, sgn, n2 - n, re, im); im = -im; }
+                km.push_back(re); km.push_back(im);
+            }
+        };
+        if (g.xpair) { add_full(nx2, dx, 1.0); add_full(nx2, dx, -1.0); }
+        else { add_half(nx2, dx, 1.0); add_half(nx2, dx, -1.0); }
+        add_half(nz2, dz, 1.0); add_half(nz2, dz, -1.0);
+        rc = upload_any(elt, g.kmul.ptr, km, st);
+    }
+    // CPML tables
+    if (rc == SM_OK) {
+        const double pi = 4.0 * std::atan(1.0);
+        const double alpha_max = cp.alpha_max ? pi * f0 : 0.0;
+        std::vector<double> sx[3][T_NST], sz[3][T_NST], kx[K_NK], kz[K_NK];
+        cpml_axis(cp, np, dx, dt, alpha_max, sx, kx);
+        cpml_axis(cp, np, dz, dt, alpha_max, sz, kz);
+        std::vector<double> t;
+        for (int s3 = 0; s3 < 3; ++s3) {
+            for (int r = 0; r < T_NST; ++r) t.insert(t.end(), sx[s3][r].begin(), sx[s3][r].end());
+            for (int r = 0; r < T_NST; ++r) t.insert(t.end(), sz[s3][r].begin(), sz[s3][r].end());
+        }
+        for (int r = 0; r < K_NK; ++r) t.insert(t.end(), kx[r].begin(), kx[r].end());
+        for (int r = 0; r < K_NK; ++r) t.insert(t.end(), kz[r].begin(), kz[r].end());
+        rc = upload_any(elt, g.tabs.ptr, t, st);
+    }
+    // plans (propagate.c:386-497): x transforms of stride nz2 (paired:
+    // complex, stride nz2/2, in place layout), z transforms of unit stride
+    if (rc == SM_OK) {
+        const hipfftType f_t = elt == 8 ? HIPFFT_D2Z : HIPFFT_R2C, i_t = elt == 8 ? HIPFFT_Z2D : HIPFFT_C2R;
+        const hipfftType c_t = elt == 8 ? HIPFFT_Z2Z : HIPFFT_C2C;
+        const int NX = (int)nx2, NZ = (int)nz2, NN = (int)g.N;
+        if (g.xpair)
+            rc = make_plan(&g.px_f, NX, NZ / 2, NZ / 2, 1, NZ / 2, 1, NN / 2, NN / 2, c_t, st);
+        else {
+            rc = make_plan(&g.px_f, NX, NZ, NZ, 1, 1, g.nxh, NN, g.nxh * NZ, f_t, st);
+            if (rc == SM_OK) rc = make_plan(&g.px_i, NX, NZ, 1, g.nxh, NZ, 1, g.nxh * NZ, NN, i_t, st);
+        }
+        if (rc == SM_OK) rc = make_plan(&g.pz_f, NZ, NX, 1, NZ, 1, g.nzh, NN, NX * g.nzh, f_t, st);
+        if (rc == SM_OK) rc = make_plan(&g.pz_i, NZ, NX, 1, g.nzh, 1, NZ, NX * g.nzh, NN, i_t, st);
+    }
+    return rc;
+}
+
+// device bytes of the machinery
+inline size_t grid_bytes(size_t nx2, size_t nz2, size_t np, size_t elt)
+{
+    return spec_bytes(nx2, nz2, elt) + kmul_bytes(nx2, nz2, elt) + table_bytes(np, elt);
+}
+
+}  // namespace pstd
+}  // namespace sm

@@ -7,8 +7,9 @@ agree to FFT rounding rather than bit for bit: traces and final fields are
 held to SH_FP64_TOL relative L2.  FP32 storage is held to SH_FP32_TOL.
 Decks: layered VTI viscoelastic with layers on every side, a model wider
 than deep (nx2 > nz2: the dx_vy layer indexed over nx2 columns),
-kappa_max > 1 with alpha_max = 0, no layers (abs 0) with odd nx, and
-several sources on one cell.
+kappa_max > 1 with alpha_max = 0, no layers (abs 0) with odd nx, several
+sources on one cell, and an odd nz2 (x derivatives by real transforms
+instead of complex transforms of column pairs).
 """
 import math
 
@@ -88,6 +89,7 @@ CASES = [
     dict(nx=33, nz=28, np_=5, seed=3, abs_=False),                     # no layers, odd nx2
     dict(nx=30, nz=36, np_=7, seed=4, srcs=[(S.SYZ, 20, 25, 1.0), (S.SYZ, 20, 25, -0.4),
                                             (S.SXY, 20, 25, 0.7), (S.SXY, 21, 24, 0.2)]),
+    dict(nx=31, nz=27, np_=6, seed=6),                                 # odd nz2: real x transforms
 ]
 
 
