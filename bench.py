@@ -40,10 +40,15 @@ STATE_BYTES = {"f32": 4, "bf16": 2, "f16": 2}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="a_iso_3d", choices=["a_iso_3d", "e_cyl_iso", "ve_vti_sh_pml"],
+    ap.add_argument("--workload", default="a_iso_3d",
+                    choices=["a_iso_3d", "e_cyl_iso", "ve_vti_sh_pml", "ve_vti_pml"],
                     help="a_iso_3d (the headline, default), e_cyl_iso (the 2.5D cylindrical "
-                         "elastic solver on a borehole model) or ve_vti_sh_pml (viscoelastic VTI SH, "
-                         "pseudo-spectral); the last two store FP32 or FP64")
+                         "elastic solver on a borehole model), ve_vti_sh_pml or ve_vti_pml "
+                         "(viscoelastic VTI SH or P-SV, pseudo-spectral); the last three store "
+                         "FP32 or FP64")
+    ap.add_argument("--hipfft", action="store_true",
+                    help="pseudo-spectral workloads: every derivative through hipFFT instead of the "
+                         "hand-written LDS transforms")
     ap.add_argument("--steps", type=int, default=5, help="timed shots")
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
     ap.add_argument("--n", type=int, default=256, help="interior nodes per axis")
@@ -69,8 +74,9 @@ def parse():
 # trace error of the timed storage against the FP64 run of the same shot
 # (relative L2 over all traces); the bench fails above it
 ERROR_TOL = {"f32": 2e-5, "f16": 4e-2, "bf16": 3e-1}
-# FP32 pseudo-spectral SH: FFT rounding over 1000 steps of a 1M-cell grid
-SH_F32_TOL = 1e-5    # measured 1.05e-6 at 1000 steps of the 1000^2 layered shot
+# FP32 pseudo-spectral workloads: FFT rounding over 1000 steps of a 1M-cell grid
+PSTD_F32_TOL = {"ve_vti_sh_pml": 1e-5,   # measured 1.05e-6 and 1.18e-6 at 1000 steps of the 1000^2 shot
+                "ve_vti_pml": 1e-4}
 
 
 def cpu_baseline(shot, nsteps_sample: int) -> dict:
@@ -309,47 +315,73 @@ def main_cyl(args) -> None:
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline_sh(shot, nsteps_sample: int) -> dict:
+def cpu_baseline_pstd(kind, shot, nsteps_sample: int) -> dict:
     """Oracle (FP64 numpy restatement with pocketfft, one thread) on the same
     model; time per step from the difference of two runs."""
-    from oracle import oracle_sh as S
     from oracle import pyoracle as O
     cfg = dict(nx=shot.nx, nz=shot.nz, np=shot.np, dx=shot.h, dz=shot.h, dt=shot.dt, vmax=5000.0,
                alpha_max=np.pi * shot.f0, iwipe=1)
     L, fct = O.source_functions([1.0], [shot.f0], 0.5 * shot.dt)
-    src = [(5, int(shot.src_idx[0]), fct / (shot.h * shot.h))]
+    src = [(PSTD[kind]["src_type"], int(shot.src_idx[0]), fct / (shot.h * shot.h))]
+    if kind == "ve_vti_sh_pml":
+        from oracle import oracle_sh as S
+        run = lambda n: S.sh_run(cfg, shot.mu, shot.rho, shot.gamma, shot.Q, shot.f, src, shot.rec_idx[:1],
+                                 shot.rec_comp[:1], n)
+    else:
+        from oracle import oracle_ve as V
+        mat = V.ve_material(*shot.model)
+        run = lambda n: V.ve_run(cfg, mat, src, shot.rec_idx[:1], shot.rec_comp[:1], n)
     times = []
     for n in (1, 1 + nsteps_sample):
         t0 = time.perf_counter()
-        S.sh_run(cfg, shot.mu, shot.rho, shot.gamma, shot.Q, shot.f, src, shot.rec_idx[:1],
-                 shot.rec_comp[:1], n)
+        run(n)
         times.append(time.perf_counter() - t0)
     per_step = max(times[1] - times[0], 1e-9) / nsteps_sample
+    nx2, nz2 = shot.nx + 2 * shot.np, shot.nz + 2 * shot.np
     return {"value": shot.cells / per_step / 1e9, "unit": "Gcell-updates/s", "cores": 1,
             "kind": "port",
-            "sample": f"{shot.mu.shape[0]}x{shot.mu.shape[1]} padded model, {nsteps_sample} time steps "
+            "sample": f"{nx2}x{nz2} padded model, {nsteps_sample} time steps "
                       f"(difference of a {1 + nsteps_sample}-step and a 1-step run), FP64 numpy, one thread"}
 
 
-def main_sh(args) -> None:
-    """ve_vti_sh_pml on the seeded layered VTI viscoelastic model (1000 x 1000
-    nodes of 5 m plus a 20-cell CPML, Syz source, 200 Vy receivers, 1000
-    steps); one bench step is one shot."""
+# the pseudo-spectral workloads: viscoelastic VTI SH (ve_vti_sh_pml) and P-SV
+# (ve_vti_pml), one bench step = one shot of 1000 steps on 1000 x 1000 nodes
+PSTD = {
+    "ve_vti_sh_pml": dict(src_type=5, src_name="Syz", rec_name="Vy",
+                          metric="Gcell-updates/s & %HBM roofline, 2D viscoelastic VTI SH, pseudo-spectral RK4 "
+                                 "(ve_vti_sh_pml)"),
+    "ve_vti_pml": dict(src_type=6, src_name="Bulk", rec_name="Vz",
+                       metric="Gcell-updates/s & %HBM roofline, 2D viscoelastic VTI P-SV, pseudo-spectral RK4 "
+                              "(ve_vti_pml)"),
+}
+
+
+def main_pstd(args) -> None:
+    """ve_vti_sh_pml or ve_vti_pml on a seeded layered VTI viscoelastic model
+    (1000 x 1000 nodes of 5 m plus a 20-cell CPML, one source, 200
+    receivers, 1000 steps); one bench step is one shot."""
     from seismod_amd import synth
-    from seismod_amd.solver import Context, ShConfig, ViscoelasticSH, source_functions
+    from seismod_amd.solver import (Context, ShConfig, VeConfig, ViscoelasticPSV, ViscoelasticSH,
+                                    source_functions)
+    kind = args.workload
+    spec = PSTD[kind]
     if args.storage not in ("f32", "f64"):
-        print("bench.py: ve_vti_sh_pml stores FP32 or FP64", file=sys.stderr)
+        print(f"bench.py: {kind} stores FP32 or FP64", file=sys.stderr)
         sys.exit(2)
     nsteps = args.nsteps or 1000
-    shot = synth.sh_layered_shot(nsteps=nsteps)
+    sh = kind == "ve_vti_sh_pml"
+    shot = synth.sh_layered_shot(nsteps=nsteps) if sh else synth.ve_layered_shot(nsteps=nsteps)
     ctx = Context(0)
 
     def make(storage):
-        cfg = ShConfig(nx=shot.nx, nz=shot.nz, np=shot.np, dx=shot.h, dz=shot.h, dt=shot.dt, f0=shot.f0,
-                       storage=storage)
-        sim = ViscoelasticSH(ctx, cfg, shot.mu, shot.rho, shot.gamma, shot.Q, shot.f)
+        common = dict(nx=shot.nx, nz=shot.nz, np=shot.np, dx=shot.h, dz=shot.h, dt=shot.dt, f0=shot.f0,
+                      storage=storage, hipfft=args.hipfft)
+        if sh:
+            sim = ViscoelasticSH(ctx, ShConfig(**common), shot.mu, shot.rho, shot.gamma, shot.Q, shot.f)
+        else:
+            sim = ViscoelasticPSV(ctx, VeConfig(L=shot.L, **common), *shot.model)
         L, fct = source_functions([1.0], [shot.f0], 0.5 * shot.dt)
-        sim.set_sources(np.array([5], np.intc), shot.src_idx, L, fct / (shot.h * shot.h))
+        sim.set_sources(np.array([spec["src_type"]], np.intc), shot.src_idx, L, fct / (shot.h * shot.h))
         sim.set_receivers(shot.rec_idx, shot.rec_comp, nsteps)
         return sim
 
@@ -378,24 +410,28 @@ def main_sh(args) -> None:
         print("bench.py: traces are not finite or all zero", file=sys.stderr)
         sys.exit(1)
     elt = 4 if args.storage == "f32" else 8
-    alg_b = synth.algorithmic_bytes_sh(elt)
+    alg_b = synth.algorithmic_bytes_sh(elt) if sh else synth.algorithmic_bytes_ve(shot.L, elt)
     avg_ms = kms / max(timed, 1)
     achieved = alg_b * shot.cells / (avg_ms * 1e-3) / 1e9
-    nx2, nz2 = shot.mu.shape
+    nx2, nz2 = shot.nx + 2 * shot.np, shot.nz + 2 * shot.np
     line = {
-        "metric": "Gcell-updates/s & %HBM roofline, 2D viscoelastic VTI SH, pseudo-spectral RK4 (ve_vti_sh_pml)",
+        "metric": spec["metric"],
         "value": round(shot.cells * nsteps * args.steps / wall / 1e9, 3),
         "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.storage == "f32" else "f64",
         "data": "synthetic (seeded layered VTI viscoelastic model)",
-        "config": {"workload": f"ve_vti_sh_pml {shot.nx}x{shot.nz} + CPML {shot.np}, h = {shot.h} m, "
-                               f"Syz {shot.f0:g} Hz, {nsteps} time steps per shot, {len(shot.rec_idx)} receivers",
+        "config": {"workload": f"{kind} {shot.nx}x{shot.nz} + CPML {shot.np}, h = {shot.h} m, "
+                               f"{spec['src_name']} {shot.f0:g} Hz, {nsteps} time steps per shot, "
+                               f"{len(shot.rec_idx)} {spec['rec_name']} receivers"
+                               + ("" if sh else f", L = {shot.L}"),
                    "padded_grid": [nx2, nz2], "time_steps_per_shot": nsteps, "dt_s": shot.dt,
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
-        "roofline": {"bound": "hbm", "kernel": "one step (4 RK stages of hipFFT derivatives and pointwise kernels)",
+        "roofline": {"bound": "hbm", "kernel": "one step (4 RK stages of FFT derivatives ("
+                                               + ("hipFFT" if args.hipfft else "LDS transforms")
+                                               + ") and a fused pointwise kernel)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_cell": alg_b, "kernel_ms_avg": round(avg_ms, 5),
@@ -408,18 +444,17 @@ def main_sh(args) -> None:
         tr64 = ref.traces(0, nsteps)
         ref.close()
         e = float(np.linalg.norm(tr - tr64) / np.linalg.norm(tr64))
+        tol = PSTD_F32_TOL[kind]
         line["error_vs_fp64"] = {"traces_rel_l2": e, "against": "FP64 run of the same shot on the GPU "
                                  "(the FP64 path equals the FP64 oracle to FFT rounding, 1e-15)",
-                                 "traces": len(shot.rec_idx), "samples": nsteps,
-                                 "tolerance": SH_F32_TOL}
-        if e > SH_F32_TOL:
+                                 "traces": len(shot.rec_idx), "samples": nsteps, "tolerance": tol}
+        if e > tol:
             print(json.dumps(line), flush=True)
-            print(f"bench.py: trace error {e:.3e} against the FP64 run exceeds {SH_F32_TOL}",
-                  file=sys.stderr)
+            print(f"bench.py: trace error {e:.3e} against the FP64 run exceeds {tol}", file=sys.stderr)
             sys.exit(1)
     ctx.close()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline_sh(shot, max(1, args.cpu_steps // 5))
+        line["cpu_baseline"] = cpu_baseline_pstd(kind, shot, max(1, args.cpu_steps // 5))
     print(json.dumps(line), flush=True)
 
 
@@ -432,8 +467,8 @@ def main():
     if args.workload == "e_cyl_iso":
         main_cyl(args)
         return
-    if args.workload == "ve_vti_sh_pml":
-        main_sh(args)
+    if args.workload in PSTD:
+        main_pstd(args)
         return
     if args.storage == "f64":
         print("bench.py: the a_iso_3d bench stores f32, bf16 or f16", file=sys.stderr)

@@ -428,6 +428,11 @@ size_t sm_cyl_bytes_needed(const sm_cyl_params *p, size_t nrec, size_t max_steps
 /* 145-168 (add_src_ve_sh_rk4)                                          */
 /* ------------------------------------------------------------------ */
 typedef struct sm_sh sm_sh;
+/* flags of sm_sh_params and sm_ve_params (no counterpart in the reference):
+ * take every derivative through hipFFT even where the hand-written
+ * LDS transforms apply (axis lengths whose factors are 2, 3, 5, 7, 11, 13);
+ * both give the same results to rounding */
+#define SM_PSTD_HIPFFT 1
 typedef struct {
     size_t nx, nz;            /* interior nodes (struct grid) */
     size_t np;                /* CPML layers on every side (nl); forced to 0
@@ -439,7 +444,7 @@ typedef struct {
     double f0;                /* frequency of source 0 */
     int abs;                  /* iwipe: 1 absorbing layers, 0 none */
     int storage;              /* SM_STORE_F64 or SM_STORE_F32 */
-    int flags;                /* 0 */
+    int flags;                /* 0 or SM_PSTD_HIPFFT */
 } sm_sh_params;
 
 /* Build a solver from the padded model as read_model_ve_sh leaves it
@@ -474,6 +479,73 @@ int sm_sh_get_field(sm_sh *s, int comp, double *out);
 int sm_sh_reset(sm_sh *s);
 int sm_sh_device_bytes(const sm_sh *s, size_t *bytes);
 size_t sm_sh_bytes_needed(const sm_sh_params *p, size_t nrec, size_t max_steps);
+
+/* ------------------------------------------------------------------ */
+/* Viscoelastic P-SV waves in a 2D VTI medium (ve_vti_pml) --           */
+/* pseudo-spectral derivatives (hipFFT), CPML, fourth-order Runge-Kutta, */
+/* L relaxation mechanisms for the quasi-dilatational mode and one for   */
+/* shear; replaces ve_vti_pml.c:518-784, propagate.c:325-510 and         */
+/* 1959-2278 (init_fftw_data_ve, propagateVE_CPML, partial_*), pml.c:    */
+/* 1553-1631 and 1959-2106 (alloc_cpml_ve, compute_cpml), src.c:114-142  */
+/* (add_src_ve_rk4)                                                      */
+/* ------------------------------------------------------------------ */
+typedef struct sm_ve sm_ve;
+typedef struct {
+    size_t nx, nz;            /* interior nodes (struct grid) */
+    size_t np;                /* CPML layers on every side; forced to 0 when
+                                 abs is 0 (io_utils.c:293) */
+    double dx, dz;            /* m */
+    double dt;                /* s */
+    double vmax, pml_order, Rc, kappa_max;   /* struct abs_params */
+    int alpha_max;            /* 0 -> alpha = 0, else alpha_max = pi*f0 */
+    double f0;                /* frequency of source 0 */
+    int abs;                  /* iwipe: 1 absorbing layers, 0 none */
+    int L;                    /* quasi-dilatational relaxation mechanisms, >= 1 */
+    int storage;              /* SM_STORE_F64 or SM_STORE_F32 */
+    int flags;                /* 0 or SM_PSTD_HIPFFT */
+} sm_ve_params;
+
+/* Build a solver from the padded model as read_model_ve leaves it
+ * (io_utils.c:1155-1314): K and mu in GPa, rho (kg/m^3), epsilon, delta,
+ * Q2 and f2 hold nx2*nz2 values (nx2 = nx+2np, index i*nz2+j), Q1 and f1
+ * L such arrays one after the other.  Computes the material of
+ * ve_vti_pml.c:518-683, the CPML tables of the three Runge-Kutta stage sets
+ * and the FFT plans, and zeroes the fields.  Needs nx, nz >= 2, rho, Q, f > 0
+ * and a finite material (SM_ERR_ARG).  As in the reference, the e3 memory
+ * variable does not feed back into tau_xz (c55_0_ij is never set there;
+ * zero-filled memory) and the relaxation sum of tau_xx and tau_zz reads the
+ * state at flat offset 6 nx2 nz2 + n + m (propagate.c:2190-2195). */
+int sm_ve_create(sm_ctx *ctx, const sm_ve_params *p, const double *K, const double *mu,
+                 const double *rho, const double *eps, const double *delta, const double *Q1,
+                 const double *f1, const double *Q2, const double *f2, sm_ve **out);
+int sm_ve_destroy(sm_ve *s);
+/* The moduli ve_vti_pml writes with write_field_nc (ve_vti_pml.c:695-699),
+ * in MPa on the padded grid, from the same inputs as sm_ve_create: c11,
+ * c13, c33 and K0 at the nodes, c55 at (i+1/2, j+1/2).  Host only. */
+int sm_ve_moduli(size_t nx2, size_t nz2, int L, const double *K, const double *mu,
+                 const double *rho, const double *eps, const double *delta, const double *Q1,
+                 const double *f1, const double *Q2, const double *f2, double *c11, double *c13,
+                 double *c33, double *c55, double *K0);
+/* Sources on padded indices: SM_SX adds to tau_xx, SM_SZ to tau_zz,
+ * SM_BULK to both, SM_SXZ to tau_xz, other types inject nothing
+ * (add_src_ve_rk4).  fct as for sm_sh_set_sources. */
+int sm_ve_set_sources(sm_ve *s, size_t nsrc, const int *type, const size_t *idx,
+                      const size_t *length, const double *fct);
+/* receivers on padded indices; comp SM_VX, SM_VZ, SM_TXX, SM_TZZ or SM_TXZ */
+int sm_ve_set_receivers(sm_ve *s, size_t nrec, const size_t *idx, const int *comp,
+                        size_t max_steps);
+int sm_ve_run(sm_ve *s, size_t nsteps);
+/* Per-step timing (no counterpart in the reference): HIP events around the
+ * four Runge-Kutta stages of every step. */
+int sm_ve_set_timing(sm_ve *s, int on);
+int sm_ve_get_timing(const sm_ve *s, double *ms_total, size_t *steps);
+int sm_ve_current_step(const sm_ve *s, size_t *it);
+int sm_ve_get_traces(sm_ve *s, size_t it0, size_t count, double *out);
+/* padded nx2*nz2 field: SM_VX, SM_VZ, SM_TXX, SM_TZZ or SM_TXZ */
+int sm_ve_get_field(sm_ve *s, int comp, double *out);
+int sm_ve_reset(sm_ve *s);
+int sm_ve_device_bytes(const sm_ve *s, size_t *bytes);
+size_t sm_ve_bytes_needed(const sm_ve_params *p, size_t nrec, size_t max_steps);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libseismod_hip.so"
 SM_OK, SM_ERR_ARG, SM_ERR_HIP, SM_ERR_CFL, SM_ERR_NOMEM, SM_ERR_STATE, SM_ERR_NODEV = range(7)
 SM_STORE_F32, SM_STORE_F64, SM_STORE_BF16, SM_STORE_F16 = range(4)
 SM_VX, SM_VY, SM_VZ, SM_TXX, SM_TZZ, SM_TXZ, SM_P = 0, 1, 2, 5, 6, 7, 8
+SM_PSTD_HIPFFT = 1
 SM_TXY, SM_TYZ = 9, 10
 SM_DIV, SM_CURL, SM_VR, SM_VT, SM_TRR, SM_TRT, SM_TRZ, SM_TTT, SM_TTZ = range(11, 20)
 SM_BULK_S, SM_FR, SM_FT, SM_FZ, SM_KURKJIAN = 8, 9, 10, 11, 12
@@ -64,6 +65,16 @@ class ShParams(C.Structure):
         ("vmax", C.c_double), ("pml_order", C.c_double), ("Rc", C.c_double),
         ("kappa_max", C.c_double), ("alpha_max", C.c_int), ("f0", C.c_double),
         ("abs", C.c_int), ("storage", C.c_int), ("flags", C.c_int),
+    ]
+
+
+class VeParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_size_t), ("nz", C.c_size_t), ("np", C.c_size_t),
+        ("dx", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("vmax", C.c_double), ("pml_order", C.c_double), ("Rc", C.c_double),
+        ("kappa_max", C.c_double), ("alpha_max", C.c_int), ("f0", C.c_double),
+        ("abs", C.c_int), ("L", C.c_int), ("storage", C.c_int), ("flags", C.c_int),
     ]
 
 
@@ -168,6 +179,20 @@ _SIGS = {
     "sm_sh_reset": (C.c_int, [C.c_void_p]),
     "sm_sh_device_bytes": (C.c_int, [C.c_void_p, _size_p]),
     "sm_sh_bytes_needed": (C.c_size_t, [C.POINTER(ShParams), C.c_size_t, C.c_size_t]),
+    "sm_ve_create": (C.c_int, [C.c_void_p, C.POINTER(VeParams)] + [_dbl_p] * 9 + [C.POINTER(C.c_void_p)]),
+    "sm_ve_moduli": (C.c_int, [C.c_size_t, C.c_size_t, C.c_int] + [_dbl_p] * 14),
+    "sm_ve_destroy": (C.c_int, [C.c_void_p]),
+    "sm_ve_set_sources": (C.c_int, [C.c_void_p, C.c_size_t, _int_p, _size_p, _size_p, _dbl_p]),
+    "sm_ve_set_receivers": (C.c_int, [C.c_void_p, C.c_size_t, _size_p, _int_p, C.c_size_t]),
+    "sm_ve_run": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "sm_ve_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "sm_ve_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),
+    "sm_ve_current_step": (C.c_int, [C.c_void_p, _size_p]),
+    "sm_ve_get_traces": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, _dbl_p]),
+    "sm_ve_get_field": (C.c_int, [C.c_void_p, C.c_int, _dbl_p]),
+    "sm_ve_reset": (C.c_int, [C.c_void_p]),
+    "sm_ve_device_bytes": (C.c_int, [C.c_void_p, _size_p]),
+    "sm_ve_bytes_needed": (C.c_size_t, [C.POINTER(VeParams), C.c_size_t, C.c_size_t]),
 }
 
 

@@ -18,6 +18,15 @@
 //   z (unit stride): real-to-complex and complex-to-real transforms.
 // Results equal the real-transform derivatives to rounding.
 //
+// When an axis length factors into 2, 3, 5, 7, 11 and 13, its derivatives
+// skip hipFFT: one pstd_dfft launch takes every derivative of a stage along
+// that axis.  A workgroup loads a tile of row pairs (z) or column pairs (x)
+// into LDS as complex numbers, runs a mixed-radix Stockham FFT in LDS,
+// multiplies by the whole-spectrum multiplier, runs the inverse as the
+// conjugate of the forward transform of the conjugate, and writes the two
+// derivatives: one HBM read and one write per value instead of six
+// (forward transform, multiplier pass, inverse transform).
+//
 // CPML (pml.c:1959-2106): per stage (fp1, fp23, fp4) and axis the rows b,
 // c at nodes, bh, ch at inner half nodes, bH, cH at outer half nodes; the
 // k rows are shared by the stages.  Every row has np + 1 entries; entry np
@@ -72,6 +81,14 @@ struct Grid {
     sm_dbuf tabs;                   // per stage 2 x T_NST rows, then 2 x K_NK rows (np + 1 each)
     size_t kx_n = 0;                // entries per x multiplier row (nx2 paired, nxh otherwise)
     hipfftHandle px_f = 0, px_i = 0, pz_f = 0, pz_i = 0;
+    // hand-written transforms per axis (0 x, 1 z): radices packed 4 bits a
+    // pass (index into FFT_RADICES), pairs per workgroup, twiddles
+    // e^{-2 pi i k/n} and whole-spectrum multipliers (fwd then bwd)
+    bool own[2] = {false, false};
+    int nr[2] = {0, 0};
+    uint64_t rcode[2] = {0, 0};
+    int P[2] = {1, 1};
+    sm_dbuf tw[2], fmul[2];
 };
 
 // one layer set as the stage kernels see it
@@ -183,6 +200,296 @@ int deriv(Grid &g, const T *f, int axis, bool fwd, T *out)
     return SM_OK;
 }
 
+// ------------------------------------------------------------------
+// hand-written FFT derivatives
+// ------------------------------------------------------------------
+constexpr int FFT_MAXPASS = 12;
+constexpr int FFT_RADICES[9] = {2, 3, 4, 5, 7, 8, 11, 13, 16};
+
+// radices of n, largest powers of two first; false if n has another factor
+inline bool fft_factor(int n, int &nr, uint64_t &code)
+{
+    nr = 0;
+    code = 0;
+    int m = n;
+    auto push = [&](int r) {
+        int q = 0;
+        while (FFT_RADICES[q] != r) ++q;
+        code |= (uint64_t)q << (4 * nr);
+        ++nr;
+    };
+    while (m % 16 == 0 && nr < FFT_MAXPASS) { push(16); m /= 16; }
+    if (m % 8 == 0) { push(8); m /= 8; }
+    else if (m % 4 == 0) { push(4); m /= 4; }
+    else if (m % 2 == 0) { push(2); m /= 2; }
+    for (int r : {3, 5, 7, 11, 13})
+        while (m % r == 0) {
+            if (nr >= FFT_MAXPASS) return false;
+            push(r);
+            m /= r;
+        }
+    return m == 1 && n >= 2 && nr > 0;
+}
+
+template <typename T>
+__device__ __forceinline__ Cx<T> cmul(Cx<T> a, Cx<T> b)
+{
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> cadd(Cx<T> a, Cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ Cx<T> csub(Cx<T> a, Cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+// -i a
+template <typename T>
+__device__ __forceinline__ Cx<T> cmi(Cx<T> a) { return {a.im, -a.re}; }
+
+// forward DFT of v[0..R) in place, w = e^{-2 pi i / R}; tw holds
+// e^{-2 pi i k / n} and n is a multiple of R (w^m = tw[m n / R])
+template <typename T, int R>
+__device__ __forceinline__ void dft(Cx<T> *v, const Cx<T> *tw, int n)
+{
+    if constexpr (R == 2) {
+        const Cx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b); v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        const Cx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]), a2 = cadd(v[1], v[3]), a3 = csub(v[1], v[3]);
+        v[0] = cadd(a0, a2); v[2] = csub(a0, a2);
+        v[1] = cadd(a1, cmi(a3)); v[3] = csub(a1, cmi(a3));
+    } else if constexpr (R == 8 || R == 16) {
+        // two half-length DFTs of the even and odd samples, combined with
+        // w^k (from the table)
+        constexpr int H = R / 2;
+        Cx<T> e[H], o[H];
+#pragma unroll
+        for (int t = 0; t < H; ++t) { e[t] = v[2 * t]; o[t] = v[2 * t + 1]; }
+        dft<T, H>(e, tw, n);
+        dft<T, H>(o, tw, n);
+        const int step = n / R;
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            const Cx<T> wo = k == 0 ? o[k] : cmul(o[k], tw[k * step]);
+            v[k] = cadd(e[k], wo);
+            v[k + H] = csub(e[k], wo);
+        }
+    } else {
+        // odd R: pair t with R - t; X_s = A_s - i B_s, X_{R-s} = A_s + i B_s
+        // with A_s = v0 + sum (v_t + v_{R-t}) cos, B_s = sum (v_t - v_{R-t}) sin
+        constexpr int H = (R - 1) / 2;
+        const int step = n / R;
+        Cx<T> sp[H], sm[H];
+        Cx<T> x0 = v[0];
+#pragma unroll
+        for (int t = 1; t <= H; ++t) {
+            sp[t - 1] = cadd(v[t], v[R - t]);
+            sm[t - 1] = csub(v[t], v[R - t]);
+            x0 = cadd(x0, sp[t - 1]);
+        }
+        Cx<T> out[R];
+        out[0] = x0;
+#pragma unroll
+        for (int s = 1; s <= H; ++s) {
+            Cx<T> A = v[0], B = {T(0), T(0)};
+#pragma unroll
+            for (int t = 1; t <= H; ++t) {
+                const Cx<T> w = tw[((s * t) % R) * step];   // (cos, -sin)
+                A.re += sp[t - 1].re * w.re; A.im += sp[t - 1].im * w.re;
+                B.re -= sm[t - 1].re * w.im; B.im -= sm[t - 1].im * w.im;
+            }
+            out[s] = {A.re + B.im, A.im - B.re};
+            out[R - s] = {A.re - B.im, A.im + B.re};
+        }
+#pragma unroll
+        for (int t = 0; t < R; ++t) v[t] = out[t];
+    }
+}
+
+// one Stockham pass of radix R over P transforms of length n (src -> dst);
+// Ns = product of the earlier radices
+template <typename T, int R>
+__device__ __forceinline__ void fft_pass(const Cx<T> *src, Cx<T> *dst, int n, int P, int Ns, const Cx<T> *tw)
+{
+    const int nb = n / R;
+    const int tws = n / (Ns * R);
+    for (int l = threadIdx.x; l < nb * P; l += blockDim.x) {
+        const int p = l / nb, j = l - p * nb;
+        const Cx<T> *s = src + p * n;
+        Cx<T> *d = dst + p * n;
+        const int k = j % Ns;
+        Cx<T> v[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) v[t] = s[j + t * nb];
+        if (k) {
+#pragma unroll
+            for (int t = 1; t < R; ++t) v[t] = cmul(v[t], tw[k * t * tws]);
+        }
+        dft<T, R>(v, tw, n);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int t = 0; t < R; ++t) d[base + t * Ns] = v[t];
+    }
+}
+
+// forward FFT of the P transforms in a (scratch b); returns the buffer
+// holding the result, after a barrier.  Inlined at one call site: an
+// out-of-line call would keep a stack frame in scratch memory.
+template <typename T>
+__device__ __forceinline__ Cx<T> *fft_fwd(Cx<T> *a, Cx<T> *b, int n, int P, int nr, uint64_t code,
+                                          const Cx<T> *tw)
+{
+    int Ns = 1;
+    for (int q = 0; q < nr; ++q) {
+        const int ri = (int)((code >> (4 * q)) & 15);
+        switch (ri) {
+        case 0: fft_pass<T, 2>(a, b, n, P, Ns, tw); Ns *= 2; break;
+        case 1: fft_pass<T, 3>(a, b, n, P, Ns, tw); Ns *= 3; break;
+        case 2: fft_pass<T, 4>(a, b, n, P, Ns, tw); Ns *= 4; break;
+        case 3: fft_pass<T, 5>(a, b, n, P, Ns, tw); Ns *= 5; break;
+        case 4: fft_pass<T, 7>(a, b, n, P, Ns, tw); Ns *= 7; break;
+        case 5: fft_pass<T, 8>(a, b, n, P, Ns, tw); Ns *= 8; break;
+        case 6: fft_pass<T, 11>(a, b, n, P, Ns, tw); Ns *= 11; break;
+        case 7: fft_pass<T, 13>(a, b, n, P, Ns, tw); Ns *= 13; break;
+        default: fft_pass<T, 16>(a, b, n, P, Ns, tw); Ns *= 16; break;
+        }
+        __syncthreads();
+        Cx<T> *t = a; a = b; b = t;
+    }
+    return a;
+}
+
+// up to 8 derivatives of one stage along one axis: job j differentiates
+// field ((fields >> 4j) & 15) of x into d + ((slots >> 4j) & 15) N, with
+// the forward multiplier if bit j of fwd is set
+template <typename T>
+struct DfftArgs {
+    const T *x;
+    T *d;
+    unsigned N;
+    uint32_t fields, slots, fwd;
+    int njobs;
+    const Cx<T> *tw, *mul_f, *mul_b;
+    int n, nr;
+    uint64_t code;
+    int nx2, nz2;
+    int P, ntiles, nblocks;
+};
+
+extern __shared__ __align__(16) unsigned char pstd_lds[];
+
+// AX 1: z (unit stride), row pairs (r, r + 1); AX 0: x (stride nz2),
+// column pairs (c, c + 1); a missing partner reads as 0 and is not written
+template <typename T, int AX>
+__global__ __launch_bounds__(512) void pstd_dfft(const DfftArgs<T> a)
+{
+    // workgroups of one XCD take consecutive tiles (adjacent columns share
+    // cache lines)
+    const int per = (a.nblocks + 7) / 8;
+    const int b = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (b >= a.nblocks) return;
+    const int jb = b / a.ntiles, tile = b - jb * a.ntiles;
+    const T *f = a.x + (size_t)((a.fields >> (4 * jb)) & 15) * a.N;
+    T *out = a.d + (size_t)((a.slots >> (4 * jb)) & 15) * a.N;
+    const Cx<T> *mul = ((a.fwd >> jb) & 1) ? a.mul_f : a.mul_b;
+    const int n = a.n, P = a.P, tot = P * n;
+    // LDS: two buffers of P transforms, then the twiddles
+    Cx<T> *A = (Cx<T> *)pstd_lds, *B = A + tot, *tw = B + tot;
+    for (int l = threadIdx.x; l < n; l += blockDim.x) tw[l] = a.tw[l];
+    for (int l = threadIdx.x; l < tot; l += blockDim.x) {
+        int p, k;
+        if (AX == 1) { p = l / n; k = l - p * n; }
+        else { k = l / P; p = l - k * P; }
+        const int r0 = 2 * (tile * P + p);
+        T re = T(0), im = T(0);
+        if (AX == 1) {
+            if (r0 < a.nx2) re = f[(size_t)r0 * a.nz2 + k];
+            if (r0 + 1 < a.nx2) im = f[(size_t)(r0 + 1) * a.nz2 + k];
+        } else {
+            if (r0 < a.nz2) re = f[(size_t)k * a.nz2 + r0];
+            if (r0 + 1 < a.nz2) im = f[(size_t)k * a.nz2 + r0 + 1];
+        }
+        A[p * n + k] = {re, im};
+    }
+    __syncthreads();
+    // Z = DFT(f); y = IDFT(M Z) = conj(DFT(conj(M Z)))
+    Cx<T> *c = A;
+    for (int rep = 0; rep < 2; ++rep) {
+        c = fft_fwd<T>(c, c == A ? B : A, n, P, a.nr, a.code, tw);
+        if (rep == 1) break;
+        for (int l = threadIdx.x; l < tot; l += blockDim.x) {
+            const Cx<T> z = cmul(c[l], mul[l % n]);
+            c[l] = {z.re, -z.im};
+        }
+        __syncthreads();
+    }
+    for (int l = threadIdx.x; l < tot; l += blockDim.x) {
+        int p, k;
+        if (AX == 1) { p = l / n; k = l - p * n; }
+        else { k = l / P; p = l - k * P; }
+        const int r0 = 2 * (tile * P + p);
+        const Cx<T> y = c[p * n + k];
+        if (AX == 1) {
+            if (r0 < a.nx2) out[(size_t)r0 * a.nz2 + k] = y.re;
+            if (r0 + 1 < a.nx2) out[(size_t)(r0 + 1) * a.nz2 + k] = -y.im;
+        } else {
+            if (r0 < a.nz2) out[(size_t)k * a.nz2 + r0] = y.re;
+            if (r0 + 1 < a.nz2) out[(size_t)k * a.nz2 + r0 + 1] = -y.im;
+        }
+    }
+}
+
+struct DJob {
+    int field, slot;
+    bool fwd;
+};
+
+inline size_t dfft_lds(int n, int P, size_t elt) { return (2 * (size_t)P + 1) * n * 2 * elt; }
+
+// threads of a pstd_dfft workgroup: enough for the largest pass in one
+// round (P n / R butterflies), at most 512
+inline unsigned dfft_threads(int n, int P, int nr, uint64_t code)
+{
+    int most = 0;
+    for (int q = 0; q < nr; ++q) most = std::max(most, P * n / FFT_RADICES[(code >> (4 * q)) & 15]);
+    return (unsigned)std::min(512, std::max(64, (most + 63) / 64 * 64));
+}
+
+// the derivatives jobs[0..nj) along axis of the fields of x into d
+template <typename T>
+int deriv_batch(Grid &g, int axis, const T *x, T *d, const DJob *jobs, int nj)
+{
+    if (!g.own[axis]) {
+        for (int q = 0; q < nj; ++q) {
+            int rc = deriv<T>(g, x + (size_t)jobs[q].field * g.N, axis, jobs[q].fwd, d + (size_t)jobs[q].slot * g.N);
+            if (rc != SM_OK) return rc;
+        }
+        return SM_OK;
+    }
+    DfftArgs<T> a;
+    a.x = x; a.d = d; a.N = (unsigned)g.N;
+    a.fields = a.slots = a.fwd = 0;
+    for (int q = 0; q < nj; ++q) {
+        a.fields |= (uint32_t)jobs[q].field << (4 * q);
+        a.slots |= (uint32_t)jobs[q].slot << (4 * q);
+        a.fwd |= (uint32_t)(jobs[q].fwd ? 1 : 0) << q;
+    }
+    a.njobs = nj;
+    const int n = axis == 0 ? g.nx2 : g.nz2, other = axis == 0 ? g.nz2 : g.nx2;
+    a.tw = (const Cx<T> *)g.tw[axis].ptr;
+    a.mul_f = (const Cx<T> *)g.fmul[axis].ptr;
+    a.mul_b = a.mul_f + n;
+    a.n = n; a.nr = g.nr[axis]; a.code = g.rcode[axis];
+    a.nx2 = g.nx2; a.nz2 = g.nz2;
+    a.P = g.P[axis];
+    a.ntiles = ((other + 1) / 2 + a.P - 1) / a.P;
+    a.nblocks = nj * a.ntiles;
+    const unsigned grid = (unsigned)((a.nblocks + 7) / 8 * 8);
+    const size_t lds = dfft_lds(n, a.P, sizeof(T));
+    const unsigned nt = dfft_threads(n, a.P, a.nr, a.code);
+    if (axis == 0) pstd_dfft<T, 0><<<grid, nt, lds, g.ctx->stream>>>(a);
+    else pstd_dfft<T, 1><<<grid, nt, lds, g.ctx->stream>>>(a);
+    return SM_OK;
+}
+
 // device views of the CPML tables: stage 0 (fp1), 1 (fp23), 2 (fp4); axis 0 x, 1 z
 template <typename T>
 const T *tab(const Grid &g, int stage, int axis, int row)
@@ -280,7 +587,7 @@ inline void release(Grid &g)
 {
     for (hipfftHandle *h : {&g.px_f, &g.px_i, &g.pz_f, &g.pz_i})
         if (*h) { (void)hipfftDestroy(*h); *h = 0; }
-    for (sm_dbuf *b : {&g.spec, &g.kmul, &g.tabs}) dfree(*b);
+    for (sm_dbuf *b : {&g.spec, &g.kmul, &g.tabs, &g.tw[0], &g.tw[1], &g.fmul[0], &g.fmul[1]}) dfree(*b);
 }
 
 inline size_t spec_bytes(size_t nx2, size_t nz2, size_t elt)
@@ -296,9 +603,10 @@ inline size_t kmul_bytes(size_t nx2, size_t nz2, size_t elt)
     return 2 * (std::max(nx2, nx2 / 2 + 1) + nz2 / 2 + 1) * 2 * elt;
 }
 
-// plans, multipliers and CPML tables for an nx2 x nz2 grid with np layers
+// plans, multipliers and CPML tables for an nx2 x nz2 grid with np layers;
+// own: hand-written transforms on the axes whose length allows them
 inline int init(Grid &g, sm_ctx *ctx, size_t nx2, size_t nz2, size_t np, size_t elt, double dx, double dz,
-                double dt, double f0, const Cpml &cp)
+                double dt, double f0, const Cpml &cp, bool own = true)
 {
     g.ctx = ctx;
     g.nx2 = (int)nx2; g.nz2 = (int)nz2; g.np = (int)np;
@@ -357,20 +665,71 @@ inline int init(Grid &g, sm_ctx *ctx, size_t nx2, size_t nz2, size_t np, size_t 
         for (int r = 0; r < K_NK; ++r) t.insert(t.end(), kz[r].begin(), kz[r].end());
         rc = upload_any(elt, g.tabs.ptr, t, st);
     }
-    // plans (propagate.c:386-497): x transforms of stride nz2 (paired:
-    // complex, stride nz2/2, in place layout), z transforms of unit stride
+    // hand-written transforms: twiddles e^{-2 pi i k/n} and the multipliers
+    // over the whole spectrum; P pairs per workgroup within 80 KiB of LDS
+    // (two workgroups per CU)
+    int lds_max = 65536;
+    if (own) {
+        const size_t want = 81920;
+        const bool ok = elt == 8
+            ? hipFuncSetAttribute((const void *)pstd_dfft<double, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess &&
+              hipFuncSetAttribute((const void *)pstd_dfft<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess
+            : hipFuncSetAttribute((const void *)pstd_dfft<float, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess &&
+              hipFuncSetAttribute((const void *)pstd_dfft<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess;
+        (void)hipGetLastError();
+        if (ok) lds_max = (int)want;
+    }
+    for (int ax = 0; ax < 2 && rc == SM_OK && own; ++ax) {
+        const size_t n = ax == 0 ? nx2 : nz2;
+        const double d = ax == 0 ? dx : dz;
+        if (!fft_factor((int)n, g.nr[ax], g.rcode[ax]) || dfft_lds((int)n, 1, elt) > (size_t)lds_max) continue;
+        // x: 4 column pairs per workgroup (adjacent columns share cache
+        // lines); z: 2 row pairs (more workgroups)
+        g.P[ax] = std::max(1, std::min(ax == 0 ? 4 : 2, (int)(lds_max / dfft_lds((int)n, 1, elt))));
+        while (g.P[ax] > 1 && dfft_lds((int)n, g.P[ax], elt) > (size_t)lds_max) --g.P[ax];
+        std::vector<double> tw, fm;
+        const double two_pi = 8.0 * std::atan(1.0);
+        for (size_t k = 0; k < n; ++k) {
+            const double th = two_pi * (double)k / (double)n;
+            tw.push_back(std::cos(th)); tw.push_back(-std::sin(th));
+        }
+        for (double sgn : {1.0, -1.0}) {
+            const double dn = two_pi / (d * n * n);
+            auto h = [&](size_t m, double &re, double &im) {
+                const double k = m * dn, ph = sgn * 0.5 * k * d * n;
+                re = -k * std::sin(ph); im = k * std::cos(ph);
+            };
+            for (size_t m = 0; m < n; ++m) {
+                double re, im;
+                if (2 * m < n) h(m, re, im);
+                else if (2 * m == n) { h(m, re, im); im = 0.0; }
+                else { h(n - m, re, im); im = -im; }
+                fm.push_back(re); fm.push_back(im);
+            }
+        }
+        rc = dalloc(g.tw[ax], tw.size() * elt);
+        if (rc == SM_OK) rc = dalloc(g.fmul[ax], fm.size() * elt);
+        if (rc == SM_OK) rc = upload_any(elt, g.tw[ax].ptr, tw, st);
+        if (rc == SM_OK) rc = upload_any(elt, g.fmul[ax].ptr, fm, st);
+        if (rc == SM_OK) g.own[ax] = true;
+    }
+    // plans (propagate.c:386-497) for the other axes: x transforms of stride
+    // nz2 (paired: complex, stride nz2/2), z transforms of unit stride
     if (rc == SM_OK) {
         const hipfftType f_t = elt == 8 ? HIPFFT_D2Z : HIPFFT_R2C, i_t = elt == 8 ? HIPFFT_Z2D : HIPFFT_C2R;
         const hipfftType c_t = elt == 8 ? HIPFFT_Z2Z : HIPFFT_C2C;
         const int NX = (int)nx2, NZ = (int)nz2, NN = (int)g.N;
-        if (g.xpair)
+        if (g.own[0]) {
+        } else if (g.xpair) {
             rc = make_plan(&g.px_f, NX, NZ / 2, NZ / 2, 1, NZ / 2, 1, NN / 2, NN / 2, c_t, st);
-        else {
+        } else {
             rc = make_plan(&g.px_f, NX, NZ, NZ, 1, 1, g.nxh, NN, g.nxh * NZ, f_t, st);
             if (rc == SM_OK) rc = make_plan(&g.px_i, NX, NZ, 1, g.nxh, NZ, 1, g.nxh * NZ, NN, i_t, st);
         }
-        if (rc == SM_OK) rc = make_plan(&g.pz_f, NZ, NX, 1, NZ, 1, g.nzh, NN, NX * g.nzh, f_t, st);
-        if (rc == SM_OK) rc = make_plan(&g.pz_i, NZ, NX, 1, g.nzh, 1, NZ, NX * g.nzh, NN, i_t, st);
+        if (rc == SM_OK && !g.own[1]) {
+            rc = make_plan(&g.pz_f, NZ, NX, 1, NZ, 1, g.nzh, NN, NX * g.nzh, f_t, st);
+            if (rc == SM_OK) rc = make_plan(&g.pz_i, NZ, NX, 1, g.nzh, 1, NZ, NX * g.nzh, NN, i_t, st);
+        }
     }
     return rc;
 }
@@ -378,7 +737,15 @@ inline int init(Grid &g, sm_ctx *ctx, size_t nx2, size_t nz2, size_t np, size_t 
 // device bytes of the machinery
 inline size_t grid_bytes(size_t nx2, size_t nz2, size_t np, size_t elt)
 {
-    return spec_bytes(nx2, nz2, elt) + kmul_bytes(nx2, nz2, elt) + table_bytes(np, elt);
+    return spec_bytes(nx2, nz2, elt) + kmul_bytes(nx2, nz2, elt) + table_bytes(np, elt) +
+           3 * (nx2 + nz2) * 2 * elt;
+}
+
+inline size_t grid_device_bytes(const Grid &g)
+{
+    size_t b = g.spec.bytes + g.kmul.bytes + g.tabs.bytes;
+    for (int ax = 0; ax < 2; ++ax) b += g.tw[ax].bytes + g.fmul[ax].bytes;
+    return b;
 }
 
 }  // namespace pstd

@@ -171,10 +171,11 @@ int stage(sm_sh *s, int tabs, bool last, const T *x, T a, T b, int set_t, int ac
     const int np = (int)s->g.np;
     T *d = (T *)s->der.ptr;
     int rc;
-    if ((rc = pstd::deriv<T>(s->g, x + W_TXY * N, 0, false, d)) != SM_OK) return rc;           // D_x^- tau_xy
-    if ((rc = pstd::deriv<T>(s->g, x + W_TYZ * N, 1, false, d + N)) != SM_OK) return rc;       // D_z^- tau_yz
-    if ((rc = pstd::deriv<T>(s->g, x + W_VY * N, 1, true, d + 2 * N)) != SM_OK) return rc;     // D_z^+ v_y
-    if ((rc = pstd::deriv<T>(s->g, x + W_VY * N, 0, true, d + 3 * N)) != SM_OK) return rc;     // D_x^+ v_y
+    // D_x^- tau_xy, D_x^+ v_y; D_z^- tau_yz, D_z^+ v_y (field, slot, forward)
+    const pstd::DJob xj[2] = {{W_TXY, L_DX_TXY, false}, {W_VY, L_DX_VY, true}};
+    const pstd::DJob zj[2] = {{W_TYZ, L_DZ_TYZ, false}, {W_VY, L_DZ_VY, true}};
+    if ((rc = pstd::deriv_batch<T>(s->g, 0, x, d, xj, 2)) != SM_OK) return rc;
+    if ((rc = pstd::deriv_batch<T>(s->g, 1, x, d, zj, 2)) != SM_OK) return rc;
     ShStage<T> k;
     k.d = d; k.mat = (const T *)s->mat.ptr; k.x = x; k.ws = STATE<T>(s, s->iws);
     k.w = STATE<T>(s, s->iw); k.wt = STATE<T>(s, s->iwt);
@@ -365,7 +366,7 @@ int sm_sh_create(sm_ctx *ctx, const sm_sh_params *pp, const double *mu, const do
              "sm_sh_create: storage must be SM_STORE_F64 or SM_STORE_F32");
     SM_CHECK(p.nx >= 2 && p.nz >= 2, SM_ERR_ARG, "sm_sh_create: the model needs at least 2 x 2 nodes");
     SM_CHECK(p.dx > 0 && p.dz > 0 && p.dt > 0, SM_ERR_ARG, "sm_sh_create: spacings and dt must be > 0");
-    SM_CHECK(p.flags == 0, SM_ERR_ARG, "sm_sh_create: unknown flags %#x", p.flags);
+    SM_CHECK((p.flags & ~SM_PSTD_HIPFFT) == 0, SM_ERR_ARG, "sm_sh_create: unknown flags %#x", p.flags);
     if (!p.abs) p.np = 0;   // io_utils.c:293: no absorbing layers without abs
     const size_t nx2 = p.nx + 2 * p.np, nz2 = p.nz + 2 * p.np, N = nx2 * nz2;
     SM_CHECK(nx2 < (1u << 20) && nz2 < (1u << 20) && N < ((size_t)1 << 30), SM_ERR_ARG,
@@ -391,7 +392,8 @@ int sm_sh_create(sm_ctx *ctx, const sm_sh_params *pp, const double *mu, const do
     pstd::Cpml cp;
     cp.abs = p.abs; cp.vmax = p.vmax; cp.pml_order = p.pml_order; cp.Rc = p.Rc; cp.kappa_max = p.kappa_max;
     cp.alpha_max = p.alpha_max;
-    if (rc == SM_OK) rc = pstd::init(s->g, ctx, nx2, nz2, p.np, e, p.dx, p.dz, p.dt, p.f0, cp);
+    if (rc == SM_OK)
+        rc = pstd::init(s->g, ctx, nx2, nz2, p.np, e, p.dx, p.dz, p.dt, p.f0, cp, !(p.flags & SM_PSTD_HIPFFT));
     if (rc == SM_OK) rc = pstd::upload_any(e, s->mat.ptr, mat, ctx->stream);
     if (rc == SM_OK) rc = zero_state(s);
     if (rc != SM_OK) { release(s); delete s; return rc; }
@@ -539,7 +541,7 @@ int sm_sh_reset(sm_sh *s)
 int sm_sh_device_bytes(const sm_sh *s, size_t *bytes)
 {
     SM_CHECK(s && bytes, SM_ERR_ARG, "sm_sh_device_bytes: NULL argument");
-    size_t b = s->g.spec.bytes + s->g.kmul.bytes + s->g.tabs.bytes;
+    size_t b = pstd::grid_device_bytes(s->g);
     for (const sm_dbuf *d : {&s->state, &s->der, &s->mat, &s->mem, &s->src, &s->rcv, &s->trace}) b += d->bytes;
     *bytes = b;
     return SM_OK;

@@ -477,6 +477,89 @@ void sm_read_model_ve_sh(const char *fn, const sm_grid *g, size_t np, double *mu
             }
 }
 
+void sm_read_grid_ve(const char *fn, sm_grid *g, int *L)
+{
+    /* io_utils.c:356-382: the keyword compared in lower case, then L */
+    FILE *f = open_or_die(fn);
+    memset(g, 0, sizeof(*g));
+    char type[64];
+    if (fscanf(f, "%zu %zu", &g->nx, &g->nz) != 2 || fscanf(f, "%lf %lf", &g->dx, &g->dz) != 2 ||
+        fscanf(f, "%lf %lf", &g->x0, &g->z0) != 2 || fscanf(f, "%63s", type) != 1)
+        die("Error: something wrong with file %s\n", fn);
+    for (char *c = type; *c; ++c) *c = (char)tolower((unsigned char)*c);
+    if (strcmp(type, "vti_viscoelastic") != 0)
+        die("Error, model file %s not for a viscoelastic VTI medium\n", fn);
+    if (fscanf(f, "%d", L) != 1 || *L < 1)
+        die("Error: something wrong with file %s\n", fn);
+    g->ny = 1;
+    g->dy = g->dx;
+    fclose(f);
+}
+
+void sm_read_model_ve(const char *fn, const sm_grid *g, size_t np, int L, double *K, double *mu,
+                      double *rho, double *eps, double *delta, double *Q1, double *f1, double *Q2,
+                      double *f2)
+{
+    /* io_utils.c:1155-1314: per node "Vp0 Vs0 rho epsilon delta", L pairs
+     * "Q1 f1", then "Q2 f2", x outer and z inner; mu = Vs0^2 rho 1e-9 and
+     * K = 1e-9 (Vp0^2 rho - 4/3 Vs0^2 rho) in GPa; one z column (copied to
+     * every x) when nothing can be read where column 1 starts; values
+     * copied into the absorbing layers from the nearest model node.  Any
+     * other shortfall is an error. */
+    FILE *f = open_or_die(fn);
+    char type[64];
+    if (fscanf(f, "%*d %*d") < 0 || fscanf(f, "%*f %*f") < 0 || fscanf(f, "%*f %*f") < 0 ||
+        fscanf(f, "%63s", type) != 1 || strcmp(type, "vti_viscoelastic") != 0)
+        die("Error, model in %s not vti_viscoelastic\n", fn);
+    if (fscanf(f, "%*d") < 0) die("Error: something wrong with file %s\n", fn);
+    const size_t nx = g->nx, nz = g->nz, nz2 = nz + 2 * np, N = (nx + 2 * np) * nz2;
+    const size_t per = 7 + 2 * (size_t)L, ncell = nx * nz;
+    size_t nv = 0;
+    double *v = read_body(f, fn, per * ncell, &nv);
+    fclose(f);
+    int flag1d = 0;
+    if (nv < per * ncell) {
+        if (nx > 1 && nv == per * nz) flag1d = 1;
+        else die("Error: something wrong with file %s\n", fn);
+    }
+    for (size_t i = 0; i < nx; ++i)
+        for (size_t j = 0; j < nz; ++j) {
+            const double *c = v + per * ((flag1d ? 0 : i) * nz + j);
+            const size_t ind = (i + np) * nz2 + j + np;
+            const double vp0 = c[0], vs0 = c[1];
+            rho[ind] = c[2];
+            eps[ind] = c[3];
+            delta[ind] = c[4];
+            const double mu_pa = vs0 * vs0 * rho[ind];
+            K[ind] = 1.e-9 * (vp0 * vp0 * rho[ind] - 4. / 3. * mu_pa);
+            mu[ind] = mu_pa * 1.e-9;
+            for (int m = 0; m < L; ++m) {
+                Q1[(size_t)m * N + ind] = c[5 + 2 * m];
+                f1[(size_t)m * N + ind] = c[6 + 2 * m];
+            }
+            Q2[ind] = c[5 + 2 * L];
+            f2[ind] = c[6 + 2 * L];
+        }
+    free(v);
+    const int na = 7 + 2 * L;
+    double **a = malloc((size_t)na * sizeof(double *));
+    a[0] = K; a[1] = mu; a[2] = rho; a[3] = eps; a[4] = delta; a[5] = Q2; a[6] = f2;
+    for (int m = 0; m < L; ++m) { a[7 + 2 * m] = Q1 + (size_t)m * N; a[8 + 2 * m] = f1 + (size_t)m * N; }
+    for (size_t i = 0; i < nx; ++i)
+        for (size_t j = 0; j < np; ++j)
+            for (int q = 0; q < na; ++q) {
+                a[q][(i + np) * nz2 + j] = a[q][(i + np) * nz2 + np];
+                a[q][(i + np) * nz2 + np + nz + j] = a[q][(i + np) * nz2 + np + nz - 1];
+            }
+    for (size_t i = 0; i < np; ++i)
+        for (size_t j = 0; j < nz2; ++j)
+            for (int q = 0; q < na; ++q) {
+                a[q][i * nz2 + j] = a[q][np * nz2 + j];
+                a[q][(i + nx + np) * nz2 + j] = a[q][(nx + np - 1) * nz2 + j];
+            }
+    free(a);
+}
+
 void sm_read_model_a(const char *fn, const sm_grid *g, size_t np, double *vp, double *rho)
 {
     /* io_utils.c:1029-1153 */
@@ -868,9 +951,12 @@ static void att_f64(buf *o, const char *name, const double *v, size_t n)
     for (size_t i = 0; i < n; ++i) put_f64(o, v[i]);
 }
 
-static int snapshot_nc(const double *data, double t, size_t nx, size_t nz, size_t np,
-                       double x0, double z0, double dx, double dz, const char *basename,
-                       size_t nr, int plstr, int cyl)
+/* field: NULL for a snapshot (ssh/<base>/<base>_<nr+1>_<t>.nc, values in
+ * mm/s), else write_field_nc's <base>_<field>.nc with the values unscaled
+ * and the given units */
+static int grid_nc(const double *data, double t, size_t nx, size_t nz, size_t np,
+                   double x0, double z0, double dx, double dz, const char *basename,
+                   size_t nr, int plstr, int cyl, const char *field, const char *units)
 {
     /* write_snapshot_nc / write_snapshot3d_nc (io_utils.c:1822-2180): dims
      * x, y; double coordinate variables x(x), y(y); float z(y, x) in mm/s.
@@ -879,10 +965,15 @@ static int snapshot_nc(const double *data, double t, size_t nx, size_t nz, size_
      * strips the reference still skips np rows (io_utils.c:1985-1991), and
      * so does this writer. */
     char fn[3 * SM_NAME_LEN + 64];
-    make_dir("ssh");
-    snprintf(fn, sizeof(fn), "ssh/%s", basename);
-    make_dir(fn);
-    snprintf(fn, sizeof(fn), "ssh/%s/%s_%zu_%lf.nc", basename, basename, nr + 1, t);
+    const double scale = field ? 1.0 : 1.e3;
+    if (field) {
+        snprintf(fn, sizeof(fn), "%s_%s.nc", basename, field);
+    } else {
+        make_dir("ssh");
+        snprintf(fn, sizeof(fn), "ssh/%s", basename);
+        make_dir(fn);
+        snprintf(fn, sizeof(fn), "ssh/%s/%s_%zu_%lf.nc", basename, basename, nr + 1, t);
+    }
     const size_t nx2 = cyl ? nx + np : nx + 2 * np, nz2 = nz + 2 * np;
     const size_t mx = plstr ? nx2 : nx, mz = plstr ? nz2 : nz;
     const size_t ox = plstr ? 0 : np, oz = plstr ? 0 : np;
@@ -894,10 +985,10 @@ static int snapshot_nc(const double *data, double t, size_t nx, size_t nz, size_
         xr[0] = x0; xr[1] = x0 + (nx - 1) * dx;
         yr[0] = z0; yr[1] = z0 + (nz - 1) * dz;
     }
-    zr[0] = zr[1] = 1.e3 * data[ox * nz2 + oz];
+    zr[0] = zr[1] = scale * data[ox * nz2 + oz];
     for (size_t i = 0; i < mx; ++i)
         for (size_t j = 0; j < mz; ++j) {
-            const double v = 1.e3 * data[(i + ox) * nz2 + j + oz];
+            const double v = scale * data[(i + ox) * nz2 + j + oz];
             zr[0] = zr[0] < v ? zr[0] : v;
             zr[1] = zr[1] > v ? zr[1] : v;
         }
@@ -928,7 +1019,10 @@ static int snapshot_nc(const double *data, double t, size_t nx, size_t nz, size_
             put_u32(&h, NC_DOUBLE);
         } else {
             put_u32(&h, 4);
-            att_text(&h, "units", "mm/s", 5);
+            /* write_snapshot_nc counts the NUL of "mm/s", write_field_nc
+             * gives strlen(units) */
+            if (field) att_text(&h, "units", units, strlen(units));
+            else att_text(&h, "units", "mm/s", 5);
             const double one = 1.0;
             att_f64(&h, "scale_factor", &one, 1);
             put_name(&h, "_FillValue"); put_u32(&h, NC_FLOAT); put_u32(&h, 1);
@@ -953,7 +1047,8 @@ static int snapshot_nc(const double *data, double t, size_t nx, size_t nz, size_
         put_f64(&h, plstr ? z0 + ((1.0 * j) - np) * dz : z0 + j * dz);
     for (size_t j = 0; j < mz; ++j)
         for (size_t i = 0; i < mx; ++i)
-            put_f32(&h, (float)data[(i + ox) * nz2 + j + oz] * 1.e3f);
+            put_f32(&h, field ? (float)data[(i + ox) * nz2 + j + oz]
+                              : (float)data[(i + ox) * nz2 + j + oz] * 1.e3f);
     put_pad(&h);
     FILE *f = fopen(fn, "wb");
     if (!f) {
@@ -971,14 +1066,21 @@ int sm_write_snapshot_nc(const double *data, double t, size_t nx, size_t nz, siz
                          double x0, double z0, double dx, double dz, const char *basename,
                          size_t nr, int plstr)
 {
-    return snapshot_nc(data, t, nx, nz, np, x0, z0, dx, dz, basename, nr, plstr, 0);
+    return grid_nc(data, t, nx, nz, np, x0, z0, dx, dz, basename, nr, plstr, 0, NULL, NULL);
+}
+
+int sm_write_field_nc(const double *data, const char *field, const char *units, size_t nx, size_t nz,
+                      size_t np, double x0, double z0, double dx, double dz, const char *basename,
+                      int plstr)
+{
+    return grid_nc(data, 0.0, nx, nz, np, x0, z0, dx, dz, basename, 0, plstr, 0, field, units);
 }
 
 int sm_write_snapshot_cyl_nc(const double *data, double t, size_t nx, size_t nz, size_t np,
                              double x0, double z0, double dx, double dz, const char *basename,
                              size_t nr, int plstr)
 {
-    return snapshot_nc(data, t, nx, nz, np, x0, z0, dx, dz, basename, nr, plstr, 1);
+    return grid_nc(data, t, nx, nz, np, x0, z0, dx, dz, basename, nr, plstr, 1, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------ */

@@ -101,6 +101,14 @@ void sm_read_output2d_np(const char *fn, const sm_grid *g, size_t np, sm_records
 void sm_read_grid_ve_sh(const char *fn, sm_grid *g);
 void sm_read_model_ve_sh(const char *fn, const sm_grid *g, size_t np, double *mu, double *rho,
                          double *gamma, double *Q, double *fr);
+/* read_grid_params_ve / read_model_ve (io_utils.c:356-382, 1155-1314):
+ * the grid and the number L of quasi-dilatational relaxation mechanisms;
+ * padded (nx + 2np) x (nz + 2np) arrays, K and mu in GPa, Q1 and f1 holding
+ * L arrays one after the other */
+void sm_read_grid_ve(const char *fn, sm_grid *g, int *L);
+void sm_read_model_ve(const char *fn, const sm_grid *g, size_t np, int L, double *K, double *mu,
+                      double *rho, double *eps, double *delta, double *Q1, double *f1, double *Q2,
+                      double *f2);
 void sm_read_output3d(const char *fn, const sm_grid *g, size_t np, sm_records *out);
 /* check_dt_trc (io_utils.c:3561-3574) */
 void sm_check_dt_trc(const sm_records *out, double dt);
@@ -112,6 +120,12 @@ void sm_close_traces(sm_records *out);
 int sm_write_snapshot_nc(const double *data, double t, size_t nx, size_t nz, size_t np,
                          double x0, double z0, double dx, double dz, const char *basename,
                          size_t nr, int plstr);
+/* write_field_nc (io_utils.c:2183-2352): a padded nx2 x nz2 field written
+ * unscaled to <basename>_<field>.nc in the current directory, with the
+ * given units */
+int sm_write_field_nc(const double *data, const char *field, const char *units, size_t nx, size_t nz,
+                      size_t np, double x0, double z0, double dx, double dz, const char *basename,
+                      int plstr);
 /* the same for e_cyl_iso's cylindrical grid: data is (nx+np) x (nz+2np) */
 int sm_write_snapshot_cyl_nc(const double *data, double t, size_t nx, size_t nz, size_t np,
                              double x0, double z0, double dx, double dz, const char *basename,

@@ -610,11 +610,12 @@ class ShConfig:
     alpha_max: int = 1
     abs: bool = True
     storage: str = "f64"
+    hipfft: bool = False      # every derivative through hipFFT (SM_PSTD_HIPFFT)
 
     def params(self) -> L.ShParams:
         return L.ShParams(self.nx, self.nz, self.np, self.dx, self.dz, self.dt, self.vmax,
                           self.pml_order, self.Rc, self.kappa_max, self.alpha_max, self.f0,
-                          1 if self.abs else 0, _STORAGE[self.storage], 0)
+                          1 if self.abs else 0, _STORAGE[self.storage], L.SM_PSTD_HIPFFT if self.hipfft else 0)
 
     @property
     def shape(self) -> tuple[int, int]:
@@ -626,10 +627,83 @@ class ShConfig:
         return int(lib.sm_sh_bytes_needed(C.byref(p), nrec, max_steps))
 
 
-class ViscoelasticSH:
+class _Pstd:
+    """Common driver of the pseudo-spectral solvers: `_pre` is the C prefix
+    (sm_sh, sm_ve), FIELDS the recordable components."""
+
+    _pre = ""
+    FIELDS: dict = {}
+
+    def _f(self, name):
+        return getattr(lib, f"{self._pre}_{name}")
+
+    def set_sources(self, types, idx, lengths, fct) -> None:
+        t = np.ascontiguousarray(types, dtype=np.intc)
+        i = np.ascontiguousarray(idx, dtype=np.uintp)
+        ln = np.ascontiguousarray(lengths, dtype=np.uintp)
+        f = _f64(fct)
+        check(self._f("set_sources")(self.handle, len(t), t.ctypes.data_as(C.POINTER(C.c_int)), _sp(i),
+                                     _sp(ln), _dp(f)), f"{self._pre}_set_sources")
+
+    def set_receivers(self, idx, comp, max_steps: int) -> None:
+        i = np.ascontiguousarray(idx, dtype=np.uintp)
+        c = np.ascontiguousarray(comp, dtype=np.intc)
+        check(self._f("set_receivers")(self.handle, len(i), _sp(i), c.ctypes.data_as(C.POINTER(C.c_int)),
+                                       max_steps), f"{self._pre}_set_receivers")
+        self.nrec = len(i)
+
+    def run(self, nsteps: int) -> None:
+        check(self._f("run")(self.handle, nsteps), f"{self._pre}_run")
+
+    def set_timing(self, on: bool) -> None:
+        check(self._f("set_timing")(self.handle, 1 if on else 0), f"{self._pre}_set_timing")
+
+    def timing(self) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_size_t()
+        check(self._f("get_timing")(self.handle, C.byref(ms), C.byref(n)), f"{self._pre}_get_timing")
+        return ms.value, n.value
+
+    @property
+    def step(self) -> int:
+        it = C.c_size_t()
+        check(self._f("current_step")(self.handle, C.byref(it)), f"{self._pre}_current_step")
+        return it.value
+
+    def traces(self, it0: int, count: int) -> np.ndarray:
+        out = np.zeros((count, self.nrec), dtype=np.float64)
+        check(self._f("get_traces")(self.handle, it0, count, _dp(out)), f"{self._pre}_get_traces")
+        return out
+
+    def field(self, name: str) -> np.ndarray:
+        out = np.zeros(self.cfg.shape, dtype=np.float64)
+        check(self._f("get_field")(self.handle, self.FIELDS[name], _dp(out)), f"{self._pre}_get_field")
+        return out
+
+    def reset(self) -> None:
+        check(self._f("reset")(self.handle), f"{self._pre}_reset")
+
+    def device_bytes(self) -> int:
+        b = C.c_size_t()
+        check(self._f("device_bytes")(self.handle, C.byref(b)), f"{self._pre}_device_bytes")
+        return b.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            check(self._f("destroy")(self.handle), f"{self._pre}_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ViscoelasticSH(_Pstd):
     """Viscoelastic SH waves in a 2D VTI medium (ve_vti_sh_pml): hipFFT
     derivatives, CPML, fourth-order Runge-Kutta."""
 
+    _pre = "sm_sh"
     FIELDS = {"vy": L.SM_VY, "txy": L.SM_TXY, "tyz": L.SM_TYZ}
 
     def __init__(self, ctx: Context, cfg: ShConfig, mu, rho, gamma, Q, f):
@@ -646,63 +720,76 @@ class ViscoelasticSH:
         self.handle = h
         self.nrec = 0
 
-    def set_sources(self, types, idx, lengths, fct) -> None:
-        t = np.ascontiguousarray(types, dtype=np.intc)
-        i = np.ascontiguousarray(idx, dtype=np.uintp)
-        ln = np.ascontiguousarray(lengths, dtype=np.uintp)
-        f = _f64(fct)
-        check(lib.sm_sh_set_sources(self.handle, len(t), t.ctypes.data_as(C.POINTER(C.c_int)), _sp(i),
-                                    _sp(ln), _dp(f)), "sm_sh_set_sources")
 
-    def set_receivers(self, idx, comp, max_steps: int) -> None:
-        i = np.ascontiguousarray(idx, dtype=np.uintp)
-        c = np.ascontiguousarray(comp, dtype=np.intc)
-        check(lib.sm_sh_set_receivers(self.handle, len(i), _sp(i), c.ctypes.data_as(C.POINTER(C.c_int)),
-                                      max_steps), "sm_sh_set_receivers")
-        self.nrec = len(i)
+@dataclass
+class VeConfig:
+    """ve_vti_pml grid: nx by nz interior nodes and np CPML layers on every
+    side (none when abs is False), padded (nx + 2np) by (nz + 2np); L
+    quasi-dilatational relaxation mechanisms."""
+    nx: int
+    nz: int
+    np: int
+    dx: float
+    dz: float
+    dt: float
+    f0: float
+    L: int = 1
+    vmax: float = 5000.0
+    pml_order: float = 2.0
+    Rc: float = 1e-5
+    kappa_max: float = 1.0
+    alpha_max: int = 1
+    abs: bool = True
+    storage: str = "f64"
+    hipfft: bool = False      # every derivative through hipFFT (SM_PSTD_HIPFFT)
 
-    def run(self, nsteps: int) -> None:
-        check(lib.sm_sh_run(self.handle, nsteps), "sm_sh_run")
-
-    def set_timing(self, on: bool) -> None:
-        check(lib.sm_sh_set_timing(self.handle, 1 if on else 0), "sm_sh_set_timing")
-
-    def timing(self) -> tuple[float, int]:
-        ms, n = C.c_double(), C.c_size_t()
-        check(lib.sm_sh_get_timing(self.handle, C.byref(ms), C.byref(n)), "sm_sh_get_timing")
-        return ms.value, n.value
+    def params(self) -> L.VeParams:
+        return L.VeParams(self.nx, self.nz, self.np, self.dx, self.dz, self.dt, self.vmax,
+                          self.pml_order, self.Rc, self.kappa_max, self.alpha_max, self.f0,
+                          1 if self.abs else 0, self.L, _STORAGE[self.storage],
+                          L.SM_PSTD_HIPFFT if self.hipfft else 0)
 
     @property
-    def step(self) -> int:
-        it = C.c_size_t()
-        check(lib.sm_sh_current_step(self.handle, C.byref(it)), "sm_sh_current_step")
-        return it.value
+    def shape(self) -> tuple[int, int]:
+        np_ = self.np if self.abs else 0
+        return (self.nx + 2 * np_, self.nz + 2 * np_)
 
-    def traces(self, it0: int, count: int) -> np.ndarray:
-        out = np.zeros((count, self.nrec), dtype=np.float64)
-        check(lib.sm_sh_get_traces(self.handle, it0, count, _dp(out)), "sm_sh_get_traces")
-        return out
+    def bytes_needed(self, nrec: int = 0, max_steps: int = 0) -> int:
+        p = self.params()
+        return int(lib.sm_ve_bytes_needed(C.byref(p), nrec, max_steps))
 
-    def field(self, name: str) -> np.ndarray:
-        out = np.zeros(self.cfg.shape, dtype=np.float64)
-        check(lib.sm_sh_get_field(self.handle, self.FIELDS[name], _dp(out)), "sm_sh_get_field")
-        return out
 
-    def reset(self) -> None:
-        check(lib.sm_sh_reset(self.handle), "sm_sh_reset")
+def ve_moduli(L_: int, K, mu, rho, eps, delta, Q1, f1, Q2, f2) -> dict:
+    """c11, c13, c33, c55 and K0 (MPa) as ve_vti_pml writes them
+    (ve_vti_pml.c:695-699), from read_model_ve's arrays; host only."""
+    K = _f64(K)
+    nx2, nz2 = K.shape
+    arrs = [_f64(a).reshape(-1) for a in (K, mu, rho, eps, delta, Q1, f1, Q2, f2)]
+    out = {k: np.zeros((nx2, nz2)) for k in ("c11", "c13", "c33", "c55", "K0")}
+    check(lib.sm_ve_moduli(nx2, nz2, L_, *(_dp(a) for a in arrs), *(_dp(o) for o in out.values())),
+          "sm_ve_moduli")
+    return out
 
-    def device_bytes(self) -> int:
-        b = C.c_size_t()
-        check(lib.sm_sh_device_bytes(self.handle, C.byref(b)), "sm_sh_device_bytes")
-        return b.value
 
-    def close(self) -> None:
-        if getattr(self, "handle", None):
-            check(lib.sm_sh_destroy(self.handle), "sm_sh_destroy")
-            self.handle = None
+class ViscoelasticPSV(_Pstd):
+    """Viscoelastic P-SV waves in a 2D VTI medium (ve_vti_pml): hipFFT
+    derivatives, CPML, fourth-order Runge-Kutta, L + 1 relaxation
+    mechanisms."""
 
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+    _pre = "sm_ve"
+    FIELDS = {"vx": L.SM_VX, "vz": L.SM_VZ, "txx": L.SM_TXX, "tzz": L.SM_TZZ, "txz": L.SM_TXZ}
+
+    def __init__(self, ctx: Context, cfg: VeConfig, K, mu, rho, eps, delta, Q1, f1, Q2, f2):
+        self.ctx, self.cfg = ctx, cfg
+        n = int(np.prod(cfg.shape))
+        self._m = [_f64(a).reshape(-1) for a in (K, mu, rho, eps, delta, Q1, f1, Q2, f2)]
+        want = [n, n, n, n, n, cfg.L * n, cfg.L * n, n, n]
+        if any(a.size != w for a, w in zip(self._m, want)):
+            raise ValueError(f"model must hold {n} padded cells ({cfg.L} arrays for Q1 and f1)")
+        self._p = cfg.params()
+        h = C.c_void_p()
+        ctx.adopt(self)
+        check(lib.sm_ve_create(ctx.handle, C.byref(self._p), *(_dp(a) for a in self._m), C.byref(h)),
+              "sm_ve_create")
+        self.handle = h
+        self.nrec = 0

@@ -396,11 +396,81 @@ def sh_layered_shot(n: int = 1000, np_: int = 20, h: float = 5.0, seed: int = 7,
 
 
 def algorithmic_bytes_sh(elt: int = 8) -> float:
-    """HBM bytes per padded cell per step of the ve_vti_sh_pml update, as a
-    streaming minimum for its structure: per Runge-Kutta stage, three FFT
-    derivatives each reading a field and writing its derivative (2 values),
-    the right-hand sides reading the five state values and nine material
-    arrays and writing five rates (19), and the stage update reading W, Ws
-    and the rates and writing W and the next stage's input (25): 50 values
-    per stage, 200 per step."""
-    return elt * 200.0
+    """HBM bytes per padded cell per step of the ve_vti_sh_pml update, as the
+    streaming minimum of its structure when every FFT derivative is one pass
+    (read the field, write the derivative) and the rest of a Runge-Kutta
+    stage is one pointwise pass: 4 stages x 4 derivatives x 2 values (32),
+    and per stage the layer corrections, right-hand sides and update reading
+    the 4 derivatives, 9 material arrays, the 5 step-start values, the 5
+    accumulated values (stages 2-4) and the stage input's 2 relaxation
+    variables (stages 2-4), writing 5 accumulated values and 5 next-stage
+    inputs (stages 1-3): 28 + 35 + 35 + 30 = 128.  160 values per step."""
+    return elt * 160.0
+
+
+@dataclass
+class VeShot:
+    nx: int
+    nz: int
+    np: int
+    h: float
+    dt: float
+    f0: float
+    nsteps: int
+    L: int
+    model: tuple            # read_model_ve's padded arrays: K, mu, rho, eps, delta, Q1, f1, Q2, f2
+    src_idx: np.ndarray     # one Bulk source
+    rec_idx: np.ndarray
+    rec_comp: np.ndarray
+
+    @property
+    def cells(self) -> int:
+        return int(self.model[0].size)
+
+
+def ve_layered_shot(n: int = 1000, np_: int = 20, h: float = 5.0, seed: int = 11, f0: float = 25.0,
+                    nsteps: int = 1000, nrec: int = 200, L: int = 2) -> VeShot:
+    """ve_vti_pml workload: n x n nodes of h metres plus np CPML layers,
+    5-10 horizontal layers with Vp0 ~ U[1800, 4000] m/s (sorted with depth),
+    Vp0/Vs0 ~ U[1.7, 2.2], rho ~ U[1800, 2600], epsilon ~ U[0, 0.25],
+    delta ~ U[-0.05, 0.15], L quasi-dilatational mechanisms with Q ~
+    U[20, 100] and relaxation frequencies spread over [5, 60] Hz, shear
+    Q ~ U[20, 100] at f0; a Bulk source at (n/2, 20) nodes and a line of
+    nrec Vz receivers at depth 10 nodes.  dt at 40% of the RK4
+    pseudo-spectral stability limit."""
+    rng = np.random.default_rng(seed)
+    nl = int(rng.integers(5, 11))
+    tops = np.sort(rng.choice(np.arange(1, n), size=nl - 1, replace=False))
+    lay = np.searchsorted(tops, np.arange(n), side="right")
+    one = np.ones((n, 1))
+    vp = np.sort(rng.uniform(1800.0, 4000.0, nl))[lay] * one
+    vs = vp / rng.uniform(1.7, 2.2, nl)[lay]
+    rho = rng.uniform(1800.0, 2600.0, nl)[lay] * one
+    eps = rng.uniform(0.0, 0.25, nl)[lay] * one
+    dlt = rng.uniform(-0.05, 0.15, nl)[lay] * one
+    Q1 = [rng.uniform(20.0, 100.0, nl)[lay] * one for _ in range(L)]
+    f1 = [np.full((n, n), 5.0 + 55.0 * (m + 0.5) / L) for m in range(L)]
+    Q2 = rng.uniform(20.0, 100.0, nl)[lay] * one
+    pad = lambda a: np.pad(a, np_, mode="edge")
+    mu_pa = vs * vs * rho
+    model = (pad(1e-9 * (vp * vp * rho - 4. / 3. * mu_pa)), pad(mu_pa * 1e-9), pad(rho), pad(eps), pad(dlt),
+             np.stack([pad(q) for q in Q1]), np.stack([pad(q) for q in f1]), pad(Q2), pad(np.full((n, n), f0)))
+    vmax = float((vp * np.sqrt(1 + 2 * eps)).max())
+    dt = 0.4 * 2.0 * h / (np.pi * vmax * np.sqrt(2.0))
+    nz2 = n + 2 * np_
+    src = np.array([(n // 2 + np_) * nz2 + 20 + np_], dtype=np.uintp)
+    xs = np.linspace(np_, n + np_ - 1, nrec).astype(int)
+    rec = np.array([x * nz2 + 10 + np_ for x in xs], dtype=np.uintp)
+    return VeShot(nx=n, nz=n, np=np_, h=h, dt=float(dt), f0=f0, nsteps=nsteps, L=L, model=model, src_idx=src,
+                  rec_idx=rec, rec_comp=np.full(nrec, 2, dtype=np.intc))
+
+
+def algorithmic_bytes_ve(L: int = 2, elt: int = 8) -> float:
+    """HBM bytes per padded cell per step of the ve_vti_pml update, counted as
+    for algorithmic_bytes_sh: 4 stages x 8 derivatives x 2 values (64), and
+    the fused pointwise pass per stage reading the 8 derivatives, 12 + 2L
+    material arrays, the 7 + L step-start values, the 7 + L accumulated
+    values (stages 2-4) and the stage input's L + 2 relaxation variables
+    (stages 2-4), writing 7 + L accumulated values and 7 + L next-stage
+    inputs (stages 1-3): 184 + 25 L.  248 + 25 L values per step."""
+    return elt * (248.0 + 25.0 * L)

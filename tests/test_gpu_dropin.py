@@ -495,3 +495,97 @@ def test_ve_vti_sh_pml_segy(tmp_path):
         pos += 240 + 4 * nsr
         ntr += 1
     assert ntr == 1 and pos == len(b)
+
+
+def deck_ve(tmp, nx=44, nz=38, np_=8, h=5.0, dt_ms=0.25, time_s=0.05, template=1, L=2):
+    rng = np.random.default_rng(23)
+    top = np.arange(nz)[None, :] < nz // 2
+    ones = np.ones((nx, 1))
+    vp = np.where(top, 2400.0, 3200.0) * ones * (1 + 0.02 * rng.uniform(-1, 1, (nx, nz)))
+    vs = np.where(top, 1300.0, 1800.0) * ones
+    rho = np.where(top, 1900.0, 2300.0) * ones
+    eps = np.where(top, 0.1, 0.2) * ones
+    dlt = np.where(top, 0.05, 0.1) * ones
+    Q1 = [np.where(top, 40.0 + 10 * m, 80.0) * ones for m in range(L)]
+    f1 = [np.full((nx, nz), 20.0 + 15 * m) for m in range(L)]
+    Q2 = np.where(top, 30.0, 60.0) * ones
+    f2 = np.full((nx, nz), 40.0)
+    with open(tmp / "model.dat", "w") as f:
+        f.write(f"{nx} {nz}\n{h} {h}\n0.0 0.0\nvti_viscoelastic\n{L}\n")
+        for i in range(nx):
+            for j in range(nz):
+                vals = [vp, vs, rho, eps, dlt] + [a for m in range(L) for a in (Q1[m], f1[m])] + [Q2, f2]
+                f.write(" ".join(repr(float(a[i, j])) for a in vals) + "\n")
+    (tmp / "src.dat").write_text(f"2\n{template}\nBulk\n2.0\n60.0\n110.0 60.0\nSxz\n0.5\n60.0\n70.0 120.0\n")
+    recs = [("Vx", "Trace", "60.0 25.0", 0.0, dt_ms), ("Vz", "Trace", "150.0 75.0", 0.01, 2 * dt_ms),
+            ("txx", "Trace", "190.0 160.0", 0.0, dt_ms), ("tzz", "Trace", "30.0 150.0", 0.0, dt_ms),
+            ("txz", "Trace", "100.0 100.0", 0.0, dt_ms), ("Vz", "Snapshot", None, 0.025, 0.0)]
+    with open(tmp / "rcv.dat", "w") as f:
+        f.write(f"{len(recs)}\n")
+        for comp, kind, xz, t0, rdt in recs:
+            f.write(f"{comp}\n{kind}\n" + (f"{xz}\n" if xz else "") + f"{t0}\n{rdt}\n")
+    (tmp / "run.par").write_text(
+        "model.dat # model, file\nsrc.dat # source, file\nrcv.dat # output, records\n"
+        f"ve # basename, name\n{time_s} # time, s\n{dt_ms} # dt, ms\n1 # abs, on\n{np_} # nl, layers\n"
+        "3 # kappa_max, PML\n1 # plstr, strips\n")
+    nx2, nz2 = nx + 2 * np_, nz + 2 * np_
+    pad = lambda a: np.pad(a, np_, mode="edge")
+    cell = lambda x, z: (int(round(x / h)) + np_) * nz2 + int(round(z / h)) + np_
+    comp = {"Vx": 0, "Vz": 2, "txx": 5, "tzz": 6, "txz": 7}
+    rec = [(cell(*map(float, xz.split())), comp[c], t0, rdt) for c, k, xz, t0, rdt in recs if xz]
+    mu_pa = vs * vs * rho
+    model = (pad(1e-9 * (vp * vp * rho - 4. / 3. * mu_pa)), pad(mu_pa * 1e-9), pad(rho), pad(eps), pad(dlt),
+             np.stack([pad(q) for q in Q1]), np.stack([pad(q) for q in f1]), pad(Q2), pad(f2))
+    return dict(nx=nx, nz=nz, np=np_, h=h, dt=dt_ms * 1e-3, model=model, rec=rec, template=template,
+                srcs=[(6, cell(110.0, 60.0), 2.0), (4, cell(70.0, 120.0), 0.5)], snap_t0=0.025)
+
+
+def oracle_ve(d, nsteps):
+    from oracle import oracle_ve as V
+    import math
+    nz2 = d["nz"] + 2 * d["np"]
+    pts = []
+    w = [0.5, 0.125, 0.25, 0.125, 0.25, 0.25, 0.125, 0.25, 0.125]
+    di = [0, -1, 0, 1, -1, 1, -1, 0, 1]
+    dj = [0, -1, -1, -1, 0, 0, 1, 1, 1]
+    for typ, c, A in d["srcs"]:
+        if d["template"] == 9:
+            pts += [(typ, c + di[q] * nz2 + dj[q], w[q] * A) for q in range(9)]
+        else:
+            pts.append((typ, c, A))
+    L, fct = O.source_functions([p[2] for p in pts], [60.0] * len(pts), 0.5 * d["dt"])
+    fct = fct / (d["h"] * d["h"])
+    off = np.concatenate([[0], np.cumsum(L)]).astype(np.int64)
+    src = [(p[0], p[1], fct[off[k]:off[k + 1]]) for k, p in enumerate(pts)]
+    cfg = dict(nx=d["nx"], nz=d["nz"], np=d["np"], dx=d["h"], dz=d["h"], dt=d["dt"], vmax=5000.0,
+               kappa_max=3.0, alpha_max=math.pi * 60.0, iwipe=1)
+    mat = V.ve_material(*d["model"])
+    tr, W = V.ve_run(cfg, mat, src, [r[0] for r in d["rec"]], [r[1] for r in d["rec"]], nsteps)
+    return tr, W, mat
+
+
+@pytest.mark.parametrize("template", [1, 9])
+def test_ve_vti_pml_program(tmp_path, template):
+    """The drop-in ve_vti_pml (FP64) against the oracle: every trace line,
+    the Vz snapshot (with the absorbing strips) and the five moduli files
+    of ve_vti_pml.c:695-699."""
+    from seismod_amd.solver import nsteps as sm_nsteps, record_due
+    d = deck_ve(tmp_path, template=template)
+    run("ve_vti_pml", tmp_path, "f64")
+    ns = sm_nsteps(0.05, d["dt"])
+    tr, _, mat = oracle_ve(d, ns)
+    for q, (_, _, t0, rdt) in enumerate(d["rec"]):
+        got = read_trc(tmp_path / "trc" / "ve" / f"ve_{q + 1}.trc")
+        _close_lines(got, trace_lines(tr[:, q], d["dt"], t0, rdt * 1e-3), q)
+    snaps = sorted((tmp_path / "ssh" / "ve").glob("ve_6_*.nc"))
+    assert len(snaps) == 1
+    it_snap = [it for it in range(ns) if record_due(it, d["dt"], d["snap_t0"], 0.0)]
+    _, W, _ = oracle_ve(d, it_snap[0] + 1)
+    want = (W[4].T.astype(np.float32) * np.float32(1e3)).astype(np.float64)
+    z = nc_field(snaps[0])
+    assert z.shape == want.shape
+    assert np.abs(z - want).max() <= 1e-5 * np.abs(want).max()
+    for name in ("c11", "c13", "c33", "c55", "K0"):
+        z = nc_field(tmp_path / f"ve_{name}.nc")
+        want = mat[name].T.astype(np.float32).astype(np.float64)
+        assert np.allclose(z, want, rtol=2.5e-7, atol=0), name   # one FP32 ulp

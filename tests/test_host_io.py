@@ -532,3 +532,80 @@ def test_read_source_and_records_padded_2d(tmp_path):
     LIB.sm_read_output2d_np(str(tmp_path / "r.dat").encode(), C.byref(g), C.c_size_t(np_), C.byref(out))
     assert (out.r[0].comp, out.r[0].idx) == (1, (4 + np_) * nz2 - 2 + np_)
     assert (out.r[1].comp, out.r[1].idx) == (9, (20 + np_) * nz2 + 5 + np_)
+
+
+@pytest.mark.parametrize("column", [False, True])
+def test_read_model_ve_layout_and_padding(tmp_path, column):
+    """read_grid_params_ve / read_model_ve (io_utils.c:356-382, 1155-1314):
+    per node "Vp0 Vs0 rho epsilon delta", L pairs "Q1 f1", "Q2 f2", z
+    fastest, into the (nx+2np) x (nz+2np) grid; mu = Vs0^2 rho 1e-9 and
+    K = 1e-9 (Vp0^2 rho - 4/3 Vs0^2 rho) in GPa; one column fills every x;
+    edges copied into all four layers."""
+    rng = np.random.default_rng(8)
+    nx, nz, np_, L = 4, 5, 2, 2
+    nv = 7 + 2 * L
+    m = np.stack([rng.uniform(2000, 4500, (nx, nz)), rng.uniform(900, 2200, (nx, nz)),
+                  rng.uniform(1500, 2700, (nx, nz)), rng.uniform(0, 0.3, (nx, nz)),
+                  rng.uniform(-0.1, 0.2, (nx, nz))] + [rng.uniform(5, 90, (nx, nz)) for _ in range(2 * L + 2)])
+    if column:
+        m[:] = m[:, :1, :]
+    fn = tmp_path / "ve.dat"
+    with open(fn, "w") as f:
+        f.write(f"{nx} {nz}\n2.0 2.5\n0.0 0.0\nvti_viscoelastic\n{L}\n")
+        for i in range(1 if column else nx):
+            for j in range(nz):
+                f.write(" ".join(repr(float(m[q, i, j])) for q in range(nv)) + "\n")
+    g = Grid()
+    Lr = C.c_int()
+    LIB.sm_read_grid_ve(str(fn).encode(), C.byref(g), C.byref(Lr))
+    assert (g.nx, g.nz, g.dx, g.dz, Lr.value) == (nx, nz, 2.0, 2.5, L)
+    shp = (nx + 2 * np_, nz + 2 * np_)
+    out = [np.zeros(shp) for _ in range(5)] + [np.zeros((L,) + shp), np.zeros((L,) + shp)] + \
+        [np.zeros(shp), np.zeros(shp)]
+    LIB.sm_read_model_ve(str(fn).encode(), C.byref(g), C.c_size_t(np_), C.c_int(L),
+                         *(o.ctypes.data_as(C.c_void_p) for o in out))
+    vp, vs, rho = m[0], m[1], m[2]
+    mu_pa = vs * vs * rho
+    want = [1e-9 * (vp * vp * rho - 4. / 3. * mu_pa), mu_pa * 1e-9, rho, m[3], m[4]]
+    for q in range(5):
+        assert np.array_equal(out[q], np.pad(want[q], np_, mode="edge")), q
+    for k in range(L):
+        assert np.array_equal(out[5][k], np.pad(m[5 + 2 * k], np_, mode="edge"))
+        assert np.array_equal(out[6][k], np.pad(m[6 + 2 * k], np_, mode="edge"))
+    assert np.array_equal(out[7], np.pad(m[5 + 2 * L], np_, mode="edge"))
+    assert np.array_equal(out[8], np.pad(m[6 + 2 * L], np_, mode="edge"))
+
+
+def test_field_netcdf(tmp_path, monkeypatch):
+    """write_field_nc (io_utils.c:2183-2352): <base>_<field>.nc in the current
+    directory, values unscaled, units of strlen(units) bytes."""
+    from scipy.io import netcdf_file
+    monkeypatch.chdir(tmp_path)
+    nx, nz, np_ = 6, 5, 2
+    data = np.random.default_rng(4).uniform(1e3, 3e4, (nx + 2 * np_, nz + 2 * np_))
+    LIB.sm_write_field_nc(data.ctypes.data_as(C.c_void_p), b"c11", b"MPa", C.c_size_t(nx), C.c_size_t(nz),
+                          C.c_size_t(np_), C.c_double(10.0), C.c_double(0.0), C.c_double(5.0), C.c_double(5.0),
+                          b"shot", C.c_int(1))
+    with netcdf_file(str(tmp_path / "shot_c11.nc"), "r", mmap=False) as f:
+        z = f.variables["z"]
+        assert np.array_equal(z[:], data.astype(np.float32).T)
+        assert z.units == b"MPa"
+        assert np.allclose(z.actual_range, [data.min(), data.max()])
+        assert np.allclose(f.variables["x"][:], 10.0 + (np.arange(nx + 2 * np_) - np_) * 5.0)
+
+
+def test_ve_moduli_against_oracle():
+    """sm_ve_moduli (host arithmetic of libseismod_hip, no GPU call) against
+    the oracle's material (ve_vti_pml.c:518-648)."""
+    from oracle import oracle_ve as V
+    from seismod_amd.solver import ve_moduli
+    rng = np.random.default_rng(9)
+    shp, L = (9, 7), 2
+    vp = rng.uniform(2200, 4000, shp); vs = vp / rng.uniform(1.6, 2.1, shp); rho = rng.uniform(1800, 2600, shp)
+    model = (1e-9 * (vp * vp * rho - 4. / 3. * vs * vs * rho), vs * vs * rho * 1e-9, rho,
+             rng.uniform(0, 0.25, shp), rng.uniform(-0.05, 0.15, shp), rng.uniform(20, 90, (L,) + shp),
+             rng.uniform(5, 60, (L,) + shp), rng.uniform(15, 70, shp), rng.uniform(5, 60, shp))
+    got = ve_moduli(L, *model)
+    m = V.ve_material(*model)
+    for k in ("c11", "c13", "c33", "c55", "K0"):
+        assert np.allclose(got[k], m[k], rtol=1e-14, atol=0), k
