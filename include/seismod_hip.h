@@ -51,6 +51,8 @@ extern "C" {
 #define SM_VX 0
 #define SM_VY 1
 #define SM_VZ 2
+#define SM_QX 3
+#define SM_QZ 4
 #define SM_TXX 5
 #define SM_TZZ 6
 #define SM_TXZ 7
@@ -546,6 +548,67 @@ int sm_ve_get_field(sm_ve *s, int comp, double *out);
 int sm_ve_reset(sm_ve *s);
 int sm_ve_device_bytes(const sm_ve *s, size_t *bytes);
 size_t sm_ve_bytes_needed(const sm_ve_params *p, size_t nrec, size_t max_steps);
+
+/* ------------------------------------------------------------------ */
+/* Poroviscoelastic waves in a 2D isotropic medium (pve_iso_pml) --     */
+/* Biot solid and fluid motion (Carcione and Helle 1999), one relaxation */
+/* mechanism for the fluid pressure, pseudo-spectral derivatives, CPML, */
+/* fourth-order Runge-Kutta with the stiff viscous coupling solved      */
+/* analytically at the start of every step; replaces pve_iso_pml.c:     */
+/* 795-1145 and 1249-1328, propagate.c:49-285 and 884-1329             */
+/* (init_fftw_data, propagateCPML), compute_div / compute_curl          */
+/* (propagate.c:2672-2704), src.c:77-112 (add_src_rk4)                  */
+/* ------------------------------------------------------------------ */
+typedef struct sm_pve sm_pve;
+typedef struct {
+    size_t nx, nz;            /* interior nodes (struct grid) */
+    size_t np;                /* CPML layers on every side; forced to 0 when
+                                 abs is 0 (io_utils.c:293) */
+    double dx, dz;            /* m */
+    double dt;                /* s */
+    double vmax, pml_order, Rc, kappa_max;   /* struct abs_params */
+    int alpha_max;            /* 0 -> alpha = 0, else alpha_max = pi*f0 */
+    double f0;                /* frequency of source 0 */
+    int abs;                  /* iwipe: 1 absorbing layers, 0 none */
+    int storage;              /* SM_STORE_F64 or SM_STORE_F32 */
+    int flags;                /* 0 or SM_PSTD_HIPFFT */
+} sm_pve_params;
+
+/* Build a solver from the padded model as read_model leaves it
+ * (io_utils.c:403-625), nx2*nz2 values each (index i*nz2+j): K_m, K_s,
+ * K_f and mu in GPa, phi, rho_s and rho_f in kg/m^3, tortuosity T, eta in
+ * cP, kappa in mD, Q and the relaxation frequency fq in Hz.  Computes the
+ * material and the analytic-step coefficients of pve_iso_pml.c:795-1145,
+ * the CPML tables and the FFT plans, and zeroes the fields.  Needs nx,
+ * nz >= 2, K_m, mu, eta >= 0 (mu = 0 gives a zero shear modulus at the
+ * cell corners it touches, 1/(sum of 1/mu)), positive K_s, K_f, porosity,
+ * densities, T, kappa, Q and fq, and a finite material (SM_ERR_ARG). */
+int sm_pve_create(sm_ctx *ctx, const sm_pve_params *p, const double *K_m, const double *K_s,
+                  const double *K_f, const double *phi, const double *mu, const double *rho_s,
+                  const double *rho_f, const double *T, const double *eta, const double *kappa,
+                  const double *Q, const double *fq, sm_pve **out);
+int sm_pve_destroy(sm_pve *s);
+/* Sources on padded indices (add_src_rk4): SM_SF adds to p, SM_SX to
+ * tau_xx, SM_SZ to tau_zz, SM_BULK to p, tau_xx and tau_zz, SM_BULK_S to
+ * tau_xx and tau_zz, SM_SXZ to tau_xz; fct as for sm_sh_set_sources. */
+int sm_pve_set_sources(sm_pve *s, size_t nsrc, const int *type, const size_t *idx,
+                       const size_t *length, const double *fct);
+/* receivers on padded indices: SM_VX, SM_VZ, SM_QX, SM_QZ, SM_TXX, SM_TZZ,
+ * SM_TXZ, SM_P, or SM_DIV / SM_CURL (D_x^- v_x + D_z^- v_z and
+ * D_x^+ v_z - D_z^+ v_x of the last Runge-Kutta stage's input, without the
+ * layer corrections, as compute_div and compute_curl take them) */
+int sm_pve_set_receivers(sm_pve *s, size_t nrec, const size_t *idx, const int *comp,
+                         size_t max_steps);
+int sm_pve_run(sm_pve *s, size_t nsteps);
+int sm_pve_set_timing(sm_pve *s, int on);
+int sm_pve_get_timing(const sm_pve *s, double *ms_total, size_t *steps);
+int sm_pve_current_step(const sm_pve *s, size_t *it);
+int sm_pve_get_traces(sm_pve *s, size_t it0, size_t count, double *out);
+/* padded nx2*nz2 field of any receiver component */
+int sm_pve_get_field(sm_pve *s, int comp, double *out);
+int sm_pve_reset(sm_pve *s);
+int sm_pve_device_bytes(const sm_pve *s, size_t *bytes);
+size_t sm_pve_bytes_needed(const sm_pve_params *p, size_t nrec, size_t max_steps);
 
 #ifdef __cplusplus
 }

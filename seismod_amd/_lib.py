@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libseismod_hip.so"
 SM_OK, SM_ERR_ARG, SM_ERR_HIP, SM_ERR_CFL, SM_ERR_NOMEM, SM_ERR_STATE, SM_ERR_NODEV = range(7)
 SM_STORE_F32, SM_STORE_F64, SM_STORE_BF16, SM_STORE_F16 = range(4)
 SM_VX, SM_VY, SM_VZ, SM_TXX, SM_TZZ, SM_TXZ, SM_P = 0, 1, 2, 5, 6, 7, 8
+SM_QX, SM_QZ = 3, 4
 SM_PSTD_HIPFFT = 1
 SM_TXY, SM_TYZ = 9, 10
 SM_DIV, SM_CURL, SM_VR, SM_VT, SM_TRR, SM_TRT, SM_TRZ, SM_TTT, SM_TTZ = range(11, 20)
@@ -76,6 +77,10 @@ class VeParams(C.Structure):
         ("kappa_max", C.c_double), ("alpha_max", C.c_int), ("f0", C.c_double),
         ("abs", C.c_int), ("L", C.c_int), ("storage", C.c_int), ("flags", C.c_int),
     ]
+
+
+# sm_pve_params has the layout of sm_sh_params
+PveParams = ShParams
 
 
 class E2dParams(C.Structure):
@@ -193,6 +198,19 @@ _SIGS = {
     "sm_ve_reset": (C.c_int, [C.c_void_p]),
     "sm_ve_device_bytes": (C.c_int, [C.c_void_p, _size_p]),
     "sm_ve_bytes_needed": (C.c_size_t, [C.POINTER(VeParams), C.c_size_t, C.c_size_t]),
+    "sm_pve_create": (C.c_int, [C.c_void_p, C.POINTER(PveParams)] + [_dbl_p] * 12 + [C.POINTER(C.c_void_p)]),
+    "sm_pve_destroy": (C.c_int, [C.c_void_p]),
+    "sm_pve_set_sources": (C.c_int, [C.c_void_p, C.c_size_t, _int_p, _size_p, _size_p, _dbl_p]),
+    "sm_pve_set_receivers": (C.c_int, [C.c_void_p, C.c_size_t, _size_p, _int_p, C.c_size_t]),
+    "sm_pve_run": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "sm_pve_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "sm_pve_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),
+    "sm_pve_current_step": (C.c_int, [C.c_void_p, _size_p]),
+    "sm_pve_get_traces": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, _dbl_p]),
+    "sm_pve_get_field": (C.c_int, [C.c_void_p, C.c_int, _dbl_p]),
+    "sm_pve_reset": (C.c_int, [C.c_void_p]),
+    "sm_pve_device_bytes": (C.c_int, [C.c_void_p, _size_p]),
+    "sm_pve_bytes_needed": (C.c_size_t, [C.POINTER(PveParams), C.c_size_t, C.c_size_t]),
 }
 
 

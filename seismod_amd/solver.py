@@ -700,8 +700,8 @@ class _Pstd:
 
 
 class ViscoelasticSH(_Pstd):
-    """Viscoelastic SH waves in a 2D VTI medium (ve_vti_sh_pml): hipFFT
-    derivatives, CPML, fourth-order Runge-Kutta."""
+    """Viscoelastic SH waves in a 2D VTI medium (ve_vti_sh_pml): pseudo-
+    spectral derivatives, CPML, fourth-order Runge-Kutta."""
 
     _pre = "sm_sh"
     FIELDS = {"vy": L.SM_VY, "txy": L.SM_TXY, "tyz": L.SM_TYZ}
@@ -772,8 +772,8 @@ def ve_moduli(L_: int, K, mu, rho, eps, delta, Q1, f1, Q2, f2) -> dict:
 
 
 class ViscoelasticPSV(_Pstd):
-    """Viscoelastic P-SV waves in a 2D VTI medium (ve_vti_pml): hipFFT
-    derivatives, CPML, fourth-order Runge-Kutta, L + 1 relaxation
+    """Viscoelastic P-SV waves in a 2D VTI medium (ve_vti_pml): pseudo-
+    spectral derivatives, CPML, fourth-order Runge-Kutta, L + 1 relaxation
     mechanisms."""
 
     _pre = "sm_ve"
@@ -791,5 +791,46 @@ class ViscoelasticPSV(_Pstd):
         ctx.adopt(self)
         check(lib.sm_ve_create(ctx.handle, C.byref(self._p), *(_dp(a) for a in self._m), C.byref(h)),
               "sm_ve_create")
+        self.handle = h
+        self.nrec = 0
+
+
+@dataclass
+class PveConfig(ShConfig):
+    """pve_iso_pml grid: nx by nz interior nodes and np CPML layers on every
+    side (none when abs is False), padded (nx + 2np) by (nz + 2np)."""
+
+    def params(self) -> L.PveParams:
+        return super().params()
+
+    def bytes_needed(self, nrec: int = 0, max_steps: int = 0) -> int:
+        p = self.params()
+        return int(lib.sm_pve_bytes_needed(C.byref(p), nrec, max_steps))
+
+
+class Poroviscoelastic(_Pstd):
+    """Poroviscoelastic waves in a 2D isotropic medium (pve_iso_pml): Biot
+    solid and fluid motion, one relaxation mechanism for the fluid pressure,
+    pseudo-spectral derivatives, CPML, fourth-order Runge-Kutta with the
+    viscous coupling stepped analytically.  Model arrays as read_model
+    leaves them: K_m, K_s, K_f, mu (GPa), phi, rho_s, rho_f (kg/m^3), T,
+    eta (cP), kappa (mD), Q, fq (Hz)."""
+
+    _pre = "sm_pve"
+    FIELDS = {"vx": L.SM_VX, "vz": L.SM_VZ, "qx": L.SM_QX, "qz": L.SM_QZ, "txx": L.SM_TXX,
+              "tzz": L.SM_TZZ, "txz": L.SM_TXZ, "p": L.SM_P, "div": L.SM_DIV, "curl": L.SM_CURL}
+
+    def __init__(self, ctx: Context, cfg: PveConfig, K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, eta, kappa,
+                 Q, fq):
+        self.ctx, self.cfg = ctx, cfg
+        n = int(np.prod(cfg.shape))
+        self._m = [_f64(a).reshape(-1) for a in (K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, eta, kappa, Q, fq)]
+        if any(a.size != n for a in self._m):
+            raise ValueError(f"model must hold {n} padded cells")
+        self._p = cfg.params()
+        h = C.c_void_p()
+        ctx.adopt(self)
+        check(lib.sm_pve_create(ctx.handle, C.byref(self._p), *(_dp(a) for a in self._m), C.byref(h)),
+              "sm_pve_create")
         self.handle = h
         self.nrec = 0
