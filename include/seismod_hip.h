@@ -600,6 +600,18 @@ int sm_pve_set_sources(sm_pve *s, size_t nsrc, const int *type, const size_t *id
 int sm_pve_set_receivers(sm_pve *s, size_t nrec, const size_t *idx, const int *comp,
                          size_t max_steps);
 int sm_pve_run(sm_pve *s, size_t nsteps);
+/* Kinetic energy of a region after every step, from the current step on
+ * (saveEnergy, pve_iso_pml.c:1106-1119 and 1391-1404): the sum over padded
+ * cells i1 <= i <= i2, j1 <= j <= j2 of 0.5 rho11 (v_x^2 + v_z^2) +
+ * rho12 (v_x q_x + v_z q_z) + 0.5 rho22 (q_x^2 + q_z^2), rho11, rho12 and
+ * rho22 given per region cell (i outer, (i2-i1+1)(j2-j1+1) values each);
+ * accumulated in FP64 in a fixed order (results repeat bit for bit).
+ * max_steps 0 stops recording.  Adds 24 bytes per region cell and 8 per
+ * step to the device memory. */
+int sm_pve_set_energy(sm_pve *s, size_t i1, size_t i2, size_t j1, size_t j2, const double *rho11,
+                      const double *rho12, const double *rho22, size_t max_steps);
+/* count values from row it0 of the energy record */
+int sm_pve_get_energy(sm_pve *s, size_t it0, size_t count, double *out);
 int sm_pve_set_timing(sm_pve *s, int on);
 int sm_pve_get_timing(const sm_pve *s, double *ms_total, size_t *steps);
 int sm_pve_current_step(const sm_pve *s, size_t *it);

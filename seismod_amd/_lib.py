@@ -203,6 +203,8 @@ _SIGS = {
     "sm_pve_set_sources": (C.c_int, [C.c_void_p, C.c_size_t, _int_p, _size_p, _size_p, _dbl_p]),
     "sm_pve_set_receivers": (C.c_int, [C.c_void_p, C.c_size_t, _size_p, _int_p, C.c_size_t]),
     "sm_pve_run": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "sm_pve_set_energy": (C.c_int, [C.c_void_p] + [C.c_size_t] * 4 + [_dbl_p] * 3 + [C.c_size_t]),
+    "sm_pve_get_energy": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, _dbl_p]),
     "sm_pve_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sm_pve_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),
     "sm_pve_current_step": (C.c_int, [C.c_void_p, _size_p]),

@@ -172,6 +172,50 @@ __global__ void pve_gather(const T *w, const T *dd, size_t N, int nrec, const in
     trace[row * nrec + r] = v;
 }
 
+// Kinetic energy of a region, summed in FP64 in a fixed order (one partial
+// sum per block, then one block over the partials): 0.5 rho11 |v|^2 +
+// rho12 v.q + 0.5 rho22 |q|^2 per cell, coef = rho11, rho12, rho22 of the
+// region's ni x nj cells (i outer) one array after the other.
+constexpr unsigned E_BLOCKS = 256;
+
+__device__ inline double block_sum(double a, double *red)
+{
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (unsigned k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void pve_energy_part(const T *w, size_t N, const double *coef, unsigned ni,
+                                                      unsigned nj, unsigned i1, unsigned j1, unsigned nz2,
+                                                      double *part)
+{
+    __shared__ double red[256];
+    const unsigned ncell = ni * nj;
+    double acc = 0.0;
+    for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+        const size_t n = (size_t)(c / nj + i1) * nz2 + c % nj + j1;
+        const double vx = w[W_VX * N + n], vz = w[W_VZ * N + n], qx = w[W_QX * N + n], qz = w[W_QZ * N + n];
+        acc += 0.5 * coef[c] * (vx * vx + vz * vz) + coef[ncell + c] * (vx * qx + vz * qz) +
+               0.5 * coef[2 * (size_t)ncell + c] * (qx * qx + qz * qz);
+    }
+    const double t = block_sum(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void pve_energy_sum(const double *part, unsigned nb, double *out)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x) acc += part[b];
+    const double t = block_sum(acc, red);
+    if (threadIdx.x == 0) out[0] = t;
+}
+
 }  // namespace
 }  // namespace sm
 
@@ -188,6 +232,9 @@ struct sm_pve {
     size_t mem_dx = 0, mem_dz = 0;
     sm_dbuf src, rcv, trace;
     size_t nsrc = 0, nrec = 0, max_steps = 0, rcv_step0 = 0, step = 0;
+    // kinetic energy of a region: coefficients, partial sums, one value per step
+    sm_dbuf ecoef, epart, etrace;
+    size_t ei1 = 0, ej1 = 0, eni = 0, enj = 0, e_max = 0, e_step0 = 0;
     bool need_div = false, need_curl = false;
     bool timing = false;
     double timed_ms = 0.0;
@@ -199,7 +246,9 @@ namespace {
 void release(sm_pve *s)
 {
     pstd::release(s->g);
-    for (sm_dbuf *b : {&s->state, &s->der, &s->mat, &s->mem, &s->src, &s->rcv, &s->trace}) dfree(*b);
+    for (sm_dbuf *b : {&s->state, &s->der, &s->mat, &s->mem, &s->src, &s->rcv, &s->trace, &s->ecoef, &s->epart,
+                       &s->etrace})
+        dfree(*b);
 }
 
 template <typename T> T *STATE(sm_pve *s, int b) { return (T *)s->state.ptr + (size_t)b * W_NF * s->N; }
@@ -325,6 +374,15 @@ int run(sm_pve *s, size_t nsteps)
                                                                  (int)s->nrec, ridx, rcomp, (T *)s->trace.ptr,
                                                                  (int64_t)((size_t)it - s->rcv_step0));
         }
+        if (s->e_max && (size_t)it >= s->e_step0 && (size_t)it - s->e_step0 < s->e_max) {
+            const size_t nc = s->eni * s->enj;
+            const unsigned nb = (unsigned)std::min<size_t>(E_BLOCKS, (nc + 255) / 256);
+            pve_energy_part<T><<<nb, 256, 0, st>>>(STATE<T>(s, s->iw), N, (const double *)s->ecoef.ptr,
+                                                   (unsigned)s->eni, (unsigned)s->enj, (unsigned)s->ei1,
+                                                   (unsigned)s->ej1, (unsigned)s->g.nz2, (double *)s->epart.ptr);
+            pve_energy_sum<<<1, 256, 0, st>>>((const double *)s->epart.ptr, nb,
+                                              (double *)s->etrace.ptr + ((size_t)it - s->e_step0));
+        }
     }
     SM_HIP(hipGetLastError());
     s->step += nsteps;
@@ -347,6 +405,7 @@ int zero_state(sm_pve *s)
     SM_HIP(hipMemsetAsync(s->mem.ptr, 0, s->mem.bytes, st));
     SM_HIP(hipMemsetAsync(s->der.ptr, 0, s->der.bytes, st));
     if (s->trace.ptr) SM_HIP(hipMemsetAsync(s->trace.ptr, 0, s->trace.bytes, st));
+    if (s->etrace.ptr) SM_HIP(hipMemsetAsync(s->etrace.ptr, 0, s->etrace.bytes, st));
     SM_HIP(hipStreamSynchronize(st));
     s->iw = 0; s->iws = 1; s->iwt = 2;
     return SM_OK;
@@ -595,6 +654,48 @@ int sm_pve_set_receivers(sm_pve *s, size_t nrec, const size_t *idx, const int *c
     return SM_OK;
 }
 
+int sm_pve_set_energy(sm_pve *s, size_t i1, size_t i2, size_t j1, size_t j2, const double *rho11,
+                      const double *rho12, const double *rho22, size_t max_steps)
+{
+    SM_CHECK(s, SM_ERR_ARG, "sm_pve_set_energy: NULL solver");
+    SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    for (sm_dbuf *b : {&s->ecoef, &s->epart, &s->etrace}) dfree(*b);
+    s->e_max = 0;
+    if (max_steps == 0) return SM_OK;
+    SM_CHECK(rho11 && rho12 && rho22, SM_ERR_ARG, "sm_pve_set_energy: NULL array");
+    SM_CHECK(i1 <= i2 && i2 < (size_t)s->g.nx2 && j1 <= j2 && j2 < (size_t)s->g.nz2, SM_ERR_ARG,
+             "sm_pve_set_energy: region [%zu, %zu] x [%zu, %zu] outside the %d x %d grid", i1, i2, j1, j2,
+             s->g.nx2, s->g.nz2);
+    const size_t ni = i2 - i1 + 1, nj = j2 - j1 + 1, nc = ni * nj;
+    std::vector<double> h(3 * nc);
+    std::memcpy(h.data(), rho11, nc * sizeof(double));
+    std::memcpy(h.data() + nc, rho12, nc * sizeof(double));
+    std::memcpy(h.data() + 2 * nc, rho22, nc * sizeof(double));
+    int rc = dalloc(s->ecoef, h.size() * sizeof(double));
+    if (rc == SM_OK) rc = dalloc(s->epart, E_BLOCKS * sizeof(double));
+    if (rc == SM_OK) rc = dalloc(s->etrace, max_steps * sizeof(double));
+    if (rc != SM_OK) return rc;
+    SM_HIP(hipMemcpyAsync(s->ecoef.ptr, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
+    SM_HIP(hipMemsetAsync(s->etrace.ptr, 0, s->etrace.bytes, s->ctx->stream));
+    SM_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->ei1 = i1; s->ej1 = j1; s->eni = ni; s->enj = nj;
+    s->e_max = max_steps; s->e_step0 = s->step;
+    return SM_OK;
+}
+
+int sm_pve_get_energy(sm_pve *s, size_t it0, size_t count, double *out)
+{
+    SM_CHECK(s && (count == 0 || out), SM_ERR_ARG, "sm_pve_get_energy: NULL argument");
+    SM_CHECK(it0 + count <= s->e_max, SM_ERR_ARG, "sm_pve_get_energy: rows [%zu, %zu) beyond %zu", it0,
+             it0 + count, s->e_max);
+    if (count == 0) return SM_OK;
+    SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    SM_HIP(hipMemcpyAsync(out, (const double *)s->etrace.ptr + it0, count * sizeof(double), hipMemcpyDeviceToHost,
+                          s->ctx->stream));
+    SM_HIP(hipStreamSynchronize(s->ctx->stream));
+    return SM_OK;
+}
+
 int sm_pve_run(sm_pve *s, size_t nsteps)
 {
     SM_CHECK(s, SM_ERR_ARG, "sm_pve_run: NULL solver");
@@ -674,6 +775,7 @@ int sm_pve_reset(sm_pve *s)
     SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
     s->step = 0;
     s->rcv_step0 = 0;
+    s->e_step0 = 0;
     return zero_state(s);
 }
 
@@ -681,7 +783,9 @@ int sm_pve_device_bytes(const sm_pve *s, size_t *bytes)
 {
     SM_CHECK(s && bytes, SM_ERR_ARG, "sm_pve_device_bytes: NULL argument");
     size_t b = pstd::grid_device_bytes(s->g);
-    for (const sm_dbuf *d : {&s->state, &s->der, &s->mat, &s->mem, &s->src, &s->rcv, &s->trace}) b += d->bytes;
+    for (const sm_dbuf *d : {&s->state, &s->der, &s->mat, &s->mem, &s->src, &s->rcv, &s->trace, &s->ecoef,
+                             &s->epart, &s->etrace})
+        b += d->bytes;
     *bytes = b;
     return SM_OK;
 }

@@ -9,6 +9,7 @@
 #define _POSIX_C_SOURCE 200809L
 #include "sm_io.h"
 
+#include <complex.h>
 #include <ctype.h>
 #include <errno.h>
 #include <math.h>
@@ -115,8 +116,12 @@ void sm_process_args(int argc, char *const argv[], sm_params *p, const char *usa
         else if (!strcmp(keyword, "shotpoint")) p->shotpt_no = atoi(value);
         else if (!strcmp(keyword, "simulateCMP")) p->simulate_cmp = atoi(value);
         else if (!strcmp(keyword, "azimuthal_mode")) p->mode = atoi(value);
-        /* other keywords (check_model, chkpt_inc, pml_median, saveEnergy,
-         * EnergyROI) do not affect these programs */
+        else if (!strcmp(keyword, "check_model")) p->check_model = (short)atoi(value);
+        else if (!strcmp(keyword, "chkpt_inc")) p->chkpt_inc = atoi(value);
+        else if (!strcmp(keyword, "pml_median")) p->pml_median = (short)atoi(value);
+        else if (!strcmp(keyword, "saveEnergy")) p->save_energy = (short)atoi(value);
+        else if (!strcmp(keyword, "EnergyROI"))
+            sscanf(value, "%lf/%lf/%lf/%lf", &p->roi[0], &p->roi[1], &p->roi[2], &p->roi[3]);
     }
     fclose(fid);
     if (p->iwipe == 0) p->np = 0;   /* io_utils.c:293 */
@@ -558,6 +563,125 @@ void sm_read_model_ve(const char *fn, const sm_grid *g, size_t np, int L, double
                 a[q][(i + nx + np) * nz2 + j] = a[q][(nx + np - 1) * nz2 + j];
             }
     free(a);
+}
+
+void sm_read_grid_pve(const char *fn, size_t np, sm_grid *g, int *with_pml)
+{
+    /* io_utils.c:296-328: the keyword compared in lower case; with the
+     * layers in the file, nx and nz count them */
+    FILE *f = open_or_die(fn);
+    memset(g, 0, sizeof(*g));
+    char type[64];
+    if (fscanf(f, "%zu %zu", &g->nx, &g->nz) != 2 || fscanf(f, "%lf %lf", &g->dx, &g->dz) != 2 ||
+        fscanf(f, "%lf %lf", &g->x0, &g->z0) != 2 || fscanf(f, "%63s", type) != 1)
+        die("Error: something wrong with file %s\n", fn);
+    fclose(f);
+    for (char *c = type; *c; ++c) *c = (char)tolower((unsigned char)*c);
+    *with_pml = strcmp(type, "isotropic_pml") == 0;
+    if (!*with_pml && strcmp(type, "isotropic") != 0) die("Error, model in %s not isotropic\n", fn);
+    if (*with_pml) {
+        if (g->nx <= 2 * np || g->nz <= 2 * np)
+            die("Error: the padded model in %s is not larger than its absorbing layers\n", fn);
+        g->nx -= 2 * np;
+        g->nz -= 2 * np;
+    }
+    g->ny = 1;
+    g->dy = g->dx;
+}
+
+void sm_read_model_pve(const char *fn, const sm_grid *g, size_t np, int with_pml, double *const vals[12])
+{
+    /* io_utils.c:403-625.  "isotropic": a short file holding exactly one z
+     * column is laterally uniform, any other shortfall an error; the layers
+     * take the nearest model node's values (Qdamping is never set, so Q is
+     * copied too).  "isotropic_pml": every padded node from the file (the
+     * reference leaves a short last node uninitialised; here it is an
+     * error). */
+    FILE *f = open_or_die(fn);
+    char type[64];
+    if (fscanf(f, "%*d %*d") < 0 || fscanf(f, "%*f %*f") < 0 || fscanf(f, "%*f %*f") < 0 ||
+        fscanf(f, "%63s", type) != 1)
+        die("Error: something wrong with file %s\n", fn);
+    const size_t nx = g->nx, nz = g->nz, nx2 = nx + 2 * np, nz2 = nz + 2 * np;
+    if (with_pml) {
+        const size_t N = nx2 * nz2;
+        size_t nv = 0;
+        double *v = read_body(f, fn, 12 * N, &nv);
+        fclose(f);
+        if (nv < 12 * N) die("Error: something wrong with file %s\n", fn);
+        for (size_t n = 0; n < N; ++n)
+            for (int q = 0; q < 12; ++q) vals[q][n] = v[12 * n + (size_t)q];
+        free(v);
+        return;
+    }
+    const size_t ncell = nx * nz;
+    size_t nv = 0;
+    double *v = read_body(f, fn, 12 * ncell, &nv);
+    fclose(f);
+    int flag1d = 0;
+    if (nv < 12 * ncell) {
+        if (nx > 1 && nv == 12 * nz) flag1d = 1;
+        else die("Error: something wrong with file %s\n", fn);
+    }
+    for (size_t i = 0; i < nx; ++i)
+        for (size_t j = 0; j < nz; ++j) {
+            const double *c = v + 12 * ((flag1d ? 0 : i) * nz + j);
+            for (int q = 0; q < 12; ++q) vals[q][(i + np) * nz2 + j + np] = c[q];
+        }
+    free(v);
+    for (size_t i = 0; i < nx; ++i)
+        for (size_t j = 0; j < np; ++j)
+            for (int q = 0; q < 12; ++q) {
+                double *a = vals[q];
+                a[(i + np) * nz2 + j] = a[(i + np) * nz2 + np];
+                a[(i + np) * nz2 + np + nz + j] = a[(i + np) * nz2 + np + nz - 1];
+            }
+    for (size_t i = 0; i < np; ++i)
+        for (size_t j = 0; j < nz2; ++j)
+            for (int q = 0; q < 12; ++q) {
+                double *a = vals[q];
+                a[i * nz2 + j] = a[np * nz2 + j];
+                a[(i + nx + np) * nz2 + j] = a[(nx + np - 1) * nz2 + j];
+            }
+}
+
+void sm_pve_dispersion(size_t n, double *const v[12], double f, double *const vel[3], double *const att[3])
+{
+    /* the node material in the units of the time loop (pve_iso_pml.c:795-
+     * 829: MPa, Mg/m^3, eta scaled so that eta/kappa is in Mg/m^3/s), then
+     * the complex density rho_bar = T rho_f / phi - i (eta/kappa) / omega
+     * and fluid modulus M (1 + i omega tau_e) / ((1 + varphi)(1 + i omega
+     * tau_s)), and the roots of the dispersion relation */
+    const double pi = 4.0 * atan(1.0);
+    const double omega = 2. * pi * f;
+    for (size_t c = 0; c < n; ++c) {
+        const double K_m = v[0][c] * 1000., K_s = v[1][c] * 1000., K_f = v[2][c] * 1000., phi = v[3][c];
+        const double mu = v[4][c] * 1000., rho_s = v[5][c] * 1.e-6, rho_f = v[6][c] * 1.e-6, T = v[7][c];
+        const double eta = v[8][c] * 1.013249965828145e6, kappa = v[9][c], Q = v[10][c];
+        const double tau0 = 1. / (2. * pi * v[11][c]);
+        const double E = K_m + 4. / 3. * mu;
+        const double D = K_s * (1. + phi * (K_s / K_f - 1.));
+        const double M = K_s * K_s / (D - K_m);
+        const double alpha = 1. - K_m / K_s;
+        double tau_e = tau0, tau_s = tau0, varphi = 0.0;
+        if (Q < 1.e4) {
+            tau_e = tau0 / Q * (sqrt(Q * Q + 1.) + 1.);
+            tau_s = tau0 / Q * (sqrt(Q * Q + 1.) - 1.);
+            varphi = tau_e / tau_s - 1.;
+        }
+        const double rho = (1. - phi) * rho_s + phi * rho_f;
+        const double complex rho_bar = T / phi * rho_f - I / omega * (eta / kappa);
+        const double complex rho_c = rho - rho_f * rho_f / rho_bar;
+        const double complex M_c = M / (1. + varphi) * (1. + I * omega * tau_e) / (1. + I * omega * tau_s);
+        const double complex A = M_c * (rho - 2. * alpha * rho_f) + rho_bar * (E + alpha * alpha * M_c);
+        const double complex tmp = csqrt(A * A - 4. * M_c * E * rho_c * rho_bar);
+        const double complex V[3] = {csqrt((A - tmp) / (2. * rho_c * rho_bar)),
+                                     csqrt((A + tmp) / (2. * rho_c * rho_bar)), csqrt(mu / rho_c)};
+        for (int k = 0; k < 3; ++k) {
+            vel[k][c] = 1. / creal(1. / V[k]);
+            att[k][c] = 17.372 * pi * cimag(V[k]) / creal(V[k]);
+        }
+    }
 }
 
 void sm_read_model_a(const char *fn, const sm_grid *g, size_t np, double *vp, double *rho)

@@ -30,6 +30,12 @@ typedef struct {
     double vmax, pml_order, Rc, kappa_max;
     int alpha_max;
     int mode;           /* azimuthal_mode (e_cyl_iso) */
+    /* keywords of the poroviscoelastic programs (io_utils.c:257-280) */
+    int check_model;    /* phase velocities and attenuation: 1 print, 2 also store */
+    int chkpt_inc;      /* checkpoint increment (0: none) */
+    int pml_median;     /* median smoothing of the material in the layers */
+    int save_energy;    /* saveEnergy: kinetic energy of a region per step */
+    double roi[4];      /* EnergyROI xmin/xmax/zmin/zmax (m) */
 } sm_params;
 
 typedef struct {
@@ -109,6 +115,22 @@ void sm_read_grid_ve(const char *fn, sm_grid *g, int *L);
 void sm_read_model_ve(const char *fn, const sm_grid *g, size_t np, int L, double *K, double *mu,
                       double *rho, double *eps, double *delta, double *Q1, double *f1, double *Q2,
                       double *f2);
+/* read_grid_params / read_model (io_utils.c:296-328, 403-625) for the
+ * poroviscoelastic programs: type "isotropic" (nx x nz nodes, or one z
+ * column, values copied into the absorbing layers from the nearest model
+ * node) or "isotropic_pml" (the padded grid in the file: nx and nz include
+ * the 2 np layer nodes, which read_grid_params subtracts); 12 values per
+ * node "K_m K_s K_f phi mu rho_s rho_f T eta kappa Q f0", x outer and z
+ * inner, into padded (nx + 2np) x (nz + 2np) arrays.  *with_pml tells
+ * which type the file holds. */
+void sm_read_grid_pve(const char *fn, size_t np, sm_grid *g, int *with_pml);
+void sm_read_model_pve(const char *fn, const sm_grid *g, size_t np, int with_pml, double *const vals[12]);
+/* Phase velocity (m/s) and attenuation (17.372 pi Im V / Re V) of the slow
+ * and fast P waves and the S wave at frequency f (Hz) for n nodes of the
+ * 12 model values as sm_read_model_pve gives them, from the complex moduli
+ * and densities of the poroviscoelastic model (pve_iso_pml.c:864-904):
+ * vel[0..2] and att[0..2] receive n values each (slow P, fast P, S). */
+void sm_pve_dispersion(size_t n, double *const vals[12], double f, double *const vel[3], double *const att[3]);
 void sm_read_output3d(const char *fn, const sm_grid *g, size_t np, sm_records *out);
 /* check_dt_trc (io_utils.c:3561-3574) */
 void sm_check_dt_trc(const sm_records *out, double dt);

@@ -834,3 +834,20 @@ class Poroviscoelastic(_Pstd):
               "sm_pve_create")
         self.handle = h
         self.nrec = 0
+        self.energy_steps = 0
+
+    def set_energy(self, i1: int, i2: int, j1: int, j2: int, rho11, rho12, rho22, max_steps: int) -> None:
+        """Record the kinetic energy of padded cells [i1, i2] x [j1, j2] after
+        every step; rho11, rho12, rho22 hold one value per region cell."""
+        ni, nj = i2 - i1 + 1, j2 - j1 + 1
+        a = [_f64(r).reshape(-1) for r in (rho11, rho12, rho22)]
+        if any(r.size != ni * nj for r in a):
+            raise ValueError(f"energy coefficients must hold {ni * nj} values")
+        check(lib.sm_pve_set_energy(self.handle, i1, i2, j1, j2, *(_dp(r) for r in a), max_steps),
+              "sm_pve_set_energy")
+        self.energy_steps = max_steps
+
+    def energy(self, it0: int, count: int) -> np.ndarray:
+        out = np.zeros(count, dtype=np.float64)
+        check(lib.sm_pve_get_energy(self.handle, it0, count, _dp(out)), "sm_pve_get_energy")
+        return out

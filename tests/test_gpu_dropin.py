@@ -589,3 +589,124 @@ def test_ve_vti_pml_program(tmp_path, template):
         z = nc_field(tmp_path / f"ve_{name}.nc")
         want = mat[name].T.astype(np.float32).astype(np.float64)
         assert np.allclose(z, want, rtol=2.5e-7, atol=0), name   # one FP32 ulp
+
+
+# ------------------------------------------------------- poroviscoelastic --
+def deck_pve(tmp, nx=44, nz=38, np_=8, h=5.0, dt_ms=0.25, time_s=0.05, template=1, with_pml=False):
+    rng = np.random.default_rng(29)
+    top = np.arange(nz)[None, :] < nz // 2
+    ones = np.ones((nx, 1))
+    two = lambda a, b: np.where(top, a, b) * ones
+    vals = [two(8.0, 12.0) * (1 + 0.02 * rng.uniform(-1, 1, (nx, nz))), two(36.0, 38.0), two(2.25, 2.0),
+            two(0.3, 0.2), two(5.0, 8.0), two(2650.0, 2700.0), two(1000.0, 1040.0), two(2.0, 2.5),
+            two(1.0, 0.8), two(800.0, 300.0), two(40.0, 2e4), two(30.0, 50.0)]
+    npad = lambda a: np.pad(a, np_, mode="edge")
+    if with_pml:
+        file_vals = [npad(a) for a in vals]
+        head = f"{nx + 2 * np_} {nz + 2 * np_}\n{h} {h}\n0.0 0.0\nisotropic_pml\n"
+    else:
+        file_vals = vals
+        head = f"{nx} {nz}\n{h} {h}\n0.0 0.0\nisotropic\n"
+    with open(tmp / "model.dat", "w") as f:
+        f.write(head)
+        for i in range(file_vals[0].shape[0]):
+            for j in range(file_vals[0].shape[1]):
+                f.write(" ".join(repr(float(a[i, j])) for a in file_vals) + "\n")
+    (tmp / "src.dat").write_text(f"2\n{template}\nBulk\n2.0\n60.0\n110.0 60.0\nSf\n0.5\n60.0\n70.0 120.0\n")
+    recs = [("Vx", "Trace", "60.0 25.0", 0.0, dt_ms), ("qz", "Trace", "150.0 75.0", 0.01, 2 * dt_ms),
+            ("p", "Trace", "100.0 80.0", 0.0, dt_ms), ("txz", "Trace", "30.0 150.0", 0.0, dt_ms),
+            ("div", "Trace", "120.0 90.0", 0.0, dt_ms), ("curl", "Trace", "80.0 100.0", 0.0, dt_ms),
+            ("p", "Snapshot", None, 0.025, 0.0)]
+    with open(tmp / "rcv.dat", "w") as f:
+        f.write(f"{len(recs)}\n")
+        for comp, kind, xz, t0, rdt in recs:
+            f.write(f"{comp}\n{kind}\n" + (f"{xz}\n" if xz else "") + f"{t0}\n{rdt}\n")
+    roi = (50.0, 75.0, 40.0, 60.0)
+    (tmp / "run.par").write_text(
+        "model.dat # model, file\nsrc.dat # source, file\nrcv.dat # output, records\n"
+        f"pve # basename, name\n{time_s} # time, s\n{dt_ms} # dt, ms\n1 # abs, on\n{np_} # nl, layers\n"
+        "3 # kappa_max, PML\n1 # plstr, strips\n2 # check_model, store\n1 # saveEnergy, E(t)\n"
+        f"{roi[0]}/{roi[1]}/{roi[2]}/{roi[3]} # EnergyROI, region\n")
+    nz2 = nz + 2 * np_
+    cell = lambda x, z: (int(round(x / h)) + np_) * nz2 + int(round(z / h)) + np_
+    comp = {"Vx": 0, "qz": 4, "p": 8, "txz": 7, "div": 11, "curl": 12}
+    rec = [(cell(*map(float, xz.split())), comp[c], t0, rdt) for c, k, xz, t0, rdt in recs if xz]
+    ij = [(i, j) for i in range(int(round(roi[0] / h)), int(round(roi[1] / h)) + 1)
+          for j in range(int(round(roi[2] / h)), int(round(roi[3] / h)) + 1)]
+    return dict(nx=nx, nz=nz, np=np_, h=h, dt=dt_ms * 1e-3, model=tuple(npad(a) for a in vals), rec=rec,
+                template=template, srcs=[(6, cell(110.0, 60.0), 2.0), (7, cell(70.0, 120.0), 0.5)], snap_t0=0.025,
+                roi_cells=ij, raw=vals)
+
+
+def oracle_pve(d, nsteps, extra_rec=()):
+    from oracle import oracle_pve as B
+    import math
+    nz2 = d["nz"] + 2 * d["np"]
+    pts = []
+    w = [0.5, 0.125, 0.25, 0.125, 0.25, 0.25, 0.125, 0.25, 0.125]
+    di = [0, -1, 0, 1, -1, 1, -1, 0, 1]
+    dj = [0, -1, -1, -1, 0, 0, 1, 1, 1]
+    for typ, c, A in d["srcs"]:
+        if d["template"] == 9:
+            pts += [(typ, c + di[q] * nz2 + dj[q], w[q] * A) for q in range(9)]
+        else:
+            pts.append((typ, c, A))
+    L, fct = O.source_functions([p[2] for p in pts], [60.0] * len(pts), 0.5 * d["dt"])
+    fct = fct / (d["h"] * d["h"])
+    off = np.concatenate([[0], np.cumsum(L)]).astype(np.int64)
+    src = [(p[0], p[1], fct[off[k]:off[k + 1]]) for k, p in enumerate(pts)]
+    cfg = dict(nx=d["nx"], nz=d["nz"], np=d["np"], dx=d["h"], dz=d["h"], dt=d["dt"], vmax=5000.0,
+               kappa_max=3.0, alpha_max=math.pi * 60.0, iwipe=1)
+    mat = B.pve_material(*d["model"], d["dt"])
+    rec = list(d["rec"]) + list(extra_rec)
+    return B.pve_run(cfg, mat, src, [r[0] for r in rec], [r[1] for r in rec], nsteps)
+
+
+@pytest.mark.parametrize("template,with_pml", [(1, False), (9, True)])
+def test_pve_iso_pml_program(tmp_path, template, with_pml):
+    """The drop-in pve_iso_pml (FP64) against the oracle: every trace line
+    (divergence and curl included), the p snapshot (with the absorbing
+    strips), the kinetic energy of the region in E/pve_E.dat (eq. 7.145 of
+    Carcione 2007 from the oracle's velocities at every region cell) and the
+    check_model files at the source frequency."""
+    from seismod_amd.solver import nsteps as sm_nsteps, record_due
+    d = deck_pve(tmp_path, template=template, with_pml=with_pml)
+    r = run("pve_iso_pml", tmp_path, "f64")
+    assert "Max/min phase velocities" in r.stdout
+    ns = sm_nsteps(0.05, d["dt"])
+    nz2 = d["nz"] + 2 * d["np"]
+    p = d["np"]
+    roi = [(i + p) * nz2 + j + p for i, j in d["roi_cells"]]
+    extra = [(c, k, 0.0, 0.0) for k in (0, 2, 3, 4) for c in roi]
+    tr, _ = oracle_pve(d, ns, extra)
+    for q, (_, _, t0, rdt) in enumerate(d["rec"]):
+        got = read_trc(tmp_path / "trc" / "pve" / f"pve_{q + 1}.trc")
+        _close_lines(got, trace_lines(tr[:, q], d["dt"], t0, rdt * 1e-3), q)
+    # energy: rho11, rho12, rho22 of eq. 7.153-7.160 per region cell
+    nr, nq = len(d["rec"]), len(roi)
+    vx, vz, qx, qz = (tr[:, nr + k * nq:nr + (k + 1) * nq] for k in range(4))
+    phi, rs, rf, T = (np.array([d["raw"][q][i, j] for i, j in d["roi_cells"]]) for q in (3, 5, 6, 7))
+    r12 = -phi * rf * (T - 1)
+    r11 = (1 - phi) * rs - r12
+    r22 = phi * rf - r12
+    E = (0.5 * r11 * (vx * vx + vz * vz) + r12 * (vx * qx + vz * qz) + 0.5 * r22 * (qx * qx + qz * qz)).sum(1)
+    lines = (tmp_path / "E" / "pve_E.dat").read_text().splitlines()
+    assert len(lines) == ns
+    got = np.array([[float(x) for x in l.split()] for l in lines])
+    assert np.allclose(got[:, 0], d["dt"] * (np.arange(ns) + 1), rtol=1e-6)
+    assert np.abs(E).max() > 0
+    assert np.abs(got[:, 1] - E).max() <= 1e-6 * np.abs(E).max()   # printed with 7 digits
+    snaps = sorted((tmp_path / "ssh" / "pve").glob("pve_7_*.nc"))
+    assert len(snaps) == 1
+    it_snap = [it for it in range(ns) if record_due(it, d["dt"], d["snap_t0"], 0.0)]
+    _, W = oracle_pve(d, it_snap[0] + 1)
+    want = (W[3].T.astype(np.float32) * np.float32(1e3)).astype(np.float64)
+    z = nc_field(snaps[0])
+    assert z.shape == want.shape
+    assert np.abs(z - want).max() <= 1e-5 * np.abs(want).max()
+    import test_host_io as H
+    vel, att = H._dispersion([a.reshape(-1) for a in d["model"]], 60.0)
+    for k, name in enumerate(("vel_slowP", "vel_fastP", "vel_S", "att_slowP", "att_fastP", "att_S")):
+        src = (vel if k < 3 else att)[k % 3].reshape(d["model"][0].shape)
+        assert np.array_equal(nc_field(tmp_path / f"pve_{name}.nc"), src.T.astype(np.float32).astype(np.float64))
+

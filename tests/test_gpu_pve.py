@@ -244,3 +244,41 @@ def test_pve_own_transforms_match_hipfft(ctx, case, storage):
     e_tr, e_f = errors(d, tr, f, tr_h, f_h)
     note("pve_own_vs_hipfft", case=ids(case), storage=storage, **errors.last)
     assert e_tr < tol and e_f < tol, errors.last
+
+
+def test_pve_energy_of_a_region(ctx):
+    """The kinetic energy record (saveEnergy): the quadratic form of v and q
+    summed over a region after every step, against the oracle's v and q at
+    every region cell; the same bits on a second run (a fixed summation
+    order); a region outside the grid is refused."""
+    from seismod_amd._lib import SM_ERR_ARG, SeismodError
+    from seismod_amd.solver import Poroviscoelastic
+    d = deck(seed=9, nsteps=60)
+    nx2, nz2 = d["model"][0].shape
+    i1, i2, j1, j2 = nx2 // 2 - 3, nx2 // 2 + 2, nz2 // 2 - 2, nz2 // 2 + 4
+    cells = [i * nz2 + j for i in range(i1, i2 + 1) for j in range(j1, j2 + 1)]
+    rng = np.random.default_rng(3)
+    r11, r22 = rng.uniform(2000, 2600, len(cells)), rng.uniform(1500, 2500, len(cells))
+    r12 = -rng.uniform(100, 400, len(cells))
+    d["ridx"] = np.array(cells * 4, np.uintp)
+    d["rcomp"] = np.repeat(np.array([B.VX, B.VZ, B.QX, B.QZ], np.intc), len(cells))
+    tr_o, _ = run_oracle(d)
+    vx, vz, qx, qz = (tr_o[:, k * len(cells):(k + 1) * len(cells)] for k in range(4))
+    want = (0.5 * r11 * (vx * vx + vz * vz) + r12 * (vx * qx + vz * qz) + 0.5 * r22 * (qx * qx + qz * qz)).sum(1)
+    got = []
+    for _ in range(2):
+        s = Poroviscoelastic(ctx, config(d), *d["model"])
+        s.set_sources(d["types"], d["idx"], d["Lf"], d["fct"])
+        s.set_energy(i1, i2, j1, j2, r11, r12, r22, d["nsteps"])
+        s.run(d["nsteps"])
+        got.append(s.energy(0, d["nsteps"]))
+        with pytest.raises(SeismodError) as e:
+            big = np.ones((nx2 - i1 + 1) * (j2 - j1 + 1))
+            s.set_energy(i1, nx2, j1, j2, big, big, big, 5)
+        assert e.value.status == SM_ERR_ARG
+        s.close()
+    assert np.abs(want).max() > 0
+    err = float(np.abs(got[0] - want).max() / np.abs(want).max())
+    note("pve_energy_vs_oracle", err=err)
+    assert err < PVE_FP64_TOL, err
+    assert np.array_equal(got[0], got[1])

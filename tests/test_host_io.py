@@ -28,7 +28,9 @@ class Params(C.Structure):
                 ("iwipe", C.c_int), ("plot_strips", C.c_int), ("segy", C.c_int),
                 ("shotpt_no", C.c_int), ("simulate_cmp", C.c_int), ("np", C.c_size_t),
                 ("vmax", C.c_double), ("pml_order", C.c_double), ("Rc", C.c_double),
-                ("kappa_max", C.c_double), ("alpha_max", C.c_int)]
+                ("kappa_max", C.c_double), ("alpha_max", C.c_int), ("mode", C.c_int),
+                ("check_model", C.c_int), ("chkpt_inc", C.c_int), ("pml_median", C.c_int),
+                ("save_energy", C.c_int), ("roi", C.c_double * 4)]
 
 
 class Grid(C.Structure):
@@ -609,3 +611,157 @@ def test_ve_moduli_against_oracle():
     m = V.ve_material(*model)
     for k in ("c11", "c13", "c33", "c55", "K0"):
         assert np.allclose(got[k], m[k], rtol=1e-14, atol=0), k
+
+
+def test_par_poroviscoelastic_keywords(tmp_path):
+    """check_model, chkpt_inc, pml_median, saveEnergy and EnergyROI
+    (io_utils.c:257-280)."""
+    p = parse_par(tmp_path, "2 # check_model, store\n500 # chkpt_inc,\n1 # pml_median,\n"
+                            "1 # saveEnergy, E(t)\n10/250/5.5/180 # EnergyROI, region\n")
+    assert (p.check_model, p.chkpt_inc, p.pml_median, p.save_energy) == (2, 500, 1, 1)
+    assert list(p.roi) == [10.0, 250.0, 5.5, 180.0]
+
+
+def _read_pve(fn, np_):
+    g = Grid()
+    wp = C.c_int()
+    LIB.sm_read_grid_pve(str(fn).encode(), C.c_size_t(np_), C.byref(g), C.byref(wp))
+    shp = (g.nx + 2 * np_, g.nz + 2 * np_)
+    out = [np.zeros(shp) for _ in range(12)]
+    arr = (C.c_void_p * 12)(*(o.ctypes.data_as(C.c_void_p) for o in out))
+    LIB.sm_read_model_pve(str(fn).encode(), C.byref(g), C.c_size_t(np_), wp, arr)
+    return g, wp.value, np.stack(out)
+
+
+def _read_pve_subprocess(fn, np_):
+    code = (f"import numpy as np, sys; sys.path.insert(0, {str(ROOT / 'tests')!r});"
+            f"import test_host_io as T; g, w, a = T._read_pve({str(fn)!r}, {np_});"
+            "np.save(sys.argv[1], a)")
+    out = Path(str(fn) + ".npy")
+    r = subprocess.run(["python3", "-c", code, str(out)], capture_output=True, text=True)
+    return r.returncode, r.stderr, (np.load(out) if r.returncode == 0 else None)
+
+
+def _pve_values(rng, nx, nz):
+    lo = [2, 30, 1.5, 0.1, 3, 2500, 900, 1.5, 0.5, 100, 10, 10]
+    hi = [15, 40, 3.0, 0.35, 10, 2800, 1100, 3.0, 2.0, 2000, 100, 80]
+    return np.stack([rng.uniform(a, b, (nx, nz)) for a, b in zip(lo, hi)])
+
+
+@pytest.mark.parametrize("kind", ["full", "column", "with_pml"])
+def test_read_model_pve_layout_and_padding(tmp_path, kind):
+    """read_grid_params / read_model (io_utils.c:296-328, 403-625): 12 values
+    per node, z fastest; "isotropic": one column fills every x and the
+    nearest model node fills the layers; "isotropic_pml": the file holds the
+    padded grid and its header counts the layers."""
+    rng = np.random.default_rng(12)
+    nx, nz, np_ = 4, 5, 2
+    if kind == "with_pml":
+        m = _pve_values(rng, nx + 2 * np_, nz + 2 * np_)
+        head = f"{nx + 2 * np_} {nz + 2 * np_}\n2.0 2.5\n1.0 0.0\nIsotropic_PML\n"
+    else:
+        m = _pve_values(rng, nx, nz)
+        if kind == "column":
+            m[:] = m[:, :1, :]
+        head = f"{nx} {nz}\n2.0 2.5\n1.0 0.0\nisotropic\n"
+    fn = tmp_path / "pve.dat"
+    with open(fn, "w") as f:
+        f.write(head)
+        for i in range(1 if kind == "column" else m.shape[1]):
+            for j in range(m.shape[2]):
+                f.write(" ".join(repr(float(m[q, i, j])) for q in range(12)) + "\n")
+    g, wp, out = _read_pve(fn, np_)
+    assert (g.nx, g.nz, g.dx, g.dz, g.x0, wp) == (nx, nz, 2.0, 2.5, 1.0, kind == "with_pml")
+    for q in range(12):
+        want = m[q] if kind == "with_pml" else np.pad(m[q], np_, mode="edge")
+        assert np.array_equal(out[q], want), q
+
+
+@pytest.mark.parametrize("cut", ["short", "bad_type", "pml_short", "pml_too_small"])
+def test_read_model_pve_errors(tmp_path, cut):
+    rng = np.random.default_rng(13)
+    nx, nz, np_ = 4, 5, 2
+    m = _pve_values(rng, nx + 2 * np_, nz + 2 * np_)
+    toks = [repr(float(v)) for v in m.transpose(1, 2, 0).reshape(-1)]
+    if cut == "short":
+        head, body = f"{nx} {nz}\n2 2\n0 0\nisotropic\n", toks[:12 * (nz + 1)]
+    elif cut == "bad_type":
+        head, body = f"{nx} {nz}\n2 2\n0 0\nelastic\n", toks[:12 * nx * nz]
+    elif cut == "pml_short":
+        head, body = f"{nx + 2 * np_} {nz + 2 * np_}\n2 2\n0 0\nisotropic_pml\n", toks[:-3]
+    else:
+        head, body = f"{2 * np_} {nz}\n2 2\n0 0\nisotropic_pml\n", toks
+    fn = tmp_path / f"{cut}.dat"
+    fn.write_text(head + "\n".join(body) + "\n")
+    rc, err, _ = _read_pve_subprocess(fn, np_)
+    assert rc == 1
+    assert {"short": "something wrong", "bad_type": "not isotropic", "pml_short": "something wrong",
+            "pml_too_small": "not larger than its absorbing layers"}[cut] in err
+
+
+def _dispersion(vals, f):
+    n = vals[0].size
+    vel = [np.zeros(n) for _ in range(3)]
+    att = [np.zeros(n) for _ in range(3)]
+    arr = (C.c_void_p * 12)(*(np.ascontiguousarray(v, np.float64).ctypes.data_as(C.c_void_p) for v in vals))
+    LIB.sm_pve_dispersion(C.c_size_t(n), arr, C.c_double(f), (C.c_void_p * 3)(*(a.ctypes.data_as(C.c_void_p)
+                                                                                for a in vel)),
+                          (C.c_void_p * 3)(*(a.ctypes.data_as(C.c_void_p) for a in att)))
+    return np.stack(vel), np.stack(att)
+
+
+def test_pve_dispersion_limits():
+    """The check_model velocities (pve_iso_pml.c:864-904) against closed
+    forms independent of the dispersion relation's algebra:
+      - lossless (eta = 0, Q >= 1e4): Biot's fast and slow P velocities, the
+        square roots of the eigenvalues of R^-1 C (C the undrained stiffness
+        [[E + alpha^2 M, alpha M], [alpha M, M]], R the density matrix
+        [[rho, rho_f], [rho_f, T rho_f / phi]]), and S = sqrt(mu / (rho -
+        phi rho_f / T)); no attenuation;
+      - viscous fluid at low frequency (eta/kappa >> omega rho_f T / phi):
+        the fast P velocity tends to Gassmann's sqrt((K_G + 4/3 mu) / rho)
+        and S to sqrt(mu / rho) (the fluid moves with the frame);
+      - a planted mistake (a sign of the coupling term) moves the lossless
+        velocities by more than 1%."""
+    rng = np.random.default_rng(21)
+    n = 16
+    vals = _pve_values(rng, 1, n).reshape(12, n)
+    loss = vals.copy()
+    loss[8] = 0.0
+    loss[10] = 2e4
+    vel, att = _dispersion(loss, 40.0)
+    Km, Ks, Kf, phi, mu = loss[0] * 1e3, loss[1] * 1e3, loss[2] * 1e3, loss[3], loss[4] * 1e3
+    rs, rf, T = loss[5] * 1e-6, loss[6] * 1e-6, loss[7]
+    alpha = 1 - Km / Ks
+    M = 1. / ((alpha - phi) / Ks + phi / Kf)
+    E = Km + 4. / 3. * mu
+    rho = (1 - phi) * rs + phi * rf
+    for c in range(n):
+        C_ = np.array([[E[c] + alpha[c] ** 2 * M[c], alpha[c] * M[c]], [alpha[c] * M[c], M[c]]])
+        R = np.array([[rho[c], rf[c]], [rf[c], T[c] * rf[c] / phi[c]]])
+        lam = np.sort(np.linalg.eigvals(np.linalg.solve(R, C_)).real)
+        assert np.allclose(vel[:2, c], np.sqrt(lam), rtol=1e-12), c
+        bad = np.array([[E[c] + alpha[c] ** 2 * M[c], -alpha[c] * M[c]], [-alpha[c] * M[c], M[c]]])
+        lam_bad = np.sort(np.linalg.eigvals(np.linalg.solve(R, bad)).real)
+        assert np.abs(np.sqrt(lam_bad) / vel[:2, c] - 1).max() > 1e-2
+    assert np.allclose(vel[2], np.sqrt(mu / (rho - phi * rf / T)), rtol=1e-12)
+    assert np.abs(att).max() < 1e-12
+    visc = vals.copy()
+    visc[10] = 2e4
+    visc[8] = 1e3            # eta 1000 cP, kappa 100-2000 mD
+    vel, _ = _dispersion(visc, 1e-3)
+    Kg = Km + (1 - Km / Ks) ** 2 / (phi / Kf + (1 - phi) / Ks - Km / Ks ** 2)
+    assert np.allclose(vel[1], np.sqrt((Kg + 4. / 3. * mu) / rho), rtol=1e-6)
+    assert np.allclose(vel[2], np.sqrt(mu / rho), rtol=1e-6)
+
+
+@pytest.mark.parametrize("extra,msg", [("500 # chkpt_inc, every 500\n", "checkpointing"),
+                                       ("1 # pml_median, on\n", "pml_median")])
+def test_pve_iso_pml_refuses_unavailable_options(tmp_path, extra, msg):
+    """pve_iso_pml stops with a message, before reading the model or using
+    a GPU, when checkpoints or the median smoothing of the layers are asked
+    for; and takes -h like the other programs."""
+    assert run_bin("pve_iso_pml", "-h").returncode == 0
+    (tmp_path / "run.par").write_text("model.dat # model,\nsrc.dat # source,\nrcv.dat # output,\n" + extra)
+    r = run_bin("pve_iso_pml", "-p", "run.par", cwd=tmp_path)
+    assert r.returncode == 1 and msg in r.stderr, r.stderr
