@@ -474,3 +474,95 @@ def algorithmic_bytes_ve(L: int = 2, elt: int = 8) -> float:
     (stages 2-4), writing 7 + L accumulated values and 7 + L next-stage
     inputs (stages 1-3): 184 + 25 L.  248 + 25 L values per step."""
     return elt * (248.0 + 25.0 * L)
+
+
+@dataclass
+class PveShot:
+    nx: int
+    nz: int
+    np: int
+    h: float
+    dt: float
+    f0: float
+    nsteps: int
+    model: tuple            # read_model's padded arrays: K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, eta, kappa, Q, fq
+    src_idx: np.ndarray     # one Bulk source
+    rec_idx: np.ndarray
+    rec_comp: np.ndarray
+
+    @property
+    def cells(self) -> int:
+        return int(self.model[0].size)
+
+
+def biot_fast_velocity(K_m, K_s, K_f, phi, mu, rho_s, rho_f, T) -> np.ndarray:
+    """Fast P velocity (m/s) of the unrelaxed, inviscid Biot medium: the
+    square root of the larger eigenvalue of R^-1 C, C the undrained stiffness
+    [[K_m + 4/3 mu + alpha^2 M, alpha M], [alpha M, M]] and R the density
+    matrix [[rho, rho_f], [rho_f, T rho_f / phi]] (moduli in GPa, densities
+    in kg/m^3; elementwise)."""
+    Km, Ks, Kf, G = (np.asarray(a, np.float64) * 1e9 for a in (K_m, K_s, K_f, mu))
+    phi, rs, rf, T = (np.asarray(a, np.float64) for a in (phi, rho_s, rho_f, T))
+    alpha = 1 - Km / Ks
+    M = 1. / ((alpha - phi) / Ks + phi / Kf)
+    c11, c12, c22 = Km + 4. / 3. * G + alpha * alpha * M, alpha * M, M
+    r11, r12, r22 = (1 - phi) * rs + phi * rf, rf, T * rf / phi
+    # eigenvalues of R^-1 C from det(C - lambda R) = 0
+    a = r11 * r22 - r12 * r12
+    b = -(c11 * r22 + c22 * r11 - 2 * c12 * r12)
+    c = c11 * c22 - c12 * c12
+    return np.sqrt((-b + np.sqrt(b * b - 4 * a * c)) / (2 * a))
+
+
+def pve_layered_shot(n: int = 1000, np_: int = 20, h: float = 5.0, seed: int = 13, f0: float = 25.0,
+                     nsteps: int = 1000, nrec: int = 200) -> PveShot:
+    """pve_iso_pml workload: n x n nodes of h metres plus np CPML layers,
+    5-10 horizontal layers of water-saturated sandstone: K_s ~ U[33, 39] GPa,
+    K_m ~ U[4, 16] GPa (stiffer with depth), K_f = 2.25 GPa, phi ~
+    U[0.1, 0.32], mu ~ U[3, 12] GPa, rho_s ~ U[2600, 2700], rho_f = 1000
+    kg/m^3, T ~ U[1.5, 3], eta = 1 cP, kappa ~ U[50, 2000] mD, Q ~ U[20,
+    100] at f0; a Bulk source at (n/2, 20) nodes and a line of nrec Vz
+    receivers at depth 10 nodes.  dt at 40% of the RK4 pseudo-spectral
+    stability limit of the fast P wave."""
+    rng = np.random.default_rng(seed)
+    nl = int(rng.integers(5, 11))
+    tops = np.sort(rng.choice(np.arange(1, n), size=nl - 1, replace=False))
+    lay = np.searchsorted(tops, np.arange(n), side="right")
+    one = np.ones((n, 1))
+    layer = lambda lo, hi, sort=False: (np.sort(rng.uniform(lo, hi, nl)) if sort else
+                                        rng.uniform(lo, hi, nl))[lay] * one
+    K_s = layer(33.0, 39.0)
+    K_m = layer(4.0, 16.0, sort=True)
+    K_f = np.full((n, n), 2.25)
+    phi = layer(0.1, 0.32)
+    mu = layer(3.0, 12.0, sort=True)
+    rho_s = layer(2600.0, 2700.0)
+    rho_f = np.full((n, n), 1000.0)
+    T = layer(1.5, 3.0)
+    eta = np.full((n, n), 1.0)
+    kappa = layer(50.0, 2000.0)
+    Q = layer(20.0, 100.0)
+    fq = np.full((n, n), f0)
+    vals = (K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, eta, kappa, Q, fq)
+    vmax = float(biot_fast_velocity(*vals[:8]).max())
+    dt = 0.4 * 2.0 * h / (np.pi * vmax * np.sqrt(2.0))
+    pad = lambda a: np.pad(a, np_, mode="edge")
+    nz2 = n + 2 * np_
+    src = np.array([(n // 2 + np_) * nz2 + 20 + np_], dtype=np.uintp)
+    xs = np.linspace(np_, n + np_ - 1, nrec).astype(int)
+    rec = np.array([x * nz2 + 10 + np_ for x in xs], dtype=np.uintp)
+    return PveShot(nx=n, nz=n, np=np_, h=h, dt=float(dt), f0=f0, nsteps=nsteps, model=tuple(pad(a) for a in vals),
+                   src_idx=src, rec_idx=rec, rec_comp=np.full(nrec, 2, dtype=np.intc))
+
+
+def algorithmic_bytes_pve(elt: int = 8) -> float:
+    """HBM bytes per padded cell per step of the pve_iso_pml update, counted as
+    for algorithmic_bytes_sh: 4 stages x 12 derivatives x 2 values (96); the
+    analytic step reading v, q and their 4 coefficients and writing v and q
+    (12); the fused pointwise pass per stage reading the 12 derivatives, 13
+    material arrays and the 9 step-start values, writing 9 accumulated
+    values, with 9 accumulated values read in stages 2-4, 9 next-stage
+    inputs written in stages 1-3 and the stage input's relaxation variable
+    read in stages 2-4 (the step-start values in stage 1): 52 + 62 + 62 + 53
+    = 229.  337 values per step."""
+    return elt * 337.0
