@@ -423,7 +423,7 @@ size_t sm_cyl_bytes_needed(const sm_cyl_params *p, size_t nrec, size_t max_steps
 
 /* ------------------------------------------------------------------ */
 /* Viscoelastic SH waves in a 2D VTI medium (ve_vti_sh_pml) --          */
-/* pseudo-spectral derivatives (hipFFT), CPML, fourth-order Runge-Kutta; */
+/* pseudo-spectral derivatives (LDS FFTs), CPML, fourth-order Runge-Kutta;*/
 /* replaces ve_vti_sh_pml.c:475-700, propagate.c:540-591, 2280-2510     */
 /* (init_fftw_data_ve_sh, propagateVE_SH_CPML, partial_*_sh), pml.c:    */
 /* 1689-1757 and 1959-2106 (alloc_cpml_ve_sh, compute_cpml), src.c:     */
@@ -484,7 +484,7 @@ size_t sm_sh_bytes_needed(const sm_sh_params *p, size_t nrec, size_t max_steps);
 
 /* ------------------------------------------------------------------ */
 /* Viscoelastic P-SV waves in a 2D VTI medium (ve_vti_pml) --           */
-/* pseudo-spectral derivatives (hipFFT), CPML, fourth-order Runge-Kutta, */
+/* pseudo-spectral derivatives (LDS FFTs), CPML, fourth-order Runge-Kutta,*/
 /* L relaxation mechanisms for the quasi-dilatational mode and one for   */
 /* shear; replaces ve_vti_pml.c:518-784, propagate.c:325-510 and         */
 /* 1959-2278 (init_fftw_data_ve, propagateVE_CPML, partial_*), pml.c:    */

@@ -7,8 +7,9 @@
 // (D_x^- tau_xy, D_z^- tau_yz, D_z^+ v_y, D_x^+ v_y), corrects them in the
 // CPML layers and forms the pointwise right-hand sides, which the
 // Runge-Kutta combination then folds into the state.  Here a stage is the
-// four derivatives (pstd.hpp: batched hipFFT transforms and a multiplier
-// pass each) followed by ONE fused kernel that applies the layer
+// four derivatives (pstd.hpp: one LDS transform launch per axis, or
+// hipFFT plans and a multiplier pass for other lengths) followed by ONE
+// fused kernel that applies the layer
 // corrections and memory-variable updates, the right-hand sides and the
 // Runge-Kutta update of every state value.  The step-start copies of the
 // state are replaced by rotating three state buffers (w, ws, wt), and the

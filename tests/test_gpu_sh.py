@@ -1,8 +1,9 @@
-"""GPU parity of the ve_vti_sh_pml solver (viscoelastic SH, 2D VTI, hipFFT
-derivatives, CPML, RK4) against its FP64 oracle (oracle/oracle_sh.py),
-through the C ABI.
+"""GPU parity of the ve_vti_sh_pml solver (viscoelastic SH, 2D VTI,
+pseudo-spectral derivatives, CPML, RK4) against its FP64 oracle
+(oracle/oracle_sh.py), through the C ABI.
 
-The GPU FFTs are rocFFT's and the oracle's are numpy's, so FP64 results
+The GPU FFTs (hand-written LDS transforms, hipFFT for other lengths) and
+the oracle's (numpy) differ in rounding, so FP64 results
 agree to FFT rounding rather than bit for bit: traces and final fields are
 held to SH_FP64_TOL relative L2.  FP32 storage is held to SH_FP32_TOL.
 Decks: layered VTI viscoelastic with layers on every side, a model wider
