@@ -587,6 +587,20 @@ int sm_pve_create(sm_ctx *ctx, const sm_pve_params *p, const double *K_m, const 
                   const double *K_f, const double *phi, const double *mu, const double *rho_s,
                   const double *rho_f, const double *T, const double *eta, const double *kappa,
                   const double *Q, const double *fq, sm_pve **out);
+/* A VTI Biot medium (pve_vti_pml; read_modelVTI, io_utils.c:627-800, and
+ * pve_vti_pml.c:787-1145): the inputs of sm_pve_create with tortuosity and
+ * permeability along x (T1, kappa1) and z (T3, kappa3) and Thomsen's
+ * epsilon and delta of the frame; c33 = K_m + 4/3 mu, c55 = mu,
+ * c11 = (1 + 2 epsilon) c33, c13 from delta, the Biot modulus and the
+ * effective-stress coefficients alpha1, alpha3 of the anisotropic frame.
+ * The same time loop and records (the stage update of propagate.c:
+ * 1514-1958).  Holds two material arrays more than the isotropic solver
+ * (sm_pve_bytes_needed counts them). */
+int sm_pve_create_vti(sm_ctx *ctx, const sm_pve_params *p, const double *K_m, const double *K_s,
+                      const double *K_f, const double *phi, const double *mu, const double *rho_s,
+                      const double *rho_f, const double *T1, const double *T3, const double *eta,
+                      const double *kappa1, const double *kappa3, const double *Q, const double *fq,
+                      const double *epsilon, const double *delta, sm_pve **out);
 int sm_pve_destroy(sm_pve *s);
 /* Sources on padded indices (add_src_rk4): SM_SF adds to p, SM_SX to
  * tau_xx, SM_SZ to tau_zz, SM_BULK to p, tau_xx and tau_zz, SM_BULK_S to
@@ -606,10 +620,16 @@ int sm_pve_run(sm_pve *s, size_t nsteps);
  * rho12 (v_x q_x + v_z q_z) + 0.5 rho22 (q_x^2 + q_z^2), rho11, rho12 and
  * rho22 given per region cell (i outer, (i2-i1+1)(j2-j1+1) values each);
  * accumulated in FP64 in a fixed order (results repeat bit for bit).
- * max_steps 0 stops recording.  Adds 24 bytes per region cell and 8 per
- * step to the device memory. */
+ * max_steps 0 stops recording. */
 int sm_pve_set_energy(sm_pve *s, size_t i1, size_t i2, size_t j1, size_t j2, const double *rho11,
                       const double *rho12, const double *rho22, size_t max_steps);
+/* The same with different coefficients along x and z (pve_vti_pml.c:
+ * 1106-1121 and 1334-1345): the sum of 0.5 ax v_x^2 + bx v_x q_x +
+ * 0.5 cx q_x^2 + 0.5 az v_z^2 + bz v_z q_z + 0.5 cz q_z^2.  Adds 48 bytes
+ * per region cell and 8 per step (so does sm_pve_set_energy). */
+int sm_pve_set_energy_axes(sm_pve *s, size_t i1, size_t i2, size_t j1, size_t j2, const double *ax,
+                           const double *bx, const double *cx, const double *az, const double *bz, const double *cz,
+                           size_t max_steps);
 /* count values from row it0 of the energy record */
 int sm_pve_get_energy(sm_pve *s, size_t it0, size_t count, double *out);
 int sm_pve_set_timing(sm_pve *s, int on);

@@ -107,6 +107,70 @@ def pve_material(K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, eta, kappa, Q, f0, dt)
     return out
 
 
+def pve_vti_material(K_m, K_s, K_f, phi, mu, rho_s, rho_f, T1, T3, eta, kappa1, kappa3, Q, f0, epsilon, delta,
+                     dt):
+    """Material of pve_vti_pml.c:787-1145 (VTI Biot medium after Carcione
+    1996): c33 = K_m + 4/3 mu, c55 = mu, Thomsen's c11 and c13, the Biot
+    modulus and effective-stress coefficients alpha1, alpha3 of the
+    anisotropic frame, tortuosities T1, T3 and permeabilities kappa1,
+    kappa3 along x and z (the x ones at (i+1/2, j), the z ones at
+    (i, j+1/2)).  Inputs as read_modelVTI leaves them.  Returns the stored
+    arrays under the keys the propagation reads."""
+    pi = 4.0 * math.atan(1.0)
+    a = lambda x: np.asarray(x, np.float64)
+    K_m, K_s, K_f, mu = a(K_m) * 1000., a(K_s) * 1000., a(K_f) * 1000., a(mu) * 1000.
+    eta = a(eta) * ETA_SCALE
+    rho_s, rho_f = a(rho_s) * 1.e-6, a(rho_f) * 1.e-6
+    phi, T1, T3, kappa1, kappa3, Q, f0 = a(phi), a(T1), a(T3), a(kappa1), a(kappa3), a(Q), a(f0)
+    epsilon, delta = a(epsilon), a(delta)
+    tau0 = 1. / (2. * pi * f0)
+    D = K_s * (1. + phi * (K_s / K_f - 1.))
+    c33 = K_m + 4. / 3. * mu
+    c55 = mu
+    c11 = 2. * epsilon * c33 + c33
+    c12 = c11 - 2. * c55
+    c13 = np.sqrt(2. * delta * c33 * (c33 - c55) + (c33 - c55) * (c33 - c55)) - c55
+    M = K_s * K_s / (D - (2. * c11 + c33 + 2. * c12 + 4. * c13) / 9.)
+    alpha1 = 1. - 1. / 3. * (c11 + c12 + c13) / K_s
+    alpha3 = 1. - 1. / 3. * (2. * c13 + c33) / K_s
+    m1 = T1 * rho_f / phi
+    m3 = T3 * rho_f / phi
+    lowQ = Q < 1.e4
+    Qs = np.where(lowQ, Q, 1.0)
+    tau_e = tau0 / Qs * (np.sqrt(Qs * Qs + 1.) + 1.)
+    tau_s = np.where(lowQ, tau0 / Qs * (np.sqrt(Qs * Qs + 1.) - 1.), tau0)
+    varphi = np.where(lowQ, tau_e / np.where(lowQ, tau_s, 1.0) - 1., 0.0)
+    rho = (1. - phi) * rho_s + phi * rho_f
+    z = np.zeros_like
+    rho_i, rho_f_i, nk_i, m_i, cq_i, cv_i = z(M), z(M), z(M), z(M), z(M), z(M)
+    rho_j, rho_f_j, nk_j, m_j, cq_j, cv_j = z(M), z(M), z(M), z(M), z(M), z(M)
+    c55_ij = z(M)
+
+    def coeffs(r, rf, mm, nkk):
+        tmp = 1. / (rf * rf - r * mm)
+        beta12 = tmp * rf
+        beta22 = tmp * -r
+        cq = np.exp(-nkk * beta22 * dt)
+        return cq, beta12 / beta22 * (cq - 1.)
+    nk1, nk3 = eta / kappa1, eta / kappa3
+    rho_i[:-1] = 0.5 * (rho[:-1] + rho[1:]); rho_f_i[:-1] = 0.5 * (rho_f[:-1] + rho_f[1:])
+    nk_i[:-1] = 0.5 * (nk1[:-1] + nk1[1:]); m_i[:-1] = 0.5 * (m1[:-1] + m1[1:])
+    cq_i[:-1], cv_i[:-1] = coeffs(rho_i[:-1], rho_f_i[:-1], m_i[:-1], nk_i[:-1])
+    rho_j[:, :-1] = 0.5 * (rho[:, :-1] + rho[:, 1:]); rho_f_j[:, :-1] = 0.5 * (rho_f[:, :-1] + rho_f[:, 1:])
+    nk_j[:, :-1] = 0.5 * (nk3[:, :-1] + nk3[:, 1:]); m_j[:, :-1] = 0.5 * (m3[:, :-1] + m3[:, 1:])
+    cq_j[:, :-1], cv_j[:, :-1] = coeffs(rho_j[:, :-1], rho_f_j[:, :-1], m_j[:, :-1], nk_j[:, :-1])
+    with np.errstate(divide="ignore"):
+        c55_ij[:-1, :-1] = 4.0 / (1. / mu[:-1, :-1] + 1. / mu[1:, :-1] + 1. / mu[:-1, 1:] + 1. / mu[1:, 1:])
+    for x in (rho_i, rho_f_i, nk_i, m_i, cq_i, cv_i, c55_ij):
+        x[-1, :] = x[-2, :]
+    for x in (rho_j, rho_f_j, nk_j, m_j, cq_j, cv_j, c55_ij):
+        x[:, -1] = x[:, -2]
+    return dict(c11=c11, c13=c13, c33=c33, alpha1=alpha1, alpha3=alpha3, M=M, ts=1. / tau_s,
+                varphi=M * varphi / (1. + varphi), mu_ij=c55_ij, cq_i=cq_i, cv_i=cv_i, cq_j=cq_j, cv_j=cv_j,
+                m_i=rho_f_i * rho_f_i / rho_i - m_i, m_j=rho_f_j * rho_f_j / rho_j - m_j, rf_i=rho_f_i / rho_i,
+                rf_j=rho_f_j / rho_j, irho_i=1. / rho_i, irho_j=1. / rho_j)
+
+
 class PveState:
     """W (9 components) and the CPML memory variables mem[0..2]
     (alloc_cpml, pml.c:1409-1491)."""
@@ -159,7 +223,10 @@ def propagate(Wt, delta, mat, fp, mem, np_, mx_f, mx_b, mz_f, mz_b):
     _x_layer(t1, "dx_vx", mem, np_, np_ - 1, *Bx)
     t2 = deriv(vz, 1, mz_b)
     _z_layer(t2, "dz_vz", mem, np_, np_ - 1, *Bz)
-    eps = mat["alpha"] * (t1 + t2)
+    if "c11" in mat:         # propagateVTI_CPML (propagate.c:1786)
+        eps = mat["alpha1"] * t1 + mat["alpha3"] * t2
+    else:
+        eps = mat["alpha"] * (t1 + t2)
     t3 = deriv(qx, 0, mx_b)
     _x_layer(t3, "dx_qx", mem, np_, np_ - 1, *Bx)
     eps = eps + t3
@@ -170,8 +237,12 @@ def propagate(Wt, delta, mat, fp, mem, np_, mx_f, mx_b, mz_f, mz_b):
     delta[I_E] = -mat["ts"] * (mat["varphi"] * eps + e)
     eps = mat["M"] * eps + e
     delta[I_P] = -eps
-    delta[I_TXX] = mat["E"] * t1 + mat["Emu"] * t2 + mat["alpha"] * eps
-    delta[I_TZZ] = mat["Emu"] * t1 + mat["E"] * t2 + mat["alpha"] * eps
+    if "c11" in mat:         # propagate.c:1871-1880
+        delta[I_TXX] = mat["c11"] * t1 + mat["c13"] * t2 + mat["alpha1"] * eps
+        delta[I_TZZ] = mat["c13"] * t1 + mat["c33"] * t2 + mat["alpha3"] * eps
+    else:
+        delta[I_TXX] = mat["E"] * t1 + mat["Emu"] * t2 + mat["alpha"] * eps
+        delta[I_TZZ] = mat["Emu"] * t1 + mat["E"] * t2 + mat["alpha"] * eps
     # tau_xz (1253-1311)
     t1 = deriv(vx, 1, mz_f)
     _z_layer(t1, "dz_vx", mem, np_, np_, *Hz)
@@ -220,7 +291,7 @@ def pve_run(cfg, mat, src, rec_idx, rec_comp, nsteps):
     (nsteps x nrec; DIV and CURL from the last stage's input) and the final
     W."""
     np_ = cfg["np"]
-    nx2, nz2 = mat["E"].shape
+    nx2, nz2 = mat["M"].shape
     tabs = cpml_tables(np_, cfg["dx"], cfg["dz"], cfg["dt"], cfg.get("vmax", 5000.0), cfg.get("pml_order", 2.0),
                        cfg.get("Rc", 1e-5), cfg.get("kappa_max", 1.0), cfg.get("alpha_max", 0.0),
                        cfg.get("iwipe", 1))
@@ -238,11 +309,13 @@ def pve_run(cfg, mat, src, rec_idx, rec_comp, nsteps):
         Ws[I_VZ] = W[I_VZ] + mat["cv_j"] * W[I_QZ]
         Ws[I_QX] = mat["cq_i"] * W[I_QX]
         Ws[I_QZ] = mat["cq_j"] * W[I_QZ]
-        # 1270-1285: over 2np + 1 rows here (2np in the viscoelastic programs)
-        n1 = (2 * np_ + 1) * nz2
+        # 1270-1285: over 2np + 1 rows here (2np in the viscoelastic programs
+        # and pve_vti_pml.c:1226-1241)
+        rows = 2 * np_ if "c11" in mat else 2 * np_ + 1
+        n1 = rows * nz2
         for nm in PveState.DX:
             mem[0][nm][:n1] = mem[1][nm][:n1]
-        n2 = (2 * np_ + 1) * nx2
+        n2 = rows * nx2
         for nm in PveState.DZ:
             mem[0][nm][:n2] = mem[1][nm][:n2]
         Wt = Ws.copy()

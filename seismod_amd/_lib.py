@@ -199,11 +199,13 @@ _SIGS = {
     "sm_ve_device_bytes": (C.c_int, [C.c_void_p, _size_p]),
     "sm_ve_bytes_needed": (C.c_size_t, [C.POINTER(VeParams), C.c_size_t, C.c_size_t]),
     "sm_pve_create": (C.c_int, [C.c_void_p, C.POINTER(PveParams)] + [_dbl_p] * 12 + [C.POINTER(C.c_void_p)]),
+    "sm_pve_create_vti": (C.c_int, [C.c_void_p, C.POINTER(PveParams)] + [_dbl_p] * 16 + [C.POINTER(C.c_void_p)]),
     "sm_pve_destroy": (C.c_int, [C.c_void_p]),
     "sm_pve_set_sources": (C.c_int, [C.c_void_p, C.c_size_t, _int_p, _size_p, _size_p, _dbl_p]),
     "sm_pve_set_receivers": (C.c_int, [C.c_void_p, C.c_size_t, _size_p, _int_p, C.c_size_t]),
     "sm_pve_run": (C.c_int, [C.c_void_p, C.c_size_t]),
     "sm_pve_set_energy": (C.c_int, [C.c_void_p] + [C.c_size_t] * 4 + [_dbl_p] * 3 + [C.c_size_t]),
+    "sm_pve_set_energy_axes": (C.c_int, [C.c_void_p] + [C.c_size_t] * 4 + [_dbl_p] * 6 + [C.c_size_t]),
     "sm_pve_get_energy": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, _dbl_p]),
     "sm_pve_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sm_pve_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),

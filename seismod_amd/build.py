@@ -2,7 +2,7 @@
 
 Everything is built in-tree so the artefacts travel with the source tree:
   seismod_amd/lib/libseismod_hip.so   HIP kernels + C ABI (hipcc, gfx950)
-  seismod_amd/bin/e_iso, a_iso_3d, e_cyl_iso, ve_vti_sh_pml, ve_vti_pml, pve_iso_pml
+  seismod_amd/bin/e_iso, a_iso_3d, e_cyl_iso, ve_vti_sh_pml, ve_vti_pml, pve_iso_pml, pve_vti_pml
                                        drop-in programs (gcc, C99)
   oracle/_build/liboracle.so           FP64 CPU restatement (test checker)
 
@@ -72,13 +72,13 @@ def build_lib() -> Path:
 
 
 def build_host() -> list[Path]:
-    """Drop-in e_iso, a_iso_3d, e_cyl_iso, ve_vti_sh_pml, ve_vti_pml and pve_iso_pml programs linked
+    """Drop-in e_iso, a_iso_3d, e_cyl_iso, ve_vti_sh_pml, ve_vti_pml, pve_iso_pml and pve_vti_pml programs linked
     against the library, and the host I/O layer alone as libsm_hostio.so (tested on CPU)."""
     BINDIR.mkdir(exist_ok=True)
     LIBDIR.mkdir(exist_ok=True)
     _run(["gcc", *C_FLAGS, "-fopenmp", "-fPIC", "-shared", f"-I{HOST}", str(HOST / "sm_io.c"),
           "-lm", "-o", str(LIBDIR / "libsm_hostio.so")])
-    programs = ("e_iso", "a_iso_3d", "e_cyl_iso", "ve_vti_sh_pml", "ve_vti_pml", "pve_iso_pml")
+    programs = ("e_iso", "a_iso_3d", "e_cyl_iso", "ve_vti_sh_pml", "ve_vti_pml", "pve_iso_pml", "pve_vti_pml")
     common = sorted(p for p in HOST.glob("*.c") if p.stem not in programs)
     outs = []
     for prog in programs:

@@ -29,8 +29,10 @@ namespace {
 using pstd::Layer;
 
 enum { W_TXX, W_TZZ, W_TXZ, W_P, W_VX, W_VZ, W_QX, W_QZ, W_E, W_NF };
+// material arrays; the VTI medium stores c11 in M_EM, c13 in M_EMU, alpha1
+// in M_ALPHA and adds c33 and alpha3
 enum { M_EM, M_EMU, M_M, M_ALPHA, M_TS, M_VPHI, M_IRHOI, M_RFI, M_MI, M_IRHOJ, M_RFJ, M_MJ, M_MUIJ, M_CVI, M_CQI,
-       M_CVJ, M_CQJ, M_NMAT };
+       M_CVJ, M_CQJ, M_NMAT, M_C33 = M_NMAT, M_ALPHA3, M_NMAT_VTI };
 enum { L_DX_TXX, L_DZ_TXZ, L_DX_P, L_DX_TXZ, L_DZ_TZZ, L_DZ_P, L_DX_VX, L_DZ_VZ, L_DX_QX, L_DZ_QZ, L_DZ_VX,
        L_DX_VZ, L_NSET };
 // derivative slots of the divergence and curl records (after the 12 of a stage)
@@ -39,7 +41,7 @@ enum { D_DIVX = L_NSET, D_DIVZ, D_CURLX, D_CURLZ, D_NSLOT };
 template <typename T>
 struct PveStage {
     const T *d;          // the twelve derivatives in L_* order (N each)
-    const T *mat;        // M_NMAT x N
+    const T *mat;        // M_NMAT (isotropic) or M_NMAT_VTI arrays of N
     const T *x;          // stage input state
     const T *ws;         // state at the start of the step
     T *w, *wt;
@@ -58,8 +60,9 @@ __device__ __forceinline__ void rk(const PveStage<T> &a, size_t q, T d)
     a.w[q] = (a.acc ? a.w[q] : s) + a.b * d;
 }
 
-// propagate.c:884-1329 from the twelve derivatives on, then eq. (79)-(83)
-template <typename T>
+// propagate.c:884-1329 (isotropic) or 1514-1958 (VTI) from the twelve
+// derivatives on, then eq. (79)-(83)
+template <typename T, bool VTI>
 __global__ __launch_bounds__(256) void pve_stage(const PveStage<T> a)
 {
     const unsigned n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -89,7 +92,9 @@ __global__ __launch_bounds__(256) void pve_stage(const PveStage<T> a)
     // epsilon = alpha (D_x^- v_x + D_z^- v_z) + D_x^- q_x + D_z^- q_z
     // (1100-1211); e, p, tau_xx, tau_zz (1213-1251)
     const T alpha = m[M_ALPHA * N + n];
-    T eps = alpha * (d[L_DX_VX] + d[L_DZ_VZ]);
+    T eps;
+    if (VTI) eps = alpha * d[L_DX_VX] + m[M_ALPHA3 * N + n] * d[L_DZ_VZ];
+    else eps = alpha * (d[L_DX_VX] + d[L_DZ_VZ]);
     eps += d[L_DX_QX];
     eps += d[L_DZ_QZ];
     const T e = a.x[W_E * N + n];
@@ -97,8 +102,13 @@ __global__ __launch_bounds__(256) void pve_stage(const PveStage<T> a)
     const T eps2 = m[M_M * N + n] * eps + e;
     rk(a, W_P * N + n, -eps2);
     const T E = m[M_EM * N + n], Emu = m[M_EMU * N + n];
-    rk(a, W_TXX * N + n, E * d[L_DX_VX] + Emu * d[L_DZ_VZ] + alpha * eps2);
-    rk(a, W_TZZ * N + n, Emu * d[L_DX_VX] + E * d[L_DZ_VZ] + alpha * eps2);
+    if (VTI) {   // c11, c13, c33, alpha1, alpha3 (propagate.c:1864-1880)
+        rk(a, W_TXX * N + n, E * d[L_DX_VX] + Emu * d[L_DZ_VZ] + alpha * eps2);
+        rk(a, W_TZZ * N + n, Emu * d[L_DX_VX] + m[M_C33 * N + n] * d[L_DZ_VZ] + m[M_ALPHA3 * N + n] * eps2);
+    } else {
+        rk(a, W_TXX * N + n, E * d[L_DX_VX] + Emu * d[L_DZ_VZ] + alpha * eps2);
+        rk(a, W_TZZ * N + n, Emu * d[L_DX_VX] + E * d[L_DZ_VZ] + alpha * eps2);
+    }
     // tau_xz (1259-1311)
     rk(a, W_TXZ * N + n, m[M_MUIJ * N + n] * (d[L_DZ_VX] + d[L_DX_VZ]));
 }
@@ -173,9 +183,10 @@ __global__ void pve_gather(const T *w, const T *dd, size_t N, int nrec, const in
 }
 
 // Kinetic energy of a region, summed in FP64 in a fixed order (one partial
-// sum per block, then one block over the partials): 0.5 rho11 |v|^2 +
-// rho12 v.q + 0.5 rho22 |q|^2 per cell, coef = rho11, rho12, rho22 of the
-// region's ni x nj cells (i outer) one array after the other.
+// sum per block, then one block over the partials): per cell 0.5 a_x v_x^2
+// + b_x v_x q_x + 0.5 c_x q_x^2 and the same along z, coef = a_x, b_x, c_x,
+// a_z, b_z, c_z of the region's ni x nj cells (i outer) one array after the
+// other.
 constexpr unsigned E_BLOCKS = 256;
 
 __device__ inline double block_sum(double a, double *red)
@@ -200,8 +211,10 @@ __global__ __launch_bounds__(256) void pve_energy_part(const T *w, size_t N, con
     for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
         const size_t n = (size_t)(c / nj + i1) * nz2 + c % nj + j1;
         const double vx = w[W_VX * N + n], vz = w[W_VZ * N + n], qx = w[W_QX * N + n], qz = w[W_QZ * N + n];
-        acc += 0.5 * coef[c] * (vx * vx + vz * vz) + coef[ncell + c] * (vx * qx + vz * qz) +
-               0.5 * coef[2 * (size_t)ncell + c] * (qx * qx + qz * qz);
+        const double *k = coef + c;
+        const size_t nc = ncell;
+        acc += 0.5 * k[0] * vx * vx + k[nc] * vx * qx + 0.5 * k[2 * nc] * qx * qx + 0.5 * k[3 * nc] * vz * vz +
+               k[4 * nc] * vz * qz + 0.5 * k[5 * nc] * qz * qz;
     }
     const double t = block_sum(acc, red);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -226,7 +239,8 @@ struct sm_pve {
     sm_pve_params p{};
     pstd::Grid g;
     size_t N = 0, elt = 8;
-    sm_dbuf state, der, mat;        // 3 x 9 N; D_NSLOT N; M_NMAT N
+    sm_dbuf state, der, mat;        // 3 x 9 N; D_NSLOT N; M_NMAT (M_NMAT_VTI) N
+    bool vti = false;
     int iw = 0, iws = 1, iwt = 2;
     sm_dbuf mem;                    // 3 copies x (6 dx_* of (2np+1) nz2, 6 dz_* of (2np+1) nx2)
     size_t mem_dx = 0, mem_dz = 0;
@@ -302,10 +316,13 @@ int stage(sm_pve *s, int tabs, bool last, const T *x, T a, T b, int set_t, int a
     k.a = a; k.b = b; k.set_t = set_t; k.acc = acc;
     k.nx2 = s->g.nx2; k.nz2 = s->g.nz2; k.np = np; k.N = (unsigned)N;
     // m2 <- m1 over (2np+1) rows after every propagation (propagate.c:
-    // 1313-1328); m0 <- m1 at the start of a step (pve_iso_pml.c:1270-1285,
-    // whose (2np+1)-row copy equals the 2np rows the layers use)
+    // 1313-1328 and 1940-1957); m0 <- m1 at the start of a step, over 2np+1
+    // rows (pve_iso_pml.c:1270-1285) or 2np rows (pve_vti_pml.c:1226-1241:
+    // the far layers' innermost row of m0 then stays 0), folded into the
+    // last stage
     const size_t r2x = (size_t)(2 * np + 1) * s->g.nz2, r2z = (size_t)(2 * np + 1) * s->g.nx2;
-    const size_t r0x = last ? r2x : 0, r0z = last ? r2z : 0;
+    const size_t r0rows = s->vti ? (size_t)(2 * np) : (size_t)(2 * np + 1);
+    const size_t r0x = last ? r0rows * s->g.nz2 : 0, r0z = last ? r0rows * s->g.nx2 : 0;
     using namespace pstd;
     auto L3 = [&](int set, int mode, int lastn, int ax, int r1, int k1, int r2, int k2) {
         k.Ls[set] = layer<T>(s->g, tabs, MEM<T>(s, 0, set), MEM<T>(s, 1, set), MEM<T>(s, 2, set), mode, lastn, ax,
@@ -323,7 +340,8 @@ int stage(sm_pve *s, int tabs, bool last, const T *x, T a, T b, int set_t, int a
     L3(L_DZ_QZ, 1, np - 1, 1, T_B, K_K, T_B, K_K);
     L3(L_DZ_VX, 1, np, 1, T_BH, K_KH, T_BHE, K_KHE);
     L3(L_DX_VZ, 0, np, 0, T_BH, K_KH, T_BHE, K_KHE);
-    pve_stage<T><<<pstd::blocks(N), 256, 0, st>>>(k);
+    if (s->vti) pve_stage<T, true><<<pstd::blocks(N), 256, 0, st>>>(k);
+    else pve_stage<T, false><<<pstd::blocks(N), 256, 0, st>>>(k);
     return SM_OK;
 }
 
@@ -421,38 +439,58 @@ int read_back(sm_pve *s, const void *dev, size_t n, double *out)
     return SM_OK;
 }
 
-// the material of pve_iso_pml.c:795-1145 in M_* order
-bool material(size_t nx2, size_t nz2, double dt, const double *K_m_, const double *K_s_, const double *K_f_,
-              const double *phi, const double *mu_, const double *rho_s_, const double *rho_f_, const double *T,
-              const double *eta_, const double *kappa, const double *Q, const double *fq, std::vector<double> &all)
+// model inputs of either medium (the VTI ones null for the isotropic one)
+struct PveModel {
+    const double *K_m, *K_s, *K_f, *phi, *mu, *rho_s, *rho_f, *T1, *T3, *eta, *kappa1, *kappa3, *Q, *fq, *eps,
+        *delta;
+};
+
+// the material of pve_iso_pml.c:795-1145 or pve_vti_pml.c:787-1145 in M_*
+// order
+bool material(size_t nx2, size_t nz2, double dt, bool vti, const PveModel &in, std::vector<double> &all)
 {
     const size_t N = nx2 * nz2;
     const double pi = 4.0 * std::atan(1.0);
-    std::vector<std::vector<double>> m(M_NMAT, std::vector<double>(N, 0.0));
-    std::vector<double> mu(N), rho_f(N), eta(N), rho(N), mm(N), tau_s(N), vphi(N);
+    std::vector<std::vector<double>> m(vti ? M_NMAT_VTI : M_NMAT, std::vector<double>(N, 0.0));
+    std::vector<double> mu(N), rho_f(N), rho(N), m1(N), m3(N), nk1(N), nk3(N), tau_s(N), vphi(N);
     for (size_t n = 0; n < N; ++n) {
-        const double K_m = K_m_[n] * 1000., K_s = K_s_[n] * 1000., K_f = K_f_[n] * 1000.;
-        mu[n] = mu_[n] * 1000.;
-        eta[n] = eta_[n] * 1.013249965828145e6;
-        const double rho_s = rho_s_[n] * 1.e-6;
-        rho_f[n] = rho_f_[n] * 1.e-6;
-        const double tau0 = 1. / (2. * pi * fq[n]);
-        const double E = K_m + 4. / 3. * mu[n];
-        const double D = K_s * (1. + phi[n] * (K_s / K_f - 1.));
-        const double M = K_s * K_s / (D - K_m);
-        m[M_EM][n] = E;
-        m[M_M][n] = M;
-        m[M_ALPHA][n] = 1. - K_m / K_s;
-        mm[n] = T[n] * rho_f[n] / phi[n];
-        if (Q[n] < 1.e4) {
-            const double tau_e = tau0 / Q[n] * (std::sqrt(Q[n] * Q[n] + 1.) + 1.);
-            tau_s[n] = tau0 / Q[n] * (std::sqrt(Q[n] * Q[n] + 1.) - 1.);
+        const double K_m = in.K_m[n] * 1000., K_s = in.K_s[n] * 1000., K_f = in.K_f[n] * 1000.;
+        mu[n] = in.mu[n] * 1000.;
+        const double eta = in.eta[n] * 1.013249965828145e6;
+        const double rho_s = in.rho_s[n] * 1.e-6;
+        rho_f[n] = in.rho_f[n] * 1.e-6;
+        const double phi = in.phi[n];
+        const double tau0 = 1. / (2. * pi * in.fq[n]);
+        const double D = K_s * (1. + phi * (K_s / K_f - 1.));
+        if (vti) {   // eq. 9-15 of Carcione 1996, Thomsen's c11 and c13
+            const double c33 = K_m + 4. / 3. * mu[n], c55 = mu[n];
+            const double c11 = 2. * in.eps[n] * c33 + c33, c12 = c11 - 2. * c55;
+            const double c13 = std::sqrt(2. * in.delta[n] * c33 * (c33 - c55) + (c33 - c55) * (c33 - c55)) - c55;
+            m[M_EM][n] = c11;
+            m[M_EMU][n] = c13;
+            m[M_C33][n] = c33;
+            m[M_M][n] = K_s * K_s / (D - (2. * c11 + c33 + 2. * c12 + 4. * c13) / 9.);
+            m[M_ALPHA][n] = 1. - 1. / 3. * (c11 + c12 + c13) / K_s;
+            m[M_ALPHA3][n] = 1. - 1. / 3. * (2. * c13 + c33) / K_s;
+        } else {
+            m[M_EM][n] = K_m + 4. / 3. * mu[n];
+            m[M_M][n] = K_s * K_s / (D - K_m);
+            m[M_ALPHA][n] = 1. - K_m / K_s;
+        }
+        m1[n] = in.T1[n] * rho_f[n] / phi;
+        m3[n] = in.T3[n] * rho_f[n] / phi;
+        nk1[n] = eta / in.kappa1[n];
+        nk3[n] = eta / in.kappa3[n];
+        const double Q = in.Q[n];
+        if (Q < 1.e4) {
+            const double tau_e = tau0 / Q * (std::sqrt(Q * Q + 1.) + 1.);
+            tau_s[n] = tau0 / Q * (std::sqrt(Q * Q + 1.) - 1.);
             vphi[n] = tau_e / tau_s[n] - 1.;
         } else {
             tau_s[n] = tau0;
             vphi[n] = 0.0;
         }
-        rho[n] = (1. - phi[n]) * rho_s + phi[n] * rho_f[n];
+        rho[n] = (1. - phi) * rho_s + phi * rho_f[n];
     }
     auto coeffs = [&](double r, double rf, double mi, double nk, double &cq, double &cv) {
         const double tmp = 1. / (rf * rf - r * mi);
@@ -467,8 +505,8 @@ bool material(size_t nx2, size_t nz2, double dt, const double *K_m_, const doubl
             const size_t a = i * nz2 + j, b = (i + 1) * nz2 + j;
             rho_i[a] = 0.5 * (rho[a] + rho[b]);
             rho_f_i[a] = 0.5 * (rho_f[a] + rho_f[b]);
-            const double nk = 0.5 * (eta[a] / kappa[a] + eta[b] / kappa[b]);
-            m_i[a] = 0.5 * (mm[a] + mm[b]);
+            const double nk = 0.5 * (nk1[a] + nk1[b]);
+            m_i[a] = 0.5 * (m1[a] + m1[b]);
             coeffs(rho_i[a], rho_f_i[a], m_i[a], nk, m[M_CQI][a], m[M_CVI][a]);
         }
     for (size_t i = 0; i < nx2; ++i)
@@ -476,8 +514,8 @@ bool material(size_t nx2, size_t nz2, double dt, const double *K_m_, const doubl
             const size_t a = i * nz2 + j, b = a + 1;
             rho_j[a] = 0.5 * (rho[a] + rho[b]);
             rho_f_j[a] = 0.5 * (rho_f[a] + rho_f[b]);
-            const double nk = 0.5 * (eta[a] / kappa[a] + eta[b] / kappa[b]);
-            m_j[a] = 0.5 * (mm[a] + mm[b]);
+            const double nk = 0.5 * (nk3[a] + nk3[b]);
+            m_j[a] = 0.5 * (m3[a] + m3[b]);
             coeffs(rho_j[a], rho_f_j[a], m_j[a], nk, m[M_CQJ][a], m[M_CVJ][a]);
         }
     std::vector<double> &muij = m[M_MUIJ];
@@ -486,18 +524,18 @@ bool material(size_t nx2, size_t nz2, double dt, const double *K_m_, const doubl
             const size_t a = i * nz2 + j, b = (i + 1) * nz2 + j, c = a + 1, d = b + 1;
             muij[a] = 4.0 / (1. / mu[a] + 1. / mu[b] + 1. / mu[c] + 1. / mu[d]);
         }
-    // last row, then last column (1076-1105)
+    // last row, then last column (pve_iso_pml.c:1076-1105)
     for (size_t j = 0; j < nz2; ++j)
         for (std::vector<double> *v : {&rho_i, &rho_f_i, &m_i, &m[M_CQI], &m[M_CVI], &muij})
             (*v)[(nx2 - 1) * nz2 + j] = (*v)[(nx2 - 2) * nz2 + j];
     for (size_t i = 0; i < nx2; ++i)
         for (std::vector<double> *v : {&rho_j, &rho_f_j, &m_j, &m[M_CQJ], &m[M_CVJ], &muij})
             (*v)[i * nz2 + nz2 - 1] = (*v)[i * nz2 + nz2 - 2];
-    // stored forms (1131-1145)
+    // stored forms (pve_iso_pml.c:1131-1145)
     for (size_t n = 0; n < N; ++n) {
         m[M_TS][n] = 1. / tau_s[n];
         m[M_VPHI][n] = m[M_M][n] * vphi[n] / (1. + vphi[n]);
-        m[M_EMU][n] = m[M_EM][n] - 2. * mu[n];
+        if (!vti) m[M_EMU][n] = m[M_EM][n] - 2. * mu[n];
         m[M_MI][n] = rho_f_i[n] * rho_f_i[n] / rho_i[n] - m_i[n];
         m[M_MJ][n] = rho_f_j[n] * rho_f_j[n] / rho_j[n] - m_j[n];
         m[M_RFI][n] = rho_f_i[n] / rho_i[n];
@@ -506,57 +544,44 @@ bool material(size_t nx2, size_t nz2, double dt, const double *K_m_, const doubl
         m[M_IRHOJ][n] = 1. / rho_j[n];
     }
     all.clear();
-    all.reserve(M_NMAT * N);
+    all.reserve(m.size() * N);
     for (auto &a : m) all.insert(all.end(), a.begin(), a.end());
     for (double v : all)
         if (!std::isfinite(v)) return false;
     return true;
 }
 
-}  // namespace
-
-extern "C" {
-
-size_t sm_pve_bytes_needed(const sm_pve_params *p, size_t nrec, size_t max_steps)
+int create(sm_ctx *ctx, const sm_pve_params *pp, bool vti, const PveModel &in, sm_pve **out)
 {
-    if (!p || (p->storage != SM_STORE_F64 && p->storage != SM_STORE_F32)) return 0;
-    const size_t e = p->storage == SM_STORE_F64 ? 8 : 4;
-    const size_t np = p->abs ? p->np : 0;
-    const size_t nx2 = p->nx + 2 * np, nz2 = p->nz + 2 * np, N = nx2 * nz2;
-    const size_t mem = 3 * 6 * (2 * np + 1) * (nx2 + nz2);
-    return e * (3 * W_NF * N + D_NSLOT * N + M_NMAT * N + mem) + pstd::grid_bytes(nx2, nz2, np, e) +
-           e * nrec * max_steps + 12 * nrec;
-}
-
-int sm_pve_create(sm_ctx *ctx, const sm_pve_params *pp, const double *K_m, const double *K_s, const double *K_f,
-                  const double *phi, const double *mu, const double *rho_s, const double *rho_f, const double *T,
-                  const double *eta, const double *kappa, const double *Q, const double *fq, sm_pve **out)
-{
-    SM_CHECK(ctx && pp && K_m && K_s && K_f && phi && mu && rho_s && rho_f && T && eta && kappa && Q && fq && out,
-             SM_ERR_ARG, "sm_pve_create: NULL argument");
+    const char *fn = vti ? "sm_pve_create_vti" : "sm_pve_create";
+    SM_CHECK(ctx && pp && out, SM_ERR_ARG, "%s: NULL argument", fn);
+    SM_CHECK(in.K_m && in.K_s && in.K_f && in.phi && in.mu && in.rho_s && in.rho_f && in.T1 && in.T3 && in.eta &&
+                 in.kappa1 && in.kappa3 && in.Q && in.fq && (!vti || (in.eps && in.delta)),
+             SM_ERR_ARG, "%s: NULL model array", fn);
     sm_pve_params p = *pp;
     SM_CHECK(p.storage == SM_STORE_F64 || p.storage == SM_STORE_F32, SM_ERR_ARG,
-             "sm_pve_create: storage must be SM_STORE_F64 or SM_STORE_F32");
-    SM_CHECK(p.nx >= 2 && p.nz >= 2, SM_ERR_ARG, "sm_pve_create: the model needs at least 2 x 2 nodes");
-    SM_CHECK(p.dx > 0 && p.dz > 0 && p.dt > 0, SM_ERR_ARG, "sm_pve_create: spacings and dt must be > 0");
-    SM_CHECK((p.flags & ~SM_PSTD_HIPFFT) == 0, SM_ERR_ARG, "sm_pve_create: unknown flags %#x", p.flags);
+             "%s: storage must be SM_STORE_F64 or SM_STORE_F32", fn);
+    SM_CHECK(p.nx >= 2 && p.nz >= 2, SM_ERR_ARG, "%s: the model needs at least 2 x 2 nodes", fn);
+    SM_CHECK(p.dx > 0 && p.dz > 0 && p.dt > 0, SM_ERR_ARG, "%s: spacings and dt must be > 0", fn);
+    SM_CHECK((p.flags & ~SM_PSTD_HIPFFT) == 0, SM_ERR_ARG, "%s: unknown flags %#x", fn, p.flags);
     if (!p.abs) p.np = 0;   // io_utils.c:293: no absorbing layers without abs
     const size_t nx2 = p.nx + 2 * p.np, nz2 = p.nz + 2 * p.np, N = nx2 * nz2;
-    SM_CHECK(nx2 < (1u << 20) && nz2 < (1u << 20) && N * W_NF < ((size_t)1 << 31), SM_ERR_ARG,
-             "sm_pve_create: grid too large");
+    SM_CHECK(nx2 < (1u << 20) && nz2 < (1u << 20) && N * W_NF < ((size_t)1 << 31), SM_ERR_ARG, "%s: grid too large",
+             fn);
     for (size_t n = 0; n < N; ++n)
-        SM_CHECK(K_m[n] >= 0 && K_s[n] > 0 && K_f[n] > 0 && phi[n] > 0 && mu[n] >= 0 && rho_s[n] > 0 &&
-                     rho_f[n] > 0 && T[n] > 0 && eta[n] >= 0 && kappa[n] > 0 && Q[n] > 0 && fq[n] > 0,
-                 SM_ERR_ARG, "sm_pve_create: cell %zu has a property out of range", n);
+        SM_CHECK(in.K_m[n] >= 0 && in.K_s[n] > 0 && in.K_f[n] > 0 && in.phi[n] > 0 && in.mu[n] >= 0 &&
+                     in.rho_s[n] > 0 && in.rho_f[n] > 0 && in.T1[n] > 0 && in.T3[n] > 0 && in.eta[n] >= 0 &&
+                     in.kappa1[n] > 0 && in.kappa3[n] > 0 && in.Q[n] > 0 && in.fq[n] > 0,
+                 SM_ERR_ARG, "%s: cell %zu has a property out of range", fn, n);
     std::vector<double> mat;
-    SM_CHECK(material(nx2, nz2, p.dt, K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, eta, kappa, Q, fq, mat), SM_ERR_ARG,
-             "sm_pve_create: material not finite");
+    SM_CHECK(material(nx2, nz2, p.dt, vti, in, mat), SM_ERR_ARG, "%s: material not finite", fn);
     SM_CHECK(set_device(ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
 
     sm_pve *s = new sm_pve;
     s->ctx = ctx;
     s->p = p;
     s->N = N;
+    s->vti = vti;
     s->elt = p.storage == SM_STORE_F64 ? 8 : 4;
     const size_t e = s->elt;
     s->mem_dx = (2 * p.np + 1) * nz2;
@@ -575,6 +600,38 @@ int sm_pve_create(sm_ctx *ctx, const sm_pve_params *pp, const double *K_m, const
     if (rc != SM_OK) { release(s); delete s; return rc; }
     *out = s;
     return SM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t sm_pve_bytes_needed(const sm_pve_params *p, size_t nrec, size_t max_steps)
+{
+    if (!p || (p->storage != SM_STORE_F64 && p->storage != SM_STORE_F32)) return 0;
+    const size_t e = p->storage == SM_STORE_F64 ? 8 : 4;
+    const size_t np = p->abs ? p->np : 0;
+    const size_t nx2 = p->nx + 2 * np, nz2 = p->nz + 2 * np, N = nx2 * nz2;
+    const size_t mem = 3 * 6 * (2 * np + 1) * (nx2 + nz2);
+    return e * (3 * W_NF * N + D_NSLOT * N + M_NMAT_VTI * N + mem) + pstd::grid_bytes(nx2, nz2, np, e) +
+           e * nrec * max_steps + 12 * nrec;
+}
+
+int sm_pve_create(sm_ctx *ctx, const sm_pve_params *pp, const double *K_m, const double *K_s, const double *K_f,
+                  const double *phi, const double *mu, const double *rho_s, const double *rho_f, const double *T,
+                  const double *eta, const double *kappa, const double *Q, const double *fq, sm_pve **out)
+{
+    const PveModel in{K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, T, eta, kappa, kappa, Q, fq, nullptr, nullptr};
+    return create(ctx, pp, false, in, out);
+}
+
+int sm_pve_create_vti(sm_ctx *ctx, const sm_pve_params *pp, const double *K_m, const double *K_s, const double *K_f,
+                      const double *phi, const double *mu, const double *rho_s, const double *rho_f, const double *T1,
+                      const double *T3, const double *eta, const double *kappa1, const double *kappa3, const double *Q,
+                      const double *fq, const double *epsilon, const double *delta, sm_pve **out)
+{
+    const PveModel in{K_m, K_s, K_f, phi, mu, rho_s, rho_f, T1, T3, eta, kappa1, kappa3, Q, fq, epsilon, delta};
+    return create(ctx, pp, true, in, out);
 }
 
 int sm_pve_destroy(sm_pve *s)
@@ -654,23 +711,23 @@ int sm_pve_set_receivers(sm_pve *s, size_t nrec, const size_t *idx, const int *c
     return SM_OK;
 }
 
-int sm_pve_set_energy(sm_pve *s, size_t i1, size_t i2, size_t j1, size_t j2, const double *rho11,
-                      const double *rho12, const double *rho22, size_t max_steps)
+int sm_pve_set_energy_axes(sm_pve *s, size_t i1, size_t i2, size_t j1, size_t j2, const double *ax,
+                           const double *bx, const double *cx, const double *az, const double *bz, const double *cz,
+                           size_t max_steps)
 {
     SM_CHECK(s, SM_ERR_ARG, "sm_pve_set_energy: NULL solver");
     SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
     for (sm_dbuf *b : {&s->ecoef, &s->epart, &s->etrace}) dfree(*b);
     s->e_max = 0;
     if (max_steps == 0) return SM_OK;
-    SM_CHECK(rho11 && rho12 && rho22, SM_ERR_ARG, "sm_pve_set_energy: NULL array");
+    SM_CHECK(ax && bx && cx && az && bz && cz, SM_ERR_ARG, "sm_pve_set_energy: NULL array");
     SM_CHECK(i1 <= i2 && i2 < (size_t)s->g.nx2 && j1 <= j2 && j2 < (size_t)s->g.nz2, SM_ERR_ARG,
              "sm_pve_set_energy: region [%zu, %zu] x [%zu, %zu] outside the %d x %d grid", i1, i2, j1, j2,
              s->g.nx2, s->g.nz2);
     const size_t ni = i2 - i1 + 1, nj = j2 - j1 + 1, nc = ni * nj;
-    std::vector<double> h(3 * nc);
-    std::memcpy(h.data(), rho11, nc * sizeof(double));
-    std::memcpy(h.data() + nc, rho12, nc * sizeof(double));
-    std::memcpy(h.data() + 2 * nc, rho22, nc * sizeof(double));
+    std::vector<double> h(6 * nc);
+    const double *src[6] = {ax, bx, cx, az, bz, cz};
+    for (int q = 0; q < 6; ++q) std::memcpy(h.data() + q * nc, src[q], nc * sizeof(double));
     int rc = dalloc(s->ecoef, h.size() * sizeof(double));
     if (rc == SM_OK) rc = dalloc(s->epart, E_BLOCKS * sizeof(double));
     if (rc == SM_OK) rc = dalloc(s->etrace, max_steps * sizeof(double));
@@ -681,6 +738,12 @@ int sm_pve_set_energy(sm_pve *s, size_t i1, size_t i2, size_t j1, size_t j2, con
     s->ei1 = i1; s->ej1 = j1; s->eni = ni; s->enj = nj;
     s->e_max = max_steps; s->e_step0 = s->step;
     return SM_OK;
+}
+
+int sm_pve_set_energy(sm_pve *s, size_t i1, size_t i2, size_t j1, size_t j2, const double *rho11,
+                      const double *rho12, const double *rho22, size_t max_steps)
+{
+    return sm_pve_set_energy_axes(s, i1, i2, j1, j2, rho11, rho12, rho22, rho11, rho12, rho22, max_steps);
 }
 
 int sm_pve_get_energy(sm_pve *s, size_t it0, size_t count, double *out)

@@ -1,334 +1,10 @@
 /*
- * pve_iso_pml -- poroviscoelastic wave propagation in 2D isotropic media
- * (Biot solid and fluid motion after Carcione and Helle 1999, one
- * relaxation mechanism for the fluid pressure, pseudo-spectral derivatives,
- * CPML, fourth-order Runge-Kutta with the viscous coupling stepped
- * analytically), running the time loop on an MI355X through libseismod_hip.
- *
- * Drop-in for seismod's pve_iso_pml: same command line, .par keywords,
- * model ("isotropic" or "isotropic_pml", "K_m K_s K_f phi mu rho_s rho_f T
- * eta kappa Q f0" per node), source (Sx, Sz, Sxz, Bulk, Sf, Bulk_s; 1- or
- * 9-point template) and record (Vx, Vz, qx, qz, txx, tzz, txz, p, div,
- * curl) files, and the same outputs: trc/, ssh/, segy/, the kinetic energy
- * of a region in E/<base>_E.dat (saveEnergy, EnergyROI) and the phase
- * velocity and attenuation check (check_model: 1 prints the extremes, 2
- * also writes <base>_vel_slowP.nc ... <base>_att_S.nc), pve_iso_pml.c:
- * 441-1440.  Checkpoints (chkpt_inc, -c) and the median smoothing of the
- * layers (pml_median) are not available: the program says so and stops.
- *
- * Set-up and update arithmetic live in the library (sm_pve_*); this program
- * reads the inputs, drives the time loop and writes the records.
- *
- * Environment: SEISMOD_STORAGE=f32 runs the FP32 kernels; the default is
- * FP64 (the reference's precision).  SEISMOD_DEVICE selects the GPU.
+ * pve_iso_pml -- drop-in for seismod's pve_iso_pml: poroviscoelastic wave
+ * propagation in 2D isotropic media on an MI355X (pve_driver.c).
  */
-#include <math.h>
-#include <stdio.h>
-#include <stdlib.h>
-#include <string.h>
-#include <sys/stat.h>
-
-#include "seismod_hip.h"
-#include "sm_io.h"
-
-static const char *usage_text =
-    "Format of parameter file:\n\n"
-    "One line per parameter, first word is the value of the parameter,\n"
-    "next is a keyword comprise between a # and a comma, and a comment\n"
-    "can be added after the comma, i.e we have on each line\n\n"
-    "value     # keyword, optional comment\n\n"
-    "Keywords: model, source, output, basename, time (s), dt (ms), segy, shotpoint,\n"
-    "simulateCMP, abs, nl, plstr, kappa_max, alpha_max, Rc, check_model (0, 1 display,\n"
-    "2 store), saveEnergy, EnergyROI (xmin/xmax/zmin/zmax)\n\n"
-    "Model file: nx nz / dx dz / x0 z0 / isotropic (or isotropic_pml: the absorbing\n"
-    "layers included) / then one line per node, x outer and z inner:\n"
-    "K_m[GPa] K_s[GPa] K_f[GPa] phi mu[GPa] rho_s[kg/m3] rho_f[kg/m3] T eta[cP]\n"
-    "kappa[mD] Q f0[Hz]  (one column only: laterally uniform)\n\n"
-    "Source file: nsrc / template points (1 or 9) / then per source: type\n"
-    "(Sx, Sz, Sxz, Bulk, Sf, Bulk_s), strength, frequency [Hz], x z [m]\n\n"
-    "Record file: nrec / then per record: component (Vx, Vz, qx, qz, txx, tzz, txz,\n"
-    "p, div, curl), Trace x z t0[s] dt[ms] or Snapshot t0[s] dt[ms]\n";
-
-static void check(int st, const char *where)
-{
-    if (st != SM_OK) {
-        fprintf(stderr, "Error in %s: %s (%s)\n", where, sm_strerror(st), sm_last_error());
-        exit(1);
-    }
-}
-
-static int recorded(int c)
-{
-    return c == SM_VX || c == SM_VZ || c == SM_QX || c == SM_QZ || c == SM_TXX || c == SM_TZZ || c == SM_TXZ ||
-           c == SM_P || c == SM_DIV || c == SM_CURL;
-}
-
-enum { KM, KS, KF, PHI, MU, RHOS, RHOF, TORT, ETA, KAPPA, QQ, FQ };   /* the 12 model values */
-
-/* Extremes of the phase velocities and attenuation of the slow and fast P
- * waves and the S wave at 1 Hz, the source frequency and 1/dt, printed when
- * verbose; check_model 2 also writes the values at the source frequency
- * (pve_iso_pml.c:846-1006 and 1164-1185). */
-static void check_model(const sm_params *p, const sm_grid *g, size_t np, double *const v[12], double f0)
-{
-    const size_t n = (g->nx + 2 * np) * (g->nz + 2 * np);
-    double ext[6][6];   /* slow P, fast P, S velocity, then attenuation: max at 3 frequencies, then min */
-    for (int k = 0; k < 6; ++k)
-        for (int q = 0; q < 6; ++q) ext[k][q] = q < 3 ? 0.0 : 1.e9;
-    double *w[6];
-    for (int k = 0; k < 6; ++k) w[k] = malloc(n * sizeof(double));
-    const double freq[3] = {1.0, f0, 1. / p->dt};
-    /* the source frequency last, so that w holds its values for the files */
-    for (int q = 0; q < 3; ++q) {
-        const int fq = q == 2 ? 1 : q == 1 ? 2 : 0;
-        sm_pve_dispersion(n, v, freq[fq], w, w + 3);
-        for (int k = 0; k < 6; ++k)
-            for (size_t c = 0; c < n; ++c) {
-                if (w[k][c] > ext[k][fq]) ext[k][fq] = w[k][c];
-                if (w[k][c] < ext[k][fq + 3]) ext[k][fq + 3] = w[k][c];
-            }
-    }
-    if (p->check_model > 1) {
-        if (sm_verbose > 1) {
-            fprintf(stdout, "  Saving phase velocity and attenuation @ src frequency ...");
-            fflush(stdout);
-        }
-        const char *names[6] = {"vel_slowP", "vel_fastP", "vel_S", "att_slowP", "att_fastP", "att_S"};
-        for (int k = 0; k < 6; ++k)
-            sm_write_field_nc(w[k], names[k], k < 3 ? "m/s" : "1/m", g->nx, g->nz, np, g->x0, g->z0, g->dx,
-                              g->dz, p->basename, 1);
-    }
-    for (int k = 0; k < 6; ++k) free(w[k]);
-    if (sm_verbose) {
-        const char *label[3] = {"   Low frequency (1 Hz)    - ", "   Src frequency (%lg Hz)   - ",
-                                "  High frequency (%lg Hz) - "};
-        const char *wave[3] = {"slow P", "fast P", "     S"};
-        for (int a = 0; a < 2; ++a) {
-            fprintf(stdout, a == 0 ? "Max/min phase velocities:\n" : "Max/min attenuation coefficient:\n");
-            for (int q = 0; q < 3; ++q)
-                for (int k = 0; k < 3; ++k) {
-                    if (k == 0) fprintf(stdout, label[q], freq[q]);
-                    else fprintf(stdout, "                             ");
-                    fprintf(stdout, "%s : %lf    %lf\n", wave[k], ext[3 * a + k][q], ext[3 * a + k][q + 3]);
-                }
-        }
-        fflush(stdout);
-    }
-}
+#include "pve_driver.h"
 
 int main(int argc, char *argv[])
 {
-    sm_params p;
-    sm_set_defaults(&p, 0);
-    sm_process_args(argc, argv, &p, usage_text);
-    if (p.chkpt_inc > 0) {
-        fprintf(stderr, "Error: checkpointing (chkpt_inc %d) is not available in this pve_iso_pml\n", p.chkpt_inc);
-        return 1;
-    }
-    if (p.pml_median) {
-        fprintf(stderr, "Error: median smoothing of the absorbing layers (pml_median) is not available in "
-                        "this pve_iso_pml\n");
-        return 1;
-    }
-    const size_t np = p.np;   /* 0 when abs is 0 */
-    const double dt = p.dt;
-    if (sm_verbose) {
-        fprintf(stdout, "\n *** pve_iso - PoroViscoElastic wave propagation in 2D isotropic media ***\n\n");
-        if (sm_verbose > 1) {
-            fprintf(stdout, "  Model file: \t%s\n", p.modelfile);
-            fprintf(stdout, "  Source file:\t%s\n", p.sourcefile);
-            fprintf(stdout, "  Output file:\t%s\n", p.outputfile);
-            fprintf(stdout, "  Time window:\t%lg (s)\n", p.duration);
-            fprintf(stdout, "  Time step:  \t%lg (ms)\n", p.dt * 1.e3);
-        }
-        fprintf(stdout, "Reading size of model ... ");
-    }
-    sm_grid g;
-    int with_pml = 0;
-    sm_read_grid_pve(p.modelfile, np, &g, &with_pml);
-    const size_t nx2 = g.nx + 2 * np, nz2 = g.nz + 2 * np, n = nx2 * nz2;
-    double *v[12];
-    for (int q = 0; q < 12; ++q) v[q] = calloc(n, sizeof(double));
-    if (sm_verbose) { fprintf(stdout, "done.\nReading properties of materials ... "); fflush(stdout); }
-    sm_read_model_pve(p.modelfile, &g, np, with_pml, v);
-
-    if (sm_verbose) { fprintf(stdout, "done.\nReading source parameters ... "); fflush(stdout); }
-    sm_sources src;
-    sm_read_source2d_np(p.sourcefile, &g, np, &src);
-    const size_t ns = src.nsrc * src.ntemplate;
-    const double f0 = ns ? src.s[0].f : 0.0;
-    /* compute_src_fct(&src, dt, 0.5) then normalize_src_fct (src.c:43-75):
-     * the functions at half steps, divided by the cell area */
-    double *A = malloc((ns ? ns : 1) * sizeof(double)), *f = malloc((ns ? ns : 1) * sizeof(double));
-    size_t *len = malloc((ns ? ns : 1) * sizeof(size_t)), *sidx = malloc((ns ? ns : 1) * sizeof(size_t));
-    int *stype = malloc((ns ? ns : 1) * sizeof(int));
-    size_t total = 0;
-    for (size_t k = 0; k < ns; ++k) {
-        A[k] = src.s[k].A; f[k] = src.s[k].f;
-        len[k] = sm_ricker_length(f[k], 0.5 * dt);
-        sidx[k] = src.s[k].idx;
-        stype[k] = src.s[k].type;
-        total += len[k];
-    }
-    double *fct = malloc((total ? total : 1) * sizeof(double));
-    check(sm_compute_src_fct(ns, A, f, 0.5 * dt, len, fct), "sm_compute_src_fct");
-    const double i_area = 1. / (g.dx * g.dz);
-    for (size_t k = 0; k < total; ++k) fct[k] *= i_area;
-
-    if (sm_verbose) { fprintf(stdout, "done.\nReading output parameters ... "); fflush(stdout); }
-    sm_records out;
-    memset(&out, 0, sizeof(out));
-    snprintf(out.basename, SM_NAME_LEN, "%s", p.basename);
-    sm_read_output2d_np(p.outputfile, &g, np, &out);
-    sm_check_dt_trc(&out, dt);
-    if (sm_verbose) fprintf(stdout, "done.\n");
-
-    /* kinetic energy of a region (pve_iso_pml.c:758-783, 1106-1119): the
-     * region must lie in the model (the reference checks it only when
-     * verbose, and reads outside its arrays otherwise) */
-    size_t i1E = 0, i2E = 0, j1E = 0, j2E = 0;
-    FILE *efid = NULL;
-    double *rho11 = NULL, *rho12 = NULL, *rho22 = NULL;
-    if (p.save_energy == 1) {
-        if (p.roi[0] < g.x0 || p.roi[1] > g.x0 + (double)(g.nx - 1) * g.dx || p.roi[2] < g.z0 ||
-            p.roi[3] > g.z0 + (double)(g.nz - 1) * g.dz || p.roi[1] < p.roi[0] || p.roi[3] < p.roi[2]) {
-            fprintf(stderr, "Region of interest for saving energy not fitting in the modeling grid\n");
-            return 1;
-        }
-        i1E = (size_t)lround((p.roi[0] - g.x0) / g.dx);
-        i2E = (size_t)lround((p.roi[1] - g.x0) / g.dx);
-        j1E = (size_t)lround((p.roi[2] - g.z0) / g.dz);
-        j2E = (size_t)lround((p.roi[3] - g.z0) / g.dz);
-        struct stat sb;
-        if (stat("E", &sb) < 0) mkdir("E", S_IRWXU);
-        char fname[SM_NAME_LEN + 16];
-        snprintf(fname, sizeof(fname), "E/%s_E.dat", p.basename);
-        efid = fopen(fname, "w+");
-        if (!efid) {
-            fprintf(stderr, "Error, cannot open %s for writing\n", fname);
-            return 1;
-        }
-        if (sm_verbose) {
-            fprintf(stdout, "Saving kinetic energy as fct of time in %s\n", fname);
-            fprintf(stdout, "  Region of interest is %lg/%lg/%lg/%lg\n", p.roi[0], p.roi[1], p.roi[2], p.roi[3]);
-        }
-        const size_t npts = (i2E - i1E + 1) * (j2E - j1E + 1);
-        rho11 = malloc(npts * sizeof(double)); rho12 = malloc(npts * sizeof(double));
-        rho22 = malloc(npts * sizeof(double));
-        for (size_t i = i1E, k = 0; i <= i2E; ++i)
-            for (size_t j = j1E; j <= j2E; ++j, ++k) {
-                const size_t ij = (i + np) * nz2 + j + np;
-                /* eq. 7.153-7.160 of Carcione (2007), densities through the
-                 * reference's 1e-6 scaling and back */
-                const double rs = v[RHOS][ij] * 1.e-6 * 1.e6, rf = v[RHOF][ij] * 1.e-6 * 1.e6, phi = v[PHI][ij];
-                rho12[k] = -phi * rf * (v[TORT][ij] - 1.);
-                rho11[k] = (1. - phi) * rs - rho12[k];
-                rho22[k] = phi * rf - rho12[k];
-            }
-    }
-
-    if (sm_verbose) { fprintf(stdout, "Computing CPML parameters ... done.\nPreprocessing data ... "); fflush(stdout); }
-    if (p.check_model > 0) check_model(&p, &g, np, v, f0);
-    const char *dev = getenv("SEISMOD_DEVICE");
-    sm_ctx *ctx = NULL;
-    check(sm_ctx_create(dev ? atoi(dev) : 0, &ctx), "sm_ctx_create");
-    const char *stor = getenv("SEISMOD_STORAGE");
-    sm_pve_params vp;
-    memset(&vp, 0, sizeof(vp));
-    vp.nx = g.nx; vp.nz = g.nz; vp.np = np; vp.dx = g.dx; vp.dz = g.dz; vp.dt = dt;
-    vp.vmax = p.vmax; vp.pml_order = p.pml_order; vp.Rc = p.Rc; vp.kappa_max = p.kappa_max;
-    vp.alpha_max = p.alpha_max; vp.f0 = f0; vp.abs = p.iwipe;
-    vp.storage = (stor && !strcmp(stor, "f32")) ? SM_STORE_F32 : SM_STORE_F64;
-    sm_pve *sim = NULL;
-    check(sm_pve_create(ctx, &vp, v[KM], v[KS], v[KF], v[PHI], v[MU], v[RHOS], v[RHOF], v[TORT], v[ETA], v[KAPPA],
-                        v[QQ], v[FQ], &sim),
-          "sm_pve_create");
-    check(sm_pve_set_sources(sim, ns, stype, sidx, len, fct), "sm_pve_set_sources");
-    if (sm_verbose) fprintf(stdout, "done.\n");
-
-    const size_t nsteps = sm_nsteps(p.duration, dt);   /* pve_iso_pml.c:1243-1244 */
-    if (efid)
-        check(sm_pve_set_energy(sim, i1E + np, i2E + np, j1E + np, j2E + np, rho11, rho12, rho22, nsteps),
-              "sm_pve_set_energy");
-    /* records of other components write nothing (pve_iso_pml.c:1330-1389) */
-    size_t ntr = 0;
-    for (size_t r = 0; r < out.nrec; ++r) ntr += out.r[r].type == REC_TRACE && recorded(out.r[r].comp);
-    size_t *ridx = malloc((ntr ? ntr : 1) * sizeof(size_t));
-    int *rcomp = malloc((ntr ? ntr : 1) * sizeof(int));
-    size_t *rmap = malloc((ntr ? ntr : 1) * sizeof(size_t));
-    for (size_t r = 0, q = 0; r < out.nrec; ++r)
-        if (out.r[r].type == REC_TRACE && recorded(out.r[r].comp)) {
-            ridx[q] = out.r[r].idx; rcomp[q] = out.r[r].comp; rmap[q] = r; ++q;
-            if (p.segy == 1) {   /* pve_iso_pml.c:719-726 */
-                out.r[r].nsamples = (size_t)round(p.duration / out.r[r].dt);
-                out.r[r].data = calloc(out.r[r].nsamples + 1, sizeof(float));
-            }
-        }
-    check(sm_pve_set_receivers(sim, ntr, ridx, rcomp, nsteps), "sm_pve_set_receivers");
-
-    if (sm_verbose) {
-        fprintf(stdout, "Starting main loop (%zu iterations)\n", nsteps);
-        fflush(stdout);
-    }
-    double *field = malloc(n * sizeof(double));
-    size_t done = 0;
-    for (size_t it = 0; it < nsteps; ++it) {
-        const double t = dt * (it + 1);
-        if ((it + 1) % 50 == 0 && sm_verbose) {
-            fprintf(stdout, "  Iteration %zu, t = %g s\n", it + 1, t);
-            fflush(stdout);
-        }
-        int snap = 0;
-        for (size_t r = 0; r < out.nrec; ++r)
-            if (out.r[r].type == REC_SNAPSHOT && recorded(out.r[r].comp) &&
-                sm_record_due(it, dt, out.r[r].t0, out.r[r].dt))
-                snap = 1;
-        if (!snap && it + 1 < nsteps) continue;
-        check(sm_pve_run(sim, it + 1 - done), "sm_pve_run");
-        done = it + 1;
-        for (size_t r = 0; r < out.nrec; ++r)
-            if (out.r[r].type == REC_SNAPSHOT && recorded(out.r[r].comp) &&
-                sm_record_due(it, dt, out.r[r].t0, out.r[r].dt)) {
-                if (sm_verbose > 1) printf("Writing snapshot, t = %g s\n", t);
-                check(sm_pve_get_field(sim, out.r[r].comp, field), "sm_pve_get_field");
-                sm_write_snapshot_nc(field, t, g.nx, g.nz, np, g.x0, g.z0, g.dx, g.dz, out.basename, r,
-                                     p.plot_strips);
-            }
-    }
-    if (ntr) {
-        double *tr = malloc(nsteps * ntr * sizeof(double));
-        check(sm_pve_get_traces(sim, 0, nsteps, tr), "sm_pve_get_traces");
-        for (size_t it = 0; it < nsteps; ++it)
-            for (size_t q = 0; q < ntr; ++q) {
-                sm_record *r = &out.r[rmap[q]];
-                const double t = dt * (it + 1);
-                if (!sm_record_due(it, dt, r->t0, r->dt)) continue;
-                if (p.segy == 1) {
-                    /* fill_data_segy (io_utils.c:1770-1774) */
-                    const size_t s = (size_t)round(t / r->dt);
-                    if (s < r->nsamples) r->data[s] = (float)tr[it * ntr + q];
-                } else {
-                    sm_write_trace(&out, rmap[q], t, tr[it * ntr + q]);
-                }
-            }
-        free(tr);
-    }
-    if (efid) {
-        double *E = malloc((nsteps ? nsteps : 1) * sizeof(double));
-        check(sm_pve_get_energy(sim, 0, nsteps, E), "sm_pve_get_energy");
-        for (size_t it = 0; it < nsteps; ++it) fprintf(efid, "%le   %le\n", dt * (it + 1), E[it]);
-        fclose(efid);
-        free(E);
-    }
-    if (p.segy == 1) sm_save_segy(&p, &out, &src);
-    sm_close_traces(&out);
-    check(sm_pve_destroy(sim), "sm_pve_destroy");
-    check(sm_ctx_destroy(ctx), "sm_ctx_destroy");
-    for (int q = 0; q < 12; ++q) free(v[q]);
-    free(A); free(f); free(len); free(sidx); free(stype); free(fct); free(ridx); free(rcomp); free(rmap);
-    free(field); free(rho11); free(rho12); free(rho22);
-    sm_free_sources(&src);
-    sm_free_records(&out);
-    if (sm_verbose) fprintf(stdout, "\nEnd of computation\n");
-    return 0;
+    return sm_pve_program(argc, argv, 0);
 }

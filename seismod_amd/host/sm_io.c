@@ -589,6 +589,61 @@ void sm_read_grid_pve(const char *fn, size_t np, sm_grid *g, int *with_pml)
     g->dy = g->dx;
 }
 
+/* ncol values per node into cols[0..ncol), padded (nx + 2np) x (nz + 2np):
+ * from the whole padded grid in the file (with_pml), or from nx x nz nodes
+ * (or one z column: laterally uniform) copied into the layers from the
+ * nearest model node */
+static void read_pve_columns(const char *fn, const sm_grid *g, size_t np, int with_pml, int ncol,
+                             double *const cols[])
+{
+    FILE *f = open_or_die(fn);
+    char type[64];
+    if (fscanf(f, "%*d %*d") < 0 || fscanf(f, "%*f %*f") < 0 || fscanf(f, "%*f %*f") < 0 ||
+        fscanf(f, "%63s", type) != 1)
+        die("Error: something wrong with file %s\n", fn);
+    const size_t nx = g->nx, nz = g->nz, nx2 = nx + 2 * np, nz2 = nz + 2 * np, nc = (size_t)ncol;
+    if (with_pml) {
+        const size_t N = nx2 * nz2;
+        size_t nv = 0;
+        double *v = read_body(f, fn, nc * N, &nv);
+        fclose(f);
+        if (nv < nc * N) die("Error: something wrong with file %s\n", fn);
+        for (size_t n = 0; n < N; ++n)
+            for (size_t q = 0; q < nc; ++q) cols[q][n] = v[nc * n + q];
+        free(v);
+        return;
+    }
+    const size_t ncell = nx * nz;
+    size_t nv = 0;
+    double *v = read_body(f, fn, nc * ncell, &nv);
+    fclose(f);
+    int flag1d = 0;
+    if (nv < nc * ncell) {
+        if (nx > 1 && nv == nc * nz) flag1d = 1;
+        else die("Error: something wrong with file %s\n", fn);
+    }
+    for (size_t i = 0; i < nx; ++i)
+        for (size_t j = 0; j < nz; ++j) {
+            const double *c = v + nc * ((flag1d ? 0 : i) * nz + j);
+            for (size_t q = 0; q < nc; ++q) cols[q][(i + np) * nz2 + j + np] = c[q];
+        }
+    free(v);
+    for (size_t i = 0; i < nx; ++i)
+        for (size_t j = 0; j < np; ++j)
+            for (size_t q = 0; q < nc; ++q) {
+                double *a = cols[q];
+                a[(i + np) * nz2 + j] = a[(i + np) * nz2 + np];
+                a[(i + np) * nz2 + np + nz + j] = a[(i + np) * nz2 + np + nz - 1];
+            }
+    for (size_t i = 0; i < np; ++i)
+        for (size_t j = 0; j < nz2; ++j)
+            for (size_t q = 0; q < nc; ++q) {
+                double *a = cols[q];
+                a[i * nz2 + j] = a[np * nz2 + j];
+                a[(i + nx + np) * nz2 + j] = a[(nx + np - 1) * nz2 + j];
+            }
+}
+
 void sm_read_model_pve(const char *fn, const sm_grid *g, size_t np, int with_pml, double *const vals[12])
 {
     /* io_utils.c:403-625.  "isotropic": a short file holding exactly one z
@@ -597,52 +652,43 @@ void sm_read_model_pve(const char *fn, const sm_grid *g, size_t np, int with_pml
      * copied too).  "isotropic_pml": every padded node from the file (the
      * reference leaves a short last node uninitialised; here it is an
      * error). */
+    read_pve_columns(fn, g, np, with_pml, 12, vals);
+}
+
+void sm_read_grid_pve_vti(const char *fn, sm_grid *g, int *with_pml)
+{
+    /* io_utils.c:296-328 and 627-647: read_grid_params subtracts the layers
+     * only for "isotropic_pml", so nx and nz of an "anisotropic__vti_pml"
+     * file (two underscores, as read_modelVTI compares it) count interior
+     * nodes while its body holds the padded grid */
     FILE *f = open_or_die(fn);
+    memset(g, 0, sizeof(*g));
     char type[64];
-    if (fscanf(f, "%*d %*d") < 0 || fscanf(f, "%*f %*f") < 0 || fscanf(f, "%*f %*f") < 0 ||
-        fscanf(f, "%63s", type) != 1)
+    if (fscanf(f, "%zu %zu", &g->nx, &g->nz) != 2 || fscanf(f, "%lf %lf", &g->dx, &g->dz) != 2 ||
+        fscanf(f, "%lf %lf", &g->x0, &g->z0) != 2 || fscanf(f, "%63s", type) != 1)
         die("Error: something wrong with file %s\n", fn);
-    const size_t nx = g->nx, nz = g->nz, nx2 = nx + 2 * np, nz2 = nz + 2 * np;
-    if (with_pml) {
-        const size_t N = nx2 * nz2;
-        size_t nv = 0;
-        double *v = read_body(f, fn, 12 * N, &nv);
-        fclose(f);
-        if (nv < 12 * N) die("Error: something wrong with file %s\n", fn);
-        for (size_t n = 0; n < N; ++n)
-            for (int q = 0; q < 12; ++q) vals[q][n] = v[12 * n + (size_t)q];
-        free(v);
-        return;
-    }
-    const size_t ncell = nx * nz;
-    size_t nv = 0;
-    double *v = read_body(f, fn, 12 * ncell, &nv);
     fclose(f);
-    int flag1d = 0;
-    if (nv < 12 * ncell) {
-        if (nx > 1 && nv == 12 * nz) flag1d = 1;
-        else die("Error: something wrong with file %s\n", fn);
+    for (char *c = type; *c; ++c) *c = (char)tolower((unsigned char)*c);
+    *with_pml = strcmp(type, "anisotropic__vti_pml") == 0;
+    if (!*with_pml && strcmp(type, "anisotropic_vti") != 0) die("Error, model in %s not VTI anisotropic\n", fn);
+    g->ny = 1;
+    g->dy = g->dx;
+}
+
+void sm_read_model_pve_vti(const char *fn, const sm_grid *g, size_t np, int with_pml, double *const vals[16])
+{
+    /* io_utils.c:627-888: per node "K_m K_s K_f phi mu rho_s rho_f T3 eta
+     * kappa3 Q f0 epsilon delta kappa3/kappa1 T3/T1", into vals in the order
+     * K_m K_s K_f phi mu rho_s rho_f T1 T3 eta kappa1 kappa3 Q f0 epsilon
+     * delta; the same column and layer rules as sm_read_model_pve */
+    double *const cols[16] = {vals[0], vals[1], vals[2], vals[3], vals[4], vals[5], vals[6], vals[8],
+                              vals[9], vals[11], vals[12], vals[13], vals[14], vals[15], vals[10], vals[7]};
+    read_pve_columns(fn, g, np, with_pml, 16, cols);
+    const size_t N = (g->nx + 2 * np) * (g->nz + 2 * np);
+    for (size_t n = 0; n < N; ++n) {
+        vals[10][n] = vals[11][n] / vals[10][n];   /* kappa1 = kappa3 / (kappa3/kappa1) */
+        vals[7][n] = vals[8][n] / vals[7][n];      /* T1 = T3 / (T3/T1) */
     }
-    for (size_t i = 0; i < nx; ++i)
-        for (size_t j = 0; j < nz; ++j) {
-            const double *c = v + 12 * ((flag1d ? 0 : i) * nz + j);
-            for (int q = 0; q < 12; ++q) vals[q][(i + np) * nz2 + j + np] = c[q];
-        }
-    free(v);
-    for (size_t i = 0; i < nx; ++i)
-        for (size_t j = 0; j < np; ++j)
-            for (int q = 0; q < 12; ++q) {
-                double *a = vals[q];
-                a[(i + np) * nz2 + j] = a[(i + np) * nz2 + np];
-                a[(i + np) * nz2 + np + nz + j] = a[(i + np) * nz2 + np + nz - 1];
-            }
-    for (size_t i = 0; i < np; ++i)
-        for (size_t j = 0; j < nz2; ++j)
-            for (int q = 0; q < 12; ++q) {
-                double *a = vals[q];
-                a[i * nz2 + j] = a[np * nz2 + j];
-                a[(i + nx + np) * nz2 + j] = a[(nx + np - 1) * nz2 + j];
-            }
 }
 
 void sm_pve_dispersion(size_t n, double *const v[12], double f, double *const vel[3], double *const att[3])

@@ -125,6 +125,15 @@ void sm_read_model_ve(const char *fn, const sm_grid *g, size_t np, int L, double
  * which type the file holds. */
 void sm_read_grid_pve(const char *fn, size_t np, sm_grid *g, int *with_pml);
 void sm_read_model_pve(const char *fn, const sm_grid *g, size_t np, int with_pml, double *const vals[12]);
+/* read_grid_params / read_modelVTI (io_utils.c:296-328, 627-888) for
+ * pve_vti_pml: type "anisotropic_vti" (nx x nz nodes or one z column) or
+ * "anisotropic__vti_pml" (the padded grid in the file; nx and nz still
+ * count interior nodes); 16 values per node "K_m K_s K_f phi mu rho_s rho_f
+ * T3 eta kappa3 Q f0 epsilon delta kappa3/kappa1 T3/T1", returned in vals
+ * as K_m K_s K_f phi mu rho_s rho_f T1 T3 eta kappa1 kappa3 Q f0 epsilon
+ * delta (padded arrays) */
+void sm_read_grid_pve_vti(const char *fn, sm_grid *g, int *with_pml);
+void sm_read_model_pve_vti(const char *fn, const sm_grid *g, size_t np, int with_pml, double *const vals[16]);
 /* Phase velocity (m/s) and attenuation (17.372 pi Im V / Re V) of the slow
  * and fast P waves and the S wave at frequency f (Hz) for n nodes of the
  * 12 model values as sm_read_model_pve gives them, from the complex moduli

@@ -847,7 +847,42 @@ class Poroviscoelastic(_Pstd):
               "sm_pve_set_energy")
         self.energy_steps = max_steps
 
+    def set_energy_axes(self, i1: int, i2: int, j1: int, j2: int, ax, bx, cx, az, bz, cz, max_steps: int) -> None:
+        """As set_energy with separate coefficients along x and z: the sum of
+        0.5 ax vx^2 + bx vx qx + 0.5 cx qx^2 and the z counterparts."""
+        ni, nj = i2 - i1 + 1, j2 - j1 + 1
+        a = [_f64(r).reshape(-1) for r in (ax, bx, cx, az, bz, cz)]
+        if any(r.size != ni * nj for r in a):
+            raise ValueError(f"energy coefficients must hold {ni * nj} values")
+        check(lib.sm_pve_set_energy_axes(self.handle, i1, i2, j1, j2, *(_dp(r) for r in a), max_steps),
+              "sm_pve_set_energy_axes")
+        self.energy_steps = max_steps
+
     def energy(self, it0: int, count: int) -> np.ndarray:
         out = np.zeros(count, dtype=np.float64)
         check(lib.sm_pve_get_energy(self.handle, it0, count, _dp(out)), "sm_pve_get_energy")
         return out
+
+
+class PoroviscoelasticVTI(Poroviscoelastic):
+    """Poroviscoelastic waves in a 2D VTI Biot medium (pve_vti_pml): the
+    isotropic solver's time loop with c11, c13, c33 from Thomsen's epsilon
+    and delta, tortuosity and permeability along x (T1, kappa1) and z (T3,
+    kappa3).  Model arrays as read_modelVTI leaves them."""
+
+    def __init__(self, ctx: Context, cfg: PveConfig, K_m, K_s, K_f, phi, mu, rho_s, rho_f, T1, T3, eta, kappa1,
+                 kappa3, Q, fq, epsilon, delta):
+        self.ctx, self.cfg = ctx, cfg
+        n = int(np.prod(cfg.shape))
+        self._m = [_f64(a).reshape(-1) for a in (K_m, K_s, K_f, phi, mu, rho_s, rho_f, T1, T3, eta, kappa1, kappa3,
+                                                  Q, fq, epsilon, delta)]
+        if any(a.size != n for a in self._m):
+            raise ValueError(f"model must hold {n} padded cells")
+        self._p = cfg.params()
+        h = C.c_void_p()
+        ctx.adopt(self)
+        check(lib.sm_pve_create_vti(ctx.handle, C.byref(self._p), *(_dp(a) for a in self._m), C.byref(h)),
+              "sm_pve_create_vti")
+        self.handle = h
+        self.nrec = 0
+        self.energy_steps = 0

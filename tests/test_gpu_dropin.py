@@ -592,7 +592,7 @@ def test_ve_vti_pml_program(tmp_path, template):
 
 
 # ------------------------------------------------------- poroviscoelastic --
-def deck_pve(tmp, nx=44, nz=38, np_=8, h=5.0, dt_ms=0.25, time_s=0.05, template=1, with_pml=False):
+def deck_pve(tmp, nx=44, nz=38, np_=8, h=5.0, dt_ms=0.25, time_s=0.05, template=1, with_pml=False, vti=False):
     rng = np.random.default_rng(29)
     top = np.arange(nz)[None, :] < nz // 2
     ones = np.ones((nx, 1))
@@ -601,12 +601,16 @@ def deck_pve(tmp, nx=44, nz=38, np_=8, h=5.0, dt_ms=0.25, time_s=0.05, template=
             two(0.3, 0.2), two(5.0, 8.0), two(2650.0, 2700.0), two(1000.0, 1040.0), two(2.0, 2.5),
             two(1.0, 0.8), two(800.0, 300.0), two(40.0, 2e4), two(30.0, 50.0)]
     npad = lambda a: np.pad(a, np_, mode="edge")
+    if vti:
+        # file columns: ... T3 eta kappa3 Q f0 epsilon delta kappa3/kappa1 T3/T1
+        vals = vals + [two(0.12, 0.2), two(0.05, 0.1), two(3.0, 1.5), two(1.4, 1.2)]
     if with_pml:
         file_vals = [npad(a) for a in vals]
-        head = f"{nx + 2 * np_} {nz + 2 * np_}\n{h} {h}\n0.0 0.0\nisotropic_pml\n"
+        head = (f"{nx} {nz}\n{h} {h}\n0.0 0.0\nanisotropic__vti_pml\n" if vti else
+                f"{nx + 2 * np_} {nz + 2 * np_}\n{h} {h}\n0.0 0.0\nisotropic_pml\n")
     else:
         file_vals = vals
-        head = f"{nx} {nz}\n{h} {h}\n0.0 0.0\nisotropic\n"
+        head = f"{nx} {nz}\n{h} {h}\n0.0 0.0\n" + ("anisotropic_vti\n" if vti else "isotropic\n")
     with open(tmp / "model.dat", "w") as f:
         f.write(head)
         for i in range(file_vals[0].shape[0]):
@@ -617,6 +621,8 @@ def deck_pve(tmp, nx=44, nz=38, np_=8, h=5.0, dt_ms=0.25, time_s=0.05, template=
             ("p", "Trace", "100.0 80.0", 0.0, dt_ms), ("txz", "Trace", "30.0 150.0", 0.0, dt_ms),
             ("div", "Trace", "120.0 90.0", 0.0, dt_ms), ("curl", "Trace", "80.0 100.0", 0.0, dt_ms),
             ("p", "Snapshot", None, 0.025, 0.0)]
+    if vti:       # pve_vti_pml records no div or curl
+        recs[4:6] = [("tzz", "Trace", "120.0 90.0", 0.0, dt_ms), ("qx", "Trace", "80.0 100.0", 0.0, dt_ms)]
     with open(tmp / "rcv.dat", "w") as f:
         f.write(f"{len(recs)}\n")
         for comp, kind, xz, t0, rdt in recs:
@@ -625,15 +631,20 @@ def deck_pve(tmp, nx=44, nz=38, np_=8, h=5.0, dt_ms=0.25, time_s=0.05, template=
     (tmp / "run.par").write_text(
         "model.dat # model, file\nsrc.dat # source, file\nrcv.dat # output, records\n"
         f"pve # basename, name\n{time_s} # time, s\n{dt_ms} # dt, ms\n1 # abs, on\n{np_} # nl, layers\n"
-        "3 # kappa_max, PML\n1 # plstr, strips\n2 # check_model, store\n1 # saveEnergy, E(t)\n"
+        "3 # kappa_max, PML\n1 # plstr, strips\n" + ("" if vti else "2 # check_model, store\n") +
+        "1 # saveEnergy, E(t)\n"
         f"{roi[0]}/{roi[1]}/{roi[2]}/{roi[3]} # EnergyROI, region\n")
     nz2 = nz + 2 * np_
     cell = lambda x, z: (int(round(x / h)) + np_) * nz2 + int(round(z / h)) + np_
-    comp = {"Vx": 0, "qz": 4, "p": 8, "txz": 7, "div": 11, "curl": 12}
+    comp = {"Vx": 0, "qz": 4, "p": 8, "txz": 7, "div": 11, "curl": 12, "tzz": 6, "qx": 3}
     rec = [(cell(*map(float, xz.split())), comp[c], t0, rdt) for c, k, xz, t0, rdt in recs if xz]
     ij = [(i, j) for i in range(int(round(roi[0] / h)), int(round(roi[1] / h)) + 1)
           for j in range(int(round(roi[2] / h)), int(round(roi[3] / h)) + 1)]
-    return dict(nx=nx, nz=nz, np=np_, h=h, dt=dt_ms * 1e-3, model=tuple(npad(a) for a in vals), rec=rec,
+    model = [npad(a) for a in vals]
+    if vti:       # the solver's order: ... T1 T3 eta kappa1 kappa3 Q f0 epsilon delta
+        K_m, K_s, K_f, phi, mu, rs, rf, T3, eta, k3, Q, f0, e_, d_, rk, rt = model
+        model = [K_m, K_s, K_f, phi, mu, rs, rf, T3 / rt, T3, eta, k3 / rk, k3, Q, f0, e_, d_]
+    return dict(nx=nx, nz=nz, np=np_, h=h, dt=dt_ms * 1e-3, model=tuple(model), rec=rec, vti=vti,
                 template=template, srcs=[(6, cell(110.0, 60.0), 2.0), (7, cell(70.0, 120.0), 0.5)], snap_t0=0.025,
                 roi_cells=ij, raw=vals)
 
@@ -657,7 +668,7 @@ def oracle_pve(d, nsteps, extra_rec=()):
     src = [(p[0], p[1], fct[off[k]:off[k + 1]]) for k, p in enumerate(pts)]
     cfg = dict(nx=d["nx"], nz=d["nz"], np=d["np"], dx=d["h"], dz=d["h"], dt=d["dt"], vmax=5000.0,
                kappa_max=3.0, alpha_max=math.pi * 60.0, iwipe=1)
-    mat = B.pve_material(*d["model"], d["dt"])
+    mat = (B.pve_vti_material if d["vti"] else B.pve_material)(*d["model"], d["dt"])
     rec = list(d["rec"]) + list(extra_rec)
     return B.pve_run(cfg, mat, src, [r[0] for r in rec], [r[1] for r in rec], nsteps)
 
@@ -710,3 +721,41 @@ def test_pve_iso_pml_program(tmp_path, template, with_pml):
         src = (vel if k < 3 else att)[k % 3].reshape(d["model"][0].shape)
         assert np.array_equal(nc_field(tmp_path / f"pve_{name}.nc"), src.T.astype(np.float32).astype(np.float64))
 
+
+
+@pytest.mark.parametrize("template,with_pml", [(1, False), (9, True)])
+def test_pve_vti_pml_program(tmp_path, template, with_pml):
+    """The drop-in pve_vti_pml (FP64) against the oracle: every trace line,
+    the p snapshot and the kinetic energy of the region (eq. 7.171 of
+    Carcione 2007 with the x and z tortuosities, pve_vti_pml.c:1112-1120 and
+    1334-1345)."""
+    from seismod_amd.solver import nsteps as sm_nsteps, record_due
+    d = deck_pve(tmp_path, template=template, with_pml=with_pml, vti=True)
+    run("pve_vti_pml", tmp_path, "f64")
+    ns = sm_nsteps(0.05, d["dt"])
+    nz2 = d["nz"] + 2 * d["np"]
+    p = d["np"]
+    roi = [(i + p) * nz2 + j + p for i, j in d["roi_cells"]]
+    extra = [(c, k, 0.0, 0.0) for k in (0, 2, 3, 4) for c in roi]
+    tr, _ = oracle_pve(d, ns, extra)
+    for q, (_, _, t0, rdt) in enumerate(d["rec"]):
+        got = read_trc(tmp_path / "trc" / "pve" / f"pve_{q + 1}.trc")
+        _close_lines(got, trace_lines(tr[:, q], d["dt"], t0, rdt * 1e-3), q)
+    nr, nq = len(d["rec"]), len(roi)
+    vx, vz, qx, qz = (tr[:, nr + k * nq:nr + (k + 1) * nq] for k in range(4))
+    phi, rs, rf, T3, rt = (np.array([d["raw"][q][i, j] for i, j in d["roi_cells"]]) for q in (3, 5, 6, 7, 15))
+    T1 = T3 / rt
+    f1, f2 = (1 - phi) * rs, phi * rf
+    rx, rz = (T1 - 1) / (1 / phi - 1), (T3 - 1) / (1 / phi - 1)
+    E = 0.5 * (f1 * (vx * vx + vz * vz) - rx * (vx - qx) ** 2 - rz * (vz - qz) ** 2 + f2 * (qx * qx + qz * qz)).sum(1)
+    got = np.array([[float(x) for x in l.split()] for l in (tmp_path / "E" / "pve_E.dat").read_text().splitlines()])
+    assert got.shape == (ns, 2) and np.abs(E).max() > 0
+    assert np.abs(got[:, 1] - E).max() <= 1e-6 * np.abs(E).max()
+    snaps = sorted((tmp_path / "ssh" / "pve").glob("pve_7_*.nc"))
+    assert len(snaps) == 1
+    it_snap = [it for it in range(ns) if record_due(it, d["dt"], d["snap_t0"], 0.0)]
+    _, W = oracle_pve(d, it_snap[0] + 1)
+    want = (W[3].T.astype(np.float32) * np.float32(1e3)).astype(np.float64)
+    z = nc_field(snaps[0])
+    assert z.shape == want.shape
+    assert np.abs(z - want).max() <= 1e-5 * np.abs(want).max()

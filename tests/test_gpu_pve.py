@@ -11,8 +11,10 @@ its scale (errors()); FP32 storage to PVE_FP32_TOL.  Decks: layered rock with la
 receiver component (divergence and curl included), a model wider than deep
 with kappa_max > 1 and alpha_max = 0, no layers (abs 0) with odd nx2, an
 odd nz2 (x derivatives by real transforms), every source type on one cell
-with a layer of zero shear and frame moduli, and layers with the lossless
-branch (Q >= 1e4) and an inviscid fluid (eta = 0).
+with a layer of zero shear and frame moduli, layers with the lossless
+branch (Q >= 1e4) and an inviscid fluid (eta = 0), and VTI Biot media
+(pve_vti_pml: Thomsen's epsilon and delta, tortuosity and permeability
+different along x and z) with layers, kappa_max > 1 and without layers.
 """
 import math
 
@@ -50,7 +52,7 @@ def biot_vmax(K_m, K_s, K_f, phi, mu, rho_s, rho_f, T):
 
 
 def deck(nx=40, nz=50, np_=8, seed=0, h=10.0, nsteps=120, f0=25.0, kappa_max=1.0, alpha_max=1, abs_=True,
-         srcs=None, fluid_layer=False, lossless_layer=False):
+         srcs=None, fluid_layer=False, lossless_layer=False, vti=False):
     rng = np.random.default_rng(seed)
     npad = np_ if abs_ else 0
     nx2, nz2 = nx + 2 * npad, nz + 2 * npad
@@ -82,6 +84,12 @@ def deck(nx=40, nz=50, np_=8, seed=0, h=10.0, nsteps=120, f0=25.0, kappa_max=1.0
         eta[:, lay == 2] = 0.0
     model = (K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, eta, kappa, Q, fq)
     vmax = biot_vmax(*model[:8])
+    if vti:
+        # tortuosity and permeability along z, Thomsen's epsilon and delta
+        T3, kappa3 = prop(1.5, 3.0), prop(100, 2000)
+        eps, dl = prop(0.0, 0.25), prop(-0.05, 0.15)
+        model = (K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, T3, eta, kappa, kappa3, Q, fq, eps, dl)
+        vmax *= float(np.sqrt(1 + 2 * eps.max()))
     dt = 0.2 * h / vmax
     if srcs is None:
         srcs = [(B.BULK, nx2 // 2, nz2 // 2, 1.0), (B.SXZ, nx2 // 3, nz2 // 3, 0.6), (B.SF, nx2 // 2 + 3, nz2 // 2 - 5, 0.8)]
@@ -97,7 +105,7 @@ def deck(nx=40, nz=50, np_=8, seed=0, h=10.0, nsteps=120, f0=25.0, kappa_max=1.0
     cfg = dict(nx=nx, nz=nz, np=npad, dx=h, dz=h, dt=dt, vmax=vmax, kappa_max=kappa_max,
                alpha_max=(math.pi * f0 if alpha_max else 0.0), iwipe=1 if abs_ else 0)
     return dict(cfg=cfg, model=model, types=types, idx=idx, Lf=Lf, fct=fct, ridx=ridx, rcomp=rcomp, nsteps=nsteps,
-                f0=f0, alpha_max=alpha_max, abs=abs_, np=np_)
+                f0=f0, alpha_max=alpha_max, abs=abs_, np=np_, vti=vti)
 
 
 def config(d, storage="f64", hipfft=False):
@@ -108,9 +116,13 @@ def config(d, storage="f64", hipfft=False):
                      storage=storage, hipfft=hipfft)
 
 
+def solver(ctx, d, storage="f64", hipfft=False):
+    from seismod_amd.solver import Poroviscoelastic, PoroviscoelasticVTI
+    return (PoroviscoelasticVTI if d["vti"] else Poroviscoelastic)(ctx, config(d, storage, hipfft), *d["model"])
+
+
 def run_gpu(ctx, d, storage, hipfft=False):
-    from seismod_amd.solver import Poroviscoelastic
-    s = Poroviscoelastic(ctx, config(d, storage, hipfft), *d["model"])
+    s = solver(ctx, d, storage, hipfft)
     s.set_sources(d["types"], d["idx"], d["Lf"], d["fct"])
     s.set_receivers(d["ridx"], d["rcomp"], d["nsteps"])
     s.run(d["nsteps"] // 2)
@@ -125,7 +137,7 @@ def run_gpu(ctx, d, storage, hipfft=False):
 def run_oracle(d):
     off = np.concatenate([[0], np.cumsum(d["Lf"])]).astype(np.int64)
     src = [(int(t), int(i), d["fct"][off[k]:off[k + 1]]) for k, (t, i) in enumerate(zip(d["types"], d["idx"]))]
-    mat = B.pve_material(*d["model"], d["cfg"]["dt"])
+    mat = (B.pve_vti_material if d["vti"] else B.pve_material)(*d["model"], d["cfg"]["dt"])
     tr, W = B.pve_run(d["cfg"], mat, src, d["ridx"], d["rcomp"], d["nsteps"])
     return tr, W[:8]
 
@@ -138,11 +150,15 @@ CASES = [
     dict(nx=30, nz=36, np_=7, seed=6, fluid_layer=True,
          srcs=[(B.SX, 20, 25, 1.0), (B.SZ, 20, 25, -0.4), (B.BULK, 20, 25, 0.7), (B.SXZ, 20, 25, 0.5),
                (B.SF, 20, 25, 0.3), (B.BULK_S, 20, 25, -0.6), (B.SXZ, 21, 24, 0.2), (3, 22, 22, 1.0)]),
+    # VTI Biot media (pve_vti_pml)
+    dict(seed=11, vti=True),
+    dict(nx=70, nz=20, np_=6, seed=13, kappa_max=3.0, alpha_max=0, vti=True),
+    dict(nx=33, nz=28, np_=5, seed=14, abs_=False, vti=True, lossless_layer=True),
 ]
 
 
 def ids(c):
-    return f"seed{c['seed']}"
+    return f"seed{c['seed']}" + ("_vti" if c.get("vti") else "")
 
 
 def rel(a, b):
@@ -193,7 +209,7 @@ def test_pve_fp64_matches_oracle(ctx, case):
         assert np.allclose(dc[k].reshape(-1)[d["ridx"][sel].astype(np.int64)], tr[-1, sel], rtol=0, atol=0)
 
 
-@pytest.mark.parametrize("case", CASES[:2], ids=ids)
+@pytest.mark.parametrize("case", CASES[:2] + CASES[5:6], ids=ids)
 def test_pve_fp32_tolerance(ctx, case):
     d = deck(**case)
     tr_o, f_o = run_oracle(d)
@@ -282,3 +298,30 @@ def test_pve_energy_of_a_region(ctx):
     note("pve_energy_vs_oracle", err=err)
     assert err < PVE_FP64_TOL, err
     assert np.array_equal(got[0], got[1])
+
+
+def test_pve_energy_per_axis(ctx):
+    """sm_pve_set_energy_axes (the VTI program's energy, eq. 7.171 of
+    Carcione 2007: different coefficients along x and z) on a VTI deck
+    against the oracle's v and q at every region cell."""
+    d = deck(seed=15, nsteps=50, vti=True)
+    nx2, nz2 = d["model"][0].shape
+    i1, i2, j1, j2 = nx2 // 2 - 2, nx2 // 2 + 3, nz2 // 2 - 3, nz2 // 2 + 2
+    cells = [i * nz2 + j for i in range(i1, i2 + 1) for j in range(j1, j2 + 1)]
+    rng = np.random.default_rng(4)
+    co = [rng.uniform(-500, 2500, len(cells)) for _ in range(6)]
+    d["ridx"] = np.array(cells * 4, np.uintp)
+    d["rcomp"] = np.repeat(np.array([B.VX, B.VZ, B.QX, B.QZ], np.intc), len(cells))
+    tr_o, _ = run_oracle(d)
+    vx, vz, qx, qz = (tr_o[:, k * len(cells):(k + 1) * len(cells)] for k in range(4))
+    want = (0.5 * co[0] * vx * vx + co[1] * vx * qx + 0.5 * co[2] * qx * qx + 0.5 * co[3] * vz * vz +
+            co[4] * vz * qz + 0.5 * co[5] * qz * qz).sum(1)
+    s = solver(ctx, d)
+    s.set_sources(d["types"], d["idx"], d["Lf"], d["fct"])
+    s.set_energy_axes(i1, i2, j1, j2, *co, d["nsteps"])
+    s.run(d["nsteps"])
+    got = s.energy(0, d["nsteps"])
+    s.close()
+    err = float(np.abs(got - want).max() / np.abs(want).max())
+    note("pve_energy_axes_vs_oracle", err=err)
+    assert err < PVE_FP64_TOL, err

@@ -765,3 +765,56 @@ def test_pve_iso_pml_refuses_unavailable_options(tmp_path, extra, msg):
     (tmp_path / "run.par").write_text("model.dat # model,\nsrc.dat # source,\nrcv.dat # output,\n" + extra)
     r = run_bin("pve_iso_pml", "-p", "run.par", cwd=tmp_path)
     assert r.returncode == 1 and msg in r.stderr, r.stderr
+
+
+def _read_pve_vti(fn, np_):
+    g = Grid()
+    wp = C.c_int()
+    LIB.sm_read_grid_pve_vti(str(fn).encode(), C.byref(g), C.byref(wp))
+    shp = (g.nx + 2 * np_, g.nz + 2 * np_)
+    out = [np.zeros(shp) for _ in range(16)]
+    arr = (C.c_void_p * 16)(*(o.ctypes.data_as(C.c_void_p) for o in out))
+    LIB.sm_read_model_pve_vti(str(fn).encode(), C.byref(g), C.c_size_t(np_), wp, arr)
+    return g, wp.value, np.stack(out)
+
+
+@pytest.mark.parametrize("kind", ["full", "column", "with_pml"])
+def test_read_model_pve_vti_layout_and_padding(tmp_path, kind):
+    """read_grid_params / read_modelVTI (io_utils.c:296-328, 627-888): 16
+    values per node, kappa1 = kappa3 / column 15 and T1 = T3 / column 16;
+    "anisotropic__vti_pml" holds the padded grid while its header still
+    counts interior nodes (read_grid_params subtracts the layers only for
+    "isotropic_pml")."""
+    rng = np.random.default_rng(14)
+    nx, nz, np_ = 4, 5, 2
+    shp = (nx + 2 * np_, nz + 2 * np_) if kind == "with_pml" else (nx, nz)
+    m = np.concatenate([_pve_values(rng, *shp), np.stack([rng.uniform(0, 0.3, shp), rng.uniform(-0.1, 0.2, shp),
+                                                          rng.uniform(1, 5, shp), rng.uniform(0.5, 2, shp)])])
+    if kind == "column":
+        m[:] = m[:, :1, :]
+    head = f"{nx} {nz}\n2.0 2.5\n1.0 0.0\n" + ("Anisotropic__VTI_pml\n" if kind == "with_pml" else "anisotropic_vti\n")
+    fn = tmp_path / "pvevti.dat"
+    # file columns: K_m K_s K_f phi mu rho_s rho_f T3 eta kappa3 Q f0 eps delta kappa3/kappa1 T3/T1
+    with open(fn, "w") as f:
+        f.write(head)
+        for i in range(1 if kind == "column" else m.shape[1]):
+            for j in range(m.shape[2]):
+                f.write(" ".join(repr(float(m[q, i, j])) for q in range(16)) + "\n")
+    g, wp, out = _read_pve_vti(fn, np_)
+    assert (g.nx, g.nz, wp) == (nx, nz, kind == "with_pml")
+    pad = (lambda a: a) if kind == "with_pml" else (lambda a: np.pad(a, np_, mode="edge"))
+    order = {0: 0, 1: 1, 2: 2, 3: 3, 4: 4, 5: 5, 6: 6, 8: 7, 9: 8, 11: 9, 12: 10, 13: 11, 14: 12, 15: 13}
+    for q_out, q_in in order.items():
+        assert np.array_equal(out[q_out], pad(m[q_in])), q_out
+    assert np.array_equal(out[10], pad(m[9] / m[14]))     # kappa1
+    assert np.array_equal(out[7], pad(m[7] / m[15]))      # T1
+
+
+def test_pve_vti_pml_refuses_check_model(tmp_path):
+    (tmp_path / "run.par").write_text("model.dat # model,\nsrc.dat # source,\nrcv.dat # output,\n"
+                                      "1 # check_model, display\n")
+    r = run_bin("pve_vti_pml", "-p", "run.par", cwd=tmp_path)
+    assert r.returncode == 1 and "check_model" in r.stderr, r.stderr
+    assert run_bin("pve_vti_pml", "-h").returncode == 0
+    r = run_bin("pve_vti_pml", "-h")
+    assert "anisotropic_vti" in r.stdout

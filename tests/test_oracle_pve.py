@@ -177,3 +177,135 @@ def test_cpml_absorbs():
         inner = W[4:6, np_:np_ + n, np_:np_ + n]
         e.append(float(np.sum(inner * inner)))
     assert e[1] < 1e-3 * e[0], e
+
+
+# ---------------------------------------------------------------- VTI ------
+def vti_rock(shape, eps=0.15, delta=0.08, T1=1.8, T3=2.6, kappa1=1500.0, kappa3=400.0, eta=0.0, Q=1e5, f0=30.0,
+             **iso):
+    K_m, K_s, K_f, phi, mu, rho_s, rho_f = rock(shape, **iso)[:7]
+    one = np.ones(shape)
+    return (K_m, K_s, K_f, phi, mu, rho_s, rho_f, T1 * one, T3 * one, eta * one, kappa1 * one, kappa3 * one,
+            Q * one, f0 * one, eps * one, delta * one)
+
+
+def test_vti_material_against_anisotropic_biot_theory():
+    """pve_vti_pml.c:787-1145 against relations written independently:
+    Thomsen's c11 = (1 + 2 eps) c33 and (c13 + c55)^2 = (c33 - c55)^2 +
+    2 delta c33 (c33 - c55); the effective-stress coefficients alpha_i =
+    1 - sum_j C_ij / (3 K_s) and M = K_s / (1 - K*/K_s - phi (1 - K_s/K_f)),
+    K* = sum_ij C_ij / 9 over the upper-left 3 x 3 block of the VTI frame
+    (Carcione 1996, eq. 9-11); directional mass m_i = T_i rho_f / phi;
+    and the isotropic limit (eps = delta = 0, T1 = T3, kappa1 = kappa3)
+    equal to the isotropic material."""
+    rng = np.random.default_rng(9)
+    shape = (6, 7)
+    K_s = rng.uniform(30, 40, shape); K_m = rng.uniform(2, 15, shape); K_f = rng.uniform(1.5, 3.0, shape)
+    phi = rng.uniform(0.1, 0.35, shape); mu = rng.uniform(3, 10, shape)
+    rho_s = rng.uniform(2500, 2800, shape); rho_f = rng.uniform(900, 1100, shape)
+    T1, T3 = rng.uniform(1.5, 3, shape), rng.uniform(1.5, 3, shape)
+    eta = rng.uniform(0.5, 2, shape); k1, k3 = rng.uniform(100, 2000, shape), rng.uniform(100, 2000, shape)
+    Q = rng.uniform(10, 100, shape); f0 = rng.uniform(10, 80, shape)
+    eps, dl = rng.uniform(0, 0.3, shape), rng.uniform(-0.05, 0.2, shape)
+    dt = 1e-4
+    m = B.pve_vti_material(K_m, K_s, K_f, phi, mu, rho_s, rho_f, T1, T3, eta, k1, k3, Q, f0, eps, dl, dt)
+    Km, Ks, Kf, MU = K_m * 1e3, K_s * 1e3, K_f * 1e3, mu * 1e3
+    c33 = Km + 4. / 3. * MU
+    assert np.allclose(m["c33"], c33, rtol=1e-14)
+    assert np.allclose(m["c11"], (1 + 2 * eps) * c33, rtol=1e-14)
+    assert np.allclose((m["c13"] + MU) ** 2, (c33 - MU) ** 2 + 2 * dl * c33 * (c33 - MU), rtol=1e-12)
+    c12 = m["c11"] - 2 * MU
+    C = np.array([[m["c11"], c12, m["c13"]], [c12, m["c11"], m["c13"]], [m["c13"], m["c13"], c33]])
+    assert np.allclose(m["alpha1"], 1 - C[0].sum(0) / (3 * Ks), rtol=1e-13)
+    assert np.allclose(m["alpha3"], 1 - C[2].sum(0) / (3 * Ks), rtol=1e-13)
+    Kstar = C.sum((0, 1)) / 9
+    assert np.allclose(m["M"], Ks / (1 - Kstar / Ks - phi * (1 - Ks / Kf)), rtol=1e-12)
+    rf = rho_f * 1e-6
+    rho = (1 - phi) * rho_s * 1e-6 + phi * rf
+    ri, rfi = 0.5 * (rho[:-1] + rho[1:]), 0.5 * (rf[:-1] + rf[1:])
+    m1i = 0.5 * (T1 * rf / phi)[:-1] + 0.5 * (T1 * rf / phi)[1:]
+    assert np.allclose(m["m_i"][:-1], rfi * rfi / ri - m1i, rtol=1e-12)
+    rj, rfj = 0.5 * (rho[:, :-1] + rho[:, 1:]), 0.5 * (rf[:, :-1] + rf[:, 1:])
+    m3j = 0.5 * (T3 * rf / phi)[:, :-1] + 0.5 * (T3 * rf / phi)[:, 1:]
+    assert np.allclose(m["m_j"][:, :-1], rfj * rfj / rj - m3j, rtol=1e-12)
+    nk3 = eta * B.ETA_SCALE / k3
+    lam = -0.5 * (nk3[:, :-1] + nk3[:, 1:]) * rj / (rj * m3j - rfj * rfj)
+    assert np.allclose(m["cq_j"][:, :-1], np.exp(lam * dt), rtol=1e-13)
+    # isotropic limit
+    zero = np.zeros(shape)
+    mv = B.pve_vti_material(K_m, K_s, K_f, phi, mu, rho_s, rho_f, T1, T1, eta, k1, k1, Q, f0, zero, zero, dt)
+    mi = B.pve_material(K_m, K_s, K_f, phi, mu, rho_s, rho_f, T1, eta, k1, Q, f0, dt)
+    assert np.allclose(mv["c11"], mi["E"], rtol=1e-14) and np.allclose(mv["c33"], mi["E"], rtol=1e-14)
+    assert np.allclose(mv["c13"], mi["Emu"], rtol=1e-12)
+    for k in ("M", "ts", "varphi", "mu_ij", "cq_i", "cv_i", "cq_j", "cv_j", "m_i", "m_j", "rf_i", "irho_j"):
+        assert np.allclose(mv[k], mi[k], rtol=1e-11), k
+    assert np.allclose(mv["alpha1"], mi["alpha"], rtol=1e-12) and np.allclose(mv["alpha3"], mi["alpha"], rtol=1e-12)
+
+
+def _run_vti(model, h, dt, f0, nsteps, cells, recs):
+    mat = B.pve_vti_material(*model, dt)
+    nx2, nz2 = mat["M"].shape
+    Lf = int(math.ceil((1.0 / f0) * 2.0 / (0.5 * dt)))
+    fct = _ricker(f0, 0.5 * dt, Lf) / (h * h)
+    src = [(B.BULK, i * nz2 + j, fct) for i, j in cells]
+    cfg = dict(np=0, dx=h, dz=h, dt=dt, vmax=4000.0, kappa_max=1.0, alpha_max=math.pi * f0, iwipe=0)
+    return B.pve_run(cfg, mat, src, [i * nz2 + j for i, j, _ in recs], [c for _, _, c in recs], nsteps)
+
+
+@pytest.mark.parametrize("axis", ["x", "z"])
+def test_vti_line_source_fast_and_slow_waves(axis):
+    """Lossless VTI Biot medium (eps 0.15, delta 0.08, T1 1.8, T3 2.6), a
+    line of Bulk sources: waves along x see C = [[c11 + a1^2 M, a1 M],
+    [a1 M, M]] and R = [[rho, rho_f], [rho_f, T1 rho_f / phi]], waves along
+    z the c33, alpha3, T3 counterparts; v and q against the two-wave closed
+    form, and the cross stress (tau_zz for waves along x, tau_xx along z)
+    against sum_k g_k [(c13 + a1 a3 M) phi_v + a_other M phi_q]_k
+    s(t - |x|/c_k) / (2 c_k^3 h), which pins c13.  Held to 0.5%."""
+    h, f0, dt = 2.0, 50.0, 1e-4
+    n_along, n_across = 128, 200
+    shape = (n_along, n_across) if axis == "x" else (n_across, n_along)
+    model = vti_rock(shape)
+    mat = B.pve_vti_material(*model, dt)
+    K_m, K_s, K_f, phi, mu, rho_s, rho_f, T1, T3 = (a[0, 0] for a in model[:9])
+    eps, dl = model[14][0, 0], model[15][0, 0]
+    Km, Ks, Kf, MU = K_m * 1e3, K_s * 1e3, K_f * 1e3, mu * 1e3
+    c33 = Km + 4. / 3. * MU
+    c11 = (1 + 2 * eps) * c33
+    c13 = math.sqrt((c33 - MU) ** 2 + 2 * dl * c33 * (c33 - MU)) - MU
+    c12 = c11 - 2 * MU
+    a1, a3 = 1 - (c11 + c12 + c13) / (3 * Ks), 1 - (2 * c13 + c33) / (3 * Ks)
+    M = Ks / (1 - (2 * c11 + c33 + 2 * c12 + 4 * c13) / 9 / Ks - phi * (1 - Ks / Kf))
+    rs, rf = rho_s * 1e-6, rho_f * 1e-6
+    rho = (1 - phi) * rs + phi * rf
+    cl, al, ao, Tl = (c11, a1, a3, T1) if axis == "x" else (c33, a3, a1, T3)
+    C = np.array([[cl + al * al * M, al * M], [al * M, M]])
+    R = np.array([[rho, rf], [rf, Tl * rf / phi]])
+    lam, vec = np.linalg.eig(np.linalg.solve(R, C))
+    g = np.linalg.solve(vec, np.linalg.solve(R, np.array([1.0, -1.0])))
+    c = np.sqrt(lam.real)
+    c0, mid = n_along // 2, n_across // 2
+    offs = [-15, 12]
+    if axis == "x":
+        comps = (B.VX, B.QX, B.TZZ)
+        recs = [(c0 + o, mid, comp) for o in offs for comp in comps]
+        cells = [(c0, j) for j in range(n_across)]
+    else:
+        comps = (B.VZ, B.QZ, B.TXX)
+        recs = [(mid, c0 + o, comp) for o in offs for comp in comps]
+        cells = [(i, c0) for i in range(n_across)]
+    nsteps = 500
+    tr, _ = _run_vti(model, h, dt, f0, nsteps, cells, recs)
+    t = (np.arange(nsteps) + 1) * dt
+    for k, o in enumerate(offs):
+        x = (o + 0.5) * h
+        want = np.zeros((3, nsteps))
+        for q in range(2):
+            s = _ricker_at(t - abs(x) / c[q], f0)
+            want[:2] += np.outer(vec[:, q].real, -np.sign(x) * g[q] * s / h / (2 * c[q] ** 2))
+            # the cross stress sits on the node, half a cell nearer the source
+            xs = o * h
+            s2 = _ricker_at(t - abs(xs) / c[q], f0)
+            want[2] += g[q] * ((c13 + a1 * a3 * M) * vec[0, q].real + ao * M * vec[1, q].real) * s2 / (
+                2 * c[q] ** 3 * h)
+        for comp in range(3):
+            got = tr[:, 3 * k + comp]
+            assert rel_l2(got, want[comp]) < 5e-3, (axis, o, comp, rel_l2(got, want[comp]))
