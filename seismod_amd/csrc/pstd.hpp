@@ -218,17 +218,25 @@ inline bool fft_factor(int n, int &nr, uint64_t &code)
         code |= (uint64_t)q << (4 * nr);
         ++nr;
     };
-    while (m % 16 == 0 && nr < FFT_MAXPASS) { push(16); m /= 16; }
-    if (m % 8 == 0) { push(8); m /= 8; }
-    else if (m % 4 == 0) { push(4); m /= 4; }
-    else if (m % 2 == 0) { push(2); m /= 2; }
-    for (int r : {3, 5, 7, 11, 13})
+    // odd radices first: the first Stockham pass writes with a stride of R
+    // elements, which for a power of two puts a wave's lanes on a few LDS
+    // banks; later passes write runs of Ns consecutive elements
+    int two = 0;
+    while (m % 2 == 0) { m /= 2; ++two; }
+    for (int r : {13, 11, 7, 5, 3})
         while (m % r == 0) {
             if (nr >= FFT_MAXPASS) return false;
             push(r);
             m /= r;
         }
-    return m == 1 && n >= 2 && nr > 0;
+    if (two % 4 == 3) push(8);
+    else if (two % 4 == 2) push(4);
+    else if (two % 4 == 1) push(2);
+    for (int q = 0; q < two / 4; ++q) {
+        if (nr >= FFT_MAXPASS) return false;
+        push(16);
+    }
+    return m == 1 && n >= 2 && nr > 0 && nr <= FFT_MAXPASS;
 }
 
 template <typename T>
@@ -244,10 +252,10 @@ __device__ __forceinline__ Cx<T> csub(Cx<T> a, Cx<T> b) { return {a.re - b.re, a
 template <typename T>
 __device__ __forceinline__ Cx<T> cmi(Cx<T> a) { return {a.im, -a.re}; }
 
-// forward DFT of v[0..R) in place, w = e^{-2 pi i / R}; tw holds
-// e^{-2 pi i k / n} and n is a multiple of R (w^m = tw[m n / R])
+// forward DFT of v[0..R) in place, w = e^{-2 pi i / R} = r[rs] (r holds
+// the roots of unity of the pass's radix; rs > 1 inside a larger radix)
 template <typename T, int R>
-__device__ __forceinline__ void dft(Cx<T> *v, const Cx<T> *tw, int n)
+__device__ __forceinline__ void dft(Cx<T> *v, const Cx<T> *r, int rs)
 {
     if constexpr (R == 2) {
         const Cx<T> a = v[0], b = v[1];
@@ -263,12 +271,11 @@ __device__ __forceinline__ void dft(Cx<T> *v, const Cx<T> *tw, int n)
         Cx<T> e[H], o[H];
 #pragma unroll
         for (int t = 0; t < H; ++t) { e[t] = v[2 * t]; o[t] = v[2 * t + 1]; }
-        dft<T, H>(e, tw, n);
-        dft<T, H>(o, tw, n);
-        const int step = n / R;
+        dft<T, H>(e, r, 2 * rs);
+        dft<T, H>(o, r, 2 * rs);
 #pragma unroll
         for (int k = 0; k < H; ++k) {
-            const Cx<T> wo = k == 0 ? o[k] : cmul(o[k], tw[k * step]);
+            const Cx<T> wo = k == 0 ? o[k] : cmul(o[k], r[k * rs]);
             v[k] = cadd(e[k], wo);
             v[k + H] = csub(e[k], wo);
         }
@@ -276,7 +283,6 @@ __device__ __forceinline__ void dft(Cx<T> *v, const Cx<T> *tw, int n)
         // odd R: pair t with R - t; X_s = A_s - i B_s, X_{R-s} = A_s + i B_s
         // with A_s = v0 + sum (v_t + v_{R-t}) cos, B_s = sum (v_t - v_{R-t}) sin
         constexpr int H = (R - 1) / 2;
-        const int step = n / R;
         Cx<T> sp[H], sm[H];
         Cx<T> x0 = v[0];
 #pragma unroll
@@ -292,7 +298,7 @@ __device__ __forceinline__ void dft(Cx<T> *v, const Cx<T> *tw, int n)
             Cx<T> A = v[0], B = {T(0), T(0)};
 #pragma unroll
             for (int t = 1; t <= H; ++t) {
-                const Cx<T> w = tw[((s * t) % R) * step];   // (cos, -sin)
+                const Cx<T> w = r[((s * t) % R) * rs];   // (cos, -sin)
                 A.re += sp[t - 1].re * w.re; A.im += sp[t - 1].im * w.re;
                 B.re -= sm[t - 1].re * w.im; B.im -= sm[t - 1].im * w.im;
             }
@@ -305,25 +311,28 @@ __device__ __forceinline__ void dft(Cx<T> *v, const Cx<T> *tw, int n)
 }
 
 // one Stockham pass of radix R over P transforms of length n (src -> dst);
-// Ns = product of the earlier radices
+// Ns = product of the earlier radices; twp[(t - 1) Ns + k] = e^{-2 pi i k t
+// / (Ns R)} (consecutive k on consecutive LDS banks), roots the R-th roots
+// of unity
 template <typename T, int R>
-__device__ __forceinline__ void fft_pass(const Cx<T> *src, Cx<T> *dst, int n, int P, int Ns, const Cx<T> *tw)
+__device__ __forceinline__ void fft_pass(const Cx<T> *src, Cx<T> *dst, int n, int P, int Ns, const Cx<T> *twp,
+                                         const Cx<T> *roots)
 {
     const int nb = n / R;
-    const int tws = n / (Ns * R);
+    const int ls = n + 1;   // LDS stride of one transform (odd: columns of p on different banks)
     for (int l = threadIdx.x; l < nb * P; l += blockDim.x) {
         const int p = l / nb, j = l - p * nb;
-        const Cx<T> *s = src + p * n;
-        Cx<T> *d = dst + p * n;
+        const Cx<T> *s = src + p * ls;
+        Cx<T> *d = dst + p * ls;
         const int k = j % Ns;
         Cx<T> v[R];
 #pragma unroll
         for (int t = 0; t < R; ++t) v[t] = s[j + t * nb];
         if (k) {
 #pragma unroll
-            for (int t = 1; t < R; ++t) v[t] = cmul(v[t], tw[k * t * tws]);
+            for (int t = 1; t < R; ++t) v[t] = cmul(v[t], twp[(t - 1) * Ns + k]);
         }
-        dft<T, R>(v, tw, n);
+        dft<T, R>(v, roots, 1);
         const int base = (j - k) * R + k;
 #pragma unroll
         for (int t = 0; t < R; ++t) d[base + t * Ns] = v[t];
@@ -340,17 +349,21 @@ __device__ __forceinline__ Cx<T> *fft_fwd(Cx<T> *a, Cx<T> *b, int n, int P, int 
     int Ns = 1;
     for (int q = 0; q < nr; ++q) {
         const int ri = (int)((code >> (4 * q)) & 15);
+        const int R = FFT_RADICES[ri];
+        const Cx<T> *roots = tw + (R - 1) * Ns;
         switch (ri) {
-        case 0: fft_pass<T, 2>(a, b, n, P, Ns, tw); Ns *= 2; break;
-        case 1: fft_pass<T, 3>(a, b, n, P, Ns, tw); Ns *= 3; break;
-        case 2: fft_pass<T, 4>(a, b, n, P, Ns, tw); Ns *= 4; break;
-        case 3: fft_pass<T, 5>(a, b, n, P, Ns, tw); Ns *= 5; break;
-        case 4: fft_pass<T, 7>(a, b, n, P, Ns, tw); Ns *= 7; break;
-        case 5: fft_pass<T, 8>(a, b, n, P, Ns, tw); Ns *= 8; break;
-        case 6: fft_pass<T, 11>(a, b, n, P, Ns, tw); Ns *= 11; break;
-        case 7: fft_pass<T, 13>(a, b, n, P, Ns, tw); Ns *= 13; break;
-        default: fft_pass<T, 16>(a, b, n, P, Ns, tw); Ns *= 16; break;
+        case 0: fft_pass<T, 2>(a, b, n, P, Ns, tw, roots); break;
+        case 1: fft_pass<T, 3>(a, b, n, P, Ns, tw, roots); break;
+        case 2: fft_pass<T, 4>(a, b, n, P, Ns, tw, roots); break;
+        case 3: fft_pass<T, 5>(a, b, n, P, Ns, tw, roots); break;
+        case 4: fft_pass<T, 7>(a, b, n, P, Ns, tw, roots); break;
+        case 5: fft_pass<T, 8>(a, b, n, P, Ns, tw, roots); break;
+        case 6: fft_pass<T, 11>(a, b, n, P, Ns, tw, roots); break;
+        case 7: fft_pass<T, 13>(a, b, n, P, Ns, tw, roots); break;
+        default: fft_pass<T, 16>(a, b, n, P, Ns, tw, roots); break;
         }
+        tw = roots + R;   // the next pass's table
+        Ns *= R;
         __syncthreads();
         Cx<T> *t = a; a = b; b = t;
     }
@@ -368,7 +381,7 @@ struct DfftArgs {
     uint32_t fields, slots, fwd;
     int njobs;
     const Cx<T> *tw, *mul_f, *mul_b;
-    int n, nr;
+    int n, nr, tsz;
     uint64_t code;
     int nx2, nz2;
     int P, ntiles, nblocks;
@@ -390,10 +403,11 @@ __global__ __launch_bounds__(512) void pstd_dfft(const DfftArgs<T> a)
     const T *f = a.x + (size_t)((a.fields >> (4 * jb)) & 15) * a.N;
     T *out = a.d + (size_t)((a.slots >> (4 * jb)) & 15) * a.N;
     const Cx<T> *mul = ((a.fwd >> jb) & 1) ? a.mul_f : a.mul_b;
-    const int n = a.n, P = a.P, tot = P * n;
-    // LDS: two buffers of P transforms, then the twiddles
-    Cx<T> *A = (Cx<T> *)pstd_lds, *B = A + tot, *tw = B + tot;
-    for (int l = threadIdx.x; l < n; l += blockDim.x) tw[l] = a.tw[l];
+    const int n = a.n, P = a.P, tot = P * n, ls = n + 1;
+    // LDS: two buffers of P transforms (stride n + 1), then the per-pass
+    // twiddles and roots
+    Cx<T> *A = (Cx<T> *)pstd_lds, *B = A + P * ls, *tw = B + P * ls;
+    for (int l = threadIdx.x; l < a.tsz; l += blockDim.x) tw[l] = a.tw[l];
     for (int l = threadIdx.x; l < tot; l += blockDim.x) {
         int p, k;
         if (AX == 1) { p = l / n; k = l - p * n; }
@@ -407,7 +421,7 @@ __global__ __launch_bounds__(512) void pstd_dfft(const DfftArgs<T> a)
             if (r0 < a.nz2) re = f[(size_t)k * a.nz2 + r0];
             if (r0 + 1 < a.nz2) im = f[(size_t)k * a.nz2 + r0 + 1];
         }
-        A[p * n + k] = {re, im};
+        A[p * ls + k] = {re, im};
     }
     __syncthreads();
     // Z = DFT(f); y = IDFT(M Z) = conj(DFT(conj(M Z)))
@@ -416,8 +430,9 @@ __global__ __launch_bounds__(512) void pstd_dfft(const DfftArgs<T> a)
         c = fft_fwd<T>(c, c == A ? B : A, n, P, a.nr, a.code, tw);
         if (rep == 1) break;
         for (int l = threadIdx.x; l < tot; l += blockDim.x) {
-            const Cx<T> z = cmul(c[l], mul[l % n]);
-            c[l] = {z.re, -z.im};
+            const int p = l / n, k = l - p * n;
+            const Cx<T> z = cmul(c[p * ls + k], mul[k]);
+            c[p * ls + k] = {z.re, -z.im};
         }
         __syncthreads();
     }
@@ -426,7 +441,7 @@ __global__ __launch_bounds__(512) void pstd_dfft(const DfftArgs<T> a)
         if (AX == 1) { p = l / n; k = l - p * n; }
         else { k = l / P; p = l - k * P; }
         const int r0 = 2 * (tile * P + p);
-        const Cx<T> y = c[p * n + k];
+        const Cx<T> y = c[p * ls + k];
         if (AX == 1) {
             if (r0 < a.nx2) out[(size_t)r0 * a.nz2 + k] = y.re;
             if (r0 + 1 < a.nx2) out[(size_t)(r0 + 1) * a.nz2 + k] = -y.im;
@@ -442,7 +457,22 @@ struct DJob {
     bool fwd;
 };
 
-inline size_t dfft_lds(int n, int P, size_t elt) { return (2 * (size_t)P + 1) * n * 2 * elt; }
+// complex entries of the per-pass tables: (R - 1) Ns twiddles and R roots
+inline int tw_count(int nr, uint64_t code)
+{
+    int c = 0, Ns = 1;
+    for (int q = 0; q < nr; ++q) {
+        const int R = FFT_RADICES[(code >> (4 * q)) & 15];
+        c += (R - 1) * Ns + R;
+        Ns *= R;
+    }
+    return c;
+}
+
+inline size_t dfft_lds(int n, int P, size_t elt, int tsz)
+{
+    return (2 * (size_t)P * (n + 1) + (size_t)tsz) * 2 * elt;
+}
 
 // threads of a pstd_dfft workgroup: enough for the largest pass in one
 // round (P n / R butterflies), at most 512
@@ -477,13 +507,13 @@ int deriv_batch(Grid &g, int axis, const T *x, T *d, const DJob *jobs, int nj)
     a.tw = (const Cx<T> *)g.tw[axis].ptr;
     a.mul_f = (const Cx<T> *)g.fmul[axis].ptr;
     a.mul_b = a.mul_f + n;
-    a.n = n; a.nr = g.nr[axis]; a.code = g.rcode[axis];
+    a.n = n; a.nr = g.nr[axis]; a.code = g.rcode[axis]; a.tsz = tw_count(a.nr, a.code);
     a.nx2 = g.nx2; a.nz2 = g.nz2;
     a.P = g.P[axis];
     a.ntiles = ((other + 1) / 2 + a.P - 1) / a.P;
     a.nblocks = nj * a.ntiles;
     const unsigned grid = (unsigned)((a.nblocks + 7) / 8 * 8);
-    const size_t lds = dfft_lds(n, a.P, sizeof(T));
+    const size_t lds = dfft_lds(n, a.P, sizeof(T), a.tsz);
     const unsigned nt = dfft_threads(n, a.P, a.nr, a.code);
     if (axis == 0) pstd_dfft<T, 0><<<grid, nt, lds, g.ctx->stream>>>(a);
     else pstd_dfft<T, 1><<<grid, nt, lds, g.ctx->stream>>>(a);
@@ -682,16 +712,29 @@ inline int init(Grid &g, sm_ctx *ctx, size_t nx2, size_t nz2, size_t np, size_t 
     for (int ax = 0; ax < 2 && rc == SM_OK && own; ++ax) {
         const size_t n = ax == 0 ? nx2 : nz2;
         const double d = ax == 0 ? dx : dz;
-        if (!fft_factor((int)n, g.nr[ax], g.rcode[ax]) || dfft_lds((int)n, 1, elt) > (size_t)lds_max) continue;
+        if (!fft_factor((int)n, g.nr[ax], g.rcode[ax])) continue;
+        const int tsz = tw_count(g.nr[ax], g.rcode[ax]);
+        if (dfft_lds((int)n, 1, elt, tsz) > (size_t)lds_max) continue;
         // x: 4 column pairs per workgroup (adjacent columns share cache
         // lines); z: 2 row pairs (more workgroups)
-        g.P[ax] = std::max(1, std::min(ax == 0 ? 4 : 2, (int)(lds_max / dfft_lds((int)n, 1, elt))));
-        while (g.P[ax] > 1 && dfft_lds((int)n, g.P[ax], elt) > (size_t)lds_max) --g.P[ax];
+        g.P[ax] = std::max(1, std::min(ax == 0 ? 4 : 2, (int)(lds_max / dfft_lds((int)n, 1, elt, tsz))));
+        while (g.P[ax] > 1 && dfft_lds((int)n, g.P[ax], elt, tsz) > (size_t)lds_max) --g.P[ax];
         std::vector<double> tw, fm;
         const double two_pi = 8.0 * std::atan(1.0);
-        for (size_t k = 0; k < n; ++k) {
-            const double th = two_pi * (double)k / (double)n;
-            tw.push_back(std::cos(th)); tw.push_back(-std::sin(th));
+        // per pass: e^{-2 pi i k t / (Ns R)} at (t - 1) Ns + k, then the
+        // R-th roots of unity
+        for (int q = 0, Ns = 1; q < g.nr[ax]; ++q) {
+            const int R = FFT_RADICES[(g.rcode[ax] >> (4 * q)) & 15];
+            for (int t = 1; t < R; ++t)
+                for (int k = 0; k < Ns; ++k) {
+                    const double th = two_pi * (double)k * t / ((double)Ns * R);
+                    tw.push_back(std::cos(th)); tw.push_back(-std::sin(th));
+                }
+            for (int m = 0; m < R; ++m) {
+                const double th = two_pi * (double)m / R;
+                tw.push_back(std::cos(th)); tw.push_back(-std::sin(th));
+            }
+            Ns *= R;
         }
         for (double sgn : {1.0, -1.0}) {
             const double dn = two_pi / (d * n * n);
@@ -737,8 +780,10 @@ inline int init(Grid &g, sm_ctx *ctx, size_t nx2, size_t nz2, size_t np, size_t 
 // device bytes of the machinery
 inline size_t grid_bytes(size_t nx2, size_t nz2, size_t np, size_t elt)
 {
+    // per axis: twiddles and roots (n - 1 + the sum of the radices) and two
+    // multiplier tables of n
     return spec_bytes(nx2, nz2, elt) + kmul_bytes(nx2, nz2, elt) + table_bytes(np, elt) +
-           3 * (nx2 + nz2) * 2 * elt;
+           (3 * (nx2 + nz2) + 2 * 16 * FFT_MAXPASS) * 2 * elt;
 }
 
 inline size_t grid_device_bytes(const Grid &g)
