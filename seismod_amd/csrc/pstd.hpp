@@ -310,61 +310,131 @@ __device__ __forceinline__ void dft(Cx<T> *v, const Cx<T> *r, int rs)
     }
 }
 
-// one Stockham pass of radix R over P transforms of length n (src -> dst);
-// Ns = product of the earlier radices; twp[(t - 1) Ns + k] = e^{-2 pi i k t
-// / (Ns R)} (consecutive k on consecutive LDS banks), roots the R-th roots
-// of unity
-template <typename T, int R>
-__device__ __forceinline__ void fft_pass(const Cx<T> *src, Cx<T> *dst, int n, int P, int Ns, const Cx<T> *twp,
+// where a pass reads and writes: the LDS buffers (transform p at p ls), or,
+// for the first forward and the last inverse pass, the field f and the
+// derivative out in global memory (pair p: rows or columns r0, r0 + 1 with
+// r0 = r0base + 2p, a missing partner read as 0 and not written); with mul,
+// the pass writes conj(mul[k] v) (the multiplier between the transforms)
+template <typename T>
+struct PassIO {
+    const Cx<T> *src;
+    Cx<T> *dst;
+    const T *f;
+    T *out;
+    const Cx<T> *mul;
+    int ls, r0base, lim, nz2;
+    bool in_g, out_g;
+};
+
+template <typename T, int AX>
+__device__ __forceinline__ Cx<T> gload(const PassIO<T> io, int p, int k)
+{
+    const int r0 = io.r0base + 2 * p;
+    T re = T(0), im = T(0);
+    if (AX == 1) {
+        if (r0 < io.lim) re = io.f[(size_t)r0 * io.nz2 + k];
+        if (r0 + 1 < io.lim) im = io.f[(size_t)(r0 + 1) * io.nz2 + k];
+    } else {
+        if (r0 < io.lim) re = io.f[(size_t)k * io.nz2 + r0];
+        if (r0 + 1 < io.lim) im = io.f[(size_t)k * io.nz2 + r0 + 1];
+    }
+    return {re, im};
+}
+
+template <typename T, int AX>
+__device__ __forceinline__ void gstore(const PassIO<T> io, int p, int k, Cx<T> y)
+{
+    const int r0 = io.r0base + 2 * p;
+    if (AX == 1) {
+        if (r0 < io.lim) io.out[(size_t)r0 * io.nz2 + k] = y.re;
+        if (r0 + 1 < io.lim) io.out[(size_t)(r0 + 1) * io.nz2 + k] = -y.im;
+    } else {
+        if (r0 < io.lim) io.out[(size_t)k * io.nz2 + r0] = y.re;
+        if (r0 + 1 < io.lim) io.out[(size_t)k * io.nz2 + r0 + 1] = -y.im;
+    }
+}
+
+// one Stockham pass of radix R over P transforms of length n; Ns = product
+// of the earlier radices; twp[(t - 1) Ns + k] = e^{-2 pi i k t / (Ns R)}
+// (consecutive k on consecutive LDS banks), roots the R-th roots of unity.
+// Passes that touch global memory along x take the pairs fastest (2P
+// adjacent values of a row per lane group).
+template <typename T, int R, int AX>
+__device__ __forceinline__ void fft_pass(const PassIO<T> io, int n, int P, int Ns, const Cx<T> *twp,
                                          const Cx<T> *roots)
 {
-    const int nb = n / R;
-    const int ls = n + 1;   // LDS stride of one transform (odd: columns of p on different banks)
+    const int nb = n / R, ls = io.ls;
+    const bool pf = AX == 0 && (io.in_g || io.out_g);
     for (int l = threadIdx.x; l < nb * P; l += blockDim.x) {
-        const int p = l / nb, j = l - p * nb;
-        const Cx<T> *s = src + p * ls;
-        Cx<T> *d = dst + p * ls;
+        int p, j;
+        if (pf) { j = l / P; p = l - j * P; }
+        else { p = l / nb; j = l - p * nb; }
         const int k = j % Ns;
         Cx<T> v[R];
+        if (io.in_g) {
 #pragma unroll
-        for (int t = 0; t < R; ++t) v[t] = s[j + t * nb];
+            for (int t = 0; t < R; ++t) v[t] = gload<T, AX>(io, p, j + t * nb);
+        } else {
+            const Cx<T> *s = io.src + p * ls;
+#pragma unroll
+            for (int t = 0; t < R; ++t) v[t] = s[j + t * nb];
+        }
         if (k) {
 #pragma unroll
             for (int t = 1; t < R; ++t) v[t] = cmul(v[t], twp[(t - 1) * Ns + k]);
         }
         dft<T, R>(v, roots, 1);
         const int base = (j - k) * R + k;
+        if (io.mul) {
 #pragma unroll
-        for (int t = 0; t < R; ++t) d[base + t * Ns] = v[t];
+            for (int t = 0; t < R; ++t) {
+                const Cx<T> z = cmul(v[t], io.mul[base + t * Ns]);
+                v[t] = {z.re, -z.im};
+            }
+        }
+        if (io.out_g) {
+#pragma unroll
+            for (int t = 0; t < R; ++t) gstore<T, AX>(io, p, base + t * Ns, v[t]);
+        } else {
+            Cx<T> *d = io.dst + p * ls;
+#pragma unroll
+            for (int t = 0; t < R; ++t) d[base + t * Ns] = v[t];
+        }
     }
 }
 
-// forward FFT of the P transforms in a (scratch b); returns the buffer
-// holding the result, after a barrier.  Inlined at one call site: an
-// out-of-line call would keep a stack frame in scratch memory.
-template <typename T>
-__device__ __forceinline__ Cx<T> *fft_fwd(Cx<T> *a, Cx<T> *b, int n, int P, int nr, uint64_t code,
-                                          const Cx<T> *tw)
+// a forward FFT of P transforms: from global memory (in_g) or LDS buffer a
+// (scratch b), into LDS (returned, after a barrier) or global memory
+// (out_g), the last pass applying mul if given.  Inlined: an out-of-line
+// call would keep a stack frame in scratch memory.
+template <typename T, int AX>
+__device__ __forceinline__ Cx<T> *fft_run(PassIO<T> io, Cx<T> *a, Cx<T> *b, int n, int P, int nr, uint64_t code,
+                                          const Cx<T> *tw, bool in_g, bool out_g, const Cx<T> *mul)
 {
     int Ns = 1;
     for (int q = 0; q < nr; ++q) {
         const int ri = (int)((code >> (4 * q)) & 15);
         const int R = FFT_RADICES[ri];
         const Cx<T> *roots = tw + (R - 1) * Ns;
+        io.src = a;
+        io.dst = b;
+        io.in_g = in_g && q == 0;
+        io.out_g = out_g && q == nr - 1;
+        io.mul = q == nr - 1 ? mul : nullptr;
         switch (ri) {
-        case 0: fft_pass<T, 2>(a, b, n, P, Ns, tw, roots); break;
-        case 1: fft_pass<T, 3>(a, b, n, P, Ns, tw, roots); break;
-        case 2: fft_pass<T, 4>(a, b, n, P, Ns, tw, roots); break;
-        case 3: fft_pass<T, 5>(a, b, n, P, Ns, tw, roots); break;
-        case 4: fft_pass<T, 7>(a, b, n, P, Ns, tw, roots); break;
-        case 5: fft_pass<T, 8>(a, b, n, P, Ns, tw, roots); break;
-        case 6: fft_pass<T, 11>(a, b, n, P, Ns, tw, roots); break;
-        case 7: fft_pass<T, 13>(a, b, n, P, Ns, tw, roots); break;
-        default: fft_pass<T, 16>(a, b, n, P, Ns, tw, roots); break;
+        case 0: fft_pass<T, 2, AX>(io, n, P, Ns, tw, roots); break;
+        case 1: fft_pass<T, 3, AX>(io, n, P, Ns, tw, roots); break;
+        case 2: fft_pass<T, 4, AX>(io, n, P, Ns, tw, roots); break;
+        case 3: fft_pass<T, 5, AX>(io, n, P, Ns, tw, roots); break;
+        case 4: fft_pass<T, 7, AX>(io, n, P, Ns, tw, roots); break;
+        case 5: fft_pass<T, 8, AX>(io, n, P, Ns, tw, roots); break;
+        case 6: fft_pass<T, 11, AX>(io, n, P, Ns, tw, roots); break;
+        case 7: fft_pass<T, 13, AX>(io, n, P, Ns, tw, roots); break;
+        default: fft_pass<T, 16, AX>(io, n, P, Ns, tw, roots); break;
         }
         tw = roots + R;   // the next pass's table
         Ns *= R;
-        __syncthreads();
+        if (!io.out_g) __syncthreads();
         Cx<T> *t = a; a = b; b = t;
     }
     return a;
@@ -403,53 +473,21 @@ __global__ __launch_bounds__(512) void pstd_dfft(const DfftArgs<T> a)
     const T *f = a.x + (size_t)((a.fields >> (4 * jb)) & 15) * a.N;
     T *out = a.d + (size_t)((a.slots >> (4 * jb)) & 15) * a.N;
     const Cx<T> *mul = ((a.fwd >> jb) & 1) ? a.mul_f : a.mul_b;
-    const int n = a.n, P = a.P, tot = P * n, ls = n + 1;
+    const int n = a.n, P = a.P, ls = n + 1;
     // LDS: two buffers of P transforms (stride n + 1), then the per-pass
     // twiddles and roots
     Cx<T> *A = (Cx<T> *)pstd_lds, *B = A + P * ls, *tw = B + P * ls;
     for (int l = threadIdx.x; l < a.tsz; l += blockDim.x) tw[l] = a.tw[l];
-    for (int l = threadIdx.x; l < tot; l += blockDim.x) {
-        int p, k;
-        if (AX == 1) { p = l / n; k = l - p * n; }
-        else { k = l / P; p = l - k * P; }
-        const int r0 = 2 * (tile * P + p);
-        T re = T(0), im = T(0);
-        if (AX == 1) {
-            if (r0 < a.nx2) re = f[(size_t)r0 * a.nz2 + k];
-            if (r0 + 1 < a.nx2) im = f[(size_t)(r0 + 1) * a.nz2 + k];
-        } else {
-            if (r0 < a.nz2) re = f[(size_t)k * a.nz2 + r0];
-            if (r0 + 1 < a.nz2) im = f[(size_t)k * a.nz2 + r0 + 1];
-        }
-        A[p * ls + k] = {re, im};
-    }
     __syncthreads();
-    // Z = DFT(f); y = IDFT(M Z) = conj(DFT(conj(M Z)))
+    PassIO<T> io;
+    io.f = f; io.out = out; io.ls = ls; io.r0base = 2 * tile * P;
+    io.lim = AX == 1 ? a.nx2 : a.nz2; io.nz2 = a.nz2;
+    // Z = DFT(f) from global memory, written as conj(M Z); y = IDFT(M Z) =
+    // conj(DFT(conj(M Z))) into global memory (one call site: inlined once)
     Cx<T> *c = A;
-    for (int rep = 0; rep < 2; ++rep) {
-        c = fft_fwd<T>(c, c == A ? B : A, n, P, a.nr, a.code, tw);
-        if (rep == 1) break;
-        for (int l = threadIdx.x; l < tot; l += blockDim.x) {
-            const int p = l / n, k = l - p * n;
-            const Cx<T> z = cmul(c[p * ls + k], mul[k]);
-            c[p * ls + k] = {z.re, -z.im};
-        }
-        __syncthreads();
-    }
-    for (int l = threadIdx.x; l < tot; l += blockDim.x) {
-        int p, k;
-        if (AX == 1) { p = l / n; k = l - p * n; }
-        else { k = l / P; p = l - k * P; }
-        const int r0 = 2 * (tile * P + p);
-        const Cx<T> y = c[p * ls + k];
-        if (AX == 1) {
-            if (r0 < a.nx2) out[(size_t)r0 * a.nz2 + k] = y.re;
-            if (r0 + 1 < a.nx2) out[(size_t)(r0 + 1) * a.nz2 + k] = -y.im;
-        } else {
-            if (r0 < a.nz2) out[(size_t)k * a.nz2 + r0] = y.re;
-            if (r0 + 1 < a.nz2) out[(size_t)k * a.nz2 + r0 + 1] = -y.im;
-        }
-    }
+    for (int rep = 0; rep < 2; ++rep)
+        c = fft_run<T, AX>(io, c, c == A ? B : A, n, P, a.nr, a.code, tw, rep == 0, rep == 1,
+                           rep == 0 ? mul : nullptr);
 }
 
 struct DJob {
@@ -715,9 +753,10 @@ inline int init(Grid &g, sm_ctx *ctx, size_t nx2, size_t nz2, size_t np, size_t 
         if (!fft_factor((int)n, g.nr[ax], g.rcode[ax])) continue;
         const int tsz = tw_count(g.nr[ax], g.rcode[ax]);
         if (dfft_lds((int)n, 1, elt, tsz) > (size_t)lds_max) continue;
-        // x: 4 column pairs per workgroup (adjacent columns share cache
-        // lines); z: 2 row pairs (more workgroups)
-        g.P[ax] = std::max(1, std::min(ax == 0 ? 4 : 2, (int)(lds_max / dfft_lds((int)n, 1, elt, tsz))));
+        // 4 column or row pairs per workgroup where LDS allows (adjacent
+        // x columns share cache lines; measured on 1040-point axes, FP32:
+        // 4/4 pairs 9% faster per step than 4/2, 2/2 and 1/1 slower still)
+        g.P[ax] = std::max(1, std::min(4, (int)(lds_max / dfft_lds((int)n, 1, elt, tsz))));
         while (g.P[ax] > 1 && dfft_lds((int)n, g.P[ax], elt, tsz) > (size_t)lds_max) --g.P[ax];
         std::vector<double> tw, fm;
         const double two_pi = 8.0 * std::atan(1.0);
