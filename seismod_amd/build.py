@@ -103,10 +103,18 @@ def build_oracle() -> Path:
     return out
 
 
+def build_ref() -> Path | None:
+    """oracle/_ref: the reference's own sources that build alone (checkers
+    of the tests), when the reference is present."""
+    import importlib
+    return importlib.import_module("oracle.build_ref").build()
+
+
 def build_all() -> None:
     build_lib()
     build_host()
     build_oracle()
+    build_ref()
 
 
 if __name__ == "__main__":

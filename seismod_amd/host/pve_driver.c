@@ -13,12 +13,12 @@
  * Sf, Bulk_s; 1- or 9-point template) and record (Vx, Vz, qx, qz, txx, tzz,
  * txz, p, and for the isotropic program div and curl) files, and the same
  * outputs: trc/, ssh/, segy/, the kinetic energy of a region in
- * E/<base>_E.dat (saveEnergy, EnergyROI) and, for the isotropic program, the
- * phase velocity and attenuation check (check_model: 1 prints the
- * extremes, 2 also writes <base>_vel_slowP.nc ... <base>_att_S.nc).
- * Checkpoints (chkpt_inc, -c), the median smoothing of the layers
- * (pml_median) and the VTI program's check_model are not available: the
- * program says so and stops.
+ * E/<base>_E.dat (saveEnergy, EnergyROI) and the phase velocity and
+ * attenuation check (check_model: 1 prints the extremes, 2 also writes
+ * <base>_vel_slowP.nc ... <base>_att_S.nc, or for the VTI program
+ * <base>_vel_qP1_x.nc ... <base>_att_qS_z.nc).  Checkpoints (chkpt_inc,
+ * -c) and the median smoothing of the layers (pml_median) are not
+ * available: the program says so and stops.
  *
  * Set-up and update arithmetic live in the library (sm_pve_*); this driver
  * reads the inputs, drives the time loop and writes the records.
@@ -26,6 +26,7 @@
  * Environment: SEISMOD_STORAGE=f32 runs the FP32 kernels; the default is
  * FP64 (the reference's precision).  SEISMOD_DEVICE selects the GPU.
  */
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -69,8 +70,8 @@ static const char *usage_vti =
     "can be added after the comma, i.e we have on each line\n\n"
     "value     # keyword, optional comment\n\n"
     "Keywords: model, source, output, basename, time (s), dt (ms), segy, shotpoint,\n"
-    "simulateCMP, abs, nl, plstr, kappa_max, alpha_max, Rc, saveEnergy, EnergyROI\n"
-    "(xmin/xmax/zmin/zmax)\n\n"
+    "simulateCMP, abs, nl, plstr, kappa_max, alpha_max, Rc, check_model (0, 1 display,\n"
+    "2 store), saveEnergy, EnergyROI (xmin/xmax/zmin/zmax)\n\n"
     "Model file: nx nz / dx dz / x0 z0 / anisotropic_vti (or anisotropic__vti_pml: the\n"
     "absorbing layers included, nx nz still the interior nodes) / then one line per\n"
     "node, x outer and z inner: K_m[GPa] K_s[GPa] K_f[GPa] phi mu[GPa] rho_s[kg/m3]\n"
@@ -146,6 +147,70 @@ static void check_model(const sm_params *p, const sm_grid *g, size_t np, double 
     }
 }
 
+/* pve_vti_pml's check_model (pve_vti_pml.c:931-1100): a warning when the
+ * frame is stiffer than the solid (alpha1 or alpha3 < 0), then the extremes
+ * of the qP1, qP2 and qS phase velocities and attenuation along x and z at
+ * the source frequency; check_model 2 also writes them */
+static void check_model_vti(const sm_params *p, const sm_grid *g, size_t np, double *const v[V_N], double f0)
+{
+    const size_t nx2 = g->nx + 2 * np, nz2 = g->nz + 2 * np, n = nx2 * nz2;
+    for (size_t c = 0; c < n; ++c) {
+        const double K_s = v[V_KS][c] * 1000., mu = v[V_MU][c] * 1000., c33 = v[V_KM][c] * 1000. + 4. / 3. * mu;
+        const double c11 = 2. * v[V_EPS][c] * c33 + c33, c12 = c11 - 2. * mu;
+        const double c13 = sqrt(2. * v[V_DELTA][c] * c33 * (c33 - mu) + (c33 - mu) * (c33 - mu)) - mu;
+        if (1. - (c11 + c12 + c13) / (3. * K_s) < 0.0 || 1. - (2. * c13 + c33) / (3. * K_s) < 0.0) {
+            fprintf(stdout, "\n***\nWarning: bulk modulus of drained matrix larger than modulus of the solid\n***\n");
+            break;
+        }
+    }
+    if (sm_verbose) {
+        fprintf(stdout, "done.\nComputing phase velocity and attenuation ... ");
+        fflush(stdout);
+    }
+    double *vel[6], *att[6];
+    for (int k = 0; k < 6; ++k) { vel[k] = malloc(n * sizeof(double)); att[k] = malloc(n * sizeof(double)); }
+    sm_pve_dispersion_vti(nx2, nz2, v, f0, vel, att);
+    double ext[12][2];   /* vel then att of qP1_x .. qS_z: max, min */
+    for (int k = 0; k < 12; ++k) {
+        const double *a = k < 6 ? vel[k] : att[k - 6];
+        ext[k][0] = -DBL_MAX; ext[k][1] = DBL_MAX;
+        for (size_t c = 0; c < n; ++c) {
+            if (a[c] > ext[k][0]) ext[k][0] = a[c];
+            if (a[c] < ext[k][1]) ext[k][1] = a[c];
+        }
+    }
+    if (sm_verbose) {
+        fprintf(stdout, "done.\n");
+        for (int ax = 0; ax < 2; ++ax)
+            for (int w = 0; w < 2; ++w) {
+                const int b = 6 * w + 3 * ax;   /* qP1, qP2, qS of this axis */
+                fprintf(stdout, w == 0 ? "Max/min phase velocities in %s:\n" : "Max/min attenuation coefficient in %s:\n",
+                        ax == 0 ? "X" : "Z");
+                fprintf(stdout, "   Src frequency (%lg Hz)   - slow P : %lg    %lg\n", f0, ext[b + 1][0], ext[b + 1][1]);
+                fprintf(stdout, "                             fast P : %lg    %lg\n", ext[b][0], ext[b][1]);
+                fprintf(stdout, "                                  S : %lg    %lg\n", ext[b + 2][0], ext[b + 2][1]);
+            }
+        fflush(stdout);
+    }
+    if (p->check_model > 1) {
+        if (sm_verbose > 1) {
+            fprintf(stdout, "  Saving phase velocity and attenuation ...");
+            fflush(stdout);
+        }
+        const char *names[6] = {"qP1_x", "qP2_x", "qS_x", "qP1_z", "qP2_z", "qS_z"};
+        char name[32];
+        for (int k = 0; k < 6; ++k) {
+            snprintf(name, sizeof(name), "vel_%s", names[k]);
+            sm_write_field_nc(vel[k], name, "m/s", g->nx, g->nz, np, g->x0, g->z0, g->dx, g->dz, p->basename, 1);
+        }
+        for (int k = 0; k < 6; ++k) {
+            snprintf(name, sizeof(name), "att_%s", names[k]);
+            sm_write_field_nc(att[k], name, "1/m", g->nx, g->nz, np, g->x0, g->z0, g->dx, g->dz, p->basename, 1);
+        }
+    }
+    for (int k = 0; k < 6; ++k) { free(vel[k]); free(att[k]); }
+}
+
 int sm_pve_program(int argc, char *argv[], int vti)
 {
     sm_params p;
@@ -159,10 +224,6 @@ int sm_pve_program(int argc, char *argv[], int vti)
     if (p.pml_median) {
         fprintf(stderr, "Error: median smoothing of the absorbing layers (pml_median) is not available in "
                         "this %s\n", prog);
-        return 1;
-    }
-    if (vti && p.check_model > 0) {
-        fprintf(stderr, "Error: the phase velocity check (check_model) is not available in this %s\n", prog);
         return 1;
     }
     const size_t np = p.np;   /* 0 when abs is 0 */
@@ -276,7 +337,10 @@ int sm_pve_program(int argc, char *argv[], int vti)
     }
 
     if (sm_verbose) { fprintf(stdout, "Computing CPML parameters ... done.\nPreprocessing data ... "); fflush(stdout); }
-    if (p.check_model > 0 && !vti) check_model(&p, &g, np, v, f0);
+    if (p.check_model > 0) {
+        if (vti) check_model_vti(&p, &g, np, v, f0);
+        else check_model(&p, &g, np, v, f0);
+    }
     const char *dev = getenv("SEISMOD_DEVICE");
     sm_ctx *ctx = NULL;
     check(sm_ctx_create(dev ? atoi(dev) : 0, &ctx), "sm_ctx_create");

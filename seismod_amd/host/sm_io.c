@@ -730,6 +730,70 @@ void sm_pve_dispersion(size_t n, double *const v[12], double f, double *const ve
     }
 }
 
+/* phase velocities of the fast (vf) and slow (vs) P waves of a Biot medium
+ * along one axis: det(C - V^2 R) = 0 with the undrained stiffness
+ * C = [[c11, c12], [c12, c22]] and the density matrix R = [[rho, rho_f],
+ * [rho_f, Y]], Y = m - i (eta/kappa)/omega */
+static void biot_axis(double c11, double c12, double c22, double rho, double rho_f, double complex Y,
+                      double complex *vf, double complex *vs)
+{
+    const double complex a = rho * Y - rho_f * rho_f;
+    const double complex b = -(c11 * Y + c22 * rho - 2. * c12 * rho_f);
+    const double c = c11 * c22 - c12 * c12;
+    const double complex d = csqrt(b * b - 4. * a * c);
+    const double complex v1 = csqrt((-b + d) / (2. * a)), v2 = csqrt((-b - d) / (2. * a));
+    if (creal(v1) >= creal(v2)) { *vf = v1; *vs = v2; }
+    else { *vf = v2; *vs = v1; }
+}
+
+void sm_pve_dispersion_vti(size_t nx2, size_t nz2, double *const v[16], double f, double *const vel[6],
+                           double *const att[6])
+{
+    /* the VTI medium's node material (pve_vti_pml.c:787-829), the
+     * permeability terms and c55 at their staggered positions as the
+     * reference has them when it runs the check (837-929), the undrained
+     * stiffness c + alpha alpha M, and plane waves along x and z: qP from
+     * biot_axis with the axis' stiffness, alpha, tortuosity and
+     * permeability; qS polarised across the axis, V^2 = c55 / (rho -
+     * rho_f^2 / Y) with the Y of the polarisation axis */
+    const double pi = 4.0 * atan(1.0), omega = 2. * pi * f;
+    const size_t N = nx2 * nz2;
+    for (size_t c = 0; c < N; ++c) {
+        const size_t i = c / nz2, j = c % nz2;
+        const double K_m = v[0][c] * 1000., K_s = v[1][c] * 1000., K_f = v[2][c] * 1000., phi = v[3][c];
+        const double mu = v[4][c] * 1000., rho_s = v[5][c] * 1.e-6, rho_f = v[6][c] * 1.e-6;
+        const double D = K_s * (1. + phi * (K_s / K_f - 1.));
+        const double c33 = K_m + 4. / 3. * mu, c11 = 2. * v[14][c] * c33 + c33, c12 = c11 - 2. * mu;
+        const double c13 = sqrt(2. * v[15][c] * c33 * (c33 - mu) + (c33 - mu) * (c33 - mu)) - mu;
+        const double M = K_s * K_s / (D - (2. * c11 + c33 + 2. * c12 + 4. * c13) / 9.);
+        const double a1 = 1. - 1. / 3. * (c11 + c12 + c13) / K_s, a3 = 1. - 1. / 3. * (2. * c13 + c33) / K_s;
+        const double rho = (1. - phi) * rho_s + phi * rho_f;
+        const double m1 = v[7][c] * rho_f / phi, m3 = v[8][c] * rho_f / phi;
+        /* eta / kappa averaged to (i+1/2, j) and (i, j+1/2), the last row or
+         * column copied from its neighbour */
+        const size_t ia = i + 1 < nx2 ? i : i - 1, ja = j + 1 < nz2 ? j : j - 1;
+        const size_t a = ia * nz2 + j, b = a + nz2, a3i = i * nz2 + ja, b3 = a3i + 1;
+        const double E = 1.013249965828145e6;
+        const double nk1 = 0.5 * (v[9][a] * E / v[10][a] + v[9][b] * E / v[10][b]);
+        const double nk3 = 0.5 * (v[9][a3i] * E / v[11][a3i] + v[9][b3] * E / v[11][b3]);
+        /* harmonic mean of mu at (i+1/2, j+1/2), last row and column copied */
+        const size_t ic = i + 1 < nx2 ? i : i - 1, jc = j + 1 < nz2 ? j : j - 1;
+        const size_t q00 = ic * nz2 + jc;
+        const double c55 = 4.0 / (1. / (v[4][q00] * 1000.) + 1. / (v[4][q00 + nz2] * 1000.) +
+                                  1. / (v[4][q00 + 1] * 1000.) + 1. / (v[4][q00 + nz2 + 1] * 1000.));
+        const double complex Y1 = m1 - I * nk1 / omega, Y3 = m3 - I * nk3 / omega;
+        double complex V[6];
+        biot_axis(c11 + a1 * a1 * M, a1 * M, M, rho, rho_f, Y1, &V[0], &V[1]);
+        V[2] = csqrt(c55 / (rho - rho_f * rho_f / Y3));
+        biot_axis(c33 + a3 * a3 * M, a3 * M, M, rho, rho_f, Y3, &V[3], &V[4]);
+        V[5] = csqrt(c55 / (rho - rho_f * rho_f / Y1));
+        for (int k = 0; k < 6; ++k) {
+            vel[k][c] = 1. / creal(1. / V[k]);
+            att[k][c] = -omega * cimag(1. / V[k]);
+        }
+    }
+}
+
 void sm_read_model_a(const char *fn, const sm_grid *g, size_t np, double *vp, double *rho)
 {
     /* io_utils.c:1029-1153 */

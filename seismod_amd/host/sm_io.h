@@ -140,6 +140,13 @@ void sm_read_model_pve_vti(const char *fn, const sm_grid *g, size_t np, int with
  * and densities of the poroviscoelastic model (pve_iso_pml.c:864-904):
  * vel[0..2] and att[0..2] receive n values each (slow P, fast P, S). */
 void sm_pve_dispersion(size_t n, double *const vals[12], double f, double *const vel[3], double *const att[3]);
+/* pve_vti_pml's check_model (pve_vti_pml.c:931-1100): phase velocity (m/s)
+ * and attenuation (-omega Im 1/V) of the fast and slow qP and the qS waves
+ * along x and along z at frequency f, for the padded nx2 x nz2 grid of the
+ * 16 VTI model values (sm_read_model_pve_vti's order); vel and att receive
+ * qP1_x, qP2_x, qS_x, qP1_z, qP2_z, qS_z */
+void sm_pve_dispersion_vti(size_t nx2, size_t nz2, double *const vals[16], double f, double *const vel[6],
+                           double *const att[6]);
 void sm_read_output3d(const char *fn, const sm_grid *g, size_t np, sm_records *out);
 /* check_dt_trc (io_utils.c:3561-3574) */
 void sm_check_dt_trc(const sm_records *out, double dt);

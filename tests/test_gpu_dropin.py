@@ -631,8 +631,7 @@ def deck_pve(tmp, nx=44, nz=38, np_=8, h=5.0, dt_ms=0.25, time_s=0.05, template=
     (tmp / "run.par").write_text(
         "model.dat # model, file\nsrc.dat # source, file\nrcv.dat # output, records\n"
         f"pve # basename, name\n{time_s} # time, s\n{dt_ms} # dt, ms\n1 # abs, on\n{np_} # nl, layers\n"
-        "3 # kappa_max, PML\n1 # plstr, strips\n" + ("" if vti else "2 # check_model, store\n") +
-        "1 # saveEnergy, E(t)\n"
+        "3 # kappa_max, PML\n1 # plstr, strips\n2 # check_model, store\n1 # saveEnergy, E(t)\n"
         f"{roi[0]}/{roi[1]}/{roi[2]}/{roi[3]} # EnergyROI, region\n")
     nz2 = nz + 2 * np_
     cell = lambda x, z: (int(round(x / h)) + np_) * nz2 + int(round(z / h)) + np_
@@ -728,10 +727,12 @@ def test_pve_vti_pml_program(tmp_path, template, with_pml):
     """The drop-in pve_vti_pml (FP64) against the oracle: every trace line,
     the p snapshot and the kinetic energy of the region (eq. 7.171 of
     Carcione 2007 with the x and z tortuosities, pve_vti_pml.c:1112-1120 and
-    1334-1345)."""
+    1334-1345) and the check_model files along x and z (pve_vti_pml.c:
+    931-1100)."""
     from seismod_amd.solver import nsteps as sm_nsteps, record_due
     d = deck_pve(tmp_path, template=template, with_pml=with_pml, vti=True)
-    run("pve_vti_pml", tmp_path, "f64")
+    r = run("pve_vti_pml", tmp_path, "f64")
+    assert "Max/min phase velocities in X" in r.stdout and "Max/min attenuation coefficient in Z" in r.stdout
     ns = sm_nsteps(0.05, d["dt"])
     nz2 = d["nz"] + 2 * d["np"]
     p = d["np"]
@@ -759,3 +760,9 @@ def test_pve_vti_pml_program(tmp_path, template, with_pml):
     z = nc_field(snaps[0])
     assert z.shape == want.shape
     assert np.abs(z - want).max() <= 1e-5 * np.abs(want).max()
+    import test_host_io as H
+    vel, att = H._dispersion_vti(list(d["model"]), 60.0)
+    for k, nm in enumerate(("qP1_x", "qP2_x", "qS_x", "qP1_z", "qP2_z", "qS_z")):
+        for pre, arr in (("vel", vel), ("att", att)):
+            z = nc_field(tmp_path / f"pve_{pre}_{nm}.nc")
+            assert np.array_equal(z, arr[k].T.astype(np.float32).astype(np.float64)), (pre, nm)

@@ -810,11 +810,120 @@ def test_read_model_pve_vti_layout_and_padding(tmp_path, kind):
     assert np.array_equal(out[7], pad(m[7] / m[15]))      # T1
 
 
-def test_pve_vti_pml_refuses_check_model(tmp_path):
-    (tmp_path / "run.par").write_text("model.dat # model,\nsrc.dat # source,\nrcv.dat # output,\n"
-                                      "1 # check_model, display\n")
-    r = run_bin("pve_vti_pml", "-p", "run.par", cwd=tmp_path)
-    assert r.returncode == 1 and "check_model" in r.stderr, r.stderr
-    assert run_bin("pve_vti_pml", "-h").returncode == 0
+def test_pve_vti_pml_usage():
     r = run_bin("pve_vti_pml", "-h")
-    assert "anisotropic_vti" in r.stdout
+    assert r.returncode == 0 and "anisotropic_vti" in r.stdout
+
+
+def _vti_model(rng, shp, uniform=False):
+    vals = list(_pve_values(rng, *shp))
+    T3, K3 = rng.uniform(1.5, 3, shp), rng.uniform(100, 2000, shp)
+    eps, dl = rng.uniform(0, 0.3, shp), rng.uniform(-0.05, 0.2, shp)
+    # K_m K_s K_f phi mu rho_s rho_f T1 T3 eta kappa1 kappa3 Q f0 epsilon delta
+    return [vals[0], vals[1], vals[2], vals[3], vals[4], vals[5], vals[6], vals[7], T3, vals[8], vals[9], K3,
+            vals[10], vals[11], eps, dl]
+
+
+def _dispersion_vti(v16, f):
+    v16 = [np.ascontiguousarray(a, np.float64) for a in v16]
+    nx2, nz2 = v16[0].shape
+    vel = [np.zeros((nx2, nz2)) for _ in range(6)]
+    att = [np.zeros((nx2, nz2)) for _ in range(6)]
+    LIB.sm_pve_dispersion_vti(C.c_size_t(nx2), C.c_size_t(nz2),
+                              (C.c_void_p * 16)(*(a.ctypes.data_as(C.c_void_p) for a in v16)), C.c_double(f),
+                              (C.c_void_p * 6)(*(a.ctypes.data_as(C.c_void_p) for a in vel)),
+                              (C.c_void_p * 6)(*(a.ctypes.data_as(C.c_void_p) for a in att)))
+    return np.stack(vel), np.stack(att)
+
+
+def _vti_check_inputs(v16):
+    """The quantities pve_vti_pml hands to its velocity routines at check
+    time: node material, eta/kappa averaged to the staggered x and z
+    positions, the harmonic mean of mu at the cell corners."""
+    from oracle import oracle_pve as B
+    mat = B.pve_vti_material(*v16, 1e-4)
+    phi, rs, rf = v16[3], v16[5] * 1e-6, v16[6] * 1e-6
+    eta = v16[9] * B.ETA_SCALE
+    nk1 = np.zeros_like(phi); nk1[:-1] = 0.5 * (eta[:-1] / v16[10][:-1] + eta[1:] / v16[10][1:]); nk1[-1] = nk1[-2]
+    nk3 = np.zeros_like(phi)
+    nk3[:, :-1] = 0.5 * (eta[:, :-1] / v16[11][:, :-1] + eta[:, 1:] / v16[11][:, 1:]); nk3[:, -1] = nk3[:, -2]
+    return dict(mat=mat, rho=(1 - phi) * rs + phi * rf, rf=rf, m1=v16[7] * rf / phi, m3=v16[8] * rf / phi,
+                nk1=nk1, nk3=nk3)
+
+
+def test_pve_dispersion_vti_against_eigen_solutions_and_the_reference():
+    """pve_vti_pml's check_model velocities (sm_pve_dispersion_vti) against
+      - the exact plane-wave solutions along the symmetry axes, written
+        independently: qP from the eigenvalues of R^-1 C with C = [[c11 +
+        a1^2 M, a1 M], [a1 M, M]] and R = [[rho, rho_f], [rho_f, Y1]],
+        Y1 = m1 - i (eta/kappa1)/omega (and the z counterparts), qS from
+        c55 / (rho - rho_f^2 / Y) with the Y across the axis: 1e-10;
+      - the reference's own VqP1, VqP2 and VqS (pve_vti_pml's general-
+        direction closed forms, built from its sources into oracle/_ref) at
+        (1, 0) and (0, 1), as complex slownesses: fast qP and qS within
+        1e-5; the slow qP within 2%
+        (the reference's closed form, a cubic solved by Cardano's formula,
+        loses that much where the slow wave is diffusive: the eigen-solution
+        above agrees with this implementation to 1e-13 there);
+      - the isotropic limit (and Q >= 1e4: the VTI check leaves M
+        unrelaxed): the velocities of sm_pve_dispersion."""
+    rng = np.random.default_rng(3)
+    shp = (5, 6)
+    v16 = _vti_model(rng, shp)
+    f = 40.0
+    om = 2 * np.pi * f
+    vel, att = _dispersion_vti(v16, f)
+    q = _vti_check_inputs(v16)
+    m = q["mat"]
+    for i in range(shp[0]):
+        for j in range(shp[1]):
+            for ax in range(2):
+                a, c, Y, Yo = (m["alpha1"], m["c11"], q["m1"] - 1j * q["nk1"] / om, q["m3"] - 1j * q["nk3"] / om) \
+                    if ax == 0 else (m["alpha3"], m["c33"], q["m3"] - 1j * q["nk3"] / om, q["m1"] - 1j * q["nk1"] / om)
+                Cm = np.array([[c[i, j] + a[i, j] ** 2 * m["M"][i, j], a[i, j] * m["M"][i, j]],
+                               [a[i, j] * m["M"][i, j], m["M"][i, j]]], complex)
+                Rm = np.array([[q["rho"][i, j], q["rf"][i, j]], [q["rf"][i, j], Y[i, j]]])
+                V = np.sqrt(np.linalg.eigvals(np.linalg.solve(Rm, Cm)))
+                V = V[np.argsort([-(1 / v).real ** -1 for v in V])]
+                Vs = np.sqrt(m["mu_ij"][i, j] / (q["rho"][i, j] - q["rf"][i, j] ** 2 / Yo[i, j]))
+                for k, Vk in enumerate((V[0], V[1], Vs)):
+                    assert abs(1 / (1 / Vk).real - vel[3 * ax + k, i, j]) < 1e-10 * vel[3 * ax + k, i, j]
+                    assert abs(-om * (1 / Vk).imag - att[3 * ax + k, i, j]) <= 1e-9 * abs(att[3 * ax + k, i, j])
+    ref = ROOT / "oracle" / "_ref" / "libvq.so"
+    if ref.exists():
+        class Cx(C.Structure):
+            _fields_ = [("re", C.c_double), ("im", C.c_double)]
+        vq = C.CDLL(str(ref))
+        for nm in ("VqP1", "VqP2", "VqS"):
+            getattr(vq, nm).restype = Cx
+            getattr(vq, nm).argtypes = [C.c_double] * 16
+        M, a1, a3 = m["M"], m["alpha1"], m["alpha3"]
+        for i in range(shp[0]):
+            for j in range(shp[1]):
+                args = (m["c11"][i, j] + a1[i, j] ** 2 * M[i, j], m["c13"][i, j] + a1[i, j] * a3[i, j] * M[i, j],
+                        m["c33"][i, j] + a3[i, j] ** 2 * M[i, j], m["mu_ij"][i, j], a1[i, j], a3[i, j], M[i, j],
+                        q["rho"][i, j], q["rf"][i, j], q["nk1"][i, j], q["nk3"][i, j], q["m1"][i, j], q["m3"][i, j], om)
+                for k, (nm, lx, lz) in enumerate([("VqP1", 1., 0.), ("VqP2", 1., 0.), ("VqS", 1., 0.),
+                                                   ("VqP1", 0., 1.), ("VqP2", 0., 1.), ("VqS", 0., 1.)]):
+                    r = getattr(vq, nm)(lx, lz, *args)
+                    # complex slowness 1/V = 1/vel - i att/omega, compared as a whole (the
+                    # attenuation of the fast wave is 1e-6 of its real part)
+                    s_ref = 1 / complex(r.re, r.im)
+                    s = complex(1 / vel[k, i, j], -att[k, i, j] / om)
+                    tol = 2e-2 if nm == "VqP2" else 1e-5
+                    assert abs(s_ref - s) < tol * abs(s), (nm, lx, i, j, abs(s_ref - s) / abs(s))
+    # isotropic limit (uniform eta/kappa and mu, so the staggered averages are the node values)
+    iso = _pve_values(rng, 1, 8).reshape(12, 8)
+    col = lambda a: np.repeat(a[None, :], 3, axis=0)
+    iso[4] = 6.0
+    iso[8] = 1.0
+    iso[9] = 500.0
+    iso[10] = 2e4            # the VTI check has no relaxation of M; the isotropic one none above Q 1e4
+    zero = np.zeros((3, 8))
+    v16 = [col(iso[q]) for q in range(7)] + [col(iso[7]), col(iso[7]), col(iso[8]), col(iso[9]), col(iso[9]),
+                                             col(iso[10]), col(iso[11]), zero, zero]
+    vel, _ = _dispersion_vti(v16, f)
+    vi, _ = _dispersion([a.reshape(-1) for a in (col(iso[q]) for q in range(12))], f)
+    vi = vi.reshape(3, 3, 8)
+    for k, ki in ((0, 1), (1, 0), (2, 2), (3, 1), (4, 0), (5, 2)):
+        assert np.allclose(vel[k], vi[ki], rtol=1e-10), k
