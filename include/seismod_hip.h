@@ -571,8 +571,13 @@ typedef struct {
     double f0;                /* frequency of source 0 */
     int abs;                  /* iwipe: 1 absorbing layers, 0 none */
     int storage;              /* SM_STORE_F64 or SM_STORE_F32 */
-    int flags;                /* 0 or SM_PSTD_HIPFFT */
+    int flags;                /* SM_PSTD_HIPFFT, SM_PVE_PML_MEDIAN */
 } sm_pve_params;
+/* flag of sm_pve_params (pml_median 1, pve_iso_pml.c:833-845): smooth the
+ * isotropic medium's node material (mu, E, M, alpha, varphi, tau_s, rho,
+ * rho_f, eta, kappa, m) in the absorbing layers with a 3 x 3 median, ring
+ * by ring (pml.c:2254-2329); not for the VTI medium (SM_ERR_ARG) */
+#define SM_PVE_PML_MEDIAN 2
 
 /* Build a solver from the padded model as read_model leaves it
  * (io_utils.c:403-625), nx2*nz2 values each (index i*nz2+j): K_m, K_s,

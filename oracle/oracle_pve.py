@@ -45,9 +45,37 @@ I_TXX, I_TZZ, I_TXZ, I_P, I_VX, I_VZ, I_QX, I_QZ, I_E = range(9)
 ETA_SCALE = 1.013249965828145e6
 
 
-def pve_material(K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, eta, kappa, Q, f0, dt):
+def pml_median(a, np_):
+    """pml_median_average (pml.c:2254-2329): ring by ring from the layer
+    next to the model outwards, each ring's nodes replaced by the lower
+    median ((n - 1)/2-th smallest) of their 3 x 3 neighbourhood inside the
+    grid, computed from the previous ring's result; the window stays 3 x 3
+    (the size update of pml.c:2324 assigns nmed its own old value).  The
+    columns of a ring stop at nz2 - np - n - 1 (pml.c:2297), short of its
+    bottom row."""
+    a = np.array(a, np.float64)
+    nx2, nz2 = a.shape
+
+    def med(src, i, j):
+        w = src[max(i - 1, 0):i + 2, max(j - 1, 0):j + 2].reshape(-1)
+        return np.sort(w)[(w.size - 1) // 2]
+    for n in range(1, np_ + 1):
+        d = np_ - n
+        out = a.copy()
+        for i in range(d, nx2 - d):
+            out[i, d] = med(a, i, d)
+            out[i, nz2 - d - 1] = med(a, i, nz2 - d - 1)
+        for j in range(d + 1, nz2 - np_ - n - 1):
+            out[d, j] = med(a, d, j)
+            out[nx2 - d - 1, j] = med(a, nx2 - d - 1, j)
+        a = out
+    return a
+
+
+def pve_material(K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, eta, kappa, Q, f0, dt, median_np=0):
     """Material of pve_iso_pml.c:795-1145 from the padded model as read_model
-    leaves it (moduli in GPa, densities in kg/m^3, eta in cP, kappa in mD).
+    leaves it (moduli in GPa, densities in kg/m^3, eta in cP, kappa in mD),
+    with the median smoothing of the layers (833-845) when median_np > 0.
     Returns the stored arrays and the analytic-step coefficients."""
     pi = 4.0 * math.atan(1.0)
     a = lambda x: np.asarray(x, np.float64)
@@ -68,6 +96,9 @@ def pve_material(K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, eta, kappa, Q, f0, dt)
     tau_s = np.where(lowQ, tau0 / Qs * (np.sqrt(Qs * Qs + 1.) - 1.), tau0)
     varphi = np.where(lowQ, tau_e / np.where(lowQ, tau_s, 1.0) - 1., 0.0)
     rho = (1. - phi) * rho_s + phi * rho_f
+    if median_np:
+        mu, E, M, alpha, varphi, tau_s, rho, rho_f, eta, kappa, m = (
+            pml_median(x, median_np) for x in (mu, E, M, alpha, varphi, tau_s, rho, rho_f, eta, kappa, m))
     # staggered, 1009-1110
     z = np.zeros_like
     rho_i, rho_f_i, nk_i, m_i, cq_i, cv_i = z(E), z(E), z(E), z(E), z(E), z(E)

@@ -14,6 +14,7 @@ SM_OK, SM_ERR_ARG, SM_ERR_HIP, SM_ERR_CFL, SM_ERR_NOMEM, SM_ERR_STATE, SM_ERR_NO
 SM_STORE_F32, SM_STORE_F64, SM_STORE_BF16, SM_STORE_F16 = range(4)
 SM_VX, SM_VY, SM_VZ, SM_TXX, SM_TZZ, SM_TXZ, SM_P = 0, 1, 2, 5, 6, 7, 8
 SM_QX, SM_QZ = 3, 4
+SM_PVE_PML_MEDIAN = 2
 SM_PSTD_HIPFFT = 1
 SM_TXY, SM_TYZ = 9, 10
 SM_DIV, SM_CURL, SM_VR, SM_VT, SM_TRR, SM_TRT, SM_TRZ, SM_TTT, SM_TTZ = range(11, 20)

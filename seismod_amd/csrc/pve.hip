@@ -445,18 +445,54 @@ struct PveModel {
         *delta;
 };
 
+// pml_median_average (pml.c:2254-2329): ring by ring from the layer next
+// to the model outwards, each ring node replaced by the lower median of its
+// 3 x 3 neighbourhood inside the grid, from the previous ring's values (the
+// window stays 3 x 3: the size update at pml.c:2324 assigns nmed its own old
+// value); a ring's columns stop at nz2 - np - n - 1 (pml.c:2297)
+void pml_median(std::vector<double> &a, size_t nx2, size_t nz2, size_t np)
+{
+    std::vector<double> out;
+    double w[9];
+    auto med = [&](long i, long j) {
+        int k = 0;
+        for (long ii = i - 1; ii <= i + 1; ++ii)
+            for (long jj = j - 1; jj <= j + 1; ++jj)
+                if (ii >= 0 && ii < (long)nx2 && jj >= 0 && jj < (long)nz2) w[k++] = a[(size_t)ii * nz2 + jj];
+        std::nth_element(w, w + (k - 1) / 2, w + k);
+        return w[(k - 1) / 2];
+    };
+    for (long n = 1; n <= (long)np; ++n) {
+        const long d = (long)np - n;
+        out = a;
+        for (long i = d; i < (long)nx2 - d; ++i) {
+            out[(size_t)i * nz2 + d] = med(i, d);
+            out[(size_t)i * nz2 + nz2 - d - 1] = med(i, (long)nz2 - d - 1);
+        }
+        for (long j = d + 1; j < (long)nz2 - (long)np - n - 1; ++j) {
+            out[(size_t)d * nz2 + j] = med(d, j);
+            out[(size_t)(nx2 - d - 1) * nz2 + j] = med((long)nx2 - d - 1, j);
+        }
+        a.swap(out);
+    }
+}
+
 // the material of pve_iso_pml.c:795-1145 or pve_vti_pml.c:787-1145 in M_*
-// order
-bool material(size_t nx2, size_t nz2, double dt, bool vti, const PveModel &in, std::vector<double> &all)
+// order; median: the smoothing of the isotropic medium's layers (833-845)
+bool material(size_t nx2, size_t nz2, double dt, bool vti, const PveModel &in, size_t median_np,
+              std::vector<double> &all)
 {
     const size_t N = nx2 * nz2;
     const double pi = 4.0 * std::atan(1.0);
     std::vector<std::vector<double>> m(vti ? M_NMAT_VTI : M_NMAT, std::vector<double>(N, 0.0));
-    std::vector<double> mu(N), rho_f(N), rho(N), m1(N), m3(N), nk1(N), nk3(N), tau_s(N), vphi(N);
+    std::vector<double> mu(N), rho_f(N), rho(N), m1(N), m3(N), nk1(N), nk3(N), tau_s(N), vphi(N), eta(N), k1(N),
+        k3(N);
     for (size_t n = 0; n < N; ++n) {
         const double K_m = in.K_m[n] * 1000., K_s = in.K_s[n] * 1000., K_f = in.K_f[n] * 1000.;
         mu[n] = in.mu[n] * 1000.;
-        const double eta = in.eta[n] * 1.013249965828145e6;
+        eta[n] = in.eta[n] * 1.013249965828145e6;
+        k1[n] = in.kappa1[n];
+        k3[n] = in.kappa3[n];
         const double rho_s = in.rho_s[n] * 1.e-6;
         rho_f[n] = in.rho_f[n] * 1.e-6;
         const double phi = in.phi[n];
@@ -479,8 +515,6 @@ bool material(size_t nx2, size_t nz2, double dt, bool vti, const PveModel &in, s
         }
         m1[n] = in.T1[n] * rho_f[n] / phi;
         m3[n] = in.T3[n] * rho_f[n] / phi;
-        nk1[n] = eta / in.kappa1[n];
-        nk3[n] = eta / in.kappa3[n];
         const double Q = in.Q[n];
         if (Q < 1.e4) {
             const double tau_e = tau0 / Q * (std::sqrt(Q * Q + 1.) + 1.);
@@ -491,6 +525,16 @@ bool material(size_t nx2, size_t nz2, double dt, bool vti, const PveModel &in, s
             vphi[n] = 0.0;
         }
         rho[n] = (1. - phi) * rho_s + phi * rho_f[n];
+    }
+    if (median_np && !vti) {
+        for (std::vector<double> *v : {&mu, &m[M_EM], &m[M_M], &m[M_ALPHA], &vphi, &tau_s, &rho, &rho_f, &eta, &k1, &m1})
+            pml_median(*v, nx2, nz2, median_np);
+        k3 = k1;
+        m3 = m1;
+    }
+    for (size_t n = 0; n < N; ++n) {
+        nk1[n] = eta[n] / k1[n];
+        nk3[n] = eta[n] / k3[n];
     }
     auto coeffs = [&](double r, double rf, double mi, double nk, double &cq, double &cv) {
         const double tmp = 1. / (rf * rf - r * mi);
@@ -563,7 +607,9 @@ int create(sm_ctx *ctx, const sm_pve_params *pp, bool vti, const PveModel &in, s
              "%s: storage must be SM_STORE_F64 or SM_STORE_F32", fn);
     SM_CHECK(p.nx >= 2 && p.nz >= 2, SM_ERR_ARG, "%s: the model needs at least 2 x 2 nodes", fn);
     SM_CHECK(p.dx > 0 && p.dz > 0 && p.dt > 0, SM_ERR_ARG, "%s: spacings and dt must be > 0", fn);
-    SM_CHECK((p.flags & ~SM_PSTD_HIPFFT) == 0, SM_ERR_ARG, "%s: unknown flags %#x", fn, p.flags);
+    SM_CHECK((p.flags & ~(SM_PSTD_HIPFFT | SM_PVE_PML_MEDIAN)) == 0, SM_ERR_ARG, "%s: unknown flags %#x", fn, p.flags);
+    SM_CHECK(!(vti && (p.flags & SM_PVE_PML_MEDIAN)), SM_ERR_ARG,
+             "%s: the median smoothing of the layers is for the isotropic medium", fn);
     if (!p.abs) p.np = 0;   // io_utils.c:293: no absorbing layers without abs
     const size_t nx2 = p.nx + 2 * p.np, nz2 = p.nz + 2 * p.np, N = nx2 * nz2;
     SM_CHECK(nx2 < (1u << 20) && nz2 < (1u << 20) && N * W_NF < ((size_t)1 << 31), SM_ERR_ARG, "%s: grid too large",
@@ -574,7 +620,8 @@ int create(sm_ctx *ctx, const sm_pve_params *pp, bool vti, const PveModel &in, s
                      in.kappa1[n] > 0 && in.kappa3[n] > 0 && in.Q[n] > 0 && in.fq[n] > 0,
                  SM_ERR_ARG, "%s: cell %zu has a property out of range", fn, n);
     std::vector<double> mat;
-    SM_CHECK(material(nx2, nz2, p.dt, vti, in, mat), SM_ERR_ARG, "%s: material not finite", fn);
+    SM_CHECK(material(nx2, nz2, p.dt, vti, in, (p.flags & SM_PVE_PML_MEDIAN) ? p.np : 0, mat), SM_ERR_ARG,
+             "%s: material not finite", fn);
     SM_CHECK(set_device(ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
 
     sm_pve *s = new sm_pve;

@@ -16,9 +16,10 @@
  * E/<base>_E.dat (saveEnergy, EnergyROI) and the phase velocity and
  * attenuation check (check_model: 1 prints the extremes, 2 also writes
  * <base>_vel_slowP.nc ... <base>_att_S.nc, or for the VTI program
- * <base>_vel_qP1_x.nc ... <base>_att_qS_z.nc).  Checkpoints (chkpt_inc,
- * -c) and the median smoothing of the layers (pml_median) are not
- * available: the program says so and stops.
+ * <base>_vel_qP1_x.nc ... <base>_att_qS_z.nc); pml_median 1 smooths the
+ * isotropic medium's layers (check_model then still reports the unsmoothed
+ * model).  Checkpoints (chkpt_inc, -c) are not available: the program says
+ * so and stops.
  *
  * Set-up and update arithmetic live in the library (sm_pve_*); this driver
  * reads the inputs, drives the time loop and writes the records.
@@ -221,11 +222,6 @@ int sm_pve_program(int argc, char *argv[], int vti)
         fprintf(stderr, "Error: checkpointing (chkpt_inc %d) is not available in this %s\n", p.chkpt_inc, prog);
         return 1;
     }
-    if (p.pml_median) {
-        fprintf(stderr, "Error: median smoothing of the absorbing layers (pml_median) is not available in "
-                        "this %s\n", prog);
-        return 1;
-    }
     const size_t np = p.np;   /* 0 when abs is 0 */
     const double dt = p.dt;
     if (sm_verbose) {
@@ -351,6 +347,9 @@ int sm_pve_program(int argc, char *argv[], int vti)
     vp.vmax = p.vmax; vp.pml_order = p.pml_order; vp.Rc = p.Rc; vp.kappa_max = p.kappa_max;
     vp.alpha_max = p.alpha_max; vp.f0 = f0; vp.abs = p.iwipe;
     vp.storage = (stor && !strcmp(stor, "f32")) ? SM_STORE_F32 : SM_STORE_F64;
+    /* pml_median 1 smooths the isotropic medium's layers (pve_iso_pml.c:833-845);
+     * pve_vti_pml reads the keyword and does nothing with it */
+    if (p.pml_median == 1 && !vti) vp.flags |= SM_PVE_PML_MEDIAN;
     sm_pve *sim = NULL;
     if (vti)
         check(sm_pve_create_vti(ctx, &vp, v[V_KM], v[V_KS], v[V_KF], v[V_PHI], v[V_MU], v[V_RHOS], v[V_RHOF],

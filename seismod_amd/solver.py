@@ -800,8 +800,13 @@ class PveConfig(ShConfig):
     """pve_iso_pml grid: nx by nz interior nodes and np CPML layers on every
     side (none when abs is False), padded (nx + 2np) by (nz + 2np)."""
 
+    pml_median: bool = False  # 3 x 3 median smoothing of the layers' material (SM_PVE_PML_MEDIAN)
+
     def params(self) -> L.PveParams:
-        return super().params()
+        p = super().params()
+        if self.pml_median:
+            p.flags |= L.SM_PVE_PML_MEDIAN
+        return p
 
     def bytes_needed(self, nrec: int = 0, max_steps: int = 0) -> int:
         p = self.params()

@@ -592,7 +592,8 @@ def test_ve_vti_pml_program(tmp_path, template):
 
 
 # ------------------------------------------------------- poroviscoelastic --
-def deck_pve(tmp, nx=44, nz=38, np_=8, h=5.0, dt_ms=0.25, time_s=0.05, template=1, with_pml=False, vti=False):
+def deck_pve(tmp, nx=44, nz=38, np_=8, h=5.0, dt_ms=0.25, time_s=0.05, template=1, with_pml=False, vti=False,
+             median=False):
     rng = np.random.default_rng(29)
     top = np.arange(nz)[None, :] < nz // 2
     ones = np.ones((nx, 1))
@@ -632,6 +633,7 @@ def deck_pve(tmp, nx=44, nz=38, np_=8, h=5.0, dt_ms=0.25, time_s=0.05, template=
         "model.dat # model, file\nsrc.dat # source, file\nrcv.dat # output, records\n"
         f"pve # basename, name\n{time_s} # time, s\n{dt_ms} # dt, ms\n1 # abs, on\n{np_} # nl, layers\n"
         "3 # kappa_max, PML\n1 # plstr, strips\n2 # check_model, store\n1 # saveEnergy, E(t)\n"
+        + ("1 # pml_median, smoothing\n" if median else "") +
         f"{roi[0]}/{roi[1]}/{roi[2]}/{roi[3]} # EnergyROI, region\n")
     nz2 = nz + 2 * np_
     cell = lambda x, z: (int(round(x / h)) + np_) * nz2 + int(round(z / h)) + np_
@@ -643,7 +645,7 @@ def deck_pve(tmp, nx=44, nz=38, np_=8, h=5.0, dt_ms=0.25, time_s=0.05, template=
     if vti:       # the solver's order: ... T1 T3 eta kappa1 kappa3 Q f0 epsilon delta
         K_m, K_s, K_f, phi, mu, rs, rf, T3, eta, k3, Q, f0, e_, d_, rk, rt = model
         model = [K_m, K_s, K_f, phi, mu, rs, rf, T3 / rt, T3, eta, k3 / rk, k3, Q, f0, e_, d_]
-    return dict(nx=nx, nz=nz, np=np_, h=h, dt=dt_ms * 1e-3, model=tuple(model), rec=rec, vti=vti,
+    return dict(nx=nx, nz=nz, np=np_, h=h, dt=dt_ms * 1e-3, model=tuple(model), rec=rec, vti=vti, median=median,
                 template=template, srcs=[(6, cell(110.0, 60.0), 2.0), (7, cell(70.0, 120.0), 0.5)], snap_t0=0.025,
                 roi_cells=ij, raw=vals)
 
@@ -667,20 +669,23 @@ def oracle_pve(d, nsteps, extra_rec=()):
     src = [(p[0], p[1], fct[off[k]:off[k + 1]]) for k, p in enumerate(pts)]
     cfg = dict(nx=d["nx"], nz=d["nz"], np=d["np"], dx=d["h"], dz=d["h"], dt=d["dt"], vmax=5000.0,
                kappa_max=3.0, alpha_max=math.pi * 60.0, iwipe=1)
-    mat = (B.pve_vti_material if d["vti"] else B.pve_material)(*d["model"], d["dt"])
+    if d["vti"]:
+        mat = B.pve_vti_material(*d["model"], d["dt"])
+    else:
+        mat = B.pve_material(*d["model"], d["dt"], median_np=d["np"] if d["median"] else 0)
     rec = list(d["rec"]) + list(extra_rec)
     return B.pve_run(cfg, mat, src, [r[0] for r in rec], [r[1] for r in rec], nsteps)
 
 
-@pytest.mark.parametrize("template,with_pml", [(1, False), (9, True)])
-def test_pve_iso_pml_program(tmp_path, template, with_pml):
+@pytest.mark.parametrize("template,with_pml,median", [(1, False, False), (9, True, False), (1, False, True)])
+def test_pve_iso_pml_program(tmp_path, template, with_pml, median):
     """The drop-in pve_iso_pml (FP64) against the oracle: every trace line
     (divergence and curl included), the p snapshot (with the absorbing
     strips), the kinetic energy of the region in E/pve_E.dat (eq. 7.145 of
     Carcione 2007 from the oracle's velocities at every region cell) and the
     check_model files at the source frequency."""
     from seismod_amd.solver import nsteps as sm_nsteps, record_due
-    d = deck_pve(tmp_path, template=template, with_pml=with_pml)
+    d = deck_pve(tmp_path, template=template, with_pml=with_pml, median=median)
     r = run("pve_iso_pml", tmp_path, "f64")
     assert "Max/min phase velocities" in r.stdout
     ns = sm_nsteps(0.05, d["dt"])

@@ -12,7 +12,8 @@ receiver component (divergence and curl included), a model wider than deep
 with kappa_max > 1 and alpha_max = 0, no layers (abs 0) with odd nx2, an
 odd nz2 (x derivatives by real transforms), every source type on one cell
 with a layer of zero shear and frame moduli, layers with the lossless
-branch (Q >= 1e4) and an inviscid fluid (eta = 0), and VTI Biot media
+branch (Q >= 1e4) and an inviscid fluid (eta = 0), the median smoothing of
+the layers (pml_median), and VTI Biot media
 (pve_vti_pml: Thomsen's epsilon and delta, tortuosity and permeability
 different along x and z) with layers, kappa_max > 1 and without layers.
 """
@@ -52,7 +53,7 @@ def biot_vmax(K_m, K_s, K_f, phi, mu, rho_s, rho_f, T):
 
 
 def deck(nx=40, nz=50, np_=8, seed=0, h=10.0, nsteps=120, f0=25.0, kappa_max=1.0, alpha_max=1, abs_=True,
-         srcs=None, fluid_layer=False, lossless_layer=False, vti=False):
+         srcs=None, fluid_layer=False, lossless_layer=False, vti=False, median=False):
     rng = np.random.default_rng(seed)
     npad = np_ if abs_ else 0
     nx2, nz2 = nx + 2 * npad, nz + 2 * npad
@@ -105,7 +106,7 @@ def deck(nx=40, nz=50, np_=8, seed=0, h=10.0, nsteps=120, f0=25.0, kappa_max=1.0
     cfg = dict(nx=nx, nz=nz, np=npad, dx=h, dz=h, dt=dt, vmax=vmax, kappa_max=kappa_max,
                alpha_max=(math.pi * f0 if alpha_max else 0.0), iwipe=1 if abs_ else 0)
     return dict(cfg=cfg, model=model, types=types, idx=idx, Lf=Lf, fct=fct, ridx=ridx, rcomp=rcomp, nsteps=nsteps,
-                f0=f0, alpha_max=alpha_max, abs=abs_, np=np_, vti=vti)
+                f0=f0, alpha_max=alpha_max, abs=abs_, np=np_, vti=vti, median=median)
 
 
 def config(d, storage="f64", hipfft=False):
@@ -113,7 +114,7 @@ def config(d, storage="f64", hipfft=False):
     c = d["cfg"]
     return PveConfig(nx=c["nx"], nz=c["nz"], np=d["np"], dx=c["dx"], dz=c["dz"], dt=c["dt"], f0=d["f0"],
                      vmax=c["vmax"], kappa_max=c["kappa_max"], alpha_max=d["alpha_max"], abs=d["abs"],
-                     storage=storage, hipfft=hipfft)
+                     storage=storage, hipfft=hipfft, pml_median=d.get("median", False))
 
 
 def solver(ctx, d, storage="f64", hipfft=False):
@@ -137,7 +138,10 @@ def run_gpu(ctx, d, storage, hipfft=False):
 def run_oracle(d):
     off = np.concatenate([[0], np.cumsum(d["Lf"])]).astype(np.int64)
     src = [(int(t), int(i), d["fct"][off[k]:off[k + 1]]) for k, (t, i) in enumerate(zip(d["types"], d["idx"]))]
-    mat = (B.pve_vti_material if d["vti"] else B.pve_material)(*d["model"], d["cfg"]["dt"])
+    if d["vti"]:
+        mat = B.pve_vti_material(*d["model"], d["cfg"]["dt"])
+    else:
+        mat = B.pve_material(*d["model"], d["cfg"]["dt"], median_np=d["cfg"]["np"] if d.get("median") else 0)
     tr, W = B.pve_run(d["cfg"], mat, src, d["ridx"], d["rcomp"], d["nsteps"])
     return tr, W[:8]
 
@@ -150,6 +154,7 @@ CASES = [
     dict(nx=30, nz=36, np_=7, seed=6, fluid_layer=True,
          srcs=[(B.SX, 20, 25, 1.0), (B.SZ, 20, 25, -0.4), (B.BULK, 20, 25, 0.7), (B.SXZ, 20, 25, 0.5),
                (B.SF, 20, 25, 0.3), (B.BULK_S, 20, 25, -0.6), (B.SXZ, 21, 24, 0.2), (3, 22, 22, 1.0)]),
+    dict(nx=30, nz=34, np_=6, seed=7, median=True),                    # median smoothing of the layers
     # VTI Biot media (pve_vti_pml)
     dict(seed=11, vti=True),
     dict(nx=70, nz=20, np_=6, seed=13, kappa_max=3.0, alpha_max=0, vti=True),
@@ -158,7 +163,7 @@ CASES = [
 
 
 def ids(c):
-    return f"seed{c['seed']}" + ("_vti" if c.get("vti") else "")
+    return f"seed{c['seed']}" + ("_vti" if c.get("vti") else "") + ("_median" if c.get("median") else "")
 
 
 def rel(a, b):
@@ -209,7 +214,7 @@ def test_pve_fp64_matches_oracle(ctx, case):
         assert np.allclose(dc[k].reshape(-1)[d["ridx"][sel].astype(np.int64)], tr[-1, sel], rtol=0, atol=0)
 
 
-@pytest.mark.parametrize("case", CASES[:2] + CASES[5:6], ids=ids)
+@pytest.mark.parametrize("case", CASES[:2] + CASES[6:7], ids=ids)
 def test_pve_fp32_tolerance(ctx, case):
     d = deck(**case)
     tr_o, f_o = run_oracle(d)

@@ -755,12 +755,11 @@ def test_pve_dispersion_limits():
     assert np.allclose(vel[2], np.sqrt(mu / rho), rtol=1e-6)
 
 
-@pytest.mark.parametrize("extra,msg", [("500 # chkpt_inc, every 500\n", "checkpointing"),
-                                       ("1 # pml_median, on\n", "pml_median")])
+@pytest.mark.parametrize("extra,msg", [("500 # chkpt_inc, every 500\n", "checkpointing")])
 def test_pve_iso_pml_refuses_unavailable_options(tmp_path, extra, msg):
     """pve_iso_pml stops with a message, before reading the model or using
-    a GPU, when checkpoints or the median smoothing of the layers are asked
-    for; and takes -h like the other programs."""
+    a GPU, when checkpoints are asked for; and takes -h like the other
+    programs."""
     assert run_bin("pve_iso_pml", "-h").returncode == 0
     (tmp_path / "run.par").write_text("model.dat # model,\nsrc.dat # source,\nrcv.dat # output,\n" + extra)
     r = run_bin("pve_iso_pml", "-p", "run.par", cwd=tmp_path)

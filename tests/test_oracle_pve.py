@@ -309,3 +309,32 @@ def test_vti_line_source_fast_and_slow_waves(axis):
         for comp in range(3):
             got = tr[:, 3 * k + comp]
             assert rel_l2(got, want[comp]) < 5e-3, (axis, o, comp, rel_l2(got, want[comp]))
+
+
+def test_pml_median_smoothing():
+    """pml_median_average (pml.c:2254-2329) as the oracle restates it: a
+    constant field stays constant; nodes outside the rings (the model, and
+    the ring columns below nz2 - np - n - 1) keep their values; a ring node
+    takes the lower median of its 3 x 3 window of the previous ring's
+    result (for the four values of a grid corner the second smallest);
+    an isolated spike in the layers is removed."""
+    np_ = 3
+    nx2, nz2 = 14, 12
+    assert np.array_equal(B.pml_median(np.full((nx2, nz2), 2.5), np_), np.full((nx2, nz2), 2.5))
+    rng = np.random.default_rng(2)
+    a = rng.uniform(0, 1, (nx2, nz2))
+    b = B.pml_median(a, np_)
+    touched = np.zeros((nx2, nz2), bool)
+    for n in range(1, np_ + 1):
+        d = np_ - n
+        touched[d:nx2 - d, d] = touched[d:nx2 - d, nz2 - d - 1] = True
+        touched[d, d + 1:nz2 - np_ - n - 1] = touched[nx2 - d - 1, d + 1:nz2 - np_ - n - 1] = True
+    assert np.array_equal(b[~touched], a[~touched])
+    spike = np.ones((nx2, nz2))
+    spike[1, 5] = 100.0
+    assert B.pml_median(spike, np_)[1, 5] == 1.0
+    # the lower median of an even window
+    z = np.zeros((nx2, nz2))
+    z[0, 0], z[0, 1], z[1, 0], z[1, 1] = 4.0, 3.0, 2.0, 1.0
+    z[1:, 1:] = np.where(z[1:, 1:] == 0, 9.0, z[1:, 1:])
+    assert B.pml_median(z, 1)[0, 0] == 2.0
