@@ -41,11 +41,12 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="a_iso_3d",
-                    choices=["a_iso_3d", "e_cyl_iso", "ve_vti_sh_pml", "ve_vti_pml", "pve_iso_pml"],
+                    choices=["a_iso_3d", "e_cyl_iso", "ve_vti_sh_pml", "ve_vti_pml", "pve_iso_pml", "pve_vti_pml"],
                     help="a_iso_3d (the headline, default), e_cyl_iso (the 2.5D cylindrical "
                          "elastic solver on a borehole model), ve_vti_sh_pml or ve_vti_pml "
-                         "(viscoelastic VTI SH or P-SV, pseudo-spectral), pve_iso_pml "
-                         "(poroviscoelastic Biot, pseudo-spectral); the last four store FP32 or FP64")
+                         "(viscoelastic VTI SH or P-SV, pseudo-spectral), pve_iso_pml or pve_vti_pml "
+                         "(poroviscoelastic Biot, isotropic or VTI, pseudo-spectral); the last five store "
+                         "FP32 or FP64")
     ap.add_argument("--hipfft", action="store_true",
                     help="pseudo-spectral workloads: every derivative through hipFFT instead of the "
                          "hand-written LDS transforms")
@@ -77,7 +78,8 @@ ERROR_TOL = {"f32": 2e-5, "f16": 4e-2, "bf16": 3e-1}
 # FP32 pseudo-spectral workloads: FFT rounding over 1000 steps of a 1M-cell grid
 PSTD_F32_TOL = {"ve_vti_sh_pml": 1e-5,   # measured 1.05e-6 and 1.18e-6 at 1000 steps of the 1000^2 shot
                 "ve_vti_pml": 1e-4,
-                "pve_iso_pml": 1e-4}
+                "pve_iso_pml": 1e-4,
+                "pve_vti_pml": 1e-4}
 
 
 def cpu_baseline(shot, nsteps_sample: int) -> dict:
@@ -328,9 +330,9 @@ def cpu_baseline_pstd(kind, shot, nsteps_sample: int) -> dict:
         from oracle import oracle_sh as S
         run = lambda n: S.sh_run(cfg, shot.mu, shot.rho, shot.gamma, shot.Q, shot.f, src, shot.rec_idx[:1],
                                  shot.rec_comp[:1], n)
-    elif kind == "pve_iso_pml":
+    elif kind in ("pve_iso_pml", "pve_vti_pml"):
         from oracle import oracle_pve as B
-        mat = B.pve_material(*shot.model, shot.dt)
+        mat = (B.pve_material if kind == "pve_iso_pml" else B.pve_vti_material)(*shot.model, shot.dt)
         run = lambda n: B.pve_run(cfg, mat, src, shot.rec_idx[:1], shot.rec_comp[:1], n)
     else:
         from oracle import oracle_ve as V
@@ -362,6 +364,9 @@ PSTD = {
     "pve_iso_pml": dict(src_type=6, src_name="Bulk", rec_name="Vz",
                         metric="Gcell-updates/s & %HBM roofline, 2D poroviscoelastic isotropic (Biot), "
                                "pseudo-spectral RK4 (pve_iso_pml)"),
+    "pve_vti_pml": dict(src_type=6, src_name="Bulk", rec_name="Vz",
+                        metric="Gcell-updates/s & %HBM roofline, 2D poroviscoelastic VTI (Biot), "
+                               "pseudo-spectral RK4 (pve_vti_pml)"),
 }
 
 
@@ -371,17 +376,18 @@ def main_pstd(args) -> None:
     plus a 20-cell CPML, one source, 200 receivers, 1000 steps); one bench
     step is one shot."""
     from seismod_amd import synth
-    from seismod_amd.solver import (Context, Poroviscoelastic, PveConfig, ShConfig, VeConfig, ViscoelasticPSV,
-                                    ViscoelasticSH, source_functions)
+    from seismod_amd.solver import (Context, Poroviscoelastic, PoroviscoelasticVTI, PveConfig, ShConfig, VeConfig,
+                                    ViscoelasticPSV, ViscoelasticSH, source_functions)
     kind = args.workload
     spec = PSTD[kind]
     if args.storage not in ("f32", "f64"):
         print(f"bench.py: {kind} stores FP32 or FP64", file=sys.stderr)
         sys.exit(2)
     nsteps = args.nsteps or 1000
-    sh, pve = kind == "ve_vti_sh_pml", kind == "pve_iso_pml"
+    sh, pve, pvti = kind == "ve_vti_sh_pml", kind == "pve_iso_pml", kind == "pve_vti_pml"
     shot = (synth.sh_layered_shot(nsteps=nsteps) if sh else
-            synth.pve_layered_shot(nsteps=nsteps) if pve else synth.ve_layered_shot(nsteps=nsteps))
+            synth.pve_layered_shot(nsteps=nsteps) if pve else
+            synth.pve_vti_layered_shot(nsteps=nsteps) if pvti else synth.ve_layered_shot(nsteps=nsteps))
     ctx = Context(0)
 
     def make(storage):
@@ -391,6 +397,8 @@ def main_pstd(args) -> None:
             sim = ViscoelasticSH(ctx, ShConfig(**common), shot.mu, shot.rho, shot.gamma, shot.Q, shot.f)
         elif pve:
             sim = Poroviscoelastic(ctx, PveConfig(**common), *shot.model)
+        elif pvti:
+            sim = PoroviscoelasticVTI(ctx, PveConfig(**common), *shot.model)
         else:
             sim = ViscoelasticPSV(ctx, VeConfig(L=shot.L, **common), *shot.model)
         L, fct = source_functions([1.0], [shot.f0], 0.5 * shot.dt)
@@ -424,7 +432,7 @@ def main_pstd(args) -> None:
         sys.exit(1)
     elt = 4 if args.storage == "f32" else 8
     alg_b = (synth.algorithmic_bytes_sh(elt) if sh else synth.algorithmic_bytes_pve(elt) if pve else
-             synth.algorithmic_bytes_ve(shot.L, elt))
+             synth.algorithmic_bytes_pve_vti(elt) if pvti else synth.algorithmic_bytes_ve(shot.L, elt))
     avg_ms = kms / max(timed, 1)
     achieved = alg_b * shot.cells / (avg_ms * 1e-3) / 1e9
     nx2, nz2 = shot.nx + 2 * shot.np, shot.nz + 2 * shot.np
@@ -434,11 +442,12 @@ def main_pstd(args) -> None:
         "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.storage == "f32" else "f64",
-        "data": "synthetic (seeded layered " + ("porous rock, Biot" if pve else "VTI viscoelastic") + " model)",
+        "data": "synthetic (seeded layered " + ("porous rock, Biot" if pve else "VTI porous rock, Biot" if pvti
+                                                 else "VTI viscoelastic") + " model)",
         "config": {"workload": f"{kind} {shot.nx}x{shot.nz} + CPML {shot.np}, h = {shot.h} m, "
                                f"{spec['src_name']} {shot.f0:g} Hz, {nsteps} time steps per shot, "
                                f"{len(shot.rec_idx)} {spec['rec_name']} receivers"
-                               + ("" if sh or pve else f", L = {shot.L}"),
+                               + ("" if sh or pve or pvti else f", L = {shot.L}"),
                    "padded_grid": [nx2, nz2], "time_steps_per_shot": nsteps, "dt_s": shot.dt,
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),

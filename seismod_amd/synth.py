@@ -566,3 +566,36 @@ def algorithmic_bytes_pve(elt: int = 8) -> float:
     read in stages 2-4 (the step-start values in stage 1): 52 + 62 + 62 + 53
     = 229.  337 values per step."""
     return elt * 337.0
+
+
+def pve_vti_layered_shot(n: int = 1000, np_: int = 20, h: float = 5.0, seed: int = 17, f0: float = 25.0,
+                         nsteps: int = 1000, nrec: int = 200) -> PveShot:
+    """pve_vti_pml workload: the layered porous rock of pve_layered_shot
+    made VTI: per layer Thomsen epsilon ~ U[0, 0.25] and delta ~
+    U[-0.05, 0.15] of the frame, tortuosity along z T3 = T1 U[1, 1.6] and
+    permeability along z kappa3 = kappa1 U[0.1, 1]; model arrays in
+    sm_pve_create_vti's order (K_m K_s K_f phi mu rho_s rho_f T1 T3 eta
+    kappa1 kappa3 Q fq epsilon delta).  dt at 40% of the RK4 pseudo-spectral
+    stability limit of the fastest (horizontal) qP wave."""
+    base = pve_layered_shot(n=n, np_=np_, h=h, seed=seed, f0=f0, nsteps=nsteps, nrec=nrec)
+    rng = np.random.default_rng(seed + 1000)
+    K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, eta, kappa, Q, fq = base.model
+    nz2 = base.model[0].shape[1]
+    # layers of the interior model, extended into the layers like the others
+    tops_src = base.model[0][n // 2 + np_]          # the K_m column identifies the layers
+    _, lay = np.unique(tops_src, return_inverse=True)
+    nl = int(lay.max()) + 1
+    per = lambda lo, hi: rng.uniform(lo, hi, nl)[lay][None, :] * np.ones((base.model[0].shape[0], 1))
+    eps, dl = per(0.0, 0.25), per(-0.05, 0.15)
+    T3, kappa3 = T * per(1.0, 1.6), kappa * per(0.1, 1.0)
+    model = (K_m, K_s, K_f, phi, mu, rho_s, rho_f, T, T3, eta, kappa, kappa3, Q, fq, eps, dl)
+    vmax = float((biot_fast_velocity(K_m, K_s, K_f, phi, mu, rho_s, rho_f, T) * np.sqrt(1 + 2 * eps)).max())
+    dt = 0.4 * 2.0 * h / (np.pi * vmax * np.sqrt(2.0))
+    return PveShot(nx=n, nz=n, np=np_, h=h, dt=float(dt), f0=f0, nsteps=nsteps, model=model, src_idx=base.src_idx,
+                   rec_idx=base.rec_idx, rec_comp=base.rec_comp)
+
+
+def algorithmic_bytes_pve_vti(elt: int = 8) -> float:
+    """As algorithmic_bytes_pve with the two further material arrays (c33,
+    alpha3) read in each of the 4 stages: 345 values per step."""
+    return elt * 345.0
