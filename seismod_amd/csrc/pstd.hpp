@@ -490,6 +490,22 @@ __global__ __launch_bounds__(512) void pstd_dfft(const DfftArgs<T> a)
                            rep == 0 ? mul : nullptr);
 }
 
+// sources the stage kernels inject themselves, after the update of their
+// cell (at most SRC_FUSED: more go through the solver's separate
+// one-thread source kernel): cell[s] of source s; type, cell, offset and
+// length of all nsrc sources in the device table tab, the functions in fct
+constexpr int SRC_FUSED = 8;
+template <typename T>
+struct StageSrc {
+    int n;                       // fused sources (0: none, or injected separately)
+    int cell[SRC_FUSED];
+    const int64_t *tab;
+    const T *fct;
+    int nsrc;
+    T dtw, dtt;                  // the stage's weights of the source function
+    int64_t it;
+};
+
 struct DJob {
     int field, slot;
     bool fwd;

@@ -50,6 +50,7 @@ struct PveStage {
     int nx2, nz2, np;
     unsigned N;
     Layer<T> Ls[L_NSET];
+    pstd::StageSrc<T> src;
 };
 
 template <typename T>
@@ -60,8 +61,28 @@ __device__ __forceinline__ void rk(const PveStage<T> &a, size_t q, T d)
     a.w[q] = (a.acc ? a.w[q] : s) + a.b * d;
 }
 
+// add_src_rk4 (src.c:77-112) for source s: p (Sf, Bulk), tau_xx (Sx, Bulk,
+// Bulk_s), tau_zz (Sz, Bulk, Bulk_s), tau_xz (Sxz)
+template <typename T>
+__device__ __forceinline__ void pve_add_src(T *w, T *wt, size_t N, int s, int nsrc, const int64_t *tab, const T *fct,
+                                            T dtw, T dtt, int64_t it)
+{
+    const int64_t k = 2 * it + 1;
+    const int64_t typ = tab[s], cell = tab[nsrc + s], off = tab[2 * nsrc + s], len = tab[3 * nsrc + s];
+    if (k >= len) return;
+    const T v = fct[off + k];
+    auto add = [&](size_t q) {
+        if (dtt != T(0)) wt[q + cell] += dtt * v;
+        w[q + cell] += dtw * v;
+    };
+    if (typ == SM_SF || typ == SM_BULK) add(W_P * N);
+    if (typ == SM_SX || typ == SM_BULK || typ == SM_BULK_S) add(W_TXX * N);
+    if (typ == SM_SZ || typ == SM_BULK || typ == SM_BULK_S) add(W_TZZ * N);
+    if (typ == SM_SXZ) add(W_TXZ * N);
+}
+
 // propagate.c:884-1329 (isotropic) or 1514-1958 (VTI) from the twelve
-// derivatives on, then eq. (79)-(83)
+// derivatives on, then eq. (79)-(83), then the sources on this cell
 template <typename T, bool VTI>
 __global__ __launch_bounds__(256) void pve_stage(const PveStage<T> a)
 {
@@ -111,6 +132,9 @@ __global__ __launch_bounds__(256) void pve_stage(const PveStage<T> a)
     }
     // tau_xz (1259-1311)
     rk(a, W_TXZ * N + n, m[M_MUIJ * N + n] * (d[L_DZ_VX] + d[L_DX_VZ]));
+    for (int s = 0; s < a.src.n; ++s)
+        if (a.src.cell[s] == (int)n)
+            pve_add_src(a.w, a.wt, N, s, a.src.nsrc, a.src.tab, a.src.fct, a.src.dtw, a.src.dtt, a.src.it);
 }
 
 // the analytic step of pve_iso_pml.c:1255-1262 on the step-start state:
@@ -128,26 +152,14 @@ __global__ void pve_split(T *ws, const T *mat, unsigned N)
     ws[W_QZ * NN + n] = mat[M_CQJ * NN + n] * qz;
 }
 
-// add_src_rk4 (src.c:77-112), in source order on one thread
+// all sources in order on one thread (more sources than the stage kernel
+// takes)
 template <typename T>
 __global__ void pve_src(T *w, T *wt, size_t N, int nsrc, const int64_t *tab, const T *fct, T dtw, T dtt,
                         int64_t it)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int64_t k = 2 * it + 1;
-    for (int s = 0; s < nsrc; ++s) {
-        const int64_t typ = tab[s], cell = tab[nsrc + s], off = tab[2 * nsrc + s], len = tab[3 * nsrc + s];
-        if (k >= len) continue;
-        const T v = fct[off + k];
-        auto add = [&](size_t q) {
-            if (dtt != T(0)) wt[q + cell] += dtt * v;
-            w[q + cell] += dtw * v;
-        };
-        if (typ == SM_SF || typ == SM_BULK) add(W_P * N);
-        if (typ == SM_SX || typ == SM_BULK || typ == SM_BULK_S) add(W_TXX * N);
-        if (typ == SM_SZ || typ == SM_BULK || typ == SM_BULK_S) add(W_TZZ * N);
-        if (typ == SM_SXZ) add(W_TXZ * N);
-    }
+    for (int s = 0; s < nsrc; ++s) pve_add_src(w, wt, N, s, nsrc, tab, fct, dtw, dtt, it);
 }
 
 __host__ __device__ inline int comp_slot(int c)
@@ -246,6 +258,7 @@ struct sm_pve {
     size_t mem_dx = 0, mem_dz = 0;
     sm_dbuf src, rcv, trace;
     size_t nsrc = 0, nrec = 0, max_steps = 0, rcv_step0 = 0, step = 0;
+    size_t src_cells[pstd::SRC_FUSED] = {};   // cells of the sources the stage kernel injects
     // kinetic energy of a region: coefficients, partial sums, one value per step
     sm_dbuf ecoef, epart, etrace;
     size_t ei1 = 0, ej1 = 0, eni = 0, enj = 0, e_max = 0, e_step0 = 0;
@@ -296,7 +309,7 @@ int div_curl(sm_pve *s, const T *x, bool div, bool curl)
 }
 
 template <typename T>
-int stage(sm_pve *s, int tabs, bool last, const T *x, T a, T b, int set_t, int acc)
+int stage(sm_pve *s, int tabs, bool last, const T *x, T a, T b, int set_t, int acc, T dtw, T dtt, int64_t it)
 {
     hipStream_t st = s->ctx->stream;
     const size_t N = s->N;
@@ -315,6 +328,15 @@ int stage(sm_pve *s, int tabs, bool last, const T *x, T a, T b, int set_t, int a
     k.w = STATE<T>(s, s->iw); k.wt = STATE<T>(s, s->iwt);
     k.a = a; k.b = b; k.set_t = set_t; k.acc = acc;
     k.nx2 = s->g.nx2; k.nz2 = s->g.nz2; k.np = np; k.N = (unsigned)N;
+    k.src.n = 0;
+    if (s->nsrc && s->nsrc <= (size_t)pstd::SRC_FUSED) {
+        k.src.n = (int)s->nsrc;
+        for (size_t q = 0; q < s->nsrc; ++q) k.src.cell[q] = (int)s->src_cells[q];
+        k.src.tab = (const int64_t *)s->src.ptr;
+        k.src.fct = (const T *)(k.src.tab + 4 * s->nsrc);
+        k.src.nsrc = (int)s->nsrc;
+        k.src.dtw = dtw; k.src.dtt = dtt; k.src.it = it;
+    }
     // m2 <- m1 over (2np+1) rows after every propagation (propagate.c:
     // 1313-1328 and 1940-1957); m0 <- m1 at the start of a step, over 2np+1
     // rows (pve_iso_pml.c:1270-1285) or 2np rows (pve_vti_pml.c:1226-1241:
@@ -376,9 +398,9 @@ int run(sm_pve *s, size_t nsteps)
         for (int q = 0; q < 4; ++q) {
             const T *x = q == 0 ? STATE<T>(s, s->iws) : STATE<T>(s, s->iwt);
             int rc = stage<T>(s, stages[q].tabs, q == 3, x, stages[q].a, stages[q].b, q < 3 ? 1 : 0,
-                              stages[q].acc);
+                              stages[q].acc, stages[q].sw, stages[q].st, it);
             if (rc != SM_OK) return rc;
-            if (s->nsrc)
+            if (s->nsrc > (size_t)pstd::SRC_FUSED)
                 pve_src<T><<<1, 64, 0, st>>>(STATE<T>(s, s->iw), STATE<T>(s, s->iwt), N, (int)s->nsrc, stab, sfct,
                                              stages[q].sw, stages[q].st, it);
         }
@@ -713,6 +735,7 @@ int sm_pve_set_sources(sm_pve *s, size_t nsrc, const int *type, const size_t *id
     int64_t *t = (int64_t *)h.data();
     size_t off = 0;
     for (size_t n = 0; n < nsrc; ++n) {
+        if (n < (size_t)pstd::SRC_FUSED) s->src_cells[n] = idx[n];
         t[n] = type[n]; t[nsrc + n] = (int64_t)idx[n]; t[2 * nsrc + n] = (int64_t)off;
         t[3 * nsrc + n] = (int64_t)length[n];
         for (size_t k = 0; k < length[n]; ++k) {

@@ -51,9 +51,27 @@ struct ShStage {
     int nx2, nz2, np;
     unsigned N;
     Layer<T> L[L_NSET];
+    pstd::StageSrc<T> src;
 };
 
-// propagate.c:2280-2462 from the four derivatives on, then eq. (79)-(83)
+// add_src_ve_sh_rk4 (src.c:144-168) for source s: Sxy to tau_xy, Syz to
+// tau_yz, other types nothing
+template <typename T>
+__device__ __forceinline__ void sh_add_src(T *w, T *wt, size_t N, int s, int nsrc, const int64_t *tab, const T *fct,
+                                           T dtw, T dtt, int64_t it)
+{
+    const int64_t k = 2 * it + 1;
+    const int64_t typ = tab[s], cell = tab[nsrc + s], off = tab[2 * nsrc + s], len = tab[3 * nsrc + s];
+    if (k >= len) return;
+    const size_t q = typ == SM_SXY ? 0 : typ == SM_SYZ ? N : ~(size_t)0;
+    if (q == ~(size_t)0) return;
+    const T v = fct[off + k];
+    if (dtt != T(0)) wt[q + cell] += dtt * v;
+    w[q + cell] += dtw * v;
+}
+
+// propagate.c:2280-2462 from the four derivatives on, then eq. (79)-(83),
+// then the sources on this cell
 template <typename T>
 __global__ __launch_bounds__(256) void sh_stage(const ShStage<T> a)
 {
@@ -84,24 +102,19 @@ __global__ __launch_bounds__(256) void sh_stage(const ShStage<T> a)
         if (a.set_t) a.wt[q] = s + a.a * dl[c];
         a.w[q] = (a.acc ? a.w[q] : s) + a.b * dl[c];
     }
+    for (int s = 0; s < a.src.n; ++s)
+        if (a.src.cell[s] == (int)n)
+            sh_add_src(a.w, a.wt, N, s, a.src.nsrc, a.src.tab, a.src.fct, a.src.dtw, a.src.dtt, a.src.it);
 }
 
-// add_src_ve_sh_rk4 (src.c:144-168), in source order on one thread
+// all sources in order on one thread (more sources than the stage kernel
+// takes)
 template <typename T>
 __global__ void sh_src(T *w, T *wt, size_t N, int nsrc, const int64_t *tab, const T *fct, T dtw, T dtt,
                        int64_t it)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int64_t k = 2 * it + 1;
-    for (int s = 0; s < nsrc; ++s) {
-        const int64_t typ = tab[s], cell = tab[nsrc + s], off = tab[2 * nsrc + s], len = tab[3 * nsrc + s];
-        if (k >= len) continue;
-        const size_t q = typ == SM_SXY ? 0 : typ == SM_SYZ ? N : ~(size_t)0;
-        if (q == ~(size_t)0) continue;
-        const T v = fct[off + k];
-        if (dtt != T(0)) wt[q + cell] += dtt * v;
-        w[q + cell] += dtw * v;
-    }
+    for (int s = 0; s < nsrc; ++s) sh_add_src(w, wt, N, s, nsrc, tab, fct, dtw, dtt, it);
 }
 
 template <typename T>
@@ -133,6 +146,7 @@ struct sm_sh {
     size_t mem_dx = 0, mem_dxv = 0, mem_dz = 0;   // entries of dx_txy, dx_vy, dz_* per copy
     sm_dbuf src, rcv, trace;
     size_t nsrc = 0, nrec = 0, max_steps = 0, rcv_step0 = 0, step = 0;
+    size_t src_cells[pstd::SRC_FUSED] = {};   // cells of the sources the stage kernel injects
     bool timing = false;
     double timed_ms = 0.0;
     size_t timed_steps = 0;
@@ -165,7 +179,7 @@ template <typename T> T *MEM(sm_sh *s, int copy, int set)   // set 0 dx_txy, 1 d
 // one propagation (stage tables `tabs`) of input x, then the Runge-Kutta
 // update of the fused kernel
 template <typename T>
-int stage(sm_sh *s, int tabs, bool last, const T *x, T a, T b, int set_t, int acc)
+int stage(sm_sh *s, int tabs, bool last, const T *x, T a, T b, int set_t, int acc, T dtw, T dtt, int64_t it)
 {
     hipStream_t st = s->ctx->stream;
     const size_t N = s->N;
@@ -182,6 +196,15 @@ int stage(sm_sh *s, int tabs, bool last, const T *x, T a, T b, int set_t, int ac
     k.w = STATE<T>(s, s->iw); k.wt = STATE<T>(s, s->iwt);
     k.a = a; k.b = b; k.set_t = set_t; k.acc = acc;
     k.nx2 = s->g.nx2; k.nz2 = s->g.nz2; k.np = np; k.N = (unsigned)N;
+    k.src.n = 0;
+    if (s->nsrc && s->nsrc <= (size_t)pstd::SRC_FUSED) {
+        k.src.n = (int)s->nsrc;
+        for (size_t q = 0; q < s->nsrc; ++q) k.src.cell[q] = (int)s->src_cells[q];
+        k.src.tab = (const int64_t *)s->src.ptr;
+        k.src.fct = (const T *)(k.src.tab + 4 * s->nsrc);
+        k.src.nsrc = (int)s->nsrc;
+        k.src.dtw = dtw; k.src.dtt = dtt; k.src.it = it;
+    }
     // copies of the reference: m2 <- m1 over (2np+1) rows of nz2 (dx_*) or
     // nx2 (dz_*) entries after every propagation (propagate.c:2451-2461);
     // m0 <- m1 over 2np rows at the start of a step (ve_vti_sh_pml.c:648-655)
@@ -239,9 +262,9 @@ int run(sm_sh *s, size_t nsteps)
         for (int q = 0; q < 4; ++q) {
             const T *x = q == 0 ? STATE<T>(s, s->iws) : STATE<T>(s, s->iwt);
             int rc = stage<T>(s, stages[q].tabs, q == 3, x, stages[q].a, stages[q].b, q < 3 ? 1 : 0,
-                              stages[q].acc);
+                              stages[q].acc, stages[q].sw, stages[q].st, it);
             if (rc != SM_OK) return rc;
-            if (s->nsrc)
+            if (s->nsrc > (size_t)pstd::SRC_FUSED)
                 sh_src<T><<<1, 64, 0, st>>>(STATE<T>(s, s->iw), STATE<T>(s, s->iwt), N, (int)s->nsrc, stab, sfct,
                                             stages[q].sw, stages[q].st, it);
         }
@@ -434,6 +457,7 @@ int sm_sh_set_sources(sm_sh *s, size_t nsrc, const int *type, const size_t *idx,
     int64_t *t = (int64_t *)h.data();
     size_t off = 0;
     for (size_t n = 0; n < nsrc; ++n) {
+        if (n < (size_t)pstd::SRC_FUSED) s->src_cells[n] = idx[n];
         t[n] = type[n]; t[nsrc + n] = (int64_t)idx[n]; t[2 * nsrc + n] = (int64_t)off;
         t[3 * nsrc + n] = (int64_t)length[n];
         for (size_t k = 0; k < length[n]; ++k) {

@@ -53,7 +53,27 @@ struct VeStage {
     int nx2, nz2, np, L;
     unsigned N;
     Layer<T> Ls[L_NSET];
+    pstd::StageSrc<T> src;
 };
+
+// add_src_ve_rk4 (src.c:114-142) for source s: Sx to tau_xx, Sz to tau_zz,
+// Bulk to both, Sxz to tau_xz, other types nothing
+template <typename T>
+__device__ __forceinline__ void ve_add_src(T *w, T *wt, size_t N, int s, int nsrc, const int64_t *tab, const T *fct,
+                                           T dtw, T dtt, int64_t it)
+{
+    const int64_t k = 2 * it + 1;
+    const int64_t typ = tab[s], cell = tab[nsrc + s], off = tab[2 * nsrc + s], len = tab[3 * nsrc + s];
+    if (k >= len) return;
+    const T v = fct[off + k];
+    auto add = [&](size_t q) {
+        if (dtt != T(0)) wt[q + cell] += dtt * v;
+        w[q + cell] += dtw * v;
+    };
+    if (typ == SM_SX || typ == SM_BULK) add(W_TXX * N);
+    if (typ == SM_SZ || typ == SM_BULK) add(W_TZZ * N);
+    if (typ == SM_SXZ) add(W_TXZ * N);
+}
 
 template <typename T>
 __device__ __forceinline__ void rk(const VeStage<T> &a, size_t q, T d)
@@ -100,27 +120,19 @@ __global__ __launch_bounds__(256) void ve_stage(const VeStage<T> a)
     const T e3 = a.x[(6 + L) * N + n];
     rk(a, (6 + L) * N + n, m[M_TS2IJ * N + n] * (m[M_ETA2IJ * N + n] * u - e3));
     rk(a, W_TXZ * N + n, m[M_C55 * N + n] * u);
+    for (int s = 0; s < a.src.n; ++s)
+        if (a.src.cell[s] == (int)n)
+            ve_add_src(a.w, a.wt, N, s, a.src.nsrc, a.src.tab, a.src.fct, a.src.dtw, a.src.dtt, a.src.it);
 }
 
-// add_src_ve_rk4 (src.c:114-142), in source order on one thread
+// all sources in order on one thread (more sources than the stage kernel
+// takes)
 template <typename T>
 __global__ void ve_src(T *w, T *wt, size_t N, int nsrc, const int64_t *tab, const T *fct, T dtw, T dtt,
                        int64_t it)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int64_t k = 2 * it + 1;
-    for (int s = 0; s < nsrc; ++s) {
-        const int64_t typ = tab[s], cell = tab[nsrc + s], off = tab[2 * nsrc + s], len = tab[3 * nsrc + s];
-        if (k >= len) continue;
-        const T v = fct[off + k];
-        auto add = [&](size_t q) {
-            if (dtt != T(0)) wt[q + cell] += dtt * v;
-            w[q + cell] += dtw * v;
-        };
-        if (typ == SM_SX || typ == SM_BULK) add(W_TXX * N);
-        if (typ == SM_SZ || typ == SM_BULK) add(W_TZZ * N);
-        if (typ == SM_SXZ) add(W_TXZ * N);
-    }
+    for (int s = 0; s < nsrc; ++s) ve_add_src(w, wt, N, s, nsrc, tab, fct, dtw, dtt, it);
 }
 
 __host__ __device__ inline int comp_slot(int c)
@@ -154,6 +166,7 @@ struct sm_ve {
     size_t mem_dx = 0, mem_dz = 0;
     sm_dbuf src, rcv, trace;
     size_t nsrc = 0, nrec = 0, max_steps = 0, rcv_step0 = 0, step = 0;
+    size_t src_cells[pstd::SRC_FUSED] = {};   // cells of the sources the stage kernel injects
     bool timing = false;
     double timed_ms = 0.0;
     size_t timed_steps = 0;
@@ -181,7 +194,7 @@ template <typename T> T *MEM(sm_ve *s, int copy, int set)
 // one propagation of x; the fused kernel writes w and the next stage's
 // input into S[iwt2] (never x)
 template <typename T>
-int stage(sm_ve *s, int tabs, bool last, const T *x, T a, T b, int set_t, int acc)
+int stage(sm_ve *s, int tabs, bool last, const T *x, T a, T b, int set_t, int acc, T dtw, T dtt, int64_t it)
 {
     hipStream_t st = s->ctx->stream;
     const size_t N = s->N;
@@ -200,6 +213,15 @@ int stage(sm_ve *s, int tabs, bool last, const T *x, T a, T b, int set_t, int ac
     k.w = STATE<T>(s, s->iw); k.wt = STATE<T>(s, s->iwt2);
     k.a = a; k.b = b; k.set_t = set_t; k.acc = acc;
     k.nx2 = s->g.nx2; k.nz2 = s->g.nz2; k.np = np; k.L = s->p.L; k.N = (unsigned)N;
+    k.src.n = 0;
+    if (s->nsrc && s->nsrc <= (size_t)pstd::SRC_FUSED) {
+        k.src.n = (int)s->nsrc;
+        for (size_t q = 0; q < s->nsrc; ++q) k.src.cell[q] = (int)s->src_cells[q];
+        k.src.tab = (const int64_t *)s->src.ptr;
+        k.src.fct = (const T *)(k.src.tab + 4 * s->nsrc);
+        k.src.nsrc = (int)s->nsrc;
+        k.src.dtw = dtw; k.src.dtt = dtt; k.src.it = it;
+    }
     // m2 <- m1 over (2np+1) rows after every propagation (propagate.c:
     // 2265-2277); m0 <- m1 over 2np rows at the start of a step
     // (ve_vti_pml.c:727-738)
@@ -257,10 +279,10 @@ int run(sm_ve *s, size_t nsteps)
         for (int q = 0; q < 4; ++q) {
             const T *x = q == 0 ? STATE<T>(s, s->iws) : STATE<T>(s, s->iwt);
             int rc = stage<T>(s, stages[q].tabs, q == 3, x, stages[q].a, stages[q].b, q < 3 ? 1 : 0,
-                              stages[q].acc);
+                              stages[q].acc, stages[q].sw, stages[q].st, it);
             if (rc != SM_OK) return rc;
             std::swap(s->iwt, s->iwt2);   // the stage's wt is the next input
-            if (s->nsrc)
+            if (s->nsrc > (size_t)pstd::SRC_FUSED)
                 ve_src<T><<<1, 64, 0, st>>>(STATE<T>(s, s->iw), STATE<T>(s, s->iwt), N, (int)s->nsrc, stab, sfct,
                                             stages[q].sw, stages[q].st, it);
         }
@@ -508,6 +530,7 @@ int sm_ve_set_sources(sm_ve *s, size_t nsrc, const int *type, const size_t *idx,
     int64_t *t = (int64_t *)h.data();
     size_t off = 0;
     for (size_t n = 0; n < nsrc; ++n) {
+        if (n < (size_t)pstd::SRC_FUSED) s->src_cells[n] = idx[n];
         t[n] = type[n]; t[nsrc + n] = (int64_t)idx[n]; t[2 * nsrc + n] = (int64_t)off;
         t[3 * nsrc + n] = (int64_t)length[n];
         for (size_t k = 0; k < length[n]; ++k) {
