@@ -637,6 +637,16 @@ int sm_pve_set_energy_axes(sm_pve *s, size_t i1, size_t i2, size_t j1, size_t j2
                            size_t max_steps);
 /* count values from row it0 of the energy record */
 int sm_pve_get_energy(sm_pve *s, size_t it0, size_t count, double *out);
+/* Checkpoints (no counterpart API: the reference's save_checkpoint and
+ * read_checkpoint2, io_utils.c:2355-2811, fwrite its structs): the state a
+ * run continues from -- step count, current fields, CPML memory variables,
+ * the traces and energy recorded so far -- as one host blob of
+ * sm_pve_state_bytes bytes.  sm_pve_set_state takes a blob from a solver of
+ * the same grid, storage, receivers and energy region (SM_ERR_ARG
+ * otherwise); the next sm_pve_run continues the saved run bit for bit. */
+int sm_pve_state_bytes(const sm_pve *s, size_t *bytes);
+int sm_pve_get_state(sm_pve *s, void *buf, size_t bytes);
+int sm_pve_set_state(sm_pve *s, const void *buf, size_t bytes);
 int sm_pve_set_timing(sm_pve *s, int on);
 int sm_pve_get_timing(const sm_pve *s, double *ms_total, size_t *steps);
 int sm_pve_current_step(const sm_pve *s, size_t *it);

@@ -208,6 +208,9 @@ _SIGS = {
     "sm_pve_set_energy": (C.c_int, [C.c_void_p] + [C.c_size_t] * 4 + [_dbl_p] * 3 + [C.c_size_t]),
     "sm_pve_set_energy_axes": (C.c_int, [C.c_void_p] + [C.c_size_t] * 4 + [_dbl_p] * 6 + [C.c_size_t]),
     "sm_pve_get_energy": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, _dbl_p]),
+    "sm_pve_state_bytes": (C.c_int, [C.c_void_p, _size_p]),
+    "sm_pve_get_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "sm_pve_set_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "sm_pve_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sm_pve_get_timing": (C.c_int, [C.c_void_p, _dbl_p, _size_p]),
     "sm_pve_current_step": (C.c_int, [C.c_void_p, _size_p]),

@@ -838,6 +838,75 @@ int sm_pve_run(sm_pve *s, size_t nsteps)
     return s->elt == 8 ? run<double>(s, nsteps) : run<float>(s, nsteps);
 }
 
+// checkpoint blob: a header of 10 64-bit words (magic, element size, N,
+// field values, memory values, trace values, energy rows, step, first
+// trace step, first energy step), then the fields, the three memory-variable
+// copies, the traces (element size) and the energy record (FP64)
+namespace {
+constexpr uint64_t CK_MAGIC = 0x31765045504D53ull;   // "SMPEPv1"
+size_t ck_mem(const sm_pve *s) { return 3 * 6 * (s->mem_dx + s->mem_dz); }
+}  // namespace
+
+int sm_pve_state_bytes(const sm_pve *s, size_t *bytes)
+{
+    SM_CHECK(s && bytes, SM_ERR_ARG, "sm_pve_state_bytes: NULL argument");
+    *bytes = 10 * sizeof(uint64_t) + s->elt * (W_NF * s->N + ck_mem(s) + s->nrec * s->max_steps) +
+             sizeof(double) * s->e_max;
+    return SM_OK;
+}
+
+int sm_pve_get_state(sm_pve *s, void *buf, size_t bytes)
+{
+    size_t need = 0;
+    SM_CHECK(s && buf, SM_ERR_ARG, "sm_pve_get_state: NULL argument");
+    sm_pve_state_bytes(s, &need);
+    SM_CHECK(bytes >= need, SM_ERR_ARG, "sm_pve_get_state: %zu bytes, %zu needed", bytes, need);
+    SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    uint64_t *h = (uint64_t *)buf;
+    const uint64_t hdr[10] = {CK_MAGIC, s->elt, s->N, W_NF * s->N, ck_mem(s), s->nrec * s->max_steps, s->e_max,
+                              s->step, s->rcv_step0, s->e_step0};
+    std::memcpy(h, hdr, sizeof(hdr));
+    char *p = (char *)buf + sizeof(hdr);
+    hipStream_t st = s->ctx->stream;
+    const size_t fw = W_NF * s->N * s->elt, mw = ck_mem(s) * s->elt, tw = s->nrec * s->max_steps * s->elt;
+    SM_HIP(hipMemcpyAsync(p, (const char *)s->state.ptr + (size_t)s->iw * W_NF * s->N * s->elt, fw,
+                          hipMemcpyDeviceToHost, st));
+    SM_HIP(hipMemcpyAsync(p + fw, s->mem.ptr, mw, hipMemcpyDeviceToHost, st));
+    if (tw) SM_HIP(hipMemcpyAsync(p + fw + mw, s->trace.ptr, tw, hipMemcpyDeviceToHost, st));
+    if (s->e_max)
+        SM_HIP(hipMemcpyAsync(p + fw + mw + tw, s->etrace.ptr, s->e_max * sizeof(double), hipMemcpyDeviceToHost, st));
+    SM_HIP(hipStreamSynchronize(st));
+    return SM_OK;
+}
+
+int sm_pve_set_state(sm_pve *s, const void *buf, size_t bytes)
+{
+    size_t need = 0;
+    SM_CHECK(s && buf, SM_ERR_ARG, "sm_pve_set_state: NULL argument");
+    sm_pve_state_bytes(s, &need);
+    SM_CHECK(bytes >= need, SM_ERR_ARG, "sm_pve_set_state: %zu bytes, %zu needed", bytes, need);
+    uint64_t h[10];
+    std::memcpy(h, buf, sizeof(h));
+    SM_CHECK(h[0] == CK_MAGIC && h[1] == s->elt && h[2] == s->N && h[3] == W_NF * s->N && h[4] == ck_mem(s) &&
+                 h[5] == s->nrec * s->max_steps && h[6] == s->e_max,
+             SM_ERR_ARG, "sm_pve_set_state: the state was saved from a solver of another shape, storage or record");
+    SM_CHECK(set_device(s->ctx) == SM_OK, SM_ERR_HIP, "hipSetDevice failed");
+    const char *p = (const char *)buf + sizeof(h);
+    hipStream_t st = s->ctx->stream;
+    const size_t fw = W_NF * s->N * s->elt, mw = ck_mem(s) * s->elt, tw = s->nrec * s->max_steps * s->elt;
+    SM_HIP(hipMemcpyAsync((char *)s->state.ptr + (size_t)s->iw * W_NF * s->N * s->elt, p, fw, hipMemcpyHostToDevice,
+                          st));
+    SM_HIP(hipMemcpyAsync(s->mem.ptr, p + fw, mw, hipMemcpyHostToDevice, st));
+    if (tw) SM_HIP(hipMemcpyAsync(s->trace.ptr, p + fw + mw, tw, hipMemcpyHostToDevice, st));
+    if (s->e_max)
+        SM_HIP(hipMemcpyAsync(s->etrace.ptr, p + fw + mw + tw, s->e_max * sizeof(double), hipMemcpyHostToDevice, st));
+    SM_HIP(hipStreamSynchronize(st));
+    s->step = h[7];
+    s->rcv_step0 = h[8];
+    s->e_step0 = h[9];
+    return SM_OK;
+}
+
 int sm_pve_set_timing(sm_pve *s, int on)
 {
     SM_CHECK(s, SM_ERR_ARG, "sm_pve_set_timing: NULL solver");

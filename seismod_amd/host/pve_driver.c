@@ -18,8 +18,11 @@
  * <base>_vel_slowP.nc ... <base>_att_S.nc, or for the VTI program
  * <base>_vel_qP1_x.nc ... <base>_att_qS_z.nc); pml_median 1 smooths the
  * isotropic medium's layers (check_model then still reports the unsmoothed
- * model).  Checkpoints (chkpt_inc, -c) are not available: the program says
- * so and stops.
+ * model).  chkpt_inc > 0 saves chkpt/<base>_<it>.dat after every
+ * chkpt_inc-th iteration and -c <file> continues a run from one; the
+ * checkpoint holds the parameters and the solver state (fields, CPML memory
+ * variables, the traces and energy so far), not the inputs, so the model,
+ * source and output files it names must still be there at the restart.
  *
  * Set-up and update arithmetic live in the library (sm_pve_*); this driver
  * reads the inputs, drives the time loop and writes the records.
@@ -212,15 +215,78 @@ static void check_model_vti(const sm_params *p, const sm_grid *g, size_t np, dou
     for (int k = 0; k < 6; ++k) { free(vel[k]); free(att[k]); }
 }
 
+/* Checkpoint files, a layout of our own (the reference's save_checkpoint and
+ * read_checkpoint1/2, io_utils.c:2355-2811, fwrite its structs and the
+ * model): 8 bytes "SMPVECK", the medium (0 isotropic, 1 VTI), the
+ * iteration after which it was saved, the run's sm_params, the state's size
+ * and the state of sm_pve_get_state.  Written as chkpt/<base>_<it>.dat
+ * (io_utils.c:2378-2383). */
+static const char ck_magic[8] = "SMPVECK";
+
+static void save_checkpoint(const sm_params *p, int vti, size_t it, sm_pve *sim)
+{
+    size_t bytes = 0;
+    check(sm_pve_state_bytes(sim, &bytes), "sm_pve_state_bytes");
+    void *blob = malloc(bytes);
+    check(sm_pve_get_state(sim, blob, bytes), "sm_pve_get_state");
+    struct stat sb;
+    if (stat("chkpt", &sb) < 0) mkdir("chkpt", S_IRWXU);
+    char fname[SM_NAME_LEN + 48];
+    snprintf(fname, sizeof(fname), "chkpt/%s_%zu.dat", p->basename, it);
+    FILE *fid = fopen(fname, "wb");
+    if (!fid) {
+        fprintf(stderr, "Error, cannot open %s for writing\n", fname);
+        exit(1);
+    }
+    const int medium = vti;
+    int ok = fwrite(ck_magic, 1, 8, fid) == 8 && fwrite(&medium, sizeof(int), 1, fid) == 1 &&
+             fwrite(&it, sizeof(size_t), 1, fid) == 1 && fwrite(p, sizeof(*p), 1, fid) == 1 &&
+             fwrite(&bytes, sizeof(size_t), 1, fid) == 1 && fwrite(blob, 1, bytes, fid) == bytes;
+    ok = (fclose(fid) == 0) && ok;
+    free(blob);
+    if (!ok) {
+        fprintf(stderr, "Error, cannot write %s\n", fname);
+        exit(1);
+    }
+}
+
+/* the parameters, the iteration and the state of a checkpoint file */
+static void *read_checkpoint(const char *fname, int vti, sm_params *p, size_t *it, size_t *bytes)
+{
+    FILE *fid = fopen(fname, "rb");
+    if (!fid) {
+        fprintf(stderr, "Error, cannot open checkpoint file %s for reading\n", fname);
+        exit(1);
+    }
+    char magic[8];
+    int medium = -1;
+    void *blob = NULL;
+    int ok = fread(magic, 1, 8, fid) == 8 && !memcmp(magic, ck_magic, 8) &&
+             fread(&medium, sizeof(int), 1, fid) == 1 && fread(it, sizeof(size_t), 1, fid) == 1 &&
+             fread(p, sizeof(*p), 1, fid) == 1 && fread(bytes, sizeof(size_t), 1, fid) == 1 &&
+             *bytes < ((size_t)1 << 40) && (blob = malloc(*bytes ? *bytes : 1)) != NULL &&
+             fread(blob, 1, *bytes, fid) == *bytes;
+    fclose(fid);
+    if (!ok || medium != vti) {
+        fprintf(stderr, "Error, %s is not a checkpoint of %s\n", fname, vti ? "pve_vti_pml" : "pve_iso_pml");
+        exit(1);
+    }
+    return blob;
+}
+
 int sm_pve_program(int argc, char *argv[], int vti)
 {
     sm_params p;
     sm_set_defaults(&p, 0);
-    sm_process_args(argc, argv, &p, vti ? usage_vti : usage_iso);
-    const char *prog = vti ? "pve_vti_pml" : "pve_iso_pml";
-    if (p.chkpt_inc > 0) {
-        fprintf(stderr, "Error: checkpointing (chkpt_inc %d) is not available in this %s\n", p.chkpt_inc, prog);
-        return 1;
+    char ckfile[SM_NAME_LEN] = "";
+    const int restart = sm_process_args_ck(argc, argv, &p, vti ? usage_vti : usage_iso, ckfile, sizeof(ckfile));
+    size_t itstart = 0, ck_bytes = 0;
+    void *ck_state = NULL;
+    if (restart) {
+        /* pve_iso_pml.c:525-531: the run continues after the saved iteration */
+        ck_state = read_checkpoint(ckfile, vti, &p, &itstart, &ck_bytes);
+        itstart++;
+        if (sm_verbose) fprintf(stdout, "\n  Starting with checkpoint file %s\n", ckfile);
     }
     const size_t np = p.np;   /* 0 when abs is 0 */
     const double dt = p.dt;
@@ -384,13 +450,18 @@ int sm_pve_program(int argc, char *argv[], int vti)
         }
     check(sm_pve_set_receivers(sim, ntr, ridx, rcomp, nsteps), "sm_pve_set_receivers");
 
+    if (ck_state) {
+        check(sm_pve_set_state(sim, ck_state, ck_bytes), "sm_pve_set_state");
+        free(ck_state);
+    }
+
     if (sm_verbose) {
         fprintf(stdout, "Starting main loop (%zu iterations)\n", nsteps);
         fflush(stdout);
     }
     double *field = malloc(n * sizeof(double));
-    size_t done = 0;
-    for (size_t it = 0; it < nsteps; ++it) {
+    size_t done = itstart;
+    for (size_t it = itstart; it < nsteps; ++it) {
         const double t = dt * (it + 1);
         if ((it + 1) % 50 == 0 && sm_verbose) {
             fprintf(stdout, "  Iteration %zu, t = %g s\n", it + 1, t);
@@ -401,9 +472,15 @@ int sm_pve_program(int argc, char *argv[], int vti)
             if (out.r[r].type == REC_SNAPSHOT && recorded(out.r[r].comp, vti) &&
                 sm_record_due(it, dt, out.r[r].t0, out.r[r].dt))
                 snap = 1;
-        if (!snap && it + 1 < nsteps) continue;
+        /* pve_iso_pml.c:1406-1411 */
+        const int ck = p.chkpt_inc > 0 && it > 0 && it % (size_t)p.chkpt_inc == 0;
+        if (!snap && !ck && it + 1 < nsteps) continue;
         check(sm_pve_run(sim, it + 1 - done), "sm_pve_run");
         done = it + 1;
+        if (ck) {
+            if (sm_verbose > 1) printf("Saving checkpoint data at iteration %zu\n", it);
+            save_checkpoint(&p, vti, it, sim);
+        }
         for (size_t r = 0; r < out.nrec; ++r)
             if (out.r[r].type == REC_SNAPSHOT && recorded(out.r[r].comp, vti) &&
                 sm_record_due(it, dt, out.r[r].t0, out.r[r].dt)) {

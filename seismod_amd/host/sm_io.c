@@ -62,16 +62,28 @@ static void usage(const char *prog, FILE *out, int code, const char *text)
 
 void sm_process_args(int argc, char *const argv[], sm_params *p, const char *usage_text)
 {
+    sm_process_args_ck(argc, argv, p, usage_text, NULL, 0);
+}
+
+int sm_process_args_ck(int argc, char *const argv[], sm_params *p, const char *usage_text, char *ckfile,
+                       size_t ckfile_len)
+{
     /* io_utils.c:156-294 */
     char paramfile[256] = "";
-    int have_p = 0, opt;
+    int have_p = 0, have_c = 0, opt;
     while ((opt = getopt(argc, argv, "c:hp:v")) != -1) {
         switch (opt) {
         case 'c':
             /* io_utils.c:167-171 returns before the .par is read, so e_iso and
-             * a_iso_3d would run on uninitialised file names; refuse instead */
-            fprintf(stderr, "Error: checkpoint restart (-c) is not available for %s\n", argv[0]);
-            exit(1);
+             * a_iso_3d would run on uninitialised file names; programs
+             * without checkpoints refuse instead */
+            if (!ckfile) {
+                fprintf(stderr, "Error: checkpoint restart (-c) is not available for %s\n", argv[0]);
+                exit(1);
+            }
+            snprintf(ckfile, ckfile_len, "%s", optarg);
+            have_c = 1;
+            break;
         case 'h':
             usage(argv[0], stdout, 0, usage_text);
             break;
@@ -86,6 +98,7 @@ void sm_process_args(int argc, char *const argv[], sm_params *p, const char *usa
             usage(argv[0], stderr, 1, usage_text);
         }
     }
+    if (have_c) return 1;   /* the parameters come from the checkpoint; -p is ignored */
     if (!have_p) usage(argv[0], stderr, 1, usage_text);
 
     FILE *fid = fopen(paramfile, "r");
@@ -125,6 +138,7 @@ void sm_process_args(int argc, char *const argv[], sm_params *p, const char *usa
     }
     fclose(fid);
     if (p->iwipe == 0) p->np = 0;   /* io_utils.c:293 */
+    return 0;
 }
 
 static FILE *open_or_die(const char *fn)

@@ -79,6 +79,11 @@ void sm_set_defaults(sm_params *p, int three_d);
  * missing -p, unreadable .par; -c (checkpoint restart) is not supported
  * by these programs and is rejected */
 void sm_process_args(int argc, char *const argv[], sm_params *p, const char *usage_text);
+/* the same for programs with checkpoints: -c names a checkpoint file, which
+ * is copied to ckfile and then 1 is returned without reading a parameter
+ * file (-p is ignored, as in the reference); 0 otherwise */
+int sm_process_args_ck(int argc, char *const argv[], sm_params *p, const char *usage_text, char *ckfile,
+                       size_t ckfile_len);
 /* read_grid_params / read_grid3d_params (io_utils.c:296-328, 384-401) */
 void sm_read_grid2d(const char *fn, sm_grid *g);
 void sm_read_grid3d(const char *fn, sm_grid *g);

@@ -863,6 +863,17 @@ class Poroviscoelastic(_Pstd):
               "sm_pve_set_energy_axes")
         self.energy_steps = max_steps
 
+    def get_state(self) -> bytes:
+        """The checkpoint blob the run continues from (sm_pve_get_state)."""
+        n = C.c_size_t()
+        check(lib.sm_pve_state_bytes(self.handle, C.byref(n)), "sm_pve_state_bytes")
+        buf = C.create_string_buffer(n.value)
+        check(lib.sm_pve_get_state(self.handle, buf, n.value), "sm_pve_get_state")
+        return buf.raw
+
+    def set_state(self, blob: bytes) -> None:
+        check(lib.sm_pve_set_state(self.handle, blob, len(blob)), "sm_pve_set_state")
+
     def energy(self, it0: int, count: int) -> np.ndarray:
         out = np.zeros(count, dtype=np.float64)
         check(lib.sm_pve_get_energy(self.handle, it0, count, _dp(out)), "sm_pve_get_energy")

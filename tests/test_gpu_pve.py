@@ -330,3 +330,61 @@ def test_pve_energy_per_axis(ctx):
     err = float(np.abs(got - want).max() / np.abs(want).max())
     note("pve_energy_axes_vs_oracle", err=err)
     assert err < PVE_FP64_TOL, err
+
+
+@pytest.mark.parametrize("case,storage", [(CASES[0], "f64"), (CASES[6], "f32")],
+                         ids=["seed1-f64", "seed11_vti-f32"])
+def test_pve_checkpoint_continues_bit_for_bit(ctx, case, storage):
+    """sm_pve_get_state / sm_pve_set_state (the state behind chkpt_inc and
+    -c, io_utils.c:2355-2811): a run stopped after k steps, saved, loaded
+    into a new solver and continued gives the same traces, energy record
+    and fields bit for bit as the run without a stop; a blob cut short or
+    saved from another grid is refused."""
+    from seismod_amd._lib import SM_ERR_ARG, SeismodError
+    d = deck(**case)
+    nx2, nz2 = d["model"][0].shape
+    i1, i2, j1, j2 = nx2 // 2 - 3, nx2 // 2 + 3, nz2 // 2 - 2, nz2 // 2 + 2
+    npts = (i2 - i1 + 1) * (j2 - j1 + 1)
+    co = [np.random.default_rng(5 + q).uniform(100, 2500, npts) for q in range(6)]
+
+    def make():
+        s = solver(ctx, d, storage)
+        s.set_sources(d["types"], d["idx"], d["Lf"], d["fct"])
+        s.set_energy_axes(i1, i2, j1, j2, *co, d["nsteps"])
+        s.set_receivers(d["ridx"], d["rcomp"], d["nsteps"])
+        return s
+
+    def finish(s):
+        out = (s.traces(0, d["nsteps"]), s.energy(0, d["nsteps"]), np.stack([s.field(k) for k in FIELDS]))
+        s.close()
+        return out
+
+    s = make()
+    s.run(d["nsteps"])
+    want = finish(s)
+    k = d["nsteps"] // 3 + 1
+    s = make()
+    s.run(k)
+    blob = s.get_state()
+    s.close()
+    s = make()
+    s.set_state(blob)
+    s.run(d["nsteps"] - k)
+    got = finish(s)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+    assert np.abs(want[1]).max() > 0 and np.abs(want[0]).max() > 0
+    s = make()
+    for bad in (blob[:-8], solver_blob_other_grid(ctx, d, storage)):
+        with pytest.raises(SeismodError) as e:
+            s.set_state(bad)
+        assert e.value.status == SM_ERR_ARG
+    s.close()
+
+
+def solver_blob_other_grid(ctx, d, storage):
+    d2 = deck(nx=d["cfg"]["nx"] + 2, nz=d["cfg"]["nz"], np_=d["np"], seed=21, vti=d["vti"])
+    s = solver(ctx, d2, storage)
+    blob = s.get_state()
+    s.close()
+    return blob

@@ -755,15 +755,44 @@ def test_pve_dispersion_limits():
     assert np.allclose(vel[2], np.sqrt(mu / rho), rtol=1e-6)
 
 
-@pytest.mark.parametrize("extra,msg", [("500 # chkpt_inc, every 500\n", "checkpointing")])
-def test_pve_iso_pml_refuses_unavailable_options(tmp_path, extra, msg):
-    """pve_iso_pml stops with a message, before reading the model or using
-    a GPU, when checkpoints are asked for; and takes -h like the other
-    programs."""
-    assert run_bin("pve_iso_pml", "-h").returncode == 0
-    (tmp_path / "run.par").write_text("model.dat # model,\nsrc.dat # source,\nrcv.dat # output,\n" + extra)
-    r = run_bin("pve_iso_pml", "-p", "run.par", cwd=tmp_path)
-    assert r.returncode == 1 and msg in r.stderr, r.stderr
+def test_process_args_with_checkpoints(tmp_path):
+    """process_args (io_utils.c:156-171): -c names a checkpoint file and
+    returns before any parameter file is read (-p is then ignored); without
+    -c the .par is read and 0 returned."""
+    par = tmp_path / "run.par"
+    par.write_text("m.dat # model,\n7 # chkpt_inc,\n")
+    ck = C.create_string_buffer(256)
+    for argv_s, ret, name in [(["x", "-c", "chk.dat"], 1, b"chk.dat"),
+                              (["x", "-p", str(tmp_path / "missing.par"), "-c", "a.dat"], 1, b"a.dat"),
+                              (["x", "-p", str(par)], 0, b"")]:
+        p = Params()
+        LIB.sm_set_defaults(C.byref(p), 0)
+        ck.value = b""
+        argv = (C.c_char_p * (len(argv_s) + 1))(*[a.encode() for a in argv_s], None)
+        C.c_int.in_dll(LIBC, "optind").value = 1   # getopt restarts at the first argument
+        r = LIB.sm_process_args_ck(len(argv_s), argv, C.byref(p), None, ck, 256)
+        assert (r, ck.value) == (ret, name)
+        assert p.chkpt_inc == (7 if ret == 0 else 0)
+
+
+def test_pve_programs_reject_foreign_checkpoints(tmp_path):
+    """-c with a file that is not a checkpoint of the program stops with a
+    message before a GPU is used; a pve_iso_pml checkpoint is refused by
+    pve_vti_pml; both programs take -h."""
+    for prog in ("pve_iso_pml", "pve_vti_pml"):
+        assert run_bin(prog, "-h").returncode == 0
+        r = run_bin(prog, "-c", "missing.dat", cwd=tmp_path)
+        assert r.returncode == 1 and "cannot open checkpoint file" in r.stderr, r.stderr
+    (tmp_path / "junk.dat").write_bytes(b"not a checkpoint at all" * 10)
+    r = run_bin("pve_iso_pml", "-c", "junk.dat", cwd=tmp_path)
+    assert r.returncode == 1 and "not a checkpoint of pve_iso_pml" in r.stderr, r.stderr
+    # the header of an isotropic checkpoint: magic, medium 0, iteration,
+    # parameters, state size 0
+    blob = b"SMPVECK\0" + (0).to_bytes(4, "little") + (5).to_bytes(8, "little") + bytes(C.sizeof(Params)) + \
+        (0).to_bytes(8, "little")
+    (tmp_path / "iso.dat").write_bytes(blob)
+    r = run_bin("pve_vti_pml", "-c", "iso.dat", cwd=tmp_path)
+    assert r.returncode == 1 and "not a checkpoint of pve_vti_pml" in r.stderr, r.stderr
 
 
 def _read_pve_vti(fn, np_):
