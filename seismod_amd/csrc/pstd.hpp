@@ -239,10 +239,16 @@ inline bool fft_factor(int n, int &nr, uint64_t &code)
     return m == 1 && n >= 2 && nr > 0 && nr <= FFT_MAXPASS;
 }
 
+// a b + c with one rounding
+__device__ __forceinline__ float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double mad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename T>
 __device__ __forceinline__ Cx<T> cmul(Cx<T> a, Cx<T> b)
 {
-    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    // fused multiply-adds: one rounding less per product (the transforms
+    // are held to a tolerance, not to the reference's FFTW bits)
+    return {mad(a.re, b.re, -(a.im * b.im)), mad(a.re, b.im, a.im * b.re)};
 }
 template <typename T>
 __device__ __forceinline__ Cx<T> cadd(Cx<T> a, Cx<T> b) { return {a.re + b.re, a.im + b.im}; }
@@ -299,8 +305,8 @@ __device__ __forceinline__ void dft(Cx<T> *v, const Cx<T> *r, int rs)
 #pragma unroll
             for (int t = 1; t <= H; ++t) {
                 const Cx<T> w = r[((s * t) % R) * rs];   // (cos, -sin)
-                A.re += sp[t - 1].re * w.re; A.im += sp[t - 1].im * w.re;
-                B.re -= sm[t - 1].re * w.im; B.im -= sm[t - 1].im * w.im;
+                A.re = mad(sp[t - 1].re, w.re, A.re); A.im = mad(sp[t - 1].im, w.re, A.im);
+                B.re = mad(-sm[t - 1].re, w.im, B.re); B.im = mad(-sm[t - 1].im, w.im, B.im);
             }
             out[s] = {A.re + B.im, A.im - B.re};
             out[R - s] = {A.re - B.im, A.im + B.re};
