@@ -466,15 +466,11 @@ struct DfftArgs {
 extern __shared__ __align__(16) unsigned char pstd_lds[];
 
 // AX 1: z (unit stride), row pairs (r, r + 1); AX 0: x (stride nz2),
-// column pairs (c, c + 1); a missing partner reads as 0 and is not written
+// column pairs (c, c + 1); a missing partner reads as 0 and is not written.
+// b: the workgroup's tile of this axis's jobs
 template <typename T, int AX>
-__global__ __launch_bounds__(512) void pstd_dfft(const DfftArgs<T> a)
+__device__ __forceinline__ void dfft_body(const DfftArgs<T> &a, int b)
 {
-    // workgroups of one XCD take consecutive tiles (adjacent columns share
-    // cache lines)
-    const int per = (a.nblocks + 7) / 8;
-    const int b = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (b >= a.nblocks) return;
     const int jb = b / a.ntiles, tile = b - jb * a.ntiles;
     const T *f = a.x + (size_t)((a.fields >> (4 * jb)) & 15) * a.N;
     T *out = a.d + (size_t)((a.slots >> (4 * jb)) & 15) * a.N;
@@ -494,6 +490,31 @@ __global__ __launch_bounds__(512) void pstd_dfft(const DfftArgs<T> a)
     for (int rep = 0; rep < 2; ++rep)
         c = fft_run<T, AX>(io, c, c == A ? B : A, n, P, a.nr, a.code, tw, rep == 0, rep == 1,
                            rep == 0 ? mul : nullptr);
+}
+
+// workgroups of one XCD take consecutive tiles (adjacent columns share
+// cache lines)
+__device__ __forceinline__ int xcd_tile(int nblocks)
+{
+    const int per = (nblocks + 7) / 8;
+    return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+}
+
+template <typename T, int AX>
+__global__ __launch_bounds__(512) void pstd_dfft(const DfftArgs<T> a)
+{
+    const int b = xcd_tile(a.nblocks);
+    if (b < a.nblocks) dfft_body<T, AX>(a, b);
+}
+
+// the derivatives of a stage along x (ax) and z (az) in one grid: the x
+// tiles first, then the z tiles
+template <typename T>
+__global__ __launch_bounds__(512) void pstd_dfft_xz(const DfftArgs<T> ax, const DfftArgs<T> az)
+{
+    const int b = xcd_tile(ax.nblocks + az.nblocks);
+    if (b < ax.nblocks) dfft_body<T, 0>(ax, b);
+    else if (b < ax.nblocks + az.nblocks) dfft_body<T, 1>(az, b - ax.nblocks);
 }
 
 // sources the stage kernels inject themselves, after the update of their
@@ -543,17 +564,12 @@ inline unsigned dfft_threads(int n, int P, int nr, uint64_t code)
     return (unsigned)std::min(512, std::max(64, (most + 63) / 64 * 64));
 }
 
-// the derivatives jobs[0..nj) along axis of the fields of x into d
+// the arguments, LDS bytes and threads of the derivatives jobs[0..nj)
+// along axis of the fields of x into d (own transforms)
 template <typename T>
-int deriv_batch(Grid &g, int axis, const T *x, T *d, const DJob *jobs, int nj)
+DfftArgs<T> dfft_args(const Grid &g, int axis, const T *x, T *d, const DJob *jobs, int nj, size_t &lds,
+                      unsigned &nt)
 {
-    if (!g.own[axis]) {
-        for (int q = 0; q < nj; ++q) {
-            int rc = deriv<T>(g, x + (size_t)jobs[q].field * g.N, axis, jobs[q].fwd, d + (size_t)jobs[q].slot * g.N);
-            if (rc != SM_OK) return rc;
-        }
-        return SM_OK;
-    }
     DfftArgs<T> a;
     a.x = x; a.d = d; a.N = (unsigned)g.N;
     a.fields = a.slots = a.fwd = 0;
@@ -572,11 +588,47 @@ int deriv_batch(Grid &g, int axis, const T *x, T *d, const DJob *jobs, int nj)
     a.P = g.P[axis];
     a.ntiles = ((other + 1) / 2 + a.P - 1) / a.P;
     a.nblocks = nj * a.ntiles;
+    lds = dfft_lds(n, a.P, sizeof(T), a.tsz);
+    nt = dfft_threads(n, a.P, a.nr, a.code);
+    return a;
+}
+
+// the derivatives jobs[0..nj) along axis of the fields of x into d
+template <typename T>
+int deriv_batch(Grid &g, int axis, const T *x, T *d, const DJob *jobs, int nj)
+{
+    if (!g.own[axis]) {
+        for (int q = 0; q < nj; ++q) {
+            int rc = deriv<T>(g, x + (size_t)jobs[q].field * g.N, axis, jobs[q].fwd, d + (size_t)jobs[q].slot * g.N);
+            if (rc != SM_OK) return rc;
+        }
+        return SM_OK;
+    }
+    size_t lds;
+    unsigned nt;
+    const DfftArgs<T> a = dfft_args<T>(g, axis, x, d, jobs, nj, lds, nt);
     const unsigned grid = (unsigned)((a.nblocks + 7) / 8 * 8);
-    const size_t lds = dfft_lds(n, a.P, sizeof(T), a.tsz);
-    const unsigned nt = dfft_threads(n, a.P, a.nr, a.code);
     if (axis == 0) pstd_dfft<T, 0><<<grid, nt, lds, g.ctx->stream>>>(a);
     else pstd_dfft<T, 1><<<grid, nt, lds, g.ctx->stream>>>(a);
+    return SM_OK;
+}
+
+// the derivatives xj[0..nx) along x and zj[0..nz) along z of the fields
+// of x into d: one launch when both axes use the own transforms
+template <typename T>
+int deriv_xz(Grid &g, const T *x, T *d, const DJob *xj, int nx, const DJob *zj, int nz)
+{
+    if (!(g.own[0] && g.own[1] && nx > 0 && nz > 0)) {
+        int rc = nx ? deriv_batch<T>(g, 0, x, d, xj, nx) : SM_OK;
+        if (rc == SM_OK && nz) rc = deriv_batch<T>(g, 1, x, d, zj, nz);
+        return rc;
+    }
+    size_t lx, lz;
+    unsigned tx, tz;
+    const DfftArgs<T> ax = dfft_args<T>(g, 0, x, d, xj, nx, lx, tx);
+    const DfftArgs<T> az = dfft_args<T>(g, 1, x, d, zj, nz, lz, tz);
+    const unsigned grid = (unsigned)((ax.nblocks + az.nblocks + 7) / 8 * 8);
+    pstd_dfft_xz<T><<<grid, std::max(tx, tz), std::max(lx, lz), g.ctx->stream>>>(ax, az);
     return SM_OK;
 }
 
@@ -763,9 +815,11 @@ inline int init(Grid &g, sm_ctx *ctx, size_t nx2, size_t nz2, size_t np, size_t 
         const size_t want = 81920;
         const bool ok = elt == 8
             ? hipFuncSetAttribute((const void *)pstd_dfft<double, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess &&
-              hipFuncSetAttribute((const void *)pstd_dfft<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess
+              hipFuncSetAttribute((const void *)pstd_dfft<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess &&
+              hipFuncSetAttribute((const void *)pstd_dfft_xz<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess
             : hipFuncSetAttribute((const void *)pstd_dfft<float, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess &&
-              hipFuncSetAttribute((const void *)pstd_dfft<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess;
+              hipFuncSetAttribute((const void *)pstd_dfft<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess &&
+              hipFuncSetAttribute((const void *)pstd_dfft_xz<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess;
         (void)hipGetLastError();
         if (ok) lds_max = (int)want;
     }

@@ -302,10 +302,7 @@ int div_curl(sm_pve *s, const T *x, bool div, bool curl)
     int nx = 0, nz = 0;
     if (div) { xj[nx++] = {W_VX, D_DIVX, false}; zj[nz++] = {W_VZ, D_DIVZ, false}; }
     if (curl) { xj[nx++] = {W_VZ, D_CURLX, true}; zj[nz++] = {W_VX, D_CURLZ, true}; }
-    int rc = SM_OK;
-    if (nx) rc = pstd::deriv_batch<T>(s->g, 0, x, d, xj, nx);
-    if (rc == SM_OK && nz) rc = pstd::deriv_batch<T>(s->g, 1, x, d, zj, nz);
-    return rc;
+    return pstd::deriv_xz<T>(s->g, x, d, xj, nx, zj, nz);
 }
 
 template <typename T>
@@ -320,8 +317,7 @@ int stage(sm_pve *s, int tabs, bool last, const T *x, T a, T b, int set_t, int a
                               {W_VX, L_DX_VX, false},  {W_QX, L_DX_QX, false}, {W_VZ, L_DX_VZ, true}};
     const pstd::DJob zj[6] = {{W_TXZ, L_DZ_TXZ, false}, {W_TZZ, L_DZ_TZZ, true}, {W_P, L_DZ_P, true},
                               {W_VZ, L_DZ_VZ, false},   {W_QZ, L_DZ_QZ, false},  {W_VX, L_DZ_VX, true}};
-    int rc = pstd::deriv_batch<T>(s->g, 0, x, d, xj, 6);
-    if (rc == SM_OK) rc = pstd::deriv_batch<T>(s->g, 1, x, d, zj, 6);
+    int rc = pstd::deriv_xz<T>(s->g, x, d, xj, 6, zj, 6);
     if (rc != SM_OK) return rc;
     PveStage<T> k;
     k.d = d; k.mat = (const T *)s->mat.ptr; k.x = x; k.ws = STATE<T>(s, s->iws);

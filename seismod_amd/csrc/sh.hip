@@ -189,8 +189,7 @@ int stage(sm_sh *s, int tabs, bool last, const T *x, T a, T b, int set_t, int ac
     // D_x^- tau_xy, D_x^+ v_y; D_z^- tau_yz, D_z^+ v_y (field, slot, forward)
     const pstd::DJob xj[2] = {{W_TXY, L_DX_TXY, false}, {W_VY, L_DX_VY, true}};
     const pstd::DJob zj[2] = {{W_TYZ, L_DZ_TYZ, false}, {W_VY, L_DZ_VY, true}};
-    if ((rc = pstd::deriv_batch<T>(s->g, 0, x, d, xj, 2)) != SM_OK) return rc;
-    if ((rc = pstd::deriv_batch<T>(s->g, 1, x, d, zj, 2)) != SM_OK) return rc;
+    if ((rc = pstd::deriv_xz<T>(s->g, x, d, xj, 2, zj, 2)) != SM_OK) return rc;
     ShStage<T> k;
     k.d = d; k.mat = (const T *)s->mat.ptr; k.x = x; k.ws = STATE<T>(s, s->iws);
     k.w = STATE<T>(s, s->iw); k.wt = STATE<T>(s, s->iwt);

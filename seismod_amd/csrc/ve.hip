@@ -205,8 +205,7 @@ int stage(sm_ve *s, int tabs, bool last, const T *x, T a, T b, int set_t, int ac
                               {W_VZ, L_DX_VZ, true}};
     const pstd::DJob zj[4] = {{W_TXZ, L_DZ_TXZ, false}, {W_TZZ, L_DZ_TZZ, true}, {W_VZ, L_DZ_VZ, false},
                               {W_VX, L_DZ_VX, true}};
-    int rc = pstd::deriv_batch<T>(s->g, 0, x, d, xj, 4);
-    if (rc == SM_OK) rc = pstd::deriv_batch<T>(s->g, 1, x, d, zj, 4);
+    int rc = pstd::deriv_xz<T>(s->g, x, d, xj, 4, zj, 4);
     if (rc != SM_OK) return rc;
     VeStage<T> k;
     k.d = d; k.mat = (const T *)s->mat.ptr; k.x = x; k.ws = STATE<T>(s, s->iws);
